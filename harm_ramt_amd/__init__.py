@@ -1,0 +1,344 @@
+"""HARM conservative timestep on MI355X (gfx950): Python host mirror of the C-ABI.
+
+The product is the shared library ``_lib/libharm_amd.so`` (HIP kernels + the
+C-ABI declared in ``include/harm_amd.h``).  This module binds it with ctypes and
+mirrors the reference's call surface -- ``fluxcalc``, ``flux_ct``, ``advance``,
+``bound_prim``, ``step_ch``, ``Utoprimgen`` (step_ch.c, advance.c,
+bounds.fishmon.c) -- on numpy arrays in the reference layout
+``p[N1+6][N2+6][8]``.  There is no CPU fallback: if the library is missing,
+importing the bindings raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libharm_amd.so")
+
+NPR = 8
+NG = 3
+NUMPFLAGS = 5
+# mnemonics (same values as the reference)
+DONOR, VANL, MINM, MC, PARA, CSSLOPE = 0, 1, 2, 3, 4, 90
+NLIMCENT, NLIMUP, NLIMDOWN = 101, 102, 103
+LAXFFLUX, HLLFLUX = 0, 1
+FLUXCTHLL, FLUXCTTOTH = 0, 1
+EOMGRMHD, EOMFFDE = 0, 1
+OUTFLOW, POLARAXIS, PERIODIC = 0, 4, 100
+X1UP, X1DN, X2UP, X2DN = 0, 1, 2, 3
+P, PK0, PK1, F1, F2, ULAST, UNEW = 0, 1, 2, 3, 4, 5, 6
+METRIC_MINK, METRIC_KS = 0, 2
+K_FLUX1, K_FLUX2, K_EMF, K_UPDATE, K_BOUND = 0, 1, 2, 3, 4
+KERNEL_NAMES = {K_FLUX1: "k_flux<1>", K_FLUX2: "k_flux<2>", K_EMF: "k_emf",
+                K_UPDATE: "k_update", K_BOUND: "k_bound_x1+k_bound_x2"}
+
+
+class HarmParams(ctypes.Structure):
+    _fields_ = [
+        ("n1", ctypes.c_int), ("n2", ctypes.c_int),
+        ("eomtype", ctypes.c_int), ("lim", ctypes.c_int), ("fluxmethod", ctypes.c_int),
+        ("fluxb", ctypes.c_int), ("timeorder", ctypes.c_int),
+        ("bc", ctypes.c_int * 4),
+        ("fixupflux", ctypes.c_int), ("poledeath", ctypes.c_int),
+        ("gam", ctypes.c_double), ("cour", ctypes.c_double), ("safe", ctypes.c_double),
+        ("defcon", ctypes.c_double), ("gammamax", ctypes.c_double), ("gammafail", ctypes.c_double),
+        ("rhominlimit", ctypes.c_double), ("uuminlimit", ctypes.c_double),
+        ("dx1", ctypes.c_double), ("dx2", ctypes.c_double),
+    ]
+
+
+class HarmGrid(ctypes.Structure):
+    _fields_ = [
+        ("n1", ctypes.c_int), ("n2", ctypes.c_int), ("metric", ctypes.c_int), ("defcoord", ctypes.c_int),
+        ("a", ctypes.c_double), ("R0", ctypes.c_double), ("Rin", ctypes.c_double), ("Rout", ctypes.c_double),
+        ("hslope", ctypes.c_double), ("startx1", ctypes.c_double), ("startx2", ctypes.c_double),
+        ("dx1", ctypes.c_double), ("dx2", ctypes.c_double), ("rhomin", ctypes.c_double),
+        ("uumin", ctypes.c_double),
+    ]
+
+
+# every symbol include/harm_amd.h declares (checked by the CPU tests)
+EXPORTS = [
+    "harm_create", "harm_destroy", "harm_last_error", "harm_set_stream", "harm_set_geometry",
+    "harm_set_array", "harm_get_array", "harm_array_ptr", "harm_get_pflag", "harm_set_time",
+    "harm_get_time", "harm_fluxcalc", "harm_fix_flux", "harm_flux_ct", "harm_advance",
+    "harm_bound_prim", "harm_step", "harm_step_n", "harm_utoprimgen", "harm_get_failures",
+    "harm_get_nstroke", "harm_prof_enable", "harm_prof_get", "harm_prof_reset",
+    "harm_grid_mks", "harm_grid_cartesian", "harm_geometry", "harm_init_fishmon",
+    "harm_init_orszag_tang",
+]
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Load libharm_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP library not built: {path} (run python -m harm_ramt_amd.build)")
+    lib = ctypes.CDLL(path)
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    vp = ctypes.c_void_p
+    sig = {
+        "harm_create": [ctypes.POINTER(HarmParams), ctypes.POINTER(vp)],
+        "harm_destroy": [vp],
+        "harm_set_stream": [vp, vp],
+        "harm_set_geometry": [vp, dp, dp, dp, dp, dp],
+        "harm_set_array": [vp, ctypes.c_int, dp],
+        "harm_get_array": [vp, ctypes.c_int, dp],
+        "harm_array_ptr": [vp, ctypes.c_int, ctypes.POINTER(vp)],
+        "harm_get_pflag": [vp, ip],
+        "harm_set_time": [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double],
+        "harm_get_time": [vp, dp, dp],
+        "harm_fluxcalc": [vp, ctypes.c_int, ctypes.c_int, dp],
+        "harm_fix_flux": [vp],
+        "harm_flux_ct": [vp],
+        "harm_advance": [vp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int, dp],
+        "harm_bound_prim": [vp, ctypes.c_int],
+        "harm_step": [vp],
+        "harm_step_n": [vp, ctypes.c_int],
+        "harm_utoprimgen": [vp, ctypes.c_int, ip, dp, dp, ip],
+        "harm_get_failures": [vp, ip, ip],
+        "harm_get_nstroke": [vp, ctypes.POINTER(ctypes.c_longlong)],
+        "harm_prof_enable": [vp, ctypes.c_int],
+        "harm_prof_get": [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong)],
+        "harm_prof_reset": [vp],
+        "harm_grid_mks": [ctypes.POINTER(HarmGrid), ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                          ctypes.c_double, ctypes.c_double, ctypes.c_double],
+        "harm_grid_cartesian": [ctypes.POINTER(HarmGrid), ctypes.c_int, ctypes.c_int],
+        "harm_geometry": [ctypes.POINTER(HarmGrid), dp, dp, dp, dp, dp],
+        "harm_init_fishmon": [ctypes.POINTER(HarmGrid), dp, dp, dp, ctypes.c_double, ctypes.c_double,
+                              ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                              ctypes.c_ulong, dp],
+        "harm_init_orszag_tang": [ctypes.POINTER(HarmGrid), ctypes.c_double, dp],
+    }
+    for name, args in sig.items():
+        f = getattr(lib, name)
+        f.argtypes = args
+        f.restype = ctypes.c_int
+    lib.harm_last_error.argtypes = []
+    lib.harm_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def _dptr(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _iptr(a):
+    assert a.dtype == np.int32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+class HarmError(RuntimeError):
+    pass
+
+
+def _chk(rc, what):
+    if rc != 0:
+        raise HarmError(f"{what}: {load_library().harm_last_error().decode()}")
+
+
+# ---------------------------------------------------------------------------
+# host-side problem generation (no GPU)
+# ---------------------------------------------------------------------------
+class Geometry:
+    """Geometry tables of set_grid() in the reference layout."""
+
+    def __init__(self, grid: HarmGrid):
+        n1m, n2m = grid.n1 + 2 * NG, grid.n2 + 2 * NG
+        self.grid = grid
+        self.gcov = np.zeros((n1m, n2m, 4, 4, 4))
+        self.gcon = np.zeros((n1m, n2m, 4, 4, 4))
+        self.gdet = np.zeros((n1m, n2m, 4))
+        self.conn = np.zeros((n1m, n2m, 4, 4, 4))
+        self.floors = np.zeros((n1m, n2m, 2))
+        lib = load_library()
+        _chk(lib.harm_geometry(ctypes.byref(grid), _dptr(self.gcov), _dptr(self.gcon), _dptr(self.gdet),
+                               _dptr(self.conn), _dptr(self.floors)), "harm_geometry")
+
+    def tables(self):
+        return self.gcov, self.gcon, self.gdet, self.conn, self.floors
+
+
+def grid_mks(n1, n2, a=0.9375, Rout=40.0, hslope=0.3, R0=0.0):
+    g = HarmGrid()
+    _chk(load_library().harm_grid_mks(ctypes.byref(g), n1, n2, a, Rout, hslope, R0), "harm_grid_mks")
+    return g
+
+
+def grid_cartesian(n1, n2):
+    g = HarmGrid()
+    _chk(load_library().harm_grid_cartesian(ctypes.byref(g), n1, n2), "harm_grid_cartesian")
+    return g
+
+
+def init_fishmon(geom: Geometry, gam=4.0 / 3.0, rin=6.0, rmax=12.0, kappa=1e-3, beta=100.0,
+                 randfact=4e-2, seed=7):
+    g = geom.grid
+    p = np.zeros((g.n1 + 2 * NG, g.n2 + 2 * NG, NPR))
+    _chk(load_library().harm_init_fishmon(ctypes.byref(g), _dptr(geom.gcov), _dptr(geom.gcon), _dptr(geom.gdet),
+                                          gam, rin, rmax, kappa, beta, randfact, seed, _dptr(p)),
+         "harm_init_fishmon")
+    return p
+
+
+def init_orszag_tang(geom: Geometry, gam=5.0 / 3.0):
+    g = geom.grid
+    p = np.zeros((g.n1 + 2 * NG, g.n2 + 2 * NG, NPR))
+    _chk(load_library().harm_init_orszag_tang(ctypes.byref(g), gam, _dptr(p)), "harm_init_orszag_tang")
+    return p
+
+
+def make_params(n1, n2, dx1, dx2, *, lim=PARA, fluxmethod=HLLFLUX, timeorder=4, bc=None, fixupflux=1,
+                poledeath=1, gam=4.0 / 3.0, cour=0.9, safe=1.3, defcon=1.0, gammamax=25.0,
+                gammafail=None, rhominlimit=1e-20, uuminlimit=1e-20, fluxb=FLUXCTTOTH, eomtype=EOMGRMHD):
+    """Parameter block with the reference's GRMHD defaults (initbase.c:119-243)."""
+    prm = HarmParams()
+    prm.n1, prm.n2 = n1, n2
+    prm.eomtype, prm.lim, prm.fluxmethod, prm.fluxb, prm.timeorder = eomtype, lim, fluxmethod, fluxb, timeorder
+    b = bc if bc is not None else (OUTFLOW, OUTFLOW, POLARAXIS, POLARAXIS)
+    for k in range(4):
+        prm.bc[k] = b[k]
+    prm.fixupflux, prm.poledeath = fixupflux, poledeath
+    prm.gam, prm.cour, prm.safe, prm.defcon = gam, cour, safe, defcon
+    prm.gammamax = gammamax
+    prm.gammafail = 100.0 * gammamax if gammafail is None else gammafail
+    prm.rhominlimit, prm.uuminlimit = rhominlimit, uuminlimit
+    prm.dx1, prm.dx2 = dx1, dx2
+    return prm
+
+
+def params_dict(prm: HarmParams):
+    return {f: (list(getattr(prm, f)) if f == "bc" else getattr(prm, f)) for f, _ in HarmParams._fields_}
+
+
+# ---------------------------------------------------------------------------
+# the GPU context: the reference's call surface
+# ---------------------------------------------------------------------------
+class Harm:
+    """One HARM grid resident on the GPU (a harm_ctx)."""
+
+    def __init__(self, prm: HarmParams, geom: Geometry = None):
+        self.lib = load_library()
+        self.prm = prm
+        self.shape = (prm.n1 + 2 * NG, prm.n2 + 2 * NG, NPR)
+        h = ctypes.c_void_p()
+        _chk(self.lib.harm_create(ctypes.byref(prm), ctypes.byref(h)), "harm_create")
+        self.h = h
+        if geom is not None:
+            self.set_geometry(*geom.tables())
+
+    def close(self):
+        if self.h:
+            self.lib.harm_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_geometry(self, gcov, gcon, gdet, conn, floors):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (gcov, gcon, gdet, conn, floors)]
+        _chk(self.lib.harm_set_geometry(self.h, *[_dptr(a) for a in arrs]), "harm_set_geometry")
+
+    def set_array(self, sel, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == self.shape
+        _chk(self.lib.harm_set_array(self.h, sel, _dptr(a)), "harm_set_array")
+
+    def get_array(self, sel):
+        a = np.zeros(self.shape)
+        _chk(self.lib.harm_get_array(self.h, sel, _dptr(a)), "harm_get_array")
+        return a
+
+    def array_ptr(self, sel):
+        v = ctypes.c_void_p()
+        _chk(self.lib.harm_array_ptr(self.h, sel, ctypes.byref(v)), "harm_array_ptr")
+        return v.value
+
+    def get_pflag(self):
+        a = np.zeros(self.shape[:2] + (NUMPFLAGS,), dtype=np.int32)
+        _chk(self.lib.harm_get_pflag(self.h, _iptr(a)), "harm_get_pflag")
+        return a
+
+    def set_time(self, t, dt, tf):
+        _chk(self.lib.harm_set_time(self.h, t, dt, tf), "harm_set_time")
+
+    def get_time(self):
+        t, dt = ctypes.c_double(), ctypes.c_double()
+        _chk(self.lib.harm_get_time(self.h, ctypes.byref(t), ctypes.byref(dt)), "harm_get_time")
+        return t.value, dt.value
+
+    # step_ch.c:741
+    def fluxcalc(self, pr_sel, dir):
+        ndt = ctypes.c_double()
+        _chk(self.lib.harm_fluxcalc(self.h, pr_sel, dir, ctypes.byref(ndt)), "harm_fluxcalc")
+        return ndt.value
+
+    # fixup.c:1357
+    def fix_flux(self):
+        _chk(self.lib.harm_fix_flux(self.h), "harm_fix_flux")
+
+    # step_ch.c:1159
+    def flux_ct(self):
+        _chk(self.lib.harm_flux_ct(self.h), "harm_flux_ct")
+
+    # advance.c:15
+    def advance(self, pi, pb, CUf, pf, Cunew, timeorder, numtimeorders):
+        cu = np.ascontiguousarray(CUf, dtype=np.float64)
+        cn = np.ascontiguousarray(Cunew, dtype=np.float64)
+        ndt = ctypes.c_double()
+        _chk(self.lib.harm_advance(self.h, pi, pb, _dptr(cu), pf, _dptr(cn), timeorder, numtimeorders,
+                                   ctypes.byref(ndt)), "harm_advance")
+        return ndt.value
+
+    # bounds.fishmon.c:13
+    def bound_prim(self, sel=P):
+        _chk(self.lib.harm_bound_prim(self.h, sel), "harm_bound_prim")
+
+    # step_ch.c:16
+    def step_ch(self):
+        _chk(self.lib.harm_step(self.h), "harm_step")
+
+    def step_n(self, n):
+        _chk(self.lib.harm_step_n(self.h, n), "harm_step_n")
+
+    # step_ch.c:468 for a batch of zones
+    def utoprimgen(self, zones, U, pr):
+        zones = np.ascontiguousarray(zones, dtype=np.int32)
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        out = np.ascontiguousarray(pr, dtype=np.float64).copy()
+        fl = np.zeros(len(zones), dtype=np.int32)
+        _chk(self.lib.harm_utoprimgen(self.h, len(zones), _iptr(zones), _dptr(U), _dptr(out), _iptr(fl)),
+             "harm_utoprimgen")
+        return out, fl
+
+    def failures(self):
+        n, f = ctypes.c_int(), ctypes.c_int()
+        _chk(self.lib.harm_get_failures(self.h, ctypes.byref(n), ctypes.byref(f)), "harm_get_failures")
+        return n.value, f.value
+
+    def nstroke(self):
+        v = ctypes.c_longlong()
+        _chk(self.lib.harm_get_nstroke(self.h, ctypes.byref(v)), "harm_get_nstroke")
+        return v.value
+
+    def prof_enable(self, on=True):
+        _chk(self.lib.harm_prof_enable(self.h, 1 if on else 0), "harm_prof_enable")
+
+    def prof_reset(self):
+        _chk(self.lib.harm_prof_reset(self.h), "harm_prof_reset")
+
+    def prof_get(self, kernel):
+        ms, cnt = ctypes.c_double(), ctypes.c_longlong()
+        _chk(self.lib.harm_prof_get(self.h, kernel, ctypes.byref(ms), ctypes.byref(cnt)), "harm_prof_get")
+        return ms.value, cnt.value

@@ -1,0 +1,348 @@
+// harm_device.h -- per-zone / per-face FP64 physics of HARM's conservative
+// step as __device__ functions for gfx950.
+//
+// Every function keeps the floating-point operation order of the reference
+// function it cites, and the library is compiled with -ffp-contract=off, so
+// that a step on the GPU is bit-identical to the reference CPU step on the
+// same inputs (IEEE add/mul/div/sqrt are correctly rounded on CDNA4).
+//
+// Geometry reaches these functions as register structs filled from the packed
+// device tables (see harm_kernels.hip):
+//   GeoU : the entries that get_state()/primtoflux()/vchar() read -- the upper
+//          triangle of gcov (phys.c:633-652 lower() reads only that), gcon^{0mu},
+//          the one diagonal gcon^{dd} of the flux direction, and gdet.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace harm {
+
+constexpr int NPR = 8;
+constexpr int RHO = 0, UU = 1, U1 = 2, U2 = 3, U3 = 4, B1 = 5, B2 = 6, B3 = 7;
+constexpr double NUMEPSILON = 2.2204460492503131e-16;
+
+// kernel-side parameter block (by value in kernel arguments)
+struct KParams {
+  int n1, n2, N2M;
+  int eomtype, lim, fluxmethod, fluxb;
+  int bc[4];
+  int fixupflux, poledeath;
+  double gam, cour, defcon, safe, gammamax, gammafail, rhominlimit, uuminlimit;
+  double dx1, dx2;
+};
+
+struct GeoU {
+  double g00, g01, g02, g03, g11, g12, g13, g22, g23, g33;  // gcov upper
+  double n00, n01, n02, n03;                                // gcon^{0 mu}
+  double ndd;                                               // gcon^{dir dir}
+  double gdet;
+};
+
+struct State {
+  double ucon[4], ucov[4], bcon[4], bcov[4];
+};
+
+__device__ __forceinline__ double dmax(double a, double b) { return (a > b) ? a : b; }  // phys.c:1580
+__device__ __forceinline__ double dmin(double a, double b) { return (a > b) ? b : a; }  // phys.c:1587
+__device__ __forceinline__ double dsign(double a) {                                      // phys.c:1562
+  return (a > 0.0) ? 1.0 : ((a < 0.0) ? -1.0 : 0.0);
+}
+
+// phys.c:848-877 gamma_calc
+__device__ __forceinline__ double gamma_calc(const double* pr, const GeoU& g) {
+  double qsq = g.g11 * pr[U1] * pr[U1] + g.g22 * pr[U2] * pr[U2] + g.g33 * pr[U3] * pr[U3] +
+               2. * (g.g12 * pr[U1] * pr[U2] + g.g13 * pr[U1] * pr[U3] + g.g23 * pr[U2] * pr[U3]);
+  return sqrt(1. + qsq);
+}
+
+// phys.c:633-652 lower
+__device__ __forceinline__ void lower(const double* v, const GeoU& g, double* w) {
+  w[0] = g.g00 * v[0] + g.g01 * v[1] + g.g02 * v[2] + g.g03 * v[3];
+  w[1] = g.g01 * v[0] + g.g11 * v[1] + g.g12 * v[2] + g.g13 * v[3];
+  w[2] = g.g02 * v[0] + g.g12 * v[1] + g.g22 * v[2] + g.g23 * v[3];
+  w[3] = g.g03 * v[0] + g.g13 * v[1] + g.g23 * v[2] + g.g33 * v[3];
+}
+
+// phys.c:716-727 get_state with ucon_calc_rel4vel :830-845 and bcon_calc :285-294
+__device__ __forceinline__ void get_state(const double* pr, const GeoU& g, State& q) {
+  double alpha = 1. / sqrt(-g.n00);
+  double beta1 = g.n01 * alpha * alpha;
+  double beta2 = g.n02 * alpha * alpha;
+  double beta3 = g.n03 * alpha * alpha;
+  double gamma = gamma_calc(pr, g);
+  q.ucon[0] = gamma / alpha;
+  q.ucon[1] = pr[U1] - gamma * beta1 / alpha;
+  q.ucon[2] = pr[U2] - gamma * beta2 / alpha;
+  q.ucon[3] = pr[U3] - gamma * beta3 / alpha;
+  lower(q.ucon, g, q.ucov);
+  q.bcon[0] = pr[B1] * q.ucov[1] + pr[B2] * q.ucov[2] + pr[B3] * q.ucov[3];
+  q.bcon[1] = (pr[B1] + q.bcon[0] * q.ucon[1]) / q.ucon[0];
+  q.bcon[2] = (pr[B2] + q.bcon[0] * q.ucon[2]) / q.ucon[0];
+  q.bcon[3] = (pr[B3] + q.bcon[0] * q.ucon[3]) / q.ucon[0];
+  lower(q.bcon, g, q.bcov);
+}
+
+__device__ __forceinline__ double dot4(const double* a, const double* b) {
+  return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
+}
+
+// phys.c:381-406 mhd_calc_0: T^dir_j, j=0..3 into mhd[0..3]
+__device__ __forceinline__ void mhd_calc(const KParams& P, const double* pr, int dir, const State& q,
+                                         double* mhd) {
+  double r, u, Pg, w;
+  if (P.eomtype == 0) {
+    r = pr[RHO];
+    u = pr[UU];
+    Pg = (P.gam - 1.) * u;
+    w = Pg + r + u;
+  } else {
+    r = u = Pg = w = 0.0;
+  }
+  (void)r;
+  double bsq = dot4(q.bcon, q.bcov);
+  double eta = w + bsq;
+  double ptot = Pg + bsq * 0.5;
+  double ed = eta * q.ucon[dir];
+  double bd = q.bcon[dir];
+#pragma unroll
+  for (int j = 0; j < 4; j++) mhd[j] = ed * q.ucov[j] + ptot * ((dir == j) ? 1. : 0.) - bd * q.bcov[j];
+}
+
+// phys.c:16-56 primtoflux (massflux :59, PRIMMAXWELL dual Faraday :155-185,
+// UtoU UNOTHING->UEVOLVE :219-265 with e[k]=gdet)
+__device__ __forceinline__ void primtoflux(const KParams& P, const double* pr, const State& q, int dir,
+                                           double gdet, double* F) {
+  double fin[NPR];
+  fin[RHO] = pr[RHO] * q.ucon[dir];
+  mhd_calc(P, pr, dir, q, &fin[UU]);
+  if (dir > 0) {
+    double Bd = pr[B1 + dir - 1];
+    fin[B1] = (pr[B1] * q.ucon[dir] - Bd * q.ucon[1]) / q.ucon[0];
+    fin[B2] = (pr[B2] * q.ucon[dir] - Bd * q.ucon[2]) / q.ucon[0];
+    fin[B3] = (pr[B3] * q.ucon[dir] - Bd * q.ucon[3]) / q.ucon[0];
+  } else {
+    fin[B1] = pr[B1];
+    fin[B2] = pr[B2];
+    fin[B3] = pr[B3];
+  }
+  if (P.eomtype == 0) fin[UU] += fin[RHO];
+#pragma unroll
+  for (int k = 0; k < NPR; k++) F[k] = fin[k] * gdet;
+}
+
+// vchar.c:37-135 vchar + :166-234 simplefast; returns 1 on the reference's
+// fail() conditions (COEFF_NEG, COEFF_SUP, VCHAR_DISCR)
+__device__ __forceinline__ int vchar(const KParams& P, const double* pr, const State& q, int dir,
+                                     const GeoU& g, double& vmax, double& vmin) {
+  double gam = P.gam;
+  double bsq = dot4(q.bcon, q.bcov);
+  double rho = pr[RHO], u = pr[UU];
+  double WW = rho + gam * u, cs2, EF, va2, cms2;
+  int fail = 0;
+  if (WW > 0.0) cs2 = gam * (gam - 1.) * u / WW;
+  else WW = cs2 = 0.0;
+  EF = bsq + WW;
+  if (EF > 0.0) va2 = bsq / EF;
+  else EF = va2 = 0.0;
+  cms2 = cs2 + va2 - cs2 * va2;
+  if (cms2 < 0.0) {
+    if (cms2 / (cs2 + va2) > -NUMEPSILON * 100.0) cms2 = 0.0;
+    else fail = 1;
+  }
+  if (cms2 > 1.) {
+    if (cms2 < 1.0 + NUMEPSILON * 100.0) cms2 = 1.0;
+    else fail = 1;
+  }
+  // simplefast: Acov=e_dir, Bcov=e_t raised with gcon (raise() keeps only the
+  // selected gcon entries: the remaining terms are x*0)
+  double Asq = g.ndd, Bsq = g.n00, AB = (dir == 1) ? g.n01 : g.n02;
+  double Au = q.ucon[dir], Bu = q.ucon[0];
+  double Au2 = Au * Au, Bu2 = Bu * Bu, AuBu = Au * Bu;
+  double A = Bu2 - (Bsq + Bu2) * cms2;
+  double B = 2. * (AuBu - (AB + AuBu) * cms2);
+  double C = Au2 - (Asq + Au2) * cms2;
+  double discr = B * B - 4. * A * C;
+  if ((discr < 0.0) && (discr > -1E-10)) discr = 0.0;
+  else if (discr < -1E-10) {
+    fail = 1;
+    discr = 0.;
+  }
+  discr = sqrt(discr);
+  double vm = -(-B + discr) / (2. * A);
+  double vp = -(-B - discr) / (2. * A);
+  if (vp < vm) {
+    vmax = vm;
+    vmin = vp;
+  } else {
+    vmax = vp;
+    vmin = vm;
+  }
+  return fail;
+}
+
+// fixup.c:1298-1354 limit_gamma: -1 changed, 1 failure, 0 untouched
+__device__ __forceinline__ int limit_gamma(double gammamax, double* pr, const GeoU& g) {
+  if (gammamax <= 1.0) return 0;
+  double gamma = gamma_calc(pr, g);
+  if ((gamma > gammamax && (gamma != 1.0))) {
+    double pref = (gammamax * gammamax - 1.) / (gamma * gamma - 1.);
+    if (pref < 0.0) return 1;
+    double f = sqrt(pref);
+    pr[U1] *= f;
+    pr[U2] *= f;
+    pr[U3] *= f;
+    return -1;
+  }
+  return 0;
+}
+
+// ---------------- reconstruction (interp.c) ----------------
+
+// interp.h:13-14
+__device__ __forceinline__ double minmod(double a, double b) {
+  return ((a) * (b) <= 0) ? 0.0 : ((fabs(a) < fabs(b)) ? (a) : (b));
+}
+
+// interp.c:106-149 slope_lim_3points
+__device__ __forceinline__ double slope3(int lim, double yl, double yc, double yr) {
+  if (lim == 3) {  // MC
+    double Dqm = 2.0 * (yc - yl), Dqp = 2.0 * (yr - yc), Dqc = 0.5 * (yr - yl);
+    return minmod(Dqc, minmod(Dqm, Dqp));
+  } else if (lim == 1) {  // VANL
+    double Dqm = (yc - yl), Dqp = (yr - yc), s = Dqm * Dqp;
+    if (s <= 0.) return 0.;
+    return (2.0 * s / (Dqm + Dqp));
+  } else if (lim == 2) {  // MINM
+    return minmod(yc - yl, yr - yc);
+  } else if (lim == 101) return 0.5 * (yr - yl);
+  else if (lim == 102) return (yc - yl);
+  else if (lim == 103) return (yr - yc);
+  return 0.0;  // DONOR
+}
+
+// interp.c:157-205 csslope
+__device__ __forceinline__ double csslope(int kk, double x1, double x2, double x3, double x4, double x5) {
+  double Dqp = (x4 - x3), Dqpp = (x5 - x4), Dqm = (x3 - x2), Dqmm = (x2 - x1);
+  double Dqc = 0.5 * (x4 - x2), Dqcm = 0.5 * (x3 - x1), Dqcp = 0.5 * (x5 - x3);
+  double s = 0.5 * (dsign(Dqp) + dsign(Dqm));
+  double sp = 0.5 * (dsign(Dqpp) + dsign(Dqp));
+  double sm = 0.5 * (dsign(Dqm) + dsign(Dqmm));
+  double alpha;
+  if (kk == RHO) alpha = 2.0;
+  else if ((kk >= U1) && (kk <= U3)) alpha = 1.25;
+  else if (kk == UU) alpha = 1.0;
+  else alpha = 1.25;
+  double Dql = alpha * dmin(fabs(Dqp), fabs(Dqm));
+  double Dqlp = alpha * dmin(fabs(Dqpp), fabs(Dqp));
+  double Dqlm = alpha * dmin(fabs(Dqm), fabs(Dqmm));
+  double Dqbp = sp * dmin(Dqlp, fabs(Dqcp));
+  double Dqbm = sm * dmin(Dqlm, fabs(Dqcm));
+  return s * dmin(fabs((1.3333333333333333333333333333333333) * Dqc -
+                       (0.1666666666666666666666666666666666) * (Dqbp + Dqbm)),
+                  Dql);
+}
+
+// interp.c:449-538 para4 (PARA2LIM=MC): left/right edge values of the middle zone
+__device__ __forceinline__ void para4(double y0, double y1, double y2, double y3, double y4, double& lout,
+                                      double& rout) {
+  double y[5] = {y0, y1, y2, y3, y4}, dq[4];
+#pragma unroll
+  for (int i = 1; i < 4; i++) {
+    double Dqm = 2.0 * (y[i] - y[i - 1]);
+    double Dqp = 2.0 * (y[i + 1] - y[i]);
+    double Dqc = 0.5 * (y[i + 1] - y[i - 1]);
+    double s = Dqm * Dqp;
+    dq[i] = (s <= 0.) ? 0. : dmin(fabs(Dqc), dmin(fabs(Dqm), fabs(Dqp))) * dsign(Dqc);
+  }
+  double l = 0.5 * (y[2] + y[1]) - (dq[2] - dq[1]) / 8.0;
+  double r = 0.5 * (y[3] + y[2]) - (dq[3] - dq[2]) / 8.0;
+  l = dmax(dmin(y[2], y[1]), l);
+  l = dmin(dmax(y[2], y[1]), l);
+  r = dmax(dmin(y[2], y[3]), r);
+  r = dmin(dmax(y[2], y[3]), r);
+  double qa = (r - y[2]) * (y[2] - l);
+  double qd = (r - l);
+  double qe = 8.0 * (y[2] - 0.5 * (l + r));
+  if (qa <= 0.) {
+    l = y[2];
+    r = y[2];
+  } else {
+    if (qd * (qd - qe) < 0.0) l = 3.0 * y[2] - 2.0 * r;
+    if (qd * (qd + qe) < 0.0) r = 3.0 * y[2] - 2.0 * l;
+  }
+  lout = l;
+  rout = r;
+}
+
+// ---------------- GRMHD inversion (utoprim_2d_final.c) ----------------
+
+struct U2P {
+  double Bsq, QdotBsq, Qtsq, Qdotn, D, wglobal, gam, gammafail;
+};
+
+// utoprim_2d_final.c:356-370 vsq_calc
+__device__ __forceinline__ double vsq_calc(const U2P& s, double W) {
+  double Wsq = W * W;
+  double Xsq = (s.Bsq + W) * (s.Bsq + W);
+  return ((Wsq * s.Qtsq + s.QdotBsq * (s.Bsq + 2. * W)) / (Wsq * Xsq));
+}
+
+// utoprim_2d_final.c:1082-1155 func_vsq2 (+ pressure_W_vsq :447, dpdW_calc_vsq :465, dpdvsq_calc :479)
+__device__ __forceinline__ void func_vsq2(const U2P& s, double W, double vsq, double& dx0, double& dx1,
+                                          double& f, double& df) {
+  double gam = s.gam, Wsq = W * W, gtmp = 1. - vsq;
+  double p_tmp = (gam - 1.) * (W * gtmp - s.D * sqrt(gtmp)) / gam;
+  double dPdW = (gam - 1.) * (1. - vsq) / gam;
+  double dPdvsq = (gam - 1.) * (0.5 * s.D / sqrt(1. - vsq) - W) / gam;
+  double Bsq = s.Bsq, QdotBsq = s.QdotBsq;
+  double t2 = -0.5 * Bsq + dPdvsq;
+  double t3 = Bsq + W;
+  double t4 = t3 * t3;
+  double t9 = 1 / Wsq;
+  double t11 = s.Qtsq - vsq * t4 + QdotBsq * (Bsq + 2.0 * W) * t9;
+  double t16 = QdotBsq * t9;
+  double t18 = -s.Qdotn - 0.5 * Bsq * (1.0 + vsq) + 0.5 * t16 - W + p_tmp;
+  double t21 = 1 / t3;
+  double t23 = 1 / W;
+  double t24 = t16 * t23;
+  double t25 = -1.0 + dPdW - t24;
+  double t35 = t25 * t3 + (Bsq - 2.0 * dPdvsq) * (QdotBsq + vsq * Wsq * W) * t9 * t23;
+  double t36 = 1 / t35;
+  dx0 = -(t2 * t11 + t4 * t18) * t21 * t36;
+  double t40 = (vsq + t24) * t3;
+  dx1 = -(-t25 * t11 - 2.0 * t40 * t18) * t21 * t36;
+  df = -t11 * t11 - t18 * t18;
+  f = -0.5 * (df);
+}
+
+// utoprim_2d_final.c:644-1012 general_newton_raphson without line search
+// (USE_LINE_SEARCH=-2), NEWCONVERGE=1, EXTRA_NEWT_ITER=2, MAX_NEWT_ITER=200
+__device__ __forceinline__ int newton2d(const U2P& s, double& x0, double& x1, int& niter) {
+  const double NEWT_TOL = 1.0e-15, MIN_NEWT_TOL = 1.0e-10;
+  double VSQ_TOO_BIG = (1.0 - 1.0 / ((s.gammafail - 1.0) + 1.0));
+  double GAMMASQ_TOO_BIG = (s.gammafail * s.gammafail);
+  double errx = 1., f = 1., df = 1., dx0, dx1;
+  int n_iter = 0, i_extra = 0, doing_extra = 0;
+  bool keep = true;
+  while (keep) {
+    func_vsq2(s, x0, x1, dx0, dx1, f, df);
+    double xo0 = x0;
+    x0 += dx0;
+    x1 += dx1;
+    errx = (x0 == 0.) ? fabs(dx0) : fabs(dx0 / x0);
+    x0 = fabs(x0);
+    x0 = (x0 / s.wglobal > GAMMASQ_TOO_BIG) ? xo0 : x0;
+    x1 = (x1 < 0.) ? 0.0 : x1;
+    x1 = (x1 > 1.) ? VSQ_TOO_BIG : x1;
+    if ((fabs(errx) <= NEWT_TOL) && (doing_extra == 0)) doing_extra = 1;
+    if (doing_extra == 1) i_extra++;
+    if (((fabs(errx) <= NEWT_TOL) && (doing_extra == 0)) || (i_extra > 2) || (n_iter >= (200 - 1))) keep = false;
+    n_iter++;
+  }
+  niter = n_iter;
+  if (!isfinite(f) || !isfinite(df) || !isfinite(x0) || !isfinite(x1)) return 1;
+  if (fabs(errx) > MIN_NEWT_TOL) return 1;
+  return 0;
+}
+
+}  // namespace harm

@@ -1,0 +1,213 @@
+/*
+ * harm_amd.h -- C-ABI of the MI355X (gfx950) HARM conservative timestep.
+ *
+ * Drop-in boundary for the reference's step_ch()/advance()/fluxcalc()/
+ * flux_ct()/Utoprimgen()/bound_prim() call surface (global.h:1383-1391,
+ * 1525-1545).  The reference keeps every array and parameter in globals
+ * (defs.h:4-83, 168-214); here they live in a context handle so the API is
+ * re-entrant, and the arrays live in GPU memory (HBM).
+ *
+ * Array layout is the reference's (set_arrays.c:116-139):
+ *   primitive-shaped      double a[N1M][N2M][NPR],  N?M = N? + 6, NPR = 8
+ *   gcov, gcon            double a[N1M][N2M][NPG][4][4]
+ *   gdet                  double a[N1M][N2M][NPG]
+ *   conn                  double a[N1M][N2M][4][4][4]     (CENT)
+ *   floors                double a[N1M][N2M][2]           (CENT; RHOMIN r^-2.7,
+ *                                                          UUMIN r^-2.7: the
+ *                                                          set_density_floors()
+ *                                                          values, fixup.c:996-1110)
+ *   pflag                 int    a[N1M][N2M][NUMPFLAGS=5]
+ * Zone (i,j), i in [-3,N1+2], j in [-3,N2+2], is stored at (i+3,j+3).
+ *
+ * Errors: every entry point returns 0 on success, >0 on error (the reference's
+ * MYFUN convention, global.h:1204-1210).  harm_last_error() gives the text.
+ * Per-zone inversion failures are not errors: they are reported in pflag
+ * (FLAGUTOPRIMFAIL, global.h:546) and counted by harm_get_failures().
+ */
+#ifndef HARM_AMD_H
+#define HARM_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- mnemonics, identical values to the reference ---- */
+/* slope limiters (step_ch.h:48-58) */
+#define HARM_DONOR 0
+#define HARM_VANL 1
+#define HARM_MINM 2
+#define HARM_MC 3
+#define HARM_PARA 4
+#define HARM_CSSLOPE 90
+#define HARM_NLIMCENT 101
+#define HARM_NLIMUP 102
+#define HARM_NLIMDOWN 103
+/* flux methods (step_ch.h:17-18) */
+#define HARM_LAXFFLUX 0
+#define HARM_HLLFLUX 1
+/* divB methods (step_ch.h:22-26): FLUXCTTOTH and FLUXCTHLL are supported */
+#define HARM_FLUXCTHLL 0
+#define HARM_FLUXCTTOTH 1
+/* equations of motion (global.h:199-200) */
+#define HARM_EOMGRMHD 0
+#define HARM_EOMFFDE 1
+/* boundary types (global.h:616-622); PERIODIC is the single-GPU stand-in for
+   the reference's periodic MPI halo (init_mpi.c:305-313) */
+#define HARM_OUTFLOW 0
+#define HARM_POLARAXIS 4
+#define HARM_PERIODIC 100
+/* boundary slots (global.h:85-88) */
+#define HARM_X1UP 0
+#define HARM_X1DN 1
+#define HARM_X2UP 2
+#define HARM_X2DN 3
+/* array selectors for the context's device arrays (defs.h names) */
+#define HARM_P 0
+#define HARM_PK0 1
+#define HARM_PK1 2
+#define HARM_F1 3
+#define HARM_F2 4
+#define HARM_ULAST 5
+#define HARM_UNEW 6
+#define HARM_NARRAYS 7
+
+#define HARM_NPR 8
+#define HARM_NG 3
+#define HARM_NUMPFLAGS 5
+
+/* Parameters that the reference keeps in globals (initbase.c:119-243). */
+typedef struct {
+  int n1, n2;          /* interior zones, N1 and N2 (global.h:126-127) */
+  int eomtype;         /* EOMTYPE */
+  int lim;             /* lim */
+  int fluxmethod;      /* fluxmethod */
+  int fluxb;           /* FLUXB */
+  int timeorder;       /* TIMEORDER (1..4) */
+  int bc[4];           /* BCtype[X1UP..X2DN] */
+  int fixupflux;       /* FIXUPFLUX (step_ch.h:82): fix_flux() on/off */
+  int poledeath;       /* POLEDEATH (bounds.fishmon.c:328) */
+  double gam;          /* adiabatic index */
+  double cour;         /* Courant factor */
+  double safe;         /* SAFE: max dt growth per step */
+  double defcon;       /* defcon */
+  double gammamax;     /* GAMMAMAX */
+  double gammafail;    /* GAMMAFAIL */
+  double rhominlimit;  /* RHOMINLIMIT */
+  double uuminlimit;   /* UUMINLIMIT */
+  double dx1, dx2;     /* dx[1], dx[2] */
+} harm_params;
+
+typedef struct harm_ctx harm_ctx;
+
+/* Context lifetime: replaces set_arrays() (set_arrays.c:109) plus the
+   globals set by init_defglobal().  Allocates all arrays on the current GPU. */
+int harm_create(const harm_params *prm, harm_ctx **out);
+int harm_destroy(harm_ctx *ctx);
+const char *harm_last_error(void);
+
+/* Run on a caller-owned HIP stream (hipStream_t passed as void*); NULL means
+   the context's own stream. */
+int harm_set_stream(harm_ctx *ctx, void *hip_stream);
+
+/* Geometry tables in the reference layout (set_grid.c:39-88 output).  Host
+   pointers; copied and packed into the device layout once. */
+int harm_set_geometry(harm_ctx *ctx, const double *gcov, const double *gcon,
+                      const double *gdet, const double *conn,
+                      const double *floors);
+
+/* Host <-> device copies of a context array (selector HARM_*). */
+int harm_set_array(harm_ctx *ctx, int sel, const double *host);
+int harm_get_array(harm_ctx *ctx, int sel, double *host);
+/* Device pointer of a context array, for zero-copy use (e.g. torch). */
+int harm_array_ptr(harm_ctx *ctx, int sel, double **dev);
+int harm_get_pflag(harm_ctx *ctx, int *host);
+
+/* Time state: t, dt, tf (defs.h:177-179). */
+int harm_set_time(harm_ctx *ctx, double t, double dt, double tf);
+int harm_get_time(harm_ctx *ctx, double *t, double *dt);
+
+/* step_ch.c:741 fluxcalc(stage, pr, F, dir, &ndt): fluxes of array pr_sel into
+   F1 (dir=1) or F2 (dir=2) and the Courant limit ndt.  Includes the
+   reference's per-face limit_gamma, vchar, HLL/LAXF. */
+int harm_fluxcalc(harm_ctx *ctx, int pr_sel, int dir, double *ndt);
+/* fixup.c:1357 fix_flux(pb, F1, F2) on the context fluxes. */
+int harm_fix_flux(harm_ctx *ctx);
+/* step_ch.c:1159 flux_ct(stage, pb, F1, F2): constrained-transport EMFs
+   rewritten into F1/F2. */
+int harm_flux_ct(harm_ctx *ctx);
+/* advance.c:15 advance(stage, pi, ulast, pb, CUf, pf, Cunew, unew, timeorder,
+   numtimeorders, &ndt): one RK substage.  pi/pb/pf may alias (advance.c:5-14). */
+int harm_advance(harm_ctx *ctx, int pi_sel, int pb_sel, const double CUf[3],
+                 int pf_sel, const double Cunew[3], int timeorder,
+                 int numtimeorders, double *ndt);
+/* bounds.fishmon.c:13 bound_prim(stage, prim) on array sel (device ghost fill:
+   outflow/polar/periodic, replacing bound_mpi's halo exchange). */
+int harm_bound_prim(harm_ctx *ctx, int sel);
+/* step_ch.c:16 step_ch(): one full timestep including the dt update
+   (step_ch.c:174-183). */
+int harm_step(harm_ctx *ctx);
+/* nsteps calls of step_ch() with no host synchronisation in between (the dt
+   update stays on the GPU); replays a captured HIP graph of one step. */
+int harm_step_n(harm_ctx *ctx, int nsteps);
+/* step_ch.c:468 Utoprimgen(EVOLVEUTOPRIM, UEVOLVE, U, geom, pr) for n zones:
+   zone[2*k],zone[2*k+1] = (i,j); U[k][8] evolved conserved state; pr[k][8]
+   guess in, solution out; pflag[k] out.  Host arrays. */
+int harm_utoprimgen(harm_ctx *ctx, int n, const int *zone, const double *U,
+                    double *pr, int *pflag);
+
+/* Failure reporting: number of zones whose last inversion set pflag, and the
+   reference's global `failed` flag (fail.c:13). */
+int harm_get_failures(harm_ctx *ctx, int *nfailzones, int *failed);
+/* Newton-iteration counter (the reference's nstroke, main.c:72-78). */
+int harm_get_nstroke(harm_ctx *ctx, long long *nstroke);
+
+/* Per-kernel timing with HIP events on the launch stream.
+   kernel ids: 0 flux dir1, 1 flux dir2, 2 emf, 3 update, 4 bound */
+#define HARM_K_FLUX1 0
+#define HARM_K_FLUX2 1
+#define HARM_K_EMF 2
+#define HARM_K_UPDATE 3
+#define HARM_K_BOUND 4
+#define HARM_K_COUNT 5
+int harm_prof_enable(harm_ctx *ctx, int on);
+int harm_prof_get(harm_ctx *ctx, int kernel, double *total_ms, long long *count);
+int harm_prof_reset(harm_ctx *ctx);
+
+/* ---- host-side problem and geometry generation (no GPU use) ---- */
+
+/* Grid/coordinate description (coord.c: set_points, bl_coord, dxdxprim). */
+#define HARM_METRIC_MINK 0  /* MINKMETRIC (metric.c:441) */
+#define HARM_METRIC_KS 2    /* KSCOORDS (metric.c:540,666) */
+typedef struct {
+  int n1, n2;
+  int metric;          /* HARM_METRIC_* */
+  int defcoord;        /* 0: MKS, r=R0+exp(X1), th=pi X2+(1-h)/2 sin(2 pi X2) */
+  double a;            /* black-hole spin */
+  double R0, Rin, Rout, hslope;
+  double startx1, startx2, dx1, dx2;
+  double rhomin, uumin;  /* floor coefficients RHOMIN, UUMIN */
+} harm_grid;
+
+/* MKS grid for a Kerr hole as the reference's init_grid() does for the torus:
+   Rin = setRin(setihor()) (coord.c:817-909), set_points() (coord.c:720). */
+int harm_grid_mks(harm_grid *g, int n1, int n2, double a, double Rout,
+                  double hslope, double R0);
+/* Unit-square Cartesian grid with Minkowski metric (X in [0,1)^2). */
+int harm_grid_cartesian(harm_grid *g, int n1, int n2);
+/* Geometry tables of set_grid() (set_grid.c:6-88) for the grid. */
+int harm_geometry(const harm_grid *g, double *gcov, double *gcon, double *gdet,
+                  double *conn, double *floors);
+/* Fishbone-Moncrief torus initial primitives (init.fishmon.c:115-352):
+   rin, rmax, kappa, beta (normalisation (gam-1)u_max/(0.5 b^2_max)),
+   randfact (u perturbation amplitude), seed.  Writes p[N1M][N2M][8]. */
+int harm_init_fishmon(const harm_grid *g, const double *gcov, const double *gcon,
+                      const double *gdet, double gam, double rin, double rmax,
+                      double kappa, double beta, double randfact,
+                      unsigned long seed, double *p);
+/* Relativistic Orszag-Tang vortex on the unit square (Minkowski). */
+int harm_init_orszag_tang(const harm_grid *g, double gam, double *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1101 @@
+/*
+ * harm_oracle.c -- CPU restatement of HARM's conservative timestep (FP64).
+ *
+ * TEST INFRASTRUCTURE ONLY (the oracle).  Loaded by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg, never by the
+ * product.  Every function restates the reference function it cites with the
+ * same floating-point operation order, so that with -ffp-contract=off it is
+ * bit-identical to the compiled reference on the same inputs.
+ *
+ * Compile-time flags of the reference that this restatement fixes:
+ *   WHICHVEL=VELREL4, WHICHEOM=WITHGDET, REMOVERESTMASSFROMUU=1, NPR=8,
+ *   MAXWELL=PRIMMAXWELL, TRUEFAST=0, HYPERHLL=0, BDIRCONT=1, EVOLVECHECKS=1,
+ *   RESCALEINTERP=0, LIMADJUST=FLUXADJUST=0, DOENO=0, CORNGDETVERSION=1,
+ *   FIXUPZONES=FIXUP1ZONE, UTOPRIMVERSION=UTOPRIM2DFINAL,
+ *   UTOPRIMTRYAGAIN=0, CHECKSOLUTION=0, rescaletype=1, EXTRAP=0.
+ */
+#include "harm_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NPR 8
+#define NDIM 4
+#define NG 3
+#define RHO 0
+#define UU 1
+#define U1 2
+#define U2 3
+#define U3 4
+#define B1 5
+#define B2 6
+#define B3 7
+#define FACE1 0
+#define FACE2 1
+#define CORN 2
+#define CENT 3
+#define NUMEPSILON (2.2204460492503131e-16)
+
+/* u2p_defs.h:40-78 with PRECISEINVERSION=1 */
+#define MAX_NEWT_ITER 200
+#define NEWT_TOL 1.0e-15
+#define MIN_NEWT_TOL 1.0e-10
+#define EXTRA_NEWT_ITER 2
+#define MAXNEGUTSQ (1E-10)
+#define MAXNEGVSQ (1.0 - 1.0 / (MAXNEGUTSQ + 1.0))
+#define FAIL_VAL 1.e30
+
+/* failure codes (global.h:525-536) */
+#define UTOPRIMNOFAIL 0
+#define UTOPRIMFAILCONVUTSQ 2
+#define UTOPRIMFAILCONVW 3
+#define UTOPRIMFAILCONVUTSQ2 4
+#define UTOPRIMFAILRHONEG 5
+#define UTOPRIMFAILUNEG 6
+#define UTOPRIMFAILRHOUNEG 7
+#define NUMPFLAGS 5
+
+struct oracle_ctx {
+  oracle_params p;
+  int N1M, N2M, nz;
+  double *gcov, *gcon, *gdet, *conn, *floors;
+  double *arr[12];
+  int *pflag;
+  double t, dt, tf;
+  long nstroke;
+  int failed;
+};
+
+typedef struct {
+  const double (*gcov)[NDIM];
+  const double (*gcon)[NDIM];
+  double g;
+  double e[NPR];
+  int i, j, p;
+} geom_t;
+
+typedef struct {
+  double ucon[NDIM], ucov[NDIM], bcon[NDIM], bcov[NDIM];
+} state_t;
+
+static inline int zi(const oracle_ctx *c, int i, int j) {
+  return (i + NG) * c->N2M + (j + NG);
+}
+#define PA(c, a, i, j) ((a) + (size_t)NPR * zi((c), (i), (j)))
+
+/* phys.c:1562-1592 */
+static inline double sign_(double a) { if (a > 0.0) return 1.0; else if (a < 0.0) return -1.0; else return 0.0; }
+static inline double max_(double a, double b) { if (a > b) return a; else return b; }
+static inline double min_(double a, double b) { if (a > b) return b; else return a; }
+/* interp.h:13-14 */
+#define MINMODB(a, b) ((fabs(a) < fabs(b)) ? (a) : (b))
+#define MINMOD(a, b) (((a) * (b) <= 0) ? 0.0 : MINMODB(a, b))
+
+/* phys.c:730-765 get_geometry + assign_eomfunc (WITHGDET: e[k]=gdet) */
+static void get_geometry(const oracle_ctx *c, int i, int j, int loc, geom_t *g) {
+  size_t z = (size_t)zi(c, i, j);
+  g->gcov = (const double (*)[NDIM])(c->gcov + (z * 4 + loc) * 16);
+  g->gcon = (const double (*)[NDIM])(c->gcon + (z * 4 + loc) * 16);
+  g->g = c->gdet[z * 4 + loc];
+  for (int k = 0; k < NPR; k++) g->e[k] = g->g;
+  g->i = i; g->j = j; g->p = loc;
+}
+
+/* phys.c:848-877 gamma_calc */
+static double gamma_calc(const double *pr, const geom_t *g) {
+  const double (*gc)[NDIM] = g->gcov;
+  double qsq = gc[1][1] * pr[U1] * pr[U1] + gc[2][2] * pr[U2] * pr[U2] +
+               gc[3][3] * pr[U3] * pr[U3] +
+               2. * (gc[1][2] * pr[U1] * pr[U2] + gc[1][3] * pr[U1] * pr[U3] +
+                     gc[2][3] * pr[U2] * pr[U3]);
+  return sqrt(1. + qsq);
+}
+
+/* phys.c:830-845 ucon_calc_rel4vel */
+static void ucon_calc(const double *pr, const geom_t *g, double *ucon) {
+  double alpha = 1. / sqrt(-g->gcon[0][0]);
+  double beta[NDIM];
+  for (int j = 1; j < NDIM; j++) beta[j] = g->gcon[0][j] * alpha * alpha;
+  double gamma = gamma_calc(pr, g);
+  ucon[0] = gamma / alpha;
+  for (int j = 1; j < NDIM; j++) ucon[j] = pr[U1 + j - 1] - gamma * beta[j] / alpha;
+}
+
+/* phys.c:633-652 lower (upper triangle of gcov) */
+static void lower(const double *ucon, const geom_t *g, double *ucov) {
+  const double (*gc)[NDIM] = g->gcov;
+  ucov[0] = gc[0][0] * ucon[0] + gc[0][1] * ucon[1] + gc[0][2] * ucon[2] + gc[0][3] * ucon[3];
+  ucov[1] = gc[0][1] * ucon[0] + gc[1][1] * ucon[1] + gc[1][2] * ucon[2] + gc[1][3] * ucon[3];
+  ucov[2] = gc[0][2] * ucon[0] + gc[1][2] * ucon[1] + gc[2][2] * ucon[2] + gc[2][3] * ucon[3];
+  ucov[3] = gc[0][3] * ucon[0] + gc[1][3] * ucon[1] + gc[2][3] * ucon[2] + gc[3][3] * ucon[3];
+}
+
+/* phys.c:285-294 bcon_calc */
+static void bcon_calc(const double *pr, const double *ucon, const double *ucov, double *bcon) {
+  bcon[0] = pr[B1] * ucov[1] + pr[B2] * ucov[2] + pr[B3] * ucov[3];
+  for (int j = 1; j <= 3; j++) bcon[j] = (pr[B1 - 1 + j] + bcon[0] * ucon[j]) / ucon[0];
+}
+
+/* phys.c:716-727 get_state */
+static void get_state(const double *pr, const geom_t *g, state_t *q) {
+  ucon_calc(pr, g, q->ucon);
+  lower(q->ucon, g, q->ucov);
+  bcon_calc(pr, q->ucon, q->ucov, q->bcon);
+  lower(q->bcon, g, q->bcov);
+}
+
+#define DOT(a, b) ((a)[0] * (b)[0] + (a)[1] * (b)[1] + (a)[2] * (b)[2] + (a)[3] * (b)[3])
+
+/* phys.c:381-406 mhd_calc_0 (T^dir_j) */
+static void mhd_calc_0(const oracle_ctx *c, const double *pr, int dir, const state_t *q, double *mhd) {
+  double r, u, P, w, bsq, eta, ptot;
+  if (c->p.eomtype == OR_GRMHD) {
+    r = pr[RHO]; u = pr[UU]; P = (c->p.gam - 1.) * u; w = P + r + u;
+  } else {
+    r = u = P = w = 0.0;
+  }
+  (void)r;
+  bsq = DOT(q->bcon, q->bcov);
+  eta = w + bsq;
+  ptot = P + bsq * 0.5;
+  for (int j = 0; j < NDIM; j++)
+    mhd[j] = eta * q->ucon[dir] * q->ucov[j] + ptot * ((dir == j) ? 1. : 0.) - q->bcon[dir] * q->bcov[j];
+}
+
+/* phys.c:16-56 primtoflux, with massflux_calc :59, dualfullfaraday_calc
+   PRIMMAXWELL :155-185 and UtoU(UNOTHING->UEVOLVE) :219-265 */
+static void primtoflux(const oracle_ctx *c, const double *pr, const state_t *q, int dir,
+                       const geom_t *g, double *flux) {
+  double fin[NPR];
+  fin[RHO] = pr[RHO] * q->ucon[dir];
+  mhd_calc_0(c, pr, dir, q, &fin[UU]);
+  if (dir > 0) {
+    fin[B1] = (pr[B1] * q->ucon[dir] - pr[B1 + dir - 1] * q->ucon[1]) / q->ucon[0];
+    fin[B2] = (pr[B2] * q->ucon[dir] - pr[B1 + dir - 1] * q->ucon[2]) / q->ucon[0];
+    fin[B3] = (pr[B3] * q->ucon[dir] - pr[B1 + dir - 1] * q->ucon[3]) / q->ucon[0];
+  } else {
+    fin[B1] = pr[B1]; fin[B2] = pr[B2]; fin[B3] = pr[B3];
+  }
+  if (c->p.eomtype != OR_FFDE) fin[UU] += fin[RHO];
+  for (int k = 0; k < NPR; k++) flux[k] = fin[k] * g->e[k];
+}
+
+/* vchar.c:166-234 simplefast, with raise() of unit covectors reduced to
+   the gcon entries it selects (exact: the other terms are x*0 added to it) */
+static int simplefast(int dir, const geom_t *g, const state_t *q, double cms2, double *vmin, double *vmax) {
+  double Asq = g->gcon[dir][dir];
+  double Bsq = g->gcon[0][0];
+  double Au = q->ucon[dir];
+  double Bu = q->ucon[0];
+  double AB = g->gcon[0][dir];
+  double Au2 = Au * Au, Bu2 = Bu * Bu, AuBu = Au * Bu;
+  double A = Bu2 - (Bsq + Bu2) * cms2;
+  double B = 2. * (AuBu - (AB + AuBu) * cms2);
+  double C = Au2 - (Asq + Au2) * cms2;
+  double discr = B * B - 4. * A * C;
+  int fail = 0;
+  if ((discr < 0.0) && (discr > -1E-10)) discr = 0.0;
+  else if (discr < -1E-10) { fail = 1; discr = 0.; }
+  discr = sqrt(discr);
+  double vm = -(-B + discr) / (2. * A);
+  double vp = -(-B - discr) / (2. * A);
+  if (vp < vm) { *vmax = vm; *vmin = vp; } else { *vmax = vp; *vmin = vm; }
+  return fail;
+}
+
+/* vchar.c:37-135 vchar (TRUEFAST=0, no smoother, no group velocity) */
+static int vchar(const oracle_ctx *c, const double *pr, const state_t *q, int dir, const geom_t *g,
+                 double *vmax, double *vmin) {
+  double gam = c->p.gam;
+  double bsq = DOT(q->bcon, q->bcov);
+  double rho = pr[RHO], u = pr[UU];
+  double WW = rho + gam * u, cs2, EF, va2, cms2;
+  int fail = 0;
+  if (WW > 0.0) cs2 = gam * (gam - 1.) * u / WW;
+  else WW = cs2 = 0.0;
+  EF = bsq + WW;
+  if (EF > 0.0) va2 = bsq / EF;
+  else EF = va2 = 0.0;
+  cms2 = cs2 + va2 - cs2 * va2;
+  if (cms2 < 0.0) {
+    if (cms2 / (cs2 + va2) > -NUMEPSILON * 100.0) cms2 = 0.0;
+    else fail = 1;
+  }
+  if (cms2 > 1.) {
+    if (cms2 < 1.0 + NUMEPSILON * 100.0) cms2 = 1.0;
+    else fail = 1;
+  }
+  if (fail) return 1;
+  return simplefast(dir, g, q, cms2, vmin, vmax);
+}
+
+/* step_ch.c:1095-1148 cminmax_calc (HYPERHLL=0) */
+static void cminmax_calc(double cmin_l, double cmin_r, double cmax_l, double cmax_r,
+                         double *cmin, double *cmax, double *ctop) {
+  double lmin = min_(cmin_l, cmin_r);
+  double lmax = max_(cmax_l, cmax_r);
+  lmin = fabs(max_(0., -lmin));
+  lmax = fabs(max_(0., lmax));
+  *cmin = lmin; *cmax = lmax; *ctop = max_(lmax, lmin);
+}
+
+/* fixup.c:1298-1354 limit_gamma; returns -1 if changed, 1 on failure */
+static int limit_gamma(double gammamax, double *pr, const geom_t *g) {
+  double realgammamax = gammamax;
+  if (realgammamax <= 1.0) return 0;
+  double gamma = gamma_calc(pr, g);
+  if ((gamma > realgammamax && (gamma != 1.0))) {
+    double pref = (realgammamax * realgammamax - 1.) / (gamma * gamma - 1.);
+    if (pref < 0.0) return 1;
+    double f = sqrt(pref);
+    pr[U1] *= f; pr[U2] *= f; pr[U3] *= f;
+    return -1;
+  }
+  return 0;
+}
+
+/* ---------------- reconstruction (interp.c) ---------------- */
+
+/* interp.c:106-149 slope_lim_3points */
+static double slope_lim_3points(int lim, double yl, double yc, double yr) {
+  double Dqm, Dqp, Dqc, s;
+  if (lim == OR_MC) {
+    Dqm = 2.0 * (yc - yl); Dqp = 2.0 * (yr - yc); Dqc = 0.5 * (yr - yl);
+    return MINMOD(Dqc, MINMOD(Dqm, Dqp));
+  } else if (lim == OR_VANL) {
+    Dqm = (yc - yl); Dqp = (yr - yc); s = Dqm * Dqp;
+    if (s <= 0.) return 0.;
+    return (2.0 * s / (Dqm + Dqp));
+  } else if (lim == OR_MINM) {
+    Dqm = (yc - yl); Dqp = (yr - yc);
+    return MINMOD(Dqm, Dqp);
+  } else if (lim == OR_NLIMCENT) return 0.5 * (yr - yl);
+  else if (lim == OR_NLIMUP) return (yc - yl);
+  else if (lim == OR_NLIMDOWN) return (yr - yc);
+  return 0.0; /* DONOR */
+}
+
+/* interp.c:157-205 csslope */
+static double csslope(int kk, double x1, double x2, double x3, double x4, double x5) {
+  double s, sm, sp, Dql, Dqlm, Dqlp, Dqp, Dqpp, Dqm, Dqmm, Dqc, Dqcm, Dqcp, Dqbp, Dqbm, alpha;
+  Dqp = (x4 - x3); Dqpp = (x5 - x4); Dqm = (x3 - x2); Dqmm = (x2 - x1);
+  Dqc = 0.5 * (x4 - x2); Dqcm = 0.5 * (x3 - x1); Dqcp = 0.5 * (x5 - x3);
+  s = 0.5 * (sign_(Dqp) + sign_(Dqm));
+  sp = 0.5 * (sign_(Dqpp) + sign_(Dqp));
+  sm = 0.5 * (sign_(Dqm) + sign_(Dqmm));
+  if (kk == RHO) alpha = 2.0;
+  else if ((kk >= U1) && (kk <= U3)) alpha = 1.25;
+  else if (kk == UU) alpha = 1.0;
+  else if ((kk >= B1) && (kk <= B3)) alpha = 1.25;
+  else alpha = 2.0;
+  Dql = alpha * min_(fabs(Dqp), fabs(Dqm));
+  Dqlp = alpha * min_(fabs(Dqpp), fabs(Dqp));
+  Dqlm = alpha * min_(fabs(Dqm), fabs(Dqmm));
+  Dqbp = sp * min_(Dqlp, fabs(Dqcp));
+  Dqbm = sm * min_(Dqlm, fabs(Dqcm));
+  /* FOURTHIRD, SIXTH are long double constants (global.h:54-55): the
+     difference is formed in long double and rounded to double by fabs() */
+  return s * min_(fabs((double)((1.3333333333333333333333333333333333L) * Dqc -
+                                (0.1666666666666666666666666666666666L) * (Dqbp + Dqbm))), Dql);
+}
+
+/* interp.c:449-538 para4 with PARA2LIM=MC */
+static void para4(double x1, double x2, double x3, double x4, double x5, double *lout, double *rout) {
+  double y[5], dq[5];
+  double Dqm, Dqc, Dqp, aDqm, aDqp, aDqc, s, l, r, qa, qd, qe;
+  y[0] = x1; y[1] = x2; y[2] = x3; y[3] = x4; y[4] = x5;
+  for (int i = 1; i < 4; i++) {
+    Dqm = 2.0 * (y[i] - y[i - 1]);
+    Dqp = 2.0 * (y[i + 1] - y[i]);
+    Dqc = 0.5 * (y[i + 1] - y[i - 1]);
+    aDqm = fabs(Dqm); aDqp = fabs(Dqp); aDqc = fabs(Dqc);
+    s = Dqm * Dqp;
+    if (s <= 0.) dq[i] = 0.;
+    else dq[i] = min_(aDqc, min_(aDqm, aDqp)) * sign_(Dqc);
+  }
+  l = 0.5 * (y[2] + y[1]) - (dq[2] - dq[1]) / 8.0;
+  r = 0.5 * (y[3] + y[2]) - (dq[3] - dq[2]) / 8.0;
+  l = max_(min_(y[2], y[1]), l);
+  l = min_(max_(y[2], y[1]), l);
+  r = max_(min_(y[2], y[3]), r);
+  r = min_(max_(y[2], y[3]), r);
+  qa = (r - y[2]) * (y[2] - l);
+  qd = (r - l);
+  qe = 8.0 * (y[2] - 0.5 * (l + r));
+  if (qa <= 0.) { l = y[2]; r = y[2]; }
+  else {
+    if (qd * (qd - qe) < 0.0) l = 3.0 * y[2] - 2.0 * r;
+    if (qd * (qd + qe) < 0.0) r = 3.0 * y[2] - 2.0 * l;
+  }
+  *lout = l; *rout = r;
+}
+
+/* interp.c:8-44 slope_lim (LIMADJUST=0, WHICHPARA=PARA4) */
+void or_slope_lim(int lim, int k, double yll, double yl, double yc, double yr, double yrr,
+                  double *dq, double *left, double *right) {
+  if (lim == OR_PARA) para4(yll, yl, yc, yr, yrr, left, right);
+  else if (lim == OR_CSSLOPE) *dq = csslope(k, yll, yl, yc, yr, yrr);
+  else *dq = slope_lim_3points(lim, yl, yc, yr);
+}
+
+/* ---------------- fluxes (step_ch.c) ---------------- */
+
+/* step_ch.c:741-1082 fluxcalc */
+int or_fluxcalc(oracle_ctx *c, int pr_sel, int dir, double *ndt) {
+  const double *pr = c->arr[pr_sel];
+  double *F = c->arr[dir == 1 ? OR_F1 : OR_F2];
+  double *dq = c->arr[dir == 1 ? OR_DQ1 : OR_DQ2];
+  double *pleft = c->arr[OR_PLEFT], *pright = c->arr[OR_PRIGHT];
+  int idel = (dir == 1), jdel = (dir == 2), face = (dir == 1) ? FACE1 : FACE2;
+  int n1 = c->p.n1, n2 = c->p.n2, lim = c->p.lim;
+  double dxd = (dir == 1) ? c->p.dx1 : c->p.dx2;
+  double hllfrac = (c->p.fluxmethod == OR_HLL) ? 1.0 : 0.0;
+  double laxffrac = (c->p.fluxmethod == OR_LAXF) ? 1.0 : 0.0;
+  int fail = 0;
+
+  /* DQZLOOP PLOOP slope_lim (step_ch.c:812-823) */
+#pragma omp parallel for schedule(static)
+  for (int i = -1; i <= n1; i++)
+    for (int j = -1; j <= n2; j++)
+      for (int k = 0; k < NPR; k++)
+        or_slope_lim(lim, k, PA(c, pr, i - 2 * idel, j - 2 * jdel)[k], PA(c, pr, i - idel, j - jdel)[k],
+                     PA(c, pr, i, j)[k], PA(c, pr, i + idel, j + jdel)[k], PA(c, pr, i + 2 * idel, j + 2 * jdel)[k],
+                     &PA(c, dq, i, j)[k], &PA(c, pleft, i, j)[k], &PA(c, pright, i, j)[k]);
+
+  double mindt = 1.e9;
+  /* FZLOOP(-jdel,-idel) (step_ch.c:836-1069) */
+#pragma omp parallel for schedule(static) reduction(min : mindt) reduction(| : fail)
+  for (int i = -jdel; i <= n1; i++)
+    for (int j = -idel; j <= n2; j++) {
+      double p_l[NPR], p_r[NPR], F_l[NPR], F_r[NPR], U_l[NPR], U_r[NPR];
+      double cmax_l, cmax_r, cmin_l, cmin_r, cmax, cmin, ctop, dtij;
+      state_t sl, sr;
+      geom_t g;
+      for (int k = 0; k < NPR; k++) {
+        if (lim != OR_PARA) {
+          p_l[k] = PA(c, pr, i - idel, j - jdel)[k] + 0.5 * PA(c, dq, i - idel, j - jdel)[k];
+          p_r[k] = PA(c, pr, i, j)[k] - 0.5 * PA(c, dq, i, j)[k];
+        } else {
+          p_l[k] = PA(c, pright, i - idel, j - jdel)[k];
+          p_r[k] = PA(c, pleft, i, j)[k];
+        }
+      }
+      get_geometry(c, i, j, face, &g);
+      if (dir == 1) p_l[B1] = p_r[B1] = 0.5 * (p_l[B1] + p_r[B1]);
+      else p_l[B2] = p_r[B2] = 0.5 * (p_l[B2] + p_r[B2]);
+      if (limit_gamma(c->p.gammamax, p_l, &g) >= 1) fail |= 1;
+      if (limit_gamma(c->p.gammamax, p_r, &g) >= 1) fail |= 1;
+      get_state(p_l, &g, &sl);
+      get_state(p_r, &g, &sr);
+      primtoflux(c, p_l, &sl, dir, &g, F_l);
+      primtoflux(c, p_r, &sr, dir, &g, F_r);
+      primtoflux(c, p_l, &sl, 0, &g, U_l);
+      primtoflux(c, p_r, &sr, 0, &g, U_r);
+      if (vchar(c, p_l, &sl, dir, &g, &cmax_l, &cmin_l)) fail |= 1;
+      if (vchar(c, p_r, &sr, dir, &g, &cmax_r, &cmin_r)) fail |= 1;
+      cminmax_calc(cmin_l, cmin_r, cmax_l, cmax_r, &cmin, &cmax, &ctop);
+      double *Fz = PA(c, F, i, j);
+      if (cmax + cmin != 0.0) {
+        for (int k = 0; k < NPR; k++)
+          Fz[k] = hllfrac * ((cmax * F_l[k] + cmin * F_r[k] - cmax * cmin * (U_r[k] - U_l[k])) / (cmax + cmin)) +
+                  laxffrac * (0.5 * (F_l[k] + F_r[k] - ctop * (U_r[k] - U_l[k])));
+      } else {
+        for (int k = 0; k < NPR; k++) Fz[k] = (0.5 * (F_l[k] + F_r[k] - ctop * (U_r[k] - U_l[k])));
+      }
+      cmax = max_(cmax, cmin);
+      dtij = c->p.cour * dxd / cmax;
+      if (dtij < mindt) mindt = dtij;
+    }
+  *ndt = mindt;
+  if (fail) c->failed = 1;
+  return fail;
+}
+
+/* fixup.c:1357-1410 fix_flux (single rank: mycpupos=0, ncpux=1) */
+void or_fix_flux(oracle_ctx *c) {
+  double *F1 = c->arr[OR_F1], *F2 = c->arr[OR_F2];
+  int n1 = c->p.n1, n2 = c->p.n2;
+  if (c->p.bc[OR_X2DN] == OR_POLARAXIS) {
+    for (int i = -1; i < n1; i++) {
+      PA(c, F1, i, -1)[B2] = -PA(c, F1, i, 0)[B2];
+      for (int k = 0; k < NPR; k++) if (k != U2) PA(c, F2, i, 0)[k] = 0.;
+    }
+  }
+  if (c->p.bc[OR_X2UP] == OR_POLARAXIS) {
+    for (int i = -1; i < n1; i++) {
+      PA(c, F1, i, n2)[B2] = -PA(c, F1, i, n2 - 1)[B2];
+      for (int k = 0; k < NPR; k++) if (k != U2) PA(c, F2, i, n2)[k] = 0.;
+    }
+  }
+  for (int j = -NG; j < n2 + NG; j++)
+    for (int i = -NG; i < n1 + NG; i++)
+      if ((i >= 0) && (i < n1) && (abs(j - (n2 - 1) / 2) <= 0))
+        for (int k = 0; k < NPR; k++) if (k != U2) PA(c, F2, i, j)[k] = 0.;
+  if (c->p.bc[OR_X1DN] == OR_OUTFLOW)
+    for (int j = -1; j < n2; j++) if (PA(c, F1, 0, j)[RHO] > 0) PA(c, F1, 0, j)[RHO] = 0;
+  if (c->p.bc[OR_X1UP] == OR_OUTFLOW)
+    for (int j = -1; j < n2; j++) if (PA(c, F1, n1, j)[RHO] < 0) PA(c, F1, n1, j)[RHO] = 0;
+}
+
+/* step_ch.c:1159-1427 flux_ct (FLUXCTTOTH/FLUXCD/FLUXCTHLL, CORNGDETVERSION=1) */
+int or_flux_ct(oracle_ctx *c) {
+  double *F1 = c->arr[OR_F1], *F2 = c->arr[OR_F2], *emf = c->arr[OR_EMF];
+  int n1 = c->p.n1, n2 = c->p.n2;
+  if (c->p.fluxb == OR_FLUXCTHLL) return 0;
+#define EMF(i, j) emf[zi(c, (i), (j))]
+  if (c->p.fluxb == OR_FLUXCTTOTH) {
+    for (int i = 0; i <= n1; i++)
+      for (int j = 0; j <= n2; j++) {
+        double ec = c->gdet[(size_t)zi(c, i, j) * 4 + CORN];
+        double e1 = c->gdet[(size_t)zi(c, i, j) * 4 + FACE1];
+        double e2 = c->gdet[(size_t)zi(c, i, j - 1) * 4 + FACE1];
+        double e3 = c->gdet[(size_t)zi(c, i, j) * 4 + FACE2];
+        double e4 = c->gdet[(size_t)zi(c, i - 1, j) * 4 + FACE2];
+        EMF(i, j) = 0.25 * (ec) * (PA(c, F1, i, j)[B2] / (e1) + PA(c, F1, i, j - 1)[B2] / (e2) -
+                                   PA(c, F2, i, j)[B1] / (e3) - PA(c, F2, i - 1, j)[B1] / (e4));
+      }
+    for (int i = 0; i <= n1; i++)
+      for (int j = 0; j <= n2 - 1; j++) {
+        PA(c, F1, i, j)[B1] = 0.;
+        PA(c, F1, i, j)[B2] = 0.5 * (EMF(i, j) + EMF(i, j + 1));
+      }
+    for (int i = 0; i <= n1 - 1; i++)
+      for (int j = 0; j <= n2; j++) {
+        PA(c, F2, i, j)[B1] = -0.5 * (EMF(i, j) + EMF(i + 1, j));
+        PA(c, F2, i, j)[B2] = 0.;
+      }
+  } else if (c->p.fluxb == OR_FLUXCD) {
+    for (int i = 0; i <= n1; i++)
+      for (int j = 0; j <= n2; j++)
+        EMF(i, j) = 0.125 * (-PA(c, F1, i, j)[B2] - PA(c, F1, i + 1, j)[B2] + PA(c, F2, i, j)[B1] + PA(c, F2, i, j + 1)[B1]);
+    for (int i = 0; i <= n1; i++)
+      for (int j = 0; j <= n2 - 1; j++) { PA(c, F1, i, j)[B1] = 0.0; PA(c, F1, i, j)[B2] = EMF(i, j); }
+    for (int i = 0; i <= n1 - 1; i++)
+      for (int j = 0; j <= n2; j++) { PA(c, F2, i, j)[B1] = -EMF(i, j); PA(c, F2, i, j)[B2] = 0.; }
+  }
+#undef EMF
+  return 0;
+}
+
+/* ---------------- sources (phys.c) ---------------- */
+
+/* phys.c:475-510 source + :520-619 source_conn (WITHGDET, cooling=0) */
+static void source(const oracle_ctx *c, const double *pr, const geom_t *g, double *dU) {
+  state_t q;
+  double mhd[NDIM][NDIM], dUconn[NPR];
+  const double *cn = c->conn + (size_t)zi(c, g->i, g->j) * 64;
+  get_state(pr, g, &q);
+  for (int j = 0; j < NDIM; j++) mhd_calc_0(c, pr, j, &q, mhd[j]);
+  for (int k = 0; k < NPR; k++) dUconn[k] = 0;
+  for (int l = 0; l < NDIM; l++)
+    for (int j = 0; j < NDIM; j++)
+      for (int k = 0; k < NDIM; k++)
+        dUconn[UU + l] += mhd[j][k] * cn[(k * 4 + l) * 4 + j];
+  /* dUcomp[GEOMSOURCE]=e*dUconn, all other components 0; dU = sum over them */
+  for (int k = 0; k < NPR; k++) dU[k] = g->e[k] * dUconn[k];
+}
+
+/* ---------------- inversion (utoprim_2d_final.c, u2p_util.c) ---------------- */
+
+typedef struct {
+  double Bsq, QdotBsq, Qtsq, Qdotn, D, wglobal, gam, gammafail;
+  long nstroke;
+} u2p_t;
+
+/* utoprim_2d_final.c:356-370 vsq_calc */
+static double vsq_calc(const u2p_t *s, double W) {
+  double Wsq = W * W;
+  double Xsq = (s->Bsq + W) * (s->Bsq + W);
+  return ((Wsq * s->Qtsq + s->QdotBsq * (s->Bsq + 2. * W)) / (Wsq * Xsq));
+}
+
+/* utoprim_2d_final.c:1082-1155 func_vsq2 with pressure_W_vsq :447, dpdW_calc_vsq :465,
+   dpdvsq_calc :479 (GAMMA=gam) */
+static void func_vsq2(const u2p_t *s, const double x[2], double dx[2], double *f, double *df) {
+  double gam = s->gam;
+  double W = x[0], vsq = x[1], Wsq = W * W;
+  double gtmp = 1. - vsq;
+  double p_tmp = (gam - 1.) * (W * gtmp - s->D * sqrt(gtmp)) / gam;
+  double dPdW = (gam - 1.) * (1. - vsq) / gam;
+  double dPdvsq = (gam - 1.) * (0.5 * s->D / sqrt(1. - vsq) - W) / gam;
+  double Bsq = s->Bsq, QdotBsq = s->QdotBsq;
+  double t2 = -0.5 * Bsq + dPdvsq;
+  double t3 = Bsq + W;
+  double t4 = t3 * t3;
+  double t9 = 1 / Wsq;
+  double t11 = s->Qtsq - vsq * t4 + QdotBsq * (Bsq + 2.0 * W) * t9;
+  double t16 = QdotBsq * t9;
+  double t18 = -s->Qdotn - 0.5 * Bsq * (1.0 + vsq) + 0.5 * t16 - W + p_tmp;
+  double t21 = 1 / t3;
+  double t23 = 1 / W;
+  double t24 = t16 * t23;
+  double t25 = -1.0 + dPdW - t24;
+  double t35 = t25 * t3 + (Bsq - 2.0 * dPdvsq) * (QdotBsq + vsq * Wsq * W) * t9 * t23;
+  double t36 = 1 / t35;
+  dx[0] = -(t2 * t11 + t4 * t18) * t21 * t36;
+  double t40 = (vsq + t24) * t3;
+  dx[1] = -(-t25 * t11 - 2.0 * t40 * t18) * t21 * t36;
+  *df = -t11 * t11 - t18 * t18;
+  *f = -0.5 * (*df);
+}
+
+/* utoprim_2d_final.c:644-1012 general_newton_raphson, do_line_search=USE_LINE_SEARCH=-2
+   (no line search), NEWCONVERGE=1, CHECK_FOR_STALL=0, OPTIMIZED=1 */
+static int general_newton_raphson(u2p_t *s, double x[2]) {
+  double f, df, dx[2], x_old[2];
+  double errx;
+  int n_iter = 0, i_extra = 0, doing_extra = 0, keep_iterating = 1;
+  double VSQ_TOO_BIG = (1.0 - 1.0 / ((s->gammafail - 1.0) + 1.0));
+  double GAMMASQ_TOO_BIG = (s->gammafail * s->gammafail);
+  errx = 1.;
+  df = f = 1.;
+  while (keep_iterating) {
+    s->nstroke++;
+    func_vsq2(s, x, dx, &f, &df);
+    errx = 0.;
+    x_old[0] = x[0]; x_old[1] = x[1];
+    x[0] += dx[0]; x[1] += dx[1];
+    errx = (x[0] == 0.) ? fabs(dx[0]) : fabs(dx[0] / x[0]);
+    /* validate_x (utoprim_2d_final.c:617-633) */
+    x[0] = fabs(x[0]);
+    x[0] = (x[0] / s->wglobal > GAMMASQ_TOO_BIG) ? x_old[0] : x[0];
+    x[1] = (x[1] < 0.) ? 0.0 : x[1];
+    x[1] = (x[1] > 1.) ? VSQ_TOO_BIG : x[1];
+    if ((fabs(errx) <= NEWT_TOL) && (doing_extra == 0) && (EXTRA_NEWT_ITER > 0)) doing_extra = 1;
+    if (doing_extra == 1) i_extra++;
+    if (((fabs(errx) <= NEWT_TOL) && (doing_extra == 0)) || (i_extra > EXTRA_NEWT_ITER) ||
+        (n_iter >= (MAX_NEWT_ITER - 1)))
+      keep_iterating = 0;
+    n_iter++;
+  }
+  if (!isfinite(f) || !isfinite(df) || !isfinite(x[0]) || !isfinite(x[1])) return 1;
+  if (fabs(errx) > MIN_NEWT_TOL) return 1;
+  return 0;
+}
+
+/* utoprim_2d_final.c:136-396 Utoprim_new_body (+ find_root_2D_gen :514-576, x1_of_x0 :594) */
+static int Utoprim_new_body(const oracle_ctx *c, const double U[NPR], const geom_t *g, double prim[NPR],
+                            int *pflag, long *nstroke) {
+  double QdotB, Bcon[4], Bcov[4], Qcov[4], Qcon[4], ncov[4], ncon[4], Qsq, Qtcon[4];
+  double rho0, u, p, w, gammasq, gamma, gtmp, W_last, W, utsq, vsq;
+  int i, j, retval, i_increase;
+  u2p_t s;
+  double gam = c->p.gam, gammafail = c->p.gammafail;
+  double UTSQ_TOO_BIG = ((gammafail - 1.0) * (gammafail - 1.0));
+  double VSQ_TOO_BIG = (1.0 - 1.0 / ((gammafail - 1.0) + 1.0));
+  double GAMMASQ_TOO_BIG = (gammafail * gammafail);
+  s.gam = gam; s.gammafail = gammafail; s.nstroke = 0;
+
+  *pflag = UTOPRIMNOFAIL;
+  for (i = B1; i <= B3; i++) prim[i] = U[i];
+  Bcon[0] = 0.;
+  for (i = 1; i < 4; i++) Bcon[i] = U[B1 + i - 1];
+  for (i = 0; i < 4; i++) { Bcov[i] = 0.; for (j = 0; j < 4; j++) Bcov[i] += g->gcov[i][j] * Bcon[j]; }
+  for (i = 0; i < 4; i++) Qcov[i] = U[UU + i];
+  for (i = 0; i < 4; i++) { Qcon[i] = 0.; for (j = 0; j < 4; j++) Qcon[i] += g->gcon[i][j] * Qcov[j]; }
+  s.Bsq = 0.;
+  for (i = 1; i < 4; i++) s.Bsq += Bcon[i] * Bcov[i];
+  QdotB = 0.;
+  for (i = 0; i < 4; i++) QdotB += Qcov[i] * Bcon[i];
+  s.QdotBsq = QdotB * QdotB;
+  {
+    double lapse = sqrt(-1. / g->gcon[0][0]);
+    ncov[0] = -lapse; ncov[1] = 0.; ncov[2] = 0.; ncov[3] = 0.;
+  }
+  for (i = 0; i < 4; i++) { ncon[i] = 0.; for (j = 0; j < 4; j++) ncon[i] += g->gcon[i][j] * ncov[j]; }
+  s.Qdotn = Qcon[0] * ncov[0];
+  Qsq = 0.;
+  for (i = 0; i < 4; i++) Qsq += Qcov[i] * Qcon[i];
+  s.Qtsq = Qsq + s.Qdotn * s.Qdotn;
+  s.D = U[RHO];
+
+  utsq = 0.;
+  for (i = 1; i < 4; i++)
+    for (j = 1; j < 4; j++) utsq += g->gcov[i][j] * prim[U1 + i - 1] * prim[U1 + j - 1];
+  if ((utsq < 0.) && (fabs(utsq) < MAXNEGUTSQ)) utsq = 0.0;
+  if (utsq < 0. || utsq > UTSQ_TOO_BIG) { *pflag = UTOPRIMFAILCONVUTSQ; return 1; }
+
+  gammasq = 1. + utsq;
+  gamma = sqrt(gammasq);
+  rho0 = s.D / gamma;
+  u = prim[UU];
+  p = (gam - 1.) * u;
+  w = rho0 + u + p;
+  W_last = w * gammasq;
+  i_increase = 0;
+  while (((W_last * W_last * W_last * (W_last + 2. * s.Bsq) - s.QdotBsq * (2. * W_last + s.Bsq)) <=
+          W_last * W_last * (s.Qtsq - s.Bsq * s.Bsq)) &&
+         (i_increase < 10)) {
+    W_last *= 10.;
+    i_increase++;
+  }
+  s.wglobal = w;
+  {
+    double x[2], vsq0;
+    x[0] = fabs(W_last);
+    vsq0 = fabs(vsq_calc(&s, W_last));
+    x[1] = (vsq0 > 1.) ? VSQ_TOO_BIG : vsq0;
+    retval = general_newton_raphson(&s, x);
+    W = x[0];
+  }
+  *nstroke += s.nstroke;
+  if ((retval != 0) || (W == FAIL_VAL)) { *pflag = retval * 100 + 1; return retval; }
+  else {
+    double Wtest = W / s.wglobal;
+    if (Wtest <= 0. || Wtest > GAMMASQ_TOO_BIG) { *pflag = UTOPRIMFAILCONVW; return retval; }
+  }
+  vsq = vsq_calc(&s, W);
+  if ((vsq < 0.) && (fabs(vsq) < MAXNEGVSQ)) vsq = 0.0;
+  if ((vsq > VSQ_TOO_BIG) || (vsq < 0.0)) { *pflag = UTOPRIMFAILCONVUTSQ2; return retval; }
+  gtmp = sqrt(1. - vsq);
+  gamma = 1. / gtmp;
+  rho0 = s.D * gtmp;
+  w = W * (1. - vsq);
+  p = (gam - 1.) * (w - rho0) / gam;
+  u = w - (rho0 + p);
+  if ((rho0 <= 0.) || (u < 0.)) {
+    if ((rho0 <= 0.) && (u >= 0.)) *pflag = UTOPRIMFAILRHONEG;
+    if ((rho0 > 0.) && (u < 0.)) *pflag = UTOPRIMFAILUNEG;
+    if ((rho0 <= 0.) && (u < 0.)) *pflag = UTOPRIMFAILRHOUNEG;
+  }
+  prim[RHO] = rho0;
+  prim[UU] = u;
+  for (i = 1; i < 4; i++) Qtcon[i] = Qcon[i] + ncon[i] * s.Qdotn;
+  for (i = 1; i < 4; i++) prim[U1 + i - 1] = gamma / (W + s.Bsq) * (Qtcon[i] + QdotB * Bcon[i] / W);
+  for (i = B1; i <= B3; i++) prim[i] = U[i];
+  return retval;
+}
+
+/* utoprim_2d_final.c:26-101 Utoprim_2d_final */
+static int Utoprim_2d_final(const oracle_ctx *c, const double U[NPR], const geom_t *g, double prim[NPR],
+                            int *pflag, long *nstroke) {
+  double U_tmp[NPR], prim_tmp[NPR], alpha;
+  int i, ret;
+  for (i = B1; i <= B3; i++) prim[i] = U[i];
+  alpha = 1.0 / sqrt(-g->gcon[0][0]);
+  for (i = RHO; i <= B3; i++) U_tmp[i] = alpha * U[i];
+  for (i = 0; i < B1; i++) prim_tmp[i] = prim[i];
+  for (i = B1; i <= B3; i++) prim_tmp[i] = alpha * prim[i];
+  ret = Utoprim_new_body(c, U_tmp, g, prim_tmp, pflag, nstroke);
+  if (ret == 0)
+    for (i = 0; i < B1; i++) prim[i] = prim_tmp[i];
+  return 0;
+}
+
+/* step_ch.c:468-552 Utoprimgen: UtoU(UEVOLVE->UNOTHING) then the GRMHD inversion */
+static int Utoprimgen(const oracle_ctx *c, const double *U, const geom_t *g, double *pr, int *pflag,
+                      long *nstroke) {
+  double Ug[NPR];
+  for (int k = 0; k < NPR; k++) Ug[k] = U[k] / g->e[k];
+  if (c->p.eomtype != OR_FFDE) Ug[UU] += -Ug[RHO];
+  return Utoprim_2d_final(c, Ug, g, pr, pflag, nstroke);
+}
+
+/* fixup.c:274-377 fixup1zone (floors from the precomputed table, then limit_gamma) */
+static int fixup1zone(const oracle_ctx *c, double *pr, const geom_t *g) {
+  if (c->p.eomtype == OR_GRMHD) {
+    const double *fl = c->floors + (size_t)zi(c, g->i, g->j) * 2;
+    double prfloor[2] = {fl[0], fl[1]};
+    if (prfloor[0] < c->p.rhominlimit) prfloor[0] = c->p.rhominlimit;
+    if (prfloor[1] < c->p.uuminlimit) prfloor[1] = c->p.uuminlimit;
+    if (prfloor[0] > pr[RHO]) pr[RHO] = prfloor[0];
+    if (prfloor[1] > pr[UU]) pr[UU] = prfloor[1];
+  }
+  if (limit_gamma(c->p.gammamax, pr, g) >= 1) return 1;
+  return 0;
+}
+
+/* advance.c:46-197 advance_standard (DOENO=0, FIXUPZONES=FIXUP1ZONE, STEPOVERNEGU=0) */
+int or_advance(oracle_ctx *c, int pi_sel, int pb_sel, const double *CUf, int pf_sel, const double *Cunew,
+               int timeorder, int numtimeorders, double *ndt) {
+  double ndt1, ndt2;
+  double *pi = c->arr[pi_sel], *pb = c->arr[pb_sel], *pf = c->arr[pf_sel];
+  double *ulast = c->arr[OR_ULAST], *unew = c->arr[OR_UNEW];
+  double *F1 = c->arr[OR_F1], *F2 = c->arr[OR_F2];
+  int n1 = c->p.n1, n2 = c->p.n2;
+  double idx1 = 1.0 / c->p.dx1, idx2 = 1.0 / c->p.dx2, dt = c->dt;
+  long nstroke = 0;
+  int fail = 0;
+  fail |= or_fluxcalc(c, pb_sel, 1, &ndt1);
+  fail |= or_fluxcalc(c, pb_sel, 2, &ndt2);
+  if (c->p.fixupflux) or_fix_flux(c);
+  or_flux_ct(c);
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : nstroke) reduction(| : fail)
+  for (int i = 0; i < n1; i++)
+    for (int j = 0; j < n2; j++) {
+      double Uf[NPR], Ui[NPR], dUgeom[NPR], dUriemann[NPR];
+      geom_t g;
+      state_t q;
+      double *pfz = PA(c, pf, i, j), *piz = PA(c, pi, i, j);
+      double *ul = PA(c, ulast, i, j), *un = PA(c, unew, i, j);
+      double pin[NPR];
+      for (int k = 0; k < NPR; k++) pin[k] = piz[k];
+      for (int k = 0; k < NPR; k++) pfz[k] = pin[k];
+      if (timeorder == 0) for (int k = 0; k < NPR; k++) ul[k] = un[k] = 0.0;
+      get_geometry(c, i, j, CENT, &g);
+      get_state(pin, &g, &q);
+      primtoflux(c, pin, &q, 0, &g, Ui);
+      {
+        double pbz[NPR];
+        for (int k = 0; k < NPR; k++) pbz[k] = PA(c, pb, i, j)[k];
+        source(c, pbz, &g, dUgeom);
+      }
+      /* flux2dUavg (advance.c:204-265) */
+      for (int k = 0; k < NPR; k++) {
+        double d1 = (-(PA(c, F1, i + 1, j)[k] - PA(c, F1, i, j)[k]) * idx1);
+        double d2 = (-(PA(c, F2, i, j + 1)[k] - PA(c, F2, i, j)[k]) * idx2);
+        dUriemann[k] = d1 + d2;
+      }
+      /* dUtoU (advance.c:273-286) */
+      for (int k = 0; k < NPR; k++)
+        ul[k] = Uf[k] = CUf[0] * Ui[k] + CUf[1] * ul[k] + CUf[2] * dt * (dUriemann[k] + dUgeom[k]);
+      for (int k = 0; k < NPR; k++)
+        un[k] += Cunew[0] * Ui[k] + Cunew[1] * dt * (dUriemann[k] + dUgeom[k]) + Cunew[2] * Uf[k];
+      int pfl;
+      if (timeorder == numtimeorders - 1) Utoprimgen(c, un, &g, pfz, &pfl, &nstroke);
+      else Utoprimgen(c, Uf, &g, pfz, &pfl, &nstroke);
+      c->pflag[(size_t)zi(c, i, j) * NUMPFLAGS + 0] = pfl;
+      if (fixup1zone(c, pfz, &g)) fail |= 1;
+    }
+  c->nstroke += nstroke;
+  *ndt = c->p.defcon * 1. / (1. / ndt1 + 1. / ndt2);
+  if (fail) c->failed = 1;
+  return fail;
+}
+
+/* fixup.c:1733-1841 inflow_check_rel4vel (dir=1, CENT: iin=-1, iout=N1) */
+static int inflow_check_rel4vel_x1(const oracle_ctx *c, double *pr, const geom_t *g) {
+  double ucon[NDIM];
+  int dofix = 0;
+  ucon_calc(pr, g, ucon);
+  if (((g->i <= -1) && (c->p.bc[OR_X1DN] == OR_OUTFLOW) && (ucon[1] > 0.)) ||
+      ((g->i >= c->p.n1) && (c->p.bc[OR_X1UP] == OR_OUTFLOW) && (ucon[1] < 0.)))
+    dofix = 1;
+  if (!dofix) return 0;
+  double gamma = gamma_calc(pr, g);
+  pr[U1] /= gamma; pr[U2] /= gamma; pr[U3] /= gamma;
+  double alpha = 1. / sqrt(-g->gcon[0][0]);
+  double betacon = g->gcon[0][1] * alpha * alpha;
+  pr[U1] = betacon / alpha;
+  double vsq = 0.;
+  for (int j = 1; j < NDIM; j++)
+    for (int k = 1; k < NDIM; k++) vsq += g->gcov[j][k] * pr[U1 + j - 1] * pr[U1 + k - 1];
+  if (vsq < 0.0) {
+    if (vsq > -NUMEPSILON * 100.0) vsq = 0.0;
+    else return 1;
+  }
+  if (vsq >= 1.0) vsq = 1.0 - 1.0 / (c->p.gammamax * c->p.gammamax);
+  gamma = 1. / sqrt(1. - vsq);
+  pr[U1] *= gamma; pr[U2] *= gamma; pr[U3] *= gamma;
+  return -1;
+}
+
+/* ghost fill: bounds.fishmon.c:13-355 (outflow EXTRAP=0 + polar axis + POLEDEATH)
+   and periodic wrap (the single-rank equivalent of the reference's MPI halo,
+   boundmpi.c:5-130: x1 faces first, then x2 faces over the full x1 range) */
+int or_bound_prim(oracle_ctx *c, int sel) {
+  double *pr = c->arr[sel];
+  int n1 = c->p.n1, n2 = c->p.n2, fail = 0;
+  /* x1 */
+  if (c->p.bc[OR_X1DN] == OR_PERIODIC) {
+    for (int j = 0; j < n2; j++)
+      for (int i = -NG; i < 0; i++) memcpy(PA(c, pr, i, j), PA(c, pr, i + n1, j), sizeof(double) * NPR);
+  } else if (c->p.bc[OR_X1DN] == OR_OUTFLOW) {
+    for (int j = 0; j < n2; j++) {
+      for (int i = -1; i >= -NG; i--) memcpy(PA(c, pr, i, j), PA(c, pr, 0, j), sizeof(double) * NPR);
+      for (int i = -1; i >= -NG; i--) {
+        geom_t g;
+        get_geometry(c, i, j, CENT, &g);
+        if (inflow_check_rel4vel_x1(c, PA(c, pr, i, j), &g) >= 1) fail = 1;
+        if (limit_gamma(c->p.gammamax, PA(c, pr, i, j), &g) >= 1) fail = 1;
+      }
+    }
+  }
+  if (c->p.bc[OR_X1UP] == OR_PERIODIC) {
+    for (int j = 0; j < n2; j++)
+      for (int i = n1; i < n1 + NG; i++) memcpy(PA(c, pr, i, j), PA(c, pr, i - n1, j), sizeof(double) * NPR);
+  } else if (c->p.bc[OR_X1UP] == OR_OUTFLOW) {
+    for (int j = 0; j < n2; j++) {
+      for (int i = n1; i <= n1 - 1 + NG; i++) memcpy(PA(c, pr, i, j), PA(c, pr, n1 - 1, j), sizeof(double) * NPR);
+      for (int i = n1; i <= n1 - 1 + NG; i++) {
+        geom_t g;
+        get_geometry(c, i, j, CENT, &g);
+        if (inflow_check_rel4vel_x1(c, PA(c, pr, i, j), &g) >= 1) fail = 1;
+        if (limit_gamma(c->p.gammamax, PA(c, pr, i, j), &g) >= 1) fail = 1;
+      }
+    }
+  }
+  /* x2 */
+  if (c->p.bc[OR_X2DN] == OR_PERIODIC) {
+    for (int i = -NG; i < n1 + NG; i++)
+      for (int j = -NG; j < 0; j++) memcpy(PA(c, pr, i, j), PA(c, pr, i, j + n2), sizeof(double) * NPR);
+  } else if (c->p.bc[OR_X2DN] == OR_POLARAXIS) {
+    for (int i = -NG; i <= n1 - 1 + NG; i++)
+      for (int j = -NG; j <= -1; j++) memcpy(PA(c, pr, i, j), PA(c, pr, i, 0 + (0 - j - 1)), sizeof(double) * NPR);
+  }
+  if (c->p.bc[OR_X2UP] == OR_PERIODIC) {
+    for (int i = -NG; i < n1 + NG; i++)
+      for (int j = n2; j < n2 + NG; j++) memcpy(PA(c, pr, i, j), PA(c, pr, i, j - n2), sizeof(double) * NPR);
+  } else if (c->p.bc[OR_X2UP] == OR_POLARAXIS) {
+    for (int i = -NG; i <= n1 - 1 + NG; i++)
+      for (int j = n2; j <= n2 - 1 + NG; j++)
+        memcpy(PA(c, pr, i, j), PA(c, pr, i, (n2 - 1) + ((n2 - 1) - j + 1)), sizeof(double) * NPR);
+  }
+  if (c->p.poledeath) {
+    if (c->p.bc[OR_X2DN] == OR_POLARAXIS)
+      for (int i = -NG; i <= n1 - 1 + NG; i++)
+        for (int j = -NG; j < 0 + 1; j++) { PA(c, pr, i, j)[U2] *= 0; PA(c, pr, i, j)[B2] *= 0.; }
+    if (c->p.bc[OR_X2UP] == OR_POLARAXIS)
+      for (int i = -NG; i <= n1 - 1 + NG; i++)
+        for (int j = n2 - 1; j <= n2 - 1 + NG; j++) { PA(c, pr, i, j)[U2] *= 0.; PA(c, pr, i, j)[B2] *= 0.; }
+  }
+  if (fail) c->failed = 1;
+  return fail;
+}
+
+/* boundsint.c:4-60 bound_pflag (FLAGUTOPRIMFAIL only): x1 outflow copies over
+   j in [-3,N2+2], then polar reflection over the full i range; periodic
+   wraps x1 (interior j) then x2 (all i), as the primitives do. */
+static void bound_pflag(oracle_ctx *c) {
+  int n1 = c->p.n1, n2 = c->p.n2;
+#define PF(i, j) c->pflag[(size_t)zi(c, (i), (j)) * NUMPFLAGS]
+  if (c->p.bc[OR_X1DN] == OR_OUTFLOW)
+    for (int j = -NG; j <= n2 - 1 + NG; j++)
+      for (int i = -1; i >= -NG; i--) PF(i, j) = PF(0, j);
+  if (c->p.bc[OR_X1UP] == OR_OUTFLOW)
+    for (int j = -NG; j <= n2 - 1 + NG; j++)
+      for (int i = n1; i <= n1 - 1 + NG; i++) PF(i, j) = PF(n1 - 1, j);
+  if (c->p.bc[OR_X1DN] == OR_PERIODIC)
+    for (int j = 0; j < n2; j++) {
+      for (int i = -NG; i < 0; i++) PF(i, j) = PF(i + n1, j);
+      for (int i = n1; i < n1 + NG; i++) PF(i, j) = PF(i - n1, j);
+    }
+  if (c->p.bc[OR_X2DN] == OR_POLARAXIS)
+    for (int i = -NG; i <= n1 - 1 + NG; i++)
+      for (int j = -NG; j <= -1; j++) PF(i, j) = PF(i, 0 + (0 - j - 1));
+  if (c->p.bc[OR_X2UP] == OR_POLARAXIS)
+    for (int i = -NG; i <= n1 - 1 + NG; i++)
+      for (int j = n2; j <= n2 - 1 + NG; j++) PF(i, j) = PF(i, (n2 - 1) + ((n2 - 1) - j + 1));
+  if (c->p.bc[OR_X2DN] == OR_PERIODIC)
+    for (int i = -NG; i < n1 + NG; i++) {
+      for (int j = -NG; j < 0; j++) PF(i, j) = PF(i, j + n2);
+      for (int j = n2; j < n2 + NG; j++) PF(i, j) = PF(i, j - n2);
+    }
+#undef PF
+}
+
+/* fixup.c:527-920 fixup_utoprim (UTOPRIMADJUST=UTOPRIMAVG, ORDERINDEPENDENT=1,
+   HANDLEUNEG=0, TODONOGOOD=NOGOODAVERAGE, STEPOVERNEGU=0): every zone whose
+   inversion failed takes the average of its good neighbours from a copy made
+   before the loop, its gamma capped at its own old gamma (or GAMMAMAX); then
+   its flag becomes UTOPRIMFAILFIXED. */
+static int fixup_utoprim(oracle_ctx *c, double *pv) {
+  int n1 = c->p.n1, n2 = c->p.n2, nfixed = 0;
+  /* cp: the reference's prc copy (fixup.c:544-552) */
+  double *cp = (double *)malloc(sizeof(double) * (size_t)c->nz * NPR);
+  memcpy(cp, pv, sizeof(double) * (size_t)c->nz * NPR);
+#define PF(i, j) c->pflag[(size_t)zi(c, (i), (j)) * NUMPFLAGS]
+#define PR(i, j, k) PA(c, cp, (i), (j))[k]
+#define GOOD(i, j) (PF((i), (j)) == UTOPRIMNOFAIL)
+#define AVG4_1(i, j, k) (0.25 * (PR(i, j + 1, k) + PR(i, j - 1, k) + PR(i - 1, j, k) + PR(i + 1, j, k)))
+#define AVG4_2(i, j, k) (0.25 * (PR(i + 1, j + 1, k) + PR(i + 1, j - 1, k) + PR(i - 1, j + 1, k) + PR(i - 1, j - 1, k)))
+  for (int i = 0; i < n1; i++)
+    for (int j = 0; j < n2; j++) {
+      int mypflag = PF(i, j);
+      if (mypflag == 0) continue;
+      double *pz = PA(c, pv, i, j);
+      geom_t g;
+      get_geometry(c, i, j, CENT, &g);
+      double gamma = gamma_calc(pz, &g);
+      if (gamma > c->p.gammamax) gamma = c->p.gammamax;
+      int k;
+      if (GOOD(i, j + 1) && GOOD(i, j - 1) && GOOD(i + 1, j) && GOOD(i - 1, j)) {
+        for (k = 0; k <= U3; k++) pz[k] = AVG4_1(i, j, k);
+      } else if (GOOD(i + 1, j + 1) && GOOD(i + 1, j - 1) && GOOD(i - 1, j + 1) && GOOD(i - 1, j - 1)) {
+        for (k = 0; k <= U3; k++) pz[k] = AVG4_2(i, j, k);
+      } else if (GOOD(i, j + 1) && GOOD(i, j - 1)) {
+        for (k = 0; k <= U3; k++) pz[k] = (0.5 * (PR(i, j + 1, k) + PR(i, j - 1, k)));
+      } else if (GOOD(i + 1, j) && GOOD(i - 1, j)) {
+        for (k = 0; k <= U3; k++) pz[k] = (0.5 * (PR(i + 1, j, k) + PR(i - 1, j, k)));
+      } else if (GOOD(i + 1, j + 1) && GOOD(i - 1, j - 1)) {
+        for (k = 0; k <= U3; k++) pz[k] = (0.5 * (PR(i + 1, j + 1, k) + PR(i - 1, j - 1, k)));
+      } else if (GOOD(i + 1, j - 1) && GOOD(i - 1, j + 1)) {
+        for (k = 0; k <= U3; k++) pz[k] = (0.5 * (PR(i + 1, j - 1, k) + PR(i - 1, j + 1, k)));
+      } else if (GOOD(i + 1, j + 1)) {
+        for (k = 0; k <= U3; k++) pz[k] = PR(i + 1, j + 1, k);
+      } else if (GOOD(i + 1, j)) {
+        for (k = 0; k <= U3; k++) pz[k] = PR(i + 1, j, k);
+      } else if (GOOD(i + 1, j - 1)) {
+        for (k = 0; k <= U3; k++) pz[k] = PR(i + 1, j - 1, k);
+      } else if (GOOD(i, j - 1)) {
+        for (k = 0; k <= U3; k++) pz[k] = PR(i, j - 1, k);
+      } else if (GOOD(i - 1, j - 1)) {
+        for (k = 0; k <= U3; k++) pz[k] = PR(i - 1, j - 1, k);
+      } else if (GOOD(i - 1, j)) {
+        for (k = 0; k <= U3; k++) pz[k] = PR(i - 1, j, k);
+      } else if (GOOD(i - 1, j + 1)) {
+        for (k = 0; k <= U3; k++) pz[k] = PR(i - 1, j + 1, k);
+      } else if (GOOD(i, j + 1)) {
+        for (k = 0; k <= U3; k++) pz[k] = PR(i, j + 1, k);
+      } else {
+        for (k = 0; k <= U3; k++) pz[k] = 0.5 * (AVG4_1(i, j, k) + AVG4_2(i, j, k));
+      }
+      if (limit_gamma(gamma, pz, &g) >= 1) c->failed = 1;
+      /* accounting (fixup.c:870-915): flag becomes UTOPRIMFAILFIXED */
+      PF(i, j) = -1;
+      nfixed++;
+    }
+#undef AVG4_1
+#undef AVG4_2
+#undef GOOD
+#undef PR
+#undef PF
+  free(cp);
+  return nfixed;
+}
+
+/* fixup.c:37-94 post_fixup: bound_pflag then fixup_utoprim (GRMHD);
+   returns the number of zones repaired */
+int or_post_fixup(oracle_ctx *c, int sel) {
+  bound_pflag(c);
+  if (c->p.eomtype != OR_GRMHD) return 0;
+  return fixup_utoprim(c, c->arr[sel]);
+}
+
+/* step_ch.c:35-188 step_ch_simplempi + setup_rktimestep :365-460 */
+int or_step(oracle_ctx *c) {
+  double CUf[4][3], Cunew[4][3], ndt = 0;
+  int pii[4], pbb[4], pff[4], nto = c->p.timeorder, fail = 0;
+  memset(CUf, 0, sizeof(CUf)); memset(Cunew, 0, sizeof(Cunew));
+  if (nto == 4) {
+    double cu[4][3] = {{1.0, 0.0, 0.5}, {1.0, 0.0, 0.5}, {1.0, 0.0, 1.0}, {1.0, 0.0, 1.0}};
+    double cn[4][3] = {{1.0, 1.0 / 6.0, 0.0}, {0.0, 1.0 / 3.0, 0.0}, {0.0, 1.0 / 3.0, 0.0}, {0.0, 1.0 / 6.0, 0.0}};
+    memcpy(CUf, cu, sizeof(cu)); memcpy(Cunew, cn, sizeof(cn));
+    int a[4] = {OR_P, OR_P, OR_P, OR_P}, b[4] = {OR_P, OR_PK0, OR_PK1, OR_PK0}, f[4] = {OR_PK0, OR_PK1, OR_PK0, OR_P};
+    memcpy(pii, a, sizeof(a)); memcpy(pbb, b, sizeof(b)); memcpy(pff, f, sizeof(f));
+  } else if (nto == 3) {
+    double cu[3][3] = {{1.0, 0.0, 1.0}, {3.0 / 4.0, 1.0 / 4.0, 1.0 / 4.0}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+    double cn[3][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 0.0}, {0.0, 0.0, 1.0}};
+    memcpy(CUf, cu, sizeof(cu)); memcpy(Cunew, cn, sizeof(cn));
+    int a[3] = {OR_P, OR_P, OR_P}, b[3] = {OR_P, OR_PK0, OR_PK1}, f[3] = {OR_PK0, OR_PK1, OR_P};
+    memcpy(pii, a, sizeof(a)); memcpy(pbb, b, sizeof(b)); memcpy(pff, f, sizeof(f));
+  } else if (nto == 2) {
+    double cu[2][3] = {{1.0, 0.0, 0.5}, {1.0, 0.0, 1.0}};
+    double cn[2][3] = {{0.0, 0.0, 0.0}, {0.0, 0.0, 1.0}};
+    memcpy(CUf, cu, sizeof(cu)); memcpy(Cunew, cn, sizeof(cn));
+    int a[2] = {OR_P, OR_P}, b[2] = {OR_P, OR_PK0}, f[2] = {OR_PK0, OR_P};
+    memcpy(pii, a, sizeof(a)); memcpy(pbb, b, sizeof(b)); memcpy(pff, f, sizeof(f));
+  } else {
+    double cu[1][3] = {{1.0, 0.0, 1.0}};
+    double cn[1][3] = {{0.0, 0.0, 1.0}};
+    memcpy(CUf, cu, sizeof(cu)); memcpy(Cunew, cn, sizeof(cn));
+    pii[0] = pbb[0] = pff[0] = OR_P;
+    nto = 1;
+  }
+  for (int to = 0; to < nto; to++) {
+    fail |= or_advance(c, pii[to], pbb[to], CUf[to], pff[to], Cunew[to], to, nto, &ndt);
+    fail |= or_bound_prim(c, pff[to]);
+    or_post_fixup(c, pff[to]);
+    fail |= or_bound_prim(c, pff[to]);
+  }
+  c->t += c->dt;
+  if (ndt > c->p.safe * c->dt) ndt = c->p.safe * c->dt;
+  c->dt = ndt;
+  if (c->t + c->dt > c->tf) c->dt = c->tf - c->t;
+  return fail;
+}
+
+/* ---------------- single-zone entry points ---------------- */
+
+int or_utoprim_2d_final(oracle_ctx *c, int i, int j, const double *Ugeomfree, double *pr, int *pflag) {
+  geom_t g;
+  long ns = 0;
+  get_geometry(c, i, j, CENT, &g);
+  Utoprim_2d_final(c, Ugeomfree, &g, pr, pflag, &ns);
+  return (int)ns;
+}
+
+int or_utoprimgen(oracle_ctx *c, int i, int j, const double *U, double *pr, int *pflag) {
+  geom_t g;
+  long ns = 0;
+  get_geometry(c, i, j, CENT, &g);
+  Utoprimgen(c, U, &g, pr, pflag, &ns);
+  return (int)ns;
+}
+
+int or_primtoU(oracle_ctx *c, int i, int j, int loc, const double *pr, double *U) {
+  return or_primtoflux(c, i, j, loc, pr, 0, U);
+}
+
+int or_primtoflux(oracle_ctx *c, int i, int j, int loc, const double *pr, int dir, double *F) {
+  geom_t g;
+  state_t q;
+  get_geometry(c, i, j, loc, &g);
+  get_state(pr, &g, &q);
+  primtoflux(c, pr, &q, dir, &g, F);
+  return 0;
+}
+
+int or_vchar(oracle_ctx *c, int i, int j, int loc, const double *pr, int dir, double *vmax, double *vmin) {
+  geom_t g;
+  state_t q;
+  get_geometry(c, i, j, loc, &g);
+  get_state(pr, &g, &q);
+  return vchar(c, pr, &q, dir, &g, vmax, vmin);
+}
+
+int or_source(oracle_ctx *c, int i, int j, const double *pr, double *dU) {
+  geom_t g;
+  get_geometry(c, i, j, CENT, &g);
+  source(c, pr, &g, dU);
+  return 0;
+}
+
+/* ---------------- context ---------------- */
+
+oracle_ctx *or_create(const oracle_params *prm) {
+  oracle_ctx *c = (oracle_ctx *)calloc(1, sizeof(oracle_ctx));
+  c->p = *prm;
+  c->N1M = prm->n1 + 2 * NG;
+  c->N2M = prm->n2 + 2 * NG;
+  c->nz = c->N1M * c->N2M;
+  c->gcov = (double *)calloc((size_t)c->nz * 64, sizeof(double));
+  c->gcon = (double *)calloc((size_t)c->nz * 64, sizeof(double));
+  c->gdet = (double *)calloc((size_t)c->nz * 4, sizeof(double));
+  c->conn = (double *)calloc((size_t)c->nz * 64, sizeof(double));
+  c->floors = (double *)calloc((size_t)c->nz * 2, sizeof(double));
+  for (int a = 0; a < 12; a++) c->arr[a] = (double *)calloc((size_t)c->nz * NPR, sizeof(double));
+  c->pflag = (int *)calloc((size_t)c->nz * NUMPFLAGS, sizeof(int));
+  c->tf = 1e30;
+#ifdef _OPENMP
+  if (prm->nthreads > 0) omp_set_num_threads(prm->nthreads);
+#endif
+  return c;
+}
+
+void or_destroy(oracle_ctx *c) {
+  if (!c) return;
+  free(c->gcov); free(c->gcon); free(c->gdet); free(c->conn); free(c->floors);
+  for (int a = 0; a < 12; a++) free(c->arr[a]);
+  free(c->pflag);
+  free(c);
+}
+
+void or_set_geometry(oracle_ctx *c, const double *gcov, const double *gcon, const double *gdet,
+                     const double *conn, const double *floors) {
+  memcpy(c->gcov, gcov, sizeof(double) * (size_t)c->nz * 64);
+  memcpy(c->gcon, gcon, sizeof(double) * (size_t)c->nz * 64);
+  memcpy(c->gdet, gdet, sizeof(double) * (size_t)c->nz * 4);
+  memcpy(c->conn, conn, sizeof(double) * (size_t)c->nz * 64);
+  memcpy(c->floors, floors, sizeof(double) * (size_t)c->nz * 2);
+}
+
+double *or_array(oracle_ctx *c, int sel) { return c->arr[sel]; }
+int *or_pflag(oracle_ctx *c) { return c->pflag; }
+void or_set_time(oracle_ctx *c, double t, double dt, double tf) { c->t = t; c->dt = dt; c->tf = tf; }
+void or_get_time(const oracle_ctx *c, double *t, double *dt) { *t = c->t; *dt = c->dt; }
+long or_nstroke(const oracle_ctx *c) { return c->nstroke; }
+int or_failed(const oracle_ctx *c) { return c->failed; }

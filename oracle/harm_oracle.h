@@ -1,0 +1,92 @@
+/*
+ * harm_oracle.h -- CPU restatement of HARM's conservative timestep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is the oracle the HIP path is checked
+ * against: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+ * leg may load it.  The product (harm_ramt_amd/) never links or calls it.
+ *
+ * Layout follows the reference exactly (defs.h:4-83, set_arrays.c:116-139):
+ *   primitive-shaped arrays  a[N1M][N2M][NPR]   N?M = N? + 6 (3 ghosts/side)
+ *   gcov, gcon               a[N1M][N2M][NPG][4][4]   (NPG: FACE1,FACE2,CORN,CENT)
+ *   gdet                     a[N1M][N2M][NPG]
+ *   conn                     a[N1M][N2M][4][4][4]     (CENT only)
+ *   floors                   a[N1M][N2M][2]           (RHO,UU floor at CENT;
+ *                                                      RHOMIN r^-2.7, UUMIN r^-2.7,
+ *                                                      fixup.c:1025-1028 precomputed)
+ * Index (i,j) with i in [-3, N1+2], j in [-3, N2+2], stored at (i+3, j+3).
+ */
+#ifndef HARM_ORACLE_H
+#define HARM_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* reference mnemonics (step_ch.h, global.h) */
+enum { OR_DONOR = 0, OR_VANL = 1, OR_MINM = 2, OR_MC = 3, OR_PARA = 4,
+       OR_CSSLOPE = 90, OR_NLIMCENT = 101, OR_NLIMUP = 102, OR_NLIMDOWN = 103 };
+enum { OR_LAXF = 0, OR_HLL = 1 };
+enum { OR_FLUXCTHLL = 0, OR_FLUXCTTOTH = 1, OR_FLUXCD = 2 };
+enum { OR_OUTFLOW = 0, OR_POLARAXIS = 4, OR_PERIODIC = 100 };
+enum { OR_X1UP = 0, OR_X1DN = 1, OR_X2UP = 2, OR_X2DN = 3 };
+enum { OR_GRMHD = 0, OR_FFDE = 1 };
+
+typedef struct {
+  int n1, n2;
+  int eomtype, lim, fluxmethod, fluxb, timeorder;
+  int bc[4];
+  int fixupflux;        /* FIXUPFLUX (step_ch.h:82) */
+  int poledeath;        /* POLEDEATH (bounds.fishmon.c:328) */
+  double gam, cour, safe, defcon, gammamax, gammafail;
+  double rhominlimit, uuminlimit;
+  double dx1, dx2;
+  int nthreads;         /* OpenMP threads, 0 = default */
+} oracle_params;
+
+typedef struct oracle_ctx oracle_ctx;
+
+oracle_ctx *or_create(const oracle_params *prm);
+void or_destroy(oracle_ctx *c);
+/* geometry tables in the reference layout (copied) */
+void or_set_geometry(oracle_ctx *c, const double *gcov, const double *gcon,
+                     const double *gdet, const double *conn, const double *floors);
+
+/* array selectors */
+enum { OR_P = 0, OR_PK0 = 1, OR_PK1 = 2, OR_F1 = 3, OR_F2 = 4, OR_ULAST = 5,
+       OR_UNEW = 6, OR_EMF = 7, OR_PLEFT = 8, OR_PRIGHT = 9, OR_DQ1 = 10, OR_DQ2 = 11 };
+double *or_array(oracle_ctx *c, int sel);
+int *or_pflag(oracle_ctx *c);
+void or_set_time(oracle_ctx *c, double t, double dt, double tf);
+void or_get_time(const oracle_ctx *c, double *t, double *dt);
+long or_nstroke(const oracle_ctx *c);
+int or_failed(const oracle_ctx *c);
+
+/* the path */
+int or_fluxcalc(oracle_ctx *c, int pr_sel, int dir, double *ndt);
+void or_fix_flux(oracle_ctx *c);
+int or_flux_ct(oracle_ctx *c);
+int or_advance(oracle_ctx *c, int pi_sel, int pb_sel, const double *CUf,
+               int pf_sel, const double *Cunew, int timeorder,
+               int numtimeorders, double *ndt);
+int or_bound_prim(oracle_ctx *c, int sel);
+int or_post_fixup(oracle_ctx *c, int sel);
+int or_step(oracle_ctx *c);
+
+/* single-zone entry points (for per-function tests) */
+int or_utoprim_2d_final(oracle_ctx *c, int i, int j, const double *Ugeomfree,
+                        double *pr, int *pflag);
+int or_utoprimgen(oracle_ctx *c, int i, int j, const double *U, double *pr,
+                  int *pflag);
+int or_primtoU(oracle_ctx *c, int i, int j, int loc, const double *pr, double *U);
+int or_primtoflux(oracle_ctx *c, int i, int j, int loc, const double *pr, int dir,
+                  double *F);
+int or_vchar(oracle_ctx *c, int i, int j, int loc, const double *pr, int dir,
+             double *vmax, double *vmin);
+int or_source(oracle_ctx *c, int i, int j, const double *pr, double *dU);
+void or_slope_lim(int lim, int k, double yll, double yl, double yc, double yr,
+                  double yrr, double *dq, double *left, double *right);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

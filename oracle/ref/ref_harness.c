@@ -1,0 +1,168 @@
+/*
+ * ref_harness.c -- driver that runs the COMPILED REFERENCE (HARM's own
+ * step_ch() and everything below it) on a given state, to produce golden
+ * vectors for the oracle.  Test infrastructure only; built by
+ * oracle/ref/Makefile into oracle/_ref/ from the reference sources in place.
+ *
+ * It replaces the reference's main.c/init() (main.c:12-90, initbase.c:297):
+ * instead of a problem initializer it reads the primitive state from a file,
+ * then calls bound_prim() and step_ch() nsteps times, exactly as main.c's loop
+ * does, writing the state after every step.
+ *
+ * For periodic problems this file also supplies bound_prim(): the reference
+ * has no single-rank periodic boundary (its periodicity is the MPI halo,
+ * boundmpi.c), so the harness provides the problem boundary file, as a HARM
+ * user does by swapping bounds.*.c.
+ *
+ * usage: ref_harness CONFIG   (key value lines, see read_config())
+ */
+#include "defs.h"
+#include "step_ch.h"
+
+#ifdef HARNESS_PERIODIC
+/* periodic ghost fill: x1 faces first (interior j), then x2 faces over the
+   full x1 range (corners), the order boundmpi.c:91-112 exchanges in */
+int bound_prim(int boundstage, FTYPE prim[][N2M][NPR])
+{
+  int i, j, k;
+  for (j = 0; j < N2; j++) {
+    for (i = -N1BND; i < 0; i++) PLOOP prim[i][j][k] = prim[i + N1][j][k];
+    for (i = N1; i < N1 + N1BND; i++) PLOOP prim[i][j][k] = prim[i - N1][j][k];
+  }
+  for (i = -N1BND; i < N1 + N1BND; i++) {
+    for (j = -N2BND; j < 0; j++) PLOOP prim[i][j][k] = prim[i][j + N2][k];
+    for (j = N2; j < N2 + N2BND; j++) PLOOP prim[i][j][k] = prim[i][j - N2][k];
+  }
+  return (0);
+}
+#endif
+
+static char cfg_key[64][64];
+static char cfg_val[64][256];
+static int ncfg = 0;
+
+static void read_config(const char *path)
+{
+  FILE *f = fopen(path, "r");
+  if (!f) { fprintf(stderr, "cannot open %s\n", path); exit(2); }
+  while (ncfg < 64 && fscanf(f, "%63s %255s", cfg_key[ncfg], cfg_val[ncfg]) == 2) ncfg++;
+  fclose(f);
+}
+
+static const char *cfg(const char *k, const char *def)
+{
+  int n;
+  for (n = 0; n < ncfg; n++) if (!strcmp(cfg_key[n], k)) return cfg_val[n];
+  return def;
+}
+static double cfgd(const char *k, double def) { const char *v = cfg(k, NULL); return v ? atof(v) : def; }
+static int cfgi(const char *k, int def) { const char *v = cfg(k, NULL); return v ? atoi(v) : def; }
+
+static void readall(const char *path, void *buf, size_t bytes)
+{
+  FILE *f = fopen(path, "rb");
+  if (!f || fread(buf, 1, bytes, f) != bytes) { fprintf(stderr, "cannot read %s (%lu bytes)\n", path, (unsigned long)bytes); exit(3); }
+  fclose(f);
+}
+
+int main(int argc, char *argv[])
+{
+  char *margv[4];
+  int s, nsteps, i, j, k;
+  FILE *out;
+  const char *geomsrc;
+  double hdr[4];
+
+  if (argc < 2) { fprintf(stderr, "usage: %s CONFIG\n", argv[0]); return 2; }
+  read_config(argv[1]);
+
+  margv[0] = argv[0]; margv[1] = "1"; margv[2] = "1"; margv[3] = NULL;
+  init_mpi(3, margv);
+  set_arrays();
+  init_defglobal();
+  init_defconsts();
+  init_defgrid();
+
+  /* runtime globals (initbase.c:119-243 defaults, overridden per problem) */
+  a = cfgd("a", 0.0);
+  R0 = cfgd("R0", 0.0);
+  Rout = cfgd("Rout", 40.0);
+  hslope = cfgd("hslope", 0.3);
+  defcoord = 0;
+  gam = cfgd("gam", 4. / 3.);
+  cour = cfgd("cour", 0.9);
+  lim = cfgi("lim", PARA);
+  fluxmethod = cfgi("fluxmethod", HLLFLUX);
+  TIMEORDER = cfgi("timeorder", 4);
+  FLUXB = FLUXCTTOTH;
+  UTOPRIMVERSION = UTOPRIM2DFINAL;
+  BCtype[X1UP] = cfgi("bc0", OUTFLOW);
+  BCtype[X1DN] = cfgi("bc1", OUTFLOW);
+  BCtype[X2UP] = cfgi("bc2", POLARAXIS);
+  BCtype[X2DN] = cfgi("bc3", POLARAXIS);
+  cooling = 0;
+  DODIAGS = 0;
+
+  geomsrc = cfg("geom", "ref");
+  if (!strcmp(geomsrc, "ref")) {
+    /* the reference's own tables: init_grid()+set_grid() as init() does */
+    Rhor = rhor_calc(0);
+    Rin = setRin(setihor());
+    set_coord_parms();
+    set_grid();
+    if (cfg("tables_out", NULL)) {
+      FILE *f = fopen(cfg("tables_out", ""), "wb");
+      fwrite(&a_gcov[0][0][0][0][0], sizeof(FTYPE), (size_t)N1M * N2M * NPG * NDIM * NDIM, f);
+      fwrite(&a_gcon[0][0][0][0][0], sizeof(FTYPE), (size_t)N1M * N2M * NPG * NDIM * NDIM, f);
+      fwrite(&a_gdet[0][0][0], sizeof(FTYPE), (size_t)N1M * N2M * NPG, f);
+      fwrite(&a_conn[0][0][0][0][0], sizeof(FTYPE), (size_t)N1M * N2M * NDIM * NDIM * NDIM, f);
+      fwrite(&Rin, sizeof(FTYPE), 1, f);
+      fwrite(&startx[1], sizeof(FTYPE), 2, f);
+      fwrite(&dx[1], sizeof(FTYPE), 2, f);
+      fclose(f);
+    }
+  } else {
+    /* tables from a file (reference layout), grid spacing from the config */
+    Rin = cfgd("Rin", 1.0);
+    set_coord_parms();
+    set_points();
+    startx[1] = cfgd("startx1", 0.0);
+    startx[2] = cfgd("startx2", 0.0);
+    dx[1] = cfgd("dx1", 1.0 / N1);
+    dx[2] = cfgd("dx2", 1.0 / N2);
+    readall(geomsrc, &a_gcov[0][0][0][0][0], sizeof(FTYPE) * (size_t)N1M * N2M * NPG * NDIM * NDIM);
+    readall(cfg("geom_gcon", ""), &a_gcon[0][0][0][0][0], sizeof(FTYPE) * (size_t)N1M * N2M * NPG * NDIM * NDIM);
+    readall(cfg("geom_gdet", ""), &a_gdet[0][0][0], sizeof(FTYPE) * (size_t)N1M * N2M * NPG);
+    readall(cfg("geom_conn", ""), &a_conn[0][0][0][0][0], sizeof(FTYPE) * (size_t)N1M * N2M * NDIM * NDIM * NDIM);
+    for (i = 0; i < N1M; i++) for (j = 0; j < N2M; j++) for (k = 0; k < NPG; k++) a_eomfunc[i][j][k] = a_gdet[i][j][k];
+    for (i = 0; i < N1M; i++) for (j = 0; j < N2M; j++) for (k = 0; k < NDIM; k++) a_conn2[i][j][k] = 0.0;
+  }
+
+  readall(cfg("prim", ""), &a_p[0][0][0], sizeof(FTYPE) * (size_t)N1M * N2M * NPR);
+  bound_prim(STAGEM1, p);
+
+  t = 0.0;
+  dt = cfgd("dt", 1e-4);
+  tf = cfgd("tf", 1e30);
+  nsteps = cfgi("nsteps", 1);
+
+  out = fopen(cfg("out", "ref_out.bin"), "wb");
+  hdr[0] = nsteps; hdr[1] = N1; hdr[2] = N2; hdr[3] = 0;
+  fwrite(hdr, sizeof(double), 4, out);
+  for (s = 0; s < nsteps; s++) {
+    double rec[4];
+    int nf = 0;
+    nstroke = 0;
+    failed = 0;
+    step_ch();
+    ZLOOP if (pflag[i][j][FLAGUTOPRIMFAIL]) nf++;
+    rec[0] = t; rec[1] = dt; rec[2] = (double)nstroke; rec[3] = (double)(failed + 2 * nf);
+    fwrite(rec, sizeof(double), 4, out);
+    fwrite(&a_p[0][0][0], sizeof(FTYPE), (size_t)N1M * N2M * NPR, out);
+  }
+  fclose(out);
+#if(USEMPI)
+  MPI_Finalize();
+#endif
+  return 0;
+}
