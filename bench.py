@@ -1,0 +1,167 @@
+"""Benchmark: FP64 zone-updates/s of HARM's conservative timestep on one MI355X.
+
+Workload (one "step" = one full step_ch(): 4 RK substages + dt update):
+Fishbone-Moncrief torus around a Kerr hole a=0.9375 in modified Kerr-Schild
+coordinates, 256x256 zones (r from inside the horizon to 40 M, hslope 0.3),
+rin=6, rmax=12, beta=100 poloidal loop, 4% seeded u perturbation, PPM
+reconstruction, HLL fluxes, Toth flux-CT, 2D Newton con-to-prim, floors,
+RK4, Courant 0.9, adaptive dt.  Synthetic initial state, all inputs resident
+in GPU memory before the timed region.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line.  `value` is the graph-replayed whole-step throughput;
+`roofline` comes from HIP events recorded around every kernel launch on the
+launch stream over an eagerly launched copy of the timed region;
+`cpu_baseline` is the oracle (FP64 C restatement, OpenMP) on the host cores,
+on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "FP64 zone-updates/s at 256² torus; batched & adjoint-step cost vs plain step"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic bytes per unit for each kernel (read + write of the data the
+# kernel must touch at least once; the unit is the kernel's work item).
+#   k_flux     per face (both directions in one launch): p of one zone (64)
+#              + packed face geometry (16 doubles, 128) + F (64)
+#              + EMF quotient q (8)                                  = 264 B
+#   k_update   per zone: pi, pb (128) + CENT geometry (36 doubles, 288)
+#              + connection (64 doubles, 512) + F1, F2 (128) + q1, q2 (16)
+#              + ulast, unew read (128) + pf, ulast, unew written (192)
+#              + pflag (4)                                          = 1396 B
+BYTES_PER_UNIT = {"k_flux": 264, "k_update": 1396}
+
+
+def units_per_launch(name, n1, n2):
+    return {"k_flux": (n1 + 1) * (n2 + 2) + (n1 + 2) * (n2 + 1), "k_update": n1 * n2}.get(name)
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE
+    (profiles/pmc_traffic.json), or None if that profile has not been taken."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        d = json.load(open(path))
+        return d.get("per_launch_bytes", {}).get(kernel)
+    except Exception:
+        return None
+
+
+def setup(n1, n2):
+    import harm_ramt_amd as H
+    g = H.grid_mks(n1, n2, a=0.9375, Rout=40.0, hslope=0.3)
+    geom = H.Geometry(g)
+    p0 = H.init_fishmon(geom, gam=4.0 / 3.0, rin=6.0, rmax=12.0, kappa=1e-3, beta=100.0, randfact=4e-2, seed=7)
+    prm = H.make_params(n1, n2, g.dx1, g.dx2, lim=H.PARA, fluxmethod=H.HLLFLUX, timeorder=4, cour=0.9)
+    return g, geom, p0, prm
+
+
+def cpu_baseline(geom, p0, prm, n1, n2, seconds):
+    import harm_ramt_amd as H
+    from oracle.oracle import Oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    o = Oracle(H.params_dict(prm), nthreads=threads)
+    o.set_geometry(*geom.tables())
+    o.set_array(0, p0)
+    o.bound_prim(0)
+    o.set_time(0.0, 1e-5, 2000.0)
+    o.step()  # first touch
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        o.step()
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds and steps >= 3:
+            break
+    return {"value": steps * n1 * n2 / el, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} steps of the same {n1}x{n2} torus from t=0 (steps 2..{steps + 1}), "
+                      f"{el:.1f} s on {threads} OpenMP threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n1", type=int, default=256)
+    ap.add_argument("--n2", type=int, default=256)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this build measures one GPU only (the path is a single-GPU grid; "
+              "multi-GPU halo exchange is design-only). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import harm_ramt_amd as H
+    n1, n2 = args.n1, args.n2
+    g, geom, p0, prm = setup(n1, n2)
+    h = H.Harm(prm, geom)
+    h.set_array(H.P, p0)
+    h.bound_prim(H.P)
+    h.set_time(0.0, 1e-5, 2000.0)   # initbase.c:164 dt=1e-5, tf well beyond the run
+
+    h.step_n(args.warmup)
+    h.get_time()                    # synchronises
+    t0 = time.perf_counter()
+    h.step_n(args.steps)            # HIP-graph replay, no host sync inside
+    t_now, dt_now = h.get_time()    # synchronises
+    el = time.perf_counter() - t0
+    value = args.steps * n1 * n2 / el
+    nfail, failed = h.failures()
+    if failed:
+        print(f"bench.py: the run set the failure flag (t={t_now})", file=sys.stderr)
+        sys.exit(1)
+
+    # per-kernel HIP-event timing over an eagerly launched copy of the timed region
+    h.prof_enable(True)
+    h.prof_reset()
+    prof_steps = min(args.steps, 100)
+    t1 = time.perf_counter()
+    h.step_n(prof_steps)
+    h.get_time()
+    el_prof = time.perf_counter() - t1
+    kern = {}
+    for kid, name in H.KERNEL_NAMES.items():
+        ms, cnt = h.prof_get(kid)
+        if cnt:
+            kern[name] = {"total_ms": ms, "launches": cnt, "avg_us": 1e3 * ms / cnt}
+    h.prof_enable(False)
+    dom = max((k for k in kern if k in BYTES_PER_UNIT), key=lambda k: kern[k]["total_ms"])
+    bytes_launch = BYTES_PER_UNIT[dom] * units_per_launch(dom, n1, n2)
+    achieved = bytes_launch / (kern[dom]["avg_us"] * 1e-6) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom),
+                "kernel": dom, "avg_launch_us": round(kern[dom]["avg_us"], 3),
+                "algorithmic_bytes_per_launch": bytes_launch}
+
+    cpu = None if args.no_cpu else cpu_baseline(geom, p0, prm, n1, n2, args.cpu_seconds)
+    out = {
+        "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * el / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"Fishbone-Moncrief torus, Kerr a=0.9375, MKS {n1}x{n2}, PPM+HLL+RK4, "
+                               f"flux-CT, 2D Newton con-to-prim, floors, adaptive dt",
+                   "n1": n1, "n2": n2, "timeorder": 4, "lim": "PARA", "flux": "HLL", "t_end": t_now,
+                   "dt_end": dt_now, "failed_inversions_repaired": nfail},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kern,
+        "eager_timed_ms_per_step": 1e3 * el_prof / prof_steps,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

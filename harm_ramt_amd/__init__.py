@@ -29,9 +29,8 @@ OUTFLOW, POLARAXIS, PERIODIC = 0, 4, 100
 X1UP, X1DN, X2UP, X2DN = 0, 1, 2, 3
 P, PK0, PK1, F1, F2, ULAST, UNEW = 0, 1, 2, 3, 4, 5, 6
 METRIC_MINK, METRIC_KS = 0, 2
-K_FLUX1, K_FLUX2, K_EMF, K_UPDATE, K_BOUND = 0, 1, 2, 3, 4
-KERNEL_NAMES = {K_FLUX1: "k_flux<1>", K_FLUX2: "k_flux<2>", K_EMF: "k_emf",
-                K_UPDATE: "k_update", K_BOUND: "k_bound_x1+k_bound_x2"}
+K_FLUX, K_EMF, K_UPDATE, K_BOUNDARY = 0, 2, 3, 4
+KERNEL_NAMES = {K_FLUX: "k_flux", K_EMF: "k_emf", K_UPDATE: "k_update", K_BOUNDARY: "k_boundary"}
 
 
 class HarmParams(ctypes.Structure):
@@ -63,7 +62,7 @@ EXPORTS = [
     "harm_create", "harm_destroy", "harm_last_error", "harm_set_stream", "harm_set_geometry",
     "harm_set_array", "harm_get_array", "harm_array_ptr", "harm_get_pflag", "harm_set_time",
     "harm_get_time", "harm_fluxcalc", "harm_fix_flux", "harm_flux_ct", "harm_advance",
-    "harm_bound_prim", "harm_step", "harm_step_n", "harm_utoprimgen", "harm_get_failures",
+    "harm_bound_prim", "harm_post_fixup", "harm_step", "harm_step_n", "harm_utoprimgen", "harm_get_failures",
     "harm_get_nstroke", "harm_prof_enable", "harm_prof_get", "harm_prof_reset",
     "harm_grid_mks", "harm_grid_cartesian", "harm_geometry", "harm_init_fishmon",
     "harm_init_orszag_tang",
@@ -99,6 +98,7 @@ def load_library(path=LIB_PATH):
         "harm_flux_ct": [vp],
         "harm_advance": [vp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int, dp],
         "harm_bound_prim": [vp, ctypes.c_int],
+        "harm_post_fixup": [vp, ctypes.c_int],
         "harm_step": [vp],
         "harm_step_n": [vp, ctypes.c_int],
         "harm_utoprimgen": [vp, ctypes.c_int, ip, dp, dp, ip],
@@ -304,6 +304,10 @@ class Harm:
     # bounds.fishmon.c:13
     def bound_prim(self, sel=P):
         _chk(self.lib.harm_bound_prim(self.h, sel), "harm_bound_prim")
+
+    # fixup.c:37
+    def post_fixup(self, sel=P):
+        _chk(self.lib.harm_post_fixup(self.h, sel), "harm_post_fixup")
 
     # step_ch.c:16
     def step_ch(self):

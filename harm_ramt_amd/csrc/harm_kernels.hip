@@ -44,6 +44,7 @@ struct DevScalars {
   double t, dt, tf;
   int failed;
   int nfail;
+  int nlist;  // entries of the failed-zone list of the current substage
   unsigned long long nstroke;
 };
 
@@ -110,165 +111,191 @@ __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
 }
 
 // ---------------------------------------------------------------------------
-// fluxcalc(): one thread per face.  dir 1 faces: i in [0,N1], j in [-1,N2];
-// dir 2 faces: i in [-1,N1], j in [0,N2] (FZLOOP(-jdel,-idel), mympi.h:287).
+// fluxcalc() for both directions in one launch (blockIdx.z = dir-1).  Each
+// face is handled by a lane pair: the even lane forms the left state (right
+// edge of the zone below), the odd lane the right state (left edge of the
+// zone above); the two half-computations (limit_gamma, get_state, primtoflux
+// for F and U, vchar) run in parallel and are exchanged with __shfl_xor, then
+// each lane writes half of the flux components.  Face ranges follow
+// FZLOOP(-jdel,-idel) (mympi.h:287): dir 1 i in [0,N1], j in [-1,N2];
+// dir 2 i in [-1,N1], j in [0,N2].
+// Besides F the kernel writes q = F1[B2]/gdet(FACE1) (dir 1) and
+// q = F2[B1]/gdet(FACE2) (dir 2): the four quotients of the corner EMF
+// (step_ch.c:1228-1230), so the EMF is a 5-flop sum wherever it is needed.
 // ---------------------------------------------------------------------------
 template <int DIR>
-__global__ __launch_bounds__(256) void k_flux(KParams P, const double* __restrict__ pr, double* __restrict__ F,
-                                              const double* __restrict__ gF, DevScalars* __restrict__ ds) {
+__device__ __forceinline__ void flux_face(const KParams& P, const double* __restrict__ pr, double* __restrict__ F,
+                                          double* __restrict__ q, const double* __restrict__ gF, int i, int j,
+                                          int side, bool valid, double& dtij, int& fail) {
   constexpr int idel = (DIR == 1), jdel = (DIR == 2);
-  const int j = blockIdx.x * blockDim.x + threadIdx.x - idel;
-  const int i = blockIdx.y * blockDim.y + threadIdx.y - jdel;
-  const bool valid = (i <= P.n1) && (j <= P.n2);
-  double dtij = 1.e9;
-  int fail = 0;
+  constexpr int kb = B1 + DIR - 1;
+  const int zstride = (DIR == 1) ? P.N2M : 1;
+  const int z0 = valid ? zidx(P, i, j) : 0;
+  double p[NPR];
+  GeoU g;
   if (valid) {
-    // stencil along DIR: zones m = -3..+2 relative to the face's right zone
-    const int zstride = (DIR == 1) ? P.N2M : 1;
-    const int z0 = zidx(P, i, j);
-    double p_l[NPR], p_r[NPR];
-    if (P.lim == 4) {  // PARA: pright[i-1], pleft[i] (step_ch.c:896-903)
-      double s[6][NPR];
+    // this lane's 5-zone stencil along DIR starts at zone (face - 3 + side)
+    double s[5][NPR];
 #pragma unroll
-      for (int m = 0; m < 6; m++) load8(pr, z0 + (m - 3) * zstride, s[m]);
+    for (int m = 0; m < 5; m++) load8(pr, z0 + (m - 3 + side) * zstride, s[m]);
+    if (P.lim == 4) {  // PARA: left state = pright[i-1], right state = pleft[i] (step_ch.c:896-903)
 #pragma unroll
       for (int k = 0; k < NPR; k++) {
-        double l, r, l2, r2;
+        double l, r;
         para4(s[0][k], s[1][k], s[2][k], s[3][k], s[4][k], l, r);
-        para4(s[1][k], s[2][k], s[3][k], s[4][k], s[5][k], l2, r2);
-        p_l[k] = r;
-        p_r[k] = l2;
+        p[k] = side ? l : r;
       }
-    } else {  // slope limiters: p[i-1]+dq/2, p[i]-dq/2
-      double s[5][NPR];
-#pragma unroll
-      for (int m = 0; m < 5; m++) load8(pr, z0 + (m - 3) * zstride, s[m]);
-      double s5[NPR];
-      load8(pr, z0 + 2 * zstride, s5);
+    } else {  // p[i-1] + dq[i-1]/2, p[i] - dq[i]/2 (step_ch.c:894-895)
 #pragma unroll
       for (int k = 0; k < NPR; k++) {
-        double dql, dqr;
-        if (P.lim == 90) {
-          dql = csslope(k, s[0][k], s[1][k], s[2][k], s[3][k], s[4][k]);
-          dqr = csslope(k, s[1][k], s[2][k], s[3][k], s[4][k], s5[k]);
-        } else {
-          dql = slope3(P.lim, s[1][k], s[2][k], s[3][k]);
-          dqr = slope3(P.lim, s[2][k], s[3][k], s[4][k]);
-        }
-        p_l[k] = s[2][k] + 0.5 * dql;
-        p_r[k] = s[3][k] - 0.5 * dqr;
+        double dq = (P.lim == 90) ? csslope(k, s[0][k], s[1][k], s[2][k], s[3][k], s[4][k])
+                                  : slope3(P.lim, s[1][k], s[2][k], s[3][k]);
+        p[k] = side ? (s[2][k] - 0.5 * dq) : (s[2][k] + 0.5 * dq);
       }
     }
-    GeoU g;
     load_geo_face(gF, z0, g);
-    // BDIRCONT (step_ch.c:944-947)
-    {
-      constexpr int kb = B1 + DIR - 1;
-      double bavg = 0.5 * (p_l[kb] + p_r[kb]);
-      p_l[kb] = bavg;
-      p_r[kb] = bavg;
-    }
-    if (limit_gamma(P.gammamax, p_l, g) >= 1) fail = 1;
-    if (limit_gamma(P.gammamax, p_r, g) >= 1) fail = 1;
-    State sl, sr;
-    get_state(p_l, g, sl);
-    get_state(p_r, g, sr);
-    double F_l[NPR], F_r[NPR], U_l[NPR], U_r[NPR];
-    primtoflux(P, p_l, sl, DIR, g.gdet, F_l);
-    primtoflux(P, p_r, sr, DIR, g.gdet, F_r);
-    primtoflux(P, p_l, sl, 0, g.gdet, U_l);
-    primtoflux(P, p_r, sr, 0, g.gdet, U_r);
-    double cmax_l, cmin_l, cmax_r, cmin_r;
-    if (vchar(P, p_l, sl, DIR, g, cmax_l, cmin_l)) fail = 1;
-    if (vchar(P, p_r, sr, DIR, g, cmax_r, cmin_r)) fail = 1;
-    // cminmax_calc (step_ch.c:1095-1148)
-    double lmin = dmin(cmin_l, cmin_r), lmax = dmax(cmax_l, cmax_r);
-    double cmin = fabs(dmax(0., -lmin)), cmax = fabs(dmax(0., lmax));
-    double ctop = dmax(cmax, cmin);
-    double hllfrac = (P.fluxmethod == 1) ? 1.0 : 0.0;
-    double laxffrac = (P.fluxmethod == 0) ? 1.0 : 0.0;
-    double Fo[NPR];
-    if (cmax + cmin != 0.0) {
+  } else {
 #pragma unroll
-      for (int k = 0; k < NPR; k++)
-        Fo[k] = hllfrac * ((cmax * F_l[k] + cmin * F_r[k] - cmax * cmin * (U_r[k] - U_l[k])) / (cmax + cmin)) +
-                laxffrac * (0.5 * (F_l[k] + F_r[k] - ctop * (U_r[k] - U_l[k])));
-    } else {
+    for (int k = 0; k < NPR; k++) p[k] = 0.0;
+    g = GeoU{-1, 0, 0, 0, 1, 0, 0, 1, 0, 1, -1, 0, 0, 0, 1, 1};
+  }
+  // BDIRCONT (step_ch.c:944-947): both lanes form 0.5*(p_l+p_r) in that order
+  {
+    double other = __shfl_xor(p[kb], 1);
+    double pl = side ? other : p[kb], prr = side ? p[kb] : other;
+    p[kb] = 0.5 * (pl + prr);
+  }
+  if (limit_gamma(P.gammamax, p, g) >= 1) fail = 1;
+  State st;
+  get_state(p, g, st);
+  double Fs[NPR], Us[NPR], cmax_s, cmin_s;
+  primtoflux(P, p, st, DIR, g.gdet, Fs);
+  primtoflux(P, p, st, 0, g.gdet, Us);
+  if (vchar(P, p, st, DIR, g, cmax_s, cmin_s)) fail = 1;
+  if (!valid) fail = 0;
+  // exchange the two half-states; each lane keeps the half of the components
+  // it will write (k = 4*side + m), selected with compile-time indices
+  double Fmine[4], Umine[4], Fthat[4], Uthat[4];
 #pragma unroll
-      for (int k = 0; k < NPR; k++) Fo[k] = (0.5 * (F_l[k] + F_r[k] - ctop * (U_r[k] - U_l[k])));
-    }
-    double cm = dmax(cmax, cmin);
-    dtij = P.cour * ((DIR == 1) ? P.dx1 : P.dx2) / cm;
-    if (!(dtij < 1.e9)) dtij = 1.e9;  // the reference's min ignores NaN and values >= 1e9
-    // fix_flux() edge rules (fixup.c:1357-1410), applied by the face that owns them
-    if (P.fixupflux) {
-      if (DIR == 1) {
-        if (i == 0 && P.bc[HARM_X1DN] == HARM_OUTFLOW && j >= -1 && j < P.n2 && Fo[RHO] > 0) Fo[RHO] = 0;
-        if (i == P.n1 && P.bc[HARM_X1UP] == HARM_OUTFLOW && j >= -1 && j < P.n2 && Fo[RHO] < 0) Fo[RHO] = 0;
-      } else {
-        bool polar_dn = (j == 0 && P.bc[HARM_X2DN] == HARM_POLARAXIS && i >= -1 && i < P.n1);
-        bool polar_up = (j == P.n2 && P.bc[HARM_X2UP] == HARM_POLARAXIS && i >= -1 && i < P.n1);
-        bool equator = (i >= 0 && i < P.n1 && j == (P.n2 - 1) / 2);
-        if (polar_dn || polar_up || equator) {
+  for (int m = 0; m < 4; m++) {
+    // the partner needs the other half: send components 4*(1-side)+m
+    double fs = side ? Fs[m] : Fs[4 + m], us = side ? Us[m] : Us[4 + m];
+    Fthat[m] = __shfl_xor(fs, 1);   // partner's value of component 4*side+m
+    Uthat[m] = __shfl_xor(us, 1);
+    Fmine[m] = side ? Fs[4 + m] : Fs[m];
+    Umine[m] = side ? Us[4 + m] : Us[m];
+  }
+  double cmax_o = __shfl_xor(cmax_s, 1), cmin_o = __shfl_xor(cmin_s, 1);
+  const double cmax_l = side ? cmax_o : cmax_s, cmin_l = side ? cmin_o : cmin_s;
+  const double cmax_r = side ? cmax_s : cmax_o, cmin_r = side ? cmin_s : cmin_o;
+  // cminmax_calc (step_ch.c:1095-1148)
+  double lmin = dmin(cmin_l, cmin_r), lmax = dmax(cmax_l, cmax_r);
+  double cmin = fabs(dmax(0., -lmin)), cmax = fabs(dmax(0., lmax));
+  double ctop = dmax(cmax, cmin);
+  double hllfrac = (P.fluxmethod == 1) ? 1.0 : 0.0;
+  double laxffrac = (P.fluxmethod == 0) ? 1.0 : 0.0;
+  double Fh[4];
 #pragma unroll
-          for (int k = 0; k < NPR; k++)
-            if (k != U2) Fo[k] = 0.;
+  for (int m = 0; m < 4; m++) {
+    double Fl = side ? Fthat[m] : Fmine[m], Fr = side ? Fmine[m] : Fthat[m];
+    double Ul = side ? Uthat[m] : Umine[m], Ur = side ? Umine[m] : Uthat[m];
+    if (cmax + cmin != 0.0)
+      Fh[m] = hllfrac * ((cmax * Fl + cmin * Fr - cmax * cmin * (Ur - Ul)) / (cmax + cmin)) +
+              laxffrac * (0.5 * (Fl + Fr - ctop * (Ur - Ul)));
+    else
+      Fh[m] = (0.5 * (Fl + Fr - ctop * (Ur - Ul)));
+  }
+  double cm = dmax(cmax, cmin);
+  dtij = P.cour * ((DIR == 1) ? P.dx1 : P.dx2) / cm;
+  if (!(dtij < 1.e9) || !valid) dtij = 1.e9;  // the reference's running min ignores NaN and >= 1e9
+  if (!valid) return;
+  // fix_flux() edge rules (fixup.c:1357-1410), applied by the face that owns them
+  bool skip_b2 = false;
+  if (P.fixupflux) {
+    if (DIR == 1) {
+      if (side == 0) {
+        if (i == 0 && P.bc[HARM_X1DN] == HARM_OUTFLOW && j >= -1 && j < P.n2 && Fh[RHO] > 0) Fh[RHO] = 0;
+        if (i == P.n1 && P.bc[HARM_X1UP] == HARM_OUTFLOW && j >= -1 && j < P.n2 && Fh[RHO] < 0) Fh[RHO] = 0;
+      } else if (i >= -1 && i < P.n1) {
+        // F1[i][-1][B2] = -F1[i][0][B2], F1[i][N2][B2] = -F1[i][N2-1][B2], with their EMF quotients
+        if (j == 0 && P.bc[HARM_X2DN] == HARM_POLARAXIS) {
+          int zg = zidx(P, i, -1);
+          F[(size_t)zg * 8 + B2] = -Fh[B2 - 4];
+          q[zg] = (-Fh[B2 - 4]) / gF[(size_t)zg * GF + 15];
         }
+        if (j == P.n2 - 1 && P.bc[HARM_X2UP] == HARM_POLARAXIS) {
+          int zg = zidx(P, i, P.n2);
+          F[(size_t)zg * 8 + B2] = -Fh[B2 - 4];
+          q[zg] = (-Fh[B2 - 4]) / gF[(size_t)zg * GF + 15];
+        }
+        skip_b2 = (j == -1 && P.bc[HARM_X2DN] == HARM_POLARAXIS) || (j == P.n2 && P.bc[HARM_X2UP] == HARM_POLARAXIS);
       }
-    }
-    bool skip_b2 = false;
-    if (DIR == 1 && P.fixupflux && i >= -1 && i < P.n1) {
-      // F1[i][-1][B2] = -F1[i][0][B2] and F1[i][N2][B2] = -F1[i][N2-1][B2]
-      if (j == 0 && P.bc[HARM_X2DN] == HARM_POLARAXIS) F[(size_t)zidx(P, i, -1) * 8 + B2] = -Fo[B2];
-      if (j == P.n2 - 1 && P.bc[HARM_X2UP] == HARM_POLARAXIS) F[(size_t)zidx(P, i, P.n2) * 8 + B2] = -Fo[B2];
-      if ((j == -1 && P.bc[HARM_X2DN] == HARM_POLARAXIS) || (j == P.n2 && P.bc[HARM_X2UP] == HARM_POLARAXIS))
-        skip_b2 = true;
-    }
-    if (skip_b2) {
-      double* Fz = F + (size_t)z0 * 8;
-#pragma unroll
-      for (int k = 0; k < NPR; k++)
-        if (k != B2) Fz[k] = Fo[k];
     } else {
-      store8(F, z0, Fo);
+      bool polar_dn = (j == 0 && P.bc[HARM_X2DN] == HARM_POLARAXIS && i >= -1 && i < P.n1);
+      bool polar_up = (j == P.n2 && P.bc[HARM_X2UP] == HARM_POLARAXIS && i >= -1 && i < P.n1);
+      bool equator = (i >= 0 && i < P.n1 && j == (P.n2 - 1) / 2);
+      if (polar_dn || polar_up || equator) {
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+          if (4 * side + m != U2) Fh[m] = 0.;
+      }
     }
   }
-  // Courant limit: wavefront min, then one 64-bit atomic per wavefront
+  double* Fz = F + (size_t)z0 * 8 + 4 * side;
+  if (skip_b2) {
+    Fz[0] = Fh[0];
+    Fz[1] = Fh[1];
+    Fz[3] = Fh[3];
+  } else {
+    reinterpret_cast<double2*>(Fz)[0] = make_double2(Fh[0], Fh[1]);
+    reinterpret_cast<double2*>(Fz)[1] = make_double2(Fh[2], Fh[3]);
+    if (side == 1) q[z0] = (DIR == 1 ? Fh[B2 - 4] : Fh[B1 - 4]) / g.gdet;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_flux(KParams P, const double* __restrict__ pr, double* __restrict__ F1,
+                                              double* __restrict__ F2, double* __restrict__ q1,
+                                              double* __restrict__ q2, const double* __restrict__ gF,
+                                              DevScalars* __restrict__ ds, int dir0) {
+  const int dir = dir0 + blockIdx.z;
+  const int side = threadIdx.x & 1;
+  const int jf = blockIdx.x * (blockDim.x >> 1) + (threadIdx.x >> 1);
+  const int ifc = blockIdx.y * blockDim.y + threadIdx.y;
+  double dtij = 1.e9;
+  int fail = 0;
+  const size_t nz = (size_t)(P.n1 + 2 * NG) * P.N2M;
+  if (dir == 1) {
+    const int i = ifc, j = jf - 1;
+    flux_face<1>(P, pr, F1, q1, gF, i, j, side, (i <= P.n1) && (j <= P.n2), dtij, fail);
+  } else {
+    const int i = ifc - 1, j = jf;
+    flux_face<2>(P, pr, F2, q2, gF + nz * GF, i, j, side, (i <= P.n1) && (j <= P.n2), dtij, fail);
+  }
   double wm = wave_min(dtij);
   if ((threadIdx.x & 63) == 0 && wm < 1.e9)
-    atomicMin(&ds->ndt_acc[DIR - 1], (unsigned long long)__double_as_longlong(wm));
+    atomicMin(&ds->ndt_acc[dir - 1], (unsigned long long)__double_as_longlong(wm));
   if (fail) atomicOr(&ds->failed, 1);
 }
 
-// ---------------------------------------------------------------------------
-// flux_ct() EMF at corners i in [0,N1], j in [0,N2] (step_ch.c:1223-1234).
-// Thread (0,0) also snapshots and resets the Courant accumulators.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_emf(KParams P, const double* __restrict__ F1, const double* __restrict__ F2,
-                                             const double* __restrict__ gF, const double* __restrict__ gC,
-                                             double* __restrict__ emf, DevScalars* __restrict__ ds) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y * blockDim.y + threadIdx.y;
-  if (i == 0 && j == 0) {
-    ds->ndt_last[0] = __longlong_as_double((long long)ds->ndt_acc[0]);
-    ds->ndt_last[1] = __longlong_as_double((long long)ds->ndt_acc[1]);
-    ds->ndt_acc[0] = (unsigned long long)__double_as_longlong(1.e9);
-    ds->ndt_acc[1] = (unsigned long long)__double_as_longlong(1.e9);
-  }
-  if (i > P.n1 || j > P.n2) return;
-  const size_t nz = (size_t)(P.n1 + 2 * NG) * P.N2M;
-  const double* gF1 = gF;
-  const double* gF2 = gF + nz * GF;
-  int z = zidx(P, i, j), zjm = zidx(P, i, j - 1), zim = zidx(P, i - 1, j);
-  double ec = gC[(size_t)z * GCN + 35];
-  double e1 = gF1[(size_t)z * GF + 15], e2 = gF1[(size_t)zjm * GF + 15];
-  double e3 = gF2[(size_t)z * GF + 15], e4 = gF2[(size_t)zim * GF + 15];
-  emf[z] = 0.25 * (ec) * (F1[(size_t)z * 8 + B2] / (e1) + F1[(size_t)zjm * 8 + B2] / (e2) -
-                          F2[(size_t)z * 8 + B1] / (e3) - F2[(size_t)zim * 8 + B1] / (e4));
+// corner EMF of flux_ct() (step_ch.c:1223-1234, CORNGDETVERSION=1) from the
+// per-face quotients written by k_flux
+__device__ __forceinline__ double emf_at(const KParams& P, const double* __restrict__ q1,
+                                         const double* __restrict__ q2, const double* __restrict__ gC, int i, int j) {
+  const int z = zidx(P, i, j);
+  return 0.25 * (gC[(size_t)z * GCN + 35]) *
+         (q1[z] + q1[zidx(P, i, j - 1)] - q2[z] - q2[zidx(P, i - 1, j)]);
 }
 
-// flux_ct() rewrite of the B fluxes (step_ch.c:1405-1412); used by the
-// standalone harm_flux_ct() entry point (the step forms them in k_update)
+// flux_ct() materialised into F1/F2 (step_ch.c:1223-1234, 1405-1412), for the
+// standalone harm_flux_ct() entry point: EMFs first, then the B fluxes
+__global__ void k_emf(KParams P, const double* __restrict__ q1, const double* __restrict__ q2,
+                      const double* __restrict__ gC, double* __restrict__ emf) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i > P.n1 || j > P.n2) return;
+  emf[zidx(P, i, j)] = emf_at(P, q1, q2, gC, i, j);
+}
+
 __global__ void k_ctapply(KParams P, double* __restrict__ F1, double* __restrict__ F2,
                           const double* __restrict__ emf) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -450,12 +477,19 @@ __global__ __launch_bounds__(256) void k_update(KParams P, RKCoef C, const doubl
                                                 const double* __restrict__ pb, double* __restrict__ pf,
                                                 double* __restrict__ ulast, double* __restrict__ unew,
                                                 const double* __restrict__ F1, const double* __restrict__ F2,
-                                                const double* __restrict__ emf, const double* __restrict__ gC,
-                                                const double* __restrict__ gConn, int* __restrict__ pflag,
+                                                const double* __restrict__ q1, const double* __restrict__ q2,
+                                                const double* __restrict__ gC, const double* __restrict__ gConn,
+                                                int* __restrict__ pflag, int* __restrict__ faillist,
                                                 DevScalars* __restrict__ ds) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y * blockDim.y + threadIdx.y;
   int niter = 0, failz = 0, fail = 0;
+  if (i == 0 && j == 0) {  // this substage's Courant limits (advance.c:190 reads them)
+    ds->ndt_last[0] = __longlong_as_double((long long)ds->ndt_acc[0]);
+    ds->ndt_last[1] = __longlong_as_double((long long)ds->ndt_acc[1]);
+    ds->ndt_acc[0] = (unsigned long long)__double_as_longlong(1.e9);
+    ds->ndt_acc[1] = (unsigned long long)__double_as_longlong(1.e9);
+  }
   if (i < P.n1 && j < P.n2) {
     const int z = zidx(P, i, j);
     const double dt = ds->dt;
@@ -514,8 +548,8 @@ __global__ __launch_bounds__(256) void k_update(KParams P, RKCoef C, const doubl
     load8(F2, z, f2a);
     load8(F2, zidx(P, i, j + 1), f2b);
     if (P.fluxb == HARM_FLUXCTTOTH) {
-      double e00 = emf[z], e01 = emf[zidx(P, i, j + 1)], e10 = emf[zidx(P, i + 1, j)],
-             e11 = emf[zidx(P, i + 1, j + 1)];
+      double e00 = emf_at(P, q1, q2, gC, i, j), e01 = emf_at(P, q1, q2, gC, i, j + 1),
+             e10 = emf_at(P, q1, q2, gC, i + 1, j), e11 = emf_at(P, q1, q2, gC, i + 1, j + 1);
       f1a[B1] = 0.;
       f1a[B2] = 0.5 * (e00 + e01);
       f1b[B1] = 0.;
@@ -554,6 +588,10 @@ __global__ __launch_bounds__(256) void k_update(KParams P, RKCoef C, const doubl
     int pfl = utoprimgen(P, C.last ? un : Uf, gc, pout, niter);
     pflag[z] = pfl;
     failz = (pfl != 0);
+    if (failz) {  // zones for fixup_utoprim (k_boundary)
+      int slot = atomicAdd(&ds->nlist, 1);
+      if (slot < P.n1 * P.n2) faillist[slot] = z;
+    }
     // fixup1zone (fixup.c:274-377): floors then limit_gamma
     if (P.eomtype == 0) {
       double fr = gc[33], fu = gc[34];
@@ -605,17 +643,15 @@ __device__ int inflow_check_x1(const KParams& P, int i, double* pr, const double
   return -1;
 }
 
-// x1 ghosts, j in [0,N2): thread (gi, j), gi in 0..5 -> i = -3..-1, N1..N1+2
-__global__ void k_bound_x1(KParams P, double* __restrict__ pr, const double* __restrict__ gC,
-                           DevScalars* __restrict__ ds) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int gi = blockIdx.y;
-  if (j >= P.n2) return;
+// x1 ghost cell (gi in 0..5 -> i = -3..-1, N1..N1+2; j in [0,N2)):
+// bounds.fishmon.c:26-110,190-262 (EXTRAP=0 copy, inflow check, limit_gamma)
+// or the periodic wrap
+__device__ void bound_x1_cell(const KParams& P, double* __restrict__ pr, const double* __restrict__ gC, int gi,
+                              int j, int& fail) {
   const bool dn = gi < NG;
   const int i = dn ? (gi - NG) : (P.n1 + gi - NG);
   const int bc = dn ? P.bc[HARM_X1DN] : P.bc[HARM_X1UP];
   double v[NPR];
-  int fail = 0;
   if (bc == HARM_PERIODIC) {
     load8(pr, zidx(P, dn ? i + P.n1 : i - P.n1, j), v);
     store8(pr, zidx(P, i, j), v);
@@ -623,35 +659,28 @@ __global__ void k_bound_x1(KParams P, double* __restrict__ pr, const double* __r
     load8(pr, zidx(P, dn ? 0 : P.n1 - 1, j), v);
     const int z = zidx(P, i, j);
     double gc[36];
+    const double2* a = reinterpret_cast<const double2*>(gC + (size_t)z * GCN);
 #pragma unroll
-    for (int m = 0; m < 36; m++) gc[m] = gC[(size_t)z * GCN + m];
+    for (int m = 0; m < 18; m++) {
+      double2 t = a[m];
+      gc[2 * m] = t.x;
+      gc[2 * m + 1] = t.y;
+    }
     GeoU g;
     geo_from_cent(gc, g);
     if (inflow_check_x1(P, i, v, gc, g) >= 1) fail = 1;
     if (limit_gamma(P.gammamax, v, g) >= 1) fail = 1;
     store8(pr, z, v);
   }
-  if (fail) atomicOr(&ds->failed, 1);
 }
 
-// x2 ghosts over i in [-3, N1+2] (corners included), then POLEDEATH zeroing of
-// U2,B2 on j in [-3,0] and [N2-1,N2+2] (bounds.fishmon.c:263-355)
-__global__ void k_bound_x2(KParams P, double* __restrict__ pr) {
-  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) - NG;
-  const int gj = blockIdx.y;  // 0..7: 0..2 dn ghosts, 3 row 0, 4 row N2-1, 5..7 up ghosts
-  if (i > P.n1 + NG - 1) return;
-  const bool dn = gj <= 3;
+// x2 ghost cell (i in [-3,N1+2], gj in 0..5 -> j = -3..-1, N2..N2+2):
+// polar reflection with POLEDEATH on the ghost's U2,B2 (bounds.fishmon.c:263-355)
+// or the periodic wrap
+__device__ void bound_x2_cell(const KParams& P, double* __restrict__ pr, int i, int gj) {
+  const bool dn = gj < NG;
+  const int j = dn ? (gj - NG) : (P.n2 + gj - NG);
   const int bc = dn ? P.bc[HARM_X2DN] : P.bc[HARM_X2UP];
-  if (gj == 3 || gj == 4) {  // interior pole rows: only POLEDEATH
-    if (bc == HARM_POLARAXIS && P.poledeath) {
-      int j = (gj == 3) ? 0 : P.n2 - 1;
-      double* a = pr + (size_t)zidx(P, i, j) * 8;
-      a[U2] *= 0.;
-      a[B2] *= 0.;
-    }
-    return;
-  }
-  const int j = dn ? (gj - NG) : (P.n2 + gj - 5);
   double v[NPR];
   if (bc == HARM_PERIODIC) {
     load8(pr, zidx(P, i, dn ? j + P.n2 : j - P.n2), v);
@@ -667,15 +696,141 @@ __global__ void k_bound_x2(KParams P, double* __restrict__ pr) {
   }
 }
 
-// step_ch.c:174-183 (t, dt update) with advance.c:190 (ndt from the last substage)
-__global__ void k_dt(KParams P, DevScalars* ds) {
-  double ndt = P.defcon * 1. / (1. / ds->ndt_last[0] + 1. / ds->ndt_last[1]);
-  double dt = ds->dt;
-  ds->t += dt;
-  if (ndt > P.safe * dt) ndt = P.safe * dt;
-  dt = ndt;
-  if (ds->t + dt > ds->tf) dt = ds->tf - ds->t;
-  ds->dt = dt;
+// bound_prim() by one workgroup: x1 faces, then x2 faces over all i (corners
+// from the x1 values), then POLEDEATH of the two interior pole rows, in the
+// reference's order (barriers between the phases)
+__device__ void bound_block(const KParams& P, double* __restrict__ pr, const double* __restrict__ gC, int& fail) {
+  const int tid = threadIdx.x, nt = blockDim.x, ni = P.n1 + 2 * NG;
+  for (int t = tid; t < 2 * NG * P.n2; t += nt) bound_x1_cell(P, pr, gC, t / P.n2, t % P.n2, fail);
+  __syncthreads();
+  for (int t = tid; t < 2 * NG * ni; t += nt) bound_x2_cell(P, pr, t % ni - NG, t / ni);
+  __syncthreads();
+  if (P.poledeath) {
+    for (int t = tid; t < 2 * ni; t += nt) {
+      const int i = t % ni - NG;
+      const bool dn = t < ni;
+      if ((dn ? P.bc[HARM_X2DN] : P.bc[HARM_X2UP]) != HARM_POLARAXIS) continue;
+      double* a = pr + (size_t)zidx(P, i, dn ? 0 : P.n2 - 1) * 8;
+      a[U2] *= 0.;
+      a[B2] *= 0.;
+    }
+  }
+  __syncthreads();
+}
+
+// pflag of a zone at distance <= 1 from the interior, as bound_pflag()
+// (boundsint.c:4-60) leaves it: x2 mapping first (polar reflection / periodic
+// wrap over all i), then x1 (outflow copy of the edge zone / periodic wrap).
+__device__ __forceinline__ int pflag_at(const KParams& P, const int* __restrict__ pflag, int i, int j) {
+  if (j < 0) j = (P.bc[HARM_X2DN] == HARM_PERIODIC) ? j + P.n2 : (0 + (0 - j - 1));
+  else if (j >= P.n2) j = (P.bc[HARM_X2UP] == HARM_PERIODIC) ? j - P.n2 : ((P.n2 - 1) + ((P.n2 - 1) - j + 1));
+  if (i < 0) i = (P.bc[HARM_X1DN] == HARM_PERIODIC) ? i + P.n1 : 0;
+  else if (i >= P.n1) i = (P.bc[HARM_X1UP] == HARM_PERIODIC) ? i - P.n1 : P.n1 - 1;
+  return pflag[zidx(P, i, j)];
+}
+
+// fixup_utoprim() for one failed zone (fixup.c:527-920; UTOPRIMAVG,
+// ORDERINDEPENDENT, HANDLEUNEG=0, TODONOGOOD=NOGOODAVERAGE): the average of its
+// good neighbours from the bounded array (read-only during this phase), gamma
+// capped at its own previous gamma or GAMMAMAX; result to pfix.
+__device__ void fixup_zone(const KParams& P, const double* __restrict__ pv, const int* __restrict__ pflag,
+                           double* __restrict__ pfix, const double* __restrict__ gC, int z, int& fail) {
+  const int i = z / P.N2M - NG, j = z % P.N2M - NG;
+  double gc[36];
+#pragma unroll
+  for (int m = 0; m < 36; m++) gc[m] = gC[(size_t)z * GCN + m];
+  GeoU g;
+  geo_from_cent(gc, g);
+  double pz[NPR];
+  load8(pv, z, pz);
+  double gamma = gamma_calc(pz, g);
+  if (gamma > P.gammamax) gamma = P.gammamax;
+  auto good = [&](int di, int dj) { return pflag_at(P, pflag, i + di, j + dj) == 0; };
+  auto val = [&](int di, int dj, int k) { return pv[(size_t)zidx(P, i + di, j + dj) * 8 + k]; };
+  auto avg41 = [&](int k) { return 0.25 * (val(0, 1, k) + val(0, -1, k) + val(-1, 0, k) + val(1, 0, k)); };
+  auto avg42 = [&](int k) { return 0.25 * (val(1, 1, k) + val(1, -1, k) + val(-1, 1, k) + val(-1, -1, k)); };
+  int mode;
+  if (good(0, 1) && good(0, -1) && good(1, 0) && good(-1, 0)) mode = 0;
+  else if (good(1, 1) && good(1, -1) && good(-1, 1) && good(-1, -1)) mode = 1;
+  else if (good(0, 1) && good(0, -1)) mode = 2;
+  else if (good(1, 0) && good(-1, 0)) mode = 3;
+  else if (good(1, 1) && good(-1, -1)) mode = 4;
+  else if (good(1, -1) && good(-1, 1)) mode = 5;
+  else if (good(1, 1)) mode = 6;
+  else if (good(1, 0)) mode = 7;
+  else if (good(1, -1)) mode = 8;
+  else if (good(0, -1)) mode = 9;
+  else if (good(-1, -1)) mode = 10;
+  else if (good(-1, 0)) mode = 11;
+  else if (good(-1, 1)) mode = 12;
+  else if (good(0, 1)) mode = 13;
+  else mode = 14;
+  for (int k = 0; k <= U3; k++) {
+    double v;
+    switch (mode) {
+      case 0: v = avg41(k); break;
+      case 1: v = avg42(k); break;
+      case 2: v = (0.5 * (val(0, 1, k) + val(0, -1, k))); break;
+      case 3: v = (0.5 * (val(1, 0, k) + val(-1, 0, k))); break;
+      case 4: v = (0.5 * (val(1, 1, k) + val(-1, -1, k))); break;
+      case 5: v = (0.5 * (val(1, -1, k) + val(-1, 1, k))); break;
+      case 6: v = val(1, 1, k); break;
+      case 7: v = val(1, 0, k); break;
+      case 8: v = val(1, -1, k); break;
+      case 9: v = val(0, -1, k); break;
+      case 10: v = val(-1, -1, k); break;
+      case 11: v = val(-1, 0, k); break;
+      case 12: v = val(-1, 1, k); break;
+      case 13: v = val(0, 1, k); break;
+      default: v = 0.5 * (avg41(k) + avg42(k)); break;
+    }
+    pz[k] = v;
+  }
+  if (limit_gamma(gamma, pz, g) >= 1) fail = 1;
+  double* o = pfix + (size_t)z * 8;
+#pragma unroll
+  for (int k = 0; k <= U3; k++) o[k] = pz[k];
+}
+
+constexpr int KB_BOUND = 1;     // bound_prim()
+constexpr int KB_FIXUP = 2;     // post_fixup(): bound_pflag + fixup_utoprim on the failed-zone list
+constexpr int KB_BOUND2 = 8;    // bound_prim() after the fixup (step_ch.c:124)
+constexpr int KB_DT = 4;        // t,dt update (step_ch.c:174-183)
+
+// Everything between two advance() calls, as one workgroup with barriers
+// between the phases: bound_prim, post_fixup, bound_prim, and after the last
+// substage the time/dt update.  The failed-zone list is filled by k_update.
+__global__ __launch_bounds__(1024) void k_boundary(KParams P, double* __restrict__ pv, const double* __restrict__ gC,
+                                                   int* __restrict__ pflag, const int* __restrict__ faillist,
+                                                   double* __restrict__ pfix, DevScalars* __restrict__ ds, int mode) {
+  int fail = 0;
+  if (mode & KB_BOUND) bound_block(P, pv, gC, fail);
+  if (mode & KB_FIXUP) {
+    int n = ds->nlist;
+    if (n > P.n1 * P.n2) n = P.n1 * P.n2;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) fixup_zone(P, pv, pflag, pfix, gC, faillist[t], fail);
+    __syncthreads();
+    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+      const int z = faillist[t];
+#pragma unroll
+      for (int k = 0; k <= U3; k++) pv[(size_t)z * 8 + k] = pfix[(size_t)z * 8 + k];
+      pflag[z] = -1;  // UTOPRIMFAILFIXED (global.h:524)
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) ds->nlist = 0;
+  }
+  if (mode & KB_BOUND2) bound_block(P, pv, gC, fail);
+  if ((mode & KB_DT) && threadIdx.x == 0) {
+    // step_ch.c:174-183 with advance.c:190 (ndt of the last substage)
+    double ndt = P.defcon * 1. / (1. / ds->ndt_last[0] + 1. / ds->ndt_last[1]);
+    double dt = ds->dt;
+    ds->t += dt;
+    if (ndt > P.safe * dt) ndt = P.safe * dt;
+    dt = ndt;
+    if (ds->t + dt > ds->tf) dt = ds->tf - ds->t;
+    ds->dt = dt;
+  }
+  if (fail) atomicOr(&ds->failed, 1);
 }
 
 __global__ void k_utoprim_batch(KParams P, int n, const int* __restrict__ zone, const double* __restrict__ U,
@@ -717,8 +872,12 @@ struct harm_ctx {
   KParams kp;
   int N1M, N2M, nz;
   double* arr[HARM_NARRAYS];
-  double* emf;
-  int* pflag;
+  double* q1;     // F1[B2]/gdet(FACE1) per face
+  double* q2;     // F2[B1]/gdet(FACE2) per face
+  double* emf;    // corner EMFs (standalone flux_ct only)
+  double* pfix;   // repaired states of failed zones
+  int* pflag;     // FLAGUTOPRIMFAIL per zone
+  int* faillist;  // zones whose inversion failed in the current substage
   double* gF;
   double* gC;
   double* gConn;
@@ -729,7 +888,9 @@ struct harm_ctx {
   bool graph_ok;
   bool geometry_set;
   bool prof;
-  hipEvent_t ev0, ev1;
+  std::vector<hipEvent_t>* evpool;  // pairs (start, end) recorded without waiting
+  std::vector<int>* evkid;          // kernel id of each recorded pair
+  int evused;
   double ktime[HARM_K_COUNT];
   long long kcount[HARM_K_COUNT];
 };
@@ -784,23 +945,28 @@ extern "C" int harm_create(const harm_params* prm, harm_ctx** out) {
   k.gammamax = prm->gammamax; k.gammafail = prm->gammafail;
   k.rhominlimit = prm->rhominlimit; k.uuminlimit = prm->uuminlimit;
   k.dx1 = prm->dx1; k.dx2 = prm->dx2;
+  c->evpool = new std::vector<hipEvent_t>();
+  c->evkid = new std::vector<int>();
   size_t nz = (size_t)c->nz;
   hipError_t e = hipSuccess;
   for (int a = 0; a < HARM_NARRAYS && e == hipSuccess; a++) {
     e = hipMalloc(&c->arr[a], nz * 8 * sizeof(double));
     if (e == hipSuccess) e = hipMemset(c->arr[a], 0, nz * 8 * sizeof(double));
   }
-  if (e == hipSuccess) e = hipMalloc(&c->emf, nz * sizeof(double));
-  if (e == hipSuccess) e = hipMemset(c->emf, 0, nz * sizeof(double));
+  double** dbufs[4] = {&c->q1, &c->q2, &c->emf, &c->pfix};
+  size_t dsz[4] = {nz, nz, nz, nz * 8};
+  for (int b = 0; b < 4 && e == hipSuccess; b++) {
+    e = hipMalloc(dbufs[b], dsz[b] * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(*dbufs[b], 0, dsz[b] * sizeof(double));
+  }
   if (e == hipSuccess) e = hipMalloc(&c->pflag, nz * sizeof(int));
   if (e == hipSuccess) e = hipMemset(c->pflag, 0, nz * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&c->faillist, nz * sizeof(int));
   if (e == hipSuccess) e = hipMalloc(&c->gF, 2 * nz * GF * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&c->gC, nz * GCN * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&c->gConn, nz * 64 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&c->ds, sizeof(DevScalars));
   if (e == hipSuccess) e = hipStreamCreate(&c->own_stream);
-  if (e == hipSuccess) e = hipEventCreate(&c->ev0);
-  if (e == hipSuccess) e = hipEventCreate(&c->ev1);
   if (e != hipSuccess) {
     seterr("harm_create: %s", hipGetErrorString(e));
     harm_destroy(c);
@@ -827,15 +993,22 @@ extern "C" int harm_create(const harm_params* prm, harm_ctx** out) {
 extern "C" int harm_destroy(harm_ctx* c) {
   if (!c) return 0;
   if (c->graph_ok) (void)hipGraphExecDestroy(c->graph);
-  for (int a = 0; a < HARM_NARRAYS; a++) if (c->arr[a]) (void)hipFree(c->arr[a]);
-  if (c->emf) (void)hipFree(c->emf);
+  for (int a = 0; a < HARM_NARRAYS; a++)
+    if (c->arr[a]) (void)hipFree(c->arr[a]);
+  double* dbufs[4] = {c->q1, c->q2, c->emf, c->pfix};
+  for (double* b : dbufs)
+    if (b) (void)hipFree(b);
   if (c->pflag) (void)hipFree(c->pflag);
+  if (c->faillist) (void)hipFree(c->faillist);
   if (c->gF) (void)hipFree(c->gF);
   if (c->gC) (void)hipFree(c->gC);
   if (c->gConn) (void)hipFree(c->gConn);
   if (c->ds) (void)hipFree(c->ds);
-  if (c->ev0) (void)hipEventDestroy(c->ev0);
-  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->evpool) {
+    for (hipEvent_t e : *c->evpool) (void)hipEventDestroy(e);
+    delete c->evpool;
+  }
+  delete c->evkid;
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   free(c);
   return 0;
@@ -936,66 +1109,77 @@ extern "C" int harm_get_time(harm_ctx* c, double* t, double* dt) {
   return 0;
 }
 
-// ---- launch helpers (optionally timed with events on the launch stream) ----
+// ---- launch helpers ----
 
 static inline dim3 grid2(int nj, int ni, dim3 b) { return dim3((nj + b.x - 1) / b.x, (ni + b.y - 1) / b.y); }
 
-static int prof_begin(harm_ctx* c) {
-  if (c->prof) HIPCHK(hipEventRecord(c->ev0, c->stream));
-  return 0;
-}
-static int prof_end(harm_ctx* c, int kid) {
-  if (c->prof) {
-    HIPCHK(hipEventRecord(c->ev1, c->stream));
-    HIPCHK(hipEventSynchronize(c->ev1));
+// Per-kernel timing: a start/end event pair is recorded around each timed
+// launch without waiting; pairs are drained (synchronise + elapsed time) when
+// the pool fills or when the totals are read.
+static int prof_drain(harm_ctx* c) {
+  if (!c->evused) return 0;
+  HIPCHK(hipEventSynchronize((*c->evpool)[2 * (c->evused - 1) + 1]));
+  for (int k = 0; k < c->evused; k++) {
     float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    HIPCHK(hipEventElapsedTime(&ms, (*c->evpool)[2 * k], (*c->evpool)[2 * k + 1]));
+    int kid = (*c->evkid)[k];
     c->ktime[kid] += ms;
     c->kcount[kid] += 1;
   }
+  c->evused = 0;
+  return 0;
+}
+static int prof_begin(harm_ctx* c, int kid) {
+  if (!c->prof) return 0;
+  if (c->evused == 2048 && prof_drain(c)) return 1;
+  if ((int)c->evpool->size() < 2 * (c->evused + 1)) {
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    c->evpool->push_back(a);
+    c->evpool->push_back(b);
+    c->evkid->push_back(0);
+  }
+  (*c->evkid)[c->evused] = kid;
+  HIPCHK(hipEventRecord((*c->evpool)[2 * c->evused], c->stream));
+  return 0;
+}
+static int prof_end(harm_ctx* c) {
+  if (!c->prof) return 0;
+  HIPCHK(hipEventRecord((*c->evpool)[2 * c->evused + 1], c->stream));
+  c->evused++;
   return 0;
 }
 
-static int launch_flux(harm_ctx* c, int pr_sel, int dir) {
+// fluxcalc for dir0..dir0+ndir-1 (both directions in one launch when ndir=2)
+static int launch_flux(harm_ctx* c, int pr_sel, int dir0, int ndir) {
   const dim3 b(64, 4);
-  if (prof_begin(c)) return 1;
-  if (dir == 1)
-    hipLaunchKernelGGL(k_flux<1>, grid2(c->kp.n2 + 2, c->kp.n1 + 1, b), b, 0, c->stream, c->kp, c->arr[pr_sel],
-                       c->arr[HARM_F1], c->gF, c->ds);
-  else
-    hipLaunchKernelGGL(k_flux<2>, grid2(c->kp.n2 + 1, c->kp.n1 + 2, b), b, 0, c->stream, c->kp, c->arr[pr_sel],
-                       c->arr[HARM_F2], c->gF + (size_t)c->nz * GF, c->ds);
+  const int nfj = c->kp.n2 + 2, nfi = c->kp.n1 + 2;  // covers both directions' face ranges
+  dim3 g((nfj + 31) / 32, (nfi + 3) / 4, ndir);
+  if (prof_begin(c, HARM_K_FLUX)) return 1;
+  hipLaunchKernelGGL(k_flux, g, b, 0, c->stream, c->kp, c->arr[pr_sel], c->arr[HARM_F1], c->arr[HARM_F2], c->q1,
+                     c->q2, c->gF, c->ds, dir0);
   HIPCHK(hipGetLastError());
-  return prof_end(c, dir == 1 ? HARM_K_FLUX1 : HARM_K_FLUX2);
-}
-
-static int launch_emf(harm_ctx* c) {
-  const dim3 b(64, 4);
-  if (prof_begin(c)) return 1;
-  hipLaunchKernelGGL(k_emf, grid2(c->kp.n2 + 1, c->kp.n1 + 1, b), b, 0, c->stream, c->kp, c->arr[HARM_F1],
-                     c->arr[HARM_F2], c->gF, c->gC, c->emf, c->ds);
-  HIPCHK(hipGetLastError());
-  return prof_end(c, HARM_K_EMF);
+  return prof_end(c);
 }
 
 static int launch_update(harm_ctx* c, int pi, int pb, int pf, const RKCoef& C) {
   const dim3 b(64, 4);
-  if (prof_begin(c)) return 1;
+  if (prof_begin(c, HARM_K_UPDATE)) return 1;
   hipLaunchKernelGGL(k_update, grid2(c->kp.n2, c->kp.n1, b), b, 0, c->stream, c->kp, C, c->arr[pi], c->arr[pb],
-                     c->arr[pf], c->arr[HARM_ULAST], c->arr[HARM_UNEW], c->arr[HARM_F1], c->arr[HARM_F2], c->emf,
-                     c->gC, c->gConn, c->pflag, c->ds);
+                     c->arr[pf], c->arr[HARM_ULAST], c->arr[HARM_UNEW], c->arr[HARM_F1], c->arr[HARM_F2], c->q1,
+                     c->q2, c->gC, c->gConn, c->pflag, c->faillist, c->ds);
   HIPCHK(hipGetLastError());
-  return prof_end(c, HARM_K_UPDATE);
+  return prof_end(c);
 }
 
-static int launch_bound(harm_ctx* c, int sel) {
-  if (prof_begin(c)) return 1;
-  hipLaunchKernelGGL(k_bound_x1, dim3((c->kp.n2 + 63) / 64, 2 * NG), dim3(64), 0, c->stream, c->kp, c->arr[sel],
-                     c->gC, c->ds);
-  hipLaunchKernelGGL(k_bound_x2, dim3((c->kp.n1 + 2 * NG + 63) / 64, 8), dim3(64), 0, c->stream, c->kp,
-                     c->arr[sel]);
+static int launch_boundary(harm_ctx* c, int sel, int mode) {
+  if (c->prm.eomtype != HARM_EOMGRMHD) mode &= ~KB_FIXUP;
+  if (prof_begin(c, HARM_K_BOUNDARY)) return 1;
+  hipLaunchKernelGGL(k_boundary, dim3(1), dim3(1024), 0, c->stream, c->kp, c->arr[sel], c->gC, c->pflag,
+                     c->faillist, c->pfix, c->ds, mode);
   HIPCHK(hipGetLastError());
-  return prof_end(c, HARM_K_BOUND);
+  return prof_end(c);
 }
 
 static void rk_setup(int timeorder, int* nto, double CUf[4][3], double Cunew[4][3], int pii[4], int pbb[4],
@@ -1036,16 +1220,14 @@ static void rk_setup(int timeorder, int* nto, double CUf[4][3], double Cunew[4][
   }
 }
 
-// one substage without host synchronisation (fluxcalc x2, emf, update)
+// advance(): fluxcalc both directions (+ fix_flux, flux_ct formed in k_update), update
 static int substage(harm_ctx* c, int pi, int pb, int pf, const double* CUf, const double* Cunew, int to, int nto) {
   RKCoef C;
   C.cuf0 = CUf[0]; C.cuf1 = CUf[1]; C.cuf2 = CUf[2];
   C.cn0 = Cunew[0]; C.cn1 = Cunew[1]; C.cn2 = Cunew[2];
   C.timeorder = to;
   C.last = (to == nto - 1);
-  if (launch_flux(c, pb, 1)) return 1;
-  if (launch_flux(c, pb, 2)) return 1;
-  if (launch_emf(c)) return 1;
+  if (launch_flux(c, pb, 1, 2)) return 1;
   if (launch_update(c, pi, pb, pf, C)) return 1;
   return 0;
 }
@@ -1056,13 +1238,9 @@ static int enqueue_step(harm_ctx* c) {
   rk_setup(c->prm.timeorder, &nto, CUf, Cunew, pii, pbb, pff);
   for (int to = 0; to < nto; to++) {
     if (substage(c, pii[to], pbb[to], pff[to], CUf[to], Cunew[to], to, nto)) return 1;
-    if (to == nto - 1) {
-      hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, c->stream, c->kp, c->ds);
-      HIPCHK(hipGetLastError());
-    }
-    // bound_prim; post_fixup (fixup_utoprim) is not applied on the GPU path yet,
-    // so the second bound_prim of step_ch.c:124 would be a repeat of this one
-    if (launch_bound(c, pff[to])) return 1;
+    // step_ch.c:104-124 bound_prim, post_fixup, bound_prim; then :174-183 after the last substage
+    int mode = KB_BOUND | KB_FIXUP | KB_BOUND2 | ((to == nto - 1) ? KB_DT : 0);
+    if (launch_boundary(c, pff[to], mode)) return 1;
   }
   return 0;
 }
@@ -1077,7 +1255,7 @@ extern "C" int harm_fluxcalc(harm_ctx* c, int pr_sel, int dir, double* ndt) {
   if (need_geom(c)) return 1;
   if (pr_sel < 0 || pr_sel > HARM_PK1) return seterr("pr_sel must be HARM_P/PK0/PK1");
   if (dir != 1 && dir != 2) return seterr("dir must be 1 or 2");
-  if (launch_flux(c, pr_sel, dir)) return 1;
+  if (launch_flux(c, pr_sel, dir, 1)) return 1;
   unsigned long long bits;
   HIPCHK(hipMemcpyAsync(&bits, &c->ds->ndt_acc[dir - 1], 8, hipMemcpyDeviceToHost, c->stream));
   unsigned long long big;
@@ -1090,8 +1268,7 @@ extern "C" int harm_fluxcalc(harm_ctx* c, int pr_sel, int dir, double* ndt) {
 }
 
 extern "C" int harm_fix_flux(harm_ctx* c) {
-  (void)c;
-  // the fix_flux() edge rules are applied inside the flux kernels when
+  // the fix_flux() edge rules are applied inside the flux kernel when
   // params.fixupflux is set; nothing is left to do here
   return need_geom(c);
 }
@@ -1099,13 +1276,12 @@ extern "C" int harm_fix_flux(harm_ctx* c) {
 extern "C" int harm_flux_ct(harm_ctx* c) {
   if (need_geom(c)) return 1;
   if (c->prm.fluxb == HARM_FLUXCTHLL) return 0;
-  // keep the Courant accumulators: k_emf snapshots/resets them, so restore after
-  DevScalars save;
-  HIPCHK(hipMemcpyAsync(&save, c->ds, sizeof(save), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  if (launch_emf(c)) return 1;
-  HIPCHK(hipMemcpyAsync(c->ds, &save, sizeof(save), hipMemcpyHostToDevice, c->stream));
   const dim3 b(64, 4);
+  if (prof_begin(c, HARM_K_EMF)) return 1;
+  hipLaunchKernelGGL(k_emf, grid2(c->kp.n2 + 1, c->kp.n1 + 1, b), b, 0, c->stream, c->kp, c->q1, c->q2, c->gC,
+                     c->emf);
+  HIPCHK(hipGetLastError());
+  if (prof_end(c)) return 1;
   hipLaunchKernelGGL(k_ctapply, grid2(c->kp.n2 + 1, c->kp.n1 + 1, b), b, 0, c->stream, c->kp, c->arr[HARM_F1],
                      c->arr[HARM_F2], c->emf);
   HIPCHK(hipGetLastError());
@@ -1130,7 +1306,15 @@ extern "C" int harm_advance(harm_ctx* c, int pi, int pb, const double CUf[3], in
 extern "C" int harm_bound_prim(harm_ctx* c, int sel) {
   if (need_geom(c)) return 1;
   if (sel < 0 || sel > HARM_PK1) return seterr("sel must be HARM_P/PK0/PK1");
-  if (launch_bound(c, sel)) return 1;
+  if (launch_boundary(c, sel, KB_BOUND)) return 1;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int harm_post_fixup(harm_ctx* c, int sel) {
+  if (need_geom(c)) return 1;
+  if (sel < 0 || sel > HARM_PK1) return seterr("sel must be HARM_P/PK0/PK1");
+  if (launch_boundary(c, sel, KB_FIXUP)) return 1;
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1234,6 +1418,7 @@ extern "C" int harm_prof_enable(harm_ctx* c, int on) {
 extern "C" int harm_prof_get(harm_ctx* c, int kernel, double* total_ms, long long* count) {
   if (!c) return seterr("null ctx");
   if (kernel < 0 || kernel >= HARM_K_COUNT) return seterr("bad kernel id %d", kernel);
+  if (prof_drain(c)) return 1;
   if (total_ms) *total_ms = c->ktime[kernel];
   if (count) *count = c->kcount[kernel];
   return 0;
@@ -1241,6 +1426,7 @@ extern "C" int harm_prof_get(harm_ctx* c, int kernel, double* total_ms, long lon
 
 extern "C" int harm_prof_reset(harm_ctx* c) {
   if (!c) return seterr("null ctx");
+  if (prof_drain(c)) return 1;
   for (int k = 0; k < HARM_K_COUNT; k++) {
     c->ktime[k] = 0;
     c->kcount[k] = 0;

@@ -132,6 +132,9 @@ int harm_get_time(harm_ctx *ctx, double *t, double *dt);
 int harm_fluxcalc(harm_ctx *ctx, int pr_sel, int dir, double *ndt);
 /* fixup.c:1357 fix_flux(pb, F1, F2) on the context fluxes. */
 int harm_fix_flux(harm_ctx *ctx);
+/* fixup.c:37 post_fixup(stage, pv, finalstep): bound_pflag + fixup_utoprim
+   (repair of zones whose inversion failed) on array sel. */
+int harm_post_fixup(harm_ctx *ctx, int sel);
 /* step_ch.c:1159 flux_ct(stage, pb, F1, F2): constrained-transport EMFs
    rewritten into F1/F2. */
 int harm_flux_ct(harm_ctx *ctx);
@@ -161,14 +164,16 @@ int harm_get_failures(harm_ctx *ctx, int *nfailzones, int *failed);
 /* Newton-iteration counter (the reference's nstroke, main.c:72-78). */
 int harm_get_nstroke(harm_ctx *ctx, long long *nstroke);
 
-/* Per-kernel timing with HIP events on the launch stream.
-   kernel ids: 0 flux dir1, 1 flux dir2, 2 emf, 3 update, 4 bound */
-#define HARM_K_FLUX1 0
-#define HARM_K_FLUX2 1
+/* Per-kernel timing with HIP events on the launch stream, ids:
+   HARM_K_FLUX     k_flux     fluxcalc, both directions in one launch
+   HARM_K_EMF      k_emf      corner EMFs (standalone harm_flux_ct only)
+   HARM_K_UPDATE   k_update   advance() per zone incl. CT fluxes and inversion
+   HARM_K_BOUNDARY k_boundary bound_prim + post_fixup + bound_prim (+ dt) */
+#define HARM_K_FLUX 0
 #define HARM_K_EMF 2
 #define HARM_K_UPDATE 3
-#define HARM_K_BOUND 4
-#define HARM_K_COUNT 5
+#define HARM_K_BOUNDARY 4
+#define HARM_K_COUNT 6
 int harm_prof_enable(harm_ctx *ctx, int on);
 int harm_prof_get(harm_ctx *ctx, int kernel, double *total_ms, long long *count);
 int harm_prof_reset(harm_ctx *ctx);

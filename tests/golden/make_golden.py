@@ -1,0 +1,127 @@
+"""Generate golden vectors from the COMPILED REFERENCE (run in the build
+container, where /root/reference exists; the GPU box only reads the .npz).
+
+For each case the script builds the input state and geometry with
+harm_ramt_amd's host generators, runs the reference's own step_ch() through
+oracle/_ref/ref_{ks,mink} (see oracle/ref/), and stores
+  - sha256 of the input state and of the geometry tables (pins the inputs),
+  - per step: t, dt, Newton iteration count (nstroke), failure count,
+  - the full primitive state p[N1+6][N2+6][8] after the listed steps.
+The reference's set_grid() tables for the torus grid are stored as a sha256
+plus a sample, pinning the product's geometry generator.
+
+    python tests/golden/make_golden.py
+"""
+import hashlib
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+import harm_ramt_amd as H  # noqa: E402
+
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+OUT_DIR = os.path.dirname(os.path.abspath(__file__))
+
+# name: (kind, n1, n2, lim, fluxmethod, timeorder, nsteps, dt0, saved steps)
+CASES = {
+    "torus_para_hll_rk4": ("ks", 32, 64, H.PARA, H.HLLFLUX, 4, 60, 1e-3, (1, 20, 60)),
+    "torus_mc_laxf_rk2": ("ks", 32, 64, H.MC, H.LAXFFLUX, 2, 10, 1e-3, (10,)),
+    "torus_vanl_hll_rk3": ("ks", 32, 64, H.VANL, H.HLLFLUX, 3, 5, 1e-3, (5,)),
+    "torus_minm_laxf_rk1": ("ks", 32, 64, H.MINM, H.LAXFFLUX, 1, 5, 1e-3, (5,)),
+    "torus_csslope_hll_rk4": ("ks", 32, 64, H.CSSLOPE, H.HLLFLUX, 4, 3, 1e-3, (3,)),
+    "ot_mc_hll_rk4": ("mink", 32, 32, H.MC, H.HLLFLUX, 4, 40, 1e-4, (1, 40)),
+    "ot_para_laxf_rk2": ("mink", 32, 32, H.PARA, H.LAXFFLUX, 2, 10, 1e-4, (10,)),
+    "ot_donor_hll_rk1": ("mink", 32, 32, H.DONOR, H.HLLFLUX, 1, 5, 1e-4, (5,)),
+    "ot_nlimcent_hll_rk3": ("mink", 32, 32, H.NLIMCENT, H.HLLFLUX, 3, 5, 1e-4, (5,)),
+}
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.float64).tobytes()).hexdigest()
+
+
+def problem(kind, n1, n2):
+    """Input state, geometry and the parameter dict shared by reference, oracle, GPU."""
+    if kind == "ks":
+        g = H.grid_mks(n1, n2)
+        geom = H.Geometry(g)
+        p0 = H.init_fishmon(geom)
+        gam, cour, bc, fix, pole = 4.0 / 3.0, 0.9, (H.OUTFLOW, H.OUTFLOW, H.POLARAXIS, H.POLARAXIS), 1, 1
+    else:
+        g = H.grid_cartesian(n1, n2)
+        geom = H.Geometry(g)
+        p0 = H.init_orszag_tang(geom)
+        gam, cour, bc, fix, pole = 5.0 / 3.0, 0.8, (H.PERIODIC,) * 4, 0, 0
+    return g, geom, p0, dict(gam=gam, cour=cour, bc=bc, fixupflux=fix, poledeath=pole)
+
+
+def run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work):
+    p0.tofile(os.path.join(work, "p0.bin"))
+    cfg = dict(prim=os.path.join(work, "p0.bin"), dt=dt0, nsteps=nsteps, out=os.path.join(work, "o.bin"),
+               lim=lim, fluxmethod=flux, timeorder=to)
+    if kind == "ks":
+        cfg.update(a=g.a, Rout=g.Rout, hslope=g.hslope, R0=g.R0, geom="ref", gam=4.0 / 3.0, cour=0.9,
+                   tables_out=os.path.join(work, "tables.bin"))
+    else:
+        names = ["gcov", "gcon", "gdet", "conn"]
+        for nm, a in zip(names, geom.tables()[:4]):
+            a.tofile(os.path.join(work, f"mk_{nm}.bin"))
+        cfg.update(geom=os.path.join(work, "mk_gcov.bin"), geom_gcon=os.path.join(work, "mk_gcon.bin"),
+                   geom_gdet=os.path.join(work, "mk_gdet.bin"), geom_conn=os.path.join(work, "mk_conn.bin"),
+                   Rin=1.0, R0=0.0, Rout=float(np.e), startx1=0.0, startx2=0.0, dx1=g.dx1, dx2=g.dx2, hslope=1.0,
+                   gam=5.0 / 3.0, cour=0.8)
+    path = os.path.join(work, "run.cfg")
+    with open(path, "w") as f:
+        for k, v in cfg.items():
+            f.write(f"{k} {v!r}\n" if isinstance(v, float) else f"{k} {v}\n")
+    subprocess.run([os.path.join(REF_DIR, "ref_" + kind), path], cwd=work, check=True, capture_output=True)
+    d = np.fromfile(os.path.join(work, "o.bin"), dtype=np.float64)[4:]
+    nz = (g.n1 + 6) * (g.n2 + 6) * 8
+    recs, states = [], []
+    for s in range(nsteps):
+        r = d[s * (4 + nz):(s + 1) * (4 + nz)]
+        recs.append(r[:4])
+        states.append(r[4:].reshape(g.n1 + 6, g.n2 + 6, 8))
+    return np.array(recs), states
+
+
+def main():
+    subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle", "ref"), "-s"])
+    with tempfile.TemporaryDirectory() as work:
+        for name, (kind, n1, n2, lim, flux, to, nsteps, dt0, saved) in CASES.items():
+            g, geom, p0, extra = problem(kind, n1, n2)
+            recs, states = run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work)
+            out = dict(kind=kind, n1=n1, n2=n2, lim=lim, fluxmethod=flux, timeorder=to, nsteps=nsteps, dt0=dt0,
+                       input_sha=sha(p0), geom_sha=sha(np.concatenate([a.ravel() for a in geom.tables()])),
+                       t=recs[:, 0], dt=recs[:, 1], nstroke=recs[:, 2], failcode=recs[:, 3],
+                       saved_steps=np.array(saved))
+            for s in saved:
+                out[f"p_{s}"] = states[s - 1]
+            np.savez_compressed(os.path.join(OUT_DIR, f"{name}.npz"), **out)
+            print(name, "nstroke", int(recs[:, 2].sum()), "failures", int((recs[:, 3] // 2).sum()))
+        # reference set_grid() tables for the 32x64 torus grid
+        kind, n1, n2 = "ks", 32, 64
+        g, geom, p0, extra = problem(kind, n1, n2)
+        tb = np.fromfile(os.path.join(work, "tables.bin"), dtype=np.float64)
+        nz = (n1 + 6) * (n2 + 6)
+        sizes = [nz * 64, nz * 64, nz * 4, nz * 64]
+        parts, o = [], 0
+        for sz in sizes:
+            parts.append(tb[o:o + sz])
+            o += sz
+        rin, sx1, sx2, dx1, dx2 = tb[o:o + 5]
+        allt = np.concatenate(parts)
+        np.savez_compressed(os.path.join(OUT_DIR, "ks_geometry_32x64.npz"), sha=hashlib.sha256(allt.tobytes()).hexdigest(),
+                            sample_idx=np.arange(0, allt.size, 997), sample=allt[::997],
+                            Rin=rin, startx=np.array([sx1, sx2]), dx=np.array([dx1, dx2]),
+                            sizes=np.array(sizes))
+        print("geometry tables sha", hashlib.sha256(allt.tobytes()).hexdigest()[:16])
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,45 @@
+"""Problem set-up shared by the oracle tests and the GPU parity tests."""
+import glob
+import hashlib
+import os
+
+import numpy as np
+
+import harm_ramt_amd as H
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.float64).tobytes()).hexdigest()
+
+
+def problem(kind, n1, n2, lim=H.PARA, fluxmethod=H.HLLFLUX, timeorder=4, seed=7):
+    """(grid, geometry, initial state, harm_params) for a torus ('ks') or an
+    Orszag-Tang ('mink') problem."""
+    if kind == "ks":
+        g = H.grid_mks(n1, n2)
+        geom = H.Geometry(g)
+        p0 = H.init_fishmon(geom, seed=seed)
+        prm = H.make_params(n1, n2, g.dx1, g.dx2, lim=lim, fluxmethod=fluxmethod, timeorder=timeorder)
+    else:
+        g = H.grid_cartesian(n1, n2)
+        geom = H.Geometry(g)
+        p0 = H.init_orszag_tang(geom)
+        prm = H.make_params(n1, n2, g.dx1, g.dx2, lim=lim, fluxmethod=fluxmethod, timeorder=timeorder,
+                            gam=5.0 / 3.0, cour=0.8, bc=(H.PERIODIC,) * 4, fixupflux=0, poledeath=0)
+    return g, geom, p0, prm
+
+
+def golden_cases():
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
+                  if not os.path.basename(p).startswith("ks_geometry"))
+
+
+def load_golden(name):
+    return np.load(os.path.join(GOLDEN, name + ".npz"))
+
+
+def same(a, b):
+    """IEEE equality elementwise (+0 == -0), NaN never equal."""
+    return np.array_equal(a, b)

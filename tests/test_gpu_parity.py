@@ -1,0 +1,185 @@
+"""Parity of the HIP path (through the C-ABI) with the oracle and with the
+reference's golden vectors.  The GPU step is required to be bit-identical
+(IEEE equality) to the CPU step: the library is built with -ffp-contract=off
+and keeps the reference's operation order.  The one exception is CSSLOPE,
+whose reference arithmetic is x87 long double (interp.c:202); there the bar
+is 1e-12 relative.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import harm_ramt_amd as H
+from oracle.oracle import Oracle
+from tests.helpers import golden_cases, load_golden, problem, same, sha
+
+pytestmark = pytest.mark.gpu
+NG = 3
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_ctx(geom, prm, p0, dt0):
+    h = H.Harm(prm, geom)
+    h.set_array(H.P, p0)
+    h.bound_prim(H.P)
+    h.set_time(0.0, dt0, 1e30)
+    return h
+
+
+def oracle_ctx(geom, prm, p0, dt0):
+    o = Oracle(H.params_dict(prm))
+    o.set_geometry(*geom.tables())
+    o.set_array(0, p0)
+    o.bound_prim(0)
+    o.set_time(0.0, dt0, 1e30)
+    return o
+
+
+def assert_close_or_same(a, b, exact, what):
+    if exact:
+        bad = np.argwhere(~((a == b) | (np.isnan(a) & np.isnan(b))))
+        assert len(bad) == 0, f"{what}: {len(bad)} values differ, first {bad[:3].tolist()}"
+    else:
+        np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-14 * np.abs(b).max(), err_msg=what)
+
+
+@pytest.mark.parametrize("name", golden_cases())
+def test_gpu_step_matches_reference(name):
+    gd = load_golden(name)
+    kind, n1, n2 = str(gd["kind"]), int(gd["n1"]), int(gd["n2"])
+    say(f"[case] {name} {kind} {n1}x{n2} lim={int(gd['lim'])} steps={int(gd['nsteps'])}")
+    g, geom, p0, prm = problem(kind, n1, n2, lim=int(gd["lim"]), fluxmethod=int(gd["fluxmethod"]),
+                               timeorder=int(gd["timeorder"]))
+    assert sha(p0) == str(gd["input_sha"])
+    exact = int(gd["lim"]) != H.CSSLOPE
+    h = gpu_ctx(geom, prm, p0, float(gd["dt0"]))
+    saved = set(int(s) for s in gd["saved_steps"])
+    ns_prev = 0
+    for s in range(1, int(gd["nsteps"]) + 1):
+        h.step_ch()
+        t, dt = h.get_time()
+        ns = h.nstroke()
+        if exact:
+            assert t == gd["t"][s - 1] and dt == gd["dt"][s - 1], f"step {s}: t,dt {t},{dt}"
+            assert ns - ns_prev == int(gd["nstroke"][s - 1]), f"step {s}: Newton iterations"
+        ns_prev = ns
+        if s in saved:
+            assert_close_or_same(h.get_array(H.P), gd[f"p_{s}"], exact, f"{name} step {s}")
+    assert h.failures()[1] == 0
+
+
+@pytest.mark.parametrize("kind,lim,flux", [("ks", H.PARA, H.HLLFLUX), ("ks", H.MC, H.LAXFFLUX),
+                                           ("mink", H.MC, H.HLLFLUX)])
+def test_gpu_entry_points_match_oracle(kind, lim, flux):
+    say(f"[case] entry points {kind} lim={lim} flux={flux}")
+    n1, n2 = 16, 32
+    g, geom, p0, prm = problem(kind, n1, n2, lim=lim, fluxmethod=flux)
+    h = gpu_ctx(geom, prm, p0, 1e-3)
+    o = oracle_ctx(geom, prm, p0, 1e-3)
+    assert same(h.get_array(H.P), o.get_array(0)), "bound_prim"
+    # fluxcalc: the GPU applies fix_flux's edge rules inside the flux kernels
+    nd1, nd2 = h.fluxcalc(H.P, 1), h.fluxcalc(H.P, 2)
+    od1, od2 = o.fluxcalc(0, 1), o.fluxcalc(0, 2)
+    if prm.fixupflux:
+        o.fix_flux()
+    assert nd1 == od1 and nd2 == od2, "Courant limits"
+    F1g, F2g, F1o, F2o = h.get_array(H.F1), h.get_array(H.F2), o.get_array(3), o.get_array(4)
+    assert same(F1g[NG:NG + n1 + 1, NG - 1:NG + n2 + 1], F1o[NG:NG + n1 + 1, NG - 1:NG + n2 + 1]), "F1"
+    assert same(F2g[NG - 1:NG + n1 + 1, NG:NG + n2 + 1], F2o[NG - 1:NG + n1 + 1, NG:NG + n2 + 1]), "F2"
+    # flux_ct
+    h.flux_ct()
+    o.flux_ct()
+    assert same(h.get_array(H.F1)[NG:NG + n1 + 1, NG:NG + n2], o.get_array(3)[NG:NG + n1 + 1, NG:NG + n2])
+    assert same(h.get_array(H.F2)[NG:NG + n1, NG:NG + n2 + 1], o.get_array(4)[NG:NG + n1, NG:NG + n2 + 1])
+    # advance: one RK4 substage (pi=p, pb=p, pf=pk0) then a second (pb=pk0, pf=pk1)
+    cu, cn = [1.0, 0.0, 0.5], [1.0, 1.0 / 6.0, 0.0]
+    ng, no = h.advance(H.P, H.P, cu, H.PK0, cn, 0, 4), o.advance(0, 0, cu, 1, cn, 0, 4)
+    assert ng == no
+    for sel in (H.PK0, H.ULAST, H.UNEW):
+        assert same(h.get_array(sel)[NG:-NG, NG:-NG], o.get_array(sel)[NG:-NG, NG:-NG]), f"advance array {sel}"
+    h.bound_prim(H.PK0)
+    o.bound_prim(1)
+    assert same(h.get_array(H.PK0), o.get_array(1)), "bound_prim(pk0)"
+    cu, cn = [1.0, 0.0, 0.5], [0.0, 1.0 / 3.0, 0.0]
+    assert h.advance(H.P, H.PK0, cu, H.PK1, cn, 1, 4) == o.advance(0, 1, cu, 2, cn, 1, 4)
+    for sel in (H.PK1, H.ULAST, H.UNEW):
+        assert same(h.get_array(sel)[NG:-NG, NG:-NG], o.get_array(sel)[NG:-NG, NG:-NG]), f"advance array {sel}"
+
+
+def test_gpu_utoprimgen_matches_oracle():
+    say("[case] utoprimgen batch 16x16 torus geometry, 4096 random states")
+    g, geom, p0, prm = problem("ks", 16, 16)
+    h = H.Harm(prm, geom)
+    o = oracle_ctx(geom, prm, p0, 1e-3)
+    rng = np.random.default_rng(11)
+    n = 4096
+    zones = rng.integers(-2, 18, size=(n, 2)).astype(np.int32)
+    U = np.zeros((n, 8))
+    guess = np.zeros((n, 8))
+    for t in range(n):
+        i, j = int(zones[t, 0]), int(zones[t, 1])
+        rho = 10 ** rng.uniform(-4, 0)
+        pr = np.array([rho, rho * 10 ** rng.uniform(-3, 1), *rng.normal(0, 1, 3), *(rng.normal(0, 0.3, 3) * np.sqrt(rho))])
+        U[t] = o.primtoU(i, j, 3, pr)
+        guess[t] = pr * (1 + rng.normal(0, 0.05, 8))
+    out_g, fl_g = h.utoprimgen(zones, U, guess)
+    fails = 0
+    for t in range(n):
+        out_o, fl_o, _ = o.utoprimgen(int(zones[t, 0]), int(zones[t, 1]), U[t], guess[t])
+        assert fl_g[t] == fl_o, f"pflag zone {t}"
+        assert same(out_g[t], out_o), f"state {t}"
+        fails += fl_o != 0
+    say(f"  inversions flagged: {fails} of {n} (both paths agree)")
+
+
+def test_gpu_graph_replay_equals_single_steps():
+    say("[case] harm_step_n (HIP graph) vs harm_step, 64x64 torus, 12 steps")
+    g, geom, p0, prm = problem("ks", 64, 64)
+    a = gpu_ctx(geom, prm, p0, 1e-3)
+    b = gpu_ctx(geom, prm, p0, 1e-3)
+    for _ in range(12):
+        a.step_ch()
+    b.step_n(12)
+    assert same(a.get_array(H.P), b.get_array(H.P))
+    assert a.get_time() == b.get_time()
+
+
+@pytest.mark.parametrize("kind,n", [("ks", 256), ("mink", 256)])
+def test_gpu_fullsize_matches_oracle(kind, n):
+    say(f"[case] full size {kind} {n}x{n}, 6 steps bitwise vs oracle")
+    lim = H.PARA if kind == "ks" else H.MC
+    g, geom, p0, prm = problem(kind, n, n, lim=lim)
+    dt0 = 1e-3 if kind == "ks" else 1e-4
+    h = gpu_ctx(geom, prm, p0, dt0)
+    o = oracle_ctx(geom, prm, p0, dt0)
+    for s in range(6):
+        h.step_ch()
+        o.step()
+    assert h.get_time() == o.get_time()
+    assert same(h.get_array(H.P), o.get_array(0))
+    assert h.nstroke() == o.nstroke()
+
+
+def test_gpu_fullsize_ot_invariants():
+    say("[case] OT 256x256 on the GPU: 30 steps, conservation and corner div B")
+    n = 256
+    g, geom, p0, prm = problem("mink", n, n, lim=H.MC)
+    h = gpu_ctx(geom, prm, p0, 1e-4)
+
+    def divb(p):
+        B1, B2 = p[NG:-NG + 1, NG:-NG + 1, 5], p[NG:-NG + 1, NG:-NG + 1, 6]
+        return (0.5 * (B1[1:, 1:] + B1[1:, :-1] - B1[:-1, 1:] - B1[:-1, :-1]) / g.dx1 +
+                0.5 * (B2[1:, 1:] + B2[:-1, 1:] - B2[1:, :-1] - B2[:-1, :-1]) / g.dx2)
+
+    d0 = divb(h.get_array(H.P))
+    h.step_n(30)
+    p = h.get_array(H.P)
+    assert np.all(np.isfinite(p))
+    assert np.max(np.abs(divb(p) - d0)) < 1e-11 * np.abs(p[..., 5:7]).max() / g.dx1
+    n_fail, failed = h.failures()
+    assert failed == 0
