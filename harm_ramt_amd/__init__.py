@@ -29,6 +29,8 @@ OUTFLOW, POLARAXIS, PERIODIC = 0, 4, 100
 X1UP, X1DN, X2UP, X2DN = 0, 1, 2, 3
 P, PK0, PK1, F1, F2, ULAST, UNEW = 0, 1, 2, 3, 4, 5, 6
 METRIC_MINK, METRIC_KS = 0, 2
+FLUX_FACE_64x4, FLUX_FACE_64x2, FLUX_PAIR = 0, 1, 2
+BOUND_KERNELS, BOUND_BLOCK = 0, 1
 K_FLUX, K_EMF, K_UPDATE, K_BOUNDARY = 0, 2, 3, 4
 KERNEL_NAMES = {K_FLUX: "k_flux", K_EMF: "k_emf", K_UPDATE: "k_update", K_BOUNDARY: "k_boundary"}
 
@@ -64,6 +66,7 @@ EXPORTS = [
     "harm_get_time", "harm_fluxcalc", "harm_fix_flux", "harm_flux_ct", "harm_advance",
     "harm_bound_prim", "harm_post_fixup", "harm_step", "harm_step_n", "harm_utoprimgen", "harm_get_failures",
     "harm_get_nstroke", "harm_prof_enable", "harm_prof_get", "harm_prof_reset",
+    "harm_set_tuning", "harm_get_tuning",
     "harm_grid_mks", "harm_grid_cartesian", "harm_geometry", "harm_init_fishmon",
     "harm_init_orszag_tang",
 ]
@@ -107,6 +110,8 @@ def load_library(path=LIB_PATH):
         "harm_prof_enable": [vp, ctypes.c_int],
         "harm_prof_get": [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong)],
         "harm_prof_reset": [vp],
+        "harm_set_tuning": [vp, ctypes.c_int, ctypes.c_int],
+        "harm_get_tuning": [vp, ip, ip],
         "harm_grid_mks": [ctypes.POINTER(HarmGrid), ctypes.c_int, ctypes.c_int, ctypes.c_double,
                           ctypes.c_double, ctypes.c_double, ctypes.c_double],
         "harm_grid_cartesian": [ctypes.POINTER(HarmGrid), ctypes.c_int, ctypes.c_int],
@@ -335,6 +340,14 @@ class Harm:
         v = ctypes.c_longlong()
         _chk(self.lib.harm_get_nstroke(self.h, ctypes.byref(v)), "harm_get_nstroke")
         return v.value
+
+    def set_tuning(self, flux_variant, bound_variant):
+        _chk(self.lib.harm_set_tuning(self.h, flux_variant, bound_variant), "harm_set_tuning")
+
+    def get_tuning(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _chk(self.lib.harm_get_tuning(self.h, ctypes.byref(a), ctypes.byref(b)), "harm_get_tuning")
+        return a.value, b.value
 
     def prof_enable(self, on=True):
         _chk(self.lib.harm_prof_enable(self.h, 1 if on else 0), "harm_prof_enable")

@@ -277,6 +277,149 @@ __global__ __launch_bounds__(256) void k_flux(KParams P, const double* __restric
   if (fail) atomicOr(&ds->failed, 1);
 }
 
+// fluxcalc() with one thread per face (the whole face on one lane); same
+// arithmetic as flux_face<DIR>, different work split.
+template <int DIR>
+__device__ __forceinline__ void flux_face_single(const KParams& P, const double* __restrict__ pr,
+                                                 double* __restrict__ F, double* __restrict__ q,
+                                                 const double* __restrict__ gF, int i, int j, double& dtij,
+                                                 int& fail) {
+  const int zstride = (DIR == 1) ? P.N2M : 1;
+  const int z0 = zidx(P, i, j);
+  double p_l[NPR], p_r[NPR];
+  if (P.lim == 4) {
+    double s[6][NPR];
+#pragma unroll
+    for (int m = 0; m < 6; m++) load8(pr, z0 + (m - 3) * zstride, s[m]);
+#pragma unroll
+    for (int k = 0; k < NPR; k++) {
+      double l, r, l2, r2;
+      para4(s[0][k], s[1][k], s[2][k], s[3][k], s[4][k], l, r);
+      para4(s[1][k], s[2][k], s[3][k], s[4][k], s[5][k], l2, r2);
+      p_l[k] = r;
+      p_r[k] = l2;
+    }
+  } else {
+    double s[6][NPR];
+#pragma unroll
+    for (int m = 0; m < 6; m++) load8(pr, z0 + (m - 3) * zstride, s[m]);
+#pragma unroll
+    for (int k = 0; k < NPR; k++) {
+      double dql, dqr;
+      if (P.lim == 90) {
+        dql = csslope(k, s[0][k], s[1][k], s[2][k], s[3][k], s[4][k]);
+        dqr = csslope(k, s[1][k], s[2][k], s[3][k], s[4][k], s[5][k]);
+      } else {
+        dql = slope3(P.lim, s[1][k], s[2][k], s[3][k]);
+        dqr = slope3(P.lim, s[2][k], s[3][k], s[4][k]);
+      }
+      p_l[k] = s[2][k] + 0.5 * dql;
+      p_r[k] = s[3][k] - 0.5 * dqr;
+    }
+  }
+  GeoU g;
+  load_geo_face(gF, z0, g);
+  {
+    constexpr int kb = B1 + DIR - 1;
+    double bavg = 0.5 * (p_l[kb] + p_r[kb]);
+    p_l[kb] = bavg;
+    p_r[kb] = bavg;
+  }
+  if (limit_gamma(P.gammamax, p_l, g) >= 1) fail = 1;
+  if (limit_gamma(P.gammamax, p_r, g) >= 1) fail = 1;
+  State sl, sr;
+  get_state(p_l, g, sl);
+  get_state(p_r, g, sr);
+  double F_l[NPR], F_r[NPR], U_l[NPR], U_r[NPR];
+  primtoflux(P, p_l, sl, DIR, g.gdet, F_l);
+  primtoflux(P, p_r, sr, DIR, g.gdet, F_r);
+  primtoflux(P, p_l, sl, 0, g.gdet, U_l);
+  primtoflux(P, p_r, sr, 0, g.gdet, U_r);
+  double cmax_l, cmin_l, cmax_r, cmin_r;
+  if (vchar(P, p_l, sl, DIR, g, cmax_l, cmin_l)) fail = 1;
+  if (vchar(P, p_r, sr, DIR, g, cmax_r, cmin_r)) fail = 1;
+  double lmin = dmin(cmin_l, cmin_r), lmax = dmax(cmax_l, cmax_r);
+  double cmin = fabs(dmax(0., -lmin)), cmax = fabs(dmax(0., lmax));
+  double ctop = dmax(cmax, cmin);
+  double hllfrac = (P.fluxmethod == 1) ? 1.0 : 0.0;
+  double laxffrac = (P.fluxmethod == 0) ? 1.0 : 0.0;
+  double Fo[NPR];
+  if (cmax + cmin != 0.0) {
+#pragma unroll
+    for (int k = 0; k < NPR; k++)
+      Fo[k] = hllfrac * ((cmax * F_l[k] + cmin * F_r[k] - cmax * cmin * (U_r[k] - U_l[k])) / (cmax + cmin)) +
+              laxffrac * (0.5 * (F_l[k] + F_r[k] - ctop * (U_r[k] - U_l[k])));
+  } else {
+#pragma unroll
+    for (int k = 0; k < NPR; k++) Fo[k] = (0.5 * (F_l[k] + F_r[k] - ctop * (U_r[k] - U_l[k])));
+  }
+  double cm = dmax(cmax, cmin);
+  dtij = P.cour * ((DIR == 1) ? P.dx1 : P.dx2) / cm;
+  if (!(dtij < 1.e9)) dtij = 1.e9;
+  bool skip_b2 = false;
+  if (P.fixupflux) {
+    if (DIR == 1) {
+      if (i == 0 && P.bc[HARM_X1DN] == HARM_OUTFLOW && j >= -1 && j < P.n2 && Fo[RHO] > 0) Fo[RHO] = 0;
+      if (i == P.n1 && P.bc[HARM_X1UP] == HARM_OUTFLOW && j >= -1 && j < P.n2 && Fo[RHO] < 0) Fo[RHO] = 0;
+      if (i >= -1 && i < P.n1) {
+        if (j == 0 && P.bc[HARM_X2DN] == HARM_POLARAXIS) {
+          int zg = zidx(P, i, -1);
+          F[(size_t)zg * 8 + B2] = -Fo[B2];
+          q[zg] = (-Fo[B2]) / gF[(size_t)zg * GF + 15];
+        }
+        if (j == P.n2 - 1 && P.bc[HARM_X2UP] == HARM_POLARAXIS) {
+          int zg = zidx(P, i, P.n2);
+          F[(size_t)zg * 8 + B2] = -Fo[B2];
+          q[zg] = (-Fo[B2]) / gF[(size_t)zg * GF + 15];
+        }
+        skip_b2 = (j == -1 && P.bc[HARM_X2DN] == HARM_POLARAXIS) || (j == P.n2 && P.bc[HARM_X2UP] == HARM_POLARAXIS);
+      }
+    } else {
+      bool polar_dn = (j == 0 && P.bc[HARM_X2DN] == HARM_POLARAXIS && i >= -1 && i < P.n1);
+      bool polar_up = (j == P.n2 && P.bc[HARM_X2UP] == HARM_POLARAXIS && i >= -1 && i < P.n1);
+      bool equator = (i >= 0 && i < P.n1 && j == (P.n2 - 1) / 2);
+      if (polar_dn || polar_up || equator) {
+#pragma unroll
+        for (int k = 0; k < NPR; k++)
+          if (k != U2) Fo[k] = 0.;
+      }
+    }
+  }
+  if (skip_b2) {
+    double* Fz = F + (size_t)z0 * 8;
+#pragma unroll
+    for (int k = 0; k < NPR; k++)
+      if (k != B2) Fz[k] = Fo[k];
+  } else {
+    store8(F, z0, Fo);
+    q[z0] = (DIR == 1 ? Fo[B2] : Fo[B1]) / g.gdet;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_flux_single(KParams P, const double* __restrict__ pr,
+                                                     double* __restrict__ F1, double* __restrict__ F2,
+                                                     double* __restrict__ q1, double* __restrict__ q2,
+                                                     const double* __restrict__ gF, DevScalars* __restrict__ ds,
+                                                     int dir0) {
+  const int dir = dir0 + blockIdx.z;
+  const int jf = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ifc = blockIdx.y * blockDim.y + threadIdx.y;
+  double dtij = 1.e9;
+  int fail = 0;
+  const size_t nz = (size_t)(P.n1 + 2 * NG) * P.N2M;
+  if (dir == 1) {
+    const int i = ifc, j = jf - 1;
+    if (i <= P.n1 && j <= P.n2) flux_face_single<1>(P, pr, F1, q1, gF, i, j, dtij, fail);
+  } else {
+    const int i = ifc - 1, j = jf;
+    if (i <= P.n1 && j <= P.n2) flux_face_single<2>(P, pr, F2, q2, gF + nz * GF, i, j, dtij, fail);
+  }
+  double wm = wave_min(dtij);
+  if ((threadIdx.x & 63) == 0 && wm < 1.e9)
+    atomicMin(&ds->ndt_acc[dir - 1], (unsigned long long)__double_as_longlong(wm));
+  if (fail) atomicOr(&ds->failed, 1);
+}
+
 // corner EMF of flux_ct() (step_ch.c:1223-1234, CORNGDETVERSION=1) from the
 // per-face quotients written by k_flux
 __device__ __forceinline__ double emf_at(const KParams& P, const double* __restrict__ q1,
@@ -797,6 +940,18 @@ constexpr int KB_FIXUP = 2;     // post_fixup(): bound_pflag + fixup_utoprim on 
 constexpr int KB_BOUND2 = 8;    // bound_prim() after the fixup (step_ch.c:124)
 constexpr int KB_DT = 4;        // t,dt update (step_ch.c:174-183)
 
+// step_ch.c:174-183 with advance.c:190: the new dt from the last substage's
+// Courant limits, clipped to SAFE*dt and to tf
+__device__ inline void dt_update(const KParams& P, DevScalars* ds) {
+  double ndt = P.defcon * 1. / (1. / ds->ndt_last[0] + 1. / ds->ndt_last[1]);
+  double dt = ds->dt;
+  ds->t += dt;
+  if (ndt > P.safe * dt) ndt = P.safe * dt;
+  dt = ndt;
+  if (ds->t + dt > ds->tf) dt = ds->tf - ds->t;
+  ds->dt = dt;
+}
+
 // Everything between two advance() calls, as one workgroup with barriers
 // between the phases: bound_prim, post_fixup, bound_prim, and after the last
 // substage the time/dt update.  The failed-zone list is filled by k_update.
@@ -820,17 +975,71 @@ __global__ __launch_bounds__(1024) void k_boundary(KParams P, double* __restrict
     if (threadIdx.x == 0) ds->nlist = 0;
   }
   if (mode & KB_BOUND2) bound_block(P, pv, gC, fail);
-  if ((mode & KB_DT) && threadIdx.x == 0) {
-    // step_ch.c:174-183 with advance.c:190 (ndt of the last substage)
-    double ndt = P.defcon * 1. / (1. / ds->ndt_last[0] + 1. / ds->ndt_last[1]);
-    double dt = ds->dt;
-    ds->t += dt;
-    if (ndt > P.safe * dt) ndt = P.safe * dt;
-    dt = ndt;
-    if (ds->t + dt > ds->tf) dt = ds->tf - ds->t;
-    ds->dt = dt;
-  }
+  if ((mode & KB_DT) && threadIdx.x == 0) dt_update(P, ds);
   if (fail) atomicOr(&ds->failed, 1);
+}
+
+// ---- the same boundary work as separate wide launches (bit-identical) ----
+
+// bound_prim() x1 faces: thread (j, gi)
+__global__ void k_bx1(KParams P, double* __restrict__ pv, const double* __restrict__ gC,
+                      DevScalars* __restrict__ ds, int reset_list) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (reset_list && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) ds->nlist = 0;
+  if (j >= P.n2) return;
+  int fail = 0;
+  bound_x1_cell(P, pv, gC, blockIdx.y, j, fail);
+  if (fail) atomicOr(&ds->failed, 1);
+}
+
+// bound_prim() x2 faces over all i (rows gj 0..5) and POLEDEATH of the two
+// interior pole rows (gj 6,7).  A ghost copied from a pole row reads its U2,B2
+// either before or after that row's x*0 and multiplies them by 0 itself, which
+// gives the same signed zero either way.  Optionally the t,dt update.
+__global__ void k_bx2(KParams P, double* __restrict__ pv, DevScalars* __restrict__ ds, int do_dt) {
+  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) - NG;
+  const int gj = blockIdx.y;
+  if (do_dt && blockIdx.x == 0 && gj == 0 && threadIdx.x == 0) dt_update(P, ds);
+  if (i > P.n1 + NG - 1) return;
+  if (gj < 2 * NG) {
+    bound_x2_cell(P, pv, i, gj);
+  } else if (P.poledeath) {
+    const bool dn = (gj == 2 * NG);
+    if ((dn ? P.bc[HARM_X2DN] : P.bc[HARM_X2UP]) != HARM_POLARAXIS) return;
+    double* a = pv + (size_t)zidx(P, i, dn ? 0 : P.n2 - 1) * 8;
+    a[U2] *= 0.;
+    a[B2] *= 0.;
+  }
+}
+
+// fixup_utoprim() over the failed-zone list: compute, then apply
+__global__ void k_fixlist(KParams P, const double* __restrict__ pv, const int* __restrict__ pflag,
+                          const int* __restrict__ faillist, double* __restrict__ pfix, const double* __restrict__ gC,
+                          DevScalars* __restrict__ ds) {
+  int n = ds->nlist, fail = 0;
+  if (n > P.n1 * P.n2) n = P.n1 * P.n2;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+    fixup_zone(P, pv, pflag, pfix, gC, faillist[t], fail);
+  if (fail) atomicOr(&ds->failed, 1);
+}
+
+// the scalar tail of a fixup-only boundary pass: clear the list, update dt
+__global__ void k_tail(KParams P, DevScalars* __restrict__ ds, int do_dt) {
+  ds->nlist = 0;
+  if (do_dt) dt_update(P, ds);
+}
+
+__global__ void k_fixapply(KParams P, double* __restrict__ pv, int* __restrict__ pflag,
+                           const int* __restrict__ faillist, const double* __restrict__ pfix,
+                           const DevScalars* __restrict__ ds) {
+  int n = ds->nlist;
+  if (n > P.n1 * P.n2) n = P.n1 * P.n2;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+    const int z = faillist[t];
+#pragma unroll
+    for (int k = 0; k <= U3; k++) pv[(size_t)z * 8 + k] = pfix[(size_t)z * 8 + k];
+    pflag[z] = -1;  // UTOPRIMFAILFIXED (global.h:524)
+  }
 }
 
 __global__ void k_utoprim_batch(KParams P, int n, const int* __restrict__ zone, const double* __restrict__ U,
@@ -888,6 +1097,9 @@ struct harm_ctx {
   bool graph_ok;
   bool geometry_set;
   bool prof;
+  int flux_variant;   // HARM_FLUX_* chosen by the autotuner or the caller
+  int bound_variant;  // HARM_BOUND_*
+  bool tuned;
   std::vector<hipEvent_t>* evpool;  // pairs (start, end) recorded without waiting
   std::vector<int>* evkid;          // kernel id of each recorded pair
   int evused;
@@ -1153,12 +1365,22 @@ static int prof_end(harm_ctx* c) {
 
 // fluxcalc for dir0..dir0+ndir-1 (both directions in one launch when ndir=2)
 static int launch_flux(harm_ctx* c, int pr_sel, int dir0, int ndir) {
-  const dim3 b(64, 4);
   const int nfj = c->kp.n2 + 2, nfi = c->kp.n1 + 2;  // covers both directions' face ranges
-  dim3 g((nfj + 31) / 32, (nfi + 3) / 4, ndir);
   if (prof_begin(c, HARM_K_FLUX)) return 1;
-  hipLaunchKernelGGL(k_flux, g, b, 0, c->stream, c->kp, c->arr[pr_sel], c->arr[HARM_F1], c->arr[HARM_F2], c->q1,
-                     c->q2, c->gF, c->ds, dir0);
+  switch (c->flux_variant) {
+    case HARM_FLUX_PAIR: {
+      const dim3 b(64, 4);
+      hipLaunchKernelGGL(k_flux, dim3((nfj + 31) / 32, (nfi + 3) / 4, ndir), b, 0, c->stream, c->kp, c->arr[pr_sel],
+                         c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2, c->gF, c->ds, dir0);
+      break;
+    }
+    default: {
+      const dim3 b = (c->flux_variant == HARM_FLUX_FACE_64x2) ? dim3(64, 2) : dim3(64, 4);
+      hipLaunchKernelGGL(k_flux_single, dim3((nfj + 63) / 64, (nfi + b.y - 1) / b.y, ndir), b, 0, c->stream,
+                         c->kp, c->arr[pr_sel], c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2, c->gF, c->ds, dir0);
+      break;
+    }
+  }
   HIPCHK(hipGetLastError());
   return prof_end(c);
 }
@@ -1176,10 +1398,90 @@ static int launch_update(harm_ctx* c, int pi, int pb, int pf, const RKCoef& C) {
 static int launch_boundary(harm_ctx* c, int sel, int mode) {
   if (c->prm.eomtype != HARM_EOMGRMHD) mode &= ~KB_FIXUP;
   if (prof_begin(c, HARM_K_BOUNDARY)) return 1;
-  hipLaunchKernelGGL(k_boundary, dim3(1), dim3(1024), 0, c->stream, c->kp, c->arr[sel], c->gC, c->pflag,
-                     c->faillist, c->pfix, c->ds, mode);
+  if (c->bound_variant == HARM_BOUND_BLOCK) {
+    hipLaunchKernelGGL(k_boundary, dim3(1), dim3(1024), 0, c->stream, c->kp, c->arr[sel], c->gC, c->pflag,
+                       c->faillist, c->pfix, c->ds, mode);
+  } else {
+    const dim3 gx1((c->kp.n2 + 63) / 64, 2 * NG), gx2((c->kp.n1 + 2 * NG + 63) / 64, 2 * NG + 2);
+    const bool bound2 = (mode & KB_BOUND2) != 0, fix = (mode & KB_FIXUP) != 0;
+    if (mode & KB_BOUND) {
+      hipLaunchKernelGGL(k_bx1, gx1, dim3(64), 0, c->stream, c->kp, c->arr[sel], c->gC, c->ds, 0);
+      hipLaunchKernelGGL(k_bx2, gx2, dim3(64), 0, c->stream, c->kp, c->arr[sel], c->ds,
+                         (!fix && !bound2 && (mode & KB_DT)) ? 1 : 0);
+    }
+    if (fix) {
+      hipLaunchKernelGGL(k_fixlist, dim3(32), dim3(64), 0, c->stream, c->kp, c->arr[sel], c->pflag, c->faillist,
+                         c->pfix, c->gC, c->ds);
+      hipLaunchKernelGGL(k_fixapply, dim3(32), dim3(64), 0, c->stream, c->kp, c->arr[sel], c->pflag, c->faillist,
+                         c->pfix, c->ds);
+    }
+    if (bound2) {
+      hipLaunchKernelGGL(k_bx1, gx1, dim3(64), 0, c->stream, c->kp, c->arr[sel], c->gC, c->ds, fix ? 1 : 0);
+      hipLaunchKernelGGL(k_bx2, gx2, dim3(64), 0, c->stream, c->kp, c->arr[sel], c->ds, (mode & KB_DT) ? 1 : 0);
+    } else if (fix) {
+      hipLaunchKernelGGL(k_tail, dim3(1), dim3(1), 0, c->stream, c->kp, c->ds, (mode & KB_DT) ? 1 : 0);
+    }
+  }
   HIPCHK(hipGetLastError());
   return prof_end(c);
+}
+
+// Autotuner: the variants below give bit-identical results; time each on the
+// live state (a few launches after one warm-up) and keep the fastest.  Flux
+// variants only write F, q and the Courant accumulators (restored after);
+// boundary variants run on PK1, which every step overwrites before reading.
+static int time_launches(harm_ctx* c, int reps, int (*fn)(harm_ctx*), float* ms) {
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  if (fn(c)) return 1;
+  HIPCHK(hipEventRecord(a, c->stream));
+  for (int r = 0; r < reps; r++)
+    if (fn(c)) return 1;
+  HIPCHK(hipEventRecord(b, c->stream));
+  HIPCHK(hipEventSynchronize(b));
+  HIPCHK(hipEventElapsedTime(ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return 0;
+}
+static int tune_flux_fn(harm_ctx* c) { return launch_flux(c, HARM_P, 1, 2); }
+static int tune_bound_fn(harm_ctx* c) { return launch_boundary(c, HARM_PK1, KB_BOUND | KB_BOUND2); }
+
+static int autotune(harm_ctx* c) {
+  if (c->tuned) return 0;
+  bool prof = c->prof;
+  c->prof = false;
+  DevScalars save;
+  HIPCHK(hipMemcpyAsync(&save, c->ds, sizeof(save), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpyAsync(c->arr[HARM_PK1], c->arr[HARM_P], (size_t)c->nz * 8 * sizeof(double),
+                        hipMemcpyDeviceToDevice, c->stream));
+  float best = 1e30f;
+  int bestv = HARM_FLUX_FACE_64x4;
+  const int fv[3] = {HARM_FLUX_FACE_64x4, HARM_FLUX_FACE_64x2, HARM_FLUX_PAIR};
+  for (int v : fv) {
+    float ms;
+    c->flux_variant = v;
+    if (time_launches(c, 5, tune_flux_fn, &ms)) return 1;
+    if (ms < best) { best = ms; bestv = v; }
+  }
+  c->flux_variant = bestv;
+  best = 1e30f;
+  bestv = HARM_BOUND_KERNELS;
+  const int bv[2] = {HARM_BOUND_KERNELS, HARM_BOUND_BLOCK};
+  for (int v : bv) {
+    float ms;
+    c->bound_variant = v;
+    if (time_launches(c, 5, tune_bound_fn, &ms)) return 1;
+    if (ms < best) { best = ms; bestv = v; }
+  }
+  c->bound_variant = bestv;
+  HIPCHK(hipMemcpyAsync(c->ds, &save, sizeof(save), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->prof = prof;
+  c->tuned = true;
+  return 0;
 }
 
 static void rk_setup(int timeorder, int* nto, double CUf[4][3], double Cunew[4][3], int pii[4], int pbb[4],
@@ -1321,6 +1623,7 @@ extern "C" int harm_post_fixup(harm_ctx* c, int sel) {
 
 extern "C" int harm_step(harm_ctx* c) {
   if (need_geom(c)) return 1;
+  if (autotune(c)) return 1;
   if (enqueue_step(c)) return 1;
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
@@ -1344,6 +1647,7 @@ static int build_graph(harm_ctx* c) {
 extern "C" int harm_step_n(harm_ctx* c, int nsteps) {
   if (need_geom(c)) return 1;
   if (nsteps < 0) return seterr("nsteps < 0");
+  if (autotune(c)) return 1;
   if (c->prof) {  // timed per kernel: eager launches
     for (int s = 0; s < nsteps; s++)
       if (enqueue_step(c)) return 1;
@@ -1431,5 +1735,26 @@ extern "C" int harm_prof_reset(harm_ctx* c) {
     c->ktime[k] = 0;
     c->kcount[k] = 0;
   }
+  return 0;
+}
+
+extern "C" int harm_set_tuning(harm_ctx* c, int flux_variant, int bound_variant) {
+  if (!c) return seterr("null ctx");
+  if (flux_variant < 0 || flux_variant > HARM_FLUX_PAIR) return seterr("bad flux variant %d", flux_variant);
+  if (bound_variant < 0 || bound_variant > HARM_BOUND_BLOCK) return seterr("bad boundary variant %d", bound_variant);
+  c->flux_variant = flux_variant;
+  c->bound_variant = bound_variant;
+  c->tuned = true;
+  if (c->graph_ok) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph_ok = false;
+  }
+  return 0;
+}
+
+extern "C" int harm_get_tuning(harm_ctx* c, int* flux_variant, int* bound_variant) {
+  if (!c) return seterr("null ctx");
+  if (flux_variant) *flux_variant = c->flux_variant;
+  if (bound_variant) *bound_variant = c->bound_variant;
   return 0;
 }

@@ -175,6 +175,16 @@ int harm_get_nstroke(harm_ctx *ctx, long long *nstroke);
 #define HARM_K_BOUNDARY 4
 #define HARM_K_COUNT 6
 int harm_prof_enable(harm_ctx *ctx, int on);
+/* (no reference counterpart) Kernel variants, all bit-identical: the first
+   harm_step/harm_step_n times them on the live state and keeps the fastest;
+   harm_set_tuning forces a choice (and disables the timing). */
+#define HARM_FLUX_FACE_64x4 0  /* one thread per face, 64x4 workgroups */
+#define HARM_FLUX_FACE_64x2 1  /* one thread per face, 64x2 workgroups */
+#define HARM_FLUX_PAIR 2       /* a lane pair per face (left/right state) */
+#define HARM_BOUND_KERNELS 0   /* boundary work as six wide launches */
+#define HARM_BOUND_BLOCK 1     /* boundary work in one workgroup with barriers */
+int harm_set_tuning(harm_ctx *ctx, int flux_variant, int bound_variant);
+int harm_get_tuning(harm_ctx *ctx, int *flux_variant, int *bound_variant);
 int harm_prof_get(harm_ctx *ctx, int kernel, double *total_ms, long long *count);
 int harm_prof_reset(harm_ctx *ctx);
 

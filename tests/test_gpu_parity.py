@@ -183,3 +183,40 @@ def test_gpu_fullsize_ot_invariants():
     assert np.max(np.abs(divb(p) - d0)) < 1e-11 * np.abs(p[..., 5:7]).max() / g.dx1
     n_fail, failed = h.failures()
     assert failed == 0
+
+
+@pytest.mark.parametrize("fv", [H.FLUX_FACE_64x4, H.FLUX_FACE_64x2, H.FLUX_PAIR])
+@pytest.mark.parametrize("bv", [H.BOUND_KERNELS, H.BOUND_BLOCK])
+def test_gpu_variants_match_oracle(fv, bv):
+    say(f"[case] kernel variants flux={fv} bound={bv}: 32x64 torus, 30 steps (with repaired inversions)")
+    g, geom, p0, prm = problem("ks", 32, 64)
+    h = gpu_ctx(geom, prm, p0, 1e-3)
+    h.set_tuning(fv, bv)
+    assert h.get_tuning() == (fv, bv)
+    o = oracle_ctx(geom, prm, p0, 1e-3)
+    for _ in range(10):
+        h.step_ch()
+    h.step_n(20)
+    for _ in range(30):
+        o.step()
+    assert h.get_time() == o.get_time()
+    assert same(h.get_array(H.P), o.get_array(0))
+    assert h.nstroke() == o.nstroke()
+    nfail, failed = h.failures()
+    assert failed == o.failed() == 0
+    assert nfail > 0, "the run is meant to exercise the fixup of failed inversions"
+
+
+def test_gpu_autotune_picks_a_variant():
+    say("[case] autotuner on the 64x64 torus")
+    g, geom, p0, prm = problem("ks", 64, 64)
+    h = gpu_ctx(geom, prm, p0, 1e-3)
+    o = oracle_ctx(geom, prm, p0, 1e-3)
+    h.step_n(3)
+    fv, bv = h.get_tuning()
+    say(f"  chosen flux={fv} bound={bv}")
+    assert fv in (H.FLUX_FACE_64x4, H.FLUX_FACE_64x2, H.FLUX_PAIR) and bv in (H.BOUND_KERNELS, H.BOUND_BLOCK)
+    for _ in range(3):
+        o.step()
+    assert same(h.get_array(H.P), o.get_array(0))
+    assert h.get_time() == o.get_time()
