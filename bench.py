@@ -28,32 +28,42 @@ sys.path.insert(0, ROOT)
 METRIC = "FP64 zone-updates/s at 256² torus; batched & adjoint-step cost vs plain step"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes per unit for each kernel (read + write of the data the
-# kernel must touch at least once; the unit is the kernel's work item).
-#   k_flux     per face (both directions in one launch): p of one zone (64)
-#              + packed face geometry (16 doubles, 128) + F (64)
-#              + EMF quotient q (8)                                  = 264 B
+# Algorithmic bytes per unit for each hot kernel (read + write of the data
+# the kernel must touch at least once; the unit is the kernel's work item).
+#   k_flux_single, k_flux  per face (both directions in one launch):
+#              p of one zone (64) + packed face geometry (16 doubles, 128)
+#              + F (64) + EMF quotient q (8)                         = 264 B
 #   k_update   per zone: pi, pb (128) + CENT geometry (36 doubles, 288)
 #              + connection (64 doubles, 512) + F1, F2 (128) + q1, q2 (16)
 #              + ulast, unew read (128) + pf, ulast, unew written (192)
 #              + pflag (4)                                          = 1396 B
-BYTES_PER_UNIT = {"k_flux": 264, "k_update": 1396}
+BYTES_PER_UNIT = {"flux": 264, "update": 1396}
 
 
-def units_per_launch(name, n1, n2):
-    return {"k_flux": (n1 + 1) * (n2 + 2) + (n1 + 2) * (n2 + 1), "k_update": n1 * n2}.get(name)
+def units_per_launch(stage, n1, n2):
+    return {"flux": (n1 + 1) * (n2 + 2) + (n1 + 2) * (n2 + 1), "update": n1 * n2}[stage]
 
 
-def traffic_from_profiles(kernel):
-    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE
-    (profiles/pmc_traffic.json), or None if that profile has not been taken."""
+def kernel_of(stage, flux_variant):
+    if stage == "update":
+        return "k_update"
+    import harm_ramt_amd as H
+    return "k_flux" if flux_variant == H.FLUX_PAIR else "k_flux_single"
+
+
+def traffic_from_profiles(kernel, variants):
+    """HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
+    (profiles/pmc_traffic.json, counters scaled by a copy kernel of known
+    size), or None if no profile of this kernel variant has been taken."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if not os.path.exists(path):
         return None
     try:
         d = json.load(open(path))
+        if d.get("variants") != variants:
+            return None
         return d.get("per_launch_bytes", {}).get(kernel)
-    except Exception:
+    except (OSError, ValueError):
         return None
 
 
@@ -89,6 +99,19 @@ def cpu_baseline(geom, p0, prm, n1, n2, seconds):
                       f"{el:.1f} s on {threads} OpenMP threads"}
 
 
+def copy_calibration(mib):
+    """A plain copy of known size: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
+    this run gives the counter reading for mib MiB read and mib MiB written."""
+    import torch
+    n = mib * (1 << 20) // 8
+    x = torch.arange(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(10):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "launches": 10}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,7 +121,15 @@ def main():
     ap.add_argument("--n2", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tuning", default=None,
+                    help="force the kernel variants as FLUX,BOUND (harm_set_tuning); default: autotuned")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB ten times (the reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE/WRITE_SIZE counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build measures one GPU only (the path is a single-GPU grid; "
               "multi-GPU halo exchange is design-only). Run with --gpus 1.", file=sys.stderr)
@@ -111,6 +142,8 @@ def main():
     h.set_array(H.P, p0)
     h.bound_prim(H.P)
     h.set_time(0.0, 1e-5, 2000.0)   # initbase.c:164 dt=1e-5, tf well beyond the run
+    if args.tuning:
+        h.set_tuning(*[int(v) for v in args.tuning.split(",")])
 
     h.step_n(args.warmup)           # the first call also times the kernel variants and keeps the fastest
     h.get_time()                    # synchronises
@@ -125,7 +158,7 @@ def main():
         print(f"bench.py: the run set the failure flag (t={t_now})", file=sys.stderr)
         sys.exit(1)
 
-    # per-kernel HIP-event timing over an eagerly launched copy of the timed region
+    # per-stage HIP-event timing over an eagerly launched copy of the timed region
     h.prof_enable(True)
     h.prof_reset()
     prof_steps = min(args.steps, 100)
@@ -133,18 +166,20 @@ def main():
     h.step_n(prof_steps)
     h.get_time()
     el_prof = time.perf_counter() - t1
-    kern = {}
-    for kid, name in H.KERNEL_NAMES.items():
+    stages = {}
+    for kid, name in H.STAGE_NAMES.items():
         ms, cnt = h.prof_get(kid)
         if cnt:
-            kern[name] = {"total_ms": ms, "launches": cnt, "avg_us": 1e3 * ms / cnt}
+            stages[name] = {"total_ms": ms, "launches": cnt, "avg_us": 1e3 * ms / cnt}
     h.prof_enable(False)
-    dom = max((k for k in kern if k in BYTES_PER_UNIT), key=lambda k: kern[k]["total_ms"])
+    dom = max((k for k in stages if k in BYTES_PER_UNIT), key=lambda k: stages[k]["total_ms"])
+    kname = kernel_of(dom, fv)
     bytes_launch = BYTES_PER_UNIT[dom] * units_per_launch(dom, n1, n2)
-    achieved = bytes_launch / (kern[dom]["avg_us"] * 1e-6) / 1e9
+    achieved = bytes_launch / (stages[dom]["avg_us"] * 1e-6) / 1e9
+    variants = {"flux": fv, "boundary": bv}
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom),
-                "kernel": dom, "avg_launch_us": round(kern[dom]["avg_us"], 3),
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(kname, variants),
+                "kernel": kname, "avg_launch_us": round(stages[dom]["avg_us"], 3),
                 "algorithmic_bytes_per_launch": bytes_launch}
 
     cpu = None if args.no_cpu else cpu_baseline(geom, p0, prm, n1, n2, args.cpu_seconds)
@@ -162,7 +197,7 @@ def main():
                                                     H.BOUND_BLOCK: "one_workgroup"}[bv]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "kernels": kern,
+        "stages": stages,
         "eager_timed_ms_per_step": 1e3 * el_prof / prof_steps,
     }
     print(json.dumps(out), flush=True)

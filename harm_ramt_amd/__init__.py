@@ -32,7 +32,7 @@ METRIC_MINK, METRIC_KS = 0, 2
 FLUX_FACE_64x4, FLUX_FACE_64x2, FLUX_PAIR = 0, 1, 2
 BOUND_KERNELS, BOUND_BLOCK = 0, 1
 K_FLUX, K_EMF, K_UPDATE, K_BOUNDARY = 0, 2, 3, 4
-KERNEL_NAMES = {K_FLUX: "k_flux", K_EMF: "k_emf", K_UPDATE: "k_update", K_BOUNDARY: "k_boundary"}
+STAGE_NAMES = {K_FLUX: "flux", K_EMF: "flux_ct", K_UPDATE: "update", K_BOUNDARY: "boundary"}
 
 
 class HarmParams(ctypes.Structure):

@@ -1,19 +1,21 @@
 // harm_kernels.hip -- HIP kernels for gfx950 and the C-ABI (include/harm_amd.h)
 // of HARM's conservative timestep.
 //
-// One RK substage is five kernels on one stream:
-//   k_flux<1>, k_flux<2>  fluxcalc() per face: reconstruction, limit_gamma,
-//                         states, fluxes, vchar, HLL/LAXF, fix_flux edge rules,
-//                         wavefront-reduced Courant minimum      (step_ch.c:741)
-//   k_emf                 flux_ct() corner EMFs (Toth)            (step_ch.c:1159)
-//   k_update              advance() per zone: U(pi), source(pb), flux
-//                         divergence with the CT fluxes formed from the EMFs,
-//                         RK combination, Newton con-to-prim, fixup1zone
-//                                                                  (advance.c:46)
-//   k_bound_x1, k_bound_x2  bound_prim() ghost fill                 (bounds.fishmon.c)
-// and one single-thread kernel after the last substage updates t and dt on
-// the device (step_ch.c:174-183), so a step never synchronises with the host
-// and a whole step is captured once into a HIP graph and replayed.
+// One RK substage is three stages on one stream:
+//   flux       fluxcalc() for both directions in one launch: reconstruction,
+//              limit_gamma, states, fluxes, vchar, HLL/LAXF, fix_flux edge
+//              rules, wavefront-reduced Courant minimum      (step_ch.c:741)
+//              k_flux_single (thread per face) or k_flux (lane pair per face)
+//   k_update   advance() per zone: U(pi), source(pb), flux divergence with
+//              the flux-CT fluxes formed from the corner EMFs (step_ch.c:1159)
+//              inline, RK combination, Newton con-to-prim, fixup1zone, and
+//              the list of zones whose inversion failed      (advance.c:46)
+//   boundary   bound_prim(), post_fixup() on the failed list, bound_prim(),
+//              and after the last substage the t,dt update (step_ch.c:85-183):
+//              k_boundary (one workgroup) or k_bx1/k_bx2/k_fixlist/k_fixapply
+// All variants of a stage give bit-identical results; the first step times
+// them and keeps the fastest.  A step never synchronises with the host, and
+// a whole step is captured once into a HIP graph and replayed.
 //
 // Device layout (HBM):
 //   primitive-shaped arrays: the reference's AoS p[N1M][N2M][8] (64 B/zone);
@@ -1426,64 +1428,6 @@ static int launch_boundary(harm_ctx* c, int sel, int mode) {
   return prof_end(c);
 }
 
-// Autotuner: the variants below give bit-identical results; time each on the
-// live state (a few launches after one warm-up) and keep the fastest.  Flux
-// variants only write F, q and the Courant accumulators (restored after);
-// boundary variants run on PK1, which every step overwrites before reading.
-static int time_launches(harm_ctx* c, int reps, int (*fn)(harm_ctx*), float* ms) {
-  hipEvent_t a, b;
-  HIPCHK(hipEventCreate(&a));
-  HIPCHK(hipEventCreate(&b));
-  if (fn(c)) return 1;
-  HIPCHK(hipEventRecord(a, c->stream));
-  for (int r = 0; r < reps; r++)
-    if (fn(c)) return 1;
-  HIPCHK(hipEventRecord(b, c->stream));
-  HIPCHK(hipEventSynchronize(b));
-  HIPCHK(hipEventElapsedTime(ms, a, b));
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
-  return 0;
-}
-static int tune_flux_fn(harm_ctx* c) { return launch_flux(c, HARM_P, 1, 2); }
-static int tune_bound_fn(harm_ctx* c) { return launch_boundary(c, HARM_PK1, KB_BOUND | KB_BOUND2); }
-
-static int autotune(harm_ctx* c) {
-  if (c->tuned) return 0;
-  bool prof = c->prof;
-  c->prof = false;
-  DevScalars save;
-  HIPCHK(hipMemcpyAsync(&save, c->ds, sizeof(save), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  HIPCHK(hipMemcpyAsync(c->arr[HARM_PK1], c->arr[HARM_P], (size_t)c->nz * 8 * sizeof(double),
-                        hipMemcpyDeviceToDevice, c->stream));
-  float best = 1e30f;
-  int bestv = HARM_FLUX_FACE_64x4;
-  const int fv[3] = {HARM_FLUX_FACE_64x4, HARM_FLUX_FACE_64x2, HARM_FLUX_PAIR};
-  for (int v : fv) {
-    float ms;
-    c->flux_variant = v;
-    if (time_launches(c, 5, tune_flux_fn, &ms)) return 1;
-    if (ms < best) { best = ms; bestv = v; }
-  }
-  c->flux_variant = bestv;
-  best = 1e30f;
-  bestv = HARM_BOUND_KERNELS;
-  const int bv[2] = {HARM_BOUND_KERNELS, HARM_BOUND_BLOCK};
-  for (int v : bv) {
-    float ms;
-    c->bound_variant = v;
-    if (time_launches(c, 5, tune_bound_fn, &ms)) return 1;
-    if (ms < best) { best = ms; bestv = v; }
-  }
-  c->bound_variant = bestv;
-  HIPCHK(hipMemcpyAsync(c->ds, &save, sizeof(save), hipMemcpyHostToDevice, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  c->prof = prof;
-  c->tuned = true;
-  return 0;
-}
-
 static void rk_setup(int timeorder, int* nto, double CUf[4][3], double Cunew[4][3], int pii[4], int pbb[4],
                      int pff[4]) {
   // step_ch.c:365-460 setup_rktimestep
@@ -1621,6 +1565,8 @@ extern "C" int harm_post_fixup(harm_ctx* c, int sel) {
   return 0;
 }
 
+static int autotune(harm_ctx* c);
+
 extern "C" int harm_step(harm_ctx* c) {
   if (need_geom(c)) return 1;
   if (autotune(c)) return 1;
@@ -1629,18 +1575,89 @@ extern "C" int harm_step(harm_ctx* c) {
   return 0;
 }
 
-static int build_graph(harm_ctx* c) {
-  if (c->graph_ok) return 0;
+static int capture_step(harm_ctx* c, hipGraphExec_t* ex) {
   hipGraph_t g;
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   int rc = enqueue_step(c);
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (rc) return 1;
   if (e != hipSuccess) return seterr("graph capture: %s", hipGetErrorString(e));
-  e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(ex, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return seterr("graph instantiate: %s", hipGetErrorString(e));
+  return 0;
+}
+
+static int build_graph(harm_ctx* c) {
+  if (c->graph_ok) return 0;
+  if (capture_step(c, &c->graph)) return 1;
   c->graph_ok = true;
+  return 0;
+}
+
+// Autotuner.  The kernel variants give bit-identical results; each
+// combination is captured as a one-step graph and timed over a few replays
+// from the live state, which is restored afterwards (P and the device
+// scalars: every other array is overwritten within a step before it is read).
+static int time_variant(harm_ctx* c, const double* keep, const DevScalars& save, float* ms) {
+  const size_t pbytes = (size_t)c->nz * 8 * sizeof(double);
+  hipGraphExec_t ex;
+  if (capture_step(c, &ex)) return 1;
+  hipEvent_t a, b;
+  HIPCHK(hipEventCreate(&a));
+  HIPCHK(hipEventCreate(&b));
+  hipError_t e = hipSuccess;
+  for (int pass = 0; pass < 2 && e == hipSuccess; pass++) {
+    e = hipMemcpyAsync(c->arr[HARM_P], keep, pbytes, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->ds, &save, sizeof(save), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && pass == 1) e = hipEventRecord(a, c->stream);
+    for (int r = 0; r < (pass ? 4 : 1) && e == hipSuccess; r++) e = hipGraphLaunch(ex, c->stream);
+    if (e == hipSuccess && pass == 1) e = hipEventRecord(b, c->stream);
+  }
+  if (e == hipSuccess) e = hipEventSynchronize(b);
+  if (e == hipSuccess) e = hipEventElapsedTime(ms, a, b);
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  (void)hipGraphExecDestroy(ex);
+  if (e != hipSuccess) return seterr("autotune: %s", hipGetErrorString(e));
+  return 0;
+}
+
+static int autotune(harm_ctx* c) {
+  if (c->tuned) return 0;
+  const size_t pbytes = (size_t)c->nz * 8 * sizeof(double);
+  DevScalars save;
+  HIPCHK(hipMemcpyAsync(&save, c->ds, sizeof(save), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double* keep = nullptr;
+  HIPCHK(hipMalloc(&keep, pbytes));
+  int rc = 0;
+  if (hipMemcpyAsync(keep, c->arr[HARM_P], pbytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+    rc = seterr("autotune: copy");
+  const bool prof = c->prof;
+  c->prof = false;
+  float best = 1e30f;
+  int bf = HARM_FLUX_FACE_64x4, bb = HARM_BOUND_KERNELS;
+  const int fv[3] = {HARM_FLUX_FACE_64x4, HARM_FLUX_FACE_64x2, HARM_FLUX_PAIR};
+  const int bv[2] = {HARM_BOUND_KERNELS, HARM_BOUND_BLOCK};
+  for (int f = 0; f < 3 && !rc; f++)
+    for (int v = 0; v < 2 && !rc; v++) {
+      float ms = 0.f;
+      c->flux_variant = fv[f];
+      c->bound_variant = bv[v];
+      rc = time_variant(c, keep, save, &ms);
+      if (!rc && ms < best) { best = ms; bf = fv[f]; bb = bv[v]; }
+    }
+  c->prof = prof;
+  c->flux_variant = bf;
+  c->bound_variant = bb;
+  hipError_t e = hipMemcpyAsync(c->arr[HARM_P], keep, pbytes, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(c->ds, &save, sizeof(save), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(keep);
+  if (rc) return 1;
+  if (e != hipSuccess) return seterr("autotune: %s", hipGetErrorString(e));
+  c->tuned = true;
   return 0;
 }
 

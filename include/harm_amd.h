@@ -164,11 +164,14 @@ int harm_get_failures(harm_ctx *ctx, int *nfailzones, int *failed);
 /* Newton-iteration counter (the reference's nstroke, main.c:72-78). */
 int harm_get_nstroke(harm_ctx *ctx, long long *nstroke);
 
-/* Per-kernel timing with HIP events on the launch stream, ids:
-   HARM_K_FLUX     k_flux     fluxcalc, both directions in one launch
-   HARM_K_EMF      k_emf      corner EMFs (standalone harm_flux_ct only)
-   HARM_K_UPDATE   k_update   advance() per zone incl. CT fluxes and inversion
-   HARM_K_BOUNDARY k_boundary bound_prim + post_fixup + bound_prim (+ dt) */
+/* (no reference counterpart) Per-stage timing with HIP events recorded on
+   the launch stream around each stage's launches, stage ids:
+   HARM_K_FLUX     fluxcalc, both directions in one launch
+                   (k_flux_single or k_flux, see harm_set_tuning)
+   HARM_K_EMF      corner EMFs + CT fluxes (standalone harm_flux_ct only)
+   HARM_K_UPDATE   k_update: advance() per zone incl. CT fluxes and inversion
+   HARM_K_BOUNDARY bound_prim + post_fixup + bound_prim (+ dt)
+                   (k_boundary, or k_bx1/k_bx2/k_fixlist/k_fixapply) */
 #define HARM_K_FLUX 0
 #define HARM_K_EMF 2
 #define HARM_K_UPDATE 3
