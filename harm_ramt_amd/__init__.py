@@ -68,7 +68,7 @@ EXPORTS = [
     "harm_get_nstroke", "harm_prof_enable", "harm_prof_get", "harm_prof_reset",
     "harm_set_tuning", "harm_get_tuning",
     "harm_grid_mks", "harm_grid_cartesian", "harm_geometry", "harm_init_fishmon",
-    "harm_init_orszag_tang",
+    "harm_init_orszag_tang", "harm_init_monopole",
 ]
 
 _lib = None
@@ -120,6 +120,7 @@ def load_library(path=LIB_PATH):
                               ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                               ctypes.c_ulong, dp],
         "harm_init_orszag_tang": [ctypes.POINTER(HarmGrid), ctypes.c_double, dp],
+        "harm_init_monopole": [ctypes.POINTER(HarmGrid), dp, ctypes.c_double, dp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -198,6 +199,14 @@ def init_orszag_tang(geom: Geometry, gam=5.0 / 3.0):
     g = geom.grid
     p = np.zeros((g.n1 + 2 * NG, g.n2 + 2 * NG, NPR))
     _chk(load_library().harm_init_orszag_tang(ctypes.byref(g), gam, _dptr(p)), "harm_init_orszag_tang")
+    return p
+
+
+def init_monopole(geom: Geometry, B0=1.0):
+    """Force-free split monopole (init.bz.c:146-155) on a Kerr grid."""
+    g = geom.grid
+    p = np.zeros((g.n1 + 2 * NG, g.n2 + 2 * NG, NPR))
+    _chk(load_library().harm_init_monopole(ctypes.byref(g), _dptr(geom.gdet), B0, _dptr(p)), "harm_init_monopole")
     return p
 
 

@@ -462,9 +462,69 @@ struct RKCoef {
   int timeorder, last;
 };
 
-// GRMHD inversion of one zone: Utoprimgen (step_ch.c:468-552) ->
-// Utoprim_2d_final (utoprim_2d_final.c:26-101) -> Utoprim_new_body (:136-396).
-// U is the evolved conserved state, pr the guess in / solution out.
+// lower() (phys.c:633-652, upper triangle of gcov) on a full CENT record
+__device__ __forceinline__ void lower_gc(const double* gc, const double* a, double* b) {
+  b[0] = gc[0] * a[0] + gc[1] * a[1] + gc[2] * a[2] + gc[3] * a[3];
+  b[1] = gc[1] * a[0] + gc[5] * a[1] + gc[6] * a[2] + gc[7] * a[3];
+  b[2] = gc[2] * a[0] + gc[6] * a[1] + gc[10] * a[2] + gc[11] * a[3];
+  b[3] = gc[3] * a[0] + gc[7] * a[1] + gc[11] * a[2] + gc[15] * a[3];
+}
+
+// Force-free inversion of one zone, phys.ffde.c:141-372 Utoprim_ffde
+// (CHECKBSQMINUSESQ=1, AVOIDUTINF=2, AVOIDCS=0, REMOVET=0; ACCOUNTDISS feeds
+// diagnostics only).  Ug is geometry-free: T^t_mu in Ug[UU..U3], B^i in
+// Ug[B1..B3].  The drift velocity's u^i goes into pr[U1..U3] (vcon2pr,
+// transforms.c:381-386); pr[RHO], pr[UU] are not touched.  Returns 1 on the
+// space-like failure (phys.ffde.c:309-312), which is fatal in the reference.
+__device__ int utoprim_ffde(const KParams& P, const double* Ug, const double* gc, double* pr) {
+  const double* gcon = gc + 16;
+  const double gdet = gc[32];
+  const double alphasq = 1. / (-gcon[0]);
+  double Bcon[4] = {0.0, Ug[B1], Ug[B2], Ug[B3]}, Bcov[4];
+  lower_gc(gc, Bcon, Bcov);
+  double Bsq = 0.0;
+#pragma unroll
+  for (int j = 1; j < 4; j++) Bsq += Bcov[j] * Bcon[j];
+  const double T1 = Ug[UU + 1], T2 = Ug[UU + 2], T3 = Ug[UU + 3];
+  const double o = 1.0 / (Bsq * gdet);
+  double Econ[4], Ecov[4];
+  Econ[0] = 0;
+  Econ[1] = (Bcov[2] * T3 - Bcov[3] * T2) * (o);
+  Econ[2] = (Bcov[3] * T1 - Bcov[1] * T3) * (o);
+  Econ[3] = (Bcov[1] * T2 - Bcov[2] * T1) * (o);
+  double realBsq = Bsq;
+  lower_gc(gc, Econ, Ecov);
+  double Esq = 0.0;
+#pragma unroll
+  for (int j = 1; j < 4; j++) Esq += Econ[j] * Ecov[j];
+  const double BsqmEsq = Bsq - Esq;
+  const double gammasq = Bsq / BsqmEsq;
+  if ((gammasq > P.gammamax * P.gammamax) || (gammasq < 1.0)) {
+    const double Gsq = 1.0 - 1.0 / (P.gammamax * P.gammamax);
+    realBsq = (sqrt(Esq * Bsq / Gsq));
+  }
+  const double ob = 1.0 / (realBsq * gdet);
+  double v1 = alphasq * (-gcon[1] + (Ecov[2] * Bcov[3] - Ecov[3] * Bcov[2]) * ob);
+  double v2 = alphasq * (-gcon[2] + (Ecov[3] * Bcov[1] - Ecov[1] * Bcov[3]) * ob);
+  double v3 = alphasq * (-gcon[3] + (Ecov[1] * Bcov[2] - Ecov[2] * Bcov[1]) * ob);
+  // ucon_calc_3vel (phys.c:885-955)
+  double discr = +gc[0] + gc[5] * v1 * v1 + gc[10] * v2 * v2 + gc[15] * v3 * v3 +
+                 2. * (gc[1] * v1 + gc[2] * v2 + gc[3] * v3 + gc[6] * v1 * v2 + gc[7] * v1 * v3 + gc[11] * v2 * v3);
+  if (discr > 0.) return 1;
+  const double ut = 1. / sqrt(-discr);
+  pr[U1] = v1 * ut;
+  pr[U2] = v2 * ut;
+  pr[U3] = v3 * ut;
+  pr[B1] = Ug[B1];
+  pr[B2] = Ug[B2];
+  pr[B3] = Ug[B3];
+  return 0;
+}
+
+// Inversion of one zone: Utoprimgen (step_ch.c:468-552) -> for GRMHD
+// Utoprim_2d_final (utoprim_2d_final.c:26-101) -> Utoprim_new_body (:136-396),
+// for FFDE Utoprim_ffde.  U is the evolved conserved state, pr the guess in /
+// solution out.  Returns the pflag value, or HARM_FFDE_SPACELIKE.
 __device__ int utoprimgen(const KParams& P, const double* U, const double* gc, double* pr, int& niter) {
   const double* gcov = gc;
   const double* gcon = gc + 16;
@@ -472,6 +532,8 @@ __device__ int utoprimgen(const KParams& P, const double* U, const double* gc, d
   double Ug[NPR];
 #pragma unroll
   for (int k = 0; k < NPR; k++) Ug[k] = U[k] / gdet;
+  niter = 0;
+  if (P.eomtype == HARM_EOMFFDE) return utoprim_ffde(P, Ug, gc, pr) ? HARM_FFDE_SPACELIKE : 0;
   Ug[UU] += -Ug[RHO];
   // Utoprim_2d_final
   double prim[NPR];
@@ -731,6 +793,10 @@ __global__ __launch_bounds__(256) void k_update(KParams P, RKCoef C, const doubl
 #pragma unroll
     for (int k = 0; k < NPR; k++) pout[k] = pin[k];
     int pfl = utoprimgen(P, C.last ? un : Uf, gc, pout, niter);
+    if (pfl == HARM_FFDE_SPACELIKE) {  // fatal in the reference (advance() returns 1)
+      fail = 1;
+      pfl = 0;
+    }
     pflag[z] = pfl;
     failz = (pfl != 0);
     if (failz) {  // zones for fixup_utoprim (k_boundary)
@@ -1115,8 +1181,7 @@ static int check_prm(const harm_params* p) {
   if (!p) return seterr("null params");
   if (p->n1 < 4 || p->n2 < 4) return seterr("n1,n2 must be >= 4 (got %d,%d)", p->n1, p->n2);
   if ((long long)(p->n1 + 6) * (p->n2 + 6) > (1LL << 28)) return seterr("grid too large");
-  if (p->eomtype != HARM_EOMGRMHD)
-    return seterr("eomtype %d: only EOMGRMHD is implemented on the GPU path", p->eomtype);
+  if (p->eomtype != HARM_EOMGRMHD && p->eomtype != HARM_EOMFFDE) return seterr("bad eomtype %d", p->eomtype);
   int l = p->lim;
   if (!(l == HARM_DONOR || l == HARM_VANL || l == HARM_MINM || l == HARM_MC || l == HARM_PARA ||
         l == HARM_CSSLOPE || l == HARM_NLIMCENT || l == HARM_NLIMUP || l == HARM_NLIMDOWN))

@@ -598,3 +598,28 @@ int harm_init_orszag_tang(const harm_grid *g, double gam, double *p) {
     }
   return 0;
 }
+
+/* Split monopole for the force-free equations (init.bz.c:146-155 and
+   init.c:187-196, PROBLEMTYPE 0): rho = u = 0, zero velocity,
+   B^r = +-B0 gdet(ihor, j) / gdet(i, j) with the sign of the hemisphere
+   (theta < pi/2 positive).  The reference leaves `horizoni` undefined; it is
+   the horizon zone index setihor() gives (coord.c:817-822), where setRin()
+   puts the horizon on the inner face. */
+int harm_init_monopole(const harm_grid *g, const double *gdet, double B0, double *p) {
+  if (!g || g->metric != HARM_METRIC_KS || !gdet || !p) return 1;
+  int ihor = (g->n1 * (0.1) > 10) ? 10 : (int)((double)g->n1 * (double)(0.1));
+  for (int i = -NG; i < g->n1 + NG; i++)
+    for (int j = -NG; j < g->n2 + NG; j++) {
+      double X[NDIM], r, th;
+      coord(g, i, j, CENT, X);
+      bl_coord(g, X, &r, &th);
+      double *pr = p + zidx(g, i, j) * NPR;
+      double gh = gdet[zidx(g, ihor, j) * 4 + CENT], gz = gdet[zidx(g, i, j) * 4 + CENT];
+      pr[RHO] = pr[UU] = 0.0;
+      pr[U1] = pr[U2] = pr[U3] = 0.0;
+      pr[B2] = pr[B3] = 0;
+      if (th < M_PI * 0.5) pr[B1] = B0 * gh / (gz);
+      else pr[B1] = -B0 * gh / (gz);
+    }
+  return 0;
+}

@@ -71,6 +71,11 @@ extern "C" {
 #define HARM_UNEW 6
 #define HARM_NARRAYS 7
 
+/* pflag value returned by harm_utoprimgen for an FFDE zone whose drift
+   velocity is space-like (phys.ffde.c:309-312); inside a step the reference
+   treats it as fatal (advance() returns 1), here it sets the failed flag */
+#define HARM_FFDE_SPACELIKE 50
+
 #define HARM_NPR 8
 #define HARM_NG 3
 #define HARM_NUMPFLAGS 5
@@ -224,6 +229,9 @@ int harm_init_fishmon(const harm_grid *g, const double *gcov, const double *gcon
                       unsigned long seed, double *p);
 /* Relativistic Orszag-Tang vortex on the unit square (Minkowski). */
 int harm_init_orszag_tang(const harm_grid *g, double gam, double *p);
+/* Force-free split monopole (init.bz.c:146-155, PROBLEMTYPE 0) on a Kerr
+   grid: rho=u=0, v=0, B^r = +-B0 gdet(ihor,j)/gdet(i,j) by hemisphere. */
+int harm_init_monopole(const harm_grid *g, const double *gdet, double B0, double *p);
 
 #ifdef __cplusplus
 }

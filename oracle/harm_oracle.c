@@ -685,12 +685,79 @@ static int Utoprim_2d_final(const oracle_ctx *c, const double U[NPR], const geom
   return 0;
 }
 
-/* step_ch.c:468-552 Utoprimgen: UtoU(UEVOLVE->UNOTHING) then the GRMHD inversion */
+/* phys.ffde.c:141-372 Utoprim_ffde (CHECKBSQMINUSESQ=1, AVOIDUTINF=2,
+   AVOIDCS=0, FIXB3=0, REMOVET=0).  U is geometry-free: T^t_mu in U[UU..U3],
+   B^i in U[B1..B3].  pr[RHO], pr[UU] are not touched.  ACCOUNTDISS only feeds
+   diagnostics (diss_account, phys.ffde.c:572-607) and is left out.
+   Returns 1 on the space-like failure of vcon2pr (phys.ffde.c:309-312). */
+static int Utoprim_ffde(const oracle_ctx *c, const double U[NPR], const geom_t *g, double *pr) {
+  double alphasq, BconOalpha[NDIM], BcovOalpha[NDIM], BsqOalphasq, T0[NDIM];
+  double EconOalpha[NDIM], EcovOalpha[NDIM], EsqOalphasq, BsqmEsqOalphasq, realBsqOalphasq;
+  double gammasq, Gsq, oBsqOalphasqgeom, oBsqgeom, vcon[NDIM], discr, ucon[NDIM];
+  int j;
+  alphasq = 1. / (-g->gcon[0][0]);
+  /* U2BBsq (phys.ffde.c:1261-1283) */
+  BconOalpha[0] = 0.0;
+  for (j = 1; j < NDIM; j++) BconOalpha[j] = U[B1 + j - 1];
+  lower(BconOalpha, g, BcovOalpha);
+  BsqOalphasq = 0.0;
+  for (j = 1; j < NDIM; j++) BsqOalphasq += BcovOalpha[j] * BconOalpha[j];
+  for (j = 0; j < NDIM; j++) T0[j] = U[UU + j];
+  /* TtoEconOalpha (phys.ffde.c:1287-1317) */
+  oBsqOalphasqgeom = 1.0 / (BsqOalphasq * g->g);
+  EconOalpha[0] = 0;
+  EconOalpha[1] = (BcovOalpha[2] * T0[3] - BcovOalpha[3] * T0[2]) * (oBsqOalphasqgeom);
+  EconOalpha[2] = (BcovOalpha[3] * T0[1] - BcovOalpha[1] * T0[3]) * (oBsqOalphasqgeom);
+  EconOalpha[3] = (BcovOalpha[1] * T0[2] - BcovOalpha[2] * T0[1]) * (oBsqOalphasqgeom);
+  /* B^2-E^2 check with the Lorentz-factor cap (phys.ffde.c:198-231) */
+  realBsqOalphasq = BsqOalphasq;
+  lower(EconOalpha, g, EcovOalpha);
+  EsqOalphasq = 0.0;
+  for (j = 1; j < NDIM; j++) EsqOalphasq += EconOalpha[j] * EcovOalpha[j];
+  BsqmEsqOalphasq = BsqOalphasq - EsqOalphasq;
+  gammasq = BsqOalphasq / BsqmEsqOalphasq;
+  if ((gammasq > c->p.gammamax * c->p.gammamax) || (gammasq < 1.0)) {
+    Gsq = 1.0 - 1.0 / (c->p.gammamax * c->p.gammamax);
+    realBsqOalphasq = (sqrt(EsqOalphasq * BsqOalphasq / Gsq));
+  }
+  /* EBcovtovcon (phys.ffde.c:551-560) */
+  oBsqgeom = 1.0 / (realBsqOalphasq * g->g);
+  vcon[1] = alphasq * (-g->gcon[0][1] + (EcovOalpha[2] * BcovOalpha[3] - EcovOalpha[3] * BcovOalpha[2]) * oBsqgeom);
+  vcon[2] = alphasq * (-g->gcon[0][2] + (EcovOalpha[3] * BcovOalpha[1] - EcovOalpha[1] * BcovOalpha[3]) * oBsqgeom);
+  vcon[3] = alphasq * (-g->gcon[0][3] + (EcovOalpha[1] * BcovOalpha[2] - EcovOalpha[2] * BcovOalpha[1]) * oBsqgeom);
+  /* vcon2pr(VELREL4) -> ucon_calc_3vel (transforms.c:348-396, phys.c:885-955);
+     the reference returns u^i itself here (transforms.c:381-386) */
+  {
+    const double (*gc)[NDIM] = g->gcov;
+    discr = +gc[0][0] + gc[1][1] * vcon[1] * vcon[1] + gc[2][2] * vcon[2] * vcon[2] +
+            gc[3][3] * vcon[3] * vcon[3] +
+            2. * (gc[0][1] * vcon[1] + gc[0][2] * vcon[2] + gc[0][3] * vcon[3] + gc[1][2] * vcon[1] * vcon[2] +
+                  gc[1][3] * vcon[1] * vcon[3] + gc[2][3] * vcon[2] * vcon[3]);
+  }
+  if (discr > 0.) return 1;
+  ucon[0] = 1. / sqrt(-discr);
+  for (j = 1; j < NDIM; j++) ucon[j] = vcon[j] * ucon[0];
+  pr[U1] = ucon[1];
+  pr[U2] = ucon[2];
+  pr[U3] = ucon[3];
+  pr[B1] = U[B1];
+  pr[B2] = U[B2];
+  pr[B3] = U[B3];
+  return 0;
+}
+
+/* step_ch.c:468-552 Utoprimgen: UtoU(UEVOLVE->UNOTHING) then the inversion
+   of the configured equations of motion.  Returns 1 on a fatal failure
+   (FFDE space-like drift), which the reference propagates out of advance(). */
 static int Utoprimgen(const oracle_ctx *c, const double *U, const geom_t *g, double *pr, int *pflag,
                       long *nstroke) {
   double Ug[NPR];
   for (int k = 0; k < NPR; k++) Ug[k] = U[k] / g->e[k];
   if (c->p.eomtype != OR_FFDE) Ug[UU] += -Ug[RHO];
+  if (c->p.eomtype == OR_FFDE) {
+    *pflag = 0;
+    return Utoprim_ffde(c, Ug, g, pr);
+  }
   return Utoprim_2d_final(c, Ug, g, pr, pflag, nstroke);
 }
 
@@ -755,8 +822,8 @@ int or_advance(oracle_ctx *c, int pi_sel, int pb_sel, const double *CUf, int pf_
       for (int k = 0; k < NPR; k++)
         un[k] += Cunew[0] * Ui[k] + Cunew[1] * dt * (dUriemann[k] + dUgeom[k]) + Cunew[2] * Uf[k];
       int pfl;
-      if (timeorder == numtimeorders - 1) Utoprimgen(c, un, &g, pfz, &pfl, &nstroke);
-      else Utoprimgen(c, Uf, &g, pfz, &pfl, &nstroke);
+      if (timeorder == numtimeorders - 1) fail |= Utoprimgen(c, un, &g, pfz, &pfl, &nstroke);
+      else fail |= Utoprimgen(c, Uf, &g, pfz, &pfl, &nstroke);
       c->pflag[(size_t)zi(c, i, j) * NUMPFLAGS + 0] = pfl;
       if (fixup1zone(c, pfz, &g)) fail |= 1;
     }
@@ -1022,7 +1089,7 @@ int or_utoprimgen(oracle_ctx *c, int i, int j, const double *U, double *pr, int 
   geom_t g;
   long ns = 0;
   get_geometry(c, i, j, CENT, &g);
-  Utoprimgen(c, U, &g, pr, pflag, &ns);
+  if (Utoprimgen(c, U, &g, pr, pflag, &ns)) *pflag = 50; /* FFDE space-like drift */
   return (int)ns;
 }
 

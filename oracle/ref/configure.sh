@@ -2,15 +2,15 @@
 # Writes a configured copy of the reference's flag headers into $OUT.
 # HARM is configured by editing these headers (global.h, metric.h, step_ch.h);
 # only #define lines are changed, the .c files are compiled where they lie.
-#   usage: configure.sh REFDIR OUT N1 N2 MCOORD FIXUPFLUX
+#   usage: configure.sh REFDIR OUT N1 N2 MCOORD FIXUPFLUX [EOMTYPE]
 set -e
-REF=$1; OUT=$2; N1=$3; N2=$4; MCOORD=$5; FIXUPFLUX=$6
+REF=$1; OUT=$2; N1=$3; N2=$4; MCOORD=$5; FIXUPFLUX=$6; EOM=${7:-EOMGRMHD}
 mkdir -p "$OUT"
 sed -e "s/^#define N1\([[:space:]]\).*/#define N1 $N1/" \
     -e "s/^#define N2\([[:space:]]\).*/#define N2 $N2/" \
     -e 's/^#define PRODUCTION 0/#define PRODUCTION 1/' \
     -e 's/^#define MAILWHENDONE 1/#define MAILWHENDONE 0/' \
-    -e 's/^#define EOMTYPE EOMFFDE/#define EOMTYPE EOMGRMHD/' \
+    -e "s/^#define EOMTYPE EOMFFDE/#define EOMTYPE $EOM/" \
     -e 's/^#define UTOPRIMTRYAGAIN 1/#define UTOPRIMTRYAGAIN 0/' \
     -e 's/^#define CHECKSOLUTION 1/#define CHECKSOLUTION 0/' \
     "$REF/global.h" > "$OUT/global.h"
@@ -18,7 +18,7 @@ sed -e "s/^#define MCOORD .*/#define MCOORD $MCOORD/" "$REF/metric.h" > "$OUT/me
 sed -e "s/^#define FIXUPFLUX 1/#define FIXUPFLUX $FIXUPFLUX/" "$REF/step_ch.h" > "$OUT/step_ch.h"
 # each edit must have matched exactly once
 for pat in "define N1 $N1" "define N2 $N2" "define PRODUCTION 1" "define MAILWHENDONE 0" \
-           "define EOMTYPE EOMGRMHD" "define UTOPRIMTRYAGAIN 0" "define CHECKSOLUTION 0"; do
+           "define EOMTYPE $EOM" "define UTOPRIMTRYAGAIN 0" "define CHECKSOLUTION 0"; do
   test "$(grep -c "^#$pat\$" "$OUT/global.h")" = 1 || { echo "configure: '$pat' not applied"; exit 1; }
 done
 grep -q "^#define MCOORD $MCOORD" "$OUT/metric.h"

@@ -5,12 +5,15 @@ For each case the script builds the input state and geometry with
 harm_ramt_amd's host generators, runs the reference's own step_ch() through
 oracle/_ref/ref_{ks,mink} (see oracle/ref/), and stores
   - sha256 of the input state and of the geometry tables (pins the inputs),
-  - per step: t, dt, Newton iteration count (nstroke), failure count,
+  - per step: t, dt, Newton iteration count (nstroke), failure code
+    (failed + 2 x number of zones with a nonzero FLAGUTOPRIMFAIL pflag; the
+    force-free inversion never writes pflag, so there it counts whatever the
+    reference's pflag array holds),
   - the full primitive state p[N1+6][N2+6][8] after the listed steps.
 The reference's set_grid() tables for the torus grid are stored as a sha256
 plus a sample, pinning the product's geometry generator.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [CASE ...]   (default: every case)
 """
 import hashlib
 import os
@@ -38,6 +41,8 @@ CASES = {
     "ot_para_laxf_rk2": ("mink", 32, 32, H.PARA, H.LAXFFLUX, 2, 10, 1e-4, (10,)),
     "ot_donor_hll_rk1": ("mink", 32, 32, H.DONOR, H.HLLFLUX, 1, 5, 1e-4, (5,)),
     "ot_nlimcent_hll_rk3": ("mink", 32, 32, H.NLIMCENT, H.HLLFLUX, 3, 5, 1e-4, (5,)),
+    "ffde_monopole_mc_laxf_rk2": ("ffde", 32, 64, H.MC, H.LAXFFLUX, 2, 200, 1e-3, (1, 100, 200)),
+    "ffde_monopole_para_hll_rk4": ("ffde", 32, 64, H.PARA, H.HLLFLUX, 4, 40, 1e-3, (40,)),
 }
 
 
@@ -52,6 +57,11 @@ def problem(kind, n1, n2):
         geom = H.Geometry(g)
         p0 = H.init_fishmon(geom)
         gam, cour, bc, fix, pole = 4.0 / 3.0, 0.9, (H.OUTFLOW, H.OUTFLOW, H.POLARAXIS, H.POLARAXIS), 1, 1
+    elif kind == "ffde":
+        g = H.grid_mks(n1, n2, a=0.9, Rout=100.0, hslope=0.1)
+        geom = H.Geometry(g)
+        p0 = H.init_monopole(geom)
+        gam, cour, bc, fix, pole = 4.0 / 3.0, 0.9, (H.OUTFLOW, H.OUTFLOW, H.POLARAXIS, H.POLARAXIS), 1, 0
     else:
         g = H.grid_cartesian(n1, n2)
         geom = H.Geometry(g)
@@ -64,7 +74,7 @@ def run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work):
     p0.tofile(os.path.join(work, "p0.bin"))
     cfg = dict(prim=os.path.join(work, "p0.bin"), dt=dt0, nsteps=nsteps, out=os.path.join(work, "o.bin"),
                lim=lim, fluxmethod=flux, timeorder=to)
-    if kind == "ks":
+    if kind in ("ks", "ffde"):
         cfg.update(a=g.a, Rout=g.Rout, hslope=g.hslope, R0=g.R0, geom="ref", gam=4.0 / 3.0, cour=0.9,
                    tables_out=os.path.join(work, "tables.bin"))
     else:
@@ -93,7 +103,9 @@ def run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work):
 def main():
     subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle", "ref"), "-s"])
     with tempfile.TemporaryDirectory() as work:
-        for name, (kind, n1, n2, lim, flux, to, nsteps, dt0, saved) in CASES.items():
+        names = sys.argv[1:] or list(CASES)
+        for name in names:
+            kind, n1, n2, lim, flux, to, nsteps, dt0, saved = CASES[name]
             g, geom, p0, extra = problem(kind, n1, n2)
             recs, states = run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work)
             out = dict(kind=kind, n1=n1, n2=n2, lim=lim, fluxmethod=flux, timeorder=to, nsteps=nsteps, dt0=dt0,
@@ -104,6 +116,8 @@ def main():
                 out[f"p_{s}"] = states[s - 1]
             np.savez_compressed(os.path.join(OUT_DIR, f"{name}.npz"), **out)
             print(name, "nstroke", int(recs[:, 2].sum()), "failures", int((recs[:, 3] // 2).sum()))
+        if sys.argv[1:]:
+            return
         # reference set_grid() tables for the 32x64 torus grid
         kind, n1, n2 = "ks", 32, 64
         g, geom, p0, extra = problem(kind, n1, n2)

@@ -15,13 +15,19 @@ def sha(a):
 
 
 def problem(kind, n1, n2, lim=H.PARA, fluxmethod=H.HLLFLUX, timeorder=4, seed=7):
-    """(grid, geometry, initial state, harm_params) for a torus ('ks') or an
-    Orszag-Tang ('mink') problem."""
+    """(grid, geometry, initial state, harm_params) for a torus ('ks'), an
+    Orszag-Tang ('mink') or a force-free split-monopole ('ffde') problem."""
     if kind == "ks":
         g = H.grid_mks(n1, n2)
         geom = H.Geometry(g)
         p0 = H.init_fishmon(geom, seed=seed)
         prm = H.make_params(n1, n2, g.dx1, g.dx2, lim=lim, fluxmethod=fluxmethod, timeorder=timeorder)
+    elif kind == "ffde":
+        g = H.grid_mks(n1, n2, a=0.9, Rout=100.0, hslope=0.1)
+        geom = H.Geometry(g)
+        p0 = H.init_monopole(geom)
+        prm = H.make_params(n1, n2, g.dx1, g.dx2, lim=lim, fluxmethod=fluxmethod, timeorder=timeorder,
+                            eomtype=H.EOMFFDE, poledeath=0)
     else:
         g = H.grid_cartesian(n1, n2)
         geom = H.Geometry(g)

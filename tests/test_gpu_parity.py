@@ -137,6 +137,26 @@ def test_gpu_utoprimgen_matches_oracle():
     say(f"  inversions flagged: {fails} of {n} (both paths agree)")
 
 
+def test_gpu_ffde_utoprimgen_matches_oracle():
+    say("[case] force-free utoprimgen batch on the 16x32 monopole geometry, 4096 random states")
+    g, geom, p0, prm = problem("ffde", 16, 32, lim=H.MC, fluxmethod=H.LAXFFLUX, timeorder=2)
+    h = H.Harm(prm, geom)
+    o = oracle_ctx(geom, prm, p0, 1e-3)
+    rng = np.random.default_rng(12)
+    n = 4096
+    zones = np.stack([rng.integers(-2, 18, n), rng.integers(-2, 34, n)], axis=1).astype(np.int32)
+    U = np.zeros((n, 8))
+    for t in range(n):
+        pr = np.array([0.0, 0.0, *rng.normal(0, 2.0, 3), *rng.normal(0, 1.0, 3)])
+        U[t] = o.primtoU(int(zones[t, 0]), int(zones[t, 1]), 3, pr)
+    guess = np.zeros((n, 8))
+    out_g, fl_g = h.utoprimgen(zones, U, guess)
+    for t in range(n):
+        out_o, fl_o, _ = o.utoprimgen(int(zones[t, 0]), int(zones[t, 1]), U[t], guess[t])
+        assert fl_g[t] == fl_o, f"flag zone {t}"
+        assert same(out_g[t], out_o), f"state {t}"
+
+
 def test_gpu_graph_replay_equals_single_steps():
     say("[case] harm_step_n (HIP graph) vs harm_step, 64x64 torus, 12 steps")
     g, geom, p0, prm = problem("ks", 64, 64)
@@ -149,12 +169,14 @@ def test_gpu_graph_replay_equals_single_steps():
     assert a.get_time() == b.get_time()
 
 
-@pytest.mark.parametrize("kind,n", [("ks", 256), ("mink", 256)])
+@pytest.mark.parametrize("kind,n", [("ks", 256), ("mink", 256), ("ffde", 512)])
 def test_gpu_fullsize_matches_oracle(kind, n):
     say(f"[case] full size {kind} {n}x{n}, 6 steps bitwise vs oracle")
-    lim = H.PARA if kind == "ks" else H.MC
-    g, geom, p0, prm = problem(kind, n, n, lim=lim)
-    dt0 = 1e-3 if kind == "ks" else 1e-4
+    if kind == "ffde":   # the reference's force-free defaults (initbase.c:151-157)
+        g, geom, p0, prm = problem(kind, n, n, lim=H.MC, fluxmethod=H.LAXFFLUX, timeorder=2)
+    else:
+        g, geom, p0, prm = problem(kind, n, n, lim=H.PARA if kind == "ks" else H.MC)
+    dt0 = 1e-4 if kind == "mink" else 1e-3
     h = gpu_ctx(geom, prm, p0, dt0)
     o = oracle_ctx(geom, prm, p0, dt0)
     for s in range(6):

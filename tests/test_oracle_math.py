@@ -225,3 +225,63 @@ def test_periodic_minkowski_conserves_totals():
         o.step()
     # periodic Minkowski: no sources, the flux differences telescope
     assert np.all(np.abs(tot[1] - tot[0]) < 1e-13 * np.abs(tot[0]).max())
+
+
+# ---------------- force-free inversion (phys.ffde.c) ----------------
+
+def _ffde_mink_oracle(n=8):
+    g, geom, p0, prm = problem("mink", n, n)
+    prm.eomtype = H.EOMFFDE
+    o = Oracle(H.params_dict(prm))
+    o.set_geometry(*geom.tables())
+    return o
+
+
+def _drift_state(rng, gamma):
+    """Random field B and a drift velocity v perpendicular to it with Lorentz
+    factor gamma; Minkowski primitives (rho = u = 0, u-tilde = gamma v)."""
+    B = rng.normal(0, 1, 3)
+    d = rng.normal(0, 1, 3)
+    d -= B * (d @ B) / (B @ B)
+    v = d / np.linalg.norm(d) * np.sqrt(1 - 1 / gamma ** 2)
+    return np.array([0.0, 0.0, *(gamma * v), *B]), v, B
+
+
+def test_ffde_inversion_recovers_drift_velocity_minkowski():
+    # With v perpendicular to B the ideal-MHD stress tensor without matter has
+    # T^t_i = B^2 v_i = (E x B)_i with E = -v x B, which the inversion undoes:
+    # the drift velocity comes back (u^i = u-tilde^i with zero shift)
+    o = _ffde_mink_oracle()
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        gamma = rng.uniform(1.0, 20.0)
+        pr, v, B = _drift_state(rng, gamma)
+        U = o.primtoU(3, 4, 3, pr)
+        assert U[0] == 0.0                       # no rest mass moved into U (phys.c:230,250)
+        out, fl, _ = o.utoprimgen(3, 4, U, np.zeros(8))
+        assert fl == 0
+        np.testing.assert_allclose(out[2:5], pr[2:5], rtol=1e-9, atol=1e-12 * gamma)
+        assert np.array_equal(out[5:], B)        # pr[B] = U[B] exactly
+        assert out[0] == 0.0 and out[1] == 0.0   # rho, u untouched
+
+
+def test_ffde_inversion_caps_lorentz_factor_at_gammamax():
+    # AVOIDUTINF=2 (phys.ffde.c:224-230): for gamma^2 = B^2/(B^2-E^2) above
+    # GAMMAMAX^2 the field is rescaled so that the drift has exactly GAMMAMAX
+    o = _ffde_mink_oracle()
+    rng = np.random.default_rng(6)
+    for gamma in (30.0, 100.0, 1e3):
+        pr, v, B = _drift_state(rng, gamma)
+        U = o.primtoU(3, 4, 3, pr)
+        out, fl, _ = o.utoprimgen(3, 4, U, np.zeros(8))
+        g_out = np.sqrt(1 + out[2:5] @ out[2:5])
+        assert g_out == pytest.approx(25.0, rel=1e-9)
+        np.testing.assert_allclose(out[2:5] / np.linalg.norm(out[2:5]), v / np.linalg.norm(v), rtol=1e-9)
+
+
+def test_ffde_characteristics_are_light_speed():
+    o = _ffde_mink_oracle()
+    pr = np.array([0.0, 0.0, 0.0, 0.0, 0.0, 0.7, -0.3, 0.2])
+    for d in (1, 2):
+        vmax, vmin, fail = o.vchar(3, 4, 3, pr, d)
+        assert fail == 0 and vmax == 1.0 and vmin == -1.0
