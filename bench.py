@@ -99,6 +99,68 @@ def cpu_baseline(geom, p0, prm, n1, n2, seconds):
                       f"{el:.1f} s on {threads} OpenMP threads"}
 
 
+def batch_sweep(nruns, steps, warmup, seed=2024):
+    """Batched sweep of independent FM tori (grids between 64x64 and 512x256,
+    spin in [0, 0.99], r_max in [10, 16], beta in [30, 300], +-2 % u
+    perturbations, all drawn from one seed): zone-updates/s of the batch
+    advanced by one graph per batch step, against the same contexts stepped
+    one after another (each with its own graph).  Both legs time `steps`
+    steps; the batched leg is timed before and after the separate leg and
+    averaged, so the slow drift of the states does not favour either."""
+    from concurrent.futures import ThreadPoolExecutor
+    import numpy as np
+    import harm_ramt_amd as H
+    rng = np.random.default_rng(seed)
+    draws = [dict(n1=int(rng.integers(4, 33)) * 16, n2=int(rng.integers(4, 17)) * 16, a=float(rng.uniform(0, 0.99)),
+                  rmax=float(rng.uniform(10, 16)), beta=float(rng.uniform(30, 300)), seed=int(rng.integers(1, 2**31)))
+             for _ in range(nruns)]
+
+    def build(d):
+        g = H.grid_mks(d["n1"], d["n2"], a=d["a"], Rout=40.0, hslope=0.3)
+        geom = H.Geometry(g)
+        p0 = H.init_fishmon(geom, rin=6.0, rmax=d["rmax"], beta=d["beta"], randfact=4e-2, seed=d["seed"])
+        prm = H.make_params(d["n1"], d["n2"], g.dx1, g.dx2, lim=H.PARA, fluxmethod=H.HLLFLUX, timeorder=4, cour=0.9)
+        return geom, p0, prm
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        built = list(ex.map(build, draws))
+    runs = []
+    for geom, p0, prm in built:
+        h = H.Harm(prm, geom)
+        h.set_array(H.P, p0)
+        h.bound_prim(H.P)
+        h.set_time(0.0, 1e-5, 2000.0)
+        runs.append(h)
+    del built
+    zones = sum(d["n1"] * d["n2"] for d in draws)
+    b = H.Batch(runs)
+    b.step_n(warmup)                       # includes the batch autotuner
+    for h in runs:
+        h.step_n(1)                        # each context's own autotuner and graph
+    t0 = time.perf_counter()
+    b.step_n(steps)
+    tb1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for h in runs:
+        h.step_n(steps)
+    ts = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.step_n(steps)
+    tb2 = time.perf_counter() - t0
+    failed = sum(h.failures()[1] for h in runs)
+    fv, bv = b.get_tuning()
+    vb = steps * zones / (0.5 * (tb1 + tb2))
+    vs = steps * zones / ts
+    b.close()
+    for h in runs:
+        h.close()
+    return {"runs": nruns, "zones_per_batch_step": zones, "steps": steps, "seed": seed,
+            "grids": [f"{d['n1']}x{d['n2']}" for d in draws],
+            "batched_zone_updates_per_s": vb, "separate_zone_updates_per_s": vs, "speedup": vb / vs,
+            "batched_ms_per_step": 1e3 * 0.5 * (tb1 + tb2) / steps, "separate_ms_per_step": 1e3 * ts / steps,
+            "variants": {"flux": fv, "boundary": bv}, "failed_runs": failed}
+
+
 def copy_calibration(mib):
     """A plain copy of known size: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
     this run gives the counter reading for mib MiB read and mib MiB written."""
@@ -121,6 +183,9 @@ def main():
     ap.add_argument("--n2", type=int, default=256)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-runs", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--tuning", default=None,
                     help="force the kernel variants as FLUX,BOUND (harm_set_tuning); default: autotuned")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -182,6 +247,8 @@ def main():
                 "kernel": kname, "avg_launch_us": round(stages[dom]["avg_us"], 3),
                 "algorithmic_bytes_per_launch": bytes_launch}
 
+    h.close()
+    batched = None if args.no_batch else batch_sweep(args.batch_runs, args.batch_steps, 3)
     cpu = None if args.no_cpu else cpu_baseline(geom, p0, prm, n1, n2, args.cpu_seconds)
     out = {
         "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -197,6 +264,7 @@ def main():
                                                     H.BOUND_BLOCK: "one_workgroup"}[bv]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
         "stages": stages,
         "eager_timed_ms_per_step": 1e3 * el_prof / prof_steps,
     }

@@ -67,6 +67,8 @@ EXPORTS = [
     "harm_bound_prim", "harm_post_fixup", "harm_step", "harm_step_n", "harm_utoprimgen", "harm_get_failures",
     "harm_get_nstroke", "harm_prof_enable", "harm_prof_get", "harm_prof_reset",
     "harm_set_tuning", "harm_get_tuning",
+    "harm_batch_create", "harm_batch_destroy", "harm_batch_step_n", "harm_batch_set_tuning",
+    "harm_batch_get_tuning",
     "harm_grid_mks", "harm_grid_cartesian", "harm_geometry", "harm_init_fishmon",
     "harm_init_orszag_tang", "harm_init_monopole",
 ]
@@ -112,6 +114,11 @@ def load_library(path=LIB_PATH):
         "harm_prof_reset": [vp],
         "harm_set_tuning": [vp, ctypes.c_int, ctypes.c_int],
         "harm_get_tuning": [vp, ip, ip],
+        "harm_batch_create": [ctypes.POINTER(vp), ctypes.c_int, ctypes.POINTER(vp)],
+        "harm_batch_destroy": [vp],
+        "harm_batch_step_n": [vp, ctypes.c_int],
+        "harm_batch_set_tuning": [vp, ctypes.c_int, ctypes.c_int],
+        "harm_batch_get_tuning": [vp, ip, ip],
         "harm_grid_mks": [ctypes.POINTER(HarmGrid), ctypes.c_int, ctypes.c_int, ctypes.c_double,
                           ctypes.c_double, ctypes.c_double, ctypes.c_double],
         "harm_grid_cartesian": [ctypes.POINTER(HarmGrid), ctypes.c_int, ctypes.c_int],
@@ -368,3 +375,39 @@ class Harm:
         ms, cnt = ctypes.c_double(), ctypes.c_longlong()
         _chk(self.lib.harm_prof_get(self.h, kernel, ctypes.byref(ms), ctypes.byref(cnt)), "harm_prof_get")
         return ms.value, cnt.value
+
+
+class Batch:
+    """Many Harm contexts advanced together (harm_batch): one launch per stage
+    over every run's tiles, one graph replay per batch step.  Keeps references
+    to the contexts so they outlive the batch."""
+
+    def __init__(self, runs):
+        self.lib = load_library()
+        self.runs = list(runs)
+        arr = (ctypes.c_void_p * len(self.runs))(*[r.h.value for r in self.runs])
+        h = ctypes.c_void_p()
+        _chk(self.lib.harm_batch_create(arr, len(self.runs), ctypes.byref(h)), "harm_batch_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.harm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step_n(self, nsteps):
+        _chk(self.lib.harm_batch_step_n(self.h, nsteps), "harm_batch_step_n")
+
+    def set_tuning(self, flux_variant, bound_variant):
+        _chk(self.lib.harm_batch_set_tuning(self.h, flux_variant, bound_variant), "harm_batch_set_tuning")
+
+    def get_tuning(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _chk(self.lib.harm_batch_get_tuning(self.h, ctypes.byref(a), ctypes.byref(b)), "harm_batch_get_tuning")
+        return a.value, b.value

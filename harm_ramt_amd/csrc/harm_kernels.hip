@@ -255,14 +255,14 @@ __device__ __forceinline__ void flux_face(const KParams& P, const double* __rest
   }
 }
 
-__global__ __launch_bounds__(256) void k_flux(KParams P, const double* __restrict__ pr, double* __restrict__ F1,
+__device__ __forceinline__ void flux_body(KParams P, const double* __restrict__ pr, double* __restrict__ F1,
                                               double* __restrict__ F2, double* __restrict__ q1,
                                               double* __restrict__ q2, const double* __restrict__ gF,
-                                              DevScalars* __restrict__ ds, int dir0) {
-  const int dir = dir0 + blockIdx.z;
+                                              DevScalars* __restrict__ ds, int dir0, const dim3 bi, const dim3 gd) {
+  const int dir = dir0 + bi.z;
   const int side = threadIdx.x & 1;
-  const int jf = blockIdx.x * (blockDim.x >> 1) + (threadIdx.x >> 1);
-  const int ifc = blockIdx.y * blockDim.y + threadIdx.y;
+  const int jf = bi.x * (blockDim.x >> 1) + (threadIdx.x >> 1);
+  const int ifc = bi.y * blockDim.y + threadIdx.y;
   double dtij = 1.e9;
   int fail = 0;
   const size_t nz = (size_t)(P.n1 + 2 * NG) * P.N2M;
@@ -277,6 +277,14 @@ __global__ __launch_bounds__(256) void k_flux(KParams P, const double* __restric
   if ((threadIdx.x & 63) == 0 && wm < 1.e9)
     atomicMin(&ds->ndt_acc[dir - 1], (unsigned long long)__double_as_longlong(wm));
   if (fail) atomicOr(&ds->failed, 1);
+}
+
+__global__ __launch_bounds__(256) void k_flux(KParams P, const double* __restrict__ pr, double* __restrict__ F1,
+                                              double* __restrict__ F2, double* __restrict__ q1,
+                                              double* __restrict__ q2, const double* __restrict__ gF,
+                                              DevScalars* __restrict__ ds, int dir0) {
+  flux_body(P, pr, F1, F2, q1, q2, gF, ds, dir0, dim3(blockIdx.x, blockIdx.y, blockIdx.z),
+      dim3(gridDim.x, gridDim.y, gridDim.z));
 }
 
 // fluxcalc() with one thread per face (the whole face on one lane); same
@@ -398,14 +406,14 @@ __device__ __forceinline__ void flux_face_single(const KParams& P, const double*
   }
 }
 
-__global__ __launch_bounds__(256) void k_flux_single(KParams P, const double* __restrict__ pr,
+__device__ __forceinline__ void flux_single_body(KParams P, const double* __restrict__ pr,
                                                      double* __restrict__ F1, double* __restrict__ F2,
                                                      double* __restrict__ q1, double* __restrict__ q2,
                                                      const double* __restrict__ gF, DevScalars* __restrict__ ds,
-                                                     int dir0) {
-  const int dir = dir0 + blockIdx.z;
-  const int jf = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ifc = blockIdx.y * blockDim.y + threadIdx.y;
+                                                     int dir0, const dim3 bi, const dim3 gd) {
+  const int dir = dir0 + bi.z;
+  const int jf = bi.x * blockDim.x + threadIdx.x;
+  const int ifc = bi.y * blockDim.y + threadIdx.y;
   double dtij = 1.e9;
   int fail = 0;
   const size_t nz = (size_t)(P.n1 + 2 * NG) * P.N2M;
@@ -420,6 +428,15 @@ __global__ __launch_bounds__(256) void k_flux_single(KParams P, const double* __
   if ((threadIdx.x & 63) == 0 && wm < 1.e9)
     atomicMin(&ds->ndt_acc[dir - 1], (unsigned long long)__double_as_longlong(wm));
   if (fail) atomicOr(&ds->failed, 1);
+}
+
+__global__ __launch_bounds__(256) void k_flux_single(KParams P, const double* __restrict__ pr,
+                                                     double* __restrict__ F1, double* __restrict__ F2,
+                                                     double* __restrict__ q1, double* __restrict__ q2,
+                                                     const double* __restrict__ gF, DevScalars* __restrict__ ds,
+                                                     int dir0) {
+  flux_single_body(P, pr, F1, F2, q1, q2, gF, ds, dir0, dim3(blockIdx.x, blockIdx.y, blockIdx.z),
+      dim3(gridDim.x, gridDim.y, gridDim.z));
 }
 
 // corner EMF of flux_ct() (step_ch.c:1223-1234, CORNGDETVERSION=1) from the
@@ -680,16 +697,16 @@ __device__ int utoprimgen(const KParams& P, const double* U, const double* gc, d
 // ---------------------------------------------------------------------------
 // advance_standard() per-zone loop (advance.c:114-181), zones i<N1, j<N2.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_update(KParams P, RKCoef C, const double* __restrict__ pi,
+__device__ __forceinline__ void update_body(KParams P, RKCoef C, const double* __restrict__ pi,
                                                 const double* __restrict__ pb, double* __restrict__ pf,
                                                 double* __restrict__ ulast, double* __restrict__ unew,
                                                 const double* __restrict__ F1, const double* __restrict__ F2,
                                                 const double* __restrict__ q1, const double* __restrict__ q2,
                                                 const double* __restrict__ gC, const double* __restrict__ gConn,
                                                 int* __restrict__ pflag, int* __restrict__ faillist,
-                                                DevScalars* __restrict__ ds) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+                                                DevScalars* __restrict__ ds, const dim3 bi, const dim3 gd) {
+  const int j = bi.x * blockDim.x + threadIdx.x;
+  const int i = bi.y * blockDim.y + threadIdx.y;
   int niter = 0, failz = 0, fail = 0;
   if (i == 0 && j == 0) {  // this substage's Courant limits (advance.c:190 reads them)
     ds->ndt_last[0] = __longlong_as_double((long long)ds->ndt_acc[0]);
@@ -821,6 +838,18 @@ __global__ __launch_bounds__(256) void k_update(KParams P, RKCoef C, const doubl
     if (nf) atomicAdd(&ds->nfail, (int)nf);
   }
   if (fail) atomicOr(&ds->failed, 1);
+}
+
+__global__ __launch_bounds__(256) void k_update(KParams P, RKCoef C, const double* __restrict__ pi,
+                                                const double* __restrict__ pb, double* __restrict__ pf,
+                                                double* __restrict__ ulast, double* __restrict__ unew,
+                                                const double* __restrict__ F1, const double* __restrict__ F2,
+                                                const double* __restrict__ q1, const double* __restrict__ q2,
+                                                const double* __restrict__ gC, const double* __restrict__ gConn,
+                                                int* __restrict__ pflag, int* __restrict__ faillist,
+                                                DevScalars* __restrict__ ds) {
+  update_body(P, C, pi, pb, pf, ulast, unew, F1, F2, q1, q2, gC, gConn, pflag, faillist, ds, dim3(blockIdx.x, blockIdx.y, blockIdx.z),
+      dim3(gridDim.x, gridDim.y, gridDim.z));
 }
 
 // inflow_check_rel4vel for x1 ghosts at CENT (fixup.c:1733-1841)
@@ -1023,9 +1052,9 @@ __device__ inline void dt_update(const KParams& P, DevScalars* ds) {
 // Everything between two advance() calls, as one workgroup with barriers
 // between the phases: bound_prim, post_fixup, bound_prim, and after the last
 // substage the time/dt update.  The failed-zone list is filled by k_update.
-__global__ __launch_bounds__(1024) void k_boundary(KParams P, double* __restrict__ pv, const double* __restrict__ gC,
+__device__ __forceinline__ void boundary_body(KParams P, double* __restrict__ pv, const double* __restrict__ gC,
                                                    int* __restrict__ pflag, const int* __restrict__ faillist,
-                                                   double* __restrict__ pfix, DevScalars* __restrict__ ds, int mode) {
+                                                   double* __restrict__ pfix, DevScalars* __restrict__ ds, int mode, const dim3 bi, const dim3 gd) {
   int fail = 0;
   if (mode & KB_BOUND) bound_block(P, pv, gC, fail);
   if (mode & KB_FIXUP) {
@@ -1047,27 +1076,40 @@ __global__ __launch_bounds__(1024) void k_boundary(KParams P, double* __restrict
   if (fail) atomicOr(&ds->failed, 1);
 }
 
+__global__ __launch_bounds__(1024) void k_boundary(KParams P, double* __restrict__ pv, const double* __restrict__ gC,
+                                                   int* __restrict__ pflag, const int* __restrict__ faillist,
+                                                   double* __restrict__ pfix, DevScalars* __restrict__ ds, int mode) {
+  boundary_body(P, pv, gC, pflag, faillist, pfix, ds, mode, dim3(blockIdx.x, blockIdx.y, blockIdx.z),
+      dim3(gridDim.x, gridDim.y, gridDim.z));
+}
+
 // ---- the same boundary work as separate wide launches (bit-identical) ----
 
 // bound_prim() x1 faces: thread (j, gi)
-__global__ void k_bx1(KParams P, double* __restrict__ pv, const double* __restrict__ gC,
-                      DevScalars* __restrict__ ds, int reset_list) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (reset_list && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) ds->nlist = 0;
+__device__ __forceinline__ void bx1_body(KParams P, double* __restrict__ pv, const double* __restrict__ gC,
+                      DevScalars* __restrict__ ds, int reset_list, const dim3 bi, const dim3 gd) {
+  const int j = bi.x * blockDim.x + threadIdx.x;
+  if (reset_list && bi.x == 0 && bi.y == 0 && threadIdx.x == 0) ds->nlist = 0;
   if (j >= P.n2) return;
   int fail = 0;
-  bound_x1_cell(P, pv, gC, blockIdx.y, j, fail);
+  bound_x1_cell(P, pv, gC, bi.y, j, fail);
   if (fail) atomicOr(&ds->failed, 1);
+}
+
+__global__ void k_bx1(KParams P, double* __restrict__ pv, const double* __restrict__ gC,
+                      DevScalars* __restrict__ ds, int reset_list) {
+  bx1_body(P, pv, gC, ds, reset_list, dim3(blockIdx.x, blockIdx.y, blockIdx.z),
+      dim3(gridDim.x, gridDim.y, gridDim.z));
 }
 
 // bound_prim() x2 faces over all i (rows gj 0..5) and POLEDEATH of the two
 // interior pole rows (gj 6,7).  A ghost copied from a pole row reads its U2,B2
 // either before or after that row's x*0 and multiplies them by 0 itself, which
 // gives the same signed zero either way.  Optionally the t,dt update.
-__global__ void k_bx2(KParams P, double* __restrict__ pv, DevScalars* __restrict__ ds, int do_dt) {
-  const int i = (int)(blockIdx.x * blockDim.x + threadIdx.x) - NG;
-  const int gj = blockIdx.y;
-  if (do_dt && blockIdx.x == 0 && gj == 0 && threadIdx.x == 0) dt_update(P, ds);
+__device__ __forceinline__ void bx2_body(KParams P, double* __restrict__ pv, DevScalars* __restrict__ ds, int do_dt, const dim3 bi, const dim3 gd) {
+  const int i = (int)(bi.x * blockDim.x + threadIdx.x) - NG;
+  const int gj = bi.y;
+  if (do_dt && bi.x == 0 && gj == 0 && threadIdx.x == 0) dt_update(P, ds);
   if (i > P.n1 + NG - 1) return;
   if (gj < 2 * NG) {
     bound_x2_cell(P, pv, i, gj);
@@ -1080,15 +1122,27 @@ __global__ void k_bx2(KParams P, double* __restrict__ pv, DevScalars* __restrict
   }
 }
 
+__global__ void k_bx2(KParams P, double* __restrict__ pv, DevScalars* __restrict__ ds, int do_dt) {
+  bx2_body(P, pv, ds, do_dt, dim3(blockIdx.x, blockIdx.y, blockIdx.z),
+      dim3(gridDim.x, gridDim.y, gridDim.z));
+}
+
 // fixup_utoprim() over the failed-zone list: compute, then apply
+__device__ __forceinline__ void fixlist_body(KParams P, const double* __restrict__ pv, const int* __restrict__ pflag,
+                          const int* __restrict__ faillist, double* __restrict__ pfix, const double* __restrict__ gC,
+                          DevScalars* __restrict__ ds, const dim3 bi, const dim3 gd) {
+  int n = ds->nlist, fail = 0;
+  if (n > P.n1 * P.n2) n = P.n1 * P.n2;
+  for (int t = bi.x * blockDim.x + threadIdx.x; t < n; t += gd.x * blockDim.x)
+    fixup_zone(P, pv, pflag, pfix, gC, faillist[t], fail);
+  if (fail) atomicOr(&ds->failed, 1);
+}
+
 __global__ void k_fixlist(KParams P, const double* __restrict__ pv, const int* __restrict__ pflag,
                           const int* __restrict__ faillist, double* __restrict__ pfix, const double* __restrict__ gC,
                           DevScalars* __restrict__ ds) {
-  int n = ds->nlist, fail = 0;
-  if (n > P.n1 * P.n2) n = P.n1 * P.n2;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
-    fixup_zone(P, pv, pflag, pfix, gC, faillist[t], fail);
-  if (fail) atomicOr(&ds->failed, 1);
+  fixlist_body(P, pv, pflag, faillist, pfix, gC, ds, dim3(blockIdx.x, blockIdx.y, blockIdx.z),
+      dim3(gridDim.x, gridDim.y, gridDim.z));
 }
 
 // the scalar tail of a fixup-only boundary pass: clear the list, update dt
@@ -1097,17 +1151,24 @@ __global__ void k_tail(KParams P, DevScalars* __restrict__ ds, int do_dt) {
   if (do_dt) dt_update(P, ds);
 }
 
-__global__ void k_fixapply(KParams P, double* __restrict__ pv, int* __restrict__ pflag,
+__device__ __forceinline__ void fixapply_body(KParams P, double* __restrict__ pv, int* __restrict__ pflag,
                            const int* __restrict__ faillist, const double* __restrict__ pfix,
-                           const DevScalars* __restrict__ ds) {
+                           const DevScalars* __restrict__ ds, const dim3 bi, const dim3 gd) {
   int n = ds->nlist;
   if (n > P.n1 * P.n2) n = P.n1 * P.n2;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+  for (int t = bi.x * blockDim.x + threadIdx.x; t < n; t += gd.x * blockDim.x) {
     const int z = faillist[t];
 #pragma unroll
     for (int k = 0; k <= U3; k++) pv[(size_t)z * 8 + k] = pfix[(size_t)z * 8 + k];
     pflag[z] = -1;  // UTOPRIMFAILFIXED (global.h:524)
   }
+}
+
+__global__ void k_fixapply(KParams P, double* __restrict__ pv, int* __restrict__ pflag,
+                           const int* __restrict__ faillist, const double* __restrict__ pfix,
+                           const DevScalars* __restrict__ ds) {
+  fixapply_body(P, pv, pflag, faillist, pfix, ds, dim3(blockIdx.x, blockIdx.y, blockIdx.z),
+      dim3(gridDim.x, gridDim.y, gridDim.z));
 }
 
 __global__ void k_utoprim_batch(KParams P, int n, const int* __restrict__ zone, const double* __restrict__ U,
@@ -1136,6 +1197,72 @@ __global__ void k_pflag_expand(int nz, const int* __restrict__ pf, int* __restri
   out[(size_t)z * 5] = pf[z];
 #pragma unroll
   for (int k = 1; k < 5; k++) out[(size_t)z * 5 + k] = 0;
+}
+
+
+// ---------------------------------------------------------------------------
+// Batched runs: many independent grids advanced by one launch per stage.
+// Each run is described by a RunDesc in device memory; a per-kernel block
+// table maps every block of the launch to (run, local block x, y, z), so the
+// per-run kernel bodies run unchanged on each run's own arrays and scalars.
+// ---------------------------------------------------------------------------
+struct RunDesc {
+  KParams P;
+  double* arr[HARM_NARRAYS];
+  double *q1, *q2, *pfix;
+  const double *gF, *gC, *gConn;
+  int *pflag, *faillist;
+  DevScalars* ds;
+};
+
+constexpr int FIXBLK = 2;  // blocks per run for the failed-list kernels
+
+__global__ __launch_bounds__(256) void kb_flux(const RunDesc* __restrict__ runs, const int4* __restrict__ tab,
+                                               int pr_sel, int dir0) {
+  const int4 e = tab[blockIdx.x];
+  const RunDesc& r = runs[e.x];
+  flux_single_body(r.P, r.arr[pr_sel], r.arr[HARM_F1], r.arr[HARM_F2], r.q1, r.q2, r.gF, r.ds, dir0,
+                   dim3(e.y, e.z, e.w), dim3(0, 0, 0));
+}
+
+__global__ __launch_bounds__(256) void kb_update(const RunDesc* __restrict__ runs, const int4* __restrict__ tab,
+                                                 RKCoef C, int pi, int pb, int pf) {
+  const int4 e = tab[blockIdx.x];
+  const RunDesc& r = runs[e.x];
+  update_body(r.P, C, r.arr[pi], r.arr[pb], r.arr[pf], r.arr[HARM_ULAST], r.arr[HARM_UNEW], r.arr[HARM_F1],
+              r.arr[HARM_F2], r.q1, r.q2, r.gC, r.gConn, r.pflag, r.faillist, r.ds, dim3(e.y, e.z, e.w),
+              dim3(0, 0, 0));
+}
+
+__global__ __launch_bounds__(1024) void kb_boundary(const RunDesc* __restrict__ runs, int sel, int mode) {
+  const RunDesc& r = runs[blockIdx.x];
+  const int m = (r.P.eomtype == HARM_EOMGRMHD) ? mode : (mode & ~KB_FIXUP);
+  boundary_body(r.P, r.arr[sel], r.gC, r.pflag, r.faillist, r.pfix, r.ds, m, dim3(0, 0, 0), dim3(1, 1, 1));
+}
+
+__global__ void kb_bx1(const RunDesc* __restrict__ runs, const int4* __restrict__ tab, int sel, int reset_list) {
+  const int4 e = tab[blockIdx.x];
+  const RunDesc& r = runs[e.x];
+  bx1_body(r.P, r.arr[sel], r.gC, r.ds, reset_list, dim3(e.y, e.z, e.w), dim3(0, 0, 0));
+}
+
+__global__ void kb_bx2(const RunDesc* __restrict__ runs, const int4* __restrict__ tab, int sel, int do_dt) {
+  const int4 e = tab[blockIdx.x];
+  const RunDesc& r = runs[e.x];
+  bx2_body(r.P, r.arr[sel], r.ds, do_dt, dim3(e.y, e.z, e.w), dim3(0, 0, 0));
+}
+
+// failed-list kernels: FIXBLK blocks per run, block b of run r is blockIdx.x = r*FIXBLK+b
+__global__ void kb_fixlist(const RunDesc* __restrict__ runs, int sel) {
+  const RunDesc& r = runs[blockIdx.x / FIXBLK];
+  fixlist_body(r.P, r.arr[sel], r.pflag, r.faillist, r.pfix, r.gC, r.ds, dim3(blockIdx.x % FIXBLK, 0, 0),
+               dim3(FIXBLK, 1, 1));
+}
+
+__global__ void kb_fixapply(const RunDesc* __restrict__ runs, int sel) {
+  const RunDesc& r = runs[blockIdx.x / FIXBLK];
+  fixapply_body(r.P, r.arr[sel], r.pflag, r.faillist, r.pfix, r.ds, dim3(blockIdx.x % FIXBLK, 0, 0),
+                dim3(FIXBLK, 1, 1));
 }
 
 }  // namespace
@@ -1838,5 +1965,259 @@ extern "C" int harm_get_tuning(harm_ctx* c, int* flux_variant, int* bound_varian
   if (!c) return seterr("null ctx");
   if (flux_variant) *flux_variant = c->flux_variant;
   if (bound_variant) *bound_variant = c->bound_variant;
+  return 0;
+}
+
+// ===========================================================================
+// Batched runs (C-ABI): contexts advanced together, one launch per stage
+// ===========================================================================
+enum { BT_FLUX4 = 0, BT_FLUX2, BT_UPDATE, BT_BX1, BT_BX2, BT_N };
+
+struct harm_batch {
+  int n;
+  std::vector<harm_ctx*>* ctx;
+  RunDesc* d_runs;
+  int4* d_tab[BT_N];
+  int ntab[BT_N];
+  hipStream_t stream;
+  hipGraphExec_t graph;
+  bool graph_ok;
+  bool tuned;
+  int flux_variant;   // HARM_FLUX_FACE_64x4 or HARM_FLUX_FACE_64x2
+  int bound_variant;  // HARM_BOUND_*
+  int timeorder;
+};
+
+static void add_blocks(std::vector<int4>& v, int run, dim3 g) {
+  for (unsigned z = 0; z < g.z; z++)
+    for (unsigned y = 0; y < g.y; y++)
+      for (unsigned x = 0; x < g.x; x++) v.push_back(make_int4(run, (int)x, (int)y, (int)z));
+}
+
+static int batch_flux(harm_batch* b, int pr_sel) {
+  const int t = (b->flux_variant == HARM_FLUX_FACE_64x2) ? BT_FLUX2 : BT_FLUX4;
+  const dim3 blk = (t == BT_FLUX2) ? dim3(64, 2) : dim3(64, 4);
+  hipLaunchKernelGGL(kb_flux, dim3(b->ntab[t]), blk, 0, b->stream, b->d_runs, b->d_tab[t], pr_sel, 1);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+static int batch_boundary(harm_batch* b, int sel, int mode) {
+  if (b->bound_variant == HARM_BOUND_BLOCK) {
+    hipLaunchKernelGGL(kb_boundary, dim3(b->n), dim3(1024), 0, b->stream, b->d_runs, sel, mode);
+  } else {
+    const bool fix = (mode & KB_FIXUP) != 0, bound2 = (mode & KB_BOUND2) != 0;
+    if (mode & KB_BOUND) {
+      hipLaunchKernelGGL(kb_bx1, dim3(b->ntab[BT_BX1]), dim3(64), 0, b->stream, b->d_runs, b->d_tab[BT_BX1], sel, 0);
+      hipLaunchKernelGGL(kb_bx2, dim3(b->ntab[BT_BX2]), dim3(64), 0, b->stream, b->d_runs, b->d_tab[BT_BX2], sel,
+                         (!fix && !bound2 && (mode & KB_DT)) ? 1 : 0);
+    }
+    if (fix) {
+      hipLaunchKernelGGL(kb_fixlist, dim3(b->n * FIXBLK), dim3(64), 0, b->stream, b->d_runs, sel);
+      hipLaunchKernelGGL(kb_fixapply, dim3(b->n * FIXBLK), dim3(64), 0, b->stream, b->d_runs, sel);
+    }
+    if (!bound2) return seterr("batch boundary pass without the second bound_prim");
+    hipLaunchKernelGGL(kb_bx1, dim3(b->ntab[BT_BX1]), dim3(64), 0, b->stream, b->d_runs, b->d_tab[BT_BX1], sel,
+                       fix ? 1 : 0);
+    hipLaunchKernelGGL(kb_bx2, dim3(b->ntab[BT_BX2]), dim3(64), 0, b->stream, b->d_runs, b->d_tab[BT_BX2], sel,
+                       (mode & KB_DT) ? 1 : 0);
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// one step_ch() of every run (each with its own t, dt and Courant limit)
+static int batch_enqueue_step(harm_batch* b) {
+  double CUf[4][3], Cunew[4][3];
+  int pii[4], pbb[4], pff[4], nto;
+  rk_setup(b->timeorder, &nto, CUf, Cunew, pii, pbb, pff);
+  for (int to = 0; to < nto; to++) {
+    RKCoef C;
+    C.cuf0 = CUf[to][0]; C.cuf1 = CUf[to][1]; C.cuf2 = CUf[to][2];
+    C.cn0 = Cunew[to][0]; C.cn1 = Cunew[to][1]; C.cn2 = Cunew[to][2];
+    C.timeorder = to;
+    C.last = (to == nto - 1);
+    if (batch_flux(b, pbb[to])) return 1;
+    hipLaunchKernelGGL(kb_update, dim3(b->ntab[BT_UPDATE]), dim3(64, 4), 0, b->stream, b->d_runs,
+                       b->d_tab[BT_UPDATE], C, pii[to], pbb[to], pff[to]);
+    HIPCHK(hipGetLastError());
+    if (batch_boundary(b, pff[to], KB_BOUND | KB_FIXUP | KB_BOUND2 | ((to == nto - 1) ? KB_DT : 0))) return 1;
+  }
+  return 0;
+}
+
+static int batch_capture(harm_batch* b, hipGraphExec_t* ex) {
+  hipGraph_t g;
+  HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  int rc = batch_enqueue_step(b);
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc) return 1;
+  if (e != hipSuccess) return seterr("batch graph capture: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return seterr("batch graph instantiate: %s", hipGetErrorString(e));
+  return 0;
+}
+
+extern "C" int harm_batch_destroy(harm_batch* b) {
+  if (!b) return 0;
+  if (b->graph_ok) (void)hipGraphExecDestroy(b->graph);
+  for (int t = 0; t < BT_N; t++) (void)hipFree(b->d_tab[t]);
+  (void)hipFree(b->d_runs);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b->ctx;
+  delete b;
+  return 0;
+}
+
+extern "C" int harm_batch_create(harm_ctx* const* ctxs, int n, harm_batch** out) {
+  if (!out) return seterr("null output pointer");
+  *out = nullptr;
+  if (!ctxs || n < 1) return seterr("batch needs at least one context");
+  for (int r = 0; r < n; r++) {
+    if (!ctxs[r]) return seterr("context %d is null", r);
+    if (!ctxs[r]->geometry_set) return seterr("context %d: geometry not set", r);
+    if (ctxs[r]->prm.timeorder != ctxs[0]->prm.timeorder)
+      return seterr("context %d: every run of a batch must use the same TIMEORDER", r);
+  }
+  HIPCHK(hipDeviceSynchronize());
+  harm_batch* b = new harm_batch();
+  memset(b, 0, sizeof(*b));
+  b->n = n;
+  b->ctx = new std::vector<harm_ctx*>(ctxs, ctxs + n);
+  b->timeorder = ctxs[0]->prm.timeorder;
+  b->flux_variant = HARM_FLUX_FACE_64x4;
+  b->bound_variant = HARM_BOUND_KERNELS;
+  std::vector<RunDesc> hr(n);
+  std::vector<int4> tab[BT_N];
+  for (int r = 0; r < n; r++) {
+    harm_ctx* c = ctxs[r];
+    RunDesc& d = hr[r];
+    d.P = c->kp;
+    for (int a = 0; a < HARM_NARRAYS; a++) d.arr[a] = c->arr[a];
+    d.q1 = c->q1; d.q2 = c->q2; d.pfix = c->pfix;
+    d.gF = c->gF; d.gC = c->gC; d.gConn = c->gConn;
+    d.pflag = c->pflag; d.faillist = c->faillist; d.ds = c->ds;
+    const int n1 = c->kp.n1, n2 = c->kp.n2, nfj = n2 + 2, nfi = n1 + 2;
+    add_blocks(tab[BT_FLUX4], r, dim3((nfj + 63) / 64, (nfi + 3) / 4, 2));
+    add_blocks(tab[BT_FLUX2], r, dim3((nfj + 63) / 64, (nfi + 1) / 2, 2));
+    add_blocks(tab[BT_UPDATE], r, grid2(n2, n1, dim3(64, 4)));
+    add_blocks(tab[BT_BX1], r, dim3((n2 + 63) / 64, 2 * NG));
+    add_blocks(tab[BT_BX2], r, dim3((n1 + 2 * NG + 63) / 64, 2 * NG + 2));
+  }
+  hipError_t e = hipMalloc(&b->d_runs, sizeof(RunDesc) * n);
+  if (e == hipSuccess) e = hipMemcpy(b->d_runs, hr.data(), sizeof(RunDesc) * n, hipMemcpyHostToDevice);
+  for (int t = 0; t < BT_N && e == hipSuccess; t++) {
+    b->ntab[t] = (int)tab[t].size();
+    e = hipMalloc(&b->d_tab[t], sizeof(int4) * tab[t].size());
+    if (e == hipSuccess)
+      e = hipMemcpy(b->d_tab[t], tab[t].data(), sizeof(int4) * tab[t].size(), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    harm_batch_destroy(b);
+    return seterr("harm_batch_create: %s", hipGetErrorString(e));
+  }
+  *out = b;
+  return 0;
+}
+
+// Batch autotuner: the same four variant combinations as one-step batch
+// graphs, timed from the live state of every run, which is then restored.
+static int batch_autotune(harm_batch* b) {
+  if (b->tuned) return 0;
+  const int n = b->n;
+  std::vector<DevScalars> save(n);
+  std::vector<double*> keep(n, nullptr);
+  int rc = 0;
+  for (int r = 0; r < n && !rc; r++) {
+    harm_ctx* c = (*b->ctx)[r];
+    if (hipMemcpy(&save[r], c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost) != hipSuccess ||
+        hipMalloc(&keep[r], (size_t)c->nz * 8 * sizeof(double)) != hipSuccess ||
+        hipMemcpy(keep[r], c->arr[HARM_P], (size_t)c->nz * 8 * sizeof(double), hipMemcpyDeviceToDevice) != hipSuccess)
+      rc = seterr("batch autotune: save");
+  }
+  auto restore = [&]() -> hipError_t {
+    hipError_t e = hipSuccess;
+    for (int r = 0; r < n && e == hipSuccess; r++) {
+      harm_ctx* c = (*b->ctx)[r];
+      e = hipMemcpyAsync(c->arr[HARM_P], keep[r], (size_t)c->nz * 8 * sizeof(double), hipMemcpyDeviceToDevice,
+                         b->stream);
+      if (e == hipSuccess) e = hipMemcpyAsync(c->ds, &save[r], sizeof(DevScalars), hipMemcpyHostToDevice, b->stream);
+    }
+    return e;
+  };
+  float best = 1e30f;
+  int bf = HARM_FLUX_FACE_64x4, bb = HARM_BOUND_KERNELS;
+  const int fv[2] = {HARM_FLUX_FACE_64x4, HARM_FLUX_FACE_64x2};
+  const int bv[2] = {HARM_BOUND_KERNELS, HARM_BOUND_BLOCK};
+  hipEvent_t ea = nullptr, eb = nullptr;
+  if (!rc && (hipEventCreate(&ea) != hipSuccess || hipEventCreate(&eb) != hipSuccess)) rc = seterr("batch autotune: events");
+  for (int f = 0; f < 2 && !rc; f++)
+    for (int v = 0; v < 2 && !rc; v++) {
+      b->flux_variant = fv[f];
+      b->bound_variant = bv[v];
+      hipGraphExec_t ex;
+      if (batch_capture(b, &ex)) { rc = 1; break; }
+      hipError_t e = restore();
+      if (e == hipSuccess) e = hipGraphLaunch(ex, b->stream);  // warm-up
+      if (e == hipSuccess) e = restore();
+      if (e == hipSuccess) e = hipEventRecord(ea, b->stream);
+      for (int k = 0; k < 3 && e == hipSuccess; k++) e = hipGraphLaunch(ex, b->stream);
+      if (e == hipSuccess) e = hipEventRecord(eb, b->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(eb);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, ea, eb);
+      (void)hipGraphExecDestroy(ex);
+      if (e != hipSuccess) { rc = seterr("batch autotune: %s", hipGetErrorString(e)); break; }
+      if (ms < best) { best = ms; bf = fv[f]; bb = bv[v]; }
+    }
+  b->flux_variant = bf;
+  b->bound_variant = bb;
+  if (!rc) {
+    hipError_t e = restore();
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    if (e != hipSuccess) rc = seterr("batch autotune: %s", hipGetErrorString(e));
+  }
+  if (ea) (void)hipEventDestroy(ea);
+  if (eb) (void)hipEventDestroy(eb);
+  for (int r = 0; r < n; r++) (void)hipFree(keep[r]);
+  if (!rc) b->tuned = true;
+  return rc;
+}
+
+extern "C" int harm_batch_step_n(harm_batch* b, int nsteps) {
+  if (!b) return seterr("null batch");
+  if (nsteps < 0) return seterr("nsteps < 0");
+  if (batch_autotune(b)) return 1;
+  if (!b->graph_ok) {
+    if (batch_capture(b, &b->graph)) return 1;
+    b->graph_ok = true;
+  }
+  for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(b->graph, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+extern "C" int harm_batch_set_tuning(harm_batch* b, int flux_variant, int bound_variant) {
+  if (!b) return seterr("null batch");
+  if (flux_variant != HARM_FLUX_FACE_64x4 && flux_variant != HARM_FLUX_FACE_64x2)
+    return seterr("batch flux variant %d: only the thread-per-face variants are batched", flux_variant);
+  if (bound_variant != HARM_BOUND_KERNELS && bound_variant != HARM_BOUND_BLOCK)
+    return seterr("bad boundary variant %d", bound_variant);
+  b->flux_variant = flux_variant;
+  b->bound_variant = bound_variant;
+  b->tuned = true;
+  if (b->graph_ok) {
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph_ok = false;
+  }
+  return 0;
+}
+
+extern "C" int harm_batch_get_tuning(harm_batch* b, int* flux_variant, int* bound_variant) {
+  if (!b) return seterr("null batch");
+  if (flux_variant) *flux_variant = b->flux_variant;
+  if (bound_variant) *bound_variant = b->bound_variant;
   return 0;
 }

@@ -196,6 +196,24 @@ int harm_get_tuning(harm_ctx *ctx, int *flux_variant, int *bound_variant);
 int harm_prof_get(harm_ctx *ctx, int kernel, double *total_ms, long long *count);
 int harm_prof_reset(harm_ctx *ctx);
 
+/* ---- batched runs (no reference counterpart: the reference advances one
+   grid per process) ----
+   A batch advances many independent contexts -- grids of different sizes,
+   metrics, spins and parameters, each with its own t, dt and Courant limit --
+   with one launch per stage covering the tiles of every run, captured once
+   as a HIP graph per batch step.  Each run's results are bit-identical to
+   stepping its context alone.  Contexts must have their geometry set and
+   share TIMEORDER; they stay owned by the caller (destroy the batch first)
+   and their arrays are updated in place (harm_get_array etc. work as usual). */
+typedef struct harm_batch harm_batch;
+int harm_batch_create(harm_ctx *const *ctxs, int n, harm_batch **out);
+int harm_batch_destroy(harm_batch *batch);
+/* nsteps calls of step_ch() for every run (first call: variant autotuning) */
+int harm_batch_step_n(harm_batch *batch, int nsteps);
+/* flux variant HARM_FLUX_FACE_64x4 or HARM_FLUX_FACE_64x2, any boundary variant */
+int harm_batch_set_tuning(harm_batch *batch, int flux_variant, int bound_variant);
+int harm_batch_get_tuning(harm_batch *batch, int *flux_variant, int *bound_variant);
+
 /* ---- host-side problem and geometry generation (no GPU use) ---- */
 
 /* Grid/coordinate description (coord.c: set_points, bl_coord, dxdxprim). */

@@ -1,0 +1,113 @@
+"""Batched runs (harm_batch) through the C-ABI: every run of a batch must be
+bit-identical to the same context stepped alone, and to the oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+import harm_ramt_amd as H
+from oracle.oracle import Oracle
+from tests.helpers import problem, same
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def torus(n1, n2, a, rmax, beta, seed):
+    g = H.grid_mks(n1, n2, a=a, Rout=40.0, hslope=0.3)
+    geom = H.Geometry(g)
+    p0 = H.init_fishmon(geom, rin=6.0, rmax=rmax, beta=beta, randfact=4e-2, seed=seed)
+    prm = H.make_params(n1, n2, g.dx1, g.dx2, lim=H.PARA, fluxmethod=H.HLLFLUX, timeorder=4, cour=0.9)
+    return geom, p0, prm
+
+
+def rk4_problems():
+    """Three RK4 runs of different size, metric and parameters."""
+    out = [torus(32, 64, 0.9375, 12.0, 100.0, 7), torus(48, 40, 0.5, 14.0, 50.0, 3)]
+    g, geom, p0, prm = problem("mink", 32, 32, lim=H.MC)
+    out.append((geom, p0, prm))
+    return out
+
+
+def start(geom, p0, prm, dt0):
+    h = H.Harm(prm, geom)
+    h.set_array(H.P, p0)
+    h.bound_prim(H.P)
+    h.set_time(0.0, dt0, 1e30)
+    return h
+
+
+DT0 = [1e-3, 1e-3, 1e-4]
+
+
+@pytest.mark.parametrize("fv", [H.FLUX_FACE_64x4, H.FLUX_FACE_64x2])
+@pytest.mark.parametrize("bv", [H.BOUND_KERNELS, H.BOUND_BLOCK])
+def test_batch_runs_equal_separate_runs(fv, bv):
+    say(f"[case] batch of 3 RK4 runs (torus 32x64, torus 48x40 a=0.5, OT 32x32), flux={fv} bound={bv}, 20 steps")
+    probs = rk4_problems()
+    batched = [start(*pp, dt) for pp, dt in zip(probs, DT0)]
+    alone = [start(*pp, dt) for pp, dt in zip(probs, DT0)]
+    b = H.Batch(batched)
+    b.set_tuning(fv, bv)
+    assert b.get_tuning() == (fv, bv)
+    b.step_n(5)
+    b.step_n(15)
+    for k, (hb, ha) in enumerate(zip(batched, alone)):
+        ha.step_n(20)
+        assert hb.get_time() == ha.get_time(), f"run {k}: t, dt"
+        assert hb.nstroke() == ha.nstroke(), f"run {k}: Newton iterations"
+        assert same(hb.get_array(H.P), ha.get_array(H.P)), f"run {k}: state"
+        assert hb.failures() == ha.failures()
+    b.close()
+
+
+def test_batch_matches_oracle_and_autotunes():
+    say("[case] autotuned batch of 3 RK4 runs vs the oracle, 12 steps")
+    probs = rk4_problems()
+    batched = [start(*pp, dt) for pp, dt in zip(probs, DT0)]
+    b = H.Batch(batched)
+    b.step_n(12)
+    fv, bv = b.get_tuning()
+    say(f"  chosen flux={fv} bound={bv}")
+    for k, ((geom, p0, prm), dt0) in enumerate(zip(probs, DT0)):
+        o = Oracle(H.params_dict(prm))
+        o.set_geometry(*geom.tables())
+        o.set_array(0, p0)
+        o.bound_prim(0)
+        o.set_time(0.0, dt0, 1e30)
+        for _ in range(12):
+            o.step()
+        assert batched[k].get_time() == o.get_time(), f"run {k}: t, dt"
+        assert same(batched[k].get_array(H.P), o.get_array(0)), f"run {k}: state"
+        assert batched[k].nstroke() == o.nstroke()
+    b.close()
+
+
+def test_batch_of_force_free_runs():
+    say("[case] batch of 2 force-free monopoles (32x64, 48x64), RK2, 30 steps")
+    probs = []
+    for n1, n2 in ((32, 64), (48, 64)):
+        g, geom, p0, prm = problem("ffde", n1, n2, lim=H.MC, fluxmethod=H.LAXFFLUX, timeorder=2)
+        probs.append((geom, p0, prm))
+    batched = [start(*pp, 1e-3) for pp in probs]
+    alone = [start(*pp, 1e-3) for pp in probs]
+    b = H.Batch(batched)
+    b.step_n(30)
+    for k, (hb, ha) in enumerate(zip(batched, alone)):
+        ha.step_n(30)
+        assert hb.get_time() == ha.get_time()
+        assert same(hb.get_array(H.P), ha.get_array(H.P)), f"ffde run {k}"
+    b.close()
+
+
+def test_batch_rejects_mixed_time_orders():
+    say("[case] batch argument checks")
+    g, geom, p0, prm = problem("mink", 16, 16, lim=H.MC, timeorder=4)
+    g2, geom2, p02, prm2 = problem("mink", 16, 16, lim=H.MC, timeorder=2)
+    a, c = start(geom, p0, prm, 1e-4), start(geom2, p02, prm2, 1e-4)
+    with pytest.raises(H.HarmError, match="TIMEORDER"):
+        H.Batch([a, c])
