@@ -69,6 +69,9 @@ EXPORTS = [
     "harm_set_tuning", "harm_get_tuning",
     "harm_batch_create", "harm_batch_destroy", "harm_batch_step_n", "harm_batch_set_tuning",
     "harm_batch_get_tuning",
+    "harm_adj_create", "harm_adj_destroy", "harm_adj_forward", "harm_adj_get_dts",
+    "harm_adj_seed_magnetic_energy", "harm_adj_seed", "harm_adj_backward", "harm_adj_get_lambda",
+    "harm_adj_get_nonsmooth",
     "harm_grid_mks", "harm_grid_cartesian", "harm_geometry", "harm_init_fishmon",
     "harm_init_orszag_tang", "harm_init_monopole",
 ]
@@ -119,6 +122,15 @@ def load_library(path=LIB_PATH):
         "harm_batch_step_n": [vp, ctypes.c_int],
         "harm_batch_set_tuning": [vp, ctypes.c_int, ctypes.c_int],
         "harm_batch_get_tuning": [vp, ip, ip],
+        "harm_adj_create": [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)],
+        "harm_adj_destroy": [vp],
+        "harm_adj_forward": [vp, dp],
+        "harm_adj_get_dts": [vp, dp],
+        "harm_adj_seed_magnetic_energy": [vp, dp],
+        "harm_adj_seed": [vp, dp],
+        "harm_adj_backward": [vp],
+        "harm_adj_get_lambda": [vp, dp],
+        "harm_adj_get_nonsmooth": [vp, ip, ip],
         "harm_grid_mks": [ctypes.POINTER(HarmGrid), ctypes.c_int, ctypes.c_int, ctypes.c_double,
                           ctypes.c_double, ctypes.c_double, ctypes.c_double],
         "harm_grid_cartesian": [ctypes.POINTER(HarmGrid), ctypes.c_int, ctypes.c_int],
@@ -410,4 +422,61 @@ class Batch:
     def get_tuning(self):
         a, b = ctypes.c_int(), ctypes.c_int()
         _chk(self.lib.harm_batch_get_tuning(self.h, ctypes.byref(a), ctypes.byref(b)), "harm_batch_get_tuning")
+        return a.value, b.value
+
+
+class Adjoint:
+    """Discrete adjoint of n steps of a Harm context (harm_adj)."""
+
+    def __init__(self, harm: "Harm", nsteps, checkpoint_every=16):
+        self.lib = load_library()
+        self.harm = harm
+        self.nsteps = nsteps
+        h = ctypes.c_void_p()
+        _chk(self.lib.harm_adj_create(harm.h, nsteps, checkpoint_every, ctypes.byref(h)), "harm_adj_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.harm_adj_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def forward(self, dts=None):
+        if dts is None:
+            _chk(self.lib.harm_adj_forward(self.h, None), "harm_adj_forward")
+        else:
+            d = np.ascontiguousarray(dts, dtype=np.float64)
+            assert d.shape == (self.nsteps,)
+            _chk(self.lib.harm_adj_forward(self.h, _dptr(d)), "harm_adj_forward")
+
+    def dts(self):
+        d = np.zeros(self.nsteps)
+        _chk(self.lib.harm_adj_get_dts(self.h, _dptr(d)), "harm_adj_get_dts")
+        return d
+
+    def seed_magnetic_energy(self):
+        J = ctypes.c_double()
+        _chk(self.lib.harm_adj_seed_magnetic_energy(self.h, ctypes.byref(J)), "harm_adj_seed_magnetic_energy")
+        return J.value
+
+    def seed(self, lam):
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        assert lam.shape == self.harm.shape
+        _chk(self.lib.harm_adj_seed(self.h, _dptr(lam)), "harm_adj_seed")
+
+    def backward(self):
+        _chk(self.lib.harm_adj_backward(self.h), "harm_adj_backward")
+        lam = np.zeros(self.harm.shape)
+        _chk(self.lib.harm_adj_get_lambda(self.h, _dptr(lam)), "harm_adj_get_lambda")
+        return lam
+
+    def nonsmooth(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _chk(self.lib.harm_adj_get_nonsmooth(self.h, ctypes.byref(a), ctypes.byref(b)), "harm_adj_get_nonsmooth")
         return a.value, b.value

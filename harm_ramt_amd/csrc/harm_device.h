@@ -1,5 +1,5 @@
 // harm_device.h -- per-zone / per-face FP64 physics of HARM's conservative
-// step as __device__ functions for gfx950.
+// step as __host__ __device__ functions (the GPU path runs them on gfx950).
 //
 // Every function keeps the floating-point operation order of the reference
 // function it cites, and the library is compiled with -ffp-contract=off, so
@@ -14,6 +14,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <math.h>
+
+// host+device: the same functions also build for the CPU (a host-only
+// compile checks the adjoint's local derivatives against finite differences)
+#define HARM_HD __host__ __device__ __forceinline__
 
 namespace harm {
 
@@ -42,21 +46,21 @@ struct State {
   double ucon[4], ucov[4], bcon[4], bcov[4];
 };
 
-__device__ __forceinline__ double dmax(double a, double b) { return (a > b) ? a : b; }  // phys.c:1580
-__device__ __forceinline__ double dmin(double a, double b) { return (a > b) ? b : a; }  // phys.c:1587
-__device__ __forceinline__ double dsign(double a) {                                      // phys.c:1562
+HARM_HD double dmax(double a, double b) { return (a > b) ? a : b; }  // phys.c:1580
+HARM_HD double dmin(double a, double b) { return (a > b) ? b : a; }  // phys.c:1587
+HARM_HD double dsign(double a) {                                      // phys.c:1562
   return (a > 0.0) ? 1.0 : ((a < 0.0) ? -1.0 : 0.0);
 }
 
 // phys.c:848-877 gamma_calc
-__device__ __forceinline__ double gamma_calc(const double* pr, const GeoU& g) {
+HARM_HD double gamma_calc(const double* pr, const GeoU& g) {
   double qsq = g.g11 * pr[U1] * pr[U1] + g.g22 * pr[U2] * pr[U2] + g.g33 * pr[U3] * pr[U3] +
                2. * (g.g12 * pr[U1] * pr[U2] + g.g13 * pr[U1] * pr[U3] + g.g23 * pr[U2] * pr[U3]);
   return sqrt(1. + qsq);
 }
 
 // phys.c:633-652 lower
-__device__ __forceinline__ void lower(const double* v, const GeoU& g, double* w) {
+HARM_HD void lower(const double* v, const GeoU& g, double* w) {
   w[0] = g.g00 * v[0] + g.g01 * v[1] + g.g02 * v[2] + g.g03 * v[3];
   w[1] = g.g01 * v[0] + g.g11 * v[1] + g.g12 * v[2] + g.g13 * v[3];
   w[2] = g.g02 * v[0] + g.g12 * v[1] + g.g22 * v[2] + g.g23 * v[3];
@@ -64,7 +68,7 @@ __device__ __forceinline__ void lower(const double* v, const GeoU& g, double* w)
 }
 
 // phys.c:716-727 get_state with ucon_calc_rel4vel :830-845 and bcon_calc :285-294
-__device__ __forceinline__ void get_state(const double* pr, const GeoU& g, State& q) {
+HARM_HD void get_state(const double* pr, const GeoU& g, State& q) {
   double alpha = 1. / sqrt(-g.n00);
   double beta1 = g.n01 * alpha * alpha;
   double beta2 = g.n02 * alpha * alpha;
@@ -82,12 +86,12 @@ __device__ __forceinline__ void get_state(const double* pr, const GeoU& g, State
   lower(q.bcon, g, q.bcov);
 }
 
-__device__ __forceinline__ double dot4(const double* a, const double* b) {
+HARM_HD double dot4(const double* a, const double* b) {
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
 }
 
 // phys.c:381-406 mhd_calc_0: T^dir_j, j=0..3 into mhd[0..3]
-__device__ __forceinline__ void mhd_calc(const KParams& P, const double* pr, int dir, const State& q,
+HARM_HD void mhd_calc(const KParams& P, const double* pr, int dir, const State& q,
                                          double* mhd) {
   double r, u, Pg, w;
   if (P.eomtype == 0) {
@@ -110,7 +114,7 @@ __device__ __forceinline__ void mhd_calc(const KParams& P, const double* pr, int
 
 // phys.c:16-56 primtoflux (massflux :59, PRIMMAXWELL dual Faraday :155-185,
 // UtoU UNOTHING->UEVOLVE :219-265 with e[k]=gdet)
-__device__ __forceinline__ void primtoflux(const KParams& P, const double* pr, const State& q, int dir,
+HARM_HD void primtoflux(const KParams& P, const double* pr, const State& q, int dir,
                                            double gdet, double* F) {
   double fin[NPR];
   fin[RHO] = pr[RHO] * q.ucon[dir];
@@ -132,7 +136,7 @@ __device__ __forceinline__ void primtoflux(const KParams& P, const double* pr, c
 
 // vchar.c:37-135 vchar + :166-234 simplefast; returns 1 on the reference's
 // fail() conditions (COEFF_NEG, COEFF_SUP, VCHAR_DISCR)
-__device__ __forceinline__ int vchar(const KParams& P, const double* pr, const State& q, int dir,
+HARM_HD int vchar(const KParams& P, const double* pr, const State& q, int dir,
                                      const GeoU& g, double& vmax, double& vmin) {
   double gam = P.gam;
   double bsq = dot4(q.bcon, q.bcov);
@@ -181,7 +185,7 @@ __device__ __forceinline__ int vchar(const KParams& P, const double* pr, const S
 }
 
 // fixup.c:1298-1354 limit_gamma: -1 changed, 1 failure, 0 untouched
-__device__ __forceinline__ int limit_gamma(double gammamax, double* pr, const GeoU& g) {
+HARM_HD int limit_gamma(double gammamax, double* pr, const GeoU& g) {
   if (gammamax <= 1.0) return 0;
   double gamma = gamma_calc(pr, g);
   if ((gamma > gammamax && (gamma != 1.0))) {
@@ -196,15 +200,89 @@ __device__ __forceinline__ int limit_gamma(double gammamax, double* pr, const Ge
   return 0;
 }
 
+// The face flux of fluxcalc() (step_ch.c:930-1065) from the reconstructed
+// left/right states: BDIRCONT averaging of the normal field, limit_gamma on
+// both states (H6), get_state, F and U of both states, vchar, cminmax_calc
+// (:1095-1148) and HLL/LAXF (:1047-1054); dtij = cour dx / max(cmax, cmin)
+// (:1062-1065).  p_l, p_r are modified (averaging, limit_gamma).
+template <int DIR>
+HARM_HD void face_flux(const KParams& P, double* p_l, double* p_r, const GeoU& g, double* Fo, double& dtij,
+                       int& fail) {
+  {
+    constexpr int kb = B1 + DIR - 1;
+    double bavg = 0.5 * (p_l[kb] + p_r[kb]);
+    p_l[kb] = bavg;
+    p_r[kb] = bavg;
+  }
+  if (limit_gamma(P.gammamax, p_l, g) >= 1) fail = 1;
+  if (limit_gamma(P.gammamax, p_r, g) >= 1) fail = 1;
+  State sl, sr;
+  get_state(p_l, g, sl);
+  get_state(p_r, g, sr);
+  double F_l[NPR], F_r[NPR], U_l[NPR], U_r[NPR];
+  primtoflux(P, p_l, sl, DIR, g.gdet, F_l);
+  primtoflux(P, p_r, sr, DIR, g.gdet, F_r);
+  primtoflux(P, p_l, sl, 0, g.gdet, U_l);
+  primtoflux(P, p_r, sr, 0, g.gdet, U_r);
+  double cmax_l, cmin_l, cmax_r, cmin_r;
+  if (vchar(P, p_l, sl, DIR, g, cmax_l, cmin_l)) fail = 1;
+  if (vchar(P, p_r, sr, DIR, g, cmax_r, cmin_r)) fail = 1;
+  double lmin = dmin(cmin_l, cmin_r), lmax = dmax(cmax_l, cmax_r);
+  double cmin = fabs(dmax(0., -lmin)), cmax = fabs(dmax(0., lmax));
+  double ctop = dmax(cmax, cmin);
+  double hllfrac = (P.fluxmethod == 1) ? 1.0 : 0.0;
+  double laxffrac = (P.fluxmethod == 0) ? 1.0 : 0.0;
+  if (cmax + cmin != 0.0) {
+#pragma unroll
+    for (int k = 0; k < NPR; k++)
+      Fo[k] = hllfrac * ((cmax * F_l[k] + cmin * F_r[k] - cmax * cmin * (U_r[k] - U_l[k])) / (cmax + cmin)) +
+              laxffrac * (0.5 * (F_l[k] + F_r[k] - ctop * (U_r[k] - U_l[k])));
+  } else {
+#pragma unroll
+    for (int k = 0; k < NPR; k++) Fo[k] = (0.5 * (F_l[k] + F_r[k] - ctop * (U_r[k] - U_l[k])));
+  }
+  double cm = dmax(cmax, cmin);
+  dtij = P.cour * ((DIR == 1) ? P.dx1 : P.dx2) / cm;
+  if (!(dtij < 1.e9)) dtij = 1.e9;
+}
+
+// source_conn (phys.c:475-619, WITHGDET): dU[UU+l] = gdet sum_jk T^j_k
+// Gamma^k_{lj} with the connection stored as conn[k][l][j]
+HARM_HD void source_conn(const KParams& P, const double* pr, const GeoU& g, const double* cn, double* dU) {
+  State qb;
+  get_state(pr, g, qb);
+  double mhd[4][4];
+#pragma unroll
+  for (int jj = 0; jj < 4; jj++) mhd_calc(P, pr, jj, qb, mhd[jj]);
+  double dUc[4];
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    double acc = 0;
+#pragma unroll
+    for (int jj = 0; jj < 4; jj++)
+#pragma unroll
+      for (int kk = 0; kk < 4; kk++) acc += mhd[jj][kk] * cn[(kk * 4 + l) * 4 + jj];
+    dUc[l] = acc;
+  }
+  dU[RHO] = g.gdet * 0.;
+  dU[UU] = g.gdet * dUc[0];
+  dU[U1] = g.gdet * dUc[1];
+  dU[U2] = g.gdet * dUc[2];
+  dU[U3] = g.gdet * dUc[3];
+  dU[B1] = g.gdet * 0.;
+  dU[B2] = g.gdet * 0.;
+  dU[B3] = g.gdet * 0.;
+}
+
 // ---------------- reconstruction (interp.c) ----------------
 
 // interp.h:13-14
-__device__ __forceinline__ double minmod(double a, double b) {
+HARM_HD double minmod(double a, double b) {
   return ((a) * (b) <= 0) ? 0.0 : ((fabs(a) < fabs(b)) ? (a) : (b));
 }
 
 // interp.c:106-149 slope_lim_3points
-__device__ __forceinline__ double slope3(int lim, double yl, double yc, double yr) {
+HARM_HD double slope3(int lim, double yl, double yc, double yr) {
   if (lim == 3) {  // MC
     double Dqm = 2.0 * (yc - yl), Dqp = 2.0 * (yr - yc), Dqc = 0.5 * (yr - yl);
     return minmod(Dqc, minmod(Dqm, Dqp));
@@ -221,7 +299,7 @@ __device__ __forceinline__ double slope3(int lim, double yl, double yc, double y
 }
 
 // interp.c:157-205 csslope
-__device__ __forceinline__ double csslope(int kk, double x1, double x2, double x3, double x4, double x5) {
+HARM_HD double csslope(int kk, double x1, double x2, double x3, double x4, double x5) {
   double Dqp = (x4 - x3), Dqpp = (x5 - x4), Dqm = (x3 - x2), Dqmm = (x2 - x1);
   double Dqc = 0.5 * (x4 - x2), Dqcm = 0.5 * (x3 - x1), Dqcp = 0.5 * (x5 - x3);
   double s = 0.5 * (dsign(Dqp) + dsign(Dqm));
@@ -243,7 +321,7 @@ __device__ __forceinline__ double csslope(int kk, double x1, double x2, double x
 }
 
 // interp.c:449-538 para4 (PARA2LIM=MC): left/right edge values of the middle zone
-__device__ __forceinline__ void para4(double y0, double y1, double y2, double y3, double y4, double& lout,
+HARM_HD void para4(double y0, double y1, double y2, double y3, double y4, double& lout,
                                       double& rout) {
   double y[5] = {y0, y1, y2, y3, y4}, dq[4];
 #pragma unroll
@@ -281,14 +359,14 @@ struct U2P {
 };
 
 // utoprim_2d_final.c:356-370 vsq_calc
-__device__ __forceinline__ double vsq_calc(const U2P& s, double W) {
+HARM_HD double vsq_calc(const U2P& s, double W) {
   double Wsq = W * W;
   double Xsq = (s.Bsq + W) * (s.Bsq + W);
   return ((Wsq * s.Qtsq + s.QdotBsq * (s.Bsq + 2. * W)) / (Wsq * Xsq));
 }
 
 // utoprim_2d_final.c:1082-1155 func_vsq2 (+ pressure_W_vsq :447, dpdW_calc_vsq :465, dpdvsq_calc :479)
-__device__ __forceinline__ void func_vsq2(const U2P& s, double W, double vsq, double& dx0, double& dx1,
+HARM_HD void func_vsq2(const U2P& s, double W, double vsq, double& dx0, double& dx1,
                                           double& f, double& df) {
   double gam = s.gam, Wsq = W * W, gtmp = 1. - vsq;
   double p_tmp = (gam - 1.) * (W * gtmp - s.D * sqrt(gtmp)) / gam;
@@ -317,7 +395,7 @@ __device__ __forceinline__ void func_vsq2(const U2P& s, double W, double vsq, do
 
 // utoprim_2d_final.c:644-1012 general_newton_raphson without line search
 // (USE_LINE_SEARCH=-2), NEWCONVERGE=1, EXTRA_NEWT_ITER=2, MAX_NEWT_ITER=200
-__device__ __forceinline__ int newton2d(const U2P& s, double& x0, double& x1, int& niter) {
+HARM_HD int newton2d(const U2P& s, double& x0, double& x1, int& niter) {
   const double NEWT_TOL = 1.0e-15, MIN_NEWT_TOL = 1.0e-10;
   double VSQ_TOO_BIG = (1.0 - 1.0 / ((s.gammafail - 1.0) + 1.0));
   double GAMMASQ_TOO_BIG = (s.gammafail * s.gammafail);

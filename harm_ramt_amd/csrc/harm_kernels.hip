@@ -30,6 +30,7 @@
 #include <vector>
 #include <mutex>
 #include "harm_device.h"
+#include "harm_adjoint.h"
 #include "../../include/harm_amd.h"
 
 using namespace harm;
@@ -329,43 +330,8 @@ __device__ __forceinline__ void flux_face_single(const KParams& P, const double*
   }
   GeoU g;
   load_geo_face(gF, z0, g);
-  {
-    constexpr int kb = B1 + DIR - 1;
-    double bavg = 0.5 * (p_l[kb] + p_r[kb]);
-    p_l[kb] = bavg;
-    p_r[kb] = bavg;
-  }
-  if (limit_gamma(P.gammamax, p_l, g) >= 1) fail = 1;
-  if (limit_gamma(P.gammamax, p_r, g) >= 1) fail = 1;
-  State sl, sr;
-  get_state(p_l, g, sl);
-  get_state(p_r, g, sr);
-  double F_l[NPR], F_r[NPR], U_l[NPR], U_r[NPR];
-  primtoflux(P, p_l, sl, DIR, g.gdet, F_l);
-  primtoflux(P, p_r, sr, DIR, g.gdet, F_r);
-  primtoflux(P, p_l, sl, 0, g.gdet, U_l);
-  primtoflux(P, p_r, sr, 0, g.gdet, U_r);
-  double cmax_l, cmin_l, cmax_r, cmin_r;
-  if (vchar(P, p_l, sl, DIR, g, cmax_l, cmin_l)) fail = 1;
-  if (vchar(P, p_r, sr, DIR, g, cmax_r, cmin_r)) fail = 1;
-  double lmin = dmin(cmin_l, cmin_r), lmax = dmax(cmax_l, cmax_r);
-  double cmin = fabs(dmax(0., -lmin)), cmax = fabs(dmax(0., lmax));
-  double ctop = dmax(cmax, cmin);
-  double hllfrac = (P.fluxmethod == 1) ? 1.0 : 0.0;
-  double laxffrac = (P.fluxmethod == 0) ? 1.0 : 0.0;
   double Fo[NPR];
-  if (cmax + cmin != 0.0) {
-#pragma unroll
-    for (int k = 0; k < NPR; k++)
-      Fo[k] = hllfrac * ((cmax * F_l[k] + cmin * F_r[k] - cmax * cmin * (U_r[k] - U_l[k])) / (cmax + cmin)) +
-              laxffrac * (0.5 * (F_l[k] + F_r[k] - ctop * (U_r[k] - U_l[k])));
-  } else {
-#pragma unroll
-    for (int k = 0; k < NPR; k++) Fo[k] = (0.5 * (F_l[k] + F_r[k] - ctop * (U_r[k] - U_l[k])));
-  }
-  double cm = dmax(cmax, cmin);
-  dtij = P.cour * ((DIR == 1) ? P.dx1 : P.dx2) / cm;
-  if (!(dtij < 1.e9)) dtij = 1.e9;
+  face_flux<DIR>(P, p_l, p_r, g, Fo, dtij, fail);
   bool skip_b2 = false;
   if (P.fixupflux) {
     if (DIR == 1) {
@@ -739,32 +705,7 @@ __device__ __forceinline__ void update_body(KParams P, RKCoef C, const double* _
     primtoflux(P, pin, q, 0, g.gdet, Ui);
     // source(pb): T^j_k Gamma^k_{lj} contraction (phys.c:475-619)
     double dUg[NPR];
-    {
-      State qb;
-      get_state(pbz, g, qb);
-      double mhd[4][4];
-#pragma unroll
-      for (int jj = 0; jj < 4; jj++) mhd_calc(P, pbz, jj, qb, mhd[jj]);
-      const double* cn = gConn + (size_t)z * 64;
-      double dUc[4];
-#pragma unroll
-      for (int l = 0; l < 4; l++) {
-        double acc = 0;
-#pragma unroll
-        for (int jj = 0; jj < 4; jj++)
-#pragma unroll
-          for (int kk = 0; kk < 4; kk++) acc += mhd[jj][kk] * cn[(kk * 4 + l) * 4 + jj];
-        dUc[l] = acc;
-      }
-      dUg[RHO] = g.gdet * 0.;
-      dUg[UU] = g.gdet * dUc[0];
-      dUg[U1] = g.gdet * dUc[1];
-      dUg[U2] = g.gdet * dUc[2];
-      dUg[U3] = g.gdet * dUc[3];
-      dUg[B1] = g.gdet * 0.;
-      dUg[B2] = g.gdet * 0.;
-      dUg[B3] = g.gdet * 0.;
-    }
+    source_conn(P, pbz, g, gConn + (size_t)z * 64, dUg);
     // flux divergence (flux2dUavg, advance.c:204-265) with the CT B fluxes
     double f1a[NPR], f1b[NPR], f2a[NPR], f2b[NPR];
     load8(F1, z, f1a);
@@ -2221,3 +2162,10 @@ extern "C" int harm_batch_get_tuning(harm_batch* b, int* flux_variant, int* boun
   if (bound_variant) *bound_variant = b->bound_variant;
   return 0;
 }
+
+// ===========================================================================
+// Discrete adjoint (C-ABI): kernels in the anonymous namespace, then the host
+// side, in harm_adjoint.inl
+// ===========================================================================
+namespace {
+#include "harm_adjoint.inl"

@@ -214,6 +214,41 @@ int harm_batch_step_n(harm_batch *batch, int nsteps);
 int harm_batch_set_tuning(harm_batch *batch, int flux_variant, int bound_variant);
 int harm_batch_get_tuning(harm_batch *batch, int *flux_variant, int *bound_variant);
 
+/* ---- discrete adjoint (no reference counterpart) ----
+   Gradient of a functional of the state after nsteps calls of step_ch()
+   with respect to the initial state, by a hand-written reverse sweep of the
+   GPU step: the time-step sequence is held at the values the forward run
+   recorded (or was given); limiters, max/min and floors are differentiated
+   on the branch the forward run took; the inversion through the
+   implicit-function theorem at its converged solution.  Checkpoints every
+   checkpoint_every steps; each segment is recomputed once, keeping every
+   RK substage state, and then reversed.  Supported: EOMGRMHD, PERIODIC
+   boundaries, FIXUPFLUX off, limiters DONOR/VANL/MINM/MC/NLIM*, HLL or
+   LAXF, with or without flux-CT (harm_adj_create reports anything else).
+   Typical use: set the state (harm_set_array + harm_bound_prim) and time,
+   harm_adj_forward, a seed, harm_adj_backward, harm_adj_get_lambda.  The
+   context is left at the final state of the forward run. */
+typedef struct harm_adj harm_adj;
+int harm_adj_create(harm_ctx *ctx, int nsteps, int checkpoint_every, harm_adj **out);
+int harm_adj_destroy(harm_adj *adj);
+/* nsteps steps from the context's current state; dts[nsteps] imposes the
+   time steps, NULL records the adaptive ones */
+int harm_adj_forward(harm_adj *adj, const double *dts);
+int harm_adj_get_dts(harm_adj *adj, double *dts);
+/* seed lambda_N = dJ/dp_N for J = sum over interior zones of
+   0.5 b^2 gdet dx1 dx2 (the magnetic energy); returns J */
+int harm_adj_seed_magnetic_energy(harm_adj *adj, double *J);
+/* seed lambda_N from a host array [N1M][N2M][NPR] */
+int harm_adj_seed(harm_adj *adj, const double *lambda);
+int harm_adj_backward(harm_adj *adj);
+/* lambda_0 = dJ/dp_0 [N1M][N2M][NPR]: interior zones (ghost zones, which
+   bound_prim overwrites, carry zero) */
+int harm_adj_get_lambda(harm_adj *adj, double *lambda);
+/* non-differentiable points met in the reverse sweep (active limit_gamma,
+   clamped wave speeds, singular inversion Jacobian) and failed inversions
+   repaired in the recomputation: both 0 means the gradient is exact */
+int harm_adj_get_nonsmooth(harm_adj *adj, int *count, int *recompute_failures);
+
 /* ---- host-side problem and geometry generation (no GPU use) ---- */
 
 /* Grid/coordinate description (coord.c: set_points, bl_coord, dxdxprim). */

@@ -1,4 +1,6 @@
+import ctypes
 import os
+import shutil
 import subprocess
 import sys
 
@@ -20,3 +22,33 @@ def _built_oracle():
     if not os.path.exists(lib):
         subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle"), "-s"])
     yield
+
+
+SHIM_SRC = os.path.join(ROOT, "tests", "adjoint_shim.cpp")
+SHIM_LIB = os.path.join(ROOT, "tests", "_shim", "adjoint_shim.so")
+
+
+def build_adjoint_shim():
+    """Host-only build of the adjoint's local derivative functions (test
+    infrastructure; __graft_entry__.build() makes it too)."""
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    os.makedirs(os.path.dirname(SHIM_LIB), exist_ok=True)
+    tmp = SHIM_LIB + ".tmp"
+    subprocess.check_call([hipcc, "-x", "hip", "--offload-host-only", "-O2", "-std=c++17", "-ffp-contract=off",
+                           "-shared", "-fPIC", SHIM_SRC, "-o", tmp])
+    os.replace(tmp, SHIM_LIB)
+
+
+@pytest.fixture(scope="session")
+def shim():
+    if not os.path.exists(SHIM_LIB) or os.path.getmtime(SHIM_LIB) < max(
+            os.path.getmtime(SHIM_SRC), os.path.getmtime(os.path.join(ROOT, "harm_ramt_amd", "csrc", "harm_adjoint.h")),
+            os.path.getmtime(os.path.join(ROOT, "harm_ramt_amd", "csrc", "harm_device.h"))):
+        build_adjoint_shim()
+    lib = ctypes.CDLL(SHIM_LIB)
+    dp = ctypes.POINTER(ctypes.c_double)
+    lib.shim_bsq.restype = ctypes.c_double
+    lib.shim_slope3.restype = ctypes.c_double
+    lib.shim_slope3.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]
+    lib.shim_slope3_coef.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
+    return lib
