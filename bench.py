@@ -161,6 +161,111 @@ def batch_sweep(nruns, steps, warmup, seed=2024):
             "variants": {"flux": fv, "boundary": bv}, "failed_runs": failed}
 
 
+def adjoint_measure(n=256, nsteps=256, K=16, seed=3):
+    """Discrete adjoint on the Orszag-Tang problem (Minkowski, periodic, MC,
+    HLL, RK4, flux-CT) at n x n over nsteps steps with checkpoints every K:
+    J = magnetic energy at the end, gradients with respect to the field
+    amplitude B0, the velocity amplitude and a seeded Gaussian density
+    perturbation field (sigma 1e-3, also part of the state).  Reports the
+    cost of forward + reverse sweep against the plain step, and the adjoint
+    against central finite differences (h = 1e-6, same time steps)."""
+    import numpy as np
+    import harm_ramt_amd as H
+    g = H.grid_cartesian(n, n)
+    geom = H.Geometry(g)
+    prm = H.make_params(n, n, g.dx1, g.dx2, lim=H.MC, fluxmethod=H.HLLFLUX, timeorder=4, gam=5.0 / 3.0, cour=0.8,
+                        bc=(H.PERIODIC,) * 4, fixupflux=0, poledeath=0)
+    rng = np.random.default_rng(seed)
+    drho = np.zeros((n + 6, n + 6))
+    drho[3:-3, 3:-3] = rng.normal(0, 1e-3, (n, n))
+
+    def state(B0, vamp, gam=5.0 / 3.0):
+        i = np.arange(-3, n + 3)[:, None]
+        j = np.arange(-3, n + 3)[None, :]
+        x = g.startx1 + (i + 0.5) * g.dx1 + 0 * j
+        y = g.startx2 + (j + 0.5) * g.dx2 + 0 * i
+        v1, v2 = -vamp * np.sin(2 * np.pi * y), vamp * np.sin(2 * np.pi * x)
+        gamma = 1 / np.sqrt(1 - v1 * v1 - v2 * v2)
+        p = np.zeros((n + 6, n + 6, 8))
+        p[..., 0] = 25 / (36 * np.pi) + drho
+        p[..., 1] = 5 / (12 * np.pi) / (gam - 1)
+        p[..., 2], p[..., 3] = gamma * v1, gamma * v2
+        p[..., 5], p[..., 6] = -B0 * np.sin(2 * np.pi * y), B0 * np.sin(4 * np.pi * x)
+        return p
+
+    B0, vamp = 1.0 / np.sqrt(4 * np.pi), 0.5
+    p0 = state(B0, vamp)
+
+    def ctx(p):
+        h = H.Harm(prm, geom)
+        h.set_array(H.P, p)
+        h.bound_prim(H.P)
+        h.set_time(0.0, 1e-4, 1e30)
+        return h
+
+    # plain step (graph replay) on the same problem
+    h = ctx(p0)
+    h.step_n(3)
+    h.get_time()
+    t0 = time.perf_counter()
+    h.step_n(nsteps)
+    h.get_time()
+    t_plain = (time.perf_counter() - t0) / nsteps
+    h.close()
+    # adjoint: forward with checkpoints, then the reverse sweep
+    h = ctx(p0)
+    a = H.Adjoint(h, nsteps, K)
+    a.forward()                                          # warm-up, records dt
+    a.close()
+    h.close()
+    h = ctx(p0)
+    a = H.Adjoint(h, nsteps, K)
+    t0 = time.perf_counter()
+    a.forward()
+    t_fwd = time.perf_counter() - t0
+    J = a.seed_magnetic_energy()
+    t0 = time.perf_counter()
+    lam = a.backward()
+    t_bwd = time.perf_counter() - t0
+    dts = a.dts()
+    nonsmooth = a.nonsmooth()
+    a.close()
+    h.close()
+
+    def J_of(p):
+        hh = ctx(p)
+        hh.set_time(0.0, dts[0], 1e30)
+        aa = H.Adjoint(hh, nsteps, nsteps)
+        aa.forward(dts)
+        val = aa.seed_magnetic_energy()
+        aa.close()
+        hh.close()
+        return val
+
+    hh = 1e-6
+    checks = {}
+    w = np.zeros_like(p0)
+    w[3:-3, 3:-3, 0] = rng.normal(0, 1, (n, n))
+    for name, plus, minus, dpdq in (
+            ("B0", state(B0 + hh, vamp), state(B0 - hh, vamp), (state(B0 + hh, vamp) - state(B0 - hh, vamp)) / (2 * hh)),
+            ("v_amplitude", state(B0, vamp + hh), state(B0, vamp - hh),
+             (state(B0, vamp + hh) - state(B0, vamp - hh)) / (2 * hh)),
+            ("rho_perturbation_direction", p0 + hh * w, p0 - hh * w, w)):
+        ad = float((lam * dpdq).sum())
+        fd = (J_of(plus) - J_of(minus)) / (2 * hh)
+        scale = float(np.abs(lam * dpdq).sum())
+        checks[name] = {"adjoint": ad, "finite_difference": fd, "abs_diff_over_term_scale": abs(ad - fd) / scale,
+                        "rel_diff": abs(ad - fd) / abs(fd)}
+    return {"workload": f"Orszag-Tang {n}x{n}, Minkowski, periodic, MC+HLL+RK4, flux-CT, {nsteps} steps, "
+                        f"checkpoints every {K}, J = magnetic energy at the end",
+            "J": J, "plain_ms_per_step": 1e3 * t_plain,
+            "adjoint_forward_ms_per_step": 1e3 * t_fwd / nsteps, "adjoint_reverse_ms_per_step": 1e3 * t_bwd / nsteps,
+            "adjoint_cost_vs_plain_step": (t_fwd + t_bwd) / nsteps / t_plain,
+            "reverse_sweep_cost_vs_plain_step": t_bwd / nsteps / t_plain,
+            "nonsmooth_points": nonsmooth[0], "failed_inversions_in_recompute": nonsmooth[1],
+            "gradient_checks": checks}
+
+
 def copy_calibration(mib):
     """A plain copy of known size: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
     this run gives the counter reading for mib MiB read and mib MiB written."""
@@ -186,6 +291,7 @@ def main():
     ap.add_argument("--batch-runs", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--tuning", default=None,
                     help="force the kernel variants as FLUX,BOUND (harm_set_tuning); default: autotuned")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -249,6 +355,7 @@ def main():
 
     h.close()
     batched = None if args.no_batch else batch_sweep(args.batch_runs, args.batch_steps, 3)
+    adjoint = None if args.no_adjoint else adjoint_measure()
     cpu = None if args.no_cpu else cpu_baseline(geom, p0, prm, n1, n2, args.cpu_seconds)
     out = {
         "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -265,6 +372,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "adjoint": adjoint,
         "stages": stages,
         "eager_timed_ms_per_step": 1e3 * el_prof / prof_steps,
     }

@@ -333,7 +333,9 @@ HARM_HD void slope3_coef(int lim, double yl, double yc, double yr, double& cl, d
 }
 
 // Solve A^T x = b for a 5x5 A (row-major) by Gaussian elimination with
-// partial pivoting; returns 1 if A is singular.
+// partial pivoting; returns 1 if A is singular.  Every loop is unrolled and
+// the pivot rows are exchanged by compare-and-swap, so all indices are
+// compile-time constants and the system stays in registers.
 HARM_HD int solve5_transpose(const double A[5][5], const double* b, double* x) {
   double M[5][6];
 #pragma unroll
@@ -342,25 +344,35 @@ HARM_HD int solve5_transpose(const double A[5][5], const double* b, double* x) {
     for (int c = 0; c < 5; c++) M[r][c] = A[c][r];
     M[r][5] = b[r];
   }
+  int singular = 0;
+#pragma unroll
   for (int c = 0; c < 5; c++) {
-    int piv = c;
-    double best = fabs(M[c][c]);
-    for (int r = c + 1; r < 5; r++)
-      if (fabs(M[r][c]) > best) { best = fabs(M[r][c]); piv = r; }
-    if (best == 0.0) return 1;
-    if (piv != c)
-      for (int k = 0; k < 6; k++) { const double t = M[c][k]; M[c][k] = M[piv][k]; M[piv][k] = t; }
+#pragma unroll
+    for (int r = c + 1; r < 5; r++) {
+      const bool sw = fabs(M[r][c]) > fabs(M[c][c]);
+#pragma unroll
+      for (int k = 0; k < 6; k++) {
+        const double t = M[c][k];
+        M[c][k] = sw ? M[r][k] : t;
+        M[r][k] = sw ? t : M[r][k];
+      }
+    }
+    if (M[c][c] == 0.0) singular = 1;
+#pragma unroll
     for (int r = c + 1; r < 5; r++) {
       const double f = M[r][c] / M[c][c];
+#pragma unroll
       for (int k = c; k < 6; k++) M[r][k] -= f * M[c][k];
     }
   }
+#pragma unroll
   for (int r = 4; r >= 0; r--) {
     double s = M[r][5];
+#pragma unroll
     for (int k = r + 1; k < 5; k++) s -= M[r][k] * x[k];
     x[r] = s / M[r][r];
   }
-  return 0;
+  return singular;
 }
 
 // Implicit-function adjoint of the inversion U -> p at its converged
@@ -371,6 +383,7 @@ HARM_HD int inversion_rev(const KParams& P, const double* p, const GeoU& g, cons
   State q;
   get_state(p, g, q);
   double A[5][5], C[5][3];
+#pragma unroll
   for (int k = 0; k < 5; k++) {
     double e[NPR] = {0., 0., 0., 0., 0., 0., 0., 0.};
     e[k] = 1.;
@@ -379,14 +392,19 @@ HARM_HD int inversion_rev(const KParams& P, const double* p, const GeoU& g, cons
     double row[NPR] = {0., 0., 0., 0., 0., 0., 0., 0.};
     primtoflux_rev(P, p, q, 0, g.gdet, e, qb, row);
     get_state_rev(p, g, q, qb, row);
+#pragma unroll
     for (int m = 0; m < 5; m++) A[k][m] = row[m];
+#pragma unroll
     for (int m = 0; m < 3; m++) C[k][m] = row[5 + m];
   }
   double x[5];
   if (solve5_transpose(A, pbar, x)) return 1;
+#pragma unroll
   for (int m = 0; m < 5; m++) Ub[m] = x[m];
+#pragma unroll
   for (int m = 0; m < 3; m++) {
     double s = pbar[5 + m];
+#pragma unroll
     for (int k = 0; k < 5; k++) s -= C[k][m] * x[k];
     Ub[5 + m] = s / g.gdet;
   }
