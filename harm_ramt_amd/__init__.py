@@ -30,7 +30,7 @@ X1UP, X1DN, X2UP, X2DN = 0, 1, 2, 3
 P, PK0, PK1, F1, F2, ULAST, UNEW = 0, 1, 2, 3, 4, 5, 6
 METRIC_MINK, METRIC_KS = 0, 2
 FLUX_FACE_64x4, FLUX_FACE_64x2, FLUX_PAIR = 0, 1, 2
-BOUND_KERNELS, BOUND_BLOCK = 0, 1
+BOUND_KERNELS, BOUND_BLOCK, BOUND_KERNELS_SEQCHECK = 0, 1, 2
 K_FLUX, K_EMF, K_UPDATE, K_BOUNDARY = 0, 2, 3, 4
 STAGE_NAMES = {K_FLUX: "flux", K_EMF: "flux_ct", K_UPDATE: "update", K_BOUNDARY: "boundary"}
 
@@ -41,7 +41,7 @@ class HarmParams(ctypes.Structure):
         ("eomtype", ctypes.c_int), ("lim", ctypes.c_int), ("fluxmethod", ctypes.c_int),
         ("fluxb", ctypes.c_int), ("timeorder", ctypes.c_int),
         ("bc", ctypes.c_int * 4),
-        ("fixupflux", ctypes.c_int), ("poledeath", ctypes.c_int),
+        ("fixupflux", ctypes.c_int), ("poledeath", ctypes.c_int), ("checksolution", ctypes.c_int),
         ("gam", ctypes.c_double), ("cour", ctypes.c_double), ("safe", ctypes.c_double),
         ("defcon", ctypes.c_double), ("gammamax", ctypes.c_double), ("gammafail", ctypes.c_double),
         ("rhominlimit", ctypes.c_double), ("uuminlimit", ctypes.c_double),
@@ -64,7 +64,7 @@ EXPORTS = [
     "harm_create", "harm_destroy", "harm_last_error", "harm_set_stream", "harm_set_geometry",
     "harm_set_array", "harm_get_array", "harm_array_ptr", "harm_get_pflag", "harm_set_time",
     "harm_get_time", "harm_fluxcalc", "harm_fix_flux", "harm_flux_ct", "harm_advance",
-    "harm_bound_prim", "harm_post_fixup", "harm_step", "harm_step_n", "harm_utoprimgen", "harm_get_failures",
+    "harm_bound_prim", "harm_pre_fixup", "harm_post_fixup", "harm_step", "harm_step_n", "harm_utoprimgen", "harm_get_failures",
     "harm_get_nstroke", "harm_prof_enable", "harm_prof_get", "harm_prof_reset",
     "harm_set_tuning", "harm_get_tuning",
     "harm_batch_create", "harm_batch_destroy", "harm_batch_step_n", "harm_batch_set_tuning",
@@ -106,6 +106,7 @@ def load_library(path=LIB_PATH):
         "harm_flux_ct": [vp],
         "harm_advance": [vp, ctypes.c_int, ctypes.c_int, dp, ctypes.c_int, dp, ctypes.c_int, ctypes.c_int, dp],
         "harm_bound_prim": [vp, ctypes.c_int],
+        "harm_pre_fixup": [vp, ctypes.c_int],
         "harm_post_fixup": [vp, ctypes.c_int],
         "harm_step": [vp],
         "harm_step_n": [vp, ctypes.c_int],
@@ -230,7 +231,7 @@ def init_monopole(geom: Geometry, B0=1.0):
 
 
 def make_params(n1, n2, dx1, dx2, *, lim=PARA, fluxmethod=HLLFLUX, timeorder=4, bc=None, fixupflux=1,
-                poledeath=1, gam=4.0 / 3.0, cour=0.9, safe=1.3, defcon=1.0, gammamax=25.0,
+                poledeath=1, checksolution=0, gam=4.0 / 3.0, cour=0.9, safe=1.3, defcon=1.0, gammamax=25.0,
                 gammafail=None, rhominlimit=1e-20, uuminlimit=1e-20, fluxb=FLUXCTTOTH, eomtype=EOMGRMHD):
     """Parameter block with the reference's GRMHD defaults (initbase.c:119-243)."""
     prm = HarmParams()
@@ -239,7 +240,7 @@ def make_params(n1, n2, dx1, dx2, *, lim=PARA, fluxmethod=HLLFLUX, timeorder=4, 
     b = bc if bc is not None else (OUTFLOW, OUTFLOW, POLARAXIS, POLARAXIS)
     for k in range(4):
         prm.bc[k] = b[k]
-    prm.fixupflux, prm.poledeath = fixupflux, poledeath
+    prm.fixupflux, prm.poledeath, prm.checksolution = fixupflux, poledeath, checksolution
     prm.gam, prm.cour, prm.safe, prm.defcon = gam, cour, safe, defcon
     prm.gammamax = gammamax
     prm.gammafail = 100.0 * gammamax if gammafail is None else gammafail
@@ -337,6 +338,10 @@ class Harm:
     # bounds.fishmon.c:13
     def bound_prim(self, sel=P):
         _chk(self.lib.harm_bound_prim(self.h, sel), "harm_bound_prim")
+
+    # fixup.c:25
+    def pre_fixup(self, sel=P):
+        _chk(self.lib.harm_pre_fixup(self.h, sel), "harm_pre_fixup")
 
     # fixup.c:37
     def post_fixup(self, sel=P):

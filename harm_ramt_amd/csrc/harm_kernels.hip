@@ -48,6 +48,7 @@ struct DevScalars {
   int failed;
   int nfail;
   int nlist;  // entries of the failed-zone list of the current substage
+  int cs_changed;  // fixup_checksolution: the second pass differs from the first
   unsigned long long nstroke;
 };
 
@@ -902,20 +903,30 @@ __device__ void bound_block(const KParams& P, double* __restrict__ pr, const dou
 // pflag of a zone at distance <= 1 from the interior, as bound_pflag()
 // (boundsint.c:4-60) leaves it: x2 mapping first (polar reflection / periodic
 // wrap over all i), then x1 (outflow copy of the edge zone / periodic wrap).
-__device__ __forceinline__ int pflag_at(const KParams& P, const int* __restrict__ pflag, int i, int j) {
+__device__ __forceinline__ int pflag_zone(const KParams& P, int i, int j) {
   if (j < 0) j = (P.bc[HARM_X2DN] == HARM_PERIODIC) ? j + P.n2 : (0 + (0 - j - 1));
   else if (j >= P.n2) j = (P.bc[HARM_X2UP] == HARM_PERIODIC) ? j - P.n2 : ((P.n2 - 1) + ((P.n2 - 1) - j + 1));
   if (i < 0) i = (P.bc[HARM_X1DN] == HARM_PERIODIC) ? i + P.n1 : 0;
   else if (i >= P.n1) i = (P.bc[HARM_X1UP] == HARM_PERIODIC) ? i - P.n1 : P.n1 - 1;
-  return pflag[zidx(P, i, j)];
+  return zidx(P, i, j);
+}
+__device__ __forceinline__ int pflag_at(const KParams& P, const int* __restrict__ pflag, int i, int j) {
+  return pflag[pflag_zone(P, i, j)];
+}
+__device__ __forceinline__ bool interior(const KParams& P, int i, int j) {
+  return i >= 0 && i < P.n1 && j >= 0 && j < P.n2;
 }
 
 // fixup_utoprim() for one failed zone (fixup.c:527-920; UTOPRIMAVG,
 // ORDERINDEPENDENT, HANDLEUNEG=0, TODONOGOOD=NOGOODAVERAGE): the average of its
 // good neighbours from the bounded array (read-only during this phase), gamma
 // capped at its own previous gamma or GAMMAMAX; result to pfix.
+// cs (nullable): zones failed by fixup_checksolution in this pass.  Ghost
+// neighbours carry the flags bound_pflag copied before that check, so a
+// ghost whose source zone was failed by it still counts as good.
 __device__ void fixup_zone(const KParams& P, const double* __restrict__ pv, const int* __restrict__ pflag,
-                           double* __restrict__ pfix, const double* __restrict__ gC, int z, int& fail) {
+                           const int* __restrict__ cs, double* __restrict__ pfix, const double* __restrict__ gC,
+                           int z, int& fail) {
   const int i = z / P.N2M - NG, j = z % P.N2M - NG;
   double gc[36];
 #pragma unroll
@@ -926,7 +937,12 @@ __device__ void fixup_zone(const KParams& P, const double* __restrict__ pv, cons
   load8(pv, z, pz);
   double gamma = gamma_calc(pz, g);
   if (gamma > P.gammamax) gamma = P.gammamax;
-  auto good = [&](int di, int dj) { return pflag_at(P, pflag, i + di, j + dj) == 0; };
+  auto good = [&](int di, int dj) {
+    const int a = i + di, b = j + dj, zm = pflag_zone(P, a, b);
+    int f = pflag[zm];
+    if (cs && !interior(P, a, b) && cs[zm]) f = 0;
+    return f == 0;
+  };
   auto val = [&](int di, int dj, int k) { return pv[(size_t)zidx(P, i + di, j + dj) * 8 + k]; };
   auto avg41 = [&](int k) { return 0.25 * (val(0, 1, k) + val(0, -1, k) + val(-1, 0, k) + val(1, 0, k)); };
   auto avg42 = [&](int k) { return 0.25 * (val(1, 1, k) + val(1, -1, k) + val(-1, 1, k) + val(-1, -1, k)); };
@@ -1001,7 +1017,7 @@ __device__ __forceinline__ void boundary_body(KParams P, double* __restrict__ pv
   if (mode & KB_FIXUP) {
     int n = ds->nlist;
     if (n > P.n1 * P.n2) n = P.n1 * P.n2;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) fixup_zone(P, pv, pflag, pfix, gC, faillist[t], fail);
+    for (int t = threadIdx.x; t < n; t += blockDim.x) fixup_zone(P, pv, pflag, nullptr, pfix, gC, faillist[t], fail);
     __syncthreads();
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
       const int z = faillist[t];
@@ -1070,19 +1086,19 @@ __global__ void k_bx2(KParams P, double* __restrict__ pv, DevScalars* __restrict
 
 // fixup_utoprim() over the failed-zone list: compute, then apply
 __device__ __forceinline__ void fixlist_body(KParams P, const double* __restrict__ pv, const int* __restrict__ pflag,
-                          const int* __restrict__ faillist, double* __restrict__ pfix, const double* __restrict__ gC,
-                          DevScalars* __restrict__ ds, const dim3 bi, const dim3 gd) {
+                          const int* __restrict__ cs, const int* __restrict__ faillist, double* __restrict__ pfix,
+                          const double* __restrict__ gC, DevScalars* __restrict__ ds, const dim3 bi, const dim3 gd) {
   int n = ds->nlist, fail = 0;
   if (n > P.n1 * P.n2) n = P.n1 * P.n2;
   for (int t = bi.x * blockDim.x + threadIdx.x; t < n; t += gd.x * blockDim.x)
-    fixup_zone(P, pv, pflag, pfix, gC, faillist[t], fail);
+    fixup_zone(P, pv, pflag, cs, pfix, gC, faillist[t], fail);
   if (fail) atomicOr(&ds->failed, 1);
 }
 
 __global__ void k_fixlist(KParams P, const double* __restrict__ pv, const int* __restrict__ pflag,
-                          const int* __restrict__ faillist, double* __restrict__ pfix, const double* __restrict__ gC,
-                          DevScalars* __restrict__ ds) {
-  fixlist_body(P, pv, pflag, faillist, pfix, gC, ds, dim3(blockIdx.x, blockIdx.y, blockIdx.z),
+                          const int* __restrict__ cs, const int* __restrict__ faillist, double* __restrict__ pfix,
+                          const double* __restrict__ gC, DevScalars* __restrict__ ds) {
+  fixlist_body(P, pv, pflag, cs, faillist, pfix, gC, ds, dim3(blockIdx.x, blockIdx.y, blockIdx.z),
       dim3(gridDim.x, gridDim.y, gridDim.z));
 }
 
@@ -1132,12 +1148,164 @@ __global__ void k_utoprim_batch(KParams P, int n, const int* __restrict__ zone, 
   for (int k = 0; k < NPR; k++) pr[(size_t)t * 8 + k] = p[k];
 }
 
-__global__ void k_pflag_expand(int nz, const int* __restrict__ pf, int* __restrict__ out) {
+// ---------------------------------------------------------------------------
+// CHECKSOLUTION: get_bsqflags (pre_fixup) and fixup_checksolution (post_fixup)
+// ---------------------------------------------------------------------------
+constexpr double BSQORHOLIMIT = 20.0;     // initbase.c:213
+constexpr double BSQOULIMIT = 100.0;      // initbase.c:214
+constexpr double GAMMAPERCDIFFMAX = 2.0;  // global.h:264
+constexpr double UPERCDIFFMAX = 10.0;     // global.h:265
+constexpr int UTOPRIMFAILGAMMAPERC = 8;   // global.h:558
+constexpr int UTOPRIMFAILUPERC = 9;       // global.h:559
+
+// get_bsqflags (fixup.c:1197-1295, LIMADJUST=FLUXADJUST=0): b^2/rho and
+// b^2/u flags (0, 1, 2) packed as f_rho | f_u << 8.  The reference also flags
+// the ghost ring, but bound_pflag overwrites those with interior copies
+// before anything reads them, so only the interior is computed.
+__global__ void k_bsqflags(KParams P, const double* __restrict__ pv, const double* __restrict__ gC,
+                           int* __restrict__ bsqf) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= P.n1 || j >= P.n2) return;
+  const int z = zidx(P, i, j);
+  int f0 = 0, f3 = 0;
+  if (P.eomtype == HARM_EOMGRMHD) {
+    double gc[36];
+#pragma unroll
+    for (int m = 0; m < 36; m++) gc[m] = gC[(size_t)z * GCN + m];
+    GeoU g;
+    geo_from_cent(gc, g);
+    double pr[NPR];
+    load8(pv, z, pr);
+    State q;
+    get_state(pr, g, q);
+    const double bsq = dot4(q.bcon, q.bcov);
+    if (bsq / pr[RHO] > BSQORHOLIMIT) f0 = 2;
+    else if (bsq / pr[RHO] > 0.9 * BSQORHOLIMIT) f0 = 1;
+    if (bsq / pr[UU] > BSQOULIMIT) f3 = 2;
+    else if (bsq / pr[UU] > 0.9 * BSQOULIMIT) f3 = 1;
+  }
+  bsqf[z] = f0 | (f3 << 8);
+}
+
+// fixup.c:400-413: gamma at the flagged zones whose inversion succeeded, over
+// DQZLOOP (i in [-1,N1], j in [-1,N2]; ghost flags are the bound_pflag copies).
+// gammacheck persists between calls like the reference's static array.
+__global__ void k_gammacheck(KParams P, const double* __restrict__ pv, const double* __restrict__ gC,
+                             const int* __restrict__ pflag, const int* __restrict__ bsqf,
+                             double* __restrict__ gammacheck) {
+  const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x) - 1;
+  const int i = (int)(blockIdx.y * blockDim.y + threadIdx.y) - 1;
+  if (i > P.n1 || j > P.n2) return;
+  const int zm = pflag_zone(P, i, j);
+  if (bsqf[zm] == 0 || pflag[zm] != 0) return;
+  const int z = zidx(P, i, j);
+  double gc[36];
+#pragma unroll
+  for (int m = 0; m < 36; m++) gc[m] = gC[(size_t)z * GCN + m];
+  GeoU g;
+  geo_from_cent(gc, g);
+  double pr[NPR];
+  load8(pv, z, pr);
+  gammacheck[z] = gamma_calc(pr, g);
+}
+
+// fixup.c:416-485 for one flagged interior zone: 0, UTOPRIMFAILGAMMAPERC or
+// UTOPRIMFAILUPERC.  Neighbour failure flags: the reference loop runs in
+// (i, j) order, so the three neighbours it has already passed ((i,j-1),
+// (i-1,j), (i-1,j-1)) show the flags it set there; those come from `prev`
+// (nullable), the others are the flags before the check.
+__device__ int cs_vote(const KParams& P, const double* __restrict__ pv, const int* __restrict__ pflag,
+                       const double* __restrict__ gammacheck, const int* __restrict__ prev, int i, int j) {
+  const int di[8] = {0, 0, 1, -1, 1, 1, 1, -1}, dj[8] = {1, -1, 0, 0, 1, -1, 1, -1};
+  const int z = zidx(P, i, j);
+  const double gz = gammacheck[z], uz = pv[(size_t)z * 8 + UU];
+  double pd[8], pd2[8];
+#pragma unroll
+  for (int l = 0; l < 8; l++) {
+    const int a = i + di[l], b = j + dj[l];
+    const int zn = zidx(P, a, b);
+    int f = pflag[pflag_zone(P, a, b)];
+    const bool earlier = interior(P, a, b) && (a < i || (a == i && b < j));
+    if (prev && earlier && prev[zn]) f = prev[zn];
+    const bool ok = (f == 0);
+    pd[l] = ok ? (gammacheck[zn] / gz) : -1;
+    pd2[l] = ok ? (pv[(size_t)zn * 8 + UU] / uz) : -1;
+  }
+  int vote0 = 0, vote1 = 0, numvotes = 0;
+  const bool grmhd = (P.eomtype == HARM_EOMGRMHD);
+#pragma unroll
+  for (int l = 0; l < 8; l++) {
+    if (pd[0] >= 0.0) {
+      if ((fabs(pd[l]) > GAMMAPERCDIFFMAX) || (fabs(pd[l]) < 1.0 / GAMMAPERCDIFFMAX)) vote0++;
+      if (grmhd && ((fabs(pd2[l]) > UPERCDIFFMAX) || (fabs(pd2[l]) < 1.0 / UPERCDIFFMAX))) vote1++;
+      numvotes++;
+    }
+  }
+  int r = 0;
+  if (vote0 > numvotes * 0.5) r = UTOPRIMFAILGAMMAPERC;
+  if (grmhd && (vote1 > numvotes * 0.5)) r = UTOPRIMFAILUPERC;
+  return r;
+}
+
+// One parallel pass of the check: dst = outcome of every flagged zone with
+// the earlier neighbours' outcomes taken from src (first pass: none).  The
+// sequential result is the unique fixed point of this map, so a second pass
+// equal to the first is exact; otherwise cs_changed is set.
+__global__ void k_cs_eval(KParams P, const double* __restrict__ pv, const int* __restrict__ pflag,
+                          const int* __restrict__ bsqf, const double* __restrict__ gammacheck,
+                          const int* __restrict__ src, int* __restrict__ dst, DevScalars* __restrict__ ds,
+                          int first) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (first && i == 0 && j == 0) ds->cs_changed = 0;
+  if (i >= P.n1 || j >= P.n2) return;
+  const int z = zidx(P, i, j);
+  int r = 0;
+  if (bsqf[z] != 0 && pflag[z] == 0) r = cs_vote(P, pv, pflag, gammacheck, first ? nullptr : src, i, j);
+  dst[z] = r;
+  if (!first && r != src[z]) atomicOr(&ds->cs_changed, 1);
+}
+
+// When the two passes differ: the reference's loop itself, in order, on one
+// thread (exact, and rare: it needs a chain of failed neighbours).
+__global__ void k_cs_sequential(KParams P, const double* __restrict__ pv, const int* __restrict__ pflag,
+                                const int* __restrict__ bsqf, const double* __restrict__ gammacheck,
+                                int* __restrict__ dst, const DevScalars* __restrict__ ds, int always) {
+  if (!always && !ds->cs_changed) return;
+  for (int i = 0; i < P.n1; i++)
+    for (int j = 0; j < P.n2; j++) dst[zidx(P, i, j)] = 0;
+  for (int i = 0; i < P.n1; i++)
+    for (int j = 0; j < P.n2; j++) {
+      const int z = zidx(P, i, j);
+      if (bsqf[z] != 0 && pflag[z] == 0) dst[z] = cs_vote(P, pv, pflag, gammacheck, dst, i, j);
+    }
+}
+
+// set the failure flags found by the check and put those zones on the list
+// fixup_utoprim repairs
+__global__ void k_cs_apply(KParams P, const int* __restrict__ cs, int* __restrict__ pflag,
+                           int* __restrict__ faillist, DevScalars* __restrict__ ds) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= P.n1 || j >= P.n2) return;
+  const int z = zidx(P, i, j);
+  if (!cs[z]) return;
+  pflag[z] = cs[z];
+  const int slot = atomicAdd(&ds->nlist, 1);
+  if (slot < P.n1 * P.n2) faillist[slot] = z;
+  atomicAdd(&ds->nfail, 1);
+}
+
+__global__ void k_pflag_expand(int nz, const int* __restrict__ pf, const int* __restrict__ bsqf,
+                               int* __restrict__ out) {
   int z = blockIdx.x * blockDim.x + threadIdx.x;
   if (z >= nz) return;
   out[(size_t)z * 5] = pf[z];
-#pragma unroll
-  for (int k = 1; k < 5; k++) out[(size_t)z * 5 + k] = 0;
+  out[(size_t)z * 5 + 1] = 0;                  // FLAGREALLIM (LIMADJUST=0)
+  out[(size_t)z * 5 + 2] = bsqf[z] & 0xff;     // FLAGBSQORHO
+  out[(size_t)z * 5 + 3] = bsqf[z] >> 8;       // FLAGBSQOU
+  out[(size_t)z * 5 + 4] = 0;                  // FLAGREALFLUX (FLUXADJUST=0)
 }
 
 
@@ -1196,7 +1364,7 @@ __global__ void kb_bx2(const RunDesc* __restrict__ runs, const int4* __restrict_
 // failed-list kernels: FIXBLK blocks per run, block b of run r is blockIdx.x = r*FIXBLK+b
 __global__ void kb_fixlist(const RunDesc* __restrict__ runs, int sel) {
   const RunDesc& r = runs[blockIdx.x / FIXBLK];
-  fixlist_body(r.P, r.arr[sel], r.pflag, r.faillist, r.pfix, r.gC, r.ds, dim3(blockIdx.x % FIXBLK, 0, 0),
+  fixlist_body(r.P, r.arr[sel], r.pflag, nullptr, r.faillist, r.pfix, r.gC, r.ds, dim3(blockIdx.x % FIXBLK, 0, 0),
                dim3(FIXBLK, 1, 1));
 }
 
@@ -1223,6 +1391,10 @@ struct harm_ctx {
   double* pfix;   // repaired states of failed zones
   int* pflag;     // FLAGUTOPRIMFAIL per zone
   int* faillist;  // zones whose inversion failed in the current substage
+  int* bsqf;      // get_bsqflags: b^2/rho | b^2/u << 8
+  double* gammacheck;  // fixup_checksolution's persistent gamma table
+  int* csA;       // fixup_checksolution outcomes, first and second pass
+  int* csB;
   double* gF;
   double* gC;
   double* gConn;
@@ -1270,6 +1442,7 @@ static int check_prm(const harm_params* p) {
       (p->bc[HARM_X2DN] == HARM_PERIODIC) != (p->bc[HARM_X2UP] == HARM_PERIODIC))
     return seterr("periodic boundaries must be set on both sides");
   if (!(p->dx1 > 0) || !(p->dx2 > 0)) return seterr("dx1, dx2 must be positive");
+  if (p->checksolution != 0 && p->checksolution != 1) return seterr("checksolution must be 0 or 1");
   return 0;
 }
 
@@ -1287,7 +1460,7 @@ extern "C" int harm_create(const harm_params* prm, harm_ctx** out) {
   k.n1 = prm->n1; k.n2 = prm->n2; k.N2M = c->N2M;
   k.eomtype = prm->eomtype; k.lim = prm->lim; k.fluxmethod = prm->fluxmethod; k.fluxb = prm->fluxb;
   for (int b = 0; b < 4; b++) k.bc[b] = prm->bc[b];
-  k.fixupflux = prm->fixupflux; k.poledeath = prm->poledeath;
+  k.fixupflux = prm->fixupflux; k.poledeath = prm->poledeath; k.checksolution = prm->checksolution;
   k.gam = prm->gam; k.cour = prm->cour; k.defcon = prm->defcon; k.safe = prm->safe;
   k.gammamax = prm->gammamax; k.gammafail = prm->gammafail;
   k.rhominlimit = prm->rhominlimit; k.uuminlimit = prm->uuminlimit;
@@ -1308,6 +1481,14 @@ extern "C" int harm_create(const harm_params* prm, harm_ctx** out) {
   }
   if (e == hipSuccess) e = hipMalloc(&c->pflag, nz * sizeof(int));
   if (e == hipSuccess) e = hipMemset(c->pflag, 0, nz * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&c->bsqf, nz * sizeof(int));
+  if (e == hipSuccess) e = hipMemset(c->bsqf, 0, nz * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&c->gammacheck, nz * sizeof(double));
+  if (e == hipSuccess) e = hipMemset(c->gammacheck, 0, nz * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&c->csA, nz * sizeof(int));
+  if (e == hipSuccess) e = hipMemset(c->csA, 0, nz * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&c->csB, nz * sizeof(int));
+  if (e == hipSuccess) e = hipMemset(c->csB, 0, nz * sizeof(int));
   if (e == hipSuccess) e = hipMalloc(&c->faillist, nz * sizeof(int));
   if (e == hipSuccess) e = hipMalloc(&c->gF, 2 * nz * GF * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&c->gC, nz * GCN * sizeof(double));
@@ -1346,6 +1527,10 @@ extern "C" int harm_destroy(harm_ctx* c) {
   for (double* b : dbufs)
     if (b) (void)hipFree(b);
   if (c->pflag) (void)hipFree(c->pflag);
+  if (c->bsqf) (void)hipFree(c->bsqf);
+  if (c->gammacheck) (void)hipFree(c->gammacheck);
+  if (c->csA) (void)hipFree(c->csA);
+  if (c->csB) (void)hipFree(c->csB);
   if (c->faillist) (void)hipFree(c->faillist);
   if (c->gF) (void)hipFree(c->gF);
   if (c->gC) (void)hipFree(c->gC);
@@ -1430,7 +1615,8 @@ extern "C" int harm_get_pflag(harm_ctx* c, int* host) {
   if (!c || !host) return seterr("null argument");
   int* tmp;
   HIPCHK(hipMalloc(&tmp, (size_t)c->nz * 5 * sizeof(int)));
-  hipLaunchKernelGGL(k_pflag_expand, dim3((c->nz + 255) / 256), dim3(256), 0, c->stream, c->nz, c->pflag, tmp);
+  hipLaunchKernelGGL(k_pflag_expand, dim3((c->nz + 255) / 256), dim3(256), 0, c->stream, c->nz, c->pflag, c->bsqf,
+                     tmp);
   hipError_t e = hipMemcpyAsync(host, tmp, (size_t)c->nz * 5 * sizeof(int), hipMemcpyDeviceToHost, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(tmp);
@@ -1530,10 +1716,15 @@ static int launch_update(harm_ctx* c, int pi, int pb, int pf, const RKCoef& C) {
   return prof_end(c);
 }
 
+static void launch_bsqflags(harm_ctx* c, int sel) {
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL(k_bsqflags, grid2(c->kp.n2, c->kp.n1, b), b, 0, c->stream, c->kp, c->arr[sel], c->gC, c->bsqf);
+}
+
 static int launch_boundary(harm_ctx* c, int sel, int mode) {
   if (c->prm.eomtype != HARM_EOMGRMHD) mode &= ~KB_FIXUP;
   if (prof_begin(c, HARM_K_BOUNDARY)) return 1;
-  if (c->bound_variant == HARM_BOUND_BLOCK) {
+  if (c->bound_variant == HARM_BOUND_BLOCK && !c->kp.checksolution) {
     hipLaunchKernelGGL(k_boundary, dim3(1), dim3(1024), 0, c->stream, c->kp, c->arr[sel], c->gC, c->pflag,
                        c->faillist, c->pfix, c->ds, mode);
   } else {
@@ -1544,9 +1735,22 @@ static int launch_boundary(harm_ctx* c, int sel, int mode) {
       hipLaunchKernelGGL(k_bx2, gx2, dim3(64), 0, c->stream, c->kp, c->arr[sel], c->ds,
                          (!fix && !bound2 && (mode & KB_DT)) ? 1 : 0);
     }
+    if (fix && c->kp.checksolution) {  // fixup_checksolution (fixup.c:388-488)
+      const dim3 b(64, 4);
+      const dim3 gi = grid2(c->kp.n2, c->kp.n1, b);
+      hipLaunchKernelGGL(k_gammacheck, grid2(c->kp.n2 + 2, c->kp.n1 + 2, b), b, 0, c->stream, c->kp, c->arr[sel],
+                         c->gC, c->pflag, c->bsqf, c->gammacheck);
+      hipLaunchKernelGGL(k_cs_eval, gi, b, 0, c->stream, c->kp, c->arr[sel], c->pflag, c->bsqf, c->gammacheck,
+                         c->csB, c->csA, c->ds, 1);
+      hipLaunchKernelGGL(k_cs_eval, gi, b, 0, c->stream, c->kp, c->arr[sel], c->pflag, c->bsqf, c->gammacheck,
+                         c->csA, c->csB, c->ds, 0);
+      hipLaunchKernelGGL(k_cs_sequential, dim3(1), dim3(1), 0, c->stream, c->kp, c->arr[sel], c->pflag, c->bsqf,
+                         c->gammacheck, c->csB, c->ds, c->bound_variant == HARM_BOUND_KERNELS_SEQCHECK ? 1 : 0);
+      hipLaunchKernelGGL(k_cs_apply, gi, b, 0, c->stream, c->kp, c->csB, c->pflag, c->faillist, c->ds);
+    }
     if (fix) {
-      hipLaunchKernelGGL(k_fixlist, dim3(32), dim3(64), 0, c->stream, c->kp, c->arr[sel], c->pflag, c->faillist,
-                         c->pfix, c->gC, c->ds);
+      hipLaunchKernelGGL(k_fixlist, dim3(32), dim3(64), 0, c->stream, c->kp, c->arr[sel], c->pflag,
+                         c->kp.checksolution ? c->csB : nullptr, c->faillist, c->pfix, c->gC, c->ds);
       hipLaunchKernelGGL(k_fixapply, dim3(32), dim3(64), 0, c->stream, c->kp, c->arr[sel], c->pflag, c->faillist,
                          c->pfix, c->ds);
     }
@@ -1616,6 +1820,9 @@ static int enqueue_step(harm_ctx* c) {
   int pii[4], pbb[4], pff[4], nto;
   rk_setup(c->prm.timeorder, &nto, CUf, Cunew, pii, pbb, pff);
   for (int to = 0; to < nto; to++) {
+    // pre_fixup (step_ch.c:92) flags every substage from pii = p, which no
+    // substage changes before its own update: once per step is the same
+    if (to == 0 && c->kp.checksolution) launch_bsqflags(c, pii[to]);
     if (substage(c, pii[to], pbb[to], pff[to], CUf[to], Cunew[to], to, nto)) return 1;
     // step_ch.c:104-124 bound_prim, post_fixup, bound_prim; then :174-183 after the last substage
     int mode = KB_BOUND | KB_FIXUP | KB_BOUND2 | ((to == nto - 1) ? KB_DT : 0);
@@ -1686,6 +1893,15 @@ extern "C" int harm_bound_prim(harm_ctx* c, int sel) {
   if (need_geom(c)) return 1;
   if (sel < 0 || sel > HARM_PK1) return seterr("sel must be HARM_P/PK0/PK1");
   if (launch_boundary(c, sel, KB_BOUND)) return 1;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int harm_pre_fixup(harm_ctx* c, int sel) {
+  if (need_geom(c)) return 1;
+  if (sel < 0 || sel > HARM_PK1) return seterr("sel must be HARM_P/PK0/PK1");
+  launch_bsqflags(c, sel);
+  HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }
@@ -1773,8 +1989,9 @@ static int autotune(harm_ctx* c) {
   int bf = HARM_FLUX_FACE_64x4, bb = HARM_BOUND_KERNELS;
   const int fv[3] = {HARM_FLUX_FACE_64x4, HARM_FLUX_FACE_64x2, HARM_FLUX_PAIR};
   const int bv[2] = {HARM_BOUND_KERNELS, HARM_BOUND_BLOCK};
+  const int nbv = c->kp.checksolution ? 1 : 2;  // the one-workgroup pass has no fixup_checksolution
   for (int f = 0; f < 3 && !rc; f++)
-    for (int v = 0; v < 2 && !rc; v++) {
+    for (int v = 0; v < nbv && !rc; v++) {
       float ms = 0.f;
       c->flux_variant = fv[f];
       c->bound_variant = bv[v];
@@ -1891,7 +2108,8 @@ extern "C" int harm_prof_reset(harm_ctx* c) {
 extern "C" int harm_set_tuning(harm_ctx* c, int flux_variant, int bound_variant) {
   if (!c) return seterr("null ctx");
   if (flux_variant < 0 || flux_variant > HARM_FLUX_PAIR) return seterr("bad flux variant %d", flux_variant);
-  if (bound_variant < 0 || bound_variant > HARM_BOUND_BLOCK) return seterr("bad boundary variant %d", bound_variant);
+  if (bound_variant < 0 || bound_variant > HARM_BOUND_KERNELS_SEQCHECK)
+    return seterr("bad boundary variant %d", bound_variant);
   c->flux_variant = flux_variant;
   c->bound_variant = bound_variant;
   c->tuned = true;
@@ -2020,6 +2238,7 @@ extern "C" int harm_batch_create(harm_ctx* const* ctxs, int n, harm_batch** out)
     if (!ctxs[r]->geometry_set) return seterr("context %d: geometry not set", r);
     if (ctxs[r]->prm.timeorder != ctxs[0]->prm.timeorder)
       return seterr("context %d: every run of a batch must use the same TIMEORDER", r);
+    if (ctxs[r]->prm.checksolution) return seterr("context %d: CHECKSOLUTION is not batched", r);
   }
   HIPCHK(hipDeviceSynchronize());
   harm_batch* b = new harm_batch();

@@ -91,6 +91,8 @@ typedef struct {
   int bc[4];           /* BCtype[X1UP..X2DN] */
   int fixupflux;       /* FIXUPFLUX (step_ch.h:82): fix_flux() on/off */
   int poledeath;       /* POLEDEATH (bounds.fishmon.c:328) */
+  int checksolution;   /* CHECKSOLUTION (global.h:259): get_bsqflags before each
+                          substage, fixup_checksolution in post_fixup */
   double gam;          /* adiabatic index */
   double cour;         /* Courant factor */
   double safe;         /* SAFE: max dt growth per step */
@@ -137,8 +139,13 @@ int harm_get_time(harm_ctx *ctx, double *t, double *dt);
 int harm_fluxcalc(harm_ctx *ctx, int pr_sel, int dir, double *ndt);
 /* fixup.c:1357 fix_flux(pb, F1, F2) on the context fluxes. */
 int harm_fix_flux(harm_ctx *ctx);
-/* fixup.c:37 post_fixup(stage, pv, finalstep): bound_pflag + fixup_utoprim
-   (repair of zones whose inversion failed) on array sel. */
+/* fixup.c:25 pre_fixup(stage, pv): get_bsqflags (b^2/rho, b^2/u flags
+   read by fixup_checksolution) from array sel; a no-op in effect unless
+   checksolution is set. */
+int harm_pre_fixup(harm_ctx *ctx, int sel);
+/* fixup.c:37 post_fixup(stage, pv, finalstep): bound_pflag,
+   fixup_checksolution (when checksolution is set) and fixup_utoprim (repair
+   of zones whose inversion failed or that the check failed) on array sel. */
 int harm_post_fixup(harm_ctx *ctx, int sel);
 /* step_ch.c:1159 flux_ct(stage, pb, F1, F2): constrained-transport EMFs
    rewritten into F1/F2. */
@@ -190,7 +197,10 @@ int harm_prof_enable(harm_ctx *ctx, int on);
 #define HARM_FLUX_FACE_64x2 1  /* one thread per face, 64x2 workgroups */
 #define HARM_FLUX_PAIR 2       /* a lane pair per face (left/right state) */
 #define HARM_BOUND_KERNELS 0   /* boundary work as six wide launches */
-#define HARM_BOUND_BLOCK 1     /* boundary work in one workgroup with barriers */
+#define HARM_BOUND_BLOCK 1     /* boundary work in one workgroup with barriers
+                                  (not with checksolution: wide launches) */
+#define HARM_BOUND_KERNELS_SEQCHECK 2  /* wide launches, fixup_checksolution
+                                          always by its sequential loop */
 int harm_set_tuning(harm_ctx *ctx, int flux_variant, int bound_variant);
 int harm_get_tuning(harm_ctx *ctx, int *flux_variant, int *bound_variant);
 int harm_prof_get(harm_ctx *ctx, int kernel, double *total_ms, long long *count);

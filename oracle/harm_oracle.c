@@ -12,7 +12,8 @@
  *   MAXWELL=PRIMMAXWELL, TRUEFAST=0, HYPERHLL=0, BDIRCONT=1, EVOLVECHECKS=1,
  *   RESCALEINTERP=0, LIMADJUST=FLUXADJUST=0, DOENO=0, CORNGDETVERSION=1,
  *   FIXUPZONES=FIXUP1ZONE, UTOPRIMVERSION=UTOPRIM2DFINAL,
- *   UTOPRIMTRYAGAIN=0, CHECKSOLUTION=0, rescaletype=1, EXTRAP=0.
+ *   UTOPRIMTRYAGAIN=0, rescaletype=1, EXTRAP=0; CHECKSOLUTION is a run-time
+ *   parameter (oracle_params.checksolution).
  */
 #include "harm_oracle.h"
 #include <math.h>
@@ -64,6 +65,7 @@ struct oracle_ctx {
   double *gcov, *gcon, *gdet, *conn, *floors;
   double *arr[12];
   int *pflag;
+  double *gammacheck; /* fixup_checksolution's static gammacheck[N1+2][N2+2] */
   double t, dt, tf;
   long nstroke;
   int failed;
@@ -926,32 +928,121 @@ int or_bound_prim(oracle_ctx *c, int sel) {
 /* boundsint.c:4-60 bound_pflag (FLAGUTOPRIMFAIL only): x1 outflow copies over
    j in [-3,N2+2], then polar reflection over the full i range; periodic
    wraps x1 (interior j) then x2 (all i), as the primitives do. */
+/* boundsint.c:4-60 bound_pflag: every pflag slot (FLOOP) of the ghost zones */
 static void bound_pflag(oracle_ctx *c) {
   int n1 = c->p.n1, n2 = c->p.n2;
-#define PF(i, j) c->pflag[(size_t)zi(c, (i), (j)) * NUMPFLAGS]
+#define PFC(i, j, i2, j2) memcpy(&c->pflag[(size_t)zi(c, (i), (j)) * NUMPFLAGS], \
+                                 &c->pflag[(size_t)zi(c, (i2), (j2)) * NUMPFLAGS], sizeof(int) * NUMPFLAGS)
   if (c->p.bc[OR_X1DN] == OR_OUTFLOW)
     for (int j = -NG; j <= n2 - 1 + NG; j++)
-      for (int i = -1; i >= -NG; i--) PF(i, j) = PF(0, j);
+      for (int i = -1; i >= -NG; i--) PFC(i, j, 0, j);
   if (c->p.bc[OR_X1UP] == OR_OUTFLOW)
     for (int j = -NG; j <= n2 - 1 + NG; j++)
-      for (int i = n1; i <= n1 - 1 + NG; i++) PF(i, j) = PF(n1 - 1, j);
+      for (int i = n1; i <= n1 - 1 + NG; i++) PFC(i, j, n1 - 1, j);
   if (c->p.bc[OR_X1DN] == OR_PERIODIC)
     for (int j = 0; j < n2; j++) {
-      for (int i = -NG; i < 0; i++) PF(i, j) = PF(i + n1, j);
-      for (int i = n1; i < n1 + NG; i++) PF(i, j) = PF(i - n1, j);
+      for (int i = -NG; i < 0; i++) PFC(i, j, i + n1, j);
+      for (int i = n1; i < n1 + NG; i++) PFC(i, j, i - n1, j);
     }
   if (c->p.bc[OR_X2DN] == OR_POLARAXIS)
     for (int i = -NG; i <= n1 - 1 + NG; i++)
-      for (int j = -NG; j <= -1; j++) PF(i, j) = PF(i, 0 + (0 - j - 1));
+      for (int j = -NG; j <= -1; j++) PFC(i, j, i, 0 + (0 - j - 1));
   if (c->p.bc[OR_X2UP] == OR_POLARAXIS)
     for (int i = -NG; i <= n1 - 1 + NG; i++)
-      for (int j = n2; j <= n2 - 1 + NG; j++) PF(i, j) = PF(i, (n2 - 1) + ((n2 - 1) - j + 1));
+      for (int j = n2; j <= n2 - 1 + NG; j++) PFC(i, j, i, (n2 - 1) + ((n2 - 1) - j + 1));
   if (c->p.bc[OR_X2DN] == OR_PERIODIC)
     for (int i = -NG; i < n1 + NG; i++) {
-      for (int j = -NG; j < 0; j++) PF(i, j) = PF(i, j + n2);
-      for (int j = n2; j < n2 + NG; j++) PF(i, j) = PF(i, j - n2);
+      for (int j = -NG; j < 0; j++) PFC(i, j, i, j + n2);
+      for (int j = n2; j < n2 + NG; j++) PFC(i, j, i, j - n2);
     }
-#undef PF
+#undef PFC
+}
+
+/* initbase.c:213-214 */
+#define BSQORHOLIMIT 20.0
+#define BSQOULIMIT 100.0
+/* global.h:264-265, 558-559; fixup.c:380 */
+#define GAMMAPERCDIFFMAX 2.0
+#define UPERCDIFFMAX 10.0
+#define UTOPRIMFAILGAMMAPERC 8
+#define UTOPRIMFAILUPERC 9
+#define MAXVOTES 8
+#define FLAGBSQORHO 2
+#define FLAGBSQOU 3
+
+/* fixup.c:1197-1295 get_bsqflags (LIMADJUST=FLUXADJUST=0): b^2/rho and
+   b^2/u flags over DQZLOOP (i in [-1,N1], j in [-1,N2]) */
+static void get_bsqflags(oracle_ctx *c, const double *pv) {
+  if (!c->p.checksolution) return;
+  for (int i = -1; i <= c->p.n1; i++)
+    for (int j = -1; j <= c->p.n2; j++) {
+      int *pf = &c->pflag[(size_t)zi(c, i, j) * NUMPFLAGS];
+      int f0 = 0, f3 = 0;
+      if (c->p.eomtype == OR_GRMHD) {
+        const double *pr = PA(c, pv, i, j);
+        geom_t g;
+        state_t q;
+        get_geometry(c, i, j, CENT, &g);
+        get_state(pr, &g, &q);
+        double bsq = DOT(q.bcon, q.bcov);
+        pf[FLAGBSQORHO] = 0;
+        if (bsq / pr[RHO] > BSQORHOLIMIT) f0 = 2;
+        else if (bsq / pr[RHO] > 0.9 * BSQORHOLIMIT) f0 = 1;
+        else f0 = 0;
+        if (bsq / pr[UU] > BSQOULIMIT) f3 = 2;
+        else if (bsq / pr[UU] > 0.9 * BSQOULIMIT) f3 = 1;
+        else f3 = 0;
+      }
+      pf[FLAGBSQORHO] = f0;
+      pf[FLAGBSQOU] = f3;
+    }
+}
+
+/* fixup.c:388-488 fixup_checksolution: zones with a b^2 flag whose inversion
+   succeeded are failed (UTOPRIMFAILGAMMAPERC / UTOPRIMFAILUPERC) when most of
+   their neighbours' gamma or u differ by large factors.  gammacheck persists
+   across calls (a function-static array in the reference) and is refreshed
+   only at flagged zones; the interior loop runs in (i, j) order and sees the
+   flags it has already changed. */
+static void fixup_checksolution(oracle_ctx *c, const double *pv) {
+  int n1 = c->p.n1, n2 = c->p.n2;
+#define PFS(i, j, s) c->pflag[(size_t)zi(c, (i), (j)) * NUMPFLAGS + (s)]
+#define GC(i, j) c->gammacheck[(size_t)((i) + 1) * (n2 + 2) + (j) + 1]
+#define FLAGGED(i, j) ((PFS(i, j, FLAGBSQORHO) || PFS(i, j, FLAGBSQOU)) && (PFS(i, j, 0) == UTOPRIMNOFAIL))
+  for (int i = -1; i <= n1; i++)
+    for (int j = -1; j <= n2; j++)
+      if (FLAGGED(i, j)) {
+        geom_t g;
+        get_geometry(c, i, j, CENT, &g);
+        GC(i, j) = gamma_calc(PA(c, pv, i, j), &g);
+      }
+  for (int i = 0; i < n1; i++)
+    for (int j = 0; j < n2; j++) {
+      if (!FLAGGED(i, j)) continue;
+      const int di[MAXVOTES] = {0, 0, 1, -1, 1, 1, 1, -1}, dj[MAXVOTES] = {1, -1, 0, 0, 1, -1, 1, -1};
+      double percdiff[MAXVOTES], percdiff2[MAXVOTES];
+      for (int l = 0; l < MAXVOTES; l++) {
+        int ok = (PFS(i + di[l], j + dj[l], 0) == UTOPRIMNOFAIL);
+        percdiff[l] = ok ? (GC(i + di[l], j + dj[l]) / GC(i, j)) : -1;
+        if (c->p.eomtype == OR_GRMHD)
+          percdiff2[l] = ok ? (PA(c, pv, i + di[l], j + dj[l])[UU] / PA(c, pv, i, j)[UU]) : -1;
+      }
+      int vote0 = 0, vote1 = 0, numvotes = 0;
+      for (int l = 0; l < MAXVOTES; l++) {
+        if (percdiff[0] >= 0.0) {
+          if ((fabs(percdiff[l]) > GAMMAPERCDIFFMAX) || (fabs(percdiff[l]) < 1.0 / GAMMAPERCDIFFMAX)) vote0++;
+          if ((c->p.eomtype == OR_GRMHD) &&
+              ((fabs(percdiff2[l]) > UPERCDIFFMAX) || (fabs(percdiff2[l]) < 1.0 / UPERCDIFFMAX)))
+            vote1++;
+          numvotes++;
+        }
+      }
+      if (vote0 > numvotes * 0.5) PFS(i, j, 0) = UTOPRIMFAILGAMMAPERC;
+      if ((c->p.eomtype == OR_GRMHD) && (vote1 > numvotes * 0.5)) PFS(i, j, 0) = UTOPRIMFAILUPERC;
+    }
+#undef FLAGGED
+#undef GC
+#undef PFS
 }
 
 /* fixup.c:527-920 fixup_utoprim (UTOPRIMADJUST=UTOPRIMAVG, ORDERINDEPENDENT=1,
@@ -1026,8 +1117,11 @@ static int fixup_utoprim(oracle_ctx *c, double *pv) {
 
 /* fixup.c:37-94 post_fixup: bound_pflag then fixup_utoprim (GRMHD);
    returns the number of zones repaired */
+/* fixup.c:37-94 post_fixup: bound_pflag, fixup_checksolution (CHECKSOLUTION),
+   fixup_utoprim (GRMHD) */
 int or_post_fixup(oracle_ctx *c, int sel) {
   bound_pflag(c);
+  if (c->p.checksolution) fixup_checksolution(c, c->arr[sel]);
   if (c->p.eomtype != OR_GRMHD) return 0;
   return fixup_utoprim(c, c->arr[sel]);
 }
@@ -1063,6 +1157,7 @@ int or_step(oracle_ctx *c) {
     nto = 1;
   }
   for (int to = 0; to < nto; to++) {
+    get_bsqflags(c, c->arr[pii[to]]); /* pre_fixup (step_ch.c:92, fixup.c:25-33) */
     fail |= or_advance(c, pii[to], pbb[to], CUf[to], pff[to], Cunew[to], to, nto, &ndt);
     fail |= or_bound_prim(c, pff[to]);
     or_post_fixup(c, pff[to]);
@@ -1136,6 +1231,7 @@ oracle_ctx *or_create(const oracle_params *prm) {
   c->floors = (double *)calloc((size_t)c->nz * 2, sizeof(double));
   for (int a = 0; a < 12; a++) c->arr[a] = (double *)calloc((size_t)c->nz * NPR, sizeof(double));
   c->pflag = (int *)calloc((size_t)c->nz * NUMPFLAGS, sizeof(int));
+  c->gammacheck = (double *)calloc((size_t)(prm->n1 + 2) * (prm->n2 + 2), sizeof(double));
   c->tf = 1e30;
 #ifdef _OPENMP
   if (prm->nthreads > 0) omp_set_num_threads(prm->nthreads);
@@ -1145,7 +1241,7 @@ oracle_ctx *or_create(const oracle_params *prm) {
 
 void or_destroy(oracle_ctx *c) {
   if (!c) return;
-  free(c->gcov); free(c->gcon); free(c->gdet); free(c->conn); free(c->floors);
+  free(c->gcov); free(c->gcon); free(c->gdet); free(c->conn); free(c->floors); free(c->gammacheck);
   for (int a = 0; a < 12; a++) free(c->arr[a]);
   free(c->pflag);
   free(c);

@@ -37,6 +37,7 @@ typedef struct {
   int bc[4];
   int fixupflux;        /* FIXUPFLUX (step_ch.h:82) */
   int poledeath;        /* POLEDEATH (bounds.fishmon.c:328) */
+  int checksolution;    /* CHECKSOLUTION (global.h:259) */
   double gam, cour, safe, defcon, gammamax, gammafail;
   double rhominlimit, uuminlimit;
   double dx1, dx2;

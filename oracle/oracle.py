@@ -20,7 +20,7 @@ class OracleParams(ctypes.Structure):
         ("n1", ctypes.c_int), ("n2", ctypes.c_int),
         ("eomtype", ctypes.c_int), ("lim", ctypes.c_int), ("fluxmethod", ctypes.c_int), ("fluxb", ctypes.c_int),
         ("timeorder", ctypes.c_int), ("bc", ctypes.c_int * 4), ("fixupflux", ctypes.c_int),
-        ("poledeath", ctypes.c_int),
+        ("poledeath", ctypes.c_int), ("checksolution", ctypes.c_int),
         ("gam", ctypes.c_double), ("cour", ctypes.c_double), ("safe", ctypes.c_double), ("defcon", ctypes.c_double),
         ("gammamax", ctypes.c_double), ("gammafail", ctypes.c_double), ("rhominlimit", ctypes.c_double),
         ("uuminlimit", ctypes.c_double), ("dx1", ctypes.c_double), ("dx2", ctypes.c_double),

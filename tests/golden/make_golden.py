@@ -43,6 +43,9 @@ CASES = {
     "ot_nlimcent_hll_rk3": ("mink", 32, 32, H.NLIMCENT, H.HLLFLUX, 3, 5, 1e-4, (5,)),
     "ffde_monopole_mc_laxf_rk2": ("ffde", 32, 64, H.MC, H.LAXFFLUX, 2, 200, 1e-3, (1, 100, 200)),
     "ffde_monopole_para_hll_rk4": ("ffde", 32, 64, H.PARA, H.HLLFLUX, 4, 40, 1e-3, (40,)),
+    # torus threaded by a weak split-monopole field, CHECKSOLUTION 1: b^2 flags
+    # in the magnetised atmosphere, zones failed by fixup_checksolution
+    "torus_field_checksolution_rk4": ("ks_mono", 32, 64, H.PARA, H.HLLFLUX, 4, 40, 1e-3, (1, 10, 40)),
 }
 
 
@@ -52,10 +55,12 @@ def sha(a):
 
 def problem(kind, n1, n2):
     """Input state, geometry and the parameter dict shared by reference, oracle, GPU."""
-    if kind == "ks":
+    if kind in ("ks", "ks_mono"):
         g = H.grid_mks(n1, n2)
         geom = H.Geometry(g)
         p0 = H.init_fishmon(geom)
+        if kind == "ks_mono":
+            p0[..., 5] += H.init_monopole(geom, 1e-2)[..., 5]
         gam, cour, bc, fix, pole = 4.0 / 3.0, 0.9, (H.OUTFLOW, H.OUTFLOW, H.POLARAXIS, H.POLARAXIS), 1, 1
     elif kind == "ffde":
         g = H.grid_mks(n1, n2, a=0.9, Rout=100.0, hslope=0.1)
@@ -74,7 +79,7 @@ def run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work):
     p0.tofile(os.path.join(work, "p0.bin"))
     cfg = dict(prim=os.path.join(work, "p0.bin"), dt=dt0, nsteps=nsteps, out=os.path.join(work, "o.bin"),
                lim=lim, fluxmethod=flux, timeorder=to)
-    if kind in ("ks", "ffde"):
+    if kind in ("ks", "ks_mono", "ffde"):
         cfg.update(a=g.a, Rout=g.Rout, hslope=g.hslope, R0=g.R0, geom="ref", gam=4.0 / 3.0, cour=0.9,
                    tables_out=os.path.join(work, "tables.bin"))
     else:
@@ -89,7 +94,8 @@ def run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work):
     with open(path, "w") as f:
         for k, v in cfg.items():
             f.write(f"{k} {v!r}\n" if isinstance(v, float) else f"{k} {v}\n")
-    subprocess.run([os.path.join(REF_DIR, "ref_" + kind), path], cwd=work, check=True, capture_output=True)
+    binary = {"ks_mono": "ref_kscs"}.get(kind, "ref_" + kind)
+    subprocess.run([os.path.join(REF_DIR, binary), path], cwd=work, check=True, capture_output=True)
     d = np.fromfile(os.path.join(work, "o.bin"), dtype=np.float64)[4:]
     nz = (g.n1 + 6) * (g.n2 + 6) * 8
     recs, states = [], []

@@ -15,13 +15,18 @@ def sha(a):
 
 
 def problem(kind, n1, n2, lim=H.PARA, fluxmethod=H.HLLFLUX, timeorder=4, seed=7):
-    """(grid, geometry, initial state, harm_params) for a torus ('ks'), an
-    Orszag-Tang ('mink') or a force-free split-monopole ('ffde') problem."""
-    if kind == "ks":
+    """(grid, geometry, initial state, harm_params) for a torus ('ks'), a
+    torus threaded by a weak split-monopole field with CHECKSOLUTION on
+    ('ks_mono'), an Orszag-Tang ('mink') or a force-free split-monopole
+    ('ffde') problem."""
+    if kind in ("ks", "ks_mono"):
         g = H.grid_mks(n1, n2)
         geom = H.Geometry(g)
         p0 = H.init_fishmon(geom, seed=seed)
-        prm = H.make_params(n1, n2, g.dx1, g.dx2, lim=lim, fluxmethod=fluxmethod, timeorder=timeorder)
+        if kind == "ks_mono":   # torus threaded by a weak split-monopole field, CHECKSOLUTION on
+            p0[..., 5] += H.init_monopole(geom, 1e-2)[..., 5]
+        prm = H.make_params(n1, n2, g.dx1, g.dx2, lim=lim, fluxmethod=fluxmethod, timeorder=timeorder,
+                            checksolution=1 if kind == "ks_mono" else 0)
     elif kind == "ffde":
         g = H.grid_mks(n1, n2, a=0.9, Rout=100.0, hslope=0.1)
         geom = H.Geometry(g)

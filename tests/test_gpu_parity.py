@@ -242,3 +242,38 @@ def test_gpu_autotune_picks_a_variant():
         o.step()
     assert same(h.get_array(H.P), o.get_array(0))
     assert h.get_time() == o.get_time()
+
+
+@pytest.mark.parametrize("fv", [H.FLUX_FACE_64x4, H.FLUX_PAIR])
+@pytest.mark.parametrize("bv", [H.BOUND_KERNELS, H.BOUND_KERNELS_SEQCHECK])
+def test_gpu_checksolution_matches_oracle(fv, bv):
+    say(f"[case] CHECKSOLUTION: field-threaded torus 32x64, 25 steps, flux={fv} bound={bv}")
+    g, geom, p0, prm = problem("ks_mono", 32, 64)
+    assert prm.checksolution == 1
+    h = gpu_ctx(geom, prm, p0, 1e-3)
+    h.set_tuning(fv, bv)
+    o = oracle_ctx(geom, prm, p0, 1e-3)
+    for s in range(25):
+        h.step_ch()
+        o.step()
+        assert h.get_time() == o.get_time(), f"step {s + 1}"
+    assert same(h.get_array(H.P), o.get_array(0))
+    assert h.nstroke() == o.nstroke()
+    pf_g = h.get_pflag()[3:-3, 3:-3]
+    pf_o = o.pflag()[3:-3, 3:-3]
+    assert np.array_equal(pf_g[..., 0], pf_o[..., 0]), "FLAGUTOPRIMFAIL"
+    assert np.array_equal(pf_g[..., 2:4], pf_o[..., 2:4]), "b^2 flags"
+    assert (pf_g[..., 2] > 0).sum() > 100, "the problem is meant to flag the magnetised atmosphere"
+
+
+def test_gpu_checksolution_autotuned_graph():
+    say("[case] CHECKSOLUTION with the autotuner and graph replay, 40 steps")
+    g, geom, p0, prm = problem("ks_mono", 32, 64)
+    h = gpu_ctx(geom, prm, p0, 1e-3)
+    o = oracle_ctx(geom, prm, p0, 1e-3)
+    h.step_n(40)
+    for _ in range(40):
+        o.step()
+    assert h.get_tuning()[1] == H.BOUND_KERNELS
+    assert same(h.get_array(H.P), o.get_array(0))
+    assert h.get_time() == o.get_time()
