@@ -42,6 +42,7 @@ class HarmParams(ctypes.Structure):
         ("fluxb", ctypes.c_int), ("timeorder", ctypes.c_int),
         ("bc", ctypes.c_int * 4),
         ("fixupflux", ctypes.c_int), ("poledeath", ctypes.c_int), ("checksolution", ctypes.c_int),
+        ("eno", ctypes.c_int),
         ("gam", ctypes.c_double), ("cour", ctypes.c_double), ("safe", ctypes.c_double),
         ("defcon", ctypes.c_double), ("gammamax", ctypes.c_double), ("gammafail", ctypes.c_double),
         ("rhominlimit", ctypes.c_double), ("uuminlimit", ctypes.c_double),
@@ -231,7 +232,7 @@ def init_monopole(geom: Geometry, B0=1.0):
 
 
 def make_params(n1, n2, dx1, dx2, *, lim=PARA, fluxmethod=HLLFLUX, timeorder=4, bc=None, fixupflux=1,
-                poledeath=1, checksolution=0, gam=4.0 / 3.0, cour=0.9, safe=1.3, defcon=1.0, gammamax=25.0,
+                poledeath=1, checksolution=0, eno=0, gam=4.0 / 3.0, cour=0.9, safe=1.3, defcon=1.0, gammamax=25.0,
                 gammafail=None, rhominlimit=1e-20, uuminlimit=1e-20, fluxb=FLUXCTTOTH, eomtype=EOMGRMHD):
     """Parameter block with the reference's GRMHD defaults (initbase.c:119-243)."""
     prm = HarmParams()
@@ -240,7 +241,7 @@ def make_params(n1, n2, dx1, dx2, *, lim=PARA, fluxmethod=HLLFLUX, timeorder=4, 
     b = bc if bc is not None else (OUTFLOW, OUTFLOW, POLARAXIS, POLARAXIS)
     for k in range(4):
         prm.bc[k] = b[k]
-    prm.fixupflux, prm.poledeath, prm.checksolution = fixupflux, poledeath, checksolution
+    prm.fixupflux, prm.poledeath, prm.checksolution, prm.eno = fixupflux, poledeath, checksolution, eno
     prm.gam, prm.cour, prm.safe, prm.defcon = gam, cour, safe, defcon
     prm.gammamax = gammamax
     prm.gammafail = 100.0 * gammamax if gammafail is None else gammafail

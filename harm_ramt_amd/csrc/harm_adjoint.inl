@@ -340,6 +340,7 @@ static int adj_check(const harm_ctx* c) {
   for (int k = 0; k < 4; k++)
     if (p.bc[k] != HARM_PERIODIC) return seterr("adjoint: only PERIODIC boundaries");
   if (p.fixupflux) return seterr("adjoint: FIXUPFLUX must be off");
+  if (p.eno) return seterr("adjoint: DOENO has no adjoint");
   if (!(p.lim == HARM_DONOR || p.lim == HARM_VANL || p.lim == HARM_MINM || p.lim == HARM_MC ||
         p.lim == HARM_NLIMCENT || p.lim == HARM_NLIMUP || p.lim == HARM_NLIMDOWN))
     return seterr("adjoint: limiter %d has no adjoint (DONOR, VANL, MINM, MC, NLIM*)", p.lim);

@@ -30,7 +30,7 @@ struct KParams {
   int n1, n2, N2M;
   int eomtype, lim, fluxmethod, fluxb;
   int bc[4];
-  int fixupflux, poledeath, checksolution;
+  int fixupflux, poledeath, checksolution, eno;
   double gam, cour, defcon, safe, gammamax, gammafail, rhominlimit, uuminlimit;
   double dx1, dx2;
 };

@@ -31,6 +31,7 @@
 #include <mutex>
 #include "harm_device.h"
 #include "harm_adjoint.h"
+#include "harm_eno.h"
 #include "../../include/harm_amd.h"
 
 using namespace harm;
@@ -662,8 +663,130 @@ __device__ int utoprimgen(const KParams& P, const double* U, const double* gc, d
 }
 
 // ---------------------------------------------------------------------------
-// advance_standard() per-zone loop (advance.c:114-181), zones i<N1, j<N2.
+// advance() per zone: advance_standard (advance.c:114-181) and the two
+// DOENO forms (advance.c:101-158 and 298-487), zones i<N1, j<N2.
 // ---------------------------------------------------------------------------
+
+// full CENT geometry record of zone z
+__device__ __forceinline__ void load_cent(const double* __restrict__ gC, int z, double* gc) {
+  const double2* a = reinterpret_cast<const double2*>(gC + (size_t)z * GCN);
+#pragma unroll
+  for (int m = 0; m < 18; m++) {
+    double2 t = a[m];
+    gc[2 * m] = t.x;
+    gc[2 * m + 1] = t.y;
+  }
+}
+
+// flux2dUavg (advance.c:204-265): each direction's flux difference, with the
+// flux-CT B fluxes formed from the corner EMFs (step_ch.c:1223-1234, 1405-1412)
+__device__ __forceinline__ void flux_div(const KParams& P, const double* __restrict__ F1,
+                                         const double* __restrict__ F2, const double* __restrict__ q1,
+                                         const double* __restrict__ q2, const double* __restrict__ gC, int i, int j,
+                                         double* d1, double* d2) {
+  const int z = zidx(P, i, j);
+  double f1a[NPR], f1b[NPR], f2a[NPR], f2b[NPR];
+  load8(F1, z, f1a);
+  load8(F1, zidx(P, i + 1, j), f1b);
+  load8(F2, z, f2a);
+  load8(F2, zidx(P, i, j + 1), f2b);
+  if (P.fluxb == HARM_FLUXCTTOTH) {
+    double e00 = emf_at(P, q1, q2, gC, i, j), e01 = emf_at(P, q1, q2, gC, i, j + 1),
+           e10 = emf_at(P, q1, q2, gC, i + 1, j), e11 = emf_at(P, q1, q2, gC, i + 1, j + 1);
+    f1a[B1] = 0.;
+    f1a[B2] = 0.5 * (e00 + e01);
+    f1b[B1] = 0.;
+    f1b[B2] = 0.5 * (e10 + e11);
+    f2a[B1] = -0.5 * (e00 + e10);
+    f2a[B2] = 0.;
+    f2b[B1] = -0.5 * (e01 + e11);
+    f2b[B2] = 0.;
+  }
+  const double idx1 = 1.0 / P.dx1, idx2 = 1.0 / P.dx2;
+#pragma unroll
+  for (int k = 0; k < NPR; k++) {
+    d1[k] = (-(f1b[k] - f1a[k]) * idx1);
+    d2[k] = (-(f2b[k] - f2a[k]) * idx2);
+  }
+}
+
+// cvtij (reconstructeno.c:133-235) of zone (i,j) along DIR for every
+// component of the AoS array u; EOMFFDE converts RHO..U3 and keeps B
+template <int TYPE, int DIR>
+__device__ __forceinline__ void eno_cvt_zone(const KParams& P, const double* __restrict__ u, int i, int j,
+                                             double* out) {
+  const int pos = (DIR == 1) ? i : j, n = (DIR == 1) ? P.n1 : P.n2;
+  double v[NPR][9];
+#pragma unroll
+  for (int o = 0; o < 9; o++) {
+    const int q = pos - 4 + o;
+    double t[NPR];
+    if (q >= 0 && q < n) {
+      load8(u, (DIR == 1) ? zidx(P, q, j) : zidx(P, i, q), t);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NPR; k++) t[k] = 0.;
+    }
+#pragma unroll
+    for (int k = 0; k < NPR; k++) v[k][o] = t[k];
+  }
+  const int nk = (P.eomtype == HARM_EOMFFDE) ? U3 + 1 : NPR;
+#pragma unroll
+  for (int k = 0; k < NPR; k++) out[k] = (k < nk) ? eno_cvt_line<TYPE>(v[k], pos, n) : v[k][4];
+}
+
+// Utoprimgen of U into pf[z] from the guess pin (the pf=pi assignment,
+// advance.c:121), fixup1zone (fixup.c:274-377), the zone's pflag and the
+// failed-zone list read by fixup_utoprim
+__device__ __forceinline__ void invert_zone(const KParams& P, const double* U, const double* pin, const double* gc,
+                                            const GeoU& g, int z, double* __restrict__ pf,
+                                            int* __restrict__ pflag, int* __restrict__ faillist,
+                                            DevScalars* __restrict__ ds, int& niter, int& failz, int& fail) {
+  double pout[NPR];
+#pragma unroll
+  for (int k = 0; k < NPR; k++) pout[k] = pin[k];
+  int pfl = utoprimgen(P, U, gc, pout, niter);
+  if (pfl == HARM_FFDE_SPACELIKE) {  // fatal in the reference (advance() returns 1)
+    fail = 1;
+    pfl = 0;
+  }
+  pflag[z] = pfl;
+  failz = (pfl != 0);
+  if (failz) {  // zones for fixup_utoprim (k_boundary)
+    int slot = atomicAdd(&ds->nlist, 1);
+    if (slot < P.n1 * P.n2) faillist[slot] = z;
+  }
+  // fixup1zone: floors then limit_gamma
+  if (P.eomtype == 0) {
+    double fr = gc[33], fu = gc[34];
+    if (fr < P.rhominlimit) fr = P.rhominlimit;
+    if (fu < P.uuminlimit) fu = P.uuminlimit;
+    if (fr > pout[RHO]) pout[RHO] = fr;
+    if (fu > pout[UU]) pout[UU] = fu;
+  }
+  if (limit_gamma(P.gammamax, pout, g) >= 1) fail = 1;
+  store8(pf, z, pout);
+}
+
+// Newton iterations, failed zones and the fatal flag of a wavefront
+__device__ __forceinline__ void count_zone(DevScalars* __restrict__ ds, int niter, int failz, int fail) {
+  unsigned long long ns = wave_sum_u64((unsigned long long)niter);
+  unsigned long long nf = wave_sum_u64((unsigned long long)failz);
+  if ((threadIdx.x & 63) == 0) {
+    if (ns) atomicAdd(&ds->nstroke, ns);
+    if (nf) atomicAdd(&ds->nfail, (int)nf);
+  }
+  if (fail) atomicOr(&ds->failed, 1);
+}
+
+// The per-zone update.  ENO (= P.eno) selects the form:
+//   0  advance_standard: Ui = U(pi), flux divergence, inversion
+//   1  advance_eno_u, middle loop (advance.c:401-438): Ui = average of U(pi)
+//      (c2a along x2 of eB, which holds c2a along x1); the averaged Uf (or
+//      unew on the last substage) goes to eA, inverted later by k_eno_invert
+//   2  DOENO=2 (advance.c:142-158): the flux divergence of each direction is
+//      converted average -> centre (a2c of eA along x1, of eB along x2)
+template <int ENO>
 __device__ __forceinline__ void update_body(KParams P, RKCoef C, const double* __restrict__ pi,
                                                 const double* __restrict__ pb, double* __restrict__ pf,
                                                 double* __restrict__ ulast, double* __restrict__ unew,
@@ -671,7 +794,8 @@ __device__ __forceinline__ void update_body(KParams P, RKCoef C, const double* _
                                                 const double* __restrict__ q1, const double* __restrict__ q2,
                                                 const double* __restrict__ gC, const double* __restrict__ gConn,
                                                 int* __restrict__ pflag, int* __restrict__ faillist,
-                                                DevScalars* __restrict__ ds, const dim3 bi, const dim3 gd) {
+                                                DevScalars* __restrict__ ds, double* eA, const double* eB,
+                                                const dim3 bi, const dim3 gd) {
   const int j = bi.x * blockDim.x + threadIdx.x;
   const int i = bi.y * blockDim.y + threadIdx.y;
   int niter = 0, failz = 0, fail = 0;
@@ -685,47 +809,30 @@ __device__ __forceinline__ void update_body(KParams P, RKCoef C, const double* _
     const int z = zidx(P, i, j);
     const double dt = ds->dt;
     double gc[36];
-    {
-      const double2* a = reinterpret_cast<const double2*>(gC + (size_t)z * GCN);
-#pragma unroll
-      for (int m = 0; m < 18; m++) {
-        double2 t = a[m];
-        gc[2 * m] = t.x;
-        gc[2 * m + 1] = t.y;
-      }
-    }
+    load_cent(gC, z, gc);
     GeoU g;
     geo_from_cent(gc, g);
     double pin[NPR], pbz[NPR];
     load8(pi, z, pin);
     load8(pb, z, pbz);
-    // Ui = U(pi)
-    State q;
     double Ui[NPR];
-    get_state(pin, g, q);
-    primtoflux(P, pin, q, 0, g.gdet, Ui);
+    if constexpr (ENO == 1) {
+      eno_cvt_zone<ENO_C2A, 2>(P, eB, i, j, Ui);
+    } else {  // Ui = U(pi)
+      State q;
+      get_state(pin, g, q);
+      primtoflux(P, pin, q, 0, g.gdet, Ui);
+    }
     // source(pb): T^j_k Gamma^k_{lj} contraction (phys.c:475-619)
     double dUg[NPR];
     source_conn(P, pbz, g, gConn + (size_t)z * 64, dUg);
-    // flux divergence (flux2dUavg, advance.c:204-265) with the CT B fluxes
-    double f1a[NPR], f1b[NPR], f2a[NPR], f2b[NPR];
-    load8(F1, z, f1a);
-    load8(F1, zidx(P, i + 1, j), f1b);
-    load8(F2, z, f2a);
-    load8(F2, zidx(P, i, j + 1), f2b);
-    if (P.fluxb == HARM_FLUXCTTOTH) {
-      double e00 = emf_at(P, q1, q2, gC, i, j), e01 = emf_at(P, q1, q2, gC, i, j + 1),
-             e10 = emf_at(P, q1, q2, gC, i + 1, j), e11 = emf_at(P, q1, q2, gC, i + 1, j + 1);
-      f1a[B1] = 0.;
-      f1a[B2] = 0.5 * (e00 + e01);
-      f1b[B1] = 0.;
-      f1b[B2] = 0.5 * (e10 + e11);
-      f2a[B1] = -0.5 * (e00 + e10);
-      f2a[B2] = 0.;
-      f2b[B1] = -0.5 * (e01 + e11);
-      f2b[B2] = 0.;
+    double d1[NPR], d2[NPR];
+    if constexpr (ENO == 2) {
+      eno_cvt_zone<ENO_A2C, 1>(P, eA, i, j, d1);
+      eno_cvt_zone<ENO_A2C, 2>(P, eB, i, j, d2);
+    } else {
+      flux_div(P, F1, F2, q1, q2, gC, i, j, d1, d2);
     }
-    const double idx1 = 1.0 / P.dx1, idx2 = 1.0 / P.dx2;
     double ul[NPR], un[NPR], Uf[NPR];
     if (C.timeorder == 0) {
 #pragma unroll
@@ -736,9 +843,7 @@ __device__ __forceinline__ void update_body(KParams P, RKCoef C, const double* _
     }
 #pragma unroll
     for (int k = 0; k < NPR; k++) {
-      double d1 = (-(f1b[k] - f1a[k]) * idx1);
-      double d2 = (-(f2b[k] - f2a[k]) * idx2);
-      double dUr = d1 + d2;
+      double dUr = d1[k] + d2[k];
       // dUtoU (advance.c:273-286)
       double dtdU = (dUr + dUg[k]);
       Uf[k] = C.cuf0 * Ui[k] + C.cuf1 * ul[k] + C.cuf2 * dt * dtdU;
@@ -747,41 +852,16 @@ __device__ __forceinline__ void update_body(KParams P, RKCoef C, const double* _
     }
     store8(ulast, z, ul);
     store8(unew, z, un);
-    // inversion, guess = pi (pf=pi assignment, advance.c:121)
-    double pout[NPR];
-#pragma unroll
-    for (int k = 0; k < NPR; k++) pout[k] = pin[k];
-    int pfl = utoprimgen(P, C.last ? un : Uf, gc, pout, niter);
-    if (pfl == HARM_FFDE_SPACELIKE) {  // fatal in the reference (advance() returns 1)
-      fail = 1;
-      pfl = 0;
+    if constexpr (ENO == 1) {
+      store8(eA, z, C.last ? un : Uf);
+    } else {
+      invert_zone(P, C.last ? un : Uf, pin, gc, g, z, pf, pflag, faillist, ds, niter, failz, fail);
     }
-    pflag[z] = pfl;
-    failz = (pfl != 0);
-    if (failz) {  // zones for fixup_utoprim (k_boundary)
-      int slot = atomicAdd(&ds->nlist, 1);
-      if (slot < P.n1 * P.n2) faillist[slot] = z;
-    }
-    // fixup1zone (fixup.c:274-377): floors then limit_gamma
-    if (P.eomtype == 0) {
-      double fr = gc[33], fu = gc[34];
-      if (fr < P.rhominlimit) fr = P.rhominlimit;
-      if (fu < P.uuminlimit) fu = P.uuminlimit;
-      if (fr > pout[RHO]) pout[RHO] = fr;
-      if (fu > pout[UU]) pout[UU] = fu;
-    }
-    if (limit_gamma(P.gammamax, pout, g) >= 1) fail = 1;
-    store8(pf, z, pout);
   }
-  unsigned long long ns = wave_sum_u64((unsigned long long)niter);
-  unsigned long long nf = wave_sum_u64((unsigned long long)failz);
-  if ((threadIdx.x & 63) == 0) {
-    if (ns) atomicAdd(&ds->nstroke, ns);
-    if (nf) atomicAdd(&ds->nfail, (int)nf);
-  }
-  if (fail) atomicOr(&ds->failed, 1);
+  count_zone(ds, niter, failz, fail);
 }
 
+template <int ENO>
 __global__ __launch_bounds__(256) void k_update(KParams P, RKCoef C, const double* __restrict__ pi,
                                                 const double* __restrict__ pb, double* __restrict__ pf,
                                                 double* __restrict__ ulast, double* __restrict__ unew,
@@ -789,9 +869,81 @@ __global__ __launch_bounds__(256) void k_update(KParams P, RKCoef C, const doubl
                                                 const double* __restrict__ q1, const double* __restrict__ q2,
                                                 const double* __restrict__ gC, const double* __restrict__ gConn,
                                                 int* __restrict__ pflag, int* __restrict__ faillist,
-                                                DevScalars* __restrict__ ds) {
-  update_body(P, C, pi, pb, pf, ulast, unew, F1, F2, q1, q2, gC, gConn, pflag, faillist, ds, dim3(blockIdx.x, blockIdx.y, blockIdx.z),
-      dim3(gridDim.x, gridDim.y, gridDim.z));
+                                                DevScalars* __restrict__ ds, double* eA, const double* eB) {
+  update_body<ENO>(P, C, pi, pb, pf, ulast, unew, F1, F2, q1, q2, gC, gConn, pflag, faillist, ds, eA, eB,
+              dim3(blockIdx.x, blockIdx.y, blockIdx.z), dim3(gridDim.x, gridDim.y, gridDim.z));
+}
+
+// ---- DOENO kernels (interior zones, thread per zone) ----
+
+// advance_eno_u first loop (advance.c:363-384): U(pi) at zone centres
+__global__ __launch_bounds__(256) void k_eno_ucenter(KParams P, const double* __restrict__ pi,
+                                                     const double* __restrict__ gC, double* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= P.n1 || j >= P.n2) return;
+  const int z = zidx(P, i, j);
+  double gc[36];
+  load_cent(gC, z, gc);
+  GeoU g;
+  geo_from_cent(gc, g);
+  double pin[NPR], U[NPR];
+  load8(pi, z, pin);
+  State q;
+  get_state(pin, g, q);
+  primtoflux(P, pin, q, 0, g.gdet, U);
+  store8(out, z, U);
+}
+
+// a2c()/c2a() along x1 of every interior zone (CZLOOP cvtij)
+template <int TYPE>
+__global__ __launch_bounds__(256) void k_eno_cvt1(KParams P, const double* __restrict__ in,
+                                                  double* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= P.n1 || j >= P.n2) return;
+  double o[NPR];
+  eno_cvt_zone<TYPE, 1>(P, in, i, j, o);
+  store8(out, zidx(P, i, j), o);
+}
+
+// DOENO=2 (advance.c:101-108): each direction's averaged flux divergence
+__global__ __launch_bounds__(256) void k_eno_fluxdiv(KParams P, const double* __restrict__ F1,
+                                                     const double* __restrict__ F2, const double* __restrict__ q1,
+                                                     const double* __restrict__ q2, const double* __restrict__ gC,
+                                                     double* __restrict__ d1out, double* __restrict__ d2out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= P.n1 || j >= P.n2) return;
+  double d1[NPR], d2[NPR];
+  flux_div(P, F1, F2, q1, q2, gC, i, j, d1, d2);
+  const int z = zidx(P, i, j);
+  store8(d1out, z, d1);
+  store8(d2out, z, d2);
+}
+
+// advance_eno_u last loop (advance.c:459-477): the averaged update converted
+// to zone centres (a2c along x2 of in, which holds a2c along x1), inverted
+// and fixed up
+__global__ __launch_bounds__(256) void k_eno_invert(KParams P, const double* __restrict__ pi,
+                                                    double* __restrict__ pf, const double* __restrict__ in,
+                                                    const double* __restrict__ gC, int* __restrict__ pflag,
+                                                    int* __restrict__ faillist, DevScalars* __restrict__ ds) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  int niter = 0, failz = 0, fail = 0;
+  if (i < P.n1 && j < P.n2) {
+    const int z = zidx(P, i, j);
+    double gc[36];
+    load_cent(gC, z, gc);
+    GeoU g;
+    geo_from_cent(gc, g);
+    double pin[NPR], U[NPR];
+    load8(pi, z, pin);
+    eno_cvt_zone<ENO_A2C, 2>(P, in, i, j, U);
+    invert_zone(P, U, pin, gc, g, z, pf, pflag, faillist, ds, niter, failz, fail);
+  }
+  count_zone(ds, niter, failz, fail);
 }
 
 // inflow_check_rel4vel for x1 ghosts at CENT (fixup.c:1733-1841)
@@ -1338,9 +1490,9 @@ __global__ __launch_bounds__(256) void kb_update(const RunDesc* __restrict__ run
                                                  RKCoef C, int pi, int pb, int pf) {
   const int4 e = tab[blockIdx.x];
   const RunDesc& r = runs[e.x];
-  update_body(r.P, C, r.arr[pi], r.arr[pb], r.arr[pf], r.arr[HARM_ULAST], r.arr[HARM_UNEW], r.arr[HARM_F1],
-              r.arr[HARM_F2], r.q1, r.q2, r.gC, r.gConn, r.pflag, r.faillist, r.ds, dim3(e.y, e.z, e.w),
-              dim3(0, 0, 0));
+  update_body<0>(r.P, C, r.arr[pi], r.arr[pb], r.arr[pf], r.arr[HARM_ULAST], r.arr[HARM_UNEW], r.arr[HARM_F1],
+              r.arr[HARM_F2], r.q1, r.q2, r.gC, r.gConn, r.pflag, r.faillist, r.ds, nullptr, nullptr,
+              dim3(e.y, e.z, e.w), dim3(0, 0, 0));
 }
 
 __global__ __launch_bounds__(1024) void kb_boundary(const RunDesc* __restrict__ runs, int sel, int mode) {
@@ -1395,6 +1547,8 @@ struct harm_ctx {
   double* gammacheck;  // fixup_checksolution's persistent gamma table
   int* csA;       // fixup_checksolution outcomes, first and second pass
   int* csB;
+  double* enoA;   // DOENO scratch (uenotmp0..2 of defs.h:80-82, two suffice)
+  double* enoB;
   double* gF;
   double* gC;
   double* gConn;
@@ -1443,6 +1597,7 @@ static int check_prm(const harm_params* p) {
     return seterr("periodic boundaries must be set on both sides");
   if (!(p->dx1 > 0) || !(p->dx2 > 0)) return seterr("dx1, dx2 must be positive");
   if (p->checksolution != 0 && p->checksolution != 1) return seterr("checksolution must be 0 or 1");
+  if (p->eno < 0 || p->eno > 2) return seterr("eno must be 0, 1 or 2");
   return 0;
 }
 
@@ -1460,7 +1615,9 @@ extern "C" int harm_create(const harm_params* prm, harm_ctx** out) {
   k.n1 = prm->n1; k.n2 = prm->n2; k.N2M = c->N2M;
   k.eomtype = prm->eomtype; k.lim = prm->lim; k.fluxmethod = prm->fluxmethod; k.fluxb = prm->fluxb;
   for (int b = 0; b < 4; b++) k.bc[b] = prm->bc[b];
-  k.fixupflux = prm->fixupflux; k.poledeath = prm->poledeath; k.checksolution = prm->checksolution;
+  // advance_eno_u calls fix_flux whatever FIXUPFLUX says (advance.c:355)
+  k.fixupflux = prm->fixupflux || prm->eno == 1;
+  k.poledeath = prm->poledeath; k.checksolution = prm->checksolution; k.eno = prm->eno;
   k.gam = prm->gam; k.cour = prm->cour; k.defcon = prm->defcon; k.safe = prm->safe;
   k.gammamax = prm->gammamax; k.gammafail = prm->gammafail;
   k.rhominlimit = prm->rhominlimit; k.uuminlimit = prm->uuminlimit;
@@ -1490,6 +1647,12 @@ extern "C" int harm_create(const harm_params* prm, harm_ctx** out) {
   if (e == hipSuccess) e = hipMalloc(&c->csB, nz * sizeof(int));
   if (e == hipSuccess) e = hipMemset(c->csB, 0, nz * sizeof(int));
   if (e == hipSuccess) e = hipMalloc(&c->faillist, nz * sizeof(int));
+  if (prm->eno) {
+    if (e == hipSuccess) e = hipMalloc(&c->enoA, nz * 8 * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(c->enoA, 0, nz * 8 * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc(&c->enoB, nz * 8 * sizeof(double));
+    if (e == hipSuccess) e = hipMemset(c->enoB, 0, nz * 8 * sizeof(double));
+  }
   if (e == hipSuccess) e = hipMalloc(&c->gF, 2 * nz * GF * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&c->gC, nz * GCN * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&c->gConn, nz * 64 * sizeof(double));
@@ -1528,6 +1691,8 @@ extern "C" int harm_destroy(harm_ctx* c) {
     if (b) (void)hipFree(b);
   if (c->pflag) (void)hipFree(c->pflag);
   if (c->bsqf) (void)hipFree(c->bsqf);
+  if (c->enoA) (void)hipFree(c->enoA);
+  if (c->enoB) (void)hipFree(c->enoB);
   if (c->gammacheck) (void)hipFree(c->gammacheck);
   if (c->csA) (void)hipFree(c->csA);
   if (c->csB) (void)hipFree(c->csB);
@@ -1706,12 +1871,33 @@ static int launch_flux(harm_ctx* c, int pr_sel, int dir0, int ndir) {
   return prof_end(c);
 }
 
+// advance() after the fluxes.  DOENO=1 (advance_eno_u): U(pi) -> A, c2a
+// along x1 A -> B, update (c2a along x2 of B; averaged Uf -> A), a2c along
+// x1 A -> B, inversion (a2c along x2 of B).  DOENO=2: the two flux
+// divergences -> A, B, then the update converts them.
 static int launch_update(harm_ctx* c, int pi, int pb, int pf, const RKCoef& C) {
   const dim3 b(64, 4);
+  const dim3 gz = grid2(c->kp.n2, c->kp.n1, b);
+  // the reference sets pf = pi before it reads pb zone by zone (advance.c:121,
+  // 367): with pf the same array as pb, the source term sees pi
+  const int pbs = (pf == pb) ? pi : pb;
   if (prof_begin(c, HARM_K_UPDATE)) return 1;
-  hipLaunchKernelGGL(k_update, grid2(c->kp.n2, c->kp.n1, b), b, 0, c->stream, c->kp, C, c->arr[pi], c->arr[pb],
-                     c->arr[pf], c->arr[HARM_ULAST], c->arr[HARM_UNEW], c->arr[HARM_F1], c->arr[HARM_F2], c->q1,
-                     c->q2, c->gC, c->gConn, c->pflag, c->faillist, c->ds);
+  if (c->kp.eno == 1) {
+    hipLaunchKernelGGL(k_eno_ucenter, gz, b, 0, c->stream, c->kp, c->arr[pi], c->gC, c->enoA);
+    hipLaunchKernelGGL(k_eno_cvt1<ENO_C2A>, gz, b, 0, c->stream, c->kp, c->enoA, c->enoB);
+  } else if (c->kp.eno == 2) {
+    hipLaunchKernelGGL(k_eno_fluxdiv, gz, b, 0, c->stream, c->kp, c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2,
+                       c->gC, c->enoA, c->enoB);
+  }
+  auto* kup = (c->kp.eno == 1) ? k_update<1> : (c->kp.eno == 2) ? k_update<2> : k_update<0>;
+  hipLaunchKernelGGL(kup, gz, b, 0, c->stream, c->kp, C, c->arr[pi], c->arr[pbs], c->arr[pf], c->arr[HARM_ULAST],
+                     c->arr[HARM_UNEW], c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2, c->gC, c->gConn, c->pflag,
+                     c->faillist, c->ds, c->enoA, c->enoB);
+  if (c->kp.eno == 1) {
+    hipLaunchKernelGGL(k_eno_cvt1<ENO_A2C>, gz, b, 0, c->stream, c->kp, c->enoA, c->enoB);
+    hipLaunchKernelGGL(k_eno_invert, gz, b, 0, c->stream, c->kp, c->arr[pi], c->arr[pf], c->enoB, c->gC, c->pflag,
+                       c->faillist, c->ds);
+  }
   HIPCHK(hipGetLastError());
   return prof_end(c);
 }
@@ -2239,6 +2425,7 @@ extern "C" int harm_batch_create(harm_ctx* const* ctxs, int n, harm_batch** out)
     if (ctxs[r]->prm.timeorder != ctxs[0]->prm.timeorder)
       return seterr("context %d: every run of a batch must use the same TIMEORDER", r);
     if (ctxs[r]->prm.checksolution) return seterr("context %d: CHECKSOLUTION is not batched", r);
+    if (ctxs[r]->prm.eno) return seterr("context %d: DOENO is not batched", r);
   }
   HIPCHK(hipDeviceSynchronize());
   harm_batch* b = new harm_batch();

@@ -93,6 +93,10 @@ typedef struct {
   int poledeath;       /* POLEDEATH (bounds.fishmon.c:328) */
   int checksolution;   /* CHECKSOLUTION (global.h:259): get_bsqflags before each
                           substage, fixup_checksolution in post_fixup */
+  int eno;             /* DOENO (global.h:347): 0 off; 1 advance_eno_u (U
+                          converted point -> average -> point around the
+                          update, advance.c:298); 2 the flux divergence
+                          converted average -> point (advance.c:101-158) */
   double gam;          /* adiabatic index */
   double cour;         /* Courant factor */
   double safe;         /* SAFE: max dt growth per step */

@@ -10,13 +10,14 @@
  * Compile-time flags of the reference that this restatement fixes:
  *   WHICHVEL=VELREL4, WHICHEOM=WITHGDET, REMOVERESTMASSFROMUU=1, NPR=8,
  *   MAXWELL=PRIMMAXWELL, TRUEFAST=0, HYPERHLL=0, BDIRCONT=1, EVOLVECHECKS=1,
- *   RESCALEINTERP=0, LIMADJUST=FLUXADJUST=0, DOENO=0, CORNGDETVERSION=1,
+ *   RESCALEINTERP=0, LIMADJUST=FLUXADJUST=0, CORNGDETVERSION=1,
  *   FIXUPZONES=FIXUP1ZONE, UTOPRIMVERSION=UTOPRIM2DFINAL,
- *   UTOPRIMTRYAGAIN=0, rescaletype=1, EXTRAP=0; CHECKSOLUTION is a run-time
- *   parameter (oracle_params.checksolution).
+ *   UTOPRIMTRYAGAIN=0, rescaletype=1, EXTRAP=0; CHECKSOLUTION and DOENO are
+ *   run-time parameters (oracle_params.checksolution, .eno).
  */
 #include "harm_oracle.h"
 #include <math.h>
+#include <stddef.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -66,6 +67,7 @@ struct oracle_ctx {
   double *arr[12];
   int *pflag;
   double *gammacheck; /* fixup_checksolution's static gammacheck[N1+2][N2+2] */
+  double *eno[3];     /* uenotmp0..2 (defs.h:80-82), DOENO only */
   double t, dt, tf;
   long nstroke;
   int failed;
@@ -777,59 +779,269 @@ static int fixup1zone(const oracle_ctx *c, double *pr, const geom_t *g) {
   return 0;
 }
 
-/* advance.c:46-197 advance_standard (DOENO=0, FIXUPZONES=FIXUP1ZONE, STEPOVERNEGU=0) */
+/* ---------------- ENO point <-> zone-average conversion ----------------
+ * reconstructeno.c / reconstructeno.h (DOENO=1 reconstructs U, DOENO=2 dU).
+ * Weights: row s holds the weights of the stencil's cells for the zone at
+ * position s of the stencil (the stencil starts s zones to its left).  Their
+ * values follow from exact polynomial reconstruction of degree order-1 (cell
+ * averages to the centre value, and back); the 3- and 5-cell tables are
+ * reconstructeno.h:18-58. */
+enum { ENO_A2C = 0, ENO_C2A = 1 };
+static const double eno_a2c3[3][3] = {
+    {23. / 24., 1. / 12., -1. / 24.}, {-1. / 24., 13. / 12., -1. / 24.}, {-1. / 24., 1. / 12., 23. / 24.}};
+static const double eno_c2a3[3][3] = {
+    {25. / 24., -1. / 12., 1. / 24.}, {1. / 24., 11. / 12., 1. / 24.}, {1. / 24., -1. / 12., 25. / 24.}};
+static const double eno_a2c5[5][5] = {
+    {563. / 640., 57. / 160., -373. / 960., 91. / 480., -71. / 1920.},
+    {-71. / 1920., 511. / 480., -13. / 960., -3. / 160., 3. / 640.},
+    {3. / 640., -29. / 480., 1067. / 960., -29. / 480., 3. / 640.},
+    {3. / 640., -3. / 160., -13. / 960., 511. / 480., -71. / 1920.},
+    {-71. / 1920., 91. / 480., -373. / 960., 57. / 160., 563. / 640.}};
+static const double eno_c2a5[5][5] = {
+    {6463. / 5760., -523. / 1440., 383. / 960., -283. / 1440., 223. / 5760.},
+    {223. / 5760., 1337. / 1440., 23. / 960., 17. / 1440., -17. / 5760.},
+    {-17. / 5760., 77. / 1440., 863. / 960., 77. / 1440., -17. / 5760.},
+    {-17. / 5760., 17. / 1440., 23. / 960., 1337. / 1440., 223. / 5760.},
+    {223. / 5760., -283. / 1440., 383. / 960., -523. / 1440., 6463. / 5760.}};
+
+/* reconstructeno.h:47-58 cvt_matrices: only orders 3 and 5 have a table */
+static const double *eno_matrix(int type, int order) {
+  if (order == 3) return type == ENO_A2C ? &eno_a2c3[0][0] : &eno_c2a3[0][0];
+  if (order == 5) return type == ENO_A2C ? &eno_a2c5[0][0] : &eno_c2a5[0][0];
+  return NULL;
+}
+
+double or_eno_weight(int type, int order, int s, int d) {
+  const double *m = eno_matrix(type, order);
+  if (!m || s < 0 || s >= order || d < 0 || d >= order) return 0.0;
+  return m[s * order + d];
+}
+
+/* reconstructeno.c:17-56 newtdiff: the order-th undivided Newton difference
+   of the values u[s*st], u[(s+1)*st], ..., by the same recursion */
+static double eno_newtdiff(const double *u, ptrdiff_t st, int order, int s) {
+  if (order <= 0) return u[s * st];
+  return eno_newtdiff(u, st, order - 1, s + 1) - eno_newtdiff(u, st, order - 1, s);
+}
+
+/* reconstructeno.c:58-131 choose_stencil on the line u[0..maxi] (min_index
+   0): grows the stencil from the zone pos towards the smoother side, with the
+   bias factor 2 towards the centred stencil; returns the left shift and may
+   lower *order when the line is shorter than the stencil */
+static int eno_choose_stencil(const double *u, ptrdiff_t st, int *order, int maxi, int pos) {
+  const double bias_factor = 2.0;
+  int is = pos, shift = 0;
+  if (maxi + 1 < *order) *order = maxi + 1;
+  const int index_pref = pos - (*order) / 2;
+  for (int m = 1; m < *order; m++) {
+    if (is == 0) continue;         /* cannot expand to the left */
+    if (is + m - 1 == maxi) {      /* cannot expand to the right */
+      is--;
+      shift++;
+      continue;
+    }
+    double leftdiff = eno_newtdiff(u, st, m, is - 1);
+    double rightdiff = eno_newtdiff(u, st, m, is);
+    double fac = (is <= index_pref) ? 1 / bias_factor : bias_factor;
+    if (fabs(leftdiff) < fac * fabs(rightdiff)) {
+      is--;
+      shift++;
+    }
+  }
+  return shift;
+}
+
+/* reconstructeno.c:133-235 cvtij for one component: u is the line through
+   the zone along the conversion direction (index 0 = first interior zone,
+   stride st), maxi = N-1 */
+static double eno_cvt(int type, const double *u, ptrdiff_t st, int maxi, int pos, int *order_out, int *shift_out) {
+  int order = 5; /* default_order */
+  int shift = eno_choose_stencil(u, st, &order, maxi, pos);
+  if (order < 3) { /* MIN_CVT_ORDER: leave the value as it is */
+    *order_out = order;
+    *shift_out = shift;
+    return u[pos * st];
+  }
+  const double *mat;
+  for (;;) { /* lower the order to one with a table, keeping the stencil near centred */
+    mat = eno_matrix(type, order);
+    if (mat) break;
+    --order;
+    if (shift > order / 2) shift--;
+  }
+  double out = 0.0;
+  for (int d = 0; d < order; d++) out += mat[shift * order + d] * u[(pos - shift + d) * st];
+  *order_out = order;
+  *shift_out = shift;
+  return out;
+}
+
+double or_eno_line(int type, const double *line, int n, int pos, int *order, int *shift) {
+  return eno_cvt(type, line, 1, n - 1, pos, order, shift);
+}
+
+/* cvtij (reconstructeno.c:133-235) of one zone along dir */
+static void eno_cvtij(const oracle_ctx *c, int dir, int type, const double *uin, int i, int j, double *uout) {
+  const int nk = (c->p.eomtype == OR_FFDE) ? U3 + 1 : NPR;
+  const ptrdiff_t st = (dir == 1) ? (ptrdiff_t)NPR * c->N2M : NPR;
+  for (int k = 0; k < nk; k++) {
+    const double *line = (dir == 1) ? PA(c, uin, 0, j) + k : PA(c, uin, i, 0) + k;
+    int order, shift;
+    uout[k] = eno_cvt(type, line, st, (dir == 1 ? c->p.n1 : c->p.n2) - 1, dir == 1 ? i : j, &order, &shift);
+  }
+  for (int k = nk; k < NPR; k++) uout[k] = PA(c, uin, i, j)[k]; /* EOMFFDE keeps B */
+}
+
+/* a2c()/c2a() over every interior zone along dir (CZLOOP cvtij) */
+static void eno_convert(const oracle_ctx *c, int dir, int type, const double *uin, double *uout) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < c->p.n1; i++)
+    for (int j = 0; j < c->p.n2; j++) eno_cvtij(c, dir, type, uin, i, j, PA(c, uout, i, j));
+}
+
+/* flux2dUavg (advance.c:204-265, FLUXB != FLUXCD) */
+static void flux2dUavg(const oracle_ctx *c, int i, int j, double *dU1, double *dU2) {
+  const double *F1 = c->arr[OR_F1], *F2 = c->arr[OR_F2];
+  double idx1 = 1.0 / c->p.dx1, idx2 = 1.0 / c->p.dx2;
+  for (int k = 0; k < NPR; k++) {
+    dU1[k] = (-(PA(c, F1, i + 1, j)[k] - PA(c, F1, i, j)[k]) * idx1);
+    dU2[k] = (-(PA(c, F2, i, j + 1)[k] - PA(c, F2, i, j)[k]) * idx2);
+  }
+}
+
+/* dUtoU (advance.c:273-286) */
+static void dUtoU(const double *dUgeom, const double *dUriemann, const double *CUf, const double *Cunew,
+                  const double *Ui, double *ul, double *Uf, double *un, double dt) {
+  for (int k = 0; k < NPR; k++)
+    ul[k] = Uf[k] = CUf[0] * Ui[k] + CUf[1] * ul[k] + CUf[2] * dt * (dUriemann[k] + dUgeom[k]);
+  for (int k = 0; k < NPR; k++)
+    un[k] += Cunew[0] * Ui[k] + Cunew[1] * dt * (dUriemann[k] + dUgeom[k]) + Cunew[2] * Uf[k];
+}
+
+/* Utoprimgen of U into pf (guess: pf as it stands) and fixup1zone
+   (advance.c:164-180 / 460-476); returns the failure flag */
+static int invert_zone(oracle_ctx *c, int i, int j, const geom_t *g, const double *U, double *pfz, long *nstroke) {
+  int pfl, fail = Utoprimgen(c, U, g, pfz, &pfl, nstroke);
+  c->pflag[(size_t)zi(c, i, j) * NUMPFLAGS + 0] = pfl;
+  if (fixup1zone(c, pfz, g)) fail |= 1;
+  return fail;
+}
+
+/* advance.c:298-487 advance_eno_u (DOENO=1): Ui(pi) at zone centres is
+   converted to zone averages (c2a along x1 then x2), updated with the flux
+   divergence and source, converted back (a2c along x1 then x2) and inverted */
+static int advance_eno_u(oracle_ctx *c, double *pi, double *pb, const double *CUf, double *pf, const double *Cunew,
+                         int timeorder, int numtimeorders) {
+  double *ulast = c->arr[OR_ULAST], *unew = c->arr[OR_UNEW];
+  double *e0 = c->eno[0], *e1 = c->eno[1], *e2 = c->eno[2];
+  const int n1 = c->p.n1, n2 = c->p.n2;
+  const double dt = c->dt;
+  long nstroke = 0;
+  int fail = 0;
+  /* pf = pi and Uicenter = U(pi) (uenotmp0) */
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n1; i++)
+    for (int j = 0; j < n2; j++) {
+      geom_t g;
+      state_t q;
+      double pin[NPR];
+      for (int k = 0; k < NPR; k++) pin[k] = PA(c, pi, i, j)[k];
+      for (int k = 0; k < NPR; k++) PA(c, pf, i, j)[k] = pin[k];
+      get_geometry(c, i, j, CENT, &g);
+      get_state(pin, &g, &q);
+      primtoflux(c, pin, &q, 0, &g, PA(c, e0, i, j));
+    }
+  eno_convert(c, 1, ENO_C2A, e0, e1); /* Uiavgtmp */
+  eno_convert(c, 2, ENO_C2A, e1, e2); /* Uiavg */
+  /* average Uf (or unew on the last substage) into Ufavg (uenotmp0) */
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n1; i++)
+    for (int j = 0; j < n2; j++) {
+      geom_t g;
+      double Ui[NPR], Uf[NPR], dUgeom[NPR], d1[NPR], d2[NPR], dUriemann[NPR], pbz[NPR];
+      double *ul = PA(c, ulast, i, j), *un = PA(c, unew, i, j);
+      get_geometry(c, i, j, CENT, &g);
+      for (int k = 0; k < NPR; k++) pbz[k] = PA(c, pb, i, j)[k];
+      source(c, pbz, &g, dUgeom);
+      for (int k = 0; k < NPR; k++) Ui[k] = PA(c, e2, i, j)[k];
+      if (timeorder == 0) for (int k = 0; k < NPR; k++) ul[k] = un[k] = 0.0;
+      flux2dUavg(c, i, j, d1, d2);
+      for (int k = 0; k < NPR; k++) dUriemann[k] = d1[k] + d2[k];
+      dUtoU(dUgeom, dUriemann, CUf, Cunew, Ui, ul, Uf, un, dt);
+      const double *src = (timeorder == numtimeorders - 1) ? un : Uf;
+      for (int k = 0; k < NPR; k++) PA(c, e0, i, j)[k] = src[k];
+    }
+  eno_convert(c, 1, ENO_A2C, e0, e1); /* Ufcentertmp */
+  eno_convert(c, 2, ENO_A2C, e1, e2); /* Ufcenter */
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : nstroke) reduction(| : fail)
+  for (int i = 0; i < n1; i++)
+    for (int j = 0; j < n2; j++) {
+      geom_t g;
+      get_geometry(c, i, j, CENT, &g);
+      fail |= invert_zone(c, i, j, &g, PA(c, e2, i, j), PA(c, pf, i, j), &nstroke);
+    }
+  c->nstroke += nstroke;
+  return fail;
+}
+
+/* advance.c:46-197 advance_standard (FIXUPZONES=FIXUP1ZONE, STEPOVERNEGU=0);
+   DOENO=2 converts each direction's averaged flux divergence to its zone
+   centre first (advance.c:101-108, 142-158); DOENO=1 is advance_eno_u */
 int or_advance(oracle_ctx *c, int pi_sel, int pb_sel, const double *CUf, int pf_sel, const double *Cunew,
                int timeorder, int numtimeorders, double *ndt) {
   double ndt1, ndt2;
   double *pi = c->arr[pi_sel], *pb = c->arr[pb_sel], *pf = c->arr[pf_sel];
   double *ulast = c->arr[OR_ULAST], *unew = c->arr[OR_UNEW];
-  double *F1 = c->arr[OR_F1], *F2 = c->arr[OR_F2];
   int n1 = c->p.n1, n2 = c->p.n2;
-  double idx1 = 1.0 / c->p.dx1, idx2 = 1.0 / c->p.dx2, dt = c->dt;
+  double dt = c->dt;
   long nstroke = 0;
   int fail = 0;
   fail |= or_fluxcalc(c, pb_sel, 1, &ndt1);
   fail |= or_fluxcalc(c, pb_sel, 2, &ndt2);
-  if (c->p.fixupflux) or_fix_flux(c);
+  /* advance_standard calls fix_flux under FIXUPFLUX (advance.c:89-91),
+     advance_eno_u always (advance.c:355) */
+  if (c->p.fixupflux || c->p.eno == 1) or_fix_flux(c);
   or_flux_ct(c);
-#pragma omp parallel for schedule(dynamic, 4) reduction(+ : nstroke) reduction(| : fail)
-  for (int i = 0; i < n1; i++)
-    for (int j = 0; j < n2; j++) {
-      double Uf[NPR], Ui[NPR], dUgeom[NPR], dUriemann[NPR];
-      geom_t g;
-      state_t q;
-      double *pfz = PA(c, pf, i, j), *piz = PA(c, pi, i, j);
-      double *ul = PA(c, ulast, i, j), *un = PA(c, unew, i, j);
-      double pin[NPR];
-      for (int k = 0; k < NPR; k++) pin[k] = piz[k];
-      for (int k = 0; k < NPR; k++) pfz[k] = pin[k];
-      if (timeorder == 0) for (int k = 0; k < NPR; k++) ul[k] = un[k] = 0.0;
-      get_geometry(c, i, j, CENT, &g);
-      get_state(pin, &g, &q);
-      primtoflux(c, pin, &q, 0, &g, Ui);
-      {
-        double pbz[NPR];
-        for (int k = 0; k < NPR; k++) pbz[k] = PA(c, pb, i, j)[k];
-        source(c, pbz, &g, dUgeom);
-      }
-      /* flux2dUavg (advance.c:204-265) */
-      for (int k = 0; k < NPR; k++) {
-        double d1 = (-(PA(c, F1, i + 1, j)[k] - PA(c, F1, i, j)[k]) * idx1);
-        double d2 = (-(PA(c, F2, i, j + 1)[k] - PA(c, F2, i, j)[k]) * idx2);
-        dUriemann[k] = d1 + d2;
-      }
-      /* dUtoU (advance.c:273-286) */
-      for (int k = 0; k < NPR; k++)
-        ul[k] = Uf[k] = CUf[0] * Ui[k] + CUf[1] * ul[k] + CUf[2] * dt * (dUriemann[k] + dUgeom[k]);
-      for (int k = 0; k < NPR; k++)
-        un[k] += Cunew[0] * Ui[k] + Cunew[1] * dt * (dUriemann[k] + dUgeom[k]) + Cunew[2] * Uf[k];
-      int pfl;
-      if (timeorder == numtimeorders - 1) fail |= Utoprimgen(c, un, &g, pfz, &pfl, &nstroke);
-      else fail |= Utoprimgen(c, Uf, &g, pfz, &pfl, &nstroke);
-      c->pflag[(size_t)zi(c, i, j) * NUMPFLAGS + 0] = pfl;
-      if (fixup1zone(c, pfz, &g)) fail |= 1;
+  if (c->p.eno == 1) {
+    fail |= advance_eno_u(c, pi, pb, CUf, pf, Cunew, timeorder, numtimeorders);
+  } else {
+    if (c->p.eno == 2) {
+#pragma omp parallel for schedule(static)
+      for (int i = 0; i < n1; i++)
+        for (int j = 0; j < n2; j++) flux2dUavg(c, i, j, PA(c, c->eno[0], i, j), PA(c, c->eno[1], i, j));
     }
-  c->nstroke += nstroke;
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : nstroke) reduction(| : fail)
+    for (int i = 0; i < n1; i++)
+      for (int j = 0; j < n2; j++) {
+        double Uf[NPR], Ui[NPR], dUgeom[NPR], dUriemann[NPR], d1[NPR], d2[NPR];
+        geom_t g;
+        state_t q;
+        double *pfz = PA(c, pf, i, j), *piz = PA(c, pi, i, j);
+        double *ul = PA(c, ulast, i, j), *un = PA(c, unew, i, j);
+        double pin[NPR];
+        for (int k = 0; k < NPR; k++) pin[k] = piz[k];
+        for (int k = 0; k < NPR; k++) pfz[k] = pin[k];
+        if (timeorder == 0) for (int k = 0; k < NPR; k++) ul[k] = un[k] = 0.0;
+        get_geometry(c, i, j, CENT, &g);
+        get_state(pin, &g, &q);
+        primtoflux(c, pin, &q, 0, &g, Ui);
+        {
+          double pbz[NPR];
+          for (int k = 0; k < NPR; k++) pbz[k] = PA(c, pb, i, j)[k];
+          source(c, pbz, &g, dUgeom);
+        }
+        if (c->p.eno == 2) {
+          eno_cvtij(c, 1, ENO_A2C, c->eno[0], i, j, d1);
+          eno_cvtij(c, 2, ENO_A2C, c->eno[1], i, j, d2);
+        } else {
+          flux2dUavg(c, i, j, d1, d2);
+        }
+        for (int k = 0; k < NPR; k++) dUriemann[k] = d1[k] + d2[k];
+        dUtoU(dUgeom, dUriemann, CUf, Cunew, Ui, ul, Uf, un, dt);
+        fail |= invert_zone(c, i, j, &g, (timeorder == numtimeorders - 1) ? un : Uf, pfz, &nstroke);
+      }
+    c->nstroke += nstroke;
+  }
   *ndt = c->p.defcon * 1. / (1. / ndt1 + 1. / ndt2);
   if (fail) c->failed = 1;
   return fail;
@@ -1232,6 +1444,7 @@ oracle_ctx *or_create(const oracle_params *prm) {
   for (int a = 0; a < 12; a++) c->arr[a] = (double *)calloc((size_t)c->nz * NPR, sizeof(double));
   c->pflag = (int *)calloc((size_t)c->nz * NUMPFLAGS, sizeof(int));
   c->gammacheck = (double *)calloc((size_t)(prm->n1 + 2) * (prm->n2 + 2), sizeof(double));
+  for (int a = 0; a < 3; a++) c->eno[a] = (double *)calloc((size_t)c->nz * NPR, sizeof(double));
   c->tf = 1e30;
 #ifdef _OPENMP
   if (prm->nthreads > 0) omp_set_num_threads(prm->nthreads);
@@ -1242,6 +1455,7 @@ oracle_ctx *or_create(const oracle_params *prm) {
 void or_destroy(oracle_ctx *c) {
   if (!c) return;
   free(c->gcov); free(c->gcon); free(c->gdet); free(c->conn); free(c->floors); free(c->gammacheck);
+  for (int a = 0; a < 3; a++) free(c->eno[a]);
   for (int a = 0; a < 12; a++) free(c->arr[a]);
   free(c->pflag);
   free(c);

@@ -38,6 +38,7 @@ typedef struct {
   int fixupflux;        /* FIXUPFLUX (step_ch.h:82) */
   int poledeath;        /* POLEDEATH (bounds.fishmon.c:328) */
   int checksolution;    /* CHECKSOLUTION (global.h:259) */
+  int eno;              /* DOENO (global.h:347): 0 off, 1 convert U, 2 convert dU */
   double gam, cour, safe, defcon, gammamax, gammafail;
   double rhominlimit, uuminlimit;
   double dx1, dx2;
@@ -86,6 +87,13 @@ int or_vchar(oracle_ctx *c, int i, int j, int loc, const double *pr, int dir,
 int or_source(oracle_ctx *c, int i, int j, const double *pr, double *dU);
 void or_slope_lim(int lim, int k, double yll, double yl, double yc, double yr,
                   double yrr, double *dq, double *left, double *right);
+/* ENO point <-> average conversion of one value along a line of n values
+   (stride 1): type 0 average-to-centre, 1 centre-to-average; returns the
+   converted value at position pos and the stencil's order and shift */
+double or_eno_line(int type, const double *line, int n, int pos, int *order, int *shift);
+/* the conversion weight of a stencil of `order` cells for the zone at
+   position s of the stencil, cell d (0 when there is no such table) */
+double or_eno_weight(int type, int order, int s, int d);
 
 #ifdef __cplusplus
 }

@@ -20,7 +20,7 @@ class OracleParams(ctypes.Structure):
         ("n1", ctypes.c_int), ("n2", ctypes.c_int),
         ("eomtype", ctypes.c_int), ("lim", ctypes.c_int), ("fluxmethod", ctypes.c_int), ("fluxb", ctypes.c_int),
         ("timeorder", ctypes.c_int), ("bc", ctypes.c_int * 4), ("fixupflux", ctypes.c_int),
-        ("poledeath", ctypes.c_int), ("checksolution", ctypes.c_int),
+        ("poledeath", ctypes.c_int), ("checksolution", ctypes.c_int), ("eno", ctypes.c_int),
         ("gam", ctypes.c_double), ("cour", ctypes.c_double), ("safe", ctypes.c_double), ("defcon", ctypes.c_double),
         ("gammamax", ctypes.c_double), ("gammafail", ctypes.c_double), ("rhominlimit", ctypes.c_double),
         ("uuminlimit", ctypes.c_double), ("dx1", ctypes.c_double), ("dx2", ctypes.c_double),
@@ -71,6 +71,8 @@ def load():
         "or_source": ([vp, ctypes.c_int, ctypes.c_int, dp, dp], ctypes.c_int),
         "or_slope_lim": ([ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                           ctypes.c_double, ctypes.c_double, dp, dp, dp], None),
+        "or_eno_line": ([ctypes.c_int, dp, ctypes.c_int, ctypes.c_int, ip, ip], ctypes.c_double),
+        "or_eno_weight": ([ctypes.c_int] * 4, ctypes.c_double),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -214,3 +216,20 @@ def slope_lim(lim, k, yll, yl, yc, yr, yrr):
     dq, l, r = ctypes.c_double(np.nan), ctypes.c_double(np.nan), ctypes.c_double(np.nan)
     lib.or_slope_lim(lim, k, yll, yl, yc, yr, yrr, ctypes.byref(dq), ctypes.byref(l), ctypes.byref(r))
     return dq.value, l.value, r.value
+
+
+ENO_A2C, ENO_C2A = 0, 1
+
+
+def eno_line(kind, line, pos):
+    """cvtij of one value: kind ENO_A2C (averages to centre) or ENO_C2A, along
+    the line of interior values; returns (value, order, shift)."""
+    lib = load()
+    a = np.ascontiguousarray(line, dtype=np.float64)
+    order, shift = ctypes.c_int(), ctypes.c_int()
+    v = lib.or_eno_line(kind, _d(a), len(a), pos, ctypes.byref(order), ctypes.byref(shift))
+    return v, order.value, shift.value
+
+
+def eno_weight(kind, order, s, d):
+    return load().or_eno_weight(kind, order, s, d)

@@ -1,10 +1,12 @@
 // adjoint_shim.cpp -- TEST INFRASTRUCTURE.  A host-only build of the
-// product's per-zone physics (harm_device.h) and its reverse-mode
-// counterparts (harm_adjoint.h), so that tests/test_adjoint_math.py can check
-// each vector-Jacobian product against central finite differences of the
-// forward function on the CPU.  Built by the tests with
+// product's per-zone physics (harm_device.h), its reverse-mode counterparts
+// (harm_adjoint.h) and the ENO conversion (harm_eno.h), so that the CPU tests
+// can check each vector-Jacobian product against central finite differences
+// of the forward function, and the conversion against the oracle's.  Built by
+// the tests with
 //   hipcc -x hip --offload-host-only -O2 -ffp-contract=off -shared -fPIC
 #include "../harm_ramt_amd/csrc/harm_adjoint.h"
+#include "../harm_ramt_amd/csrc/harm_eno.h"
 
 using namespace harm;
 
@@ -141,5 +143,17 @@ double shim_slope3(int lim, double yl, double yc, double yr) { return slope3(lim
 void shim_slope3_coef(int lim, double yl, double yc, double yr, double* c) {
   slope3_coef(lim, yl, yc, yr, c[0], c[1], c[2]);
 }
+
+// the device conversion of the value at pos of a line of n values
+double shim_eno_line(int type, const double* line, int n, int pos) {
+  double v[9];
+  for (int o = 0; o < 9; o++) {
+    const int q = pos - 4 + o;
+    v[o] = (q >= 0 && q < n) ? line[q] : 0.;
+  }
+  return type == ENO_A2C ? eno_cvt_line<ENO_A2C>(v, pos, n) : eno_cvt_line<ENO_C2A>(v, pos, n);
+}
+
+double shim_eno_weight(int type, int order, int s, int d) { return eno_weight(type, order, s, d); }
 
 }  // extern "C"

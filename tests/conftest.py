@@ -43,7 +43,8 @@ def build_adjoint_shim():
 def shim():
     if not os.path.exists(SHIM_LIB) or os.path.getmtime(SHIM_LIB) < max(
             os.path.getmtime(SHIM_SRC), os.path.getmtime(os.path.join(ROOT, "harm_ramt_amd", "csrc", "harm_adjoint.h")),
-            os.path.getmtime(os.path.join(ROOT, "harm_ramt_amd", "csrc", "harm_device.h"))):
+            os.path.getmtime(os.path.join(ROOT, "harm_ramt_amd", "csrc", "harm_device.h")),
+            os.path.getmtime(os.path.join(ROOT, "harm_ramt_amd", "csrc", "harm_eno.h"))):
         build_adjoint_shim()
     lib = ctypes.CDLL(SHIM_LIB)
     dp = ctypes.POINTER(ctypes.c_double)
@@ -51,4 +52,8 @@ def shim():
     lib.shim_slope3.restype = ctypes.c_double
     lib.shim_slope3.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double]
     lib.shim_slope3_coef.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double, dp]
+    lib.shim_eno_line.restype = ctypes.c_double
+    lib.shim_eno_line.argtypes = [ctypes.c_int, dp, ctypes.c_int, ctypes.c_int]
+    lib.shim_eno_weight.restype = ctypes.c_double
+    lib.shim_eno_weight.argtypes = [ctypes.c_int] * 4
     return lib

@@ -3,7 +3,8 @@ container, where /root/reference exists; the GPU box only reads the .npz).
 
 For each case the script builds the input state and geometry with
 harm_ramt_amd's host generators, runs the reference's own step_ch() through
-oracle/_ref/ref_{ks,mink} (see oracle/ref/), and stores
+oracle/_ref/ref_* (see oracle/ref/: one build per problem family and
+compile-time option), and stores
   - sha256 of the input state and of the geometry tables (pins the inputs),
   - per step: t, dt, Newton iteration count (nstroke), failure code
     (failed + 2 x number of zones with a nonzero FLAGUTOPRIMFAIL pflag; the
@@ -46,7 +47,16 @@ CASES = {
     # torus threaded by a weak split-monopole field, CHECKSOLUTION 1: b^2 flags
     # in the magnetised atmosphere, zones failed by fixup_checksolution
     "torus_field_checksolution_rk4": ("ks_mono", 32, 64, H.PARA, H.HLLFLUX, 4, 40, 1e-3, (1, 10, 40)),
+    # DOENO 1 (U converted centre -> average -> centre around the update) and
+    # DOENO 2 (flux divergence converted average -> centre)
+    "torus_eno_u_para_hll_rk4": ("ks", 32, 64, H.PARA, H.HLLFLUX, 4, 30, 1e-3, (1, 30)),
+    "torus_eno_du_mc_laxf_rk2": ("ks", 32, 64, H.MC, H.LAXFFLUX, 2, 30, 1e-3, (30,)),
+    "ot_eno_u_mc_hll_rk4": ("mink", 32, 32, H.MC, H.HLLFLUX, 4, 30, 1e-4, (1, 30)),
+    "ot_eno_du_para_hll_rk3": ("mink", 32, 32, H.PARA, H.HLLFLUX, 3, 30, 1e-4, (30,)),
 }
+# DOENO of the cases that set it (the reference is built once per value)
+ENO = {"torus_eno_u_para_hll_rk4": 1, "torus_eno_du_mc_laxf_rk2": 2, "ot_eno_u_mc_hll_rk4": 1,
+       "ot_eno_du_para_hll_rk3": 2}
 
 
 def sha(a):
@@ -75,10 +85,20 @@ def problem(kind, n1, n2):
     return g, geom, p0, dict(gam=gam, cour=cour, bc=bc, fixupflux=fix, poledeath=pole)
 
 
-def run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work):
+def bc_of(kind):
+    """BCtype[X1UP..X2DN] of a problem family (periodic: the stand-in value,
+    which the reference's BCtype tests treat as neither OUTFLOW nor POLARAXIS)."""
+    if kind == "mink":
+        return (H.PERIODIC,) * 4
+    return (H.OUTFLOW, H.OUTFLOW, H.POLARAXIS, H.POLARAXIS)
+
+
+def run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, eno=0):
     p0.tofile(os.path.join(work, "p0.bin"))
     cfg = dict(prim=os.path.join(work, "p0.bin"), dt=dt0, nsteps=nsteps, out=os.path.join(work, "o.bin"),
                lim=lim, fluxmethod=flux, timeorder=to)
+    for n, b in enumerate(bc_of(kind)):
+        cfg[f"bc{n}"] = b
     if kind in ("ks", "ks_mono", "ffde"):
         cfg.update(a=g.a, Rout=g.Rout, hslope=g.hslope, R0=g.R0, geom="ref", gam=4.0 / 3.0, cour=0.9,
                    tables_out=os.path.join(work, "tables.bin"))
@@ -94,7 +114,7 @@ def run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work):
     with open(path, "w") as f:
         for k, v in cfg.items():
             f.write(f"{k} {v!r}\n" if isinstance(v, float) else f"{k} {v}\n")
-    binary = {"ks_mono": "ref_kscs"}.get(kind, "ref_" + kind)
+    binary = {"ks_mono": "ref_kscs"}.get(kind, "ref_" + kind) + (f"eno{eno}" if eno else "")
     subprocess.run([os.path.join(REF_DIR, binary), path], cwd=work, check=True, capture_output=True)
     d = np.fromfile(os.path.join(work, "o.bin"), dtype=np.float64)[4:]
     nz = (g.n1 + 6) * (g.n2 + 6) * 8
@@ -113,8 +133,10 @@ def main():
         for name in names:
             kind, n1, n2, lim, flux, to, nsteps, dt0, saved = CASES[name]
             g, geom, p0, extra = problem(kind, n1, n2)
-            recs, states = run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work)
+            eno = ENO.get(name, 0)
+            recs, states = run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, eno)
             out = dict(kind=kind, n1=n1, n2=n2, lim=lim, fluxmethod=flux, timeorder=to, nsteps=nsteps, dt0=dt0,
+                       eno=eno,
                        input_sha=sha(p0), geom_sha=sha(np.concatenate([a.ravel() for a in geom.tables()])),
                        t=recs[:, 0], dt=recs[:, 1], nstroke=recs[:, 2], failcode=recs[:, 3],
                        saved_steps=np.array(saved))

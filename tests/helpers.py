@@ -14,7 +14,7 @@ def sha(a):
     return hashlib.sha256(np.ascontiguousarray(a, dtype=np.float64).tobytes()).hexdigest()
 
 
-def problem(kind, n1, n2, lim=H.PARA, fluxmethod=H.HLLFLUX, timeorder=4, seed=7):
+def problem(kind, n1, n2, lim=H.PARA, fluxmethod=H.HLLFLUX, timeorder=4, seed=7, eno=0):
     """(grid, geometry, initial state, harm_params) for a torus ('ks'), a
     torus threaded by a weak split-monopole field with CHECKSOLUTION on
     ('ks_mono'), an Orszag-Tang ('mink') or a force-free split-monopole
@@ -26,19 +26,19 @@ def problem(kind, n1, n2, lim=H.PARA, fluxmethod=H.HLLFLUX, timeorder=4, seed=7)
         if kind == "ks_mono":   # torus threaded by a weak split-monopole field, CHECKSOLUTION on
             p0[..., 5] += H.init_monopole(geom, 1e-2)[..., 5]
         prm = H.make_params(n1, n2, g.dx1, g.dx2, lim=lim, fluxmethod=fluxmethod, timeorder=timeorder,
-                            checksolution=1 if kind == "ks_mono" else 0)
+                            checksolution=1 if kind == "ks_mono" else 0, eno=eno)
     elif kind == "ffde":
         g = H.grid_mks(n1, n2, a=0.9, Rout=100.0, hslope=0.1)
         geom = H.Geometry(g)
         p0 = H.init_monopole(geom)
         prm = H.make_params(n1, n2, g.dx1, g.dx2, lim=lim, fluxmethod=fluxmethod, timeorder=timeorder,
-                            eomtype=H.EOMFFDE, poledeath=0)
+                            eomtype=H.EOMFFDE, poledeath=0, eno=eno)
     else:
         g = H.grid_cartesian(n1, n2)
         geom = H.Geometry(g)
         p0 = H.init_orszag_tang(geom)
         prm = H.make_params(n1, n2, g.dx1, g.dx2, lim=lim, fluxmethod=fluxmethod, timeorder=timeorder,
-                            gam=5.0 / 3.0, cour=0.8, bc=(H.PERIODIC,) * 4, fixupflux=0, poledeath=0)
+                            gam=5.0 / 3.0, cour=0.8, bc=(H.PERIODIC,) * 4, fixupflux=0, poledeath=0, eno=eno)
     return g, geom, p0, prm
 
 
@@ -49,6 +49,13 @@ def golden_cases():
 
 def load_golden(name):
     return np.load(os.path.join(GOLDEN, name + ".npz"))
+
+
+def golden_problem(gd):
+    """problem() with the settings a golden vector file records."""
+    eno = int(gd["eno"]) if "eno" in gd.files else 0
+    return problem(str(gd["kind"]), int(gd["n1"]), int(gd["n2"]), lim=int(gd["lim"]),
+                   fluxmethod=int(gd["fluxmethod"]), timeorder=int(gd["timeorder"]), eno=eno)
 
 
 def same(a, b):

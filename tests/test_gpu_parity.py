@@ -12,7 +12,7 @@ import pytest
 
 import harm_ramt_amd as H
 from oracle.oracle import Oracle
-from tests.helpers import golden_cases, load_golden, problem, same, sha
+from tests.helpers import golden_cases, golden_problem, load_golden, problem, same, sha
 
 pytestmark = pytest.mark.gpu
 NG = 3
@@ -53,8 +53,7 @@ def test_gpu_step_matches_reference(name):
     gd = load_golden(name)
     kind, n1, n2 = str(gd["kind"]), int(gd["n1"]), int(gd["n2"])
     say(f"[case] {name} {kind} {n1}x{n2} lim={int(gd['lim'])} steps={int(gd['nsteps'])}")
-    g, geom, p0, prm = problem(kind, n1, n2, lim=int(gd["lim"]), fluxmethod=int(gd["fluxmethod"]),
-                               timeorder=int(gd["timeorder"]))
+    g, geom, p0, prm = golden_problem(gd)
     assert sha(p0) == str(gd["input_sha"])
     exact = int(gd["lim"]) != H.CSSLOPE
     h = gpu_ctx(geom, prm, p0, float(gd["dt0"]))
@@ -181,6 +180,22 @@ def test_gpu_fullsize_matches_oracle(kind, n):
     o = oracle_ctx(geom, prm, p0, dt0)
     for s in range(6):
         h.step_ch()
+        o.step()
+    assert h.get_time() == o.get_time()
+    assert same(h.get_array(H.P), o.get_array(0))
+    assert h.nstroke() == o.nstroke()
+
+
+@pytest.mark.parametrize("kind,eno", [("ks", 1), ("ks", 2), ("mink", 1), ("mink", 2)])
+def test_gpu_eno_graph_matches_oracle(kind, eno):
+    """DOENO through the autotuned graph path at 128x128, bitwise vs the oracle."""
+    say(f"[case] DOENO={eno} {kind} 128x128, 6 steps via harm_step_n vs oracle")
+    g, geom, p0, prm = problem(kind, 128, 128, lim=H.PARA if kind == "ks" else H.MC, eno=eno)
+    dt0 = 1e-4 if kind == "mink" else 1e-3
+    h = gpu_ctx(geom, prm, p0, dt0)
+    o = oracle_ctx(geom, prm, p0, dt0)
+    h.step_n(6)
+    for _ in range(6):
         o.step()
     assert h.get_time() == o.get_time()
     assert same(h.get_array(H.P), o.get_array(0))

@@ -6,15 +6,14 @@ import pytest
 
 import harm_ramt_amd as H
 from oracle.oracle import Oracle
-from tests.helpers import golden_cases, load_golden, problem, same, sha
+from tests.helpers import golden_cases, golden_problem, load_golden, problem, same, sha
 
 
 @pytest.mark.parametrize("name", golden_cases())
 def test_oracle_matches_reference(name):
     gd = load_golden(name)
     kind, n1, n2 = str(gd["kind"]), int(gd["n1"]), int(gd["n2"])
-    g, geom, p0, prm = problem(kind, n1, n2, lim=int(gd["lim"]), fluxmethod=int(gd["fluxmethod"]),
-                               timeorder=int(gd["timeorder"]))
+    g, geom, p0, prm = golden_problem(gd)
     assert sha(p0) == str(gd["input_sha"]), "input state generator changed"
     assert sha(np.concatenate([a.ravel() for a in geom.tables()])) == str(gd["geom_sha"])
     o = Oracle(H.params_dict(prm))
