@@ -139,12 +139,16 @@ def test_adjoint_single_zone_components():
     p0 = generic(ot_state(g))
     J, lam, dts, ns, pN, after = gradient(geom, prm, p0, nsteps, 2)
     assert ns == (0, 0)
+    # h = 1e-6: at the corner zone a wave-speed max/min switches branch
+    # within 1e-5 of rho and u (the CPU reference sweep gives the same
+    # lambda there; finite differences converge to it only below that);
+    # the absolute floor is the round-off of J's difference quotient
     for (i, j) in ((3 + 5, 3 + 7), (3 + 0, 3 + 15)):
         for k in range(8):
             v = np.zeros_like(p0)
             v[i, j, k] = 1.0
-            fd = fd_along(geom, prm, p0, dts, v, h=1e-5)
-            assert lam[i, j, k] == pytest.approx(fd, rel=1e-5, abs=1e-10 * abs(J)), f"zone {(i, j)} prim {k}"
+            fd = fd_along(geom, prm, p0, dts, v, h=1e-6)
+            assert lam[i, j, k] == pytest.approx(fd, rel=1e-5, abs=3e-10 * abs(J)), f"zone {(i, j)} prim {k}"
 
 
 @pytest.mark.parametrize("lim,flux,to", [(H.NLIMUP, H.LAXFFLUX, 2), (H.DONOR, H.HLLFLUX, 3), (H.NLIMDOWN, H.HLLFLUX, 1)])

@@ -278,7 +278,7 @@ def test_gpu_checksolution_matches_oracle(fv, bv):
     pf_o = o.pflag()[3:-3, 3:-3]
     assert np.array_equal(pf_g[..., 0], pf_o[..., 0]), "FLAGUTOPRIMFAIL"
     assert np.array_equal(pf_g[..., 2:4], pf_o[..., 2:4]), "b^2 flags"
-    assert (pf_g[..., 2] > 0).sum() > 100, "the problem is meant to flag the magnetised atmosphere"
+    assert (pf_g[..., 2:4] > 0).sum() > 100, "the problem is meant to flag the magnetised atmosphere"
 
 
 def test_gpu_checksolution_autotuned_graph():
