@@ -54,13 +54,14 @@ def kernel_of(stage, flux_variant):
 def traffic_from_profiles(kernel, variants):
     """HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
     (profiles/pmc_traffic.json, counters scaled by a copy kernel of known
-    size), or None if no profile of this kernel variant has been taken."""
+    size), or None if no profile of this kernel variant has been taken.  The
+    flux and update kernels' traffic does not depend on the boundary variant."""
     path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
     if not os.path.exists(path):
         return None
     try:
         d = json.load(open(path))
-        if d.get("variants") != variants:
+        if (d.get("variants") or {}).get("flux") != variants.get("flux"):
             return None
         return d.get("per_launch_bytes", {}).get(kernel)
     except (OSError, ValueError):
@@ -368,7 +369,10 @@ def main():
                    "kernel_variants": {"flux": {H.FLUX_FACE_64x4: "face_64x4", H.FLUX_FACE_64x2: "face_64x2",
                                                 H.FLUX_PAIR: "lane_pair"}[fv],
                                        "boundary": {H.BOUND_KERNELS: "wide_launches",
-                                                    H.BOUND_BLOCK: "one_workgroup"}[bv]}},
+                                                    H.BOUND_BLOCK: "one_workgroup",
+                                                    H.BOUND_KERNELS_SEQCHECK: "wide_launches_seqcheck",
+                                                    H.BOUND_FIXONE: "wide_fills_one_workgroup_fixup",
+                                                    H.BOUND_FUSED: "one_launch_fills_one_workgroup_fixup"}[bv]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,

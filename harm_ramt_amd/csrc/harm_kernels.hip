@@ -980,32 +980,37 @@ __device__ int inflow_check_x1(const KParams& P, int i, double* pr, const double
 // x1 ghost cell (gi in 0..5 -> i = -3..-1, N1..N1+2; j in [0,N2)):
 // bounds.fishmon.c:26-110,190-262 (EXTRAP=0 copy, inflow check, limit_gamma)
 // or the periodic wrap
+// the interior zone an x1 ghost (i, j) is filled from
+__device__ __forceinline__ int x1_source(const KParams& P, int i) {
+  const bool dn = i < 0;
+  const int bc = dn ? P.bc[HARM_X1DN] : P.bc[HARM_X1UP];
+  if (bc == HARM_PERIODIC) return dn ? i + P.n1 : i - P.n1;
+  return dn ? 0 : P.n1 - 1;
+}
+
+// the value of x1 ghost (i, j) from the given source state v (in/out)
+__device__ __forceinline__ void x1_value(const KParams& P, const double* __restrict__ gC, int i, int j, double* v,
+                                         int& fail) {
+  const int bc = (i < 0) ? P.bc[HARM_X1DN] : P.bc[HARM_X1UP];
+  if (bc != HARM_OUTFLOW) return;  // periodic: the copy itself
+  double gc[36];
+  load_cent(gC, zidx(P, i, j), gc);
+  GeoU g;
+  geo_from_cent(gc, g);
+  if (inflow_check_x1(P, i, v, gc, g) >= 1) fail = 1;
+  if (limit_gamma(P.gammamax, v, g) >= 1) fail = 1;
+}
+
 __device__ void bound_x1_cell(const KParams& P, double* __restrict__ pr, const double* __restrict__ gC, int gi,
                               int j, int& fail) {
   const bool dn = gi < NG;
   const int i = dn ? (gi - NG) : (P.n1 + gi - NG);
   const int bc = dn ? P.bc[HARM_X1DN] : P.bc[HARM_X1UP];
+  if (bc != HARM_PERIODIC && bc != HARM_OUTFLOW) return;
   double v[NPR];
-  if (bc == HARM_PERIODIC) {
-    load8(pr, zidx(P, dn ? i + P.n1 : i - P.n1, j), v);
-    store8(pr, zidx(P, i, j), v);
-  } else if (bc == HARM_OUTFLOW) {
-    load8(pr, zidx(P, dn ? 0 : P.n1 - 1, j), v);
-    const int z = zidx(P, i, j);
-    double gc[36];
-    const double2* a = reinterpret_cast<const double2*>(gC + (size_t)z * GCN);
-#pragma unroll
-    for (int m = 0; m < 18; m++) {
-      double2 t = a[m];
-      gc[2 * m] = t.x;
-      gc[2 * m + 1] = t.y;
-    }
-    GeoU g;
-    geo_from_cent(gc, g);
-    if (inflow_check_x1(P, i, v, gc, g) >= 1) fail = 1;
-    if (limit_gamma(P.gammamax, v, g) >= 1) fail = 1;
-    store8(pr, z, v);
-  }
+  load8(pr, zidx(P, x1_source(P, i), j), v);
+  x1_value(P, gC, i, j, v, fail);
+  store8(pr, zidx(P, i, j), v);
 }
 
 // x2 ghost cell (i in [-3,N1+2], gj in 0..5 -> j = -3..-1, N2..N2+2):
@@ -1236,6 +1241,127 @@ __global__ void k_bx2(KParams P, double* __restrict__ pv, DevScalars* __restrict
       dim3(gridDim.x, gridDim.y, gridDim.z));
 }
 
+// bound_prim() in one launch, equal to k_bx1 then k_bx2.  Every ghost zone
+// is computed from the interior by composing the rules the reference applies
+// in turn (a corner: the x2 rule applied to the x1 rule's value).  POLEDEATH
+// zeroes U2,B2 of the interior pole rows, which the x1 rule must read first:
+// the x1 ghosts of a pole row, and the corners reflecting them, are filled
+// by one thread per pole row, which zeroes its source zones after reading
+// them all.  Every other reader of a pole row is an x2 ghost, which
+// multiplies the same U2,B2 by 0 itself, so it gets the same signed zero
+// whether it reads before or after the zeroing.
+// Threads: [x1 ghosts 2*NG*N2][x2 ghosts 2*NG*(N1+2NG)][pole-row x1 ghosts 2][pole rows 2*N1]
+__host__ __device__ __forceinline__ int bound_threads(const KParams& P) {
+  return 2 * NG * P.n2 + 2 * NG * (P.n1 + 2 * NG) + 2 + 2 * P.n1;
+}
+
+// interior zone i of a pole row is read by the x1 rule (a pole-row thread's source)
+__device__ __forceinline__ bool x1_reads(const KParams& P, int i) {
+  const int bdn = P.bc[HARM_X1DN], bup = P.bc[HARM_X1UP];
+  if (bdn == HARM_OUTFLOW && i == 0) return true;
+  if (bup == HARM_OUTFLOW && i == P.n1 - 1) return true;
+  if (bdn == HARM_PERIODIC && i >= P.n1 - NG) return true;
+  if (bup == HARM_PERIODIC && i < NG) return true;
+  return false;
+}
+
+__device__ __forceinline__ void bound_fused_body(KParams P, double* __restrict__ pv, const double* __restrict__ gC,
+                                                 DevScalars* __restrict__ ds, int reset_list, int do_dt, int t) {
+  if (t == 0) {
+    if (reset_list) ds->nlist = 0;
+    if (do_dt) dt_update(P, ds);
+  }
+  const int NI = P.n1 + 2 * NG;
+  const bool pdn = P.poledeath && P.bc[HARM_X2DN] == HARM_POLARAXIS;
+  const bool pup = P.poledeath && P.bc[HARM_X2UP] == HARM_POLARAXIS;
+  int fail = 0;
+  const int nA = 2 * NG * P.n2, nB = 2 * NG * NI;
+  if (t < nA) {  // x1 ghosts off the pole rows
+    const int gi = t / P.n2, j = t % P.n2;
+    if (!((j == 0 && pdn) || (j == P.n2 - 1 && pup))) bound_x1_cell(P, pv, gC, gi, j, fail);
+  } else if ((t -= nA) < nB) {  // x2 ghosts
+    const int i = t % NI - NG, gj = t / NI;
+    const bool dn = gj < NG;
+    const int j = dn ? (gj - NG) : (P.n2 + gj - NG);
+    const int bc = dn ? P.bc[HARM_X2DN] : P.bc[HARM_X2UP];
+    int js;
+    if (bc == HARM_PERIODIC) {
+      js = dn ? j + P.n2 : j - P.n2;
+    } else if (bc == HARM_POLARAXIS) {
+      js = dn ? (0 + (0 - j - 1)) : ((P.n2 - 1) + ((P.n2 - 1) - j + 1));
+    } else {
+      js = -1000;
+    }
+    const bool zero = (bc == HARM_POLARAXIS) && P.poledeath;
+    if (js != -1000) {
+      double v[NPR];
+      bool done = true;
+      if (i >= 0 && i < P.n1) {
+        load8(pv, zidx(P, i, js), v);
+      } else if (zero && js == (dn ? 0 : P.n2 - 1)) {
+        done = false;  // a pole-row thread fills it
+      } else {
+        const int xb = (i < 0) ? P.bc[HARM_X1DN] : P.bc[HARM_X1UP];
+        if (xb != HARM_OUTFLOW && xb != HARM_PERIODIC) {
+          done = false;
+        } else {
+          load8(pv, zidx(P, x1_source(P, i), js), v);
+          x1_value(P, gC, i, js, v, fail);
+        }
+      }
+      if (done) {
+        if (zero) {
+          v[U2] *= 0.;
+          v[B2] *= 0.;
+        }
+        store8(pv, zidx(P, i, j), v);
+      }
+    }
+  } else if ((t -= nB) < 2) {  // x1 ghosts of a pole row and the corners reflecting them
+    if (!(t == 0 ? pdn : pup)) return;
+    const int jp = (t == 0) ? 0 : P.n2 - 1, jc = (t == 0) ? -1 : P.n2;
+    const bool xok[2] = {P.bc[HARM_X1DN] == HARM_OUTFLOW || P.bc[HARM_X1DN] == HARM_PERIODIC,
+                         P.bc[HARM_X1UP] == HARM_OUTFLOW || P.bc[HARM_X1UP] == HARM_PERIODIC};
+    double src[2 * NG][NPR];  // g < NG: i = -1-g; else i = N1+g-NG
+#pragma unroll
+    for (int g = 0; g < 2 * NG; g++) {
+      const int i = (g < NG) ? -1 - g : P.n1 + g - NG;
+      if (xok[g < NG ? 0 : 1]) load8(pv, zidx(P, x1_source(P, i), jp), src[g]);
+    }
+#pragma unroll
+    for (int g = 0; g < 2 * NG; g++) {
+      const int i = (g < NG) ? -1 - g : P.n1 + g - NG;
+      if (!xok[g < NG ? 0 : 1]) continue;
+      x1_value(P, gC, i, jp, src[g], fail);
+      src[g][U2] *= 0.;
+      src[g][B2] *= 0.;
+      store8(pv, zidx(P, i, jp), src[g]);
+      store8(pv, zidx(P, i, jc), src[g]);
+    }
+#pragma unroll
+    for (int g = 0; g < 2 * NG; g++) {
+      const int i = (g < NG) ? -1 - g : P.n1 + g - NG;
+      if (!xok[g < NG ? 0 : 1]) continue;
+      double* a = pv + (size_t)zidx(P, x1_source(P, i), jp) * 8;
+      a[U2] *= 0.;
+      a[B2] *= 0.;
+    }
+  } else if ((t -= 2) < 2 * P.n1) {  // POLEDEATH of the other interior pole-row zones
+    const bool dn = t < P.n1;
+    const int i = t % P.n1;
+    if (!(dn ? pdn : pup) || x1_reads(P, i)) return;
+    double* a = pv + (size_t)zidx(P, i, dn ? 0 : P.n2 - 1) * 8;
+    a[U2] *= 0.;
+    a[B2] *= 0.;
+  }
+  if (fail) atomicOr(&ds->failed, 1);
+}
+
+__global__ __launch_bounds__(256) void k_bound(KParams P, double* __restrict__ pv, const double* __restrict__ gC,
+                                               DevScalars* __restrict__ ds, int reset_list, int do_dt) {
+  bound_fused_body(P, pv, gC, ds, reset_list, do_dt, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
 // fixup_utoprim() over the failed-zone list: compute, then apply
 __device__ __forceinline__ void fixlist_body(KParams P, const double* __restrict__ pv, const int* __restrict__ pflag,
                           const int* __restrict__ cs, const int* __restrict__ faillist, double* __restrict__ pfix,
@@ -1278,6 +1404,31 @@ __global__ void k_fixapply(KParams P, double* __restrict__ pv, int* __restrict__
                            const DevScalars* __restrict__ ds) {
   fixapply_body(P, pv, pflag, faillist, pfix, ds, dim3(blockIdx.x, blockIdx.y, blockIdx.z),
       dim3(gridDim.x, gridDim.y, gridDim.z));
+}
+
+// fixup_utoprim over the whole failed-zone list in one workgroup: every
+// repaired state from the unrepaired array, a barrier, then all written (the
+// reference reads its copy prc, fixup.c:527-920), then the list is cleared
+// and, after the last substage without a second bound_prim, the t,dt update
+__global__ __launch_bounds__(1024) void k_fixall(KParams P, double* __restrict__ pv, int* __restrict__ pflag,
+                                                 const int* __restrict__ cs, const int* __restrict__ faillist,
+                                                 double* __restrict__ pfix, const double* __restrict__ gC,
+                                                 DevScalars* __restrict__ ds, int do_dt) {
+  int n = ds->nlist, fail = 0;
+  if (n > P.n1 * P.n2) n = P.n1 * P.n2;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) fixup_zone(P, pv, pflag, cs, pfix, gC, faillist[t], fail);
+  __syncthreads();
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    const int z = faillist[t];
+#pragma unroll
+    for (int k = 0; k <= U3; k++) pv[(size_t)z * 8 + k] = pfix[(size_t)z * 8 + k];
+    pflag[z] = -1;  // UTOPRIMFAILFIXED (global.h:524)
+  }
+  if (fail) atomicOr(&ds->failed, 1);
+  if (threadIdx.x == 0) {  // every thread read nlist before the barrier
+    ds->nlist = 0;
+    if (do_dt) dt_update(P, ds);
+  }
 }
 
 __global__ void k_utoprim_batch(KParams P, int n, const int* __restrict__ zone, const double* __restrict__ U,
@@ -1915,8 +2066,13 @@ static int launch_boundary(harm_ctx* c, int sel, int mode) {
                        c->faillist, c->pfix, c->ds, mode);
   } else {
     const dim3 gx1((c->kp.n2 + 63) / 64, 2 * NG), gx2((c->kp.n1 + 2 * NG + 63) / 64, 2 * NG + 2);
+    const dim3 gb((bound_threads(c->kp) + 255) / 256);
     const bool bound2 = (mode & KB_BOUND2) != 0, fix = (mode & KB_FIXUP) != 0;
-    if (mode & KB_BOUND) {
+    const bool fused = (c->bound_variant == HARM_BOUND_FUSED);
+    if ((mode & KB_BOUND) && fused) {
+      hipLaunchKernelGGL(k_bound, gb, dim3(256), 0, c->stream, c->kp, c->arr[sel], c->gC, c->ds, 0,
+                         (!fix && !bound2 && (mode & KB_DT)) ? 1 : 0);
+    } else if (mode & KB_BOUND) {
       hipLaunchKernelGGL(k_bx1, gx1, dim3(64), 0, c->stream, c->kp, c->arr[sel], c->gC, c->ds, 0);
       hipLaunchKernelGGL(k_bx2, gx2, dim3(64), 0, c->stream, c->kp, c->arr[sel], c->ds,
                          (!fix && !bound2 && (mode & KB_DT)) ? 1 : 0);
@@ -1934,16 +2090,25 @@ static int launch_boundary(harm_ctx* c, int sel, int mode) {
                          c->gammacheck, c->csB, c->ds, c->bound_variant == HARM_BOUND_KERNELS_SEQCHECK ? 1 : 0);
       hipLaunchKernelGGL(k_cs_apply, gi, b, 0, c->stream, c->kp, c->csB, c->pflag, c->faillist, c->ds);
     }
-    if (fix) {
+    const bool fixone = (c->bound_variant == HARM_BOUND_FIXONE) || fused;
+    if (fix && fixone) {  // list repaired and cleared by one workgroup
+      hipLaunchKernelGGL(k_fixall, dim3(1), dim3(1024), 0, c->stream, c->kp, c->arr[sel], c->pflag,
+                         c->kp.checksolution ? c->csB : nullptr, c->faillist, c->pfix, c->gC, c->ds,
+                         (!bound2 && (mode & KB_DT)) ? 1 : 0);
+    } else if (fix) {
       hipLaunchKernelGGL(k_fixlist, dim3(32), dim3(64), 0, c->stream, c->kp, c->arr[sel], c->pflag,
                          c->kp.checksolution ? c->csB : nullptr, c->faillist, c->pfix, c->gC, c->ds);
       hipLaunchKernelGGL(k_fixapply, dim3(32), dim3(64), 0, c->stream, c->kp, c->arr[sel], c->pflag, c->faillist,
                          c->pfix, c->ds);
     }
-    if (bound2) {
-      hipLaunchKernelGGL(k_bx1, gx1, dim3(64), 0, c->stream, c->kp, c->arr[sel], c->gC, c->ds, fix ? 1 : 0);
+    if (bound2 && fused) {
+      hipLaunchKernelGGL(k_bound, gb, dim3(256), 0, c->stream, c->kp, c->arr[sel], c->gC, c->ds, 0,
+                         (mode & KB_DT) ? 1 : 0);
+    } else if (bound2) {
+      hipLaunchKernelGGL(k_bx1, gx1, dim3(64), 0, c->stream, c->kp, c->arr[sel], c->gC, c->ds,
+                         (fix && !fixone) ? 1 : 0);
       hipLaunchKernelGGL(k_bx2, gx2, dim3(64), 0, c->stream, c->kp, c->arr[sel], c->ds, (mode & KB_DT) ? 1 : 0);
-    } else if (fix) {
+    } else if (fix && !fixone) {
       hipLaunchKernelGGL(k_tail, dim3(1), dim3(1), 0, c->stream, c->kp, c->ds, (mode & KB_DT) ? 1 : 0);
     }
   }
@@ -2174,8 +2339,9 @@ static int autotune(harm_ctx* c) {
   float best = 1e30f;
   int bf = HARM_FLUX_FACE_64x4, bb = HARM_BOUND_KERNELS;
   const int fv[3] = {HARM_FLUX_FACE_64x4, HARM_FLUX_FACE_64x2, HARM_FLUX_PAIR};
-  const int bv[2] = {HARM_BOUND_KERNELS, HARM_BOUND_BLOCK};
-  const int nbv = c->kp.checksolution ? 1 : 2;  // the one-workgroup pass has no fixup_checksolution
+  // the one-workgroup pass has no fixup_checksolution: it is tried last
+  const int bv[4] = {HARM_BOUND_KERNELS, HARM_BOUND_FIXONE, HARM_BOUND_FUSED, HARM_BOUND_BLOCK};
+  const int nbv = c->kp.checksolution ? 3 : 4;
   for (int f = 0; f < 3 && !rc; f++)
     for (int v = 0; v < nbv && !rc; v++) {
       float ms = 0.f;
@@ -2294,7 +2460,7 @@ extern "C" int harm_prof_reset(harm_ctx* c) {
 extern "C" int harm_set_tuning(harm_ctx* c, int flux_variant, int bound_variant) {
   if (!c) return seterr("null ctx");
   if (flux_variant < 0 || flux_variant > HARM_FLUX_PAIR) return seterr("bad flux variant %d", flux_variant);
-  if (bound_variant < 0 || bound_variant > HARM_BOUND_KERNELS_SEQCHECK)
+  if (bound_variant < 0 || bound_variant > HARM_BOUND_FUSED)
     return seterr("bad boundary variant %d", bound_variant);
   c->flux_variant = flux_variant;
   c->bound_variant = bound_variant;

@@ -205,6 +205,10 @@ int harm_prof_enable(harm_ctx *ctx, int on);
                                   (not with checksolution: wide launches) */
 #define HARM_BOUND_KERNELS_SEQCHECK 2  /* wide launches, fixup_checksolution
                                           always by its sequential loop */
+#define HARM_BOUND_FIXONE 3    /* wide ghost fills, the failed-zone repair
+                                  in one workgroup (compute, barrier, write) */
+#define HARM_BOUND_FUSED 4     /* each bound_prim() in one launch, the repair
+                                  in one workgroup */
 int harm_set_tuning(harm_ctx *ctx, int flux_variant, int bound_variant);
 int harm_get_tuning(harm_ctx *ctx, int *flux_variant, int *bound_variant);
 int harm_prof_get(harm_ctx *ctx, int kernel, double *total_ms, long long *count);

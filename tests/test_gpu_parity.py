@@ -223,7 +223,7 @@ def test_gpu_fullsize_ot_invariants():
 
 
 @pytest.mark.parametrize("fv", [H.FLUX_FACE_64x4, H.FLUX_FACE_64x2, H.FLUX_PAIR])
-@pytest.mark.parametrize("bv", [H.BOUND_KERNELS, H.BOUND_BLOCK])
+@pytest.mark.parametrize("bv", [H.BOUND_KERNELS, H.BOUND_BLOCK, H.BOUND_FIXONE, H.BOUND_FUSED])
 def test_gpu_variants_match_oracle(fv, bv):
     say(f"[case] kernel variants flux={fv} bound={bv}: 32x64 torus, 30 steps (with repaired inversions)")
     g, geom, p0, prm = problem("ks", 32, 64)
@@ -244,6 +244,37 @@ def test_gpu_variants_match_oracle(fv, bv):
     assert nfail > 0, "the run is meant to exercise the fixup of failed inversions"
 
 
+@pytest.mark.parametrize("case", ["mink", "ffde", "ks_x1periodic_5", "ks_x1periodic_12", "ks_nopoledeath"])
+@pytest.mark.parametrize("bv", [H.BOUND_FUSED, H.BOUND_FIXONE])
+def test_gpu_boundary_variants_other_boundaries(case, bv):
+    """The one-launch ghost fill on the other boundary combinations: periodic
+    both ways, polar without POLEDEATH, and periodic x1 with polar x2 and
+    POLEDEATH (a pole row's x1 ghosts read the far side), including a
+    5-zone x1 extent where the two sides' source zones overlap."""
+    say(f"[case] boundary variant {bv} on {case}, 8 steps bitwise vs oracle")
+    if case == "mink":
+        g, geom, p0, prm = problem("mink", 32, 32, lim=H.MC)
+    elif case == "ffde":
+        g, geom, p0, prm = problem("ffde", 16, 32, lim=H.MC, fluxmethod=H.LAXFFLUX, timeorder=2)
+    elif case.startswith("ks_x1periodic"):
+        n1 = int(case.rsplit("_", 1)[1])
+        g, geom, p0, prm = problem("ks", n1, 32)
+        prm.bc[H.X1UP] = prm.bc[H.X1DN] = H.PERIODIC
+    else:
+        g, geom, p0, prm = problem("ks", 32, 64)
+        prm.poledeath = 0
+    dt0 = 1e-4 if case == "mink" else 1e-3
+    h = gpu_ctx(geom, prm, p0, dt0)
+    h.set_tuning(H.FLUX_FACE_64x4, bv)
+    o = oracle_ctx(geom, prm, p0, dt0)
+    for s in range(8):
+        h.step_ch()
+        o.step()
+        assert h.get_time() == o.get_time(), f"step {s + 1}"
+        assert same(h.get_array(H.P), o.get_array(0)), f"step {s + 1}"
+    assert h.nstroke() == o.nstroke()
+
+
 def test_gpu_autotune_picks_a_variant():
     say("[case] autotuner on the 64x64 torus")
     g, geom, p0, prm = problem("ks", 64, 64)
@@ -252,7 +283,7 @@ def test_gpu_autotune_picks_a_variant():
     h.step_n(3)
     fv, bv = h.get_tuning()
     say(f"  chosen flux={fv} bound={bv}")
-    assert fv in (H.FLUX_FACE_64x4, H.FLUX_FACE_64x2, H.FLUX_PAIR) and bv in (H.BOUND_KERNELS, H.BOUND_BLOCK)
+    assert fv in (H.FLUX_FACE_64x4, H.FLUX_FACE_64x2, H.FLUX_PAIR) and bv in (H.BOUND_KERNELS, H.BOUND_BLOCK, H.BOUND_FIXONE, H.BOUND_FUSED)
     for _ in range(3):
         o.step()
     assert same(h.get_array(H.P), o.get_array(0))
@@ -260,7 +291,7 @@ def test_gpu_autotune_picks_a_variant():
 
 
 @pytest.mark.parametrize("fv", [H.FLUX_FACE_64x4, H.FLUX_PAIR])
-@pytest.mark.parametrize("bv", [H.BOUND_KERNELS, H.BOUND_KERNELS_SEQCHECK])
+@pytest.mark.parametrize("bv", [H.BOUND_KERNELS, H.BOUND_KERNELS_SEQCHECK, H.BOUND_FIXONE, H.BOUND_FUSED])
 def test_gpu_checksolution_matches_oracle(fv, bv):
     say(f"[case] CHECKSOLUTION: field-threaded torus 32x64, 25 steps, flux={fv} bound={bv}")
     g, geom, p0, prm = problem("ks_mono", 32, 64)
