@@ -168,14 +168,17 @@ def adjoint_measure(n=256, nsteps=256, K=16, seed=3):
     J = magnetic energy at the end, gradients with respect to the field
     amplitude B0, the velocity amplitude and a seeded Gaussian density
     perturbation field (sigma 1e-3, also part of the state).  Reports the
-    cost of forward + reverse sweep against the plain step, and the adjoint
+    cost of forward + reverse sweep against the plain step (steady state: step
+    graphs captured and kernel variants chosen before the timed sweep), and the adjoint
     against central finite differences (h = 1e-6, same time steps)."""
     import numpy as np
     import harm_ramt_amd as H
     g = H.grid_cartesian(n, n)
     geom = H.Geometry(g)
-    prm = H.make_params(n, n, g.dx1, g.dx2, lim=H.MC, fluxmethod=H.HLLFLUX, timeorder=4, gam=5.0 / 3.0, cour=0.8,
-                        bc=(H.PERIODIC,) * 4, fixupflux=0, poledeath=0)
+    def params(lim):
+        return H.make_params(n, n, g.dx1, g.dx2, lim=lim, fluxmethod=H.HLLFLUX, timeorder=4, gam=5.0 / 3.0,
+                             cour=0.8, bc=(H.PERIODIC,) * 4, fixupflux=0, poledeath=0)
+    prm = params(H.MC)
     rng = np.random.default_rng(seed)
     drho = np.zeros((n + 6, n + 6))
     drho[3:-3, 3:-3] = rng.normal(0, 1e-3, (n, n))
@@ -197,8 +200,8 @@ def adjoint_measure(n=256, nsteps=256, K=16, seed=3):
     B0, vamp = 1.0 / np.sqrt(4 * np.pi), 0.5
     p0 = state(B0, vamp)
 
-    def ctx(p):
-        h = H.Harm(prm, geom)
+    def ctx(p, pr=None):
+        h = H.Harm(pr or prm, geom)
         h.set_array(H.P, p)
         h.bound_prim(H.P)
         h.set_time(0.0, 1e-4, 1e30)
@@ -208,19 +211,25 @@ def adjoint_measure(n=256, nsteps=256, K=16, seed=3):
     h = ctx(p0)
     h.step_n(3)
     h.get_time()
+    tuning = h.get_tuning()
     t0 = time.perf_counter()
     h.step_n(nsteps)
     h.get_time()
     t_plain = (time.perf_counter() - t0) / nsteps
     h.close()
-    # adjoint: forward with checkpoints, then the reverse sweep
+    # adjoint: forward with checkpoints, then the reverse sweep, in steady
+    # state: one untimed sweep first captures the step graphs (the kernel
+    # variants are the plain run's), then the state is reset and timed
     h = ctx(p0)
+    h.set_tuning(*tuning)
     a = H.Adjoint(h, nsteps, K)
-    a.forward()                                          # warm-up, records dt
-    a.close()
-    h.close()
-    h = ctx(p0)
-    a = H.Adjoint(h, nsteps, K)
+    a.forward()
+    a.seed_magnetic_energy()
+    a.backward()
+    h.set_array(H.P, p0)
+    h.bound_prim(H.P)
+    h.set_time(0.0, 1e-4, 1e30)
+    h.get_time()
     t0 = time.perf_counter()
     a.forward()
     t_fwd = time.perf_counter() - t0
@@ -233,8 +242,8 @@ def adjoint_measure(n=256, nsteps=256, K=16, seed=3):
     a.close()
     h.close()
 
-    def J_of(p):
-        hh = ctx(p)
+    def J_of(p, pr, dts):
+        hh = ctx(p, pr)
         hh.set_time(0.0, dts[0], 1e30)
         aa = H.Adjoint(hh, nsteps, nsteps)
         aa.forward(dts)
@@ -244,19 +253,41 @@ def adjoint_measure(n=256, nsteps=256, K=16, seed=3):
         return val
 
     hh = 1e-6
-    checks = {}
     w = np.zeros_like(p0)
     w[3:-3, 3:-3, 0] = rng.normal(0, 1, (n, n))
-    for name, plus, minus, dpdq in (
-            ("B0", state(B0 + hh, vamp), state(B0 - hh, vamp), (state(B0 + hh, vamp) - state(B0 - hh, vamp)) / (2 * hh)),
-            ("v_amplitude", state(B0, vamp + hh), state(B0, vamp - hh),
-             (state(B0, vamp + hh) - state(B0, vamp - hh)) / (2 * hh)),
-            ("rho_perturbation_direction", p0 + hh * w, p0 - hh * w, w)):
-        ad = float((lam * dpdq).sum())
-        fd = (J_of(plus) - J_of(minus)) / (2 * hh)
-        scale = float(np.abs(lam * dpdq).sum())
-        checks[name] = {"adjoint": ad, "finite_difference": fd, "abs_diff_over_term_scale": abs(ad - fd) / scale,
-                        "rel_diff": abs(ad - fd) / abs(fd)}
+
+    def checks_of(lam, pr, dts):
+        out = {}
+        for name, plus, minus, dpdq in (
+                ("B0", state(B0 + hh, vamp), state(B0 - hh, vamp),
+                 (state(B0 + hh, vamp) - state(B0 - hh, vamp)) / (2 * hh)),
+                ("v_amplitude", state(B0, vamp + hh), state(B0, vamp - hh),
+                 (state(B0, vamp + hh) - state(B0, vamp - hh)) / (2 * hh)),
+                ("rho_perturbation_direction", p0 + hh * w, p0 - hh * w, w)):
+            ad = float((lam * dpdq).sum())
+            fd = (J_of(plus, pr, dts) - J_of(minus, pr, dts)) / (2 * hh)
+            scale = float(np.abs(lam * dpdq).sum())
+            out[name] = {"adjoint": ad, "finite_difference": fd, "abs_diff_over_term_scale": abs(ad - fd) / scale,
+                         "rel_diff": abs(ad - fd) / abs(fd)}
+        return out
+
+    checks = checks_of(lam, prm, dts)
+    # MC's slope switches are kinks of J: finite differences straddle them.
+    # The same checks with a smooth limiter (NLIMCENT), where central
+    # differences converge to the derivative
+    prm_s = params(H.NLIMCENT)
+    hs = ctx(p0, prm_s)
+    as_ = H.Adjoint(hs, nsteps, K)
+    as_.forward()
+    as_.seed_magnetic_energy()
+    lam_s = as_.backward()
+    dts_s = as_.dts()
+    ns_s = as_.nonsmooth()
+    as_.close()
+    hs.close()
+    checks_smooth = checks_of(lam_s, prm_s, dts_s)
+    checks_smooth["limiter"] = "NLIMCENT"
+    checks_smooth["nonsmooth_points"] = ns_s[0]
     return {"workload": f"Orszag-Tang {n}x{n}, Minkowski, periodic, MC+HLL+RK4, flux-CT, {nsteps} steps, "
                         f"checkpoints every {K}, J = magnetic energy at the end",
             "J": J, "plain_ms_per_step": 1e3 * t_plain,
@@ -264,7 +295,7 @@ def adjoint_measure(n=256, nsteps=256, K=16, seed=3):
             "adjoint_cost_vs_plain_step": (t_fwd + t_bwd) / nsteps / t_plain,
             "reverse_sweep_cost_vs_plain_step": t_bwd / nsteps / t_plain,
             "nonsmooth_points": nonsmooth[0], "failed_inversions_in_recompute": nonsmooth[1],
-            "gradient_checks": checks}
+            "gradient_checks": checks, "gradient_checks_smooth_limiter": checks_smooth}
 
 
 def copy_calibration(mib):

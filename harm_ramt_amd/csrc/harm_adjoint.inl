@@ -25,10 +25,44 @@ __device__ __forceinline__ double adj_ld(const KParams& P, const double* ldU, in
   return (i >= 0 && i < P.n1 && j >= 0 && j < P.n2) ? ldU[(size_t)zidx(P, i, j) * 8 + k] : 0.;
 }
 
-// dt of step n: record (before the step) or impose
-__global__ void k_adj_dt(DevScalars* ds, double* dts, int n, int impose) {
+// The adjoint's steps are replayed as HIP graphs; the step index lives on
+// the device (ctr[0] = step n, ctr[1] = first step of the segment), so one
+// captured graph serves every step.
+
+// dt of step ctr[0]: record (before the step) or impose
+__global__ void k_adj_dt(DevScalars* ds, double* dts, const int* ctr, int impose) {
+  const int n = ctr[0];
   if (impose) ds->dt = dts[n];
   else dts[n] = ds->dt;
+}
+
+__global__ void k_adj_setctr(int* ctr, int n, int n0) {
+  ctr[0] = n;
+  ctr[1] = n0;
+}
+
+__global__ void k_adj_next(int* ctr, int delta) { ctr[0] += delta; }
+
+// substage state s of step ctr[0] within its segment's store
+__device__ __forceinline__ size_t adj_state(const int* ctr, int nto, int s, size_t nz8) {
+  return ((size_t)(ctr[0] - ctr[1]) * nto + s) * nz8;
+}
+
+// store the state after substage `to` (state to+1 of the step)
+__global__ void k_adj_store(const double* __restrict__ src, double* __restrict__ seg, const int* __restrict__ ctr,
+                            int nto, int to, size_t nz8) {
+  double* dst = seg + adj_state(ctr, nto, to + 1, nz8);
+  const size_t n2 = nz8 / 2;
+  const double2* s2 = reinterpret_cast<const double2*>(src);
+  double2* d2 = reinterpret_cast<double2*>(dst);
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n2; t += (size_t)gridDim.x * blockDim.x)
+    d2[t] = s2[t];
+}
+
+// slot[0] = pb of substage s (the state before it), slot[1] = its pf
+__global__ void k_adj_slot(const double* seg, const int* ctr, int nto, int s, size_t nz8, const double** slot) {
+  slot[0] = seg + adj_state(ctr, nto, s, nz8);
+  slot[1] = slot[0] + nz8;
 }
 
 // periodic bound_prim transposed: x2 ghosts (all i) fold into their sources,
@@ -100,12 +134,15 @@ __global__ void k_adj_seed_bsq(KParams P, const double* p, const double* gC, dou
 }
 
 // inversion (implicit-function theorem) + RK combination, interior zones
-__global__ void k_adj_zone(KParams P, AdjRK R, const double* __restrict__ pf, const double* __restrict__ gC,
+__global__ void k_adj_zone(KParams P, AdjRK R, const double* const* __restrict__ slot, const double* __restrict__ dts,
+                           const int* __restrict__ ctr, const double* __restrict__ gC,
                            const double* __restrict__ lp, double* __restrict__ lunew, double* __restrict__ lulast,
                            double* __restrict__ lUi, double* __restrict__ ldU, int* __restrict__ nonsmooth) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= P.n1 || j >= P.n2) return;
+  const double* __restrict__ pf = slot[1];
+  R.dt = dts[ctr[0]];
   const int z = zidx(P, i, j);
   double gc[36];
 #pragma unroll
@@ -208,11 +245,12 @@ __device__ __forceinline__ void adj_face(const KParams& P, const double* __restr
   store8(lf, 2 * z0 + 1, prb);
 }
 
-__global__ __launch_bounds__(256) void k_adj_face(KParams P, const double* __restrict__ pb,
+__global__ __launch_bounds__(256) void k_adj_face(KParams P, const double* const* __restrict__ slot,
                                                   const double* __restrict__ gF, const double* __restrict__ gC,
                                                   const double* __restrict__ ldU, const double* __restrict__ lemf,
                                                   double* __restrict__ lf1, double* __restrict__ lf2,
                                                   int* __restrict__ nonsmooth) {
+  const double* __restrict__ pb = slot[0];
   const int dir = 1 + blockIdx.z;
   const int jf = blockIdx.x * blockDim.x + threadIdx.x;
   const int ifc = blockIdx.y * blockDim.y + threadIdx.y;
@@ -231,10 +269,11 @@ __global__ __launch_bounds__(256) void k_adj_face(KParams P, const double* __res
 // gather: adjoint of every zone of pb from the face-state adjoints of the
 // faces whose reconstruction stencil holds it (pl(f) = p(f-1) + dq(f-1)/2,
 // pr(f) = p(f) - dq(f)/2, dq on the limiter's branch), plus the source term
-__global__ __launch_bounds__(256) void k_adj_gather(KParams P, const double* __restrict__ pb,
+__global__ __launch_bounds__(256) void k_adj_gather(KParams P, const double* const* __restrict__ slot,
                                                     const double* __restrict__ gC, const double* __restrict__ gConn,
                                                     const double* __restrict__ ldU, const double* __restrict__ lf1,
                                                     const double* __restrict__ lf2, double* __restrict__ lp) {
+  const double* __restrict__ pb = slot[0];
   const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x) - NG;
   const int i = (int)(blockIdx.y * blockDim.y + threadIdx.y) - NG;
   if (i > P.n1 + NG - 1 || j > P.n2 + NG - 1) return;
@@ -283,12 +322,13 @@ __global__ __launch_bounds__(256) void k_adj_gather(KParams P, const double* __r
 
 // Ui = U(p_n) transposed into the adjoint of p_n; clears lUi, lunew, lulast
 // (the adjoints of the substage accumulators, which start each step at zero)
-__global__ void k_adj_ui(KParams P, const double* __restrict__ pn, const double* __restrict__ gC,
+__global__ void k_adj_ui(KParams P, const double* const* __restrict__ slot, const double* __restrict__ gC,
                          double* __restrict__ lUi, double* __restrict__ lunew, double* __restrict__ lulast,
                          double* __restrict__ lp) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= P.n1 || j >= P.n2) return;
+  const double* __restrict__ pn = slot[0];
   const int z = zidx(P, i, j);
   double gc[36];
 #pragma unroll
@@ -330,6 +370,10 @@ struct harm_adj {
   int nto;
   double *lp, *lunew, *lulast, *lUi, *ldU, *lemf, *lf1, *lf2, *rowsum;
   int* d_ns;
+  int* d_ctr;                      // step index, first step of the segment
+  const double** d_slot;           // pb and pf of the substage being reversed
+  hipGraphExec_t gx[4];            // steps: record dt, impose dt, recompute, reverse
+  bool gx_ok[4];
   bool forward_done, seeded;
   int recompute_failures;
 };
@@ -349,6 +393,10 @@ static int adj_check(const harm_ctx* c) {
 
 extern "C" int harm_adj_destroy(harm_adj* a) {
   if (!a) return 0;
+  for (int k = 0; k < 4; k++)
+    if (a->gx_ok[k]) (void)hipGraphExecDestroy(a->gx[k]);
+  if (a->d_ctr) (void)hipFree(a->d_ctr);
+  if (a->d_slot) (void)hipFree((void*)a->d_slot);
   if (a->ckpt)
     for (double* pp : *a->ckpt) (void)hipFree(pp);
   double* bufs[] = {a->d_dts, a->pN, a->seg, a->lp, a->lunew, a->lulast, a->lUi, a->ldU, a->lemf, a->lf1, a->lf2,
@@ -396,6 +444,10 @@ extern "C" int harm_adj_create(harm_ctx* c, int nsteps, int checkpoint_every, ha
   if (e == hipSuccess && (e = hipMalloc(&a->lf2, 2 * sb)) == hipSuccess) e = hipMemset(a->lf2, 0, 2 * sb);
   if (e == hipSuccess) e = hipMalloc(&a->rowsum, sizeof(double) * c->kp.n1);
   if (e == hipSuccess && (e = hipMalloc(&a->d_ns, sizeof(int))) == hipSuccess) e = hipMemset(a->d_ns, 0, sizeof(int));
+  if (e == hipSuccess && (e = hipMalloc(&a->d_ctr, 2 * sizeof(int))) == hipSuccess)
+    e = hipMemset(a->d_ctr, 0, 2 * sizeof(int));
+  if (e == hipSuccess && (e = hipMalloc((void**)&a->d_slot, 2 * sizeof(double*))) == hipSuccess)
+    e = hipMemset((void*)a->d_slot, 0, 2 * sizeof(double*));
   if (e != hipSuccess) {
     harm_adj_destroy(a);
     return seterr("harm_adj_create: %s", hipGetErrorString(e));
@@ -405,19 +457,84 @@ extern "C" int harm_adj_create(harm_ctx* c, int nsteps, int checkpoint_every, ha
 }
 
 // one step with the dt of step n imposed (or recorded), eager launches
-static int adj_step(harm_adj* a, int n, int impose, double* store) {
+// one forward step of step ctr[0] (dt recorded or imposed); with store, the
+// state after every substage goes to the segment store; then ctr[0]++
+static int adj_step(harm_adj* a, int impose, bool store) {
   harm_ctx* c = a->c;
-  hipLaunchKernelGGL(k_adj_dt, dim3(1), dim3(1), 0, c->stream, c->ds, a->d_dts, n, impose);
+  hipLaunchKernelGGL(k_adj_dt, dim3(1), dim3(1), 0, c->stream, c->ds, a->d_dts, a->d_ctr, impose);
   double CUf[4][3], Cunew[4][3];
   int pii[4], pbb[4], pff[4], nto;
   rk_setup(c->prm.timeorder, &nto, CUf, Cunew, pii, pbb, pff);
-  const size_t sb = (size_t)c->nz * 8 * sizeof(double);
+  const size_t nz8 = (size_t)c->nz * 8;
   for (int to = 0; to < nto; to++) {
     if (substage(c, pii[to], pbb[to], pff[to], CUf[to], Cunew[to], to, nto)) return 1;
     if (launch_boundary(c, pff[to], KB_BOUND | KB_FIXUP | KB_BOUND2 | ((to == nto - 1) ? KB_DT : 0))) return 1;
-    if (store) HIPCHK(hipMemcpyAsync(store + (size_t)(to + 1) * c->nz * 8, c->arr[pff[to]], sb,
-                                     hipMemcpyDeviceToDevice, c->stream));
+    if (store)
+      hipLaunchKernelGGL(k_adj_store, dim3(512), dim3(256), 0, c->stream, c->arr[pff[to]], a->seg, a->d_ctr, nto, to,
+                         nz8);
   }
+  hipLaunchKernelGGL(k_adj_next, dim3(1), dim3(1), 0, c->stream, a->d_ctr, 1);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+static int adj_boundT(harm_adj* a);
+
+// the reverse of step ctr[0] (its substage states in the segment store),
+// then ctr[0]--
+static int adj_reverse_step(harm_adj* a) {
+  harm_ctx* c = a->c;
+  const size_t nz8 = (size_t)c->nz * 8;
+  double CUf[4][3], Cunew[4][3];
+  int pii[4], pbb[4], pff[4], nto;
+  rk_setup(c->prm.timeorder, &nto, CUf, Cunew, pii, pbb, pff);
+  const dim3 b(64, 4);
+  const dim3 gz = grid2(c->kp.n2, c->kp.n1, b);
+  const dim3 gc2 = grid2(c->kp.n2 + 1, c->kp.n1 + 1, b);
+  const dim3 gf((c->kp.n2 + 2 + 63) / 64, (c->kp.n1 + 2 + 3) / 4, 2);
+  const dim3 gall = grid2(c->kp.n2 + 2 * NG, c->kp.n1 + 2 * NG, b);
+  for (int s = nto - 1; s >= 0; s--) {
+    AdjRK R;
+    R.a0 = CUf[s][0]; R.a1 = CUf[s][1]; R.a2 = CUf[s][2];
+    R.b0 = Cunew[s][0]; R.b1 = Cunew[s][1]; R.b2 = Cunew[s][2];
+    R.dt = 0.;  // read from the dt sequence on the device
+    R.last = (s == nto - 1);
+    if (adj_boundT(a)) return 1;
+    hipLaunchKernelGGL(k_adj_slot, dim3(1), dim3(1), 0, c->stream, a->seg, a->d_ctr, nto, s, nz8, a->d_slot);
+    hipLaunchKernelGGL(k_adj_zone, gz, b, 0, c->stream, c->kp, R, a->d_slot, a->d_dts, a->d_ctr, c->gC, a->lp,
+                       a->lunew, a->lulast, a->lUi, a->ldU, a->d_ns);
+    if (c->prm.fluxb == HARM_FLUXCTTOTH)
+      hipLaunchKernelGGL(k_adj_emf, gc2, b, 0, c->stream, c->kp, a->ldU, a->lemf);
+    hipLaunchKernelGGL(k_adj_face, gf, b, 0, c->stream, c->kp, a->d_slot, c->gF, c->gC, a->ldU, a->lemf, a->lf1,
+                       a->lf2, a->d_ns);
+    hipLaunchKernelGGL(k_adj_gather, gall, b, 0, c->stream, c->kp, a->d_slot, c->gC, c->gConn, a->ldU, a->lf1,
+                       a->lf2, a->lp);
+    if (s == 0)
+      hipLaunchKernelGGL(k_adj_ui, gz, b, 0, c->stream, c->kp, a->d_slot, c->gC, a->lUi, a->lunew, a->lulast, a->lp);
+  }
+  hipLaunchKernelGGL(k_adj_next, dim3(1), dim3(1), 0, c->stream, a->d_ctr, -1);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// graph k: 0 forward step recording dt, 1 forward step imposing dt,
+// 2 recompute step (imposed dt, states stored), 3 reverse step
+static int adj_graph(harm_adj* a, int k) {
+  if (a->gx_ok[k]) return 0;
+  harm_ctx* c = a->c;
+  hipGraph_t g;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = (k == 3) ? adj_reverse_step(a) : adj_step(a, k == 0 ? 0 : 1, k == 2);
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) {
+    if (e == hipSuccess) (void)hipGraphDestroy(g);
+    return 1;
+  }
+  if (e != hipSuccess) return seterr("adjoint graph capture: %s", hipGetErrorString(e));
+  e = hipGraphInstantiate(&a->gx[k], g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) return seterr("adjoint graph instantiate: %s", hipGetErrorString(e));
+  a->gx_ok[k] = true;
   return 0;
 }
 
@@ -432,7 +549,9 @@ extern "C" int harm_adj_forward(harm_adj* a, const double* dts) {
     memcpy(a->dts->data(), dts, sizeof(double) * a->nsteps);
     HIPCHK(hipMemcpy(a->d_dts, dts, sizeof(double) * a->nsteps, hipMemcpyHostToDevice));
   }
-  int rc = 0;
+  const int gk = dts ? 1 : 0;
+  int rc = adj_graph(a, gk);
+  if (!rc) hipLaunchKernelGGL(k_adj_setctr, dim3(1), dim3(1), 0, c->stream, a->d_ctr, 0, 0);
   for (int n = 0; n < a->nsteps && !rc; n++) {
     if (n % a->K == 0) {
       const int m = n / a->K;
@@ -440,7 +559,7 @@ extern "C" int harm_adj_forward(harm_adj* a, const double* dts) {
           hipMemcpyAsync(a->d_ckds + m, c->ds, sizeof(DevScalars), hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
         rc = seterr("adjoint forward: checkpoint copy");
     }
-    if (!rc) rc = adj_step(a, n, dts ? 1 : 0, nullptr);
+    if (!rc && hipGraphLaunch(a->gx[gk], c->stream) != hipSuccess) rc = seterr("adjoint forward: graph launch");
   }
   c->prof = was_prof;
   if (rc) return 1;
@@ -510,17 +629,10 @@ extern "C" int harm_adj_backward(harm_adj* a) {
   harm_ctx* c = a->c;
   const bool was_prof = c->prof;
   c->prof = false;
-  const size_t sb = (size_t)c->nz * 8 * sizeof(double), nz8 = (size_t)c->nz * 8;
-  double CUf[4][3], Cunew[4][3];
-  int pii[4], pbb[4], pff[4], nto;
-  rk_setup(c->prm.timeorder, &nto, CUf, Cunew, pii, pbb, pff);
-  const dim3 b(64, 4);
-  const dim3 gz = grid2(c->kp.n2, c->kp.n1, b);
-  const dim3 gc2 = grid2(c->kp.n2 + 1, c->kp.n1 + 1, b);
-  const dim3 gf((c->kp.n2 + 2 + 63) / 64, (c->kp.n1 + 2 + 3) / 4, 2);
-  const dim3 gall = grid2(c->kp.n2 + 2 * NG, c->kp.n1 + 2 * NG, b);
+  const size_t sb = (size_t)c->nz * 8 * sizeof(double);
   HIPCHK(hipMemsetAsync(a->d_ns, 0, sizeof(int), c->stream));
-  int rc = 0;
+  int rc = adj_graph(a, 2);
+  if (!rc) rc = adj_graph(a, 3);
   a->recompute_failures = 0;
   for (int m = a->nseg - 1; m >= 0 && !rc; m--) {
     const int n0 = m * a->K, n1 = (n0 + a->K < a->nsteps) ? n0 + a->K : a->nsteps;
@@ -530,35 +642,16 @@ extern "C" int harm_adj_backward(harm_adj* a) {
     HIPCHK(hipMemcpyAsync(a->seg, c->arr[HARM_P], sb, hipMemcpyDeviceToDevice, c->stream));
     int nf0 = 0, nf1 = 0, dummy = 0;
     if (harm_get_failures(c, &nf0, &dummy)) return 1;
-    for (int n = n0; n < n1 && !rc; n++) rc = adj_step(a, n, 1, a->seg + (size_t)(n - n0) * nto * nz8);
+    hipLaunchKernelGGL(k_adj_setctr, dim3(1), dim3(1), 0, c->stream, a->d_ctr, n0, n0);
+    for (int n = n0; n < n1 && !rc; n++)
+      if (hipGraphLaunch(a->gx[2], c->stream) != hipSuccess) rc = seterr("adjoint recompute: graph launch");
     if (rc) break;
     if (harm_get_failures(c, &nf1, &dummy)) return 1;
     a->recompute_failures += nf1 - nf0;
-    // reverse the steps of the segment
-    for (int n = n1 - 1; n >= n0; n--) {
-      const double* base = a->seg + (size_t)(n - n0) * nto * nz8;  // base[s*nz8] = pb_s, base[(s+1)*nz8] = pf_s
-      for (int s = nto - 1; s >= 0; s--) {
-        AdjRK R;
-        R.a0 = CUf[s][0]; R.a1 = CUf[s][1]; R.a2 = CUf[s][2];
-        R.b0 = Cunew[s][0]; R.b1 = Cunew[s][1]; R.b2 = Cunew[s][2];
-        R.dt = (*a->dts)[n];
-        R.last = (s == nto - 1);
-        const double* pf = base + (size_t)(s + 1) * nz8;
-        const double* pbs = base + (size_t)s * nz8;
-        if (adj_boundT(a)) return 1;
-        hipLaunchKernelGGL(k_adj_zone, gz, b, 0, c->stream, c->kp, R, pf, c->gC, a->lp, a->lunew, a->lulast, a->lUi,
-                           a->ldU, a->d_ns);
-        if (c->prm.fluxb == HARM_FLUXCTTOTH)
-          hipLaunchKernelGGL(k_adj_emf, gc2, b, 0, c->stream, c->kp, a->ldU, a->lemf);
-        hipLaunchKernelGGL(k_adj_face, gf, b, 0, c->stream, c->kp, pbs, c->gF, c->gC, a->ldU, a->lemf, a->lf1,
-                           a->lf2, a->d_ns);
-        hipLaunchKernelGGL(k_adj_gather, gall, b, 0, c->stream, c->kp, pbs, c->gC, c->gConn, a->ldU, a->lf1, a->lf2,
-                           a->lp);
-        if (s == 0)
-          hipLaunchKernelGGL(k_adj_ui, gz, b, 0, c->stream, c->kp, pbs, c->gC, a->lUi, a->lunew, a->lulast, a->lp);
-        HIPCHK(hipGetLastError());
-      }
-    }
+    // reverse the steps of the segment, last first
+    hipLaunchKernelGGL(k_adj_setctr, dim3(1), dim3(1), 0, c->stream, a->d_ctr, n1 - 1, n0);
+    for (int n = n1 - 1; n >= n0 && !rc; n--)
+      if (hipGraphLaunch(a->gx[3], c->stream) != hipSuccess) rc = seterr("adjoint reverse: graph launch");
   }
   c->prof = was_prof;
   if (rc) return 1;

@@ -1246,13 +1246,15 @@ __global__ void k_bx2(KParams P, double* __restrict__ pv, DevScalars* __restrict
 // in turn (a corner: the x2 rule applied to the x1 rule's value).  POLEDEATH
 // zeroes U2,B2 of the interior pole rows, which the x1 rule must read first:
 // the x1 ghosts of a pole row, and the corners reflecting them, are filled
-// by one thread per pole row, which zeroes its source zones after reading
-// them all.  Every other reader of a pole row is an x2 ghost, which
+// by the first 4*NG threads of workgroup 0 (one per pole row and ghost
+// column), which read all their sources, meet at a barrier, and only then
+// zero the sources.  Every other reader of a pole row is an x2 ghost, which
 // multiplies the same U2,B2 by 0 itself, so it gets the same signed zero
 // whether it reads before or after the zeroing.
-// Threads: [x1 ghosts 2*NG*N2][x2 ghosts 2*NG*(N1+2NG)][pole-row x1 ghosts 2][pole rows 2*N1]
+// Threads: [pole-row x1 ghosts 4*NG][x1 ghosts 2*NG*N2][x2 ghosts 2*NG*(N1+2NG)][pole rows 2*N1]
+constexpr int BOUND_NP = 4 * NG;
 __host__ __device__ __forceinline__ int bound_threads(const KParams& P) {
-  return 2 * NG * P.n2 + 2 * NG * (P.n1 + 2 * NG) + 2 + 2 * P.n1;
+  return BOUND_NP + 2 * NG * P.n2 + 2 * NG * (P.n1 + 2 * NG) + 2 * P.n1;
 }
 
 // interior zone i of a pole row is read by the x1 rule (a pole-row thread's source)
@@ -1275,8 +1277,34 @@ __device__ __forceinline__ void bound_fused_body(KParams P, double* __restrict__
   const bool pdn = P.poledeath && P.bc[HARM_X2DN] == HARM_POLARAXIS;
   const bool pup = P.poledeath && P.bc[HARM_X2UP] == HARM_POLARAXIS;
   int fail = 0;
+  // pole-row x1 ghosts and the corners reflecting them (workgroup 0 only;
+  // the barrier is reached by every thread of workgroup 0)
+  if (blockIdx.x == 0) {
+    const int row = threadIdx.x / (2 * NG), g = threadIdx.x % (2 * NG);  // g < NG: i = -1-g, else N1+g-NG
+    const int i = (g < NG) ? -1 - g : P.n1 + g - NG;
+    const int xb = (g < NG) ? P.bc[HARM_X1DN] : P.bc[HARM_X1UP];
+    const bool mine = threadIdx.x < BOUND_NP && (row == 0 ? pdn : pup) &&
+                      (xb == HARM_OUTFLOW || xb == HARM_PERIODIC);
+    const int jp = (row == 0) ? 0 : P.n2 - 1, jc = (row == 0) ? -1 : P.n2;
+    double v[NPR];
+    if (mine) load8(pv, zidx(P, x1_source(P, i), jp), v);
+    __syncthreads();  // every source read before any is zeroed
+    if (mine) {
+      x1_value(P, gC, i, jp, v, fail);
+      v[U2] *= 0.;
+      v[B2] *= 0.;
+      store8(pv, zidx(P, i, jp), v);
+      store8(pv, zidx(P, i, jc), v);
+      double* a = pv + (size_t)zidx(P, x1_source(P, i), jp) * 8;
+      a[U2] *= 0.;
+      a[B2] *= 0.;
+    }
+  }
+  t -= BOUND_NP;
   const int nA = 2 * NG * P.n2, nB = 2 * NG * NI;
-  if (t < nA) {  // x1 ghosts off the pole rows
+  if (t < 0) {
+    // a pole-row thread
+  } else if (t < nA) {  // x1 ghosts off the pole rows
     const int gi = t / P.n2, j = t % P.n2;
     if (!((j == 0 && pdn) || (j == P.n2 - 1 && pup))) bound_x1_cell(P, pv, gC, gi, j, fail);
   } else if ((t -= nA) < nB) {  // x2 ghosts
@@ -1317,36 +1345,7 @@ __device__ __forceinline__ void bound_fused_body(KParams P, double* __restrict__
         store8(pv, zidx(P, i, j), v);
       }
     }
-  } else if ((t -= nB) < 2) {  // x1 ghosts of a pole row and the corners reflecting them
-    if (!(t == 0 ? pdn : pup)) return;
-    const int jp = (t == 0) ? 0 : P.n2 - 1, jc = (t == 0) ? -1 : P.n2;
-    const bool xok[2] = {P.bc[HARM_X1DN] == HARM_OUTFLOW || P.bc[HARM_X1DN] == HARM_PERIODIC,
-                         P.bc[HARM_X1UP] == HARM_OUTFLOW || P.bc[HARM_X1UP] == HARM_PERIODIC};
-    double src[2 * NG][NPR];  // g < NG: i = -1-g; else i = N1+g-NG
-#pragma unroll
-    for (int g = 0; g < 2 * NG; g++) {
-      const int i = (g < NG) ? -1 - g : P.n1 + g - NG;
-      if (xok[g < NG ? 0 : 1]) load8(pv, zidx(P, x1_source(P, i), jp), src[g]);
-    }
-#pragma unroll
-    for (int g = 0; g < 2 * NG; g++) {
-      const int i = (g < NG) ? -1 - g : P.n1 + g - NG;
-      if (!xok[g < NG ? 0 : 1]) continue;
-      x1_value(P, gC, i, jp, src[g], fail);
-      src[g][U2] *= 0.;
-      src[g][B2] *= 0.;
-      store8(pv, zidx(P, i, jp), src[g]);
-      store8(pv, zidx(P, i, jc), src[g]);
-    }
-#pragma unroll
-    for (int g = 0; g < 2 * NG; g++) {
-      const int i = (g < NG) ? -1 - g : P.n1 + g - NG;
-      if (!xok[g < NG ? 0 : 1]) continue;
-      double* a = pv + (size_t)zidx(P, x1_source(P, i), jp) * 8;
-      a[U2] *= 0.;
-      a[B2] *= 0.;
-    }
-  } else if ((t -= 2) < 2 * P.n1) {  // POLEDEATH of the other interior pole-row zones
+  } else if ((t -= nB) < 2 * P.n1) {  // POLEDEATH of the other interior pole-row zones
     const bool dn = t < P.n1;
     const int i = t % P.n1;
     if (!(dn ? pdn : pup) || x1_reads(P, i)) return;
