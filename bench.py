@@ -48,7 +48,7 @@ def kernel_of(stage, flux_variant):
     if stage == "update":
         return "k_update"
     import harm_ramt_amd as H
-    return "k_flux" if flux_variant == H.FLUX_PAIR else "k_flux_single"
+    return {H.FLUX_PAIR: "k_flux", H.FLUX_SHARED: "k_flux_shared"}.get(flux_variant, "k_flux_single")
 
 
 def traffic_from_profiles(kernel, variants):
@@ -398,7 +398,7 @@ def main():
                    "n1": n1, "n2": n2, "timeorder": 4, "lim": "PARA", "flux": "HLL", "t_end": t_now,
                    "dt_end": dt_now, "failed_inversions_repaired": nfail,
                    "kernel_variants": {"flux": {H.FLUX_FACE_64x4: "face_64x4", H.FLUX_FACE_64x2: "face_64x2",
-                                                H.FLUX_PAIR: "lane_pair"}[fv],
+                                                H.FLUX_PAIR: "lane_pair", H.FLUX_SHARED: "zone_shared"}[fv],
                                        "boundary": {H.BOUND_KERNELS: "wide_launches",
                                                     H.BOUND_BLOCK: "one_workgroup",
                                                     H.BOUND_KERNELS_SEQCHECK: "wide_launches_seqcheck",

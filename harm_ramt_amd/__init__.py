@@ -29,7 +29,7 @@ OUTFLOW, POLARAXIS, PERIODIC = 0, 4, 100
 X1UP, X1DN, X2UP, X2DN = 0, 1, 2, 3
 P, PK0, PK1, F1, F2, ULAST, UNEW = 0, 1, 2, 3, 4, 5, 6
 METRIC_MINK, METRIC_KS = 0, 2
-FLUX_FACE_64x4, FLUX_FACE_64x2, FLUX_PAIR = 0, 1, 2
+FLUX_FACE_64x4, FLUX_FACE_64x2, FLUX_PAIR, FLUX_SHARED = 0, 1, 2, 3
 BOUND_KERNELS, BOUND_BLOCK, BOUND_KERNELS_SEQCHECK, BOUND_FIXONE, BOUND_FUSED = 0, 1, 2, 3, 4
 K_FLUX, K_EMF, K_UPDATE, K_BOUNDARY = 0, 2, 3, 4
 STAGE_NAMES = {K_FLUX: "flux", K_EMF: "flux_ct", K_UPDATE: "update", K_BOUNDARY: "boundary"}
@@ -48,6 +48,60 @@ class HarmParams(ctypes.Structure):
         ("rhominlimit", ctypes.c_double), ("uuminlimit", ctypes.c_double),
         ("dx1", ctypes.c_double), ("dx2", ctypes.c_double),
     ]
+
+
+NUMENERREGIONS, NUMDIRS, NUMTSCALES, NUMFAILFLOORFLAGS = 3, 4, 4, 9
+
+
+_d, _l, _i = ctypes.c_double, ctypes.c_long, ctypes.c_int
+
+
+class RestartHeader(ctypes.Structure):
+    """harm_restart_header: the fields of the reference's restart header
+    (restart.c:554-682), named as its globals."""
+    _fields_ = [
+        ("totalsize", _i * 2), ("t", _d), ("tf", _d), ("nstep", _l), ("a", _d), ("gam", _d), ("cour", _d),
+        ("DTd", _d), ("DTavg", _d), ("DTener", _d), ("DTi", _d), ("DTr", _l), ("DTdebug", _d),
+        ("dump_cnt", _l), ("image_cnt", _l), ("rdump_cnt", _l), ("avg_cnt", _l), ("debug_cnt", _l),
+        ("dt", _d), ("lim", _i), ("TIMEORDER", _i), ("fluxmethod", _i), ("FLUXB", _i), ("UTOPRIMVERSION", _i),
+        ("failed", _i), ("R0", _d), ("Rin", _d), ("Rout", _d), ("hslope", _d), ("defcoord", _i),
+        ("BCtype", _i * 4), ("realnstep", _l), ("debugfail", _i), ("whichrestart", _i), ("cooling", _i),
+        ("restartsteps", _l * 2),
+    ] + [(n, _i) for n in ("GAMMIEDUMP", "GAMMIEIMAGE", "GAMMIEENER", "DODIAGS", "DOENERDIAG", "DOGDUMPDIAG",
+                           "DORDUMPDIAG", "DODUMPDIAG", "DOAVGDIAG", "DOIMAGEDIAG", "DOAREAMAPDIAG", "POSDEFMETRIC",
+                           "periodicx1", "periodicx2", "binaryoutput", "sortedoutput")] + [
+        (n, _d) for n in ("defcon", "SAFE", "RHOMIN", "UUMIN", "RHOMINLIMIT", "UUMINLIMIT", "BSQORHOLIMIT",
+                          "BSQOULIMIT", "GAMMAMAX", "GAMMADAMP", "GAMMAFAIL")] + [
+        ("pcumreg_tot", _d * 8 * NUMDIRS * NUMENERREGIONS), ("fladdreg_tot", _d * 8 * NUMENERREGIONS),
+        ("sourceaddreg_tot", _d * 8 * NUMENERREGIONS), ("Ureg_init_tot", _d * 8 * NUMENERREGIONS),
+        ("failfloorcount_tot", _d * NUMFAILFLOORFLAGS * NUMTSCALES), ("prMAX", _d * 8), ("rescaletype", _i),
+        ("lumvsr_tot", ctypes.POINTER(_d)),
+    ]
+
+
+def rdump_write(path, header, p, binary=False):
+    """Write a restart file of the interior of the host array p[N1+6][N2+6][8]
+    (restart_write, restart.c:182).  header: RestartHeader, its lumvsr_tot
+    pointing at N1 doubles or NULL."""
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    n1, n2 = p.shape[0] - 2 * NG, p.shape[1] - 2 * NG
+    rc = load_library().harm_rdump_write_host(os.fsencode(path), int(binary), ctypes.byref(header), n1, n2, _dptr(p))
+    if rc:
+        raise RuntimeError(f"harm_rdump_write_host {path}: error {rc}")
+
+
+def rdump_read(path, n1, n2, binary=False, p=None):
+    """Read a restart file (restart_read, restart.c:234) into the interior of
+    p (zeros when not given); returns (RestartHeader, p, lumvsr)."""
+    h = RestartHeader()
+    lum = np.zeros(n1)
+    h.lumvsr_tot = lum.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    p = np.zeros((n1 + 2 * NG, n2 + 2 * NG, NPR)) if p is None else np.ascontiguousarray(p, dtype=np.float64)
+    rc = load_library().harm_rdump_read_host(os.fsencode(path), int(binary), ctypes.byref(h), n1, n2, _dptr(p))
+    if rc:
+        raise RuntimeError(f"harm_rdump_read_host {path}: error {rc}")
+    h.lumvsr_tot = ctypes.POINTER(ctypes.c_double)()
+    return h, p, lum
 
 
 class HarmGrid(ctypes.Structure):
@@ -75,6 +129,7 @@ EXPORTS = [
     "harm_adj_get_nonsmooth",
     "harm_grid_mks", "harm_grid_cartesian", "harm_geometry", "harm_init_fishmon",
     "harm_init_orszag_tang", "harm_init_monopole",
+    "harm_rdump_write_host", "harm_rdump_read_host", "harm_rdump_write", "harm_rdump_read",
 ]
 
 _lib = None
@@ -142,6 +197,12 @@ def load_library(path=LIB_PATH):
                               ctypes.c_ulong, dp],
         "harm_init_orszag_tang": [ctypes.POINTER(HarmGrid), ctypes.c_double, dp],
         "harm_init_monopole": [ctypes.POINTER(HarmGrid), dp, ctypes.c_double, dp],
+        "harm_rdump_write_host": [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(RestartHeader), ctypes.c_int,
+                                  ctypes.c_int, dp],
+        "harm_rdump_read_host": [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(RestartHeader), ctypes.c_int,
+                                 ctypes.c_int, dp],
+        "harm_rdump_write": [vp, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(RestartHeader)],
+        "harm_rdump_read": [vp, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(RestartHeader)],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -354,6 +415,21 @@ class Harm:
 
     def step_n(self, n):
         _chk(self.lib.harm_step_n(self.h, n), "harm_step_n")
+
+    # restart.c:182 restart_write / :234 restart_read on HARM_P
+    def rdump_write(self, path, header, binary=False):
+        _chk(self.lib.harm_rdump_write(self.h, os.fsencode(path), int(binary), ctypes.byref(header)),
+             "harm_rdump_write")
+
+    def rdump_read(self, path, binary=False):
+        """Reads the interior of HARM_P (call bound_prim afterwards, as
+        restart_init does); returns (RestartHeader, lumvsr)."""
+        h = RestartHeader()
+        lum = np.zeros(self.prm.n1)
+        h.lumvsr_tot = lum.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        _chk(self.lib.harm_rdump_read(self.h, os.fsencode(path), int(binary), ctypes.byref(h)), "harm_rdump_read")
+        h.lumvsr_tot = ctypes.POINTER(ctypes.c_double)()
+        return h, lum
 
     # step_ch.c:468 for a batch of zones
     def utoprimgen(self, zones, U, pr):

@@ -4,8 +4,8 @@
     python -m harm_ramt_amd.build --all    # plus oracle/liboracle.so
 
 The product is one shared library, harm_ramt_amd/_lib/libharm_amd.so, holding
-the HIP kernels, the C-ABI of include/harm_amd.h and the host-side geometry /
-initial-state generators.  It is compiled with -ffp-contract=off so that the
+the HIP kernels, the C-ABI of include/harm_amd.h, the host-side geometry /
+initial-state generators and the restart-file reader/writer.  It is compiled with -ffp-contract=off so that the
 FP64 arithmetic keeps the reference's operation order (no fused multiply-add
 contraction), which makes the GPU step bit-identical to the CPU step.
 """
@@ -43,11 +43,13 @@ def build_lib():
     objdir = os.path.join(ROOT, "_build")
     os.makedirs(objdir, exist_ok=True)
     prob_o = os.path.join(objdir, "harm_problem.o")
+    io_o = os.path.join(objdir, "harm_io.o")
     kern_o = os.path.join(objdir, "harm_kernels.o")
     _run(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, "harm_problem.c"), "-o", prob_o])
+    _run(["gcc", *C_FLAGS, "-c", os.path.join(CSRC, "harm_io.c"), "-o", io_o])
     _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "harm_kernels.hip"), "-o", kern_o])
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", kern_o, prob_o, "-o", tmp, "-lm"])
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", kern_o, prob_o, io_o, "-o", tmp, "-lm"])
     os.replace(tmp, LIB)
     return LIB
 

@@ -290,6 +290,11 @@ __global__ __launch_bounds__(256) void k_flux(KParams P, const double* __restric
       dim3(gridDim.x, gridDim.y, gridDim.z));
 }
 
+template <int DIR>
+__device__ __forceinline__ void face_finish(const KParams& P, double* p_l, double* p_r, double* __restrict__ F,
+                                            double* __restrict__ q, const double* __restrict__ gF, int i, int j,
+                                            double& dtij, int& fail);
+
 // fluxcalc() with one thread per face (the whole face on one lane); same
 // arithmetic as flux_face<DIR>, different work split.
 template <int DIR>
@@ -330,6 +335,16 @@ __device__ __forceinline__ void flux_face_single(const KParams& P, const double*
       p_r[k] = s[3][k] - 0.5 * dqr;
     }
   }
+  face_finish<DIR>(P, p_l, p_r, F, q, gF, i, j, dtij, fail);
+}
+
+// everything of a face after the reconstruction: face geometry, fluxes and
+// wave speeds, HLL/LAXF, the fix_flux edge rules, F and the EMF quotient
+template <int DIR>
+__device__ __forceinline__ void face_finish(const KParams& P, double* p_l, double* p_r, double* __restrict__ F,
+                                            double* __restrict__ q, const double* __restrict__ gF, int i, int j,
+                                            double& dtij, int& fail) {
+  const int z0 = zidx(P, i, j);
   GeoU g;
   load_geo_face(gF, z0, g);
   double Fo[NPR];
@@ -405,6 +420,93 @@ __global__ __launch_bounds__(256) void k_flux_single(KParams P, const double* __
                                                      int dir0) {
   flux_single_body(P, pr, F1, F2, q1, q2, gF, ds, dir0, dim3(blockIdx.x, blockIdx.y, blockIdx.z),
       dim3(gridDim.x, gridDim.y, gridDim.z));
+}
+
+// Edge states of one zone along DIR from its 5-zone stencil s[0..4]
+// (s[2] the zone): l = value at its lower face, r = at its upper face, with
+// the expressions of flux_face_single (PARA: para4 edges; slopes: y -+ dq/2)
+__device__ __forceinline__ void zone_edges(const KParams& P, const double (&s)[5][NPR], double* l, double* r) {
+#pragma unroll
+  for (int k = 0; k < NPR; k++) {
+    if (P.lim == 4) {
+      para4(s[0][k], s[1][k], s[2][k], s[3][k], s[4][k], l[k], r[k]);
+    } else {
+      const double dq = (P.lim == 90) ? csslope(k, s[0][k], s[1][k], s[2][k], s[3][k], s[4][k])
+                                      : slope3(P.lim, s[1][k], s[2][k], s[3][k]);
+      r[k] = s[2][k] + 0.5 * dq;
+      l[k] = s[2][k] - 0.5 * dq;
+    }
+  }
+}
+
+// fluxcalc() with one thread per zone and direction: each zone's edge states
+// are reconstructed once and shared with the face below it -- along x2
+// between the lanes of a wavefront (63 faces per 64 lanes), along x1 through
+// LDS between the rows of a workgroup (3 face rows per 4).  Workgroup 64x4.
+constexpr int FSH_ROWS = 4;
+__host__ __device__ __forceinline__ void flux_shared_grid(int n1, int n2, int& gx, int& gy) {
+  const int gx2 = (n2 + 1 + 62) / 63, gy2 = (n1 + 2 + FSH_ROWS - 1) / FSH_ROWS;           // x2 faces
+  const int gx1 = (n2 + 2 + 63) / 64, gy1 = (n1 + 1 + FSH_ROWS - 2) / (FSH_ROWS - 1);     // x1 faces
+  gx = gx2 > gx1 ? gx2 : gx1;
+  gy = gy2 > gy1 ? gy2 : gy1;
+}
+
+__global__ __launch_bounds__(64 * FSH_ROWS) void k_flux_shared(KParams P, const double* __restrict__ pr,
+                                                              double* __restrict__ F1, double* __restrict__ F2,
+                                                              double* __restrict__ q1, double* __restrict__ q2,
+                                                              const double* __restrict__ gF,
+                                                              DevScalars* __restrict__ ds, int dir0) {
+  __shared__ double rsh[FSH_ROWS][64][NPR];
+  const int dir = dir0 + blockIdx.z;
+  const int tx = threadIdx.x, ty = threadIdx.y;
+  double dtij = 1.e9;
+  int fail = 0;
+  double s[5][NPR], l[NPR], r[NPR];
+  if (dir == 2) {
+    // lanes along x2: zone jz, row i; face (i, jz) takes r of lane tx-1
+    const int jz = 63 * (int)blockIdx.x - 1 + tx, i = FSH_ROWS * (int)blockIdx.y + ty - 1;
+    const bool zone_ok = (jz >= -1 && jz <= P.n2 && i >= -1 && i <= P.n1);
+    if (zone_ok) {
+#pragma unroll
+      for (int m = 0; m < 5; m++) load8(pr, zidx(P, i, jz - 2 + m), s[m]);
+      zone_edges(P, s, l, r);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NPR; k++) l[k] = r[k] = 0.;
+    }
+    double p_l[NPR];
+#pragma unroll
+    for (int k = 0; k < NPR; k++) p_l[k] = __shfl_up(r[k], 1);
+    if (tx >= 1 && zone_ok && jz >= 0) {
+      const size_t nz = (size_t)(P.n1 + 2 * NG) * P.N2M;
+      face_finish<2>(P, p_l, l, F2, q2, gF + nz * GF, i, jz, dtij, fail);
+    }
+  } else {
+    // lanes along x2, rows along x1: zone iz of row ty; face (iz, j) takes r
+    // of row ty-1 through LDS
+    const int j = 64 * (int)blockIdx.x + tx - 1, iz = (FSH_ROWS - 1) * (int)blockIdx.y + ty - 1;
+    const bool zone_ok = (j >= -1 && j <= P.n2 && iz >= -1 && iz <= P.n1);
+    if (zone_ok) {
+#pragma unroll
+      for (int m = 0; m < 5; m++) load8(pr, zidx(P, iz - 2 + m, j), s[m]);
+      zone_edges(P, s, l, r);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NPR; k++) l[k] = r[k] = 0.;
+    }
+#pragma unroll
+    for (int k = 0; k < NPR; k++) rsh[ty][tx][k] = r[k];
+    __syncthreads();
+    if (ty >= 1 && zone_ok && iz >= 0) {
+      double p_l[NPR];
+#pragma unroll
+      for (int k = 0; k < NPR; k++) p_l[k] = rsh[ty - 1][tx][k];
+      face_finish<1>(P, p_l, l, F1, q1, gF, iz, j, dtij, fail);
+    }
+  }
+  double wm = wave_min(dtij);
+  if ((tx & 63) == 0 && wm < 1.e9) atomicMin(&ds->ndt_acc[dir - 1], (unsigned long long)__double_as_longlong(wm));
+  if (fail) atomicOr(&ds->failed, 1);
 }
 
 // corner EMF of flux_ct() (step_ch.c:1223-1234, CORNGDETVERSION=1) from the
@@ -1926,6 +2028,28 @@ extern "C" int harm_array_ptr(harm_ctx* c, int sel, double** dev) {
   return 0;
 }
 
+extern "C" int harm_rdump_write(harm_ctx* c, const char* path, int binary, const harm_restart_header* h) {
+  if (!c || !path || !h) return seterr("null argument");
+  std::vector<double> p((size_t)c->nz * 8);
+  if (harm_get_array(c, HARM_P, p.data())) return 1;
+  const int rc = harm_rdump_write_host(path, binary, h, c->kp.n1, c->kp.n2, p.data());
+  if (rc == 2) return seterr("restart header grid %dx%d differs from the context's %dx%d", h->totalsize[0],
+                             h->totalsize[1], c->kp.n1, c->kp.n2);
+  if (rc) return seterr("harm_rdump_write %s: error %d", path, rc);
+  return 0;
+}
+
+extern "C" int harm_rdump_read(harm_ctx* c, const char* path, int binary, harm_restart_header* h) {
+  if (!c || !path || !h) return seterr("null argument");
+  std::vector<double> p((size_t)c->nz * 8);
+  if (harm_get_array(c, HARM_P, p.data())) return 1;
+  const int rc = harm_rdump_read_host(path, binary, h, c->kp.n1, c->kp.n2, p.data());
+  if (rc == 2) return seterr("restart file grid %dx%d differs from the context's %dx%d", h->totalsize[0],
+                             h->totalsize[1], c->kp.n1, c->kp.n2);
+  if (rc) return seterr("harm_rdump_read %s: error %d", path, rc);
+  return harm_set_array(c, HARM_P, p.data());
+}
+
 extern "C" int harm_get_pflag(harm_ctx* c, int* host) {
   if (!c || !host) return seterr("null argument");
   int* tmp;
@@ -2008,6 +2132,13 @@ static int launch_flux(harm_ctx* c, int pr_sel, int dir0, int ndir) {
       const dim3 b(64, 4);
       hipLaunchKernelGGL(k_flux, dim3((nfj + 31) / 32, (nfi + 3) / 4, ndir), b, 0, c->stream, c->kp, c->arr[pr_sel],
                          c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2, c->gF, c->ds, dir0);
+      break;
+    }
+    case HARM_FLUX_SHARED: {
+      int gx, gy;
+      flux_shared_grid(c->kp.n1, c->kp.n2, gx, gy);
+      hipLaunchKernelGGL(k_flux_shared, dim3(gx, gy, ndir), dim3(64, FSH_ROWS), 0, c->stream, c->kp,
+                         c->arr[pr_sel], c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2, c->gF, c->ds, dir0);
       break;
     }
     default: {
@@ -2337,11 +2468,11 @@ static int autotune(harm_ctx* c) {
   c->prof = false;
   float best = 1e30f;
   int bf = HARM_FLUX_FACE_64x4, bb = HARM_BOUND_KERNELS;
-  const int fv[3] = {HARM_FLUX_FACE_64x4, HARM_FLUX_FACE_64x2, HARM_FLUX_PAIR};
+  const int fv[4] = {HARM_FLUX_FACE_64x4, HARM_FLUX_FACE_64x2, HARM_FLUX_PAIR, HARM_FLUX_SHARED};
   // the one-workgroup pass has no fixup_checksolution: it is tried last
   const int bv[4] = {HARM_BOUND_KERNELS, HARM_BOUND_FIXONE, HARM_BOUND_FUSED, HARM_BOUND_BLOCK};
   const int nbv = c->kp.checksolution ? 3 : 4;
-  for (int f = 0; f < 3 && !rc; f++)
+  for (int f = 0; f < 4 && !rc; f++)
     for (int v = 0; v < nbv && !rc; v++) {
       float ms = 0.f;
       c->flux_variant = fv[f];
@@ -2458,7 +2589,7 @@ extern "C" int harm_prof_reset(harm_ctx* c) {
 
 extern "C" int harm_set_tuning(harm_ctx* c, int flux_variant, int bound_variant) {
   if (!c) return seterr("null ctx");
-  if (flux_variant < 0 || flux_variant > HARM_FLUX_PAIR) return seterr("bad flux variant %d", flux_variant);
+  if (flux_variant < 0 || flux_variant > HARM_FLUX_SHARED) return seterr("bad flux variant %d", flux_variant);
   if (bound_variant < 0 || bound_variant > HARM_BOUND_FUSED)
     return seterr("bad boundary variant %d", bound_variant);
   c->flux_variant = flux_variant;

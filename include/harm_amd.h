@@ -200,6 +200,9 @@ int harm_prof_enable(harm_ctx *ctx, int on);
 #define HARM_FLUX_FACE_64x4 0  /* one thread per face, 64x4 workgroups */
 #define HARM_FLUX_FACE_64x2 1  /* one thread per face, 64x2 workgroups */
 #define HARM_FLUX_PAIR 2       /* a lane pair per face (left/right state) */
+#define HARM_FLUX_SHARED 3     /* a thread per zone and direction: the zone's
+                                  edge states reconstructed once, shared with
+                                  the face below (lanes along x2, LDS along x1) */
 #define HARM_BOUND_KERNELS 0   /* boundary work as six wide launches */
 #define HARM_BOUND_BLOCK 1     /* boundary work in one workgroup with barriers
                                   (not with checksolution: wide launches) */
@@ -303,6 +306,63 @@ int harm_init_orszag_tang(const harm_grid *g, double gam, double *p);
 /* Force-free split monopole (init.bz.c:146-155, PROBLEMTYPE 0) on a Kerr
    grid: rho=u=0, v=0, B^r = +-B0 gdet(ihor,j)/gdet(i,j) by hemisphere. */
 int harm_init_monopole(const harm_grid *g, const double *gdet, double B0, double *p);
+
+/* ---- restart files (restart.c:182-264 restart_write / restart_read) ----
+   Header of write_restart_header() (restart.c:554-682; text: the
+   WRITERESTARTHEADER line of global.h:929-935), then the 8 primitives of
+   every interior zone, j outer and i inner: raw doubles in binary, in text
+   "%21.15g" separated by spaces with one zone per line (the reference's
+   default, binaryoutput = TEXTOUTPUT, keeps 15 significant digits; binary is
+   exact).  Field names are the reference's globals.  The accumulator arrays
+   have the reference build's sizes: NUMENERREGIONS 3 (DOJETDIAG 1),
+   COMPDIM*2 directions, NUMTSCALES 4, NUMFAILFLOORFLAGS 9, NPRDUMP 8. */
+#define HARM_NUMENERREGIONS 3
+#define HARM_NUMDIRS 4
+#define HARM_NUMTSCALES 4
+#define HARM_NUMFAILFLOORFLAGS 9
+typedef struct {
+  int totalsize[2];                 /* N1, N2 */
+  double t, tf;
+  long nstep;
+  double a, gam, cour;
+  double DTd, DTavg, DTener, DTi;
+  long DTr;
+  double DTdebug;
+  long dump_cnt, image_cnt, rdump_cnt, avg_cnt, debug_cnt;
+  double dt;
+  int lim, TIMEORDER, fluxmethod, FLUXB, UTOPRIMVERSION, failed;
+  double R0, Rin, Rout, hslope;
+  int defcoord;
+  int BCtype[4];                    /* X1UP, X1DN, X2UP, X2DN */
+  long realnstep;
+  int debugfail, whichrestart, cooling;
+  long restartsteps[2];
+  int GAMMIEDUMP, GAMMIEIMAGE, GAMMIEENER, DODIAGS, DOENERDIAG, DOGDUMPDIAG, DORDUMPDIAG, DODUMPDIAG,
+      DOAVGDIAG, DOIMAGEDIAG, DOAREAMAPDIAG, POSDEFMETRIC, periodicx1, periodicx2, binaryoutput, sortedoutput;
+  double defcon, SAFE, RHOMIN, UUMIN, RHOMINLIMIT, UUMINLIMIT, BSQORHOLIMIT, BSQOULIMIT, GAMMAMAX, GAMMADAMP,
+      GAMMAFAIL;
+  double pcumreg_tot[HARM_NUMENERREGIONS][HARM_NUMDIRS][8];
+  double fladdreg_tot[HARM_NUMENERREGIONS][8];
+  double sourceaddreg_tot[HARM_NUMENERREGIONS][8];
+  double Ureg_init_tot[HARM_NUMENERREGIONS][8];
+  double failfloorcount_tot[HARM_NUMTSCALES][HARM_NUMFAILFLOORFLAGS];
+  double prMAX[8];
+  int rescaletype;
+  double *lumvsr_tot;               /* N1 values (DOLUMVSR 1); NULL: written as 0, skipped on read */
+} harm_restart_header;
+
+/* Host arrays p[N1+6][N2+6][8]: write the interior of p, or read it into p
+   (ghost zones untouched).  Returns 0, 1 bad argument, 2 header grid size
+   differs from n1,n2, 3 cannot open, 4 short read/write.  In binary the
+   8-byte defcoord field holds the int and 4 zero bytes (the reference writes
+   the 4 bytes that follow the int in its memory there). */
+int harm_rdump_write_host(const char *path, int binary, const harm_restart_header *h, int n1, int n2,
+                          const double *p);
+int harm_rdump_read_host(const char *path, int binary, harm_restart_header *h, int n1, int n2, double *p);
+/* The same on a context's array HARM_P (restart_write / restart_read).  As
+   in restart_init() (restart.c:18-100), call harm_bound_prim after a read. */
+int harm_rdump_write(harm_ctx *ctx, const char *path, int binary, const harm_restart_header *h);
+int harm_rdump_read(harm_ctx *ctx, const char *path, int binary, harm_restart_header *h);
 
 #ifdef __cplusplus
 }

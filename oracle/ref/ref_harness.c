@@ -15,6 +15,8 @@
  * user does by swapping bounds.*.c.
  *
  * usage: ref_harness CONFIG   (key value lines, see read_config())
+ * With "rdump 1" it also writes the final state's restart files (see the end
+ * of main()).
  */
 #include "defs.h"
 #include "step_ch.h"
@@ -77,6 +79,10 @@ int main(int argc, char *argv[])
   read_config(argv[1]);
 
   margv[0] = argv[0]; margv[1] = "1"; margv[2] = "1"; margv[3] = NULL;
+  /* output modes as init() sets them before init_mpi() (initbase.c:42-50,
+     USEROMIO 0); init_mpi() picks the file-combine method from them */
+  binaryoutput = TEXTOUTPUT;
+  sortedoutput = SORTED;
   init_mpi(3, margv);
   set_arrays();
   init_defglobal();
@@ -161,6 +167,24 @@ int main(int argc, char *argv[])
     fwrite(&a_p[0][0][0], sizeof(FTYPE), (size_t)N1M * N2M * NPR, out);
   }
   fclose(out);
+  /* optional: the restart file of the final state, written by the
+     reference's own restart_write() (restart.c:182, through dump_gen),
+     as text (the default binaryoutput, initbase.c:48) into dumps/rdump-0 and
+     as binary into dumps/rdump-1.bin.  init_dumps() (initbase.c:989) sets the
+     column counts that init() would; the luminosity accumulator the header
+     carries (malloc'd in set_arrays.c:145, filled only by diagnostics this
+     harness does not run) is set to zero. */
+  if (cfgi("rdump", 0)) {
+    extern int init_dumps(void);
+    extern int restart_write(long dump_cnt);
+    int ii;
+    init_dumps();
+    for (ii = 0; ii < ncpux1 * N1; ii++) lumvsr_tot[ii] = 0.0;
+    binaryoutput = TEXTOUTPUT;
+    restart_write(0);
+    binaryoutput = BINARYOUTPUT;
+    restart_write(1);
+  }
 #if(USEMPI)
   MPI_Finalize();
 #endif

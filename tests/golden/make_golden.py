@@ -93,12 +93,15 @@ def bc_of(kind):
     return (H.OUTFLOW, H.OUTFLOW, H.POLARAXIS, H.POLARAXIS)
 
 
-def run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, eno=0):
+def run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, eno=0, rdump=False):
     p0.tofile(os.path.join(work, "p0.bin"))
     cfg = dict(prim=os.path.join(work, "p0.bin"), dt=dt0, nsteps=nsteps, out=os.path.join(work, "o.bin"),
                lim=lim, fluxmethod=flux, timeorder=to)
     for n, b in enumerate(bc_of(kind)):
         cfg[f"bc{n}"] = b
+    if rdump:
+        cfg["rdump"] = 1
+        os.makedirs(os.path.join(work, "dumps"), exist_ok=True)
     if kind in ("ks", "ks_mono", "ffde"):
         cfg.update(a=g.a, Rout=g.Rout, hslope=g.hslope, R0=g.R0, geom="ref", gam=4.0 / 3.0, cour=0.9,
                    tables_out=os.path.join(work, "tables.bin"))
@@ -126,10 +129,30 @@ def run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, eno=0):
     return np.array(recs), states
 
 
+# restart files written by the reference's restart_write() after the steps
+RDUMP_CASES = {"rdump_ot_mc_hll_rk4": ("mink", 32, 32, H.MC, H.HLLFLUX, 4, 3, 1e-4)}
+
+
+def make_rdump(name, work):
+    kind, n1, n2, lim, flux, to, nsteps, dt0 = RDUMP_CASES[name]
+    g, geom, p0, extra = problem(kind, n1, n2)
+    recs, states = run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, rdump=True)
+    text = open(os.path.join(work, "dumps", "rdump-0"), "rb").read()
+    binary = open(os.path.join(work, "dumps", "rdump-1.bin"), "rb").read()
+    np.savez_compressed(os.path.join(OUT_DIR, f"{name}.npz"), kind=kind, n1=n1, n2=n2, lim=lim, fluxmethod=flux,
+                        timeorder=to, nsteps=nsteps, dt0=dt0, input_sha=sha(p0), t=recs[:, 0], dt=recs[:, 1],
+                        p=states[-1], rdump_text=np.frombuffer(text, dtype=np.uint8),
+                        rdump_binary=np.frombuffer(binary, dtype=np.uint8))
+    print(name, "text", len(text), "bytes, binary", len(binary), "bytes")
+
+
 def main():
     subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle", "ref"), "-s"])
     with tempfile.TemporaryDirectory() as work:
-        names = sys.argv[1:] or list(CASES)
+        rd = [a for a in sys.argv[1:] if a in RDUMP_CASES] or ([] if sys.argv[1:] else list(RDUMP_CASES))
+        for name in rd:
+            make_rdump(name, work)
+        names = [a for a in sys.argv[1:] if a not in RDUMP_CASES] if sys.argv[1:] else list(CASES)
         for name in names:
             kind, n1, n2, lim, flux, to, nsteps, dt0, saved = CASES[name]
             g, geom, p0, extra = problem(kind, n1, n2)

@@ -222,7 +222,7 @@ def test_gpu_fullsize_ot_invariants():
     assert failed == 0
 
 
-@pytest.mark.parametrize("fv", [H.FLUX_FACE_64x4, H.FLUX_FACE_64x2, H.FLUX_PAIR])
+@pytest.mark.parametrize("fv", [H.FLUX_FACE_64x4, H.FLUX_FACE_64x2, H.FLUX_PAIR, H.FLUX_SHARED])
 @pytest.mark.parametrize("bv", [H.BOUND_KERNELS, H.BOUND_BLOCK, H.BOUND_FIXONE, H.BOUND_FUSED])
 def test_gpu_variants_match_oracle(fv, bv):
     say(f"[case] kernel variants flux={fv} bound={bv}: 32x64 torus, 30 steps (with repaired inversions)")
@@ -242,6 +242,25 @@ def test_gpu_variants_match_oracle(fv, bv):
     nfail, failed = h.failures()
     assert failed == o.failed() == 0
     assert nfail > 0, "the run is meant to exercise the fixup of failed inversions"
+
+
+@pytest.mark.parametrize("lim", [H.DONOR, H.VANL, H.MINM, H.MC, H.PARA, H.CSSLOPE, H.NLIMCENT, H.NLIMUP, H.NLIMDOWN])
+def test_gpu_shared_flux_every_limiter(lim):
+    """The zone-shared reconstruction variant with every limiter, on the torus
+    (outflow, polar axis, fix_flux) and on OT (periodic), against the oracle."""
+    for kind, n1, n2, dt0 in (("ks", 32, 64, 1e-3), ("mink", 40, 72, 1e-4)):
+        say(f"[case] FLUX_SHARED lim={lim} {kind} {n1}x{n2}, 5 steps vs oracle")
+        g, geom, p0, prm = problem(kind, n1, n2, lim=lim)
+        h = gpu_ctx(geom, prm, p0, dt0)
+        h.set_tuning(H.FLUX_SHARED, H.BOUND_FUSED)
+        o = oracle_ctx(geom, prm, p0, dt0)
+        exact = lim != H.CSSLOPE
+        for s in range(5):
+            h.step_ch()
+            o.step()
+            if exact:
+                assert h.get_time() == o.get_time(), f"{kind} step {s + 1}"
+        assert_close_or_same(h.get_array(H.P), o.get_array(0), exact, f"{kind} lim {lim}")
 
 
 @pytest.mark.parametrize("case", ["mink", "ffde", "ks_x1periodic_5", "ks_x1periodic_12", "ks_nopoledeath"])
@@ -283,14 +302,14 @@ def test_gpu_autotune_picks_a_variant():
     h.step_n(3)
     fv, bv = h.get_tuning()
     say(f"  chosen flux={fv} bound={bv}")
-    assert fv in (H.FLUX_FACE_64x4, H.FLUX_FACE_64x2, H.FLUX_PAIR) and bv in (H.BOUND_KERNELS, H.BOUND_BLOCK, H.BOUND_FIXONE, H.BOUND_FUSED)
+    assert fv in (H.FLUX_FACE_64x4, H.FLUX_FACE_64x2, H.FLUX_PAIR, H.FLUX_SHARED) and bv in (H.BOUND_KERNELS, H.BOUND_BLOCK, H.BOUND_FIXONE, H.BOUND_FUSED)
     for _ in range(3):
         o.step()
     assert same(h.get_array(H.P), o.get_array(0))
     assert h.get_time() == o.get_time()
 
 
-@pytest.mark.parametrize("fv", [H.FLUX_FACE_64x4, H.FLUX_PAIR])
+@pytest.mark.parametrize("fv", [H.FLUX_FACE_64x4, H.FLUX_PAIR, H.FLUX_SHARED])
 @pytest.mark.parametrize("bv", [H.BOUND_KERNELS, H.BOUND_KERNELS_SEQCHECK, H.BOUND_FIXONE, H.BOUND_FUSED])
 def test_gpu_checksolution_matches_oracle(fv, bv):
     say(f"[case] CHECKSOLUTION: field-threaded torus 32x64, 25 steps, flux={fv} bound={bv}")

@@ -1,0 +1,163 @@
+"""Restart files (restart.c:182-264, dump.c:641, init_mpi.c:962-990) against
+the files the compiled reference wrote itself (tests/golden/
+rdump_ot_mc_hll_rk4.npz: its restart_write() after 3 steps of the 32x32
+Orszag-Tang run, text and binary, plus its state).
+
+- Reading the reference's binary file gives its state exactly, its text file
+  the state to the 15 significant digits "%21.15g" keeps, and both headers
+  the same values; the header holds the run's grid, time, dt and scheme.
+- Writing that header and state reproduces the reference's text file byte
+  for byte, and its binary file except the 4 bytes after defcoord (the
+  reference writes an int from its address with sizeof(double)).
+- The state written by the GPU context after the same 3 steps gives the
+  same files (-m gpu).
+"""
+import numpy as np
+import pytest
+
+import harm_ramt_amd as H
+from tests.helpers import load_golden, problem, sha
+
+NAME = "rdump_ot_mc_hll_rk4"
+
+
+@pytest.fixture(scope="module")
+def gd():
+    return load_golden(NAME)
+
+
+def _files(gd, tmp_path):
+    text = tmp_path / "rdump-0"
+    binary = tmp_path / "rdump-1.bin"
+    text.write_bytes(gd["rdump_text"].tobytes())
+    binary.write_bytes(gd["rdump_binary"].tobytes())
+    return str(text), str(binary)
+
+
+def _fields(h):
+    out = {}
+    for name, _ in H.RestartHeader._fields_:
+        if name == "lumvsr_tot":
+            continue
+        v = getattr(h, name)
+        out[name] = np.ctypeslib.as_array(v).copy() if hasattr(v, "_length_") else v
+    return out
+
+
+def _defcoord_offset():
+    """byte offset of defcoord in a binary restart header: the fields before
+    it, each with its C size"""
+    off = 0
+    for name, typ in H.RestartHeader._fields_:
+        if name == "defcoord":
+            return off
+        off += H.ctypes.sizeof(typ)
+    raise AssertionError
+
+
+def test_read_reference_files(gd, tmp_path):
+    n1, n2 = int(gd["n1"]), int(gd["n2"])
+    text, binary = _files(gd, tmp_path)
+    hb, pb, lb = H.rdump_read(binary, n1, n2, binary=True)
+    ht, pt, lt = H.rdump_read(text, n1, n2, binary=False)
+    ref = gd["p"]
+    inner = (slice(3, -3), slice(3, -3))
+    assert np.array_equal(pb[inner], ref[inner]), "binary restart data is the exact state"
+    assert np.all(np.abs(pt[inner] - ref[inner]) <= 1e-14 * np.maximum(np.abs(ref[inner]), 1e-300))
+    # the text values are the reference's %21.15g of the state
+    assert np.array_equal(pt[inner], np.vectorize(lambda x: float("%21.15g" % x))(ref[inner]))
+    fb, ft = _fields(hb), _fields(ht)
+    text_of = np.vectorize(lambda x: float("%21.15g" % x))
+    for k in fb:
+        if k == "binaryoutput":   # the one field that tells the two files apart
+            continue
+        a, b = np.asarray(fb[k]), np.asarray(ft[k])
+        assert np.array_equal(text_of(a) if a.dtype.kind == "f" else a, b), k
+    # the run the header describes
+    assert list(hb.totalsize) == [n1, n2]
+    assert hb.t == gd["t"][-1] and hb.dt == gd["dt"][-1]
+    assert hb.lim == int(gd["lim"]) and hb.TIMEORDER == int(gd["timeorder"])
+    assert hb.fluxmethod == int(gd["fluxmethod"]) and hb.gam == 5.0 / 3.0 and hb.cour == 0.8
+    assert list(hb.BCtype) == [H.PERIODIC] * 4
+    assert hb.binaryoutput == 1 and ht.binaryoutput == 0
+    assert np.all(lb == 0) and np.all(lt == 0)
+
+
+@pytest.mark.parametrize("binary", [False, True])
+def test_write_reproduces_reference_files(gd, tmp_path, binary):
+    n1, n2 = int(gd["n1"]), int(gd["n2"])
+    text, bfile = _files(gd, tmp_path)
+    src = bfile if binary else text
+    h, _, lum = H.rdump_read(src, n1, n2, binary=binary)
+    lum_keep = np.ascontiguousarray(lum)
+    h.lumvsr_tot = lum_keep.ctypes.data_as(H.ctypes.POINTER(H.ctypes.c_double))
+    out = str(tmp_path / ("mine.bin" if binary else "mine"))
+    H.rdump_write(out, h, gd["p"], binary=binary)
+    mine = open(out, "rb").read()
+    ref = (gd["rdump_binary"] if binary else gd["rdump_text"]).tobytes()
+    assert len(mine) == len(ref)
+    if binary:
+        off = _defcoord_offset()
+        mine = mine[:off + 4] + mine[off + 8:]
+        ref = ref[:off + 4] + ref[off + 8:]
+    assert mine == ref
+
+
+@pytest.mark.parametrize("binary", [False, True])
+def test_round_trip(tmp_path, binary):
+    g, geom, p0, prm = problem("ks", 16, 24)
+    h = H.RestartHeader()
+    h.totalsize[0], h.totalsize[1] = 16, 24
+    h.t, h.dt, h.nstep, h.gam, h.lim, h.defcoord = 1.25, 3e-3, 17, 4.0 / 3.0, H.PARA, 0
+    h.pcumreg_tot[1][2][5] = -2.5
+    lum = np.linspace(0, 1, 16)
+    h.lumvsr_tot = lum.ctypes.data_as(H.ctypes.POINTER(H.ctypes.c_double))
+    path = str(tmp_path / "rt")
+    H.rdump_write(path, h, p0, binary=binary)
+    h2, p2, l2 = H.rdump_read(path, 16, 24, binary=binary)
+    inner = (slice(3, -3), slice(3, -3))
+    if binary:
+        assert np.array_equal(p2[inner], p0[inner]) and np.array_equal(l2, lum)
+    else:
+        assert np.allclose(p2[inner], p0[inner], rtol=1e-14, atol=0) and np.allclose(l2, lum, rtol=1e-14)
+    assert h2.nstep == 17 and h2.t == 1.25 and h2.pcumreg_tot[1][2][5] == -2.5 and h2.lim == H.PARA
+    with pytest.raises(RuntimeError):
+        H.rdump_read(path, 16, 25, binary=binary)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("binary", [False, True])
+def test_gpu_context_writes_the_reference_file(gd, tmp_path, binary):
+    """The context's own state after the same 3 steps, through harm_rdump_write."""
+    n1, n2 = int(gd["n1"]), int(gd["n2"])
+    text, bfile = _files(gd, tmp_path)
+    hdr, _, lum = H.rdump_read(bfile if binary else text, n1, n2, binary=binary)
+    g, geom, p0, prm = problem(str(gd["kind"]), n1, n2, lim=int(gd["lim"]), fluxmethod=int(gd["fluxmethod"]),
+                               timeorder=int(gd["timeorder"]))
+    assert sha(p0) == str(gd["input_sha"])
+    c = H.Harm(prm, geom)
+    c.set_array(H.P, p0)
+    c.bound_prim(H.P)
+    c.set_time(0.0, float(gd["dt0"]), 1e30)
+    for _ in range(int(gd["nsteps"])):
+        c.step_ch()
+    lum_keep = np.ascontiguousarray(lum)
+    hdr.lumvsr_tot = lum_keep.ctypes.data_as(H.ctypes.POINTER(H.ctypes.c_double))
+    out = str(tmp_path / "gpu")
+    c.rdump_write(out, hdr, binary=binary)
+    mine = open(out, "rb").read()
+    ref = (gd["rdump_binary"] if binary else gd["rdump_text"]).tobytes()
+    if binary:
+        off = _defcoord_offset()
+        mine, ref = mine[:off + 4] + mine[off + 8:], ref[:off + 4] + ref[off + 8:]
+    assert mine == ref
+    # and reading it back into a fresh context gives the state
+    c2 = H.Harm(prm, geom)
+    c2.rdump_read(out, binary=binary)
+    c2.bound_prim(H.P)
+    got = c2.get_array(H.P)[3:-3, 3:-3]
+    want = c.get_array(H.P)[3:-3, 3:-3]
+    if binary:
+        assert np.array_equal(got, want)
+    else:
+        assert np.allclose(got, want, rtol=1e-14, atol=0)
