@@ -114,6 +114,32 @@ class HarmGrid(ctypes.Structure):
     ]
 
 
+NUMENERREGIONS, NUMFLUXTERMS, NUMSOURCES, NUMFAILFLOORFLAGS = 3, 7, 7, 9
+
+
+class EnerRegion(ctypes.Structure):
+    """harm_ener_region: one energy region's accumulators (defs.h:218-239)"""
+    _fields_ = [("pdot", (ctypes.c_double * 8) * 4),
+                ("pdotterms", ((ctypes.c_double * 8) * NUMFLUXTERMS) * 4),
+                ("pcum", (ctypes.c_double * 8) * 4),
+                ("fladd", ctypes.c_double * 8),
+                ("fladdterms", (ctypes.c_double * 8) * NUMFAILFLOORFLAGS),
+                ("sourceadd", ctypes.c_double * 8),
+                ("sourceaddterms", (ctypes.c_double * 8) * NUMSOURCES)]
+
+
+def ener_regions(grid, h_over_r_jet=0.4):
+    """setflux()+setjetflux() (initbase.c:736-980) on a grid: (enerpos,
+    doflux), int32 arrays [3][4] indexed [region][X1UP,X1DN,X2UP,X2DN]"""
+    lib = load_library()
+    ep = np.zeros((NUMENERREGIONS, 4), dtype=np.int32)
+    df = np.zeros((NUMENERREGIONS, 4), dtype=np.int32)
+    rc = lib.harm_ener_regions(ctypes.byref(grid), h_over_r_jet, _iptr(ep), _iptr(df))
+    if rc != 0:
+        raise HarmError(f"harm_ener_regions: error {rc}")
+    return ep, df
+
+
 # every symbol include/harm_amd.h declares (checked by the CPU tests)
 EXPORTS = [
     "harm_create", "harm_destroy", "harm_last_error", "harm_set_stream", "harm_set_geometry",
@@ -130,6 +156,8 @@ EXPORTS = [
     "harm_grid_mks", "harm_grid_cartesian", "harm_geometry", "harm_init_fishmon",
     "harm_init_orszag_tang", "harm_init_monopole",
     "harm_rdump_write_host", "harm_rdump_read_host", "harm_rdump_write", "harm_rdump_read",
+    "harm_ener_regions", "harm_acct_setup", "harm_acct_off", "harm_acct_read", "harm_diag_flux",
+    "harm_divbmaxavg", "harm_constotal",
 ]
 
 _lib = None
@@ -203,6 +231,13 @@ def load_library(path=LIB_PATH):
                                  ctypes.c_int, dp],
         "harm_rdump_write": [vp, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(RestartHeader)],
         "harm_rdump_read": [vp, ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(RestartHeader)],
+        "harm_ener_regions": [ctypes.POINTER(HarmGrid), ctypes.c_double, ip, ip],
+        "harm_acct_setup": [vp, ip, ip, ctypes.c_double],
+        "harm_acct_off": [vp],
+        "harm_acct_read": [vp, ctypes.POINTER(EnerRegion), dp, dp, dp],
+        "harm_diag_flux": [vp, ctypes.c_double],
+        "harm_divbmaxavg": [vp, ctypes.c_int, dp, dp, ip, ip],
+        "harm_constotal": [vp, ctypes.c_int, ctypes.c_int, dp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -430,6 +465,46 @@ class Harm:
         _chk(self.lib.harm_rdump_read(self.h, os.fsencode(path), int(binary), ctypes.byref(h)), "harm_rdump_read")
         h.lumvsr_tot = ctypes.POINTER(ctypes.c_double)()
         return h, lum
+
+    # conservation accounting (diag.c diag_flux/diag_source, dump_ener.c)
+    def acct_setup(self, enerpos, doflux, dx3=2 * np.pi):
+        ep = np.ascontiguousarray(enerpos, dtype=np.int32)
+        df = np.ascontiguousarray(doflux, dtype=np.int32)
+        if ep.shape != (NUMENERREGIONS, 4) or df.shape != (NUMENERREGIONS, 4):
+            raise ValueError("enerpos and doflux are [3][4]")
+        _chk(self.lib.harm_acct_setup(self.h, _iptr(ep), _iptr(df), dx3), "harm_acct_setup")
+
+    def acct_off(self):
+        _chk(self.lib.harm_acct_off(self.h), "harm_acct_off")
+
+    def acct(self):
+        """the accumulators as a dict of arrays with a leading region axis,
+        plus pdottermsjet2[4][7][8], frdot[n1][8] and lumvsr[n1]"""
+        regs = (EnerRegion * NUMENERREGIONS)()
+        jet2 = np.zeros((4, NUMFLUXTERMS, 8))
+        frdot = np.zeros((self.prm.n1, 8))
+        lum = np.zeros(self.prm.n1)
+        _chk(self.lib.harm_acct_read(self.h, regs, _dptr(jet2), _dptr(frdot), _dptr(lum)), "harm_acct_read")
+        out = {name: np.array([np.ctypeslib.as_array(getattr(regs[r], name)) for r in range(NUMENERREGIONS)])
+               for name, _ in EnerRegion._fields_}
+        out["pdottermsjet2"], out["frdot"], out["lumvsr"] = jet2, frdot, lum
+        return out
+
+    def diag_flux(self, Dt):
+        _chk(self.lib.harm_diag_flux(self.h, Dt), "harm_diag_flux")
+
+    def divbmaxavg(self, sel=P):
+        """(divbmax, divbavg, imax, jmax) of dump_ener.c:449 divbmaxavg"""
+        mx, av = ctypes.c_double(), ctypes.c_double()
+        im, jm = ctypes.c_int(), ctypes.c_int()
+        _chk(self.lib.harm_divbmaxavg(self.h, sel, ctypes.byref(mx), ctypes.byref(av), ctypes.byref(im),
+                                      ctypes.byref(jm)), "harm_divbmaxavg")
+        return mx.value, av.value, im.value, jm.value
+
+    def constotal(self, region, sel=P):
+        v = np.zeros(8)
+        _chk(self.lib.harm_constotal(self.h, sel, region, _dptr(v)), "harm_constotal")
+        return v
 
     # step_ch.c:468 for a batch of zones
     def utoprimgen(self, zones, U, pr):

@@ -1,0 +1,575 @@
+// harm_acct.inl -- conservation accounting of the step on the device
+// (included by harm_kernels.hip after the context and step code).
+//
+// The reference accumulates, every step, the fluxes through the boundaries
+// of each energy region (diag_flux, diag.c:576-834, after the step) and the
+// volume source (diag_source, diag.c:1231-1265, on the last substage), and
+// at dump time reduces the state to div B extremes (divbmaxavg,
+// dump_ener.c:449-486) and conserved totals (constotal, dump_ener.c:562-590).
+// Its sums are serial loops, so each total here is formed in two phases:
+// every face or zone term in parallel (the same FP64 operations as the
+// reference, one thread each), then one lane per accumulator adding the terms
+// in the reference's loop order.  The serial phase is a single wavefront;
+// its length is one boundary (diag_flux), one grid row (frdot, lumvsr) or the
+// region's zones (sourceadd, constotal, divbavg).
+
+namespace {
+
+constexpr int ACC_NREG = HARM_NUMENERREGIONS;
+constexpr int ACC_NFL = HARM_NUMFLUXTERMS;
+constexpr int REG_D = (int)(sizeof(harm_ener_region) / sizeof(double));
+constexpr int OFF_PDOT = (int)(offsetof(harm_ener_region, pdot) / sizeof(double));
+constexpr int OFF_TERMS = (int)(offsetof(harm_ener_region, pdotterms) / sizeof(double));
+constexpr int OFF_PCUM = (int)(offsetof(harm_ener_region, pcum) / sizeof(double));
+constexpr int OFF_SRC = (int)(offsetof(harm_ener_region, sourceadd) / sizeof(double));
+constexpr int OFF_SRCT = (int)(offsetof(harm_ener_region, sourceaddterms) / sizeof(double));
+constexpr int OFF_JET2 = ACC_NREG * REG_D;             // pdottermsjet2[4][7][8] after the regions
+constexpr int ACC_D = OFF_JET2 + 4 * ACC_NFL * 8;
+constexpr int FACE_SLOTS = 8 + ACC_NFL * 8 + 1;        // per face: pdot term, flux terms, jet2 condition
+
+// regions by value in kernel arguments
+struct AcctRegions {
+  int ep[ACC_NREG][4], df[ACC_NREG][4];
+  int off[ACC_NREG][4], len[ACC_NREG][4];  // each boundary's run in the face list
+  double dVF, dx3;
+};
+
+struct AcctState {
+  AcctRegions R;
+  bool on;
+  double* acc;      // ACC_D doubles: harm_ener_region[3], pdottermsjet2
+  double* frdot;    // [n1][8]
+  double* lumvsr;   // [n1]
+  double* scratch;  // per-term buffer of the first phase
+  double* out;      // 4 doubles: dump-time results
+  int* faces;       // (i, j) of each boundary face, boundaries in (region, dir) order
+  size_t scratch_n;
+};
+
+__device__ __forceinline__ bool acct_in(const int* ep, int i, int j) {
+  return (i >= ep[HARM_X1DN]) && (i <= ep[HARM_X1UP]) && (j >= ep[HARM_X2DN]) && (j <= ep[HARM_X2UP]);
+}
+
+// UtoU(UEVOLVE -> UDIAG), phys.c:219-265 (e[k] = gdet)
+__device__ __forceinline__ void evolve_to_diag(const KParams& P, double gdet, const double* Uin, double* Uout) {
+  double u[NPR];
+#pragma unroll
+  for (int k = 0; k < NPR; k++) u[k] = Uin[k] / gdet;
+  if (P.eomtype == 0) u[UU] += -u[RHO];
+#pragma unroll
+  for (int k = 0; k < NPR; k++) Uout[k] = u[k] * gdet;
+}
+
+// primtoU with returntype UDIAG (phys.c:16-56, UtoU UNOTHING -> UDIAG):
+// rest mass kept in UU, times gdet
+__device__ __forceinline__ void primtoU_diag(const KParams& P, const double* pr, const State& q, double gdet,
+                                             double* U) {
+  double fin[NPR];
+  fin[RHO] = pr[RHO] * q.ucon[0];
+  mhd_calc(P, pr, 0, q, &fin[UU]);
+  fin[B1] = pr[B1];
+  fin[B2] = pr[B2];
+  fin[B3] = pr[B3];
+#pragma unroll
+  for (int k = 0; k < NPR; k++) U[k] = fin[k] * gdet;
+}
+
+// a step's fluxes as the reference's F1/F2 hold them after flux_ct()
+// (step_ch.c:1403-1412): the B fluxes of the flux-CT faces from the EMFs
+__device__ __forceinline__ void final_flux(const KParams& P, const double* __restrict__ F, const double* __restrict__ q1,
+                                          const double* __restrict__ q2, const double* __restrict__ gC, int dir, int i,
+                                          int j, double* f) {
+  load8(F, zidx(P, i, j), f);
+  if (P.fluxb != HARM_FLUXCTTOTH) return;
+  if (dir == 1 && i >= 0 && i <= P.n1 && j >= 0 && j <= P.n2 - 1) {
+    f[B1] = 0.;
+    f[B2] = 0.5 * (emf_at(P, q1, q2, gC, i, j) + emf_at(P, q1, q2, gC, i, j + 1));
+  } else if (dir == 2 && i >= 0 && i <= P.n1 - 1 && j >= 0 && j <= P.n2) {
+    f[B1] = -0.5 * (emf_at(P, q1, q2, gC, i, j) + emf_at(P, q1, q2, gC, i + 1, j));
+    f[B2] = 0.;
+  }
+}
+
+// diag_source's term of each interior zone (diag.c:1247-1259 with the
+// geometric source only; cooling is off): UDIAG form of Dt dVF dUcomp[0]
+__global__ void k_acct_srcterm(KParams P, AcctRegions R, const double* __restrict__ pb, const double* __restrict__ gC,
+                               const double* __restrict__ gConn, const DevScalars* __restrict__ ds,
+                               double* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= P.n1 || j >= P.n2) return;
+  const int z = zidx(P, i, j);
+  double gc[36];
+  load_cent(gC, z, gc);
+  GeoU g;
+  geo_from_cent(gc, g);
+  double pr[NPR], dU[NPR], ft[NPR], fd[NPR];
+  load8(pb, z, pr);
+  source_conn(P, pr, g, gConn + (size_t)z * 64, dU);
+  const double Dt = ds->dt;
+#pragma unroll
+  for (int k = 0; k < NPR; k++) ft[k] = Dt * R.dVF * (0.0 + dU[k]);  // dUcomp[GEOMSOURCE] = 0 e + e dUconn
+  evolve_to_diag(P, g.gdet, ft, fd);
+  store8(out, i * P.n2 + j, fd);
+}
+
+// sourceadd / sourceaddterms (lane = region*8 + k) and lumvsr (lanes of a
+// second group, one per x1 row), in ZLOOP order; the six zero cooling terms
+// of each zone are added as the reference adds them
+__global__ void k_acct_srcsum(KParams P, AcctRegions R, const double* __restrict__ term, double* __restrict__ acc,
+                              double* __restrict__ lumvsr) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < ACC_NREG * 8) {
+    const int r = t / 8, k = t % 8;
+    const int* ep = R.ep[r];
+    const int i0 = max(ep[HARM_X1DN], 0), i1 = min(ep[HARM_X1UP], P.n1 - 1);
+    const int j0 = max(ep[HARM_X2DN], 0), j1 = min(ep[HARM_X2UP], P.n2 - 1);
+    double* a = acc + r * REG_D;
+    double s = a[OFF_SRC + k], s0 = a[OFF_SRCT + k];
+    for (int i = i0; i <= i1; i++) {
+      const double* row = term + (size_t)i * P.n2 * 8 + k;
+      int j = j0;
+      for (; j + 8 <= j1 + 1; j += 8) {
+        double v[8];
+#pragma unroll
+        for (int m = 0; m < 8; m++) v[m] = row[(size_t)(j + m) * 8];
+#pragma unroll
+        for (int m = 0; m < 8; m++) {
+          s0 += v[m];
+          s += v[m];
+          s += 0.0;
+        }
+      }
+      for (; j <= j1; j++) {
+        const double v = row[(size_t)j * 8];
+        s0 += v;
+        s += v;
+        s += 0.0;
+      }
+    }
+    a[OFF_SRC + k] = s;
+    a[OFF_SRCT + k] = s0;
+  } else if (t >= 64 && t - 64 < P.n1) {
+    const int i = t - 64;
+    const int* ep = R.ep[0];
+    if (i < ep[HARM_X1DN] || i > ep[HARM_X1UP]) return;
+    const int j0 = max(ep[HARM_X2DN], 0), j1 = min(ep[HARM_X2UP], P.n2 - 1);
+    double l = lumvsr[i];
+    for (int j = j0; j <= j1; j++) {
+      l += term[((size_t)i * P.n2 + j) * 8 + UU];
+      l += 0.0;
+    }
+    lumvsr[i] = l;
+  }
+}
+
+// diag_flux's terms of each boundary face (diag.c:649-803)
+__global__ void k_acct_faceterm(KParams P, AcctRegions R, const int* __restrict__ faces, int nfaces,
+                                const double* __restrict__ p, const double* __restrict__ F1,
+                                const double* __restrict__ F2, const double* __restrict__ q1,
+                                const double* __restrict__ q2, const double* __restrict__ gF,
+                                const double* __restrict__ gC, double* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nfaces) return;
+  int r = 0, dir = 0;
+  for (int rr = 0; rr < ACC_NREG; rr++)
+    for (int d = 0; d < 4; d++)
+      if (R.len[rr][d] > 0 && e >= R.off[rr][d] && e < R.off[rr][d] + R.len[rr][d]) { r = rr; dir = d; }
+  const int i = faces[2 * e], j = faces[2 * e + 1];
+  const bool x1 = (dir == HARM_X1UP || dir == HARM_X1DN);
+  const int fluxdir = x1 ? 1 : 2;
+  const double surface = x1 ? P.dx2 * R.dx3 : P.dx1 * R.dx3;
+  const int z = zidx(P, i, j);
+  const size_t nz = (size_t)(P.n1 + 2 * NG) * P.N2M;
+  double* o = out + (size_t)e * FACE_SLOTS;
+  {
+    double f[NPR], ft[NPR], fd[NPR];
+    final_flux(P, x1 ? F1 : F2, q1, q2, gC, fluxdir, i, j, f);
+    const double fg = gF[((x1 ? 0 : nz) + z) * GF + 15];
+#pragma unroll
+    for (int k = 0; k < NPR; k++) ft[k] = f[k] * surface;
+    evolve_to_diag(P, fg, ft, fd);
+#pragma unroll
+    for (int k = 0; k < NPR; k++) o[k] = fd[k];
+  }
+  double gc[36];
+  load_cent(gC, z, gc);
+  GeoU g;
+  geo_from_cent(gc, g);
+  double pr[NPR];
+  load8(p, z, pr);
+  State q;
+  get_state(pr, g, q);
+  const double pgas = (P.gam - 1.0) * pr[UU];
+  double bsq = 0;
+#pragma unroll
+  for (int l = 0; l < 4; l++) bsq += (q.bcon[l]) * (q.bcov[l]);
+  int cond = 0;
+  if (r == 0) {
+    const double bsqorho = bsq / pr[RHO];
+    if (dir == HARM_X1DN) cond = (bsqorho > 3.162);  // JETBSQORHO (diag.c:572)
+    else if (dir == HARM_X1UP) cond = ((q.ucon[1] > 0.0) && (-q.ucov[0] - 1.0 > 0.0));
+  }
+  const double surgdet = (g.gdet) * surface;
+  double* tm = o + 8;  // [fl][k]
+#pragma unroll
+  for (int m = 0; m < ACC_NFL * 8; m++) tm[m] = 0.0;
+  tm[0 * 8 + RHO] = pr[RHO] * (q.ucon[fluxdir]) * surgdet;
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    const int k = UU + l;
+    const double d = (fluxdir == l) ? 1. : 0.;
+    tm[0 * 8 + k] = pgas * (q.ucon[fluxdir]) * (q.ucov[l]) * surgdet;
+    tm[1 * 8 + k] = pr[RHO] * (q.ucon[fluxdir]) * (q.ucov[l]) * surgdet;
+    tm[2 * 8 + k] = pr[UU] * (q.ucon[fluxdir]) * (q.ucov[l]) * surgdet;
+    tm[3 * 8 + k] = bsq * (q.ucon[fluxdir]) * (q.ucov[l]) * surgdet;
+    tm[4 * 8 + k] = pgas * d * surgdet;
+    tm[5 * 8 + k] = 0.5 * bsq * d * surgdet;
+    tm[6 * 8 + k] = -(q.bcon[fluxdir]) * (q.bcov[l]) * surgdet;
+  }
+#pragma unroll
+  for (int m = 1; m <= 3; m++) {
+    tm[0 * 8 + B1 + m - 1] = (q.bcon[m]) * (q.ucon[fluxdir]) * surgdet;
+    tm[1 * 8 + B1 + m - 1] = -(q.bcon[fluxdir]) * (q.ucon[m]) * surgdet;
+  }
+  o[FACE_SLOTS - 1] = (double)cond;
+}
+
+// the boundary sums of diag_flux, one lane per (region, dir, quantity):
+// slots 0-7 pdot (then pcum += pdot Dt), 8-63 pdotterms, 64-119 jet2
+// (region 0); boundaries with no flux are zeroed as the reference zeroes them
+__global__ void k_acct_facesum(AcctRegions R, const double* __restrict__ term, double* __restrict__ acc,
+                               const DevScalars* __restrict__ ds, double Dt, int dt_from_step) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int NS = 8 + 2 * ACC_NFL * 8;
+  if (t >= ACC_NREG * 4 * NS) return;
+  const int r = t / (4 * NS), dir = (t / NS) % 4, slot = t % NS;
+  const bool jet = slot >= 8 + ACC_NFL * 8;
+  if (jet && r != 0) return;
+  const int flux = R.df[r][dir] >= 0;
+  const int off = R.off[r][dir], len = flux ? R.len[r][dir] : 0;
+  const int col = jet ? slot - ACC_NFL * 8 : slot;  // column of the face record
+  double s = 0.0;
+  for (int e = off; e < off + len; e++) {
+    const double* o = term + (size_t)e * FACE_SLOTS;
+    if (jet) {
+      if (o[FACE_SLOTS - 1] != 0.0) s += o[col];
+    } else {
+      s += o[col];
+    }
+  }
+  double* a = acc + r * REG_D;
+  if (jet) {
+    acc[OFF_JET2 + dir * ACC_NFL * 8 + (slot - 8 - ACC_NFL * 8)] = s;
+  } else if (slot < 8) {
+    a[OFF_PDOT + dir * 8 + slot] = s;
+    if (flux) {
+      const double dtv = dt_from_step ? ds->dt_step : Dt;
+      a[OFF_PCUM + dir * 8 + slot] += s * dtv;
+    }
+  } else {
+    a[OFF_TERMS + dir * ACC_NFL * 8 + (slot - 8)] = s;
+  }
+}
+
+// frdot (diag.c:816-827): the x1 face terms of the interior, then one lane
+// per (row, quantity) summing over x2
+__global__ void k_acct_frterm(KParams P, AcctRegions R, const double* __restrict__ F1, const double* __restrict__ q1,
+                              const double* __restrict__ q2, const double* __restrict__ gF,
+                              const double* __restrict__ gC, double* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= P.n1 || j >= P.n2) return;
+  double f[NPR], ft[NPR], fd[NPR];
+  final_flux(P, F1, q1, q2, gC, 1, i, j, f);
+  const double surface = P.dx2 * R.dx3;
+#pragma unroll
+  for (int k = 0; k < NPR; k++) ft[k] = f[k] * surface;
+  evolve_to_diag(P, gF[(size_t)zidx(P, i, j) * GF + 15], ft, fd);
+  store8(out, i * P.n2 + j, fd);
+}
+
+__global__ void k_acct_frsum(KParams P, const double* __restrict__ term, double* __restrict__ frdot) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P.n1 * 8) return;
+  const int i = t / 8, k = t % 8;
+  const double* row = term + (size_t)i * P.n2 * 8 + k;
+  double s = 0;
+  for (int j = 0; j < P.n2; j++) s += row[(size_t)j * 8];
+  frdot[t] = s;
+}
+
+// divbmaxavg: |div B| of each corner of LOOPDIVB (SETFDIVBFLUXCT,
+// global.h:1152-1175), then the first maximum and the sum in loop order
+__global__ void k_divb(KParams P, const double* __restrict__ p, const double* __restrict__ gC,
+                       double* __restrict__ out) {
+  const int jj = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ii = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ii >= P.n1 + 4 || jj >= P.n2 + 4) return;
+  const int i = ii - 2, j = jj - 2;
+  double divb = 0.;
+  if ((j != 0) && (j != P.n2)) {
+    const double dx1 = P.dx1, dx2 = P.dx2;
+    const int z00 = zidx(P, i, j), z0m = zidx(P, i, j - 1), zm0 = zidx(P, i - 1, j), zmm = zidx(P, i - 1, j - 1);
+    const double g00 = gC[(size_t)z00 * GCN + 32], g0m = gC[(size_t)z0m * GCN + 32];
+    const double gm0 = gC[(size_t)zm0 * GCN + 32], gmm = gC[(size_t)zmm * GCN + 32];
+    const double b100 = p[(size_t)z00 * 8 + B1], b10m = p[(size_t)z0m * 8 + B1];
+    const double b1m0 = p[(size_t)zm0 * 8 + B1], b1mm = p[(size_t)zmm * 8 + B1];
+    const double b200 = p[(size_t)z00 * 8 + B2], b20m = p[(size_t)z0m * 8 + B2];
+    const double b2m0 = p[(size_t)zm0 * 8 + B2], b2mm = p[(size_t)zmm * 8 + B2];
+    const double num = ((dx1 > dx2) ? dx1 : dx2) *
+                       (0.5 * (b100 * g00 + b10m * g0m - b1m0 * gm0 - b1mm * gmm) / dx1 +
+                        0.5 * (b200 * g00 + b2m0 * gm0 - b20m * g0m - b2mm * gmm) / dx2);
+    const double sum = b100 * g00 + b10m * g0m + b1m0 * gm0 + b1mm * gmm + b200 * g00 + b2m0 * gm0 + b20m * g0m +
+                       b2mm * gmm;
+    divb = fabs(num / (gC[(size_t)z00 * GCN + 35] * 0.125 * fabs(fabs(sum) + 1.e-20)));
+  }
+  out[ii * (P.n2 + 4) + jj] = divb;
+}
+
+__global__ void k_divb_reduce(KParams P, const double* __restrict__ divb, double* __restrict__ res) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int w = P.n2 + 4;
+  double mx = 0, sum = 0;
+  int imax = 0, jmax = 0;
+  for (int ii = 0; ii < P.n1 + 4; ii++) {
+    const double* row = divb + (size_t)ii * w;
+    for (int jj = 0; jj < w; jj++) {
+      const double d = row[jj];
+      if (d > mx) { imax = ii - 2; jmax = jj - 2; mx = d; }
+      sum += d;
+    }
+  }
+  res[0] = mx;
+  res[1] = sum / (double)(P.n1 * P.n2);
+  res[2] = (double)imax;
+  res[3] = (double)jmax;
+}
+
+// constotal: U (UDIAG) dVF of each interior zone, then lane k sums the
+// region's zones in ZLOOP order
+__global__ void k_cons_term(KParams P, AcctRegions R, const double* __restrict__ p, const double* __restrict__ gC,
+                            double* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= P.n1 || j >= P.n2) return;
+  const int z = zidx(P, i, j);
+  double gc[36];
+  load_cent(gC, z, gc);
+  GeoU g;
+  geo_from_cent(gc, g);
+  double pr[NPR], U[NPR], ft[NPR];
+  load8(p, z, pr);
+  State q;
+  get_state(pr, g, q);
+  primtoU_diag(P, pr, q, g.gdet, U);
+#pragma unroll
+  for (int k = 0; k < NPR; k++) ft[k] = U[k] * R.dVF;
+  store8(out, i * P.n2 + j, ft);
+}
+
+__global__ void k_cons_sum(KParams P, AcctRegions R, int region, const double* __restrict__ term,
+                           double* __restrict__ res) {
+  const int k = threadIdx.x;
+  if (k >= NPR) return;
+  const int* ep = R.ep[region];
+  const int i0 = max(ep[HARM_X1DN], 0), i1 = min(ep[HARM_X1UP], P.n1 - 1);
+  const int j0 = max(ep[HARM_X2DN], 0), j1 = min(ep[HARM_X2UP], P.n2 - 1);
+  double s = 0.0;
+  for (int i = i0; i <= i1; i++)
+    for (int j = j0; j <= j1; j++) s += term[((size_t)i * P.n2 + j) * 8 + k];
+  res[k] = s;
+}
+
+}  // namespace
+
+static bool acct_on(const harm_ctx* c) { return c->acct && c->acct->on; }
+static void acct_set_on(harm_ctx* c, bool on) {
+  if (c->acct) c->acct->on = on;
+}
+
+static int acct_alloc(harm_ctx* c) {
+  if (c->acct) return 0;
+  AcctState* a = new AcctState();
+  memset(a, 0, sizeof(*a));
+  const int n1 = c->prm.n1, n2 = c->prm.n2;
+  a->scratch_n = std::max((size_t)n1 * n2 * 8, (size_t)(n1 + 4) * (n2 + 4));
+  a->scratch_n = std::max(a->scratch_n, (size_t)ACC_NREG * 2 * (n1 + n2 + 2) * FACE_SLOTS);
+  hipError_t e = hipMalloc(&a->acc, ACC_D * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&a->frdot, (size_t)n1 * 8 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&a->lumvsr, (size_t)n1 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&a->scratch, a->scratch_n * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&a->out, 8 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&a->faces, (size_t)ACC_NREG * 2 * (n1 + n2 + 2) * 2 * sizeof(int));
+  c->acct = a;
+  if (e != hipSuccess) return seterr("accounting buffers: %s", hipGetErrorString(e));
+  return 0;
+}
+
+static void acct_free(harm_ctx* c) {
+  AcctState* a = c->acct;
+  if (!a) return;
+  void* bufs[6] = {a->acc, a->frdot, a->lumvsr, a->scratch, a->out, a->faces};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  delete a;
+  c->acct = nullptr;
+}
+
+// diag_source on the last substage, before its update overwrites pf (the
+// source is that of the array the update reads, advance.c:121-140)
+static int acct_source(harm_ctx* c, int pi, int pb, int pf) {
+  AcctState* a = c->acct;
+  const int pbs = (pf == pb) ? pi : pb;
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL(k_acct_srcterm, grid2(c->kp.n2, c->kp.n1, b), b, 0, c->stream, c->kp, a->R, c->arr[pbs], c->gC,
+                     c->gConn, c->ds, a->scratch);
+  hipLaunchKernelGGL(k_acct_srcsum, dim3((64 + c->kp.n1 + 63) / 64), dim3(64), 0, c->stream, c->kp, a->R, a->scratch,
+                     a->acc, a->lumvsr);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// diag_flux of array HARM_P and the last fluxes; Dt < 0: the step's dt
+static int acct_flux(harm_ctx* c, double Dt) {
+  AcctState* a = c->acct;
+  int nfaces = 0;
+  for (int r = 0; r < ACC_NREG; r++)
+    for (int d = 0; d < 4; d++) nfaces = std::max(nfaces, a->R.off[r][d] + a->R.len[r][d]);
+  if (nfaces > 0)
+    hipLaunchKernelGGL(k_acct_faceterm, dim3((nfaces + 63) / 64), dim3(64), 0, c->stream, c->kp, a->R, a->faces,
+                       nfaces, c->arr[HARM_P], c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2, c->gF, c->gC,
+                       a->scratch);
+  const int nsum = ACC_NREG * 4 * (8 + 2 * ACC_NFL * 8);
+  hipLaunchKernelGGL(k_acct_facesum, dim3((nsum + 63) / 64), dim3(64), 0, c->stream, a->R, a->scratch, a->acc, c->ds,
+                     Dt, Dt < 0 ? 1 : 0);
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL(k_acct_frterm, grid2(c->kp.n2, c->kp.n1, b), b, 0, c->stream, c->kp, a->R, c->arr[HARM_F1],
+                     c->q1, c->q2, c->gF, c->gC, a->scratch);
+  hipLaunchKernelGGL(k_acct_frsum, dim3((c->kp.n1 * 8 + 63) / 64), dim3(64), 0, c->stream, c->kp, a->scratch,
+                     a->frdot);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+static void invalidate_graph(harm_ctx* c) {
+  if (c->graph_ok) {
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph_ok = false;
+  }
+}
+
+extern "C" int harm_acct_setup(harm_ctx* c, const int* enerpos, const int* doflux, double dx3) {
+  if (!c || !enerpos || !doflux) return seterr("null argument");
+  if (!(dx3 > 0.0)) return seterr("dx3 must be positive");
+  const int n1 = c->prm.n1, n2 = c->prm.n2;
+  AcctRegions R;
+  memset(&R, 0, sizeof(R));
+  std::vector<int> faces;
+  for (int r = 0; r < ACC_NREG; r++)
+    for (int d = 0; d < 4; d++) {
+      R.ep[r][d] = enerpos[r * 4 + d];
+      R.df[r][d] = doflux[r * 4 + d];
+    }
+  for (int r = 0; r < ACC_NREG; r++)
+    for (int d = 0; d < 4; d++) {
+      R.off[r][d] = (int)(faces.size() / 2);
+      if (R.df[r][d] < 0) continue;
+      const bool x1 = (d == HARM_X1UP || d == HARM_X1DN);
+      const int fixed = R.df[r][d];
+      const int a0 = x1 ? R.ep[r][HARM_X2DN] : R.ep[r][HARM_X1DN];
+      const int a1 = x1 ? R.ep[r][HARM_X2UP] : R.ep[r][HARM_X1UP];
+      if (fixed > (x1 ? n1 : n2)) return seterr("region %d dir %d: flux face %d off the grid", r, d, fixed);
+      if (a0 <= a1 && (a0 < 0 || a1 > (x1 ? n2 : n1) - 1))
+        return seterr("region %d dir %d: faces %d..%d off the grid", r, d, a0, a1);
+      for (int m = a0; m <= a1; m++) {
+        faces.push_back(x1 ? fixed : m);
+        faces.push_back(x1 ? m : fixed);
+      }
+      R.len[r][d] = (int)(faces.size() / 2) - R.off[r][d];
+    }
+  R.dx3 = dx3;
+  R.dVF = (c->prm.dx1 * c->prm.dx2) * dx3;  // set_grid.c:17-19
+  if (acct_alloc(c)) return 1;
+  AcctState* a = c->acct;
+  if (faces.size() / 2 > (size_t)ACC_NREG * 2 * (n1 + n2 + 2)) return seterr("too many boundary faces");
+  a->R = R;
+  if (!faces.empty())
+    HIPCHK(hipMemcpyAsync(a->faces, faces.data(), faces.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(a->acc, 0, ACC_D * sizeof(double), c->stream));
+  HIPCHK(hipMemsetAsync(a->frdot, 0, (size_t)n1 * 8 * sizeof(double), c->stream));
+  HIPCHK(hipMemsetAsync(a->lumvsr, 0, (size_t)n1 * sizeof(double), c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  a->on = true;
+  invalidate_graph(c);
+  return 0;
+}
+
+extern "C" int harm_acct_off(harm_ctx* c) {
+  if (!c) return seterr("null ctx");
+  if (c->acct) c->acct->on = false;
+  invalidate_graph(c);
+  return 0;
+}
+
+extern "C" int harm_acct_read(harm_ctx* c, harm_ener_region* regions, double* jet2, double* frdot, double* lumvsr) {
+  if (!c) return seterr("null ctx");
+  if (!c->acct) return seterr("accounting was never set up (harm_acct_setup)");
+  AcctState* a = c->acct;
+  std::vector<double> h(ACC_D);
+  HIPCHK(hipMemcpyAsync(h.data(), a->acc, ACC_D * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (frdot)
+    HIPCHK(hipMemcpyAsync(frdot, a->frdot, (size_t)c->prm.n1 * 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (lumvsr)
+    HIPCHK(hipMemcpyAsync(lumvsr, a->lumvsr, (size_t)c->prm.n1 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (regions) memcpy(regions, h.data(), sizeof(harm_ener_region) * ACC_NREG);
+  if (jet2) memcpy(jet2, h.data() + OFF_JET2, 4 * ACC_NFL * 8 * sizeof(double));
+  return 0;
+}
+
+extern "C" int harm_diag_flux(harm_ctx* c, double Dt) {
+  if (need_geom(c)) return 1;
+  if (!acct_on(c)) return seterr("accounting is off (harm_acct_setup)");
+  if (!(Dt >= 0.0)) return seterr("Dt must be >= 0");
+  if (acct_flux(c, Dt)) return 1;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+extern "C" int harm_divbmaxavg(harm_ctx* c, int sel, double* divbmax, double* divbavg, int* imax, int* jmax) {
+  if (need_geom(c)) return 1;
+  if (sel < 0 || sel >= HARM_NARRAYS) return seterr("bad array selector %d", sel);
+  if (acct_alloc(c)) return 1;
+  AcctState* a = c->acct;
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL(k_divb, grid2(c->kp.n2 + 4, c->kp.n1 + 4, b), b, 0, c->stream, c->kp, c->arr[sel], c->gC,
+                     a->scratch);
+  hipLaunchKernelGGL(k_divb_reduce, dim3(1), dim3(64), 0, c->stream, c->kp, a->scratch, a->out);
+  HIPCHK(hipGetLastError());
+  double r[4];
+  HIPCHK(hipMemcpyAsync(r, a->out, sizeof(r), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (divbmax) *divbmax = r[0];
+  if (divbavg) *divbavg = r[1];
+  if (imax) *imax = (int)r[2];
+  if (jmax) *jmax = (int)r[3];
+  return 0;
+}
+
+extern "C" int harm_constotal(harm_ctx* c, int sel, int region, double* vars) {
+  if (need_geom(c)) return 1;
+  if (!vars) return seterr("null argument");
+  if (sel < 0 || sel >= HARM_NARRAYS) return seterr("bad array selector %d", sel);
+  if (region < 0 || region >= ACC_NREG) return seterr("region must be 0..%d", ACC_NREG - 1);
+  if (!c->acct || !c->acct->R.dx3) return seterr("regions not set (harm_acct_setup)");
+  AcctState* a = c->acct;
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL(k_cons_term, grid2(c->kp.n2, c->kp.n1, b), b, 0, c->stream, c->kp, a->R, c->arr[sel], c->gC,
+                     a->scratch);
+  hipLaunchKernelGGL(k_cons_sum, dim3(1), dim3(64), 0, c->stream, c->kp, a->R, region, a->scratch, a->out);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(vars, a->out, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}

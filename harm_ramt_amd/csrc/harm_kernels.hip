@@ -29,6 +29,8 @@
 #include <stdlib.h>
 #include <vector>
 #include <mutex>
+#include <algorithm>
+#include <stddef.h>
 #include "harm_device.h"
 #include "harm_adjoint.h"
 #include "harm_eno.h"
@@ -51,6 +53,7 @@ struct DevScalars {
   int nlist;  // entries of the failed-zone list of the current substage
   int cs_changed;  // fixup_checksolution: the second pass differs from the first
   unsigned long long nstroke;
+  double dt_step;  // dt of the step just completed (diag_flux's Dt)
 };
 
 thread_local char g_err[512];
@@ -1258,6 +1261,7 @@ constexpr int KB_DT = 4;        // t,dt update (step_ch.c:174-183)
 __device__ inline void dt_update(const KParams& P, DevScalars* ds) {
   double ndt = P.defcon * 1. / (1. / ds->ndt_last[0] + 1. / ds->ndt_last[1]);
   double dt = ds->dt;
+  ds->dt_step = dt;
   ds->t += dt;
   if (ndt > P.safe * dt) ndt = P.safe * dt;
   dt = ndt;
@@ -1784,6 +1788,10 @@ __global__ void kb_fixapply(const RunDesc* __restrict__ runs, int sel) {
 // C-ABI
 // ===========================================================================
 
+namespace {
+struct AcctState;  // conservation accounting (harm_acct.inl)
+}
+
 struct harm_ctx {
   harm_params prm;
   KParams kp;
@@ -1819,7 +1827,14 @@ struct harm_ctx {
   int evused;
   double ktime[HARM_K_COUNT];
   long long kcount[HARM_K_COUNT];
+  AcctState* acct;  // null until accounting or a dump-time reduction is first used
 };
+
+static void acct_free(harm_ctx* c);
+static bool acct_on(const harm_ctx* c);
+static void acct_set_on(harm_ctx* c, bool on);
+static int acct_source(harm_ctx* c, int pi, int pb, int pf);
+static int acct_flux(harm_ctx* c, double Dt);
 
 extern "C" const char* harm_last_error(void) { return g_err; }
 
@@ -1958,6 +1973,7 @@ extern "C" int harm_destroy(harm_ctx* c) {
     delete c->evpool;
   }
   delete c->evkid;
+  acct_free(c);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   free(c);
   return 0;
@@ -2304,11 +2320,15 @@ static int enqueue_step(harm_ctx* c) {
     // pre_fixup (step_ch.c:92) flags every substage from pii = p, which no
     // substage changes before its own update: once per step is the same
     if (to == 0 && c->kp.checksolution) launch_bsqflags(c, pii[to]);
+    // advance.c:138-140: diag_source on the last substage
+    if (to == nto - 1 && acct_on(c) && acct_source(c, pii[to], pbb[to], pff[to])) return 1;
     if (substage(c, pii[to], pbb[to], pff[to], CUf[to], Cunew[to], to, nto)) return 1;
     // step_ch.c:104-124 bound_prim, post_fixup, bound_prim; then :174-183 after the last substage
     int mode = KB_BOUND | KB_FIXUP | KB_BOUND2 | ((to == nto - 1) ? KB_DT : 0);
     if (launch_boundary(c, pff[to], mode)) return 1;
   }
+  // step_ch.c:160 diag_flux with the step's dt
+  if (acct_on(c) && acct_flux(c, -1.0)) return 1;
   return 0;
 }
 
@@ -2466,6 +2486,9 @@ static int autotune(harm_ctx* c) {
     rc = seterr("autotune: copy");
   const bool prof = c->prof;
   c->prof = false;
+  // the timed replays must not accumulate: accounting is off meanwhile
+  const bool acct = acct_on(c);
+  if (acct) acct_set_on(c, false);
   float best = 1e30f;
   int bf = HARM_FLUX_FACE_64x4, bb = HARM_BOUND_KERNELS;
   const int fv[4] = {HARM_FLUX_FACE_64x4, HARM_FLUX_FACE_64x2, HARM_FLUX_PAIR, HARM_FLUX_SHARED};
@@ -2481,6 +2504,7 @@ static int autotune(harm_ctx* c) {
       if (!rc && ms < best) { best = ms; bf = fv[f]; bb = bv[v]; }
     }
   c->prof = prof;
+  if (acct) acct_set_on(c, true);
   c->flux_variant = bf;
   c->bound_variant = bb;
   hipError_t e = hipMemcpyAsync(c->arr[HARM_P], keep, pbytes, hipMemcpyDeviceToDevice, c->stream);
@@ -2871,3 +2895,4 @@ extern "C" int harm_batch_get_tuning(harm_batch* b, int* flux_variant, int* boun
 // ===========================================================================
 namespace {
 #include "harm_adjoint.inl"
+#include "harm_acct.inl"

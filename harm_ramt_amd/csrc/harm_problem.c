@@ -623,3 +623,79 @@ int harm_init_monopole(const harm_grid *g, const double *gdet, double B0, double
     }
   return 0;
 }
+
+/* initbase.c:736-780 setflux() and :788-980 setjetflux() on one rank
+   (mycpupos = 0, ncpux = 1): region 0 the grid, region 1 the funnel
+   theta < pi/2 - h_over_r_jet, region 2 the funnel theta > pi/2 + h_over_r_jet,
+   with the reference's edge indices (region 2 starts at the zone below the
+   first x2 face past its edge). */
+int harm_ener_regions(const harm_grid *g, double h_over_r_jet, int enerpos[HARM_NUMENERREGIONS][4],
+                      int doflux[HARM_NUMENERREGIONS][4]) {
+  if (!g || !enerpos || !doflux || g->n1 < 1 || g->n2 < 1) return 1;
+  const int N1 = g->n1, N2 = g->n2;
+  double X[NDIM], r, th;
+  int *ep, *df, jetedge;
+  /* setflux */
+  ep = enerpos[0]; df = doflux[0];
+  ep[HARM_X1DN] = 0; ep[HARM_X1UP] = N1 - 1; ep[HARM_X2DN] = 0; ep[HARM_X2UP] = N2 - 1;
+  df[HARM_X1DN] = 0; df[HARM_X1UP] = N1; df[HARM_X2DN] = 0; df[HARM_X2UP] = N2;
+  /* setjetflux */
+  const double thetajet = M_PI * 0.5 - h_over_r_jet;
+  coord(g, 0, 0, FACE2, X);
+  bl_coord(g, X, &r, &th);
+  const double startth = th;
+  coord(g, 0, N2, FACE2, X);
+  bl_coord(g, X, &r, &th);
+  const double endth = th;
+  if ((fabs(startth - thetajet) / thetajet < 1E-8) || (fabs(endth - thetajet) / thetajet < 1E-8) ||
+      (fabs(startth - (M_PI - thetajet)) / thetajet < 1E-8) || (fabs(endth - (M_PI - thetajet)) / thetajet < 1E-8))
+    return 2;
+  /* inner jet */
+  ep = enerpos[1]; df = doflux[1];
+  if ((startth <= thetajet) && (endth <= thetajet)) {
+    ep[HARM_X1DN] = 0; ep[HARM_X1UP] = N1 - 1; ep[HARM_X2DN] = 0; ep[HARM_X2UP] = N2 - 1;
+    jetedge = -100;
+  } else if ((startth < thetajet) && (endth > thetajet)) {
+    ep[HARM_X1DN] = 0; ep[HARM_X1UP] = N1 - 1; ep[HARM_X2DN] = 0; ep[HARM_X2UP] = N2 - 1;
+    jetedge = -100;
+    for (int j = 0; j <= N2; j++) {
+      coord(g, 0, j, FACE2, X);
+      bl_coord(g, X, &r, &th);
+      if (th > thetajet) { ep[HARM_X2UP] = j - 1; jetedge = j; break; }
+    }
+  } else if ((startth >= thetajet) && (endth >= thetajet)) {
+    ep[HARM_X1DN] = ep[HARM_X1UP] = ep[HARM_X2DN] = ep[HARM_X2UP] = -100;
+    jetedge = -100;
+  } else {
+    return 2;
+  }
+  df[HARM_X1DN] = (ep[HARM_X1DN] != -100) ? 0 : -100;
+  df[HARM_X1UP] = (ep[HARM_X1DN] != -100) ? N1 : -100;
+  df[HARM_X2DN] = 0;
+  df[HARM_X2UP] = (jetedge != -100) ? jetedge : -100;
+  /* outer jet */
+  ep = enerpos[2]; df = doflux[2];
+  if ((startth <= M_PI - thetajet) && (endth <= M_PI - thetajet)) {
+    ep[HARM_X1DN] = ep[HARM_X1UP] = ep[HARM_X2DN] = ep[HARM_X2UP] = -100;
+    jetedge = -100;
+  } else if ((startth < M_PI - thetajet) && (endth > M_PI - thetajet)) {
+    ep[HARM_X1DN] = 0; ep[HARM_X1UP] = N1 - 1; ep[HARM_X2DN] = 0;
+    jetedge = -100;
+    for (int j = 0; j <= N2; j++) {
+      coord(g, 0, j, FACE2, X);
+      bl_coord(g, X, &r, &th);
+      if (th > M_PI - thetajet) { ep[HARM_X2DN] = j - 1; jetedge = j - 1; break; }
+    }
+    ep[HARM_X2UP] = N2 - 1;
+  } else if ((startth >= M_PI - thetajet) && (endth >= M_PI - thetajet)) {
+    ep[HARM_X1DN] = 0; ep[HARM_X1UP] = N1 - 1; ep[HARM_X2DN] = 0; ep[HARM_X2UP] = N2 - 1;
+    jetedge = -100;
+  } else {
+    return 2;
+  }
+  df[HARM_X1DN] = (ep[HARM_X1DN] != -100) ? 0 : -100;
+  df[HARM_X1UP] = (ep[HARM_X1DN] != -100) ? N1 : -100;
+  df[HARM_X2DN] = (jetedge != -100) ? jetedge : -100;
+  df[HARM_X2UP] = N2;
+  return 0;
+}

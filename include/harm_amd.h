@@ -364,6 +364,54 @@ int harm_rdump_read_host(const char *path, int binary, harm_restart_header *h, i
 int harm_rdump_write(harm_ctx *ctx, const char *path, int binary, const harm_restart_header *h);
 int harm_rdump_read(harm_ctx *ctx, const char *path, int binary, harm_restart_header *h);
 
+/* ---- conservation accounting (diag.c:576-834 diag_flux, :1231-1265
+   diag_source; dump_ener.c:449-590 divbmaxavg, constotal) ----
+   An energy region is a zone box enerpos[X1UP,X1DN,X2UP,X2DN] with the
+   faces doflux[dir] its boundary fluxes are summed on (negative: that side
+   is not summed), as setflux()/setjetflux() set them (initbase.c:736-980).
+   With accounting on, every step does what the reference's step does:
+   diag_source() on the last substage (advance.c:138-140) and diag_flux()
+   after the step (step_ch.c:160), each sum formed in the reference's order
+   so that the totals are the reference's to the bit.  The floor/failure
+   accumulators fladd/fladdterms (diag_fixup, fixup.c:174-229) are not
+   formed on the device: they read as zero. */
+#define HARM_NUMFLUXTERMS 7
+#define HARM_NUMSOURCES 7
+typedef struct {                    /* defs.h:218-239, one region */
+  double pdot[HARM_NUMDIRS][8];     /* this step's surface integrals (UDIAG form) */
+  double pdotterms[HARM_NUMDIRS][HARM_NUMFLUXTERMS][8];
+  double pcum[HARM_NUMDIRS][8];     /* sum over steps of pdot * dt */
+  double fladd[8];
+  double fladdterms[HARM_NUMFAILFLOORFLAGS][8];
+  double sourceadd[8];              /* sum over steps of the volume source */
+  double sourceaddterms[HARM_NUMSOURCES][8];
+} harm_ener_region;
+
+/* setflux() and setjetflux() (initbase.c:736-980) on a grid: region 0 the
+   whole grid, regions 1 and 2 the polar funnels within h_over_r_jet of the
+   poles (theta from bl_coord at the x2 faces).  Returns 0, 1 bad argument,
+   2 a jet edge on a grid face (the reference's "move h_over_r_jet" exit). */
+int harm_ener_regions(const harm_grid *g, double h_over_r_jet, int enerpos[HARM_NUMENERREGIONS][4],
+                      int doflux[HARM_NUMENERREGIONS][4]);
+/* Accounting on: regions (arrays [HARM_NUMENERREGIONS][4]) and dx[3]
+   (2 pi for the reference's 2D coordinates, coord.c:728); zeroes every
+   accumulator.  Face loops must lie on the grid (indices 0..N) and zone
+   boxes are clipped to the interior as ZLOOP clips them. */
+int harm_acct_setup(harm_ctx *ctx, const int *enerpos, const int *doflux, double dx3);
+int harm_acct_off(harm_ctx *ctx);
+/* the accumulators: regions[HARM_NUMENERREGIONS], pdottermsjet2[4][7][8],
+   frdot[N1][8] (radial flux per radius), lumvsr[N1]; any may be NULL */
+int harm_acct_read(harm_ctx *ctx, harm_ener_region *regions, double *pdottermsjet2, double *frdot,
+                   double *lumvsr);
+/* diag_flux() of array HARM_P and the last fluxes with time step Dt (the
+   steps call it themselves while accounting is on) */
+int harm_diag_flux(harm_ctx *ctx, double Dt);
+/* divbmaxavg() of array sel: the largest corner |div B| (first in loop
+   order) with its corner, and the sum over the corners / (N1 N2) */
+int harm_divbmaxavg(harm_ctx *ctx, int sel, double *divbmax, double *divbavg, int *imax, int *jmax);
+/* constotal() of array sel over a region: sum of U (UDIAG form) dVF */
+int harm_constotal(harm_ctx *ctx, int sel, int region, double *vars);
+
 #ifdef __cplusplus
 }
 #endif

@@ -71,6 +71,12 @@ struct oracle_ctx {
   double t, dt, tf;
   long nstroke;
   int failed;
+  int acct;          /* conservation accounting on */
+  int finalstep;     /* the substage being advanced is the step's last */
+  or_acct A;
+  double dx3;
+  double *frdot;     /* [n1][8] */
+  double *lumvsr;    /* [n1] */
 };
 
 typedef struct {
@@ -261,6 +267,91 @@ static int limit_gamma(double gammamax, double *pr, const geom_t *g) {
   return 0;
 }
 
+/* ---------------- conservation accounting ---------------- */
+
+/* failure/floor categories (global.h:536-544) */
+#define COUNTUTOPRIMFAILCONV 0
+#define COUNTFLOORACT 1
+#define COUNTLIMITGAMMAACT 2
+#define COUNTUTOPRIMFAILRHONEG 4
+#define COUNTUTOPRIMFAILUNEG 5
+#define COUNTUTOPRIMFAILRHOUNEG 6
+#define COUNTGAMMAPERC 7
+#define COUNTUPERC 8
+
+static int in_region(const int *ep, int i, int j) {
+  return (i >= ep[OR_X1DN]) && (i <= ep[OR_X1UP]) && (j >= ep[OR_X2DN]) && (j <= ep[OR_X2UP]);
+}
+
+/* primtoU with returntype UDIAG: the geometry-free conserved quantities of
+   phys.c:16-56 (rest mass kept in UU) times gdet (UtoU UNOTHING->UDIAG,
+   phys.c:219-265) */
+static void primtoU_diag(const oracle_ctx *c, const double *pr, const state_t *q, const geom_t *g, double *U) {
+  double fin[NPR];
+  fin[RHO] = pr[RHO] * q->ucon[0];
+  mhd_calc_0(c, pr, 0, q, &fin[UU]);
+  fin[B1] = pr[B1]; fin[B2] = pr[B2]; fin[B3] = pr[B3];
+  for (int k = 0; k < NPR; k++) U[k] = fin[k] * g->g;
+}
+
+/* UtoU(UEVOLVE -> UDIAG) (phys.c:219-265): divide out e[k], add the rest
+   mass back into UU (REMOVERESTMASSFROMUU, not force-free), times gdet */
+static void evolve_to_diag(const oracle_ctx *c, const geom_t *g, const double *Uin, double *Uout) {
+  double u[NPR];
+  for (int k = 0; k < NPR; k++) u[k] = Uin[k] / g->e[k];
+  if (c->p.eomtype != OR_FFDE) u[UU] += -u[RHO];
+  for (int k = 0; k < NPR; k++) Uout[k] = u[k] * g->g;
+}
+
+/* fixup.c:174-229 diag_fixup: dVF (U(pr) - U(pr0)) in every region holding
+   zone (i,j) of the geometry, on an accounting (final) substage */
+static void diag_fixup(oracle_ctx *c, const double *pr0, const double *pr, const geom_t *g, int finalstep,
+                       int whocalled) {
+  if (!c->acct || !finalstep) return;
+  for (int r = 0; r < OR_NREG; r++) {
+    if (!in_region(c->A.enerpos[r], g->i, g->j)) continue;
+    state_t q;
+    double Ui[NPR], Uf[NPR];
+    get_state(pr0, g, &q);
+    primtoU_diag(c, pr0, &q, g, Ui);
+    get_state(pr, g, &q);
+    primtoU_diag(c, pr, &q, g, Uf);
+    for (int k = 0; k < NPR; k++) {
+      double ftemp = c->A.dVF * (Uf[k] - Ui[k]);
+      c->A.fladdterms[r][whocalled][k] += ftemp;
+      c->A.fladd[r][k] += ftemp;
+    }
+  }
+}
+
+/* limit_gamma with its accounting (fixup.c:1340-1350): every caller on the
+   path passes finalstep=-1.0, which counts */
+static int limit_gamma_acct(oracle_ctx *c, double gammamax, double *pr, const geom_t *g) {
+  double pr0[NPR];
+  if (c->acct) memcpy(pr0, pr, sizeof(pr0));
+  int r = limit_gamma(gammamax, pr, g);
+  if (r == -1) diag_fixup(c, pr0, pr, g, 1, COUNTLIMITGAMMAACT);
+  return r;
+}
+
+/* diag.c:1231-1265 diag_source on the accounting substage (Dt == dt), with
+   the cooling sources (sc >= 1) zero */
+static void diag_source(oracle_ctx *c, const geom_t *g, const double *dUgeomsrc, double Dt) {
+  for (int r = 0; r < OR_NREG; r++) {
+    if (!in_region(c->A.enerpos[r], g->i, g->j)) continue;
+    for (int sc = 0; sc < OR_NUMSOURCES; sc++) {
+      double ftemp[NPR], fdiag[NPR];
+      for (int k = 0; k < NPR; k++) ftemp[k] = Dt * c->A.dVF * (sc == 0 ? dUgeomsrc[k] : 0.0);
+      evolve_to_diag(c, g, ftemp, fdiag);
+      for (int k = 0; k < NPR; k++) {
+        c->A.sourceaddterms[r][sc][k] += fdiag[k];
+        c->A.sourceadd[r][k] += fdiag[k];
+        if (k == UU && r == 0) c->lumvsr[g->i] += fdiag[k];
+      }
+    }
+  }
+}
+
 /* ---------------- reconstruction (interp.c) ---------------- */
 
 /* interp.c:106-149 slope_lim_3points */
@@ -371,7 +462,7 @@ int or_fluxcalc(oracle_ctx *c, int pr_sel, int dir, double *ndt) {
 
   double mindt = 1.e9;
   /* FZLOOP(-jdel,-idel) (step_ch.c:836-1069) */
-#pragma omp parallel for schedule(static) reduction(min : mindt) reduction(| : fail)
+#pragma omp parallel for schedule(static) reduction(min : mindt) reduction(| : fail) if (!c->acct)
   for (int i = -jdel; i <= n1; i++)
     for (int j = -idel; j <= n2; j++) {
       double p_l[NPR], p_r[NPR], F_l[NPR], F_r[NPR], U_l[NPR], U_r[NPR];
@@ -390,8 +481,8 @@ int or_fluxcalc(oracle_ctx *c, int pr_sel, int dir, double *ndt) {
       get_geometry(c, i, j, face, &g);
       if (dir == 1) p_l[B1] = p_r[B1] = 0.5 * (p_l[B1] + p_r[B1]);
       else p_l[B2] = p_r[B2] = 0.5 * (p_l[B2] + p_r[B2]);
-      if (limit_gamma(c->p.gammamax, p_l, &g) >= 1) fail |= 1;
-      if (limit_gamma(c->p.gammamax, p_r, &g) >= 1) fail |= 1;
+      if (limit_gamma_acct(c, c->p.gammamax, p_l, &g) >= 1) fail |= 1;
+      if (limit_gamma_acct(c, c->p.gammamax, p_r, &g) >= 1) fail |= 1;
       get_state(p_l, &g, &sl);
       get_state(p_r, &g, &sr);
       primtoflux(c, p_l, &sl, dir, &g, F_l);
@@ -486,8 +577,9 @@ int or_flux_ct(oracle_ctx *c) {
 
 /* ---------------- sources (phys.c) ---------------- */
 
-/* phys.c:475-510 source + :520-619 source_conn (WITHGDET, cooling=0) */
-static void source(const oracle_ctx *c, const double *pr, const geom_t *g, double *dU) {
+/* phys.c:475-510 source + :520-619 source_conn (WITHGDET, cooling=0);
+   dUcomp0 (may be NULL) receives dUcomp[GEOMSOURCE] */
+static void source(const oracle_ctx *c, const double *pr, const geom_t *g, double *dU, double *dUcomp0) {
   state_t q;
   double mhd[NDIM][NDIM], dUconn[NPR];
   const double *cn = c->conn + (size_t)zi(c, g->i, g->j) * 64;
@@ -500,6 +592,8 @@ static void source(const oracle_ctx *c, const double *pr, const geom_t *g, doubl
         dUconn[UU + l] += mhd[j][k] * cn[(k * 4 + l) * 4 + j];
   /* dUcomp[GEOMSOURCE]=e*dUconn, all other components 0; dU = sum over them */
   for (int k = 0; k < NPR; k++) dU[k] = g->e[k] * dUconn[k];
+  if (dUcomp0)
+    for (int k = 0; k < NPR; k++) dUcomp0[k] = 0.0 * g->e[k] + g->e[k] * dUconn[k];
 }
 
 /* ---------------- inversion (utoprim_2d_final.c, u2p_util.c) ---------------- */
@@ -765,17 +859,26 @@ static int Utoprimgen(const oracle_ctx *c, const double *U, const geom_t *g, dou
   return Utoprim_2d_final(c, Ug, g, pr, pflag, nstroke);
 }
 
-/* fixup.c:274-377 fixup1zone (floors from the precomputed table, then limit_gamma) */
-static int fixup1zone(const oracle_ctx *c, double *pr, const geom_t *g) {
+/* fixup.c:274-377 fixup1zone (floors from the precomputed table, then
+   limit_gamma); a change is accounted as COUNTFLOORACT on the final substage
+   (measured from the state before the floors).  Force-free leaves pr0
+   unset in the reference (fixup.c:293-313); here it is the incoming state. */
+static int fixup1zone(oracle_ctx *c, double *pr, const geom_t *g, int finalstep) {
+  double pr0[NPR];
+  int changed = 0;
+  memcpy(pr0, pr, sizeof(pr0));
   if (c->p.eomtype == OR_GRMHD) {
     const double *fl = c->floors + (size_t)zi(c, g->i, g->j) * 2;
     double prfloor[2] = {fl[0], fl[1]};
     if (prfloor[0] < c->p.rhominlimit) prfloor[0] = c->p.rhominlimit;
     if (prfloor[1] < c->p.uuminlimit) prfloor[1] = c->p.uuminlimit;
-    if (prfloor[0] > pr[RHO]) pr[RHO] = prfloor[0];
-    if (prfloor[1] > pr[UU]) pr[UU] = prfloor[1];
+    if (prfloor[0] > pr[RHO]) { pr[RHO] = prfloor[0]; changed = 1; }
+    if (prfloor[1] > pr[UU]) { pr[UU] = prfloor[1]; changed = 1; }
   }
-  if (limit_gamma(c->p.gammamax, pr, g) >= 1) return 1;
+  int r = limit_gamma_acct(c, c->p.gammamax, pr, g);
+  if (r >= 1) return 1;
+  if (r == -1) changed = 1;
+  if (changed) diag_fixup(c, pr0, pr, g, finalstep, COUNTFLOORACT);
   return 0;
 }
 
@@ -923,7 +1026,7 @@ static void dUtoU(const double *dUgeom, const double *dUriemann, const double *C
 static int invert_zone(oracle_ctx *c, int i, int j, const geom_t *g, const double *U, double *pfz, long *nstroke) {
   int pfl, fail = Utoprimgen(c, U, g, pfz, &pfl, nstroke);
   c->pflag[(size_t)zi(c, i, j) * NUMPFLAGS + 0] = pfl;
-  if (fixup1zone(c, pfz, g)) fail |= 1;
+  if (fixup1zone(c, pfz, g, c->finalstep)) fail |= 1;
   return fail;
 }
 
@@ -954,15 +1057,16 @@ static int advance_eno_u(oracle_ctx *c, double *pi, double *pb, const double *CU
   eno_convert(c, 1, ENO_C2A, e0, e1); /* Uiavgtmp */
   eno_convert(c, 2, ENO_C2A, e1, e2); /* Uiavg */
   /* average Uf (or unew on the last substage) into Ufavg (uenotmp0) */
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (!c->acct)
   for (int i = 0; i < n1; i++)
     for (int j = 0; j < n2; j++) {
       geom_t g;
-      double Ui[NPR], Uf[NPR], dUgeom[NPR], d1[NPR], d2[NPR], dUriemann[NPR], pbz[NPR];
+      double Ui[NPR], Uf[NPR], dUgeom[NPR], d1[NPR], d2[NPR], dUriemann[NPR], pbz[NPR], dUc0[NPR];
       double *ul = PA(c, ulast, i, j), *un = PA(c, unew, i, j);
       get_geometry(c, i, j, CENT, &g);
       for (int k = 0; k < NPR; k++) pbz[k] = PA(c, pb, i, j)[k];
-      source(c, pbz, &g, dUgeom);
+      source(c, pbz, &g, dUgeom, dUc0);
+      if (c->acct && c->finalstep) diag_source(c, &g, dUc0, dt);
       for (int k = 0; k < NPR; k++) Ui[k] = PA(c, e2, i, j)[k];
       if (timeorder == 0) for (int k = 0; k < NPR; k++) ul[k] = un[k] = 0.0;
       flux2dUavg(c, i, j, d1, d2);
@@ -973,7 +1077,7 @@ static int advance_eno_u(oracle_ctx *c, double *pi, double *pb, const double *CU
     }
   eno_convert(c, 1, ENO_A2C, e0, e1); /* Ufcentertmp */
   eno_convert(c, 2, ENO_A2C, e1, e2); /* Ufcenter */
-#pragma omp parallel for schedule(dynamic, 4) reduction(+ : nstroke) reduction(| : fail)
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : nstroke) reduction(| : fail) if (!c->acct)
   for (int i = 0; i < n1; i++)
     for (int j = 0; j < n2; j++) {
       geom_t g;
@@ -996,6 +1100,7 @@ int or_advance(oracle_ctx *c, int pi_sel, int pb_sel, const double *CUf, int pf_
   double dt = c->dt;
   long nstroke = 0;
   int fail = 0;
+  c->finalstep = (timeorder == numtimeorders - 1);
   fail |= or_fluxcalc(c, pb_sel, 1, &ndt1);
   fail |= or_fluxcalc(c, pb_sel, 2, &ndt2);
   /* advance_standard calls fix_flux under FIXUPFLUX (advance.c:89-91),
@@ -1010,10 +1115,10 @@ int or_advance(oracle_ctx *c, int pi_sel, int pb_sel, const double *CUf, int pf_
       for (int i = 0; i < n1; i++)
         for (int j = 0; j < n2; j++) flux2dUavg(c, i, j, PA(c, c->eno[0], i, j), PA(c, c->eno[1], i, j));
     }
-#pragma omp parallel for schedule(dynamic, 4) reduction(+ : nstroke) reduction(| : fail)
+#pragma omp parallel for schedule(dynamic, 4) reduction(+ : nstroke) reduction(| : fail) if (!c->acct)
     for (int i = 0; i < n1; i++)
       for (int j = 0; j < n2; j++) {
-        double Uf[NPR], Ui[NPR], dUgeom[NPR], dUriemann[NPR], d1[NPR], d2[NPR];
+        double Uf[NPR], Ui[NPR], dUgeom[NPR], dUriemann[NPR], d1[NPR], d2[NPR], dUc0[NPR];
         geom_t g;
         state_t q;
         double *pfz = PA(c, pf, i, j), *piz = PA(c, pi, i, j);
@@ -1028,7 +1133,9 @@ int or_advance(oracle_ctx *c, int pi_sel, int pb_sel, const double *CUf, int pf_
         {
           double pbz[NPR];
           for (int k = 0; k < NPR; k++) pbz[k] = PA(c, pb, i, j)[k];
-          source(c, pbz, &g, dUgeom);
+          source(c, pbz, &g, dUgeom, dUc0);
+          /* advance.c:138-140: diag_source with dt4diag = dt on the last substage */
+          if (c->acct && c->finalstep) diag_source(c, &g, dUc0, dt);
         }
         if (c->p.eno == 2) {
           eno_cvtij(c, 1, ENO_A2C, c->eno[0], i, j, d1);
@@ -1277,6 +1384,8 @@ static int fixup_utoprim(oracle_ctx *c, double *pv) {
       int mypflag = PF(i, j);
       if (mypflag == 0) continue;
       double *pz = PA(c, pv, i, j);
+      double pr0[NPR];
+      memcpy(pr0, pz, sizeof(pr0));
       geom_t g;
       get_geometry(c, i, j, CENT, &g);
       double gamma = gamma_calc(pz, &g);
@@ -1313,8 +1422,21 @@ static int fixup_utoprim(oracle_ctx *c, double *pv) {
       } else {
         for (k = 0; k <= U3; k++) pz[k] = 0.5 * (AVG4_1(i, j, k) + AVG4_2(i, j, k));
       }
-      if (limit_gamma(gamma, pz, &g) >= 1) c->failed = 1;
-      /* accounting (fixup.c:870-915): flag becomes UTOPRIMFAILFIXED */
+      if (limit_gamma_acct(c, gamma, pz, &g) >= 1) c->failed = 1;
+      /* accounting (fixup.c:870-915): the failure type's category
+         (STEPOVERNEGU=0), then the flag becomes UTOPRIMFAILFIXED */
+      {
+        int type;
+        if (mypflag == 1 || mypflag == UTOPRIMFAILCONVUTSQ || mypflag >= 101 || mypflag == UTOPRIMFAILCONVW ||
+            mypflag == UTOPRIMFAILCONVUTSQ2)
+          type = COUNTUTOPRIMFAILCONV;
+        else if (mypflag == UTOPRIMFAILRHONEG) type = COUNTUTOPRIMFAILRHONEG;
+        else if (mypflag == UTOPRIMFAILUNEG) type = COUNTUTOPRIMFAILUNEG;
+        else if (mypflag == UTOPRIMFAILRHOUNEG) type = COUNTUTOPRIMFAILRHOUNEG;
+        else if (mypflag == 8) type = COUNTGAMMAPERC;
+        else type = COUNTUPERC;
+        diag_fixup(c, pr0, pz, &g, c->finalstep, type);
+      }
       PF(i, j) = -1;
       nfixed++;
     }
@@ -1375,6 +1497,8 @@ int or_step(oracle_ctx *c) {
     or_post_fixup(c, pff[to]);
     fail |= or_bound_prim(c, pff[to]);
   }
+  /* step_ch.c:160 diag_flux with the step's dt */
+  if (c->acct) or_diag_flux(c, c->dt);
   c->t += c->dt;
   if (ndt > c->p.safe * c->dt) ndt = c->p.safe * c->dt;
   c->dt = ndt;
@@ -1424,7 +1548,7 @@ int or_vchar(oracle_ctx *c, int i, int j, int loc, const double *pr, int dir, do
 int or_source(oracle_ctx *c, int i, int j, const double *pr, double *dU) {
   geom_t g;
   get_geometry(c, i, j, CENT, &g);
-  source(c, pr, &g, dU);
+  source(c, pr, &g, dU, NULL);
   return 0;
 }
 
@@ -1445,6 +1569,8 @@ oracle_ctx *or_create(const oracle_params *prm) {
   c->pflag = (int *)calloc((size_t)c->nz * NUMPFLAGS, sizeof(int));
   c->gammacheck = (double *)calloc((size_t)(prm->n1 + 2) * (prm->n2 + 2), sizeof(double));
   for (int a = 0; a < 3; a++) c->eno[a] = (double *)calloc((size_t)c->nz * NPR, sizeof(double));
+  c->frdot = (double *)calloc((size_t)prm->n1 * NPR, sizeof(double));
+  c->lumvsr = (double *)calloc((size_t)prm->n1, sizeof(double));
   c->tf = 1e30;
 #ifdef _OPENMP
   if (prm->nthreads > 0) omp_set_num_threads(prm->nthreads);
@@ -1458,6 +1584,7 @@ void or_destroy(oracle_ctx *c) {
   for (int a = 0; a < 3; a++) free(c->eno[a]);
   for (int a = 0; a < 12; a++) free(c->arr[a]);
   free(c->pflag);
+  free(c->frdot); free(c->lumvsr);
   free(c);
 }
 
@@ -1476,3 +1603,193 @@ void or_set_time(oracle_ctx *c, double t, double dt, double tf) { c->t = t; c->d
 void or_get_time(const oracle_ctx *c, double *t, double *dt) { *t = c->t; *dt = c->dt; }
 long or_nstroke(const oracle_ctx *c) { return c->nstroke; }
 int or_failed(const oracle_ctx *c) { return c->failed; }
+
+/* ---------------- conservation accounting: entry points ---------------- */
+
+void or_acct_setup(oracle_ctx *c, const int *enerpos, const int *doflux, double dx3) {
+  memset(&c->A, 0, sizeof(c->A));
+  memcpy(c->A.enerpos, enerpos, sizeof(c->A.enerpos));
+  memcpy(c->A.doflux, doflux, sizeof(c->A.doflux));
+  c->dx3 = dx3;
+  c->A.dVF = (c->p.dx1 * c->p.dx2) * dx3; /* set_grid.c:17-19 dV, dVF */
+  memset(c->frdot, 0, sizeof(double) * (size_t)c->p.n1 * NPR);
+  memset(c->lumvsr, 0, sizeof(double) * (size_t)c->p.n1);
+  c->acct = 1;
+}
+
+void or_acct_off(oracle_ctx *c) { c->acct = 0; }
+
+void or_acct_get(const oracle_ctx *c, or_acct *out, double *frdot, double *lumvsr) {
+  if (out) *out = c->A;
+  if (frdot) memcpy(frdot, c->frdot, sizeof(double) * (size_t)c->p.n1 * NPR);
+  if (lumvsr) memcpy(lumvsr, c->lumvsr, sizeof(double) * (size_t)c->p.n1);
+}
+
+/* diag.c:576-834 diag_flux (JETBSQORHO 3.162; the step had no failed zone,
+   so get_state runs on every zone) */
+void or_diag_flux(oracle_ctx *c, double Dt) {
+  const double *prim = c->arr[OR_P];
+  or_acct *A = &c->A;
+  const double gam = c->p.gam;
+  for (int r = 0; r < OR_NREG; r++) {
+    const int *doflux = A->doflux[r], *ep = A->enerpos[r];
+    for (int dir = 0; dir < 4; dir++)
+      for (int k = 0; k < NPR; k++) {
+        A->pdot[r][dir][k] = 0;
+        for (int fl = 0; fl < OR_NUMFLUXTERMS; fl++) A->pdotterms[r][dir][fl][k] = 0;
+        if (r == 0)
+          for (int fl = 0; fl < OR_NUMFLUXTERMS; fl++) A->pdottermsjet2[dir][fl][k] = 0;
+      }
+    for (int dir = 0; dir < 4; dir++) {
+      if (doflux[dir] < 0) continue;
+      int start1, stop1, start2, stop2, loc, fluxdir;
+      double surface;
+      const double *flux;
+      if (dir == OR_X1UP || dir == OR_X1DN) {
+        surface = c->p.dx2 * c->dx3;
+        start1 = stop1 = doflux[dir];
+        start2 = ep[OR_X2DN]; stop2 = ep[OR_X2UP];
+        flux = c->arr[OR_F1]; loc = FACE1; fluxdir = 1;
+      } else {
+        surface = c->p.dx1 * c->dx3;
+        start2 = stop2 = doflux[dir];
+        start1 = ep[OR_X1DN]; stop1 = ep[OR_X1UP];
+        flux = c->arr[OR_F2]; loc = FACE2; fluxdir = 2;
+      }
+      double *pdot = A->pdot[r][dir];
+      double (*terms)[8] = A->pdotterms[r][dir];
+      double (*jet2)[8] = A->pdottermsjet2[dir];
+      for (int i = start1; i <= stop1; i++)
+        for (int j = start2; j <= stop2; j++) {
+          geom_t g;
+          state_t q;
+          double Ftemp[NPR], Fd[NPR], pr[NPR];
+          get_geometry(c, i, j, loc, &g);
+          for (int k = 0; k < NPR; k++) Ftemp[k] = PA(c, flux, i, j)[k] * surface;
+          evolve_to_diag(c, &g, Ftemp, Fd);
+          for (int k = 0; k < NPR; k++) pdot[k] += Fd[k];
+          get_geometry(c, i, j, CENT, &g);
+          for (int k = 0; k < NPR; k++) pr[k] = PA(c, prim, i, j)[k];
+          get_state(pr, &g, &q);
+          double pgas = (gam - 1.0) * pr[UU];
+          double bsq = 0;
+          for (int l = 0; l < NDIM; l++) bsq += (q.bcon[l]) * (q.bcov[l]);
+          int condjet2 = 0;
+          if (r == 0) {
+            double bsqorho = bsq / pr[RHO];
+            if (dir == OR_X1DN) condjet2 = (bsqorho > 3.162);
+            else if (dir == OR_X1UP) condjet2 = ((q.ucon[1] > 0.0) && (-q.ucov[0] - 1.0 > 0.0));
+          }
+          double surgdet = (g.g) * surface;
+          double f[OR_NUMFLUXTERMS];
+          for (int k = 0; k < NPR; k++) {
+            int nf = 0;
+            if (k == RHO) {
+              f[0] = pr[k] * (q.ucon[fluxdir]) * surgdet;
+              nf = 1;
+            } else if (k >= UU && k <= U3) {
+              int l = k - UU;
+              double d = (fluxdir == l) ? 1.0 : 0.0; /* delta(), phys.c */
+              f[0] = pgas * (q.ucon[fluxdir]) * (q.ucov[l]) * surgdet;
+              f[1] = PA(c, prim, i, j)[RHO] * (q.ucon[fluxdir]) * (q.ucov[l]) * surgdet;
+              f[2] = PA(c, prim, i, j)[UU] * (q.ucon[fluxdir]) * (q.ucov[l]) * surgdet;
+              f[3] = bsq * (q.ucon[fluxdir]) * (q.ucov[l]) * surgdet;
+              f[4] = pgas * d * surgdet;
+              f[5] = 0.5 * bsq * d * surgdet;
+              f[6] = -(q.bcon[fluxdir]) * (q.bcov[l]) * surgdet;
+              nf = 7;
+            } else {
+              int m = k - B1 + 1;
+              f[0] = (q.bcon[m]) * (q.ucon[fluxdir]) * surgdet;
+              f[1] = -(q.bcon[fluxdir]) * (q.ucon[m]) * surgdet;
+              nf = 2;
+            }
+            for (int fl = 0; fl < nf; fl++) {
+              terms[fl][k] += f[fl];
+              if (condjet2) jet2[fl][k] += f[fl];
+            }
+          }
+        }
+      for (int k = 0; k < NPR; k++) A->pcum[r][dir][k] += pdot[k] * Dt;
+    }
+  }
+  /* radial flux vs radius (diag.c:816-827) */
+  const double surface = c->p.dx2 * c->dx3;
+  for (int i = 0; i < c->p.n1; i++) {
+    double *fr = c->frdot + (size_t)i * NPR;
+    for (int k = 0; k < NPR; k++) fr[k] = 0;
+    for (int j = 0; j < c->p.n2; j++) {
+      geom_t g;
+      double Ftemp[NPR], Fd[NPR];
+      get_geometry(c, i, j, FACE1, &g);
+      for (int k = 0; k < NPR; k++) Ftemp[k] = PA(c, c->arr[OR_F1], i, j)[k] * surface;
+      evolve_to_diag(c, &g, Ftemp, Fd);
+      for (int k = 0; k < NPR; k++) fr[k] += Fd[k];
+    }
+  }
+}
+
+/* dump_ener.c:449-486 divbmaxavg with SETFDIVBFLUXCT (global.h:1152-1175):
+   the corner divergence over LOOPDIVB (i in [-2,N1+2), j in [-2,N2+2)),
+   zero on the rows j=0 and j=N2; the first largest and the sum over
+   N1*N2 */
+void or_divbmaxavg(const oracle_ctx *c, int sel, double *divbmax_out, double *divbavg_out, int *imax_out,
+                   int *jmax_out) {
+  const double *p = c->arr[sel];
+  const int n1 = c->p.n1, n2 = c->p.n2;
+  const double dx1 = c->p.dx1, dx2 = c->p.dx2;
+  int imax = 0, jmax = 0;
+  double divbmax = 0, divbavg = 0;
+#define GC(i, j) c->gdet[(size_t)zi(c, (i), (j)) * 4 + CENT]
+#define BB(i, j, k) PA(c, p, (i), (j))[k]
+  for (int i = -2; i < n1 + 2; i++)
+    for (int j = -2; j < n2 + 2; j++) {
+      double divb;
+      if ((j != 0) && (j != n2)) {
+        double num = (((dx1) > (dx2)) ? (dx1) : (dx2)) *
+                     (0.5 * (BB(i, j, B1) * GC(i, j) + BB(i, j - 1, B1) * GC(i, j - 1) - BB(i - 1, j, B1) * GC(i - 1, j) -
+                             BB(i - 1, j - 1, B1) * GC(i - 1, j - 1)) /
+                          dx1 +
+                      0.5 * (BB(i, j, B2) * GC(i, j) + BB(i - 1, j, B2) * GC(i - 1, j) - BB(i, j - 1, B2) * GC(i, j - 1) -
+                             BB(i - 1, j - 1, B2) * GC(i - 1, j - 1)) /
+                          dx2);
+        double sum = BB(i, j, B1) * GC(i, j) + BB(i, j - 1, B1) * GC(i, j - 1) + BB(i - 1, j, B1) * GC(i - 1, j) +
+                     BB(i - 1, j - 1, B1) * GC(i - 1, j - 1) + BB(i, j, B2) * GC(i, j) + BB(i - 1, j, B2) * GC(i - 1, j) +
+                     BB(i, j - 1, B2) * GC(i, j - 1) + BB(i - 1, j - 1, B2) * GC(i - 1, j - 1);
+        divb = fabs(num / (c->gdet[(size_t)zi(c, i, j) * 4 + CORN] * 0.125 * fabs(fabs(sum) + 1.e-20)));
+      } else {
+        divb = 0.;
+      }
+      if (divb > divbmax) { imax = i; jmax = j; divbmax = divb; }
+      divbavg += divb;
+    }
+#undef GC
+#undef BB
+  divbavg /= (double)(n1 * n2);
+  *divbmax_out = divbmax; *divbavg_out = divbavg;
+  if (imax_out) *imax_out = imax;
+  if (jmax_out) *jmax_out = jmax;
+}
+
+/* dump_ener.c:562-590 constotal: sum over the region's zones of
+   primtoU(UDIAG) * dVF, in ZLOOP order */
+void or_constotal(const oracle_ctx *c, int sel, int region, double *vars) {
+  const double *p = c->arr[sel];
+  const int *ep = c->A.enerpos[region];
+  for (int k = 0; k < NPR; k++) vars[k] = 0.0;
+  for (int i = 0; i < c->p.n1; i++)
+    for (int j = 0; j < c->p.n2; j++) {
+      if (!in_region(ep, i, j)) continue;
+      geom_t g;
+      state_t q;
+      double U[NPR], pr[NPR];
+      get_geometry(c, i, j, CENT, &g);
+      for (int k = 0; k < NPR; k++) pr[k] = PA(c, p, i, j)[k];
+      get_state(pr, &g, &q);
+      primtoU_diag(c, pr, &q, &g, U);
+      for (int k = 0; k < NPR; k++) {
+        double ftemp = U[k] * c->A.dVF;
+        vars[k] += ftemp;
+      }
+    }
+}

@@ -95,6 +95,37 @@ double or_eno_line(int type, const double *line, int n, int pos, int *order, int
    position s of the stencil, cell d (0 when there is no such table) */
 double or_eno_weight(int type, int order, int s, int d);
 
+/* ---- conservation accounting (diag.c, fixup.c, dump_ener.c) ----
+   Energy regions are zone boxes enerpos[r][X1UP,X1DN,X2UP,X2DN] with the
+   boundary faces doflux[r][dir] (negative: no flux through that side), as
+   setflux()/setjetflux() set them (initbase.c:736-980); dx3 is dx[3]
+   (2 pi in the reference's 2D coordinates, coord.c:728). */
+#define OR_NREG 3
+#define OR_NUMFLUXTERMS 7
+#define OR_NUMSOURCES 7
+#define OR_NUMFAILFLOORFLAGS 9
+typedef struct {
+  int enerpos[OR_NREG][4], doflux[OR_NREG][4];
+  double dVF;
+  /* the accumulators of defs.h:218-239, per region */
+  double pdot[OR_NREG][4][8], pdotterms[OR_NREG][4][OR_NUMFLUXTERMS][8], pcum[OR_NREG][4][8];
+  double fladd[OR_NREG][8], fladdterms[OR_NREG][OR_NUMFAILFLOORFLAGS][8];
+  double sourceadd[OR_NREG][8], sourceaddterms[OR_NREG][OR_NUMSOURCES][8];
+  double pdottermsjet2[4][OR_NUMFLUXTERMS][8];
+} or_acct;
+/* set the regions, zero every accumulator and switch accounting on (the
+   steps then run their zone loops in the reference's order, one thread) */
+void or_acct_setup(oracle_ctx *c, const int *enerpos, const int *doflux, double dx3);
+void or_acct_off(oracle_ctx *c);
+/* the accumulators, frdot[n1][8] and lumvsr[n1] */
+void or_acct_get(const oracle_ctx *c, or_acct *out, double *frdot, double *lumvsr);
+/* diag_flux() of the current state and fluxes with time step Dt (the step
+   calls it itself when accounting is on) */
+void or_diag_flux(oracle_ctx *c, double Dt);
+/* dump-time reductions of array sel: divbmaxavg() and constotal() */
+void or_divbmaxavg(const oracle_ctx *c, int sel, double *divbmax, double *divbavg, int *imax, int *jmax);
+void or_constotal(const oracle_ctx *c, int sel, int region, double *vars);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,23 @@ class OracleParams(ctypes.Structure):
 _lib = None
 
 
+NREG, NUMFLUXTERMS, NUMSOURCES, NUMFAILFLOORFLAGS = 3, 7, 7, 9
+
+
+class OracleAcct(ctypes.Structure):
+    """or_acct: the regions and accumulators of defs.h:218-239"""
+    _fields_ = [("enerpos", (ctypes.c_int * 4) * NREG), ("doflux", (ctypes.c_int * 4) * NREG),
+                ("dVF", ctypes.c_double),
+                ("pdot", ((ctypes.c_double * 8) * 4) * NREG),
+                ("pdotterms", (((ctypes.c_double * 8) * NUMFLUXTERMS) * 4) * NREG),
+                ("pcum", ((ctypes.c_double * 8) * 4) * NREG),
+                ("fladd", (ctypes.c_double * 8) * NREG),
+                ("fladdterms", ((ctypes.c_double * 8) * NUMFAILFLOORFLAGS) * NREG),
+                ("sourceadd", (ctypes.c_double * 8) * NREG),
+                ("sourceaddterms", ((ctypes.c_double * 8) * NUMSOURCES) * NREG),
+                ("pdottermsjet2", ((ctypes.c_double * 8) * NUMFLUXTERMS) * 4)]
+
+
 def build():
     subprocess.check_call(["make", "-C", HERE, "-s"])
 
@@ -73,6 +90,12 @@ def load():
                           ctypes.c_double, ctypes.c_double, dp, dp, dp], None),
         "or_eno_line": ([ctypes.c_int, dp, ctypes.c_int, ctypes.c_int, ip, ip], ctypes.c_double),
         "or_eno_weight": ([ctypes.c_int] * 4, ctypes.c_double),
+        "or_acct_setup": ([vp, ip, ip, ctypes.c_double], None),
+        "or_acct_off": ([vp], None),
+        "or_acct_get": ([vp, ctypes.POINTER(OracleAcct), dp, dp], None),
+        "or_diag_flux": ([vp, ctypes.c_double], None),
+        "or_divbmaxavg": ([vp, ctypes.c_int, dp, dp, ip, ip], None),
+        "or_constotal": ([vp, ctypes.c_int, ctypes.c_int, dp], None),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -209,6 +232,43 @@ class Oracle:
         dU = np.zeros(NPR)
         self.lib.or_source(self.h, i, j, _d(pr), _d(dU))
         return dU
+
+    # conservation accounting (diag.c, fixup.c, dump_ener.c)
+    def acct_setup(self, enerpos, doflux, dx3=2 * np.pi):
+        ep = np.ascontiguousarray(enerpos, dtype=np.int32)
+        df = np.ascontiguousarray(doflux, dtype=np.int32)
+        assert ep.shape == (NREG, 4) and df.shape == (NREG, 4)
+        ip = ctypes.POINTER(ctypes.c_int)
+        self.lib.or_acct_setup(self.h, ep.ctypes.data_as(ip), df.ctypes.data_as(ip), dx3)
+
+    def acct_off(self):
+        self.lib.or_acct_off(self.h)
+
+    def acct(self):
+        """the accumulators as a dict of arrays (plus frdot[n1][8], lumvsr[n1])"""
+        a = OracleAcct()
+        frdot = np.zeros((self.prm.n1, NPR))
+        lum = np.zeros(self.prm.n1)
+        self.lib.or_acct_get(self.h, ctypes.byref(a), _d(frdot), _d(lum))
+        out = {name: np.ctypeslib.as_array(getattr(a, name)).copy() for name, _ in OracleAcct._fields_
+               if name != "dVF"}
+        out["dVF"] = a.dVF
+        out["frdot"], out["lumvsr"] = frdot, lum
+        return out
+
+    def diag_flux(self, Dt):
+        self.lib.or_diag_flux(self.h, Dt)
+
+    def divbmaxavg(self, sel=P):
+        mx, av = ctypes.c_double(), ctypes.c_double()
+        im, jm = ctypes.c_int(), ctypes.c_int()
+        self.lib.or_divbmaxavg(self.h, sel, ctypes.byref(mx), ctypes.byref(av), ctypes.byref(im), ctypes.byref(jm))
+        return mx.value, av.value, im.value, jm.value
+
+    def constotal(self, region, sel=P):
+        v = np.zeros(NPR)
+        self.lib.or_constotal(self.h, sel, region, _d(v))
+        return v
 
 
 def slope_lim(lim, k, yll, yl, yc, yr, yrr):

@@ -15,8 +15,9 @@
  * user does by swapping bounds.*.c.
  *
  * usage: ref_harness CONFIG   (key value lines, see read_config())
- * With "rdump 1" it also writes the final state's restart files (see the end
- * of main()).
+ * With "rdump 1" it also writes the final state's restart files, with
+ * "diag 1" the conservation accumulators and dump-time reductions (see the
+ * end of main()).
  */
 #include "defs.h"
 #include "step_ch.h"
@@ -147,6 +148,25 @@ int main(int argc, char *argv[])
   readall(cfg("prim", ""), &a_p[0][0][0], sizeof(FTYPE) * (size_t)N1M * N2M * NPR);
   bound_prim(STAGEM1, p);
 
+  /* conservation accounting (every step runs diag_flux, step_ch.c:160, and
+     diag_source, advance.c:140, whatever this option says): with "diag 1"
+     the energy regions and volume factors are set up as init() does --
+     dV and dVF as set_grid.c:17-19 forms them, h_over_r_jet as the problem
+     files set it (init.fishmon.c:21-23), setflux()/setjetflux()
+     (initbase.c:642-643), the luminosity accumulator zeroed
+     (initbase.c:89) -- and the accumulators are written after the steps */
+  if (cfgi("diag", 0)) {
+    extern int setflux(void);
+    extern int setjetflux(void);
+    int ii;
+    dV = dx[1] * dx[2];
+    dVF = dV * dx[3];
+    h_over_r_jet = cfgd("h_over_r_jet", 0.4);
+    setflux();
+    if (DOJETDIAG) setjetflux();
+    for (ii = 0; ii < ncpux1 * N1; ii++) lumvsr[ii] = 0.0;
+  }
+
   t = 0.0;
   dt = cfgd("dt", 1e-4);
   tf = cfgd("tf", 1e30);
@@ -167,6 +187,42 @@ int main(int argc, char *argv[])
     fwrite(&a_p[0][0][0], sizeof(FTYPE), (size_t)N1M * N2M * NPR, out);
   }
   fclose(out);
+  /* the accumulators after the steps, then the reference's own dump-time
+     reductions of the final state: divbmaxavg() (dump_ener.c:449) and
+     constotal() of each region (dump_ener.c:562) */
+  if (cfgi("diag", 0)) {
+    extern void divbmaxavg(FTYPE p[][N2M][NPR], SFTYPE *ptrdivbmax, SFTYPE *ptrdivbavg);
+    extern int constotal(int enerregion, SFTYPE *vars);
+    SFTYPE divbmax, divbavg, vars[NPR];
+    double d;
+    int r, dir;
+    FILE *f = fopen(cfg("diag_out", "diag.bin"), "wb");
+    for (r = 0; r < NUMENERREGIONS; r++) for (dir = 0; dir < COMPDIM * 2; dir++) {
+      d = enerposreg[r][dir]; fwrite(&d, sizeof(double), 1, f);
+    }
+    for (r = 0; r < NUMENERREGIONS; r++) for (dir = 0; dir < COMPDIM * 2; dir++) {
+      d = dofluxreg[r][dir]; fwrite(&d, sizeof(double), 1, f);
+    }
+    fwrite(pdotreg, sizeof(pdotreg), 1, f);
+    fwrite(pdottermsreg, sizeof(pdottermsreg), 1, f);
+    fwrite(pcumreg, sizeof(pcumreg), 1, f);
+    fwrite(fladdreg, sizeof(fladdreg), 1, f);
+    fwrite(fladdtermsreg, sizeof(fladdtermsreg), 1, f);
+    fwrite(sourceaddreg, sizeof(sourceaddreg), 1, f);
+    fwrite(sourceaddtermsreg, sizeof(sourceaddtermsreg), 1, f);
+    fwrite(pdottermsjet2, sizeof(pdottermsjet2), 1, f);
+    fwrite(frdot, sizeof(frdot), 1, f);
+    fwrite(lumvsr, sizeof(SFTYPE), (size_t)ncpux1 * N1, f);
+    divbmaxavg(p, &divbmax, &divbavg);
+    fwrite(&divbmax, sizeof(double), 1, f);
+    fwrite(&divbavg, sizeof(double), 1, f);
+    for (r = 0; r < NUMENERREGIONS; r++) {
+      constotal(r, vars);
+      fwrite(vars, sizeof(double), NPR, f);
+    }
+    d = dVF; fwrite(&d, sizeof(double), 1, f);
+    fclose(f);
+  }
   /* optional: the restart file of the final state, written by the
      reference's own restart_write() (restart.c:182, through dump_gen),
      as text (the default binaryoutput, initbase.c:48) into dumps/rdump-0 and

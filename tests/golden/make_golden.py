@@ -93,7 +93,7 @@ def bc_of(kind):
     return (H.OUTFLOW, H.OUTFLOW, H.POLARAXIS, H.POLARAXIS)
 
 
-def run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, eno=0, rdump=False):
+def run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, eno=0, rdump=False, diag=False):
     p0.tofile(os.path.join(work, "p0.bin"))
     cfg = dict(prim=os.path.join(work, "p0.bin"), dt=dt0, nsteps=nsteps, out=os.path.join(work, "o.bin"),
                lim=lim, fluxmethod=flux, timeorder=to)
@@ -102,6 +102,8 @@ def run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, eno=0, rd
     if rdump:
         cfg["rdump"] = 1
         os.makedirs(os.path.join(work, "dumps"), exist_ok=True)
+    if diag:
+        cfg.update(diag=1, diag_out=os.path.join(work, "diag.bin"), h_over_r_jet=0.4)
     if kind in ("ks", "ks_mono", "ffde"):
         cfg.update(a=g.a, Rout=g.Rout, hslope=g.hslope, R0=g.R0, geom="ref", gam=4.0 / 3.0, cour=0.9,
                    tables_out=os.path.join(work, "tables.bin"))
@@ -146,13 +148,57 @@ def make_rdump(name, work):
     print(name, "text", len(text), "bytes, binary", len(binary), "bytes")
 
 
+# conservation accounting of the reference's steps (diag_flux, diag_source,
+# diag_fixup accumulators of every energy region, written after the steps)
+# and its dump-time reductions divbmaxavg() and constotal() of the final state
+DIAG_CASES = {
+    "diag_torus_para_hll_rk4": ("ks", 32, 64, H.PARA, H.HLLFLUX, 4, 20, 1e-3),
+    "diag_ot_mc_hll_rk2": ("mink", 32, 32, H.MC, H.HLLFLUX, 2, 10, 1e-4),
+}
+NREG, NFL, NSRC, NFLOOR = 3, 7, 7, 9
+DIAG_LAYOUT = [("enerpos", (NREG, 4)), ("doflux", (NREG, 4)), ("pdot", (NREG, 4, 8)),
+               ("pdotterms", (NREG, 4, NFL, 8)), ("pcum", (NREG, 4, 8)), ("fladd", (NREG, 8)),
+               ("fladdterms", (NREG, NFLOOR, 8)), ("sourceadd", (NREG, 8)), ("sourceaddterms", (NREG, NSRC, 8)),
+               ("pdottermsjet2", (4, NFL, 8)), ("frdot", ("n1", 8)), ("lumvsr", ("n1",)), ("divbmax", ()),
+               ("divbavg", ()), ("constotal", (NREG, 8)), ("dVF", ())]
+
+
+def read_diag(path, n1):
+    d = np.fromfile(path, dtype=np.float64)
+    out, o = {}, 0
+    for name, shape in DIAG_LAYOUT:
+        shape = tuple(n1 if s == "n1" else s for s in shape)
+        n = int(np.prod(shape)) if shape else 1
+        out[name] = d[o:o + n].reshape(shape) if shape else d[o]
+        o += n
+    assert o == d.size, (o, d.size)
+    out["enerpos"] = out["enerpos"].astype(np.int32)
+    out["doflux"] = out["doflux"].astype(np.int32)
+    return out
+
+
+def make_diag(name, work):
+    kind, n1, n2, lim, flux, to, nsteps, dt0 = DIAG_CASES[name]
+    g, geom, p0, extra = problem(kind, n1, n2)
+    recs, states = run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, diag=True)
+    dg = read_diag(os.path.join(work, "diag.bin"), n1)
+    np.savez_compressed(os.path.join(OUT_DIR, f"{name}.npz"), kind=kind, n1=n1, n2=n2, lim=lim, fluxmethod=flux,
+                        timeorder=to, nsteps=nsteps, dt0=dt0, input_sha=sha(p0), t=recs[:, 0], dt=recs[:, 1],
+                        fail=recs[:, 3], p=states[-1], h_over_r_jet=0.4, **dg)
+    print(name, "fladd", dg["fladd"][0], "divb", dg["divbmax"], dg["divbavg"])
+
+
 def main():
     subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle", "ref"), "-s"])
     with tempfile.TemporaryDirectory() as work:
         rd = [a for a in sys.argv[1:] if a in RDUMP_CASES] or ([] if sys.argv[1:] else list(RDUMP_CASES))
         for name in rd:
             make_rdump(name, work)
-        names = [a for a in sys.argv[1:] if a not in RDUMP_CASES] if sys.argv[1:] else list(CASES)
+        dc = [a for a in sys.argv[1:] if a in DIAG_CASES] or ([] if sys.argv[1:] else list(DIAG_CASES))
+        for name in dc:
+            make_diag(name, work)
+        names = [a for a in sys.argv[1:] if a not in RDUMP_CASES and a not in DIAG_CASES] if sys.argv[1:] \
+            else list(CASES)
         for name in names:
             kind, n1, n2, lim, flux, to, nsteps, dt0, saved = CASES[name]
             g, geom, p0, extra = problem(kind, n1, n2)
