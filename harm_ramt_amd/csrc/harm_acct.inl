@@ -22,6 +22,9 @@ constexpr int OFF_PDOT = (int)(offsetof(harm_ener_region, pdot) / sizeof(double)
 constexpr int OFF_TERMS = (int)(offsetof(harm_ener_region, pdotterms) / sizeof(double));
 constexpr int OFF_PCUM = (int)(offsetof(harm_ener_region, pcum) / sizeof(double));
 constexpr int OFF_SRC = (int)(offsetof(harm_ener_region, sourceadd) / sizeof(double));
+constexpr int OFF_FLADD = (int)(offsetof(harm_ener_region, fladd) / sizeof(double));
+constexpr int OFF_FLT = (int)(offsetof(harm_ener_region, fladdterms) / sizeof(double));
+constexpr int ACC_NFLOOR = HARM_NUMFAILFLOORFLAGS;
 constexpr int OFF_SRCT = (int)(offsetof(harm_ener_region, sourceaddterms) / sizeof(double));
 constexpr int OFF_JET2 = ACC_NREG * REG_D;             // pdottermsjet2[4][7][8] after the regions
 constexpr int ACC_D = OFF_JET2 + 4 * ACC_NFL * 8;
@@ -44,6 +47,15 @@ struct AcctState {
   double* out;      // 4 doubles: dump-time results
   int* faces;       // (i, j) of each boundary face, boundaries in (region, dir) order
   size_t scratch_n;
+  // fixup events of a substage (slots of acct_base) and their compaction
+  double* pre;      // zone states before fixup1zone [nz][8]
+  int* flag;        // [nslots], category + 1 of the event in the slot
+  double* val;      // [nslots][8]
+  int* list;        // [nslots][2]: slot, category of the events in order
+  int* cnt;         // [nblocks] events per block of 256 slots, then their offsets
+  int* off;
+  int* total;
+  int nslots, nblocks;
 };
 
 __device__ __forceinline__ bool acct_in(const int* ep, int i, int j) {
@@ -58,20 +70,6 @@ __device__ __forceinline__ void evolve_to_diag(const KParams& P, double gdet, co
   if (P.eomtype == 0) u[UU] += -u[RHO];
 #pragma unroll
   for (int k = 0; k < NPR; k++) Uout[k] = u[k] * gdet;
-}
-
-// primtoU with returntype UDIAG (phys.c:16-56, UtoU UNOTHING -> UDIAG):
-// rest mass kept in UU, times gdet
-__device__ __forceinline__ void primtoU_diag(const KParams& P, const double* pr, const State& q, double gdet,
-                                             double* U) {
-  double fin[NPR];
-  fin[RHO] = pr[RHO] * q.ucon[0];
-  mhd_calc(P, pr, 0, q, &fin[UU]);
-  fin[B1] = pr[B1];
-  fin[B2] = pr[B2];
-  fin[B3] = pr[B3];
-#pragma unroll
-  for (int k = 0; k < NPR; k++) U[k] = fin[k] * gdet;
 }
 
 // a step's fluxes as the reference's F1/F2 hold them after flux_ct()
@@ -381,11 +379,185 @@ __global__ void k_cons_sum(KParams P, AcctRegions R, int region, const double* _
   res[k] = s;
 }
 
+// limit_gamma on the face states of fluxcalc (step_ch.c:978-979): the event
+// of each state it changes (faces and their order as the flux kernels)
+__global__ void k_acct_faceg(KParams P, const double* __restrict__ pr, const double* __restrict__ gF) {
+  const int dir = 1 + blockIdx.z;
+  const int jf = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ifc = blockIdx.y * blockDim.y + threadIdx.y;
+  const int i = (dir == 1) ? ifc : ifc - 1, j = (dir == 1) ? jf - 1 : jf;
+  if (i > P.n1 || j > P.n2) return;
+  double p[2][NPR];
+  if (dir == 1) face_recon<1>(P, pr, i, j, p[0], p[1]);
+  else face_recon<2>(P, pr, i, j, p[0], p[1]);
+  const int kb = B1 + dir - 1;
+  const double bavg = 0.5 * (p[0][kb] + p[1][kb]);
+  p[0][kb] = bavg;
+  p[1][kb] = bavg;
+  const size_t nz = (size_t)(P.n1 + 2 * NG) * P.N2M;
+  GeoU g;
+  load_geo_face(gF + (dir == 2 ? nz * GF : 0), zidx(P, i, j), g);
+  const int slot = (dir == 1) ? acct_base(P.n1, P.n2, 0) + (i * (P.n2 + 2) + (j + 1)) * 2
+                              : acct_base(P.n1, P.n2, 1) + ((i + 1) * (P.n2 + 1) + j) * 2;
+  for (int side = 0; side < 2; side++) {
+    double p0[NPR];
+#pragma unroll
+    for (int k = 0; k < NPR; k++) p0[k] = p[side][k];
+    if (limit_gamma(P.gammamax, p[side], g) == -1) acct_event(P, g, p0, p[side], slot + side, 2);
+  }
+}
+
+// fixup1zone's events (fixup.c:274-377) from the state before it: limit_gamma
+// (COUNTLIMITGAMMAACT), then the whole change (COUNTFLOORACT, final substage)
+__global__ void k_acct_zoneg(KParams P, const double* __restrict__ gC) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= P.n1 || j >= P.n2) return;
+  const int z = zidx(P, i, j);
+  double gc[36];
+  load_cent(gC, z, gc);
+  GeoU g;
+  geo_from_cent(gc, g);
+  double p0[NPR], p1[NPR], p2[NPR];
+  load8(P.acct_pre, z, p0);
+#pragma unroll
+  for (int k = 0; k < NPR; k++) p1[k] = p0[k];
+  int changed = 0;
+  if (P.eomtype == 0) {
+    double fr = gc[33], fu = gc[34];
+    if (fr < P.rhominlimit) fr = P.rhominlimit;
+    if (fu < P.uuminlimit) fu = P.uuminlimit;
+    if (fr > p1[RHO]) { p1[RHO] = fr; changed = 1; }
+    if (fu > p1[UU]) { p1[UU] = fu; changed = 1; }
+  }
+#pragma unroll
+  for (int k = 0; k < NPR; k++) p2[k] = p1[k];
+  const int slot = acct_base(P.n1, P.n2, 2) + (i * P.n2 + j) * 2;
+  if (limit_gamma(P.gammamax, p2, g) == -1) {
+    acct_event(P, g, p1, p2, slot, 2);
+    changed = 1;
+  }
+  if (changed) acct_event(P, g, p0, p2, slot + 1, 1);
+}
+
+// the failed zones' events from the states fixup_zone kept: limit_gamma's
+// (slot flag -1: it changed the averaged state) and the repair's
+__global__ void k_acct_fixg(KParams P, const double* __restrict__ gC) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y;
+  if (i >= P.n1 || j >= P.n2) return;
+  const int slot = acct_base(P.n1, P.n2, 3) + (i * P.n2 + j) * 2;
+  const int f = P.acct_flag[slot];
+  if (f >= 0) return;
+  const int z = zidx(P, i, j);
+  double gc[36];
+  load_cent(gC, z, gc);
+  GeoU g;
+  geo_from_cent(gc, g);
+  double pavg[NPR], pfin[NPR], pr0[NPR];
+  load8(P.acct_val, slot, pavg);
+  load8(P.acct_val, slot + 1, pfin);
+  load8(P.acct_pre, z, pr0);
+  if (f == -1) acct_event(P, g, pavg, pfin, slot, 2);
+  else P.acct_flag[slot] = 0;
+  acct_event(P, g, pr0, pfin, slot + 1, P.acct_flag[slot + 1] - 1);
+}
+
+// ordered compaction of the used slots: counts per block of 256, offsets,
+// then each block writes its events at its offset in slot order
+__global__ __launch_bounds__(256) void k_acct_count(int nslots, const int* __restrict__ flag, int* __restrict__ cnt) {
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  const int c = __syncthreads_count((s < nslots) && flag[s] != 0);
+  if (threadIdx.x == 0) cnt[blockIdx.x] = c;
+}
+
+__global__ void k_acct_scan(int nblocks, const int* __restrict__ cnt, int* __restrict__ off, int* __restrict__ total) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int run = 0;
+  for (int b = 0; b < nblocks; b++) {
+    off[b] = run;
+    run += cnt[b];
+  }
+  *total = run;
+}
+
+__global__ __launch_bounds__(256) void k_acct_compact(int nslots, int* __restrict__ flag, const int* __restrict__ off,
+                                                      int* __restrict__ list) {
+  __shared__ int wcount[4];
+  const int s = blockIdx.x * 256 + threadIdx.x;
+  const int f = (s < nslots) ? flag[s] : 0;
+  const unsigned long long m = __ballot(f != 0);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) wcount[w] = __popcll(m);
+  __syncthreads();
+  if (f != 0) {
+    int pos = off[blockIdx.x] + __popcll(m & ((1ull << lane) - 1ull));
+    for (int v = 0; v < w; v++) pos += wcount[v];
+    list[2 * pos] = s;
+    list[2 * pos + 1] = f - 1;
+    flag[s] = 0;
+  }
+}
+
+// the substage's events added in order: lanes 0-23 fladd[r][k], then
+// fladdterms[r][cat][k]; floor and repair events count on the last substage
+__global__ void k_acct_merge(KParams P, AcctRegions R, const int* __restrict__ list, const int* __restrict__ total,
+                             const double* __restrict__ val, double* __restrict__ acc, int finalstep) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  constexpr int NT = ACC_NREG * 8, NTT = ACC_NREG * ACC_NFLOOR * 8;
+  if (t >= NT + NTT) return;
+  int r, cat = -1, k;
+  if (t < NT) {
+    r = t / 8;
+    k = t % 8;
+  } else {
+    const int u = t - NT;
+    r = u / (ACC_NFLOOR * 8);
+    cat = (u / 8) % ACC_NFLOOR;
+    k = u % 8;
+  }
+  const int n1 = P.n1, n2 = P.n2;
+  const int bB = acct_base(n1, n2, 1), bC = acct_base(n1, n2, 2), bD = acct_base(n1, n2, 3);
+  double* dst = acc + r * REG_D + ((cat < 0) ? OFF_FLADD + k : OFF_FLT + cat * 8 + k);
+  double s = *dst;
+  const int E = *total;
+  const int* ep = R.ep[r];
+  for (int e = 0; e < E; e++) {
+    const int slot = list[2 * e], c = list[2 * e + 1];
+    if (cat >= 0 && c != cat) continue;
+    int i, j;
+    if (slot < bB) {
+      const int f = slot >> 1;
+      i = f / (n2 + 2);
+      j = f % (n2 + 2) - 1;
+    } else if (slot < bC) {
+      const int f = (slot - bB) >> 1;
+      i = f / (n2 + 1) - 1;
+      j = f % (n2 + 1);
+    } else {
+      const int rel = slot - ((slot < bD) ? bC : bD);
+      if ((rel & 1) && !finalstep) continue;  // COUNTFLOORACT / the repair: accounting substage only
+      const int f = rel >> 1;
+      i = f / n2;
+      j = f % n2;
+    }
+    if (!acct_in(ep, i, j)) continue;
+    s += val[(size_t)slot * 8 + k];
+  }
+  *dst = s;
+}
+
 }  // namespace
 
 static bool acct_on(const harm_ctx* c) { return c->acct && c->acct->on; }
+// accounting on/off for the steps; the kernels see its buffers only while on
 static void acct_set_on(harm_ctx* c, bool on) {
-  if (c->acct) c->acct->on = on;
+  AcctState* a = c->acct;
+  if (a) a->on = on;
+  const bool k = a && on;
+  c->kp.acct_pre = k ? a->pre : nullptr;
+  c->kp.acct_flag = k ? a->flag : nullptr;
+  c->kp.acct_val = k ? a->val : nullptr;
 }
 
 static int acct_alloc(harm_ctx* c) {
@@ -401,6 +573,16 @@ static int acct_alloc(harm_ctx* c) {
   if (e == hipSuccess) e = hipMalloc(&a->scratch, a->scratch_n * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&a->out, 8 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&a->faces, (size_t)ACC_NREG * 2 * (n1 + n2 + 2) * 2 * sizeof(int));
+  a->nslots = acct_base(n1, n2, 4);
+  a->nblocks = (a->nslots + 255) / 256;
+  if (e == hipSuccess) e = hipMalloc(&a->pre, (size_t)c->nz * 8 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&a->flag, (size_t)a->nslots * sizeof(int));
+  if (e == hipSuccess) e = hipMemset(a->flag, 0, (size_t)a->nslots * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&a->val, (size_t)a->nslots * 8 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&a->list, (size_t)a->nslots * 2 * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&a->cnt, (size_t)a->nblocks * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&a->off, (size_t)a->nblocks * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&a->total, sizeof(int));
   c->acct = a;
   if (e != hipSuccess) return seterr("accounting buffers: %s", hipGetErrorString(e));
   return 0;
@@ -409,7 +591,8 @@ static int acct_alloc(harm_ctx* c) {
 static void acct_free(harm_ctx* c) {
   AcctState* a = c->acct;
   if (!a) return;
-  void* bufs[6] = {a->acc, a->frdot, a->lumvsr, a->scratch, a->out, a->faces};
+  void* bufs[13] = {a->acc, a->frdot, a->lumvsr, a->scratch, a->out,  a->faces, a->pre,
+                     a->flag, a->val,   a->list,   a->cnt,    a->off, a->total};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete a;
@@ -426,6 +609,40 @@ static int acct_source(harm_ctx* c, int pi, int pb, int pf) {
                      c->gConn, c->ds, a->scratch);
   hipLaunchKernelGGL(k_acct_srcsum, dim3((64 + c->kp.n1 + 63) / 64), dim3(64), 0, c->stream, c->kp, a->R, a->scratch,
                      a->acc, a->lumvsr);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// the fixup events of one substage: faces (from pb, before the substage) ...
+static int acct_faces(harm_ctx* c, int pb) {
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL(k_acct_faceg, dim3((c->kp.n2 + 2 + 63) / 64, (c->kp.n1 + 2 + 3) / 4, 2), b, 0, c->stream,
+                     c->kp, c->arr[pb], c->gF);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// ... zones (after the update) ...
+static int acct_zones(harm_ctx* c) {
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL(k_acct_zoneg, grid2(c->kp.n2, c->kp.n1, b), b, 0, c->stream, c->kp, c->gC);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// ... and, after the failed zones' repair, all of them added in order
+static int acct_merge(harm_ctx* c, int finalstep) {
+  AcctState* a = c->acct;
+  if (c->prm.eomtype == HARM_EOMGRMHD) {  // fixup_utoprim runs for GRMHD only
+    const dim3 b(64, 4);
+    hipLaunchKernelGGL(k_acct_fixg, grid2(c->kp.n2, c->kp.n1, b), b, 0, c->stream, c->kp, c->gC);
+  }
+  hipLaunchKernelGGL(k_acct_count, dim3(a->nblocks), dim3(256), 0, c->stream, a->nslots, a->flag, a->cnt);
+  hipLaunchKernelGGL(k_acct_scan, dim3(1), dim3(64), 0, c->stream, a->nblocks, a->cnt, a->off, a->total);
+  hipLaunchKernelGGL(k_acct_compact, dim3(a->nblocks), dim3(256), 0, c->stream, a->nslots, a->flag, a->off, a->list);
+  const int nl = ACC_NREG * 8 * (1 + ACC_NFLOOR);
+  hipLaunchKernelGGL(k_acct_merge, dim3((nl + 63) / 64), dim3(64), 0, c->stream, c->kp, a->R, a->list, a->total,
+                     a->val, a->acc, finalstep);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -500,14 +717,15 @@ extern "C" int harm_acct_setup(harm_ctx* c, const int* enerpos, const int* doflu
   HIPCHK(hipMemsetAsync(a->frdot, 0, (size_t)n1 * 8 * sizeof(double), c->stream));
   HIPCHK(hipMemsetAsync(a->lumvsr, 0, (size_t)n1 * sizeof(double), c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  a->on = true;
+  c->kp.dVF = R.dVF;
+  acct_set_on(c, true);
   invalidate_graph(c);
   return 0;
 }
 
 extern "C" int harm_acct_off(harm_ctx* c) {
   if (!c) return seterr("null ctx");
-  if (c->acct) c->acct->on = false;
+  acct_set_on(c, false);
   invalidate_graph(c);
   return 0;
 }

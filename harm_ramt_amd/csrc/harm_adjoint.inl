@@ -416,6 +416,7 @@ extern "C" int harm_adj_create(harm_ctx* c, int nsteps, int checkpoint_every, ha
   *out = nullptr;
   if (need_geom(c)) return 1;
   if (nsteps < 1 || checkpoint_every < 1) return seterr("adjoint: nsteps and checkpoint_every must be >= 1");
+  if (acct_on(c)) return seterr("adjoint: switch conservation accounting off first (harm_acct_off)");
   if (adj_check(c)) return 1;
   harm_adj* a = new harm_adj();
   memset(a, 0, sizeof(*a));
@@ -541,6 +542,7 @@ static int adj_graph(harm_adj* a, int k) {
 extern "C" int harm_adj_forward(harm_adj* a, const double* dts) {
   if (!a) return seterr("null adjoint");
   harm_ctx* c = a->c;
+  if (acct_on(c)) return seterr("adjoint: switch conservation accounting off first (harm_acct_off)");
   if (autotune(c)) return 1;
   const bool was_prof = c->prof;
   c->prof = false;
@@ -627,6 +629,7 @@ extern "C" int harm_adj_backward(harm_adj* a) {
   if (!a) return seterr("null adjoint");
   if (!a->seeded) return seterr("adjoint: seed the final-state adjoint first");
   harm_ctx* c = a->c;
+  if (acct_on(c)) return seterr("adjoint: switch conservation accounting off first (harm_acct_off)");
   const bool was_prof = c->prof;
   c->prof = false;
   const size_t sb = (size_t)c->nz * 8 * sizeof(double);

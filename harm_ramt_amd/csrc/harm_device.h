@@ -33,6 +33,12 @@ struct KParams {
   int fixupflux, poledeath, checksolution, eno;
   double gam, cour, defcon, safe, gammamax, gammafail, rhominlimit, uuminlimit;
   double dx1, dx2;
+  // floor/failure accounting (diag_fixup), set only while it is on: dVF,
+  // the zone states before fixup1zone, and the event slots of a substage
+  double dVF;
+  double* acct_pre;
+  int* acct_flag;
+  double* acct_val;
 };
 
 struct GeoU {

@@ -298,16 +298,12 @@ __device__ __forceinline__ void face_finish(const KParams& P, double* p_l, doubl
                                             double* __restrict__ q, const double* __restrict__ gF, int i, int j,
                                             double& dtij, int& fail);
 
-// fluxcalc() with one thread per face (the whole face on one lane); same
-// arithmetic as flux_face<DIR>, different work split.
+// the left and right states of face (i,j) of direction DIR (step_ch.c:812-903)
 template <int DIR>
-__device__ __forceinline__ void flux_face_single(const KParams& P, const double* __restrict__ pr,
-                                                 double* __restrict__ F, double* __restrict__ q,
-                                                 const double* __restrict__ gF, int i, int j, double& dtij,
-                                                 int& fail) {
+__device__ __forceinline__ void face_recon(const KParams& P, const double* __restrict__ pr, int i, int j,
+                                           double* p_l, double* p_r) {
   const int zstride = (DIR == 1) ? P.N2M : 1;
   const int z0 = zidx(P, i, j);
-  double p_l[NPR], p_r[NPR];
   if (P.lim == 4) {
     double s[6][NPR];
 #pragma unroll
@@ -338,6 +334,17 @@ __device__ __forceinline__ void flux_face_single(const KParams& P, const double*
       p_r[k] = s[3][k] - 0.5 * dqr;
     }
   }
+}
+
+// fluxcalc() with one thread per face (the whole face on one lane); same
+// arithmetic as flux_face<DIR>, different work split.
+template <int DIR>
+__device__ __forceinline__ void flux_face_single(const KParams& P, const double* __restrict__ pr,
+                                                 double* __restrict__ F, double* __restrict__ q,
+                                                 const double* __restrict__ gF, int i, int j, double& dtij,
+                                                 int& fail) {
+  double p_l[NPR], p_r[NPR];
+  face_recon<DIR>(P, pr, i, j, p_l, p_r);
   face_finish<DIR>(P, p_l, p_r, F, q, gF, i, j, dtij, fail);
 }
 
@@ -840,9 +847,53 @@ __device__ __forceinline__ void eno_cvt_zone(const KParams& P, const double* __r
   for (int k = 0; k < NPR; k++) out[k] = (k < nk) ? eno_cvt_line<TYPE>(v[k], pos, n) : v[k][4];
 }
 
+// ---- floor/failure accounting (diag_fixup, fixup.c:174-229) ----
+// Every change a fixup makes to a zone or face state is an event: dVF (U(after)
+// - U(before)) in UDIAG form, booked in the energy regions holding that zone
+// or face.  The events of a substage have fixed slots in the order the
+// reference makes them: x1 faces (FZLOOP, left state then right), x2 faces,
+// zones (limit_gamma, then floors), failed zones (limit_gamma, then the repair).
+// A flag (category + 1) marks a slot in use; the slots are compacted in order
+// and summed after the substage.
+
+// primtoU with returntype UDIAG (phys.c:16-56, UtoU UNOTHING -> UDIAG):
+// rest mass kept in UU, times gdet
+__device__ __forceinline__ void primtoU_diag(const KParams& P, const double* pr, const State& q, double gdet,
+                                             double* U) {
+  double fin[NPR];
+  fin[RHO] = pr[RHO] * q.ucon[0];
+  mhd_calc(P, pr, 0, q, &fin[UU]);
+  fin[B1] = pr[B1];
+  fin[B2] = pr[B2];
+  fin[B3] = pr[B3];
+#pragma unroll
+  for (int k = 0; k < NPR; k++) U[k] = fin[k] * gdet;
+}
+
+// one event into slot `slot`: fixup.c:199-217
+__device__ void acct_event(const KParams& P, const GeoU& g, const double* pr0, const double* pr, int slot, int cat) {
+  State q;
+  double Ui[NPR], Uf[NPR];
+  get_state(pr0, g, q);
+  primtoU_diag(P, pr0, q, g.gdet, Ui);
+  get_state(pr, g, q);
+  primtoU_diag(P, pr, q, g.gdet, Uf);
+  double* o = P.acct_val + (size_t)slot * 8;
+#pragma unroll
+  for (int k = 0; k < NPR; k++) o[k] = P.dVF * (Uf[k] - Ui[k]);
+  P.acct_flag[slot] = cat + 1;
+}
+
+// slot bases of the four event classes of a substage
+__host__ __device__ __forceinline__ int acct_base(int n1, int n2, int cls) {
+  const int nA = (n1 + 1) * (n2 + 2) * 2, nB = (n1 + 2) * (n2 + 1) * 2, nC = n1 * n2 * 2;
+  return cls == 0 ? 0 : cls == 1 ? nA : cls == 2 ? nA + nB : cls == 3 ? nA + nB + nC : nA + nB + 2 * nC;
+}
+
 // Utoprimgen of U into pf[z] from the guess pin (the pf=pi assignment,
 // advance.c:121), fixup1zone (fixup.c:274-377), the zone's pflag and the
-// failed-zone list read by fixup_utoprim
+// failed-zone list read by fixup_utoprim; with accounting on, the state
+// before fixup1zone is kept for its events
 __device__ __forceinline__ void invert_zone(const KParams& P, const double* U, const double* pin, const double* gc,
                                             const GeoU& g, int z, double* __restrict__ pf,
                                             int* __restrict__ pflag, int* __restrict__ faillist,
@@ -861,6 +912,7 @@ __device__ __forceinline__ void invert_zone(const KParams& P, const double* U, c
     int slot = atomicAdd(&ds->nlist, 1);
     if (slot < P.n1 * P.n2) faillist[slot] = z;
   }
+  if (P.acct_pre) store8(P.acct_pre, z, pout);
   // fixup1zone: floors then limit_gamma
   if (P.eomtype == 0) {
     double fr = gc[33], fu = gc[34];
@@ -1245,7 +1297,25 @@ __device__ void fixup_zone(const KParams& P, const double* __restrict__ pv, cons
     }
     pz[k] = v;
   }
-  if (limit_gamma(gamma, pz, g) >= 1) fail = 1;
+  const int slot = acct_base(P.n1, P.n2, 3) + (i * P.n2 + j) * 2;
+  if (P.acct_flag) {  // the states of fixup.c:850's limit_gamma event: before it ...
+    double* o = P.acct_val + (size_t)slot * 8;
+#pragma unroll
+    for (int k = 0; k < NPR; k++) o[k] = (k <= U3) ? pz[k] : pv[(size_t)z * 8 + k];
+  }
+  const int lg = limit_gamma(gamma, pz, g);
+  if (lg >= 1) fail = 1;
+  if (P.acct_flag) {  // ... after it, and the repair's (fixup.c:870-915): the state before and its failure type
+    double* o = P.acct_val + (size_t)(slot + 1) * 8;
+#pragma unroll
+    for (int k = 0; k < NPR; k++) o[k] = (k <= U3) ? pz[k] : pv[(size_t)z * 8 + k];
+#pragma unroll
+    for (int k = 0; k < NPR; k++) P.acct_pre[(size_t)z * 8 + k] = pv[(size_t)z * 8 + k];
+    const int f = pflag[z];
+    const int cat = (f == 5) ? 4 : (f == 6) ? 5 : (f == 7) ? 6 : (f == 8) ? 7 : (f == 9) ? 8 : 0;
+    P.acct_flag[slot] = (lg == -1) ? -1 : -2;  // states stored; k_acct_fixg makes the events
+    P.acct_flag[slot + 1] = cat + 1;
+  }
   double* o = pfix + (size_t)z * 8;
 #pragma unroll
   for (int k = 0; k <= U3; k++) o[k] = pz[k];
@@ -1835,6 +1905,9 @@ static bool acct_on(const harm_ctx* c);
 static void acct_set_on(harm_ctx* c, bool on);
 static int acct_source(harm_ctx* c, int pi, int pb, int pf);
 static int acct_flux(harm_ctx* c, double Dt);
+static int acct_faces(harm_ctx* c, int pb);
+static int acct_zones(harm_ctx* c);
+static int acct_merge(harm_ctx* c, int finalstep);
 
 extern "C" const char* harm_last_error(void) { return g_err; }
 
@@ -2320,12 +2393,16 @@ static int enqueue_step(harm_ctx* c) {
     // pre_fixup (step_ch.c:92) flags every substage from pii = p, which no
     // substage changes before its own update: once per step is the same
     if (to == 0 && c->kp.checksolution) launch_bsqflags(c, pii[to]);
+    const bool acct = acct_on(c);
     // advance.c:138-140: diag_source on the last substage
-    if (to == nto - 1 && acct_on(c) && acct_source(c, pii[to], pbb[to], pff[to])) return 1;
+    if (to == nto - 1 && acct && acct_source(c, pii[to], pbb[to], pff[to])) return 1;
+    if (acct && acct_faces(c, pbb[to])) return 1;
     if (substage(c, pii[to], pbb[to], pff[to], CUf[to], Cunew[to], to, nto)) return 1;
+    if (acct && acct_zones(c)) return 1;
     // step_ch.c:104-124 bound_prim, post_fixup, bound_prim; then :174-183 after the last substage
     int mode = KB_BOUND | KB_FIXUP | KB_BOUND2 | ((to == nto - 1) ? KB_DT : 0);
     if (launch_boundary(c, pff[to], mode)) return 1;
+    if (acct && acct_merge(c, to == nto - 1)) return 1;
   }
   // step_ch.c:160 diag_flux with the step's dt
   if (acct_on(c) && acct_flux(c, -1.0)) return 1;
@@ -2761,6 +2838,9 @@ extern "C" int harm_batch_create(harm_ctx* const* ctxs, int n, harm_batch** out)
     harm_ctx* c = ctxs[r];
     RunDesc& d = hr[r];
     d.P = c->kp;
+    d.P.acct_pre = nullptr;  // a batch does no accounting
+    d.P.acct_flag = nullptr;
+    d.P.acct_val = nullptr;
     for (int a = 0; a < HARM_NARRAYS; a++) d.arr[a] = c->arr[a];
     d.q1 = c->q1; d.q2 = c->q2; d.pfix = c->pfix;
     d.gF = c->gF; d.gC = c->gC; d.gConn = c->gConn;

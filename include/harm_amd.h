@@ -371,10 +371,10 @@ int harm_rdump_read(harm_ctx *ctx, const char *path, int binary, harm_restart_he
    is not summed), as setflux()/setjetflux() set them (initbase.c:736-980).
    With accounting on, every step does what the reference's step does:
    diag_source() on the last substage (advance.c:138-140) and diag_flux()
-   after the step (step_ch.c:160), each sum formed in the reference's order
-   so that the totals are the reference's to the bit.  The floor/failure
-   accumulators fladd/fladdterms (diag_fixup, fixup.c:174-229) are not
-   formed on the device: they read as zero. */
+   after the step (step_ch.c:160), and diag_fixup() (fixup.c:174-229) for
+   every change limit_gamma, the floors and the failed-zone repair make;
+   each sum formed in the reference's order so that the totals are the
+   reference's to the bit.  The adjoint refuses a context with accounting on. */
 #define HARM_NUMFLUXTERMS 7
 #define HARM_NUMSOURCES 7
 typedef struct {                    /* defs.h:218-239, one region */

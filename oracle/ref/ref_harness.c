@@ -109,6 +109,10 @@ int main(int argc, char *argv[])
   BCtype[X2DN] = cfgi("bc3", POLARAXIS);
   cooling = 0;
   DODIAGS = 0;
+  if (cfg("gammamax", NULL)) {  /* initbase.c:218-219 sets both from GAMMAMAX */
+    GAMMAMAX = cfgd("gammamax", 25.0);
+    GAMMAFAIL = 100.0 * GAMMAMAX;
+  }
 
   geomsrc = cfg("geom", "ref");
   if (!strcmp(geomsrc, "ref")) {

@@ -93,10 +93,13 @@ def bc_of(kind):
     return (H.OUTFLOW, H.OUTFLOW, H.POLARAXIS, H.POLARAXIS)
 
 
-def run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, eno=0, rdump=False, diag=False):
+def run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, eno=0, rdump=False, diag=False,
+                  gammamax=None):
     p0.tofile(os.path.join(work, "p0.bin"))
     cfg = dict(prim=os.path.join(work, "p0.bin"), dt=dt0, nsteps=nsteps, out=os.path.join(work, "o.bin"),
                lim=lim, fluxmethod=flux, timeorder=to)
+    if gammamax is not None:
+        cfg["gammamax"] = gammamax
     for n, b in enumerate(bc_of(kind)):
         cfg[f"bc{n}"] = b
     if rdump:
@@ -155,6 +158,9 @@ DIAG_CASES = {
     "diag_torus_para_hll_rk4": ("ks", 32, 64, H.PARA, H.HLLFLUX, 4, 20, 1e-3),
     "diag_ot_mc_hll_rk2": ("mink", 32, 32, H.MC, H.HLLFLUX, 2, 10, 1e-4),
 }
+# GAMMAMAX lowered (initbase.c:218) so that limit_gamma acts on face states
+# and zones: its accounting (COUNTLIMITGAMMAACT) is exercised
+DIAG_GAMMAMAX = {"diag_ot_gammamax_mc_hll_rk2": ("mink", 32, 32, H.MC, H.HLLFLUX, 2, 10, 1e-4, 1.1)}
 NREG, NFL, NSRC, NFLOOR = 3, 7, 7, 9
 DIAG_LAYOUT = [("enerpos", (NREG, 4)), ("doflux", (NREG, 4)), ("pdot", (NREG, 4, 8)),
                ("pdotterms", (NREG, 4, NFL, 8)), ("pcum", (NREG, 4, 8)), ("fladd", (NREG, 8)),
@@ -178,13 +184,17 @@ def read_diag(path, n1):
 
 
 def make_diag(name, work):
-    kind, n1, n2, lim, flux, to, nsteps, dt0 = DIAG_CASES[name]
+    if name in DIAG_GAMMAMAX:
+        kind, n1, n2, lim, flux, to, nsteps, dt0, gmax = DIAG_GAMMAMAX[name]
+    else:
+        (kind, n1, n2, lim, flux, to, nsteps, dt0), gmax = DIAG_CASES[name], None
     g, geom, p0, extra = problem(kind, n1, n2)
-    recs, states = run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, diag=True)
+    recs, states = run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, diag=True, gammamax=gmax)
     dg = read_diag(os.path.join(work, "diag.bin"), n1)
+    more = {} if gmax is None else {"gammamax": gmax}
     np.savez_compressed(os.path.join(OUT_DIR, f"{name}.npz"), kind=kind, n1=n1, n2=n2, lim=lim, fluxmethod=flux,
                         timeorder=to, nsteps=nsteps, dt0=dt0, input_sha=sha(p0), t=recs[:, 0], dt=recs[:, 1],
-                        fail=recs[:, 3], p=states[-1], h_over_r_jet=0.4, **dg)
+                        fail=recs[:, 3], p=states[-1], h_over_r_jet=0.4, **more, **dg)
     print(name, "fladd", dg["fladd"][0], "divb", dg["divbmax"], dg["divbavg"])
 
 
@@ -194,10 +204,11 @@ def main():
         rd = [a for a in sys.argv[1:] if a in RDUMP_CASES] or ([] if sys.argv[1:] else list(RDUMP_CASES))
         for name in rd:
             make_rdump(name, work)
-        dc = [a for a in sys.argv[1:] if a in DIAG_CASES] or ([] if sys.argv[1:] else list(DIAG_CASES))
+        alld = {**DIAG_CASES, **DIAG_GAMMAMAX}
+        dc = [a for a in sys.argv[1:] if a in alld] or ([] if sys.argv[1:] else list(alld))
         for name in dc:
             make_diag(name, work)
-        names = [a for a in sys.argv[1:] if a not in RDUMP_CASES and a not in DIAG_CASES] if sys.argv[1:] \
+        names = [a for a in sys.argv[1:] if a not in RDUMP_CASES and a not in alld] if sys.argv[1:] \
             else list(CASES)
         for name in names:
             kind, n1, n2, lim, flux, to, nsteps, dt0, saved = CASES[name]

@@ -54,8 +54,12 @@ def load_golden(name):
 def golden_problem(gd):
     """problem() with the settings a golden vector file records."""
     eno = int(gd["eno"]) if "eno" in gd.files else 0
-    return problem(str(gd["kind"]), int(gd["n1"]), int(gd["n2"]), lim=int(gd["lim"]),
-                   fluxmethod=int(gd["fluxmethod"]), timeorder=int(gd["timeorder"]), eno=eno)
+    g, geom, p0, prm = problem(str(gd["kind"]), int(gd["n1"]), int(gd["n2"]), lim=int(gd["lim"]),
+                               fluxmethod=int(gd["fluxmethod"]), timeorder=int(gd["timeorder"]), eno=eno)
+    if "gammamax" in gd.files:  # GAMMAMAX and GAMMAFAIL = 100 GAMMAMAX (initbase.c:218-219)
+        prm.gammamax = float(gd["gammamax"])
+        prm.gammafail = 100.0 * prm.gammamax
+    return g, geom, p0, prm
 
 
 def same(a, b):
