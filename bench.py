@@ -324,6 +324,7 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-acct", action="store_true")
     ap.add_argument("--tuning", default=None,
                     help="force the kernel variants as FLUX,BOUND (harm_set_tuning); default: autotuned")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -385,6 +386,23 @@ def main():
                 "kernel": kname, "avg_launch_us": round(stages[dom]["avg_us"], 3),
                 "algorithmic_bytes_per_launch": bytes_launch}
 
+    # the same run continued with the reference's conservation accounting on
+    # (diag_flux, diag_source, diag_fixup every step; not part of `value`)
+    accounting = None
+    if not args.no_acct:
+        ep, df = H.ener_regions(g, 0.4)
+        h.acct_setup(ep, df)
+        h.step_n(2)
+        h.get_time()
+        na = min(args.steps, 50)
+        t2 = time.perf_counter()
+        h.step_n(na)
+        h.get_time()
+        ms_acct = 1e3 * (time.perf_counter() - t2) / na
+        accounting = {"steps": na, "ms_per_step": ms_acct, "vs_plain_step": ms_acct / (1e3 * el / args.steps),
+                      "regions": 3}
+        h.acct_off()
+
     h.close()
     batched = None if args.no_batch else batch_sweep(args.batch_runs, args.batch_steps, 3)
     adjoint = None if args.no_adjoint else adjoint_measure()
@@ -408,6 +426,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adjoint,
+        "accounting": accounting,
         "stages": stages,
         "eager_timed_ms_per_step": 1e3 * el_prof / prof_steps,
     }
