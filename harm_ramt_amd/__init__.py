@@ -128,6 +128,36 @@ class EnerRegion(ctypes.Structure):
                 ("sourceaddterms", (ctypes.c_double * 8) * NUMSOURCES)]
 
 
+DUMP_COLUMNS = 56
+
+
+class DumpHeader(ctypes.Structure):
+    """harm_dump_header: dump_header()'s fields (dump.c:48-80)"""
+    _fields_ = [("t", ctypes.c_double), ("totalsize", ctypes.c_int * 2), ("startx", ctypes.c_double * 2),
+                ("dx", ctypes.c_double * 2), ("realnstep", ctypes.c_long), ("gam", ctypes.c_double),
+                ("a", ctypes.c_double), ("R0", ctypes.c_double), ("Rin", ctypes.c_double),
+                ("Rout", ctypes.c_double), ("hslope", ctypes.c_double), ("dt", ctypes.c_double),
+                ("defcoord", ctypes.c_int)]
+
+
+def dump_write(path, header, cols, binary=False):
+    """the reference's dump file from cols[n1][n2][56] (harm_dump_write_host)"""
+    lib = load_library()
+    cols = np.ascontiguousarray(cols, dtype=np.float64)
+    n1, n2 = cols.shape[:2]
+    assert cols.shape[2] == DUMP_COLUMNS
+    rc = lib.harm_dump_write_host(os.fsencode(path), int(binary), ctypes.byref(header), n1, n2, _dptr(cols))
+    if rc != 0:
+        raise HarmError(f"harm_dump_write_host {path}: error {rc}")
+
+
+def zone_coords(grid, i, j):
+    """(X1, X2, r, th) at the centre of zone (i,j)"""
+    v = [ctypes.c_double() for _ in range(4)]
+    load_library().harm_zone_coords(ctypes.byref(grid), i, j, *[ctypes.byref(x) for x in v])
+    return tuple(x.value for x in v)
+
+
 def ener_regions(grid, h_over_r_jet=0.4):
     """setflux()+setjetflux() (initbase.c:736-980) on a grid: (enerpos,
     doflux), int32 arrays [3][4] indexed [region][X1UP,X1DN,X2UP,X2DN]"""
@@ -158,6 +188,7 @@ EXPORTS = [
     "harm_rdump_write_host", "harm_rdump_read_host", "harm_rdump_write", "harm_rdump_read",
     "harm_ener_regions", "harm_acct_setup", "harm_acct_off", "harm_acct_read", "harm_diag_flux",
     "harm_divbmaxavg", "harm_constotal",
+    "harm_current_enable", "harm_get_current", "harm_dump_write_host", "harm_dump_write", "harm_zone_coords",
 ]
 
 _lib = None
@@ -238,6 +269,12 @@ def load_library(path=LIB_PATH):
         "harm_diag_flux": [vp, ctypes.c_double],
         "harm_divbmaxavg": [vp, ctypes.c_int, dp, dp, ip, ip],
         "harm_constotal": [vp, ctypes.c_int, ctypes.c_int, dp],
+        "harm_current_enable": [vp, ctypes.c_int],
+        "harm_get_current": [vp, dp, dp],
+        "harm_dump_write_host": [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(DumpHeader), ctypes.c_int,
+                                 ctypes.c_int, dp],
+        "harm_dump_write": [vp, ctypes.POINTER(HarmGrid), ctypes.c_char_p, ctypes.c_int, ctypes.c_long],
+        "harm_zone_coords": [ctypes.POINTER(HarmGrid), ctypes.c_int, ctypes.c_int, dp, dp, dp, dp],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -500,6 +537,22 @@ class Harm:
         _chk(self.lib.harm_divbmaxavg(self.h, sel, ctypes.byref(mx), ctypes.byref(av), ctypes.byref(im),
                                       ctypes.byref(jm)), "harm_divbmaxavg")
         return mx.value, av.value, im.value, jm.value
+
+    # the current J^mu, F^{mu nu} (phys.c:1124-1460) and the dump (dump.c)
+    def current_enable(self, on=True):
+        _chk(self.lib.harm_current_enable(self.h, int(on)), "harm_current_enable")
+
+    def current(self):
+        """(jcon[N1M][N2M][4], fcon[N1M][N2M][6])"""
+        n1m, n2m = self.prm.n1 + 6, self.prm.n2 + 6
+        j = np.zeros((n1m, n2m, 4))
+        f = np.zeros((n1m, n2m, 6))
+        _chk(self.lib.harm_get_current(self.h, _dptr(j), _dptr(f)), "harm_get_current")
+        return j, f
+
+    def dump_write(self, path, grid, binary=False, realnstep=0):
+        _chk(self.lib.harm_dump_write(self.h, ctypes.byref(grid), os.fsencode(path), int(binary), realnstep),
+             "harm_dump_write")
 
     def constotal(self, region, sel=P):
         v = np.zeros(8)

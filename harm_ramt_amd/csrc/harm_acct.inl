@@ -88,6 +88,33 @@ __device__ __forceinline__ void final_flux(const KParams& P, const double* __res
   }
 }
 
+// The serial phase of a sum: wave 0 adds the items of a chunk in order from
+// LDS while waves 1-3 stage the next chunk (double buffer), so that the
+// chain of dependent adds never waits on memory.  load(item, w) gives
+// component w of an item; consume(first, count, lds) runs on wave 0.
+constexpr int STG_CH = 256;
+template <int W, class Load, class Consume>
+__device__ __forceinline__ void staged_items(int nitems, double* lds, Load load, Consume consume) {
+  const int tid = threadIdx.x;
+  const int nch = (nitems + STG_CH - 1) / STG_CH;
+  auto stage = [&](int c, double* b) {
+    for (int q = tid - 64; q < STG_CH * W; q += blockDim.x - 64) {
+      const int it = c * STG_CH + q / W;
+      b[q] = (it < nitems) ? load(it, q % W) : 0.0;
+    }
+  };
+  if (tid >= 64 && nch > 0) stage(0, lds);
+  __syncthreads();
+  for (int c = 0; c < nch; c++) {
+    if (tid >= 64) {
+      if (c + 1 < nch) stage(c + 1, lds + ((c + 1) & 1) * STG_CH * W);
+    } else {
+      consume(c * STG_CH, min(STG_CH, nitems - c * STG_CH), lds + (c & 1) * STG_CH * W);
+    }
+    __syncthreads();
+  }
+}
+
 // diag_source's term of each interior zone (diag.c:1247-1259 with the
 // geometric source only; cooling is off): UDIAG form of Dt dVF dUcomp[0]
 __global__ void k_acct_srcterm(KParams P, AcctRegions R, const double* __restrict__ pb, const double* __restrict__ gC,
@@ -111,54 +138,59 @@ __global__ void k_acct_srcterm(KParams P, AcctRegions R, const double* __restric
   store8(out, i * P.n2 + j, fd);
 }
 
-// sourceadd / sourceaddterms (lane = region*8 + k) and lumvsr (lanes of a
-// second group, one per x1 row), in ZLOOP order; the six zero cooling terms
-// of each zone are added as the reference adds them
-__global__ void k_acct_srcsum(KParams P, AcctRegions R, const double* __restrict__ term, double* __restrict__ acc,
-                              double* __restrict__ lumvsr) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < ACC_NREG * 8) {
-    const int r = t / 8, k = t % 8;
-    const int* ep = R.ep[r];
-    const int i0 = max(ep[HARM_X1DN], 0), i1 = min(ep[HARM_X1UP], P.n1 - 1);
-    const int j0 = max(ep[HARM_X2DN], 0), j1 = min(ep[HARM_X2UP], P.n2 - 1);
-    double* a = acc + r * REG_D;
-    double s = a[OFF_SRC + k], s0 = a[OFF_SRCT + k];
-    for (int i = i0; i <= i1; i++) {
-      const double* row = term + (size_t)i * P.n2 * 8 + k;
-      int j = j0;
-      for (; j + 8 <= j1 + 1; j += 8) {
-        double v[8];
-#pragma unroll
-        for (int m = 0; m < 8; m++) v[m] = row[(size_t)(j + m) * 8];
-#pragma unroll
-        for (int m = 0; m < 8; m++) {
-          s0 += v[m];
-          s += v[m];
-          s += 0.0;
-        }
-      }
-      for (; j <= j1; j++) {
-        const double v = row[(size_t)j * 8];
-        s0 += v;
-        s += v;
-        s += 0.0;
-      }
-    }
-    a[OFF_SRC + k] = s;
-    a[OFF_SRCT + k] = s0;
-  } else if (t >= 64 && t - 64 < P.n1) {
-    const int i = t - 64;
-    const int* ep = R.ep[0];
-    if (i < ep[HARM_X1DN] || i > ep[HARM_X1UP]) return;
-    const int j0 = max(ep[HARM_X2DN], 0), j1 = min(ep[HARM_X2UP], P.n2 - 1);
-    double l = lumvsr[i];
-    for (int j = j0; j <= j1; j++) {
-      l += term[((size_t)i * P.n2 + j) * 8 + UU];
-      l += 0.0;
-    }
-    lumvsr[i] = l;
+// sourceadd and sourceaddterms[0] (lane = region*8 + k) over the region's
+// zones in ZLOOP order.  The reference also adds the six cooling sources,
+// all +0.0 here; x + 0.0 differs from x only for x = -0.0, and a sum that
+// starts at +0.0 never becomes -0.0 (a sum is -0.0 only if both terms are),
+// so those adds change nothing and are left out.
+__global__ __launch_bounds__(256) void k_acct_srcsum(KParams P, AcctRegions R, const double* __restrict__ term,
+                                                     double* __restrict__ acc) {
+  __shared__ double lds[2 * STG_CH * 8];
+  const int l = threadIdx.x;
+  const int r = l / 8, k = l % 8;
+  const int* ep = R.ep[r < ACC_NREG ? r : 0];
+  double s = 0, s0 = 0;
+  if (l < ACC_NREG * 8) {
+    s = acc[r * REG_D + OFF_SRC + k];
+    s0 = acc[r * REG_D + OFF_SRCT + k];
   }
+  const int n2 = P.n2;
+  staged_items<8>(P.n1 * n2, lds, [&](int z, int w) { return term[(size_t)z * 8 + w]; },
+                  [&](int first, int cnt, const double* b) {
+                    if (l >= ACC_NREG * 8) return;
+                    int i = first / n2, j = first % n2;
+                    for (int m = 0; m < cnt; m++) {
+                      if (acct_in(ep, i, j)) {
+                        const double v = b[m * 8 + k];
+                        s0 += v;
+                        s += v;
+                      }
+                      if (++j == n2) { j = 0; i++; }
+                    }
+                  });
+  if (l < ACC_NREG * 8) {
+    acc[r * REG_D + OFF_SRC + k] = s;
+    acc[r * REG_D + OFF_SRCT + k] = s0;
+  }
+}
+
+// lumvsr (region 0, k = UU): one lane per x1 row, x2 in order
+__global__ void k_acct_lumsum(KParams P, AcctRegions R, const double* __restrict__ term, double* __restrict__ lumvsr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int* ep = R.ep[0];
+  if (i >= P.n1 || i < ep[HARM_X1DN] || i > ep[HARM_X1UP]) return;
+  const int j0 = max(ep[HARM_X2DN], 0), j1 = min(ep[HARM_X2UP], P.n2 - 1);
+  const double* row = term + (size_t)i * P.n2 * 8 + UU;
+  double l = lumvsr[i];
+  for (int j = j0; j <= j1; j += 16) {
+    double v[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) v[m] = (j + m <= j1) ? row[(size_t)(j + m) * 8] : 0.0;
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+      if (j + m <= j1) l += v[m];
+  }
+  lumvsr[i] = l;
 }
 
 // diag_flux's terms of each boundary face (diag.c:649-803)
@@ -248,13 +280,17 @@ __global__ void k_acct_facesum(AcctRegions R, const double* __restrict__ term, d
   const int off = R.off[r][dir], len = flux ? R.len[r][dir] : 0;
   const int col = jet ? slot - ACC_NFL * 8 : slot;  // column of the face record
   double s = 0.0;
-  for (int e = off; e < off + len; e++) {
-    const double* o = term + (size_t)e * FACE_SLOTS;
-    if (jet) {
-      if (o[FACE_SLOTS - 1] != 0.0) s += o[col];
-    } else {
-      s += o[col];
+  for (int e = off; e < off + len; e += 16) {
+    double v[16], cnd[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) {
+      const bool in = e + m < off + len;
+      v[m] = in ? term[(size_t)(e + m) * FACE_SLOTS + col] : 0.0;
+      cnd[m] = (in && jet) ? term[(size_t)(e + m) * FACE_SLOTS + FACE_SLOTS - 1] : 1.0;
     }
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+      if (e + m < off + len && cnd[m] != 0.0) s += v[m];
   }
   double* a = acc + r * REG_D;
   if (jet) {
@@ -293,18 +329,21 @@ __global__ void k_acct_frsum(KParams P, const double* __restrict__ term, double*
   const int i = t / 8, k = t % 8;
   const double* row = term + (size_t)i * P.n2 * 8 + k;
   double s = 0;
-  for (int j = 0; j < P.n2; j++) s += row[(size_t)j * 8];
+  for (int j = 0; j < P.n2; j += 16) {
+    double v[16];
+#pragma unroll
+    for (int m = 0; m < 16; m++) v[m] = (j + m < P.n2) ? row[(size_t)(j + m) * 8] : 0.0;
+#pragma unroll
+    for (int m = 0; m < 16; m++)
+      if (j + m < P.n2) s += v[m];
+  }
   frdot[t] = s;
 }
 
-// divbmaxavg: |div B| of each corner of LOOPDIVB (SETFDIVBFLUXCT,
-// global.h:1152-1175), then the first maximum and the sum in loop order
-__global__ void k_divb(KParams P, const double* __restrict__ p, const double* __restrict__ gC,
-                       double* __restrict__ out) {
-  const int jj = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ii = blockIdx.y * blockDim.y + threadIdx.y;
-  if (ii >= P.n1 + 4 || jj >= P.n2 + 4) return;
-  const int i = ii - 2, j = jj - 2;
+// |div B| at corner (i,j) (SETFDIVBFLUXCT, global.h:1152-1175), zero on
+// the rows j=0 and j=N2
+__device__ double corner_divb(const KParams& P, const double* __restrict__ p, const double* __restrict__ gC, int i,
+                              int j) {
   double divb = 0.;
   if ((j != 0) && (j != P.n2)) {
     const double dx1 = P.dx1, dx2 = P.dx2;
@@ -322,26 +361,44 @@ __global__ void k_divb(KParams P, const double* __restrict__ p, const double* __
                        b2mm * gmm;
     divb = fabs(num / (gC[(size_t)z00 * GCN + 35] * 0.125 * fabs(fabs(sum) + 1.e-20)));
   }
-  out[ii * (P.n2 + 4) + jj] = divb;
+  return divb;
 }
 
-__global__ void k_divb_reduce(KParams P, const double* __restrict__ divb, double* __restrict__ res) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// divbmaxavg: |div B| of each corner of LOOPDIVB, then the first maximum and
+// the sum in loop order
+__global__ void k_divb(KParams P, const double* __restrict__ p, const double* __restrict__ gC,
+                       double* __restrict__ out) {
+  const int jj = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ii = blockIdx.y * blockDim.y + threadIdx.y;
+  if (ii >= P.n1 + 4 || jj >= P.n2 + 4) return;
+  out[ii * (P.n2 + 4) + jj] = corner_divb(P, p, gC, ii - 2, jj - 2);
+}
+
+__global__ __launch_bounds__(256) void k_divb_reduce(KParams P, const double* __restrict__ divb,
+                                                     double* __restrict__ res) {
+  __shared__ double lds[2 * STG_CH];
   const int w = P.n2 + 4;
   double mx = 0, sum = 0;
   int imax = 0, jmax = 0;
-  for (int ii = 0; ii < P.n1 + 4; ii++) {
-    const double* row = divb + (size_t)ii * w;
-    for (int jj = 0; jj < w; jj++) {
-      const double d = row[jj];
-      if (d > mx) { imax = ii - 2; jmax = jj - 2; mx = d; }
-      sum += d;
-    }
+  staged_items<1>((P.n1 + 4) * w, lds, [&](int it, int) { return divb[it]; },
+                  [&](int first, int cnt, const double* b) {
+                    if (threadIdx.x != 0) return;
+                    for (int m = 0; m < cnt; m++) {
+                      const double d = b[m];
+                      if (d > mx) {
+                        imax = (first + m) / w - 2;
+                        jmax = (first + m) % w - 2;
+                        mx = d;
+                      }
+                      sum += d;
+                    }
+                  });
+  if (threadIdx.x == 0) {
+    res[0] = mx;
+    res[1] = sum / (double)(P.n1 * P.n2);
+    res[2] = (double)imax;
+    res[3] = (double)jmax;
   }
-  res[0] = mx;
-  res[1] = sum / (double)(P.n1 * P.n2);
-  res[2] = (double)imax;
-  res[3] = (double)jmax;
 }
 
 // constotal: U (UDIAG) dVF of each interior zone, then lane k sums the
@@ -366,17 +423,23 @@ __global__ void k_cons_term(KParams P, AcctRegions R, const double* __restrict__
   store8(out, i * P.n2 + j, ft);
 }
 
-__global__ void k_cons_sum(KParams P, AcctRegions R, int region, const double* __restrict__ term,
-                           double* __restrict__ res) {
+__global__ __launch_bounds__(256) void k_cons_sum(KParams P, AcctRegions R, int region,
+                                                  const double* __restrict__ term, double* __restrict__ res) {
+  __shared__ double lds[2 * STG_CH * 8];
   const int k = threadIdx.x;
-  if (k >= NPR) return;
   const int* ep = R.ep[region];
-  const int i0 = max(ep[HARM_X1DN], 0), i1 = min(ep[HARM_X1UP], P.n1 - 1);
-  const int j0 = max(ep[HARM_X2DN], 0), j1 = min(ep[HARM_X2UP], P.n2 - 1);
+  const int n2 = P.n2;
   double s = 0.0;
-  for (int i = i0; i <= i1; i++)
-    for (int j = j0; j <= j1; j++) s += term[((size_t)i * P.n2 + j) * 8 + k];
-  res[k] = s;
+  staged_items<8>(P.n1 * n2, lds, [&](int z, int w) { return term[(size_t)z * 8 + w]; },
+                  [&](int first, int cnt, const double* b) {
+                    if (k >= NPR) return;
+                    int i = first / n2, j = first % n2;
+                    for (int m = 0; m < cnt; m++) {
+                      if (acct_in(ep, i, j)) s += b[m * 8 + k];
+                      if (++j == n2) { j = 0; i++; }
+                    }
+                  });
+  if (k < NPR) res[k] = s;
 }
 
 // limit_gamma on the face states of fluxcalc (step_ch.c:978-979): the event
@@ -501,16 +564,19 @@ __global__ __launch_bounds__(256) void k_acct_compact(int nslots, int* __restric
 
 // the substage's events added in order: lanes 0-23 fladd[r][k], then
 // fladdterms[r][cat][k]; floor and repair events count on the last substage
-__global__ void k_acct_merge(KParams P, AcctRegions R, const int* __restrict__ list, const int* __restrict__ total,
-                             const double* __restrict__ val, double* __restrict__ acc, int finalstep) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(256) void k_acct_merge(KParams P, AcctRegions R, const int* __restrict__ list,
+                                                    const int* __restrict__ total, const double* __restrict__ val,
+                                                    double* __restrict__ acc, int finalstep) {
+  __shared__ double ev_val[256 * 8];
+  __shared__ int ev_i[256], ev_j[256], ev_cat[256];
+  const int t = threadIdx.x;
   constexpr int NT = ACC_NREG * 8, NTT = ACC_NREG * ACC_NFLOOR * 8;
-  if (t >= NT + NTT) return;
-  int r, cat = -1, k;
+  const bool lane = t < NT + NTT;
+  int r = 0, cat = -1, k = 0;
   if (t < NT) {
     r = t / 8;
     k = t % 8;
-  } else {
+  } else if (lane) {
     const int u = t - NT;
     r = u / (ACC_NFLOOR * 8);
     cat = (u / 8) % ACC_NFLOOR;
@@ -519,32 +585,49 @@ __global__ void k_acct_merge(KParams P, AcctRegions R, const int* __restrict__ l
   const int n1 = P.n1, n2 = P.n2;
   const int bB = acct_base(n1, n2, 1), bC = acct_base(n1, n2, 2), bD = acct_base(n1, n2, 3);
   double* dst = acc + r * REG_D + ((cat < 0) ? OFF_FLADD + k : OFF_FLT + cat * 8 + k);
-  double s = *dst;
+  double s = lane ? *dst : 0.0;
   const int E = *total;
   const int* ep = R.ep[r];
-  for (int e = 0; e < E; e++) {
-    const int slot = list[2 * e], c = list[2 * e + 1];
-    if (cat >= 0 && c != cat) continue;
-    int i, j;
-    if (slot < bB) {
-      const int f = slot >> 1;
-      i = f / (n2 + 2);
-      j = f % (n2 + 2) - 1;
-    } else if (slot < bC) {
-      const int f = (slot - bB) >> 1;
-      i = f / (n2 + 1) - 1;
-      j = f % (n2 + 1);
-    } else {
-      const int rel = slot - ((slot < bD) ? bC : bD);
-      if ((rel & 1) && !finalstep) continue;  // COUNTFLOORACT / the repair: accounting substage only
-      const int f = rel >> 1;
-      i = f / n2;
-      j = f % n2;
+  for (int base = 0; base < E; base += 256) {
+    // stage 256 events: where, which category (-1: not counted on this substage), the values
+    const int e = base + t;
+    if (e < E) {
+      const int slot = list[2 * e];
+      int c = list[2 * e + 1], i, j;
+      if (slot < bB) {
+        const int f = slot >> 1;
+        i = f / (n2 + 2);
+        j = f % (n2 + 2) - 1;
+      } else if (slot < bC) {
+        const int f = (slot - bB) >> 1;
+        i = f / (n2 + 1) - 1;
+        j = f % (n2 + 1);
+      } else {
+        const int rel = slot - ((slot < bD) ? bC : bD);
+        if ((rel & 1) && !finalstep) c = -1;  // COUNTFLOORACT / the repair: accounting substage only
+        const int f = rel >> 1;
+        i = f / n2;
+        j = f % n2;
+      }
+      ev_i[t] = i;
+      ev_j[t] = j;
+      ev_cat[t] = c;
+#pragma unroll
+      for (int m = 0; m < 8; m++) ev_val[t * 8 + m] = val[(size_t)slot * 8 + m];
     }
-    if (!acct_in(ep, i, j)) continue;
-    s += val[(size_t)slot * 8 + k];
+    __syncthreads();
+    if (lane) {
+      const int cnt = min(256, E - base);
+      for (int m = 0; m < cnt; m++) {
+        const int c = ev_cat[m];
+        if (c < 0 || (cat >= 0 && c != cat)) continue;
+        if (!acct_in(ep, ev_i[m], ev_j[m])) continue;
+        s += ev_val[m * 8 + k];
+      }
+    }
+    __syncthreads();
   }
-  *dst = s;
+  if (lane) *dst = s;
 }
 
 }  // namespace
@@ -607,8 +690,9 @@ static int acct_source(harm_ctx* c, int pi, int pb, int pf) {
   const dim3 b(64, 4);
   hipLaunchKernelGGL(k_acct_srcterm, grid2(c->kp.n2, c->kp.n1, b), b, 0, c->stream, c->kp, a->R, c->arr[pbs], c->gC,
                      c->gConn, c->ds, a->scratch);
-  hipLaunchKernelGGL(k_acct_srcsum, dim3((64 + c->kp.n1 + 63) / 64), dim3(64), 0, c->stream, c->kp, a->R, a->scratch,
-                     a->acc, a->lumvsr);
+  hipLaunchKernelGGL(k_acct_srcsum, dim3(1), dim3(256), 0, c->stream, c->kp, a->R, a->scratch, a->acc);
+  hipLaunchKernelGGL(k_acct_lumsum, dim3((c->kp.n1 + 63) / 64), dim3(64), 0, c->stream, c->kp, a->R, a->scratch,
+                     a->lumvsr);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -640,9 +724,10 @@ static int acct_merge(harm_ctx* c, int finalstep) {
   hipLaunchKernelGGL(k_acct_count, dim3(a->nblocks), dim3(256), 0, c->stream, a->nslots, a->flag, a->cnt);
   hipLaunchKernelGGL(k_acct_scan, dim3(1), dim3(64), 0, c->stream, a->nblocks, a->cnt, a->off, a->total);
   hipLaunchKernelGGL(k_acct_compact, dim3(a->nblocks), dim3(256), 0, c->stream, a->nslots, a->flag, a->off, a->list);
-  const int nl = ACC_NREG * 8 * (1 + ACC_NFLOOR);
-  hipLaunchKernelGGL(k_acct_merge, dim3((nl + 63) / 64), dim3(64), 0, c->stream, c->kp, a->R, a->list, a->total,
-                     a->val, a->acc, finalstep);
+  constexpr int nl = ACC_NREG * 8 * (1 + ACC_NFLOOR);
+  static_assert(nl <= 256, "one workgroup holds every accumulator lane");
+  hipLaunchKernelGGL(k_acct_merge, dim3(1), dim3(256), 0, c->stream, c->kp, a->R, a->list, a->total, a->val, a->acc,
+                     finalstep);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -763,7 +848,7 @@ extern "C" int harm_divbmaxavg(harm_ctx* c, int sel, double* divbmax, double* di
   const dim3 b(64, 4);
   hipLaunchKernelGGL(k_divb, grid2(c->kp.n2 + 4, c->kp.n1 + 4, b), b, 0, c->stream, c->kp, c->arr[sel], c->gC,
                      a->scratch);
-  hipLaunchKernelGGL(k_divb_reduce, dim3(1), dim3(64), 0, c->stream, c->kp, a->scratch, a->out);
+  hipLaunchKernelGGL(k_divb_reduce, dim3(1), dim3(256), 0, c->stream, c->kp, a->scratch, a->out);
   HIPCHK(hipGetLastError());
   double r[4];
   HIPCHK(hipMemcpyAsync(r, a->out, sizeof(r), hipMemcpyDeviceToHost, c->stream));
@@ -785,7 +870,7 @@ extern "C" int harm_constotal(harm_ctx* c, int sel, int region, double* vars) {
   const dim3 b(64, 4);
   hipLaunchKernelGGL(k_cons_term, grid2(c->kp.n2, c->kp.n1, b), b, 0, c->stream, c->kp, a->R, c->arr[sel], c->gC,
                      a->scratch);
-  hipLaunchKernelGGL(k_cons_sum, dim3(1), dim3(64), 0, c->stream, c->kp, a->R, region, a->scratch, a->out);
+  hipLaunchKernelGGL(k_cons_sum, dim3(1), dim3(256), 0, c->stream, c->kp, a->R, region, a->scratch, a->out);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(vars, a->out, 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));

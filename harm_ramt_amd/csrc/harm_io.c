@@ -1,5 +1,5 @@
 /*
- * harm_io.c -- restart files in the reference's format (host side).
+ * harm_io.c -- restart and dump files in the reference's format (host side).
  *
  * restart_write()/restart_read() (restart.c:182-264) go through dump_gen()
  * (dump.c:641): the header of write_restart_header() (restart.c:554-682),
@@ -199,4 +199,50 @@ int harm_rdump_read_host(const char *path, int binary, harm_restart_header *h, i
     }
   fclose(f);
   return rc == 2 ? 2 : (rc ? 4 : 0);
+}
+
+/* dump.c:48-80 dump_header: 16 fields, text on one line, binary with their
+   C sizes */
+static const char DUMP_HDR_FMT[] =
+    "%21.15g %d %d %21.15g %21.15g %21.15g %21.15g %ld %21.15g %21.15g %21.15g %21.15g %21.15g %21.15g %21.15g %d\n";
+
+int harm_dump_write_host(const char *path, int binary, const harm_dump_header *h, int n1, int n2, const double *cols) {
+  if (!path || !h || !cols) return 1;
+  if (h->totalsize[0] != n1 || h->totalsize[1] != n2) return 2;
+  FILE *f = fopen(path, binary ? "w" : "wt");
+  if (!f) return 3;
+  int rc = 0;
+  if (binary) {
+    rc |= fwrite(&h->t, sizeof(double), 1, f) != 1;
+    rc |= fwrite(&h->totalsize[0], sizeof(int), 1, f) != 1;
+    rc |= fwrite(&h->totalsize[1], sizeof(int), 1, f) != 1;
+    rc |= fwrite(&h->startx[0], sizeof(double), 1, f) != 1;
+    rc |= fwrite(&h->startx[1], sizeof(double), 1, f) != 1;
+    rc |= fwrite(&h->dx[0], sizeof(double), 1, f) != 1;
+    rc |= fwrite(&h->dx[1], sizeof(double), 1, f) != 1;
+    rc |= fwrite(&h->realnstep, sizeof(long), 1, f) != 1;
+    rc |= fwrite(&h->gam, sizeof(double), 1, f) != 1;
+    rc |= fwrite(&h->a, sizeof(double), 1, f) != 1;
+    rc |= fwrite(&h->R0, sizeof(double), 1, f) != 1;
+    rc |= fwrite(&h->Rin, sizeof(double), 1, f) != 1;
+    rc |= fwrite(&h->Rout, sizeof(double), 1, f) != 1;
+    rc |= fwrite(&h->hslope, sizeof(double), 1, f) != 1;
+    rc |= fwrite(&h->dt, sizeof(double), 1, f) != 1;
+    rc |= fwrite(&h->defcoord, sizeof(int), 1, f) != 1;
+  } else {
+    rc |= fprintf(f, DUMP_HDR_FMT, h->t, h->totalsize[0], h->totalsize[1], h->startx[0], h->startx[1], h->dx[0],
+                  h->dx[1], h->realnstep, h->gam, h->a, h->R0, h->Rin, h->Rout, h->hslope, h->dt, h->defcoord) < 0;
+  }
+  for (int j = 0; j < n2 && !rc; j++)
+    for (int i = 0; i < n1 && !rc; i++) {
+      const double *z = cols + ((size_t)i * n2 + j) * HARM_DUMP_COLUMNS;
+      if (binary) {
+        rc |= fwrite(z, sizeof(double), HARM_DUMP_COLUMNS, f) != HARM_DUMP_COLUMNS;
+      } else {
+        for (int k = 0; k < HARM_DUMP_COLUMNS; k++)
+          rc |= fprintf(f, (k < HARM_DUMP_COLUMNS - 1) ? "%21.15g " : "%21.15g\n", z[k]) < 0;
+      }
+    }
+  rc |= fclose(f) != 0;
+  return rc ? 4 : 0;
 }

@@ -1860,6 +1860,7 @@ __global__ void kb_fixapply(const RunDesc* __restrict__ runs, int sel) {
 
 namespace {
 struct AcctState;  // conservation accounting (harm_acct.inl)
+struct CurState;   // the current J, F (harm_diag.inl)
 }
 
 struct harm_ctx {
@@ -1898,7 +1899,12 @@ struct harm_ctx {
   double ktime[HARM_K_COUNT];
   long long kcount[HARM_K_COUNT];
   AcctState* acct;  // null until accounting or a dump-time reduction is first used
+  CurState* cur;    // null until the current is first enabled
 };
+
+static void cur_free(harm_ctx* c);
+static bool cur_on(const harm_ctx* c);
+static int cur_step(harm_ctx* c);
 
 static void acct_free(harm_ctx* c);
 static bool acct_on(const harm_ctx* c);
@@ -2047,6 +2053,7 @@ extern "C" int harm_destroy(harm_ctx* c) {
   }
   delete c->evkid;
   acct_free(c);
+  cur_free(c);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   free(c);
   return 0;
@@ -2404,7 +2411,8 @@ static int enqueue_step(harm_ctx* c) {
     if (launch_boundary(c, pff[to], mode)) return 1;
     if (acct && acct_merge(c, to == nto - 1)) return 1;
   }
-  // step_ch.c:160 diag_flux with the step's dt
+  // step_ch.c:143-153 the current, :160 diag_flux, with the step's dt
+  if (cur_on(c) && cur_step(c)) return 1;
   if (acct_on(c) && acct_flux(c, -1.0)) return 1;
   return 0;
 }
@@ -2976,3 +2984,4 @@ extern "C" int harm_batch_get_tuning(harm_batch* b, int* flux_variant, int* boun
 namespace {
 #include "harm_adjoint.inl"
 #include "harm_acct.inl"
+#include "harm_diag.inl"

@@ -699,3 +699,15 @@ int harm_ener_regions(const harm_grid *g, double h_over_r_jet, int enerpos[HARM_
   df[HARM_X2UP] = N2;
   return 0;
 }
+
+/* coord() and bl_coord() at a zone centre (coord.c:920-939, 334-455): the
+   coordinate columns of the dump (dump.c:102-103) */
+int harm_zone_coords(const harm_grid *g, int i, int j, double *X1, double *X2, double *r, double *th) {
+  if (!g) return 1;
+  double X[NDIM];
+  coord(g, i, j, CENT, X);
+  bl_coord(g, X, r, th);
+  *X1 = X[1];
+  *X2 = X[2];
+  return 0;
+}

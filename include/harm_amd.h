@@ -412,6 +412,42 @@ int harm_divbmaxavg(harm_ctx *ctx, int sel, double *divbmax, double *divbavg, in
 /* constotal() of array sel over a region: sum of U (UDIAG form) dVF */
 int harm_constotal(harm_ctx *ctx, int sel, int region, double *vars);
 
+/* ---- the current and the 56-column dump (phys.c:1124-1460,
+   dump.c:23-191) ----
+   The reference's step also computes the current J^mu and the Faraday
+   tensor F^{mu nu} (WHICHCURRENTCALC 1, step_ch.c:143-153: the time
+   derivative from the last three steps once three steps have run).  On, the
+   steps here do the same (from a zero history, as a fresh run). */
+int harm_current_enable(harm_ctx *ctx, int on);
+/* host arrays jcon[N1+6][N2+6][4], fcon[N1+6][N2+6][6] (either may be NULL) */
+int harm_get_current(harm_ctx *ctx, double *jcon, double *fcon);
+
+/* dump_content() columns: i, j, X1, X2, r, th, the 8 primitives, div B,
+   u^mu, u_mu, b^mu, b_mu, vmin/vmax in x1 and x2, gdet, J^mu, J_mu, the 6
+   F^{mu nu}, the 6 F_{mu nu} */
+#define HARM_DUMP_COLUMNS 56
+typedef struct {                    /* dump_header() fields (dump.c:48-80) */
+  double t;
+  int totalsize[2];
+  double startx[2], dx[2];
+  long realnstep;
+  double gam, a, R0, Rin, Rout, hslope, dt;
+  int defcoord;
+} harm_dump_header;
+/* cols[N1][N2][56] in zone order as the file holds them (j outer, i inner):
+   text "%21.15g" separated by spaces, one zone per line, or raw doubles.
+   Returns 0, 1 bad argument, 2 header grid size differs, 3 cannot open,
+   4 short write. */
+int harm_dump_write_host(const char *path, int binary, const harm_dump_header *h, int n1, int n2,
+                         const double *cols);
+/* dump() of the context's state (dump.c:23-46): the grid gives the
+   coordinate columns and header fields, the context t, dt and gam;
+   realnstep is the caller's step count.  J and F columns are zero unless
+   harm_current_enable is on. */
+int harm_dump_write(harm_ctx *ctx, const harm_grid *grid, const char *path, int binary, long realnstep);
+/* coord()/bl_coord() at zone centre (i,j) */
+int harm_zone_coords(const harm_grid *g, int i, int j, double *X1, double *X2, double *r, double *th);
+
 #ifdef __cplusplus
 }
 #endif

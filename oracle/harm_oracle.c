@@ -77,6 +77,12 @@ struct oracle_ctx {
   double dx3;
   double *frdot;     /* [n1][8] */
   double *lumvsr;    /* [n1] */
+  int cur;           /* current (jcon, fcon) computed every step */
+  double *cf;        /* cfaraday [nz][5][3] */
+  double *fcon;      /* [nz][6] */
+  double *jcon;      /* [nz][4] */
+  long long calls;   /* current_calc_1's statics */
+  double lastdt;
 };
 
 typedef struct {
@@ -1497,7 +1503,9 @@ int or_step(oracle_ctx *c) {
     or_post_fixup(c, pff[to]);
     fail |= or_bound_prim(c, pff[to]);
   }
-  /* step_ch.c:160 diag_flux with the step's dt */
+  /* step_ch.c:143-153 the current (WHICHCURRENTCALC 1), :160 diag_flux,
+     both with the step's dt */
+  if (c->cur) or_current(c);
   if (c->acct) or_diag_flux(c, c->dt);
   c->t += c->dt;
   if (ndt > c->p.safe * c->dt) ndt = c->p.safe * c->dt;
@@ -1585,6 +1593,7 @@ void or_destroy(oracle_ctx *c) {
   for (int a = 0; a < 12; a++) free(c->arr[a]);
   free(c->pflag);
   free(c->frdot); free(c->lumvsr);
+  free(c->cf); free(c->fcon); free(c->jcon);
   free(c);
 }
 
@@ -1729,6 +1738,29 @@ void or_diag_flux(oracle_ctx *c, double Dt) {
   }
 }
 
+/* SETFDIVBFLUXCT (global.h:1152-1175): |div B| at corner (i,j), zero on the
+   rows j=0 and j=N2 */
+static double corner_divb(const oracle_ctx *c, const double *p, int i, int j) {
+  const int n2 = c->p.n2;
+  const double dx1 = c->p.dx1, dx2 = c->p.dx2;
+  if (!((j != 0) && (j != n2))) return 0.;
+#define GC(i, j) c->gdet[(size_t)zi(c, (i), (j)) * 4 + CENT]
+#define BB(i, j, k) PA(c, p, (i), (j))[k]
+  double num = (((dx1) > (dx2)) ? (dx1) : (dx2)) *
+               (0.5 * (BB(i, j, B1) * GC(i, j) + BB(i, j - 1, B1) * GC(i, j - 1) - BB(i - 1, j, B1) * GC(i - 1, j) -
+                       BB(i - 1, j - 1, B1) * GC(i - 1, j - 1)) /
+                    dx1 +
+                0.5 * (BB(i, j, B2) * GC(i, j) + BB(i - 1, j, B2) * GC(i - 1, j) - BB(i, j - 1, B2) * GC(i, j - 1) -
+                       BB(i - 1, j - 1, B2) * GC(i - 1, j - 1)) /
+                    dx2);
+  double sum = BB(i, j, B1) * GC(i, j) + BB(i, j - 1, B1) * GC(i, j - 1) + BB(i - 1, j, B1) * GC(i - 1, j) +
+               BB(i - 1, j - 1, B1) * GC(i - 1, j - 1) + BB(i, j, B2) * GC(i, j) + BB(i - 1, j, B2) * GC(i - 1, j) +
+               BB(i, j - 1, B2) * GC(i, j - 1) + BB(i - 1, j - 1, B2) * GC(i - 1, j - 1);
+#undef GC
+#undef BB
+  return fabs(num / (c->gdet[(size_t)zi(c, i, j) * 4 + CORN] * 0.125 * fabs(fabs(sum) + 1.e-20)));
+}
+
 /* dump_ener.c:449-486 divbmaxavg with SETFDIVBFLUXCT (global.h:1152-1175):
    the corner divergence over LOOPDIVB (i in [-2,N1+2), j in [-2,N2+2)),
    zero on the rows j=0 and j=N2; the first largest and the sum over
@@ -1740,31 +1772,13 @@ void or_divbmaxavg(const oracle_ctx *c, int sel, double *divbmax_out, double *di
   const double dx1 = c->p.dx1, dx2 = c->p.dx2;
   int imax = 0, jmax = 0;
   double divbmax = 0, divbavg = 0;
-#define GC(i, j) c->gdet[(size_t)zi(c, (i), (j)) * 4 + CENT]
-#define BB(i, j, k) PA(c, p, (i), (j))[k]
+  (void)dx1; (void)dx2;
   for (int i = -2; i < n1 + 2; i++)
     for (int j = -2; j < n2 + 2; j++) {
-      double divb;
-      if ((j != 0) && (j != n2)) {
-        double num = (((dx1) > (dx2)) ? (dx1) : (dx2)) *
-                     (0.5 * (BB(i, j, B1) * GC(i, j) + BB(i, j - 1, B1) * GC(i, j - 1) - BB(i - 1, j, B1) * GC(i - 1, j) -
-                             BB(i - 1, j - 1, B1) * GC(i - 1, j - 1)) /
-                          dx1 +
-                      0.5 * (BB(i, j, B2) * GC(i, j) + BB(i - 1, j, B2) * GC(i - 1, j) - BB(i, j - 1, B2) * GC(i, j - 1) -
-                             BB(i - 1, j - 1, B2) * GC(i - 1, j - 1)) /
-                          dx2);
-        double sum = BB(i, j, B1) * GC(i, j) + BB(i, j - 1, B1) * GC(i, j - 1) + BB(i - 1, j, B1) * GC(i - 1, j) +
-                     BB(i - 1, j - 1, B1) * GC(i - 1, j - 1) + BB(i, j, B2) * GC(i, j) + BB(i - 1, j, B2) * GC(i - 1, j) +
-                     BB(i, j - 1, B2) * GC(i, j - 1) + BB(i - 1, j - 1, B2) * GC(i - 1, j - 1);
-        divb = fabs(num / (c->gdet[(size_t)zi(c, i, j) * 4 + CORN] * 0.125 * fabs(fabs(sum) + 1.e-20)));
-      } else {
-        divb = 0.;
-      }
+      double divb = corner_divb(c, p, i, j);
       if (divb > divbmax) { imax = i; jmax = j; divbmax = divb; }
       divbavg += divb;
     }
-#undef GC
-#undef BB
   divbavg /= (double)(n1 * n2);
   *divbmax_out = divbmax; *divbavg_out = divbavg;
   if (imax_out) *imax_out = imax;
@@ -1791,5 +1805,151 @@ void or_constotal(const oracle_ctx *c, int sel, int region, double *vars) {
         double ftemp = U[k] * c->A.dVF;
         vars[k] += ftemp;
       }
+    }
+}
+
+/* ---------------- current and the 56-column dump ---------------- */
+
+void or_current_enable(oracle_ctx *c, int on) {
+  if (!c->cf) {
+    c->cf = (double *)calloc((size_t)c->nz * 15, sizeof(double));
+    c->fcon = (double *)calloc((size_t)c->nz * 6, sizeof(double));
+    c->jcon = (double *)calloc((size_t)c->nz * 4, sizeof(double));
+  } else {
+    memset(c->cf, 0, sizeof(double) * (size_t)c->nz * 15);
+    memset(c->fcon, 0, sizeof(double) * (size_t)c->nz * 6);
+    memset(c->jcon, 0, sizeof(double) * (size_t)c->nz * 4);
+  }
+  c->calls = 0;
+  c->lastdt = 0.0;
+  c->cur = on;
+}
+
+/* phys.c:1124-1218 current_doprecalc(0..3) + current_precalc with
+   WHICHCURRENTCALC 1 (face CENT, every zone), then :1335-1460
+   current_calc_1 over LOOPP11 x LOOPP12, with its statics calls, lastdt */
+void or_current(oracle_ctx *c) {
+  const double *p = c->arr[OR_P];
+  const int n1 = c->p.n1, n2 = c->p.n2;
+  for (int i = -NG; i < n1 + NG; i++)
+    for (int j = -NG; j < n2 + NG; j++) {
+      geom_t g;
+      state_t q;
+      get_geometry(c, i, j, CENT, &g);
+      get_state(PA(c, p, i, j), &g, &q);
+      double (*f)[3] = (double (*)[3])(c->cf + (size_t)zi(c, i, j) * 15);
+      const double *bc = q.bcov, *uc = q.ucov;
+      f[3][0] = f[4][0]; f[3][1] = f[4][1]; f[3][2] = f[4][2];
+      f[4][0] = f[0][0]; f[4][1] = f[0][1]; f[4][2] = f[0][2];
+      f[0][0] = -1.0 / g.g * (-bc[3] * uc[2] + bc[2] * uc[3]);
+      f[0][1] = -1.0 / g.g * (bc[3] * uc[1] - bc[1] * uc[3]);
+      f[0][2] = -1.0 / g.g * (-bc[2] * uc[1] + bc[1] * uc[2]);
+      f[1][0] = -1.0 / g.g * (bc[3] * uc[2] - bc[2] * uc[3]);
+      f[1][1] = -1.0 / g.g * (-bc[3] * uc[0] + bc[0] * uc[3]);
+      f[1][2] = -1.0 / g.g * (bc[2] * uc[0] - bc[0] * uc[2]);
+      f[2][0] = -1.0 / g.g * (-bc[3] * uc[1] + bc[1] * uc[3]);
+      f[2][1] = -1.0 / g.g * (bc[3] * uc[0] - bc[0] * uc[3]);
+      f[2][2] = -1.0 / g.g * (-bc[1] * uc[0] + bc[0] * uc[1]);
+      double *fc = c->fcon + (size_t)zi(c, i, j) * 6;
+      fc[0] = -1.0 / g.g * (bc[3] * uc[2] - bc[2] * uc[3]);
+      fc[1] = -1.0 / g.g * (-bc[3] * uc[1] + bc[1] * uc[3]);
+      fc[2] = -1.0 / g.g * (bc[2] * uc[1] - bc[1] * uc[2]);
+      fc[3] = -1.0 / g.g * (bc[3] * uc[0] - bc[0] * uc[3]);
+      fc[4] = -1.0 / g.g * (-bc[2] * uc[0] + bc[0] * uc[2]);
+      fc[5] = -1.0 / g.g * (bc[1] * uc[0] - bc[0] * uc[1]);
+    }
+  const double dtl = c->lastdt, dtr = c->dt;
+  const double idx1 = 1.0 / (2.0 * c->p.dx1), idx2 = 1.0 / (2.0 * c->p.dx2);
+#define CF(i, j, s, m) c->cf[(size_t)zi(c, (i), (j)) * 15 + (s) * 3 + (m)]
+#define GD(i, j) c->gdet[(size_t)zi(c, (i), (j)) * 4 + CENT]
+  for (int i = -2; i < n1 + 2; i++)
+    for (int j = -2; j < n2 + 2; j++) {
+      const double gt = GD(i, j), gr = GD(i, j), grp1 = GD(i + 1, j), ghp1 = GD(i, j + 1), grm1 = GD(i - 1, j),
+                   ghm1 = GD(i, j - 1);
+      double *jc = c->jcon + (size_t)zi(c, i, j) * 4;
+      jc[0] = 1. / gt * (grp1 * CF(i + 1, j, 1, 0) - grm1 * CF(i - 1, j, 1, 0)) * idx1 +
+              1. / gt * (ghp1 * CF(i, j + 1, 2, 0) - ghm1 * CF(i, j - 1, 2, 0)) * idx2;
+      double derf[3];
+      for (int m = 0; m < 3; m++) {
+        if (c->calls >= 2) {
+          const double fl = CF(i, j, 3, m), f0 = CF(i, j, 4, m), fr = CF(i, j, 0, m);
+          const double AA = (dtr * (fl - f0) + dtl * (fr - f0)) / (dtl * dtr * (dtl + dtr));
+          const double BB = (-dtr * dtr * (fl - f0) + dtl * dtl * (fr - f0)) / (dtl * dtr * (dtl + dtr));
+          derf[m] = 2.0 * AA * dtr + BB;
+        } else {
+          derf[m] = 0;
+        }
+      }
+      jc[1] = derf[0] + 1. / gt * (ghp1 * CF(i, j + 1, 2, 1) - ghm1 * CF(i, j - 1, 2, 1)) * idx2;
+      /* the reference differences F^{hr} with geomr (zone i) at i-1 */
+      jc[2] = derf[1] + 1. / gt * (grp1 * CF(i + 1, j, 1, 1) - gr * CF(i - 1, j, 1, 1)) * idx1;
+      jc[3] = derf[2] + 1. / gt * (grp1 * CF(i + 1, j, 1, 2) - grm1 * CF(i - 1, j, 1, 2)) * idx1 +
+              1. / gt * (ghp1 * CF(i, j + 1, 2, 2) - ghm1 * CF(i, j - 1, 2, 2)) * idx2;
+    }
+#undef CF
+#undef GD
+  c->calls++;
+  c->lastdt = c->dt;
+}
+
+void or_current_get(const oracle_ctx *c, double *jcon, double *fcon) {
+  if (jcon) memcpy(jcon, c->jcon, sizeof(double) * (size_t)c->nz * 4);
+  if (fcon) memcpy(fcon, c->fcon, sizeof(double) * (size_t)c->nz * 6);
+}
+
+/* phys.c:655-688 lowerf */
+static void lowerf(const double *fcon, const geom_t *g, double *fcov) {
+  double m[NDIM][NDIM], o[NDIM][NDIM];
+  m[0][0] = m[1][1] = m[2][2] = m[3][3] = 0;
+  m[0][1] = fcon[0]; m[0][2] = fcon[1]; m[0][3] = fcon[2];
+  m[1][2] = fcon[3]; m[1][3] = fcon[4]; m[2][3] = fcon[5];
+  m[1][0] = -fcon[0]; m[2][0] = -fcon[1]; m[3][0] = -fcon[2];
+  m[2][1] = -fcon[3]; m[3][1] = -fcon[4]; m[3][2] = -fcon[5];
+  for (int j = 0; j < NDIM; j++)
+    for (int k = 0; k < NDIM; k++) {
+      o[j][k] = 0;
+      for (int jp = 0; jp < NDIM; jp++)
+        for (int kp = 0; kp < NDIM; kp++) o[j][k] += m[jp][kp] * g->gcov[j][jp] * g->gcov[k][kp];
+    }
+  fcov[0] = o[0][1]; fcov[1] = o[0][2]; fcov[2] = o[0][3];
+  fcov[3] = o[1][2]; fcov[4] = o[1][3]; fcov[5] = o[2][3];
+}
+
+/* dump.c:82-191 dump_content without its six coordinate columns (i, j,
+   X1, X2, r, th): p, divb, ucon, ucov, bcon, bcov, vmin/vmax in x1 and x2,
+   gdet, jcon, jcov, fcon, fcov -- 50 values per interior zone, out[n1][n2][50] */
+void or_dump_content(const oracle_ctx *c, double *out) {
+  const double *p = c->arr[OR_P];
+  for (int i = 0; i < c->p.n1; i++)
+    for (int j = 0; j < c->p.n2; j++) {
+      geom_t g;
+      state_t q;
+      double vmax1, vmin1, vmax2, vmin2, jcov[NDIM], fcov[6];
+      const double *pz = PA(c, p, i, j);
+      get_geometry(c, i, j, CENT, &g);
+      get_state(pz, &g, &q);
+      vchar(c, pz, &q, 1, &g, &vmax1, &vmin1);
+      vchar(c, pz, &q, 2, &g, &vmax2, &vmin2);
+      double *o = out + ((size_t)i * c->p.n2 + j) * 50;
+      int n = 0;
+      for (int k = 0; k < NPR; k++) o[n++] = pz[k];
+      o[n++] = corner_divb(c, p, i, j);
+      for (int k = 0; k < NDIM; k++) o[n++] = q.ucon[k];
+      for (int k = 0; k < NDIM; k++) o[n++] = q.ucov[k];
+      for (int k = 0; k < NDIM; k++) o[n++] = q.bcon[k];
+      for (int k = 0; k < NDIM; k++) o[n++] = q.bcov[k];
+      o[n++] = vmin1; o[n++] = vmax1; o[n++] = vmin2; o[n++] = vmax2;
+      o[n++] = g.g;
+      const double *jc = c->cur ? c->jcon + (size_t)zi(c, i, j) * 4 : NULL;
+      const double *fc = c->cur ? c->fcon + (size_t)zi(c, i, j) * 6 : NULL;
+      double zero[6] = {0, 0, 0, 0, 0, 0};
+      if (!jc) jc = zero;
+      if (!fc) fc = zero;
+      lower(jc, &g, jcov);
+      for (int k = 0; k < NDIM; k++) o[n++] = jc[k];
+      for (int k = 0; k < NDIM; k++) o[n++] = jcov[k];
+      lowerf(fc, &g, fcov);
+      for (int k = 0; k < 6; k++) o[n++] = fc[k];
+      for (int k = 0; k < 6; k++) o[n++] = fcov[k];
     }
 }

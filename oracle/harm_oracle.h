@@ -126,6 +126,16 @@ void or_diag_flux(oracle_ctx *c, double Dt);
 void or_divbmaxavg(const oracle_ctx *c, int sel, double *divbmax, double *divbavg, int *imax, int *jmax);
 void or_constotal(const oracle_ctx *c, int sel, int region, double *vars);
 
+/* the current J^mu and Faraday F^{mu nu} the reference's step computes
+   (WHICHCURRENTCALC 1, phys.c:1124-1460): on, every step updates them
+   (zeroed, with current_calc_1's call count, when switched on) */
+void or_current_enable(oracle_ctx *c, int on);
+void or_current(oracle_ctx *c);
+void or_current_get(const oracle_ctx *c, double *jcon, double *fcon);  /* [nz][4], [nz][6] */
+/* dump_content (dump.c:82-191) after its six coordinate columns: 50 values
+   per interior zone, out[n1][n2][50] */
+void or_dump_content(const oracle_ctx *c, double *out);
+
 #ifdef __cplusplus
 }
 #endif

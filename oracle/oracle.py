@@ -96,6 +96,10 @@ def load():
         "or_diag_flux": ([vp, ctypes.c_double], None),
         "or_divbmaxavg": ([vp, ctypes.c_int, dp, dp, ip, ip], None),
         "or_constotal": ([vp, ctypes.c_int, ctypes.c_int, dp], None),
+        "or_current_enable": ([vp, ctypes.c_int], None),
+        "or_current": ([vp], None),
+        "or_current_get": ([vp, dp, dp], None),
+        "or_dump_content": ([vp, dp], None),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -264,6 +268,26 @@ class Oracle:
         im, jm = ctypes.c_int(), ctypes.c_int()
         self.lib.or_divbmaxavg(self.h, sel, ctypes.byref(mx), ctypes.byref(av), ctypes.byref(im), ctypes.byref(jm))
         return mx.value, av.value, im.value, jm.value
+
+    def current_enable(self, on=True):
+        self.lib.or_current_enable(self.h, int(on))
+
+    def current_step(self):
+        """the step's current computation on the current state and dt"""
+        self.lib.or_current(self.h)
+
+    def current(self):
+        """(jcon[N1M][N2M][4], fcon[N1M][N2M][6])"""
+        j = np.zeros(self.shape[:2] + (4,))
+        f = np.zeros(self.shape[:2] + (6,))
+        self.lib.or_current_get(self.h, _d(j), _d(f))
+        return j, f
+
+    def dump_content(self):
+        """dump_content's 50 physics columns per interior zone [n1][n2][50]"""
+        out = np.zeros((self.prm.n1, self.prm.n2, 50))
+        self.lib.or_dump_content(self.h, _d(out))
+        return out
 
     def constotal(self, region, sel=P):
         v = np.zeros(NPR)

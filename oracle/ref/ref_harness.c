@@ -16,8 +16,8 @@
  *
  * usage: ref_harness CONFIG   (key value lines, see read_config())
  * With "rdump 1" it also writes the final state's restart files, with
- * "diag 1" the conservation accumulators and dump-time reductions (see the
- * end of main()).
+ * "dump 1" its 56-column dump, with "diag 1" the conservation accumulators
+ * and dump-time reductions (see the end of main()).
  */
 #include "defs.h"
 #include "step_ch.h"
@@ -244,6 +244,20 @@ int main(int argc, char *argv[])
     restart_write(0);
     binaryoutput = BINARYOUTPUT;
     restart_write(1);
+  }
+  /* optional: the 56-column dump of the final state, written by the
+     reference's own dump() (dump.c:23-191, through dump_gen) as text into
+     dumps/dump0000 and as binary into dumps/dump0001 (with dump_gen's
+     binary suffix); the current and Faraday columns are what step_ch()
+     computed every step (step_ch.c:143-153) */
+  if (cfgi("dump", 0)) {
+    extern int init_dumps(void);
+    extern int dump(long dump_cnt);
+    init_dumps();
+    binaryoutput = TEXTOUTPUT;
+    dump(0);
+    binaryoutput = BINARYOUTPUT;
+    dump(1);
   }
 #if(USEMPI)
   MPI_Finalize();

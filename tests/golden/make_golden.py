@@ -94,7 +94,7 @@ def bc_of(kind):
 
 
 def run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, eno=0, rdump=False, diag=False,
-                  gammamax=None):
+                  gammamax=None, dump=False):
     p0.tofile(os.path.join(work, "p0.bin"))
     cfg = dict(prim=os.path.join(work, "p0.bin"), dt=dt0, nsteps=nsteps, out=os.path.join(work, "o.bin"),
                lim=lim, fluxmethod=flux, timeorder=to)
@@ -107,6 +107,9 @@ def run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, eno=0, rd
         os.makedirs(os.path.join(work, "dumps"), exist_ok=True)
     if diag:
         cfg.update(diag=1, diag_out=os.path.join(work, "diag.bin"), h_over_r_jet=0.4)
+    if dump:
+        cfg["dump"] = 1
+        os.makedirs(os.path.join(work, "dumps"), exist_ok=True)
     if kind in ("ks", "ks_mono", "ffde"):
         cfg.update(a=g.a, Rout=g.Rout, hslope=g.hslope, R0=g.R0, geom="ref", gam=4.0 / 3.0, cour=0.9,
                    tables_out=os.path.join(work, "tables.bin"))
@@ -198,18 +201,40 @@ def make_diag(name, work):
     print(name, "fladd", dg["fladd"][0], "divb", dg["divbmax"], dg["divbavg"])
 
 
+# the reference's 56-column dump (dump.c) of the final state, text and
+# binary, with the current and Faraday columns its steps computed
+DUMP_CASES = {"dump_torus_para_hll_rk4": ("ks", 32, 64, H.PARA, H.HLLFLUX, 4, 5, 1e-3)}
+
+
+def make_dump(name, work):
+    kind, n1, n2, lim, flux, to, nsteps, dt0 = DUMP_CASES[name]
+    g, geom, p0, extra = problem(kind, n1, n2)
+    recs, states = run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, dump=True)
+    d = os.path.join(work, "dumps")
+    text = open(os.path.join(d, "dump0000"), "rb").read()
+    binary = open(os.path.join(d, "dump0001.bin"), "rb").read()
+    np.savez_compressed(os.path.join(OUT_DIR, f"{name}.npz"), kind=kind, n1=n1, n2=n2, lim=lim, fluxmethod=flux,
+                        timeorder=to, nsteps=nsteps, dt0=dt0, input_sha=sha(p0), t=recs[:, 0], dt=recs[:, 1],
+                        p=states[-1], dump_text=np.frombuffer(text, dtype=np.uint8),
+                        dump_binary=np.frombuffer(binary, dtype=np.uint8))
+    print(name, "text", len(text), "bytes, binary", len(binary), "bytes")
+
+
 def main():
     subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle", "ref"), "-s"])
     with tempfile.TemporaryDirectory() as work:
         rd = [a for a in sys.argv[1:] if a in RDUMP_CASES] or ([] if sys.argv[1:] else list(RDUMP_CASES))
         for name in rd:
             make_rdump(name, work)
+        du = [a for a in sys.argv[1:] if a in DUMP_CASES] or ([] if sys.argv[1:] else list(DUMP_CASES))
+        for name in du:
+            make_dump(name, work)
         alld = {**DIAG_CASES, **DIAG_GAMMAMAX}
         dc = [a for a in sys.argv[1:] if a in alld] or ([] if sys.argv[1:] else list(alld))
         for name in dc:
             make_diag(name, work)
-        names = [a for a in sys.argv[1:] if a not in RDUMP_CASES and a not in alld] if sys.argv[1:] \
-            else list(CASES)
+        names = [a for a in sys.argv[1:] if a not in RDUMP_CASES and a not in alld and a not in DUMP_CASES] \
+            if sys.argv[1:] else list(CASES)
         for name in names:
             kind, n1, n2, lim, flux, to, nsteps, dt0, saved = CASES[name]
             g, geom, p0, extra = problem(kind, n1, n2)

@@ -44,7 +44,7 @@ def problem(kind, n1, n2, lim=H.PARA, fluxmethod=H.HLLFLUX, timeorder=4, seed=7,
 
 def golden_cases():
     return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
-                  if not os.path.basename(p).startswith(("ks_geometry", "rdump_", "diag_")))
+                  if not os.path.basename(p).startswith(("ks_geometry", "rdump_", "diag_", "dump_")))
 
 
 def load_golden(name):
