@@ -52,6 +52,7 @@ struct AcctState {
   int* flag;        // [nslots], category + 1 of the event in the slot
   double* val;      // [nslots][8]
   int* list;        // [nslots][2]: slot, category of the events in order
+  double* cval;     // [nslots][8]: their values in that order
   int* cnt;         // [nblocks] events per block of 256 slots, then their offsets
   int* off;
   int* total;
@@ -92,15 +93,25 @@ __device__ __forceinline__ void final_flux(const KParams& P, const double* __res
 // LDS while waves 1-3 stage the next chunk (double buffer), so that the
 // chain of dependent adds never waits on memory.  load(item, w) gives
 // component w of an item; consume(first, count, lds) runs on wave 0.
-constexpr int STG_CH = 256;
+constexpr int STG_CH = 1024;
 template <int W, class Load, class Consume>
 __device__ __forceinline__ void staged_items(int nitems, double* lds, Load load, Consume consume) {
   const int tid = threadIdx.x;
   const int nch = (nitems + STG_CH - 1) / STG_CH;
-  auto stage = [&](int c, double* b) {
-    for (int q = tid - 64; q < STG_CH * W; q += blockDim.x - 64) {
-      const int it = c * STG_CH + q / W;
-      b[q] = (it < nitems) ? load(it, q % W) : 0.0;
+  auto stage = [&](int c, double* b) {  // 16 loads in flight per thread
+    const int step = blockDim.x - 64;
+    for (int q0 = tid - 64; q0 < STG_CH * W; q0 += 16 * step) {
+      double v[16];
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const int q = q0 + u * step, it = c * STG_CH + q / W;
+        v[u] = (q < STG_CH * W && it < nitems) ? load(it, q % W) : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 16; u++) {
+        const int q = q0 + u * step;
+        if (q < STG_CH * W) b[q] = v[u];
+      }
     }
   };
   if (tid >= 64 && nch > 0) stage(0, lds);
@@ -112,6 +123,41 @@ __device__ __forceinline__ void staged_items(int nitems, double* lds, Load load,
       consume(c * STG_CH, min(STG_CH, nitems - c * STG_CH), lds + (c & 1) * STG_CH * W);
     }
     __syncthreads();
+  }
+}
+
+// Adds column k of the chunk's zones that lie in the box [i0,i1] x [j0,j1]
+// to the chains in ZLOOP order (zones i-major from `first`, n2 per row):
+// whole row segments at a time, eight LDS values ahead of the adds.
+template <int NCH>
+__device__ __forceinline__ void box_rows(int first, int cnt, const double* b, int k, int n2, int i0, int i1, int j0,
+                                         int j1, double (&acc)[NCH]) {
+  int i = first / n2, j = first % n2, m = 0;
+  while (m < cnt) {
+    const int run = min(cnt - m, n2 - j);
+    if (i >= i0 && i <= i1) {
+      const int ja = max(j, j0), jb = min(j + run - 1, j1);
+      const double* row = b + (ptrdiff_t)(m - j) * 8 + k;  // row[jj*8] is zone (i, jj)
+      int jj = ja;
+      for (; jj + 16 <= jb + 1; jj += 16) {  // full groups: no per-value test
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) v[u] = row[(jj + u) * 8];
+#pragma unroll
+        for (int u = 0; u < 16; u++)
+#pragma unroll
+          for (int c = 0; c < NCH; c++) acc[c] += v[u];
+      }
+      for (; jj <= jb; jj++)
+#pragma unroll
+        for (int c = 0; c < NCH; c++) acc[c] += row[jj * 8];
+    }
+    m += run;
+    j += run;
+    if (j == n2) {
+      j = 0;
+      i++;
+    }
   }
 }
 
@@ -148,29 +194,21 @@ __global__ __launch_bounds__(256) void k_acct_srcsum(KParams P, AcctRegions R, c
   __shared__ double lds[2 * STG_CH * 8];
   const int l = threadIdx.x;
   const int r = l / 8, k = l % 8;
-  const int* ep = R.ep[r < ACC_NREG ? r : 0];
-  double s = 0, s0 = 0;
-  if (l < ACC_NREG * 8) {
-    s = acc[r * REG_D + OFF_SRC + k];
-    s0 = acc[r * REG_D + OFF_SRCT + k];
-  }
+  const int rr = r < ACC_NREG ? r : 0;
+  const int i0 = max(R.ep[rr][HARM_X1DN], 0), i1 = min(R.ep[rr][HARM_X1UP], P.n1 - 1);
+  const int j0 = max(R.ep[rr][HARM_X2DN], 0), j1 = min(R.ep[rr][HARM_X2UP], P.n2 - 1);
+  // sourceadd and sourceaddterms[GEOMSOURCE] receive the same terms from the
+  // same start (both zeroed together): one chain serves both
+  double ch[1] = {0};
+  if (l < ACC_NREG * 8) ch[0] = acc[r * REG_D + OFF_SRC + k];
   const int n2 = P.n2;
   staged_items<8>(P.n1 * n2, lds, [&](int z, int w) { return term[(size_t)z * 8 + w]; },
                   [&](int first, int cnt, const double* b) {
-                    if (l >= ACC_NREG * 8) return;
-                    int i = first / n2, j = first % n2;
-                    for (int m = 0; m < cnt; m++) {
-                      if (acct_in(ep, i, j)) {
-                        const double v = b[m * 8 + k];
-                        s0 += v;
-                        s += v;
-                      }
-                      if (++j == n2) { j = 0; i++; }
-                    }
+                    if (l < ACC_NREG * 8) box_rows<1>(first, cnt, b, k, n2, i0, i1, j0, j1, ch);
                   });
   if (l < ACC_NREG * 8) {
-    acc[r * REG_D + OFF_SRC + k] = s;
-    acc[r * REG_D + OFF_SRCT + k] = s0;
+    acc[r * REG_D + OFF_SRC + k] = ch[0];
+    acc[r * REG_D + OFF_SRCT + k] = ch[0];
   }
 }
 
@@ -376,21 +414,27 @@ __global__ void k_divb(KParams P, const double* __restrict__ p, const double* __
 
 __global__ __launch_bounds__(256) void k_divb_reduce(KParams P, const double* __restrict__ divb,
                                                      double* __restrict__ res) {
-  __shared__ double lds[2 * STG_CH];
+  __shared__ double lds[2 * STG_CH];  // 16 KiB
   const int w = P.n2 + 4;
   double mx = 0, sum = 0;
   int imax = 0, jmax = 0;
   staged_items<1>((P.n1 + 4) * w, lds, [&](int it, int) { return divb[it]; },
                   [&](int first, int cnt, const double* b) {
                     if (threadIdx.x != 0) return;
-                    for (int m = 0; m < cnt; m++) {
-                      const double d = b[m];
-                      if (d > mx) {
-                        imax = (first + m) / w - 2;
-                        jmax = (first + m) % w - 2;
-                        mx = d;
+                    for (int m = 0; m < cnt; m += 8) {
+                      double d[8];
+#pragma unroll
+                      for (int u = 0; u < 8; u++) d[u] = (m + u < cnt) ? b[m + u] : 0.0;
+#pragma unroll
+                      for (int u = 0; u < 8; u++) {
+                        if (m + u >= cnt) break;
+                        if (d[u] > mx) {
+                          imax = (first + m + u) / w - 2;
+                          jmax = (first + m + u) % w - 2;
+                          mx = d[u];
+                        }
+                        sum += d[u];
                       }
-                      sum += d;
                     }
                   });
   if (threadIdx.x == 0) {
@@ -427,19 +471,15 @@ __global__ __launch_bounds__(256) void k_cons_sum(KParams P, AcctRegions R, int 
                                                   const double* __restrict__ term, double* __restrict__ res) {
   __shared__ double lds[2 * STG_CH * 8];
   const int k = threadIdx.x;
-  const int* ep = R.ep[region];
+  const int i0 = max(R.ep[region][HARM_X1DN], 0), i1 = min(R.ep[region][HARM_X1UP], P.n1 - 1);
+  const int j0 = max(R.ep[region][HARM_X2DN], 0), j1 = min(R.ep[region][HARM_X2UP], P.n2 - 1);
   const int n2 = P.n2;
-  double s = 0.0;
+  double ch[1] = {0.0};
   staged_items<8>(P.n1 * n2, lds, [&](int z, int w) { return term[(size_t)z * 8 + w]; },
                   [&](int first, int cnt, const double* b) {
-                    if (k >= NPR) return;
-                    int i = first / n2, j = first % n2;
-                    for (int m = 0; m < cnt; m++) {
-                      if (acct_in(ep, i, j)) s += b[m * 8 + k];
-                      if (++j == n2) { j = 0; i++; }
-                    }
+                    if (k < NPR) box_rows<1>(first, cnt, b, k, n2, i0, i1, j0, j1, ch);
                   });
-  if (k < NPR) res[k] = s;
+  if (k < NPR) res[k] = ch[0];
 }
 
 // limit_gamma on the face states of fluxcalc (step_ch.c:978-979): the event
@@ -472,7 +512,7 @@ __global__ void k_acct_faceg(KParams P, const double* __restrict__ pr, const dou
 
 // fixup1zone's events (fixup.c:274-377) from the state before it: limit_gamma
 // (COUNTLIMITGAMMAACT), then the whole change (COUNTFLOORACT, final substage)
-__global__ void k_acct_zoneg(KParams P, const double* __restrict__ gC) {
+__global__ void k_acct_zoneg(KParams P, const double* __restrict__ gC, int finalstep) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= P.n1 || j >= P.n2) return;
@@ -500,7 +540,7 @@ __global__ void k_acct_zoneg(KParams P, const double* __restrict__ gC) {
     acct_event(P, g, p1, p2, slot, 2);
     changed = 1;
   }
-  if (changed) acct_event(P, g, p0, p2, slot + 1, 1);
+  if (changed && finalstep) acct_event(P, g, p0, p2, slot + 1, 1);
 }
 
 // the failed zones' events from the states fixup_zone kept: limit_gamma's
@@ -534,18 +574,35 @@ __global__ __launch_bounds__(256) void k_acct_count(int nslots, const int* __res
   if (threadIdx.x == 0) cnt[blockIdx.x] = c;
 }
 
-__global__ void k_acct_scan(int nblocks, const int* __restrict__ cnt, int* __restrict__ off, int* __restrict__ total) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  int run = 0;
-  for (int b = 0; b < nblocks; b++) {
+__global__ __launch_bounds__(256) void k_acct_scan(int nblocks, const int* __restrict__ cnt, int* __restrict__ off,
+                                                   int* __restrict__ total) {
+  __shared__ int part[256];
+  const int t = threadIdx.x;
+  const int per = (nblocks + 255) / 256, b0 = t * per, b1 = min(b0 + per, nblocks);
+  int sum = 0;
+  for (int b = b0; b < b1; b++) sum += cnt[b];
+  part[t] = sum;
+  __syncthreads();
+  if (t == 0) {
+    int run = 0;
+    for (int u = 0; u < 256; u++) {
+      const int v = part[u];
+      part[u] = run;
+      run += v;
+    }
+    *total = run;
+  }
+  __syncthreads();
+  int run = part[t];
+  for (int b = b0; b < b1; b++) {
     off[b] = run;
     run += cnt[b];
   }
-  *total = run;
 }
 
 __global__ __launch_bounds__(256) void k_acct_compact(int nslots, int* __restrict__ flag, const int* __restrict__ off,
-                                                      int* __restrict__ list) {
+                                                      int* __restrict__ list, const double* __restrict__ val,
+                                                      double* __restrict__ cval) {
   __shared__ int wcount[4];
   const int s = blockIdx.x * 256 + threadIdx.x;
   const int f = (s < nslots) ? flag[s] : 0;
@@ -558,20 +615,22 @@ __global__ __launch_bounds__(256) void k_acct_compact(int nslots, int* __restric
     for (int v = 0; v < w; v++) pos += wcount[v];
     list[2 * pos] = s;
     list[2 * pos + 1] = f - 1;
+#pragma unroll
+    for (int m = 0; m < 8; m++) cval[(size_t)pos * 8 + m] = val[(size_t)s * 8 + m];
     flag[s] = 0;
   }
 }
 
 // the substage's events added in order: lanes 0-23 fladd[r][k], then
 // fladdterms[r][cat][k]; floor and repair events count on the last substage
-__global__ __launch_bounds__(256) void k_acct_merge(KParams P, AcctRegions R, const int* __restrict__ list,
-                                                    const int* __restrict__ total, const double* __restrict__ val,
+__global__ __launch_bounds__(512) void k_acct_merge(KParams P, AcctRegions R, const int* __restrict__ list,
+                                                    const int* __restrict__ total, const double* __restrict__ cval,
                                                     double* __restrict__ acc, int finalstep) {
-  __shared__ double ev_val[256 * 8];
-  __shared__ int ev_i[256], ev_j[256], ev_cat[256];
+  __shared__ double ev_val[2][256 * 8];
+  __shared__ int ev_i[2][256], ev_j[2][256], ev_cat[2][256];
   const int t = threadIdx.x;
   constexpr int NT = ACC_NREG * 8, NTT = ACC_NREG * ACC_NFLOOR * 8;
-  const bool lane = t < NT + NTT;
+  const bool lane = t < NT + NTT;  // waves 0-3 add, waves 4-7 stage the next 256 events
   int r = 0, cat = -1, k = 0;
   if (t < NT) {
     r = t / 8;
@@ -587,42 +646,60 @@ __global__ __launch_bounds__(256) void k_acct_merge(KParams P, AcctRegions R, co
   double* dst = acc + r * REG_D + ((cat < 0) ? OFF_FLADD + k : OFF_FLT + cat * 8 + k);
   double s = lane ? *dst : 0.0;
   const int E = *total;
-  const int* ep = R.ep[r];
-  for (int base = 0; base < E; base += 256) {
-    // stage 256 events: where, which category (-1: not counted on this substage), the values
-    const int e = base + t;
-    if (e < E) {
-      const int slot = list[2 * e];
-      int c = list[2 * e + 1], i, j;
-      if (slot < bB) {
-        const int f = slot >> 1;
-        i = f / (n2 + 2);
-        j = f % (n2 + 2) - 1;
-      } else if (slot < bC) {
-        const int f = (slot - bB) >> 1;
-        i = f / (n2 + 1) - 1;
-        j = f % (n2 + 1);
-      } else {
-        const int rel = slot - ((slot < bD) ? bC : bD);
-        if ((rel & 1) && !finalstep) c = -1;  // COUNTFLOORACT / the repair: accounting substage only
-        const int f = rel >> 1;
-        i = f / n2;
-        j = f % n2;
-      }
-      ev_i[t] = i;
-      ev_j[t] = j;
-      ev_cat[t] = c;
-#pragma unroll
-      for (int m = 0; m < 8; m++) ev_val[t * 8 + m] = val[(size_t)slot * 8 + m];
+  const int nch = (E + 255) / 256;
+  const int x1dn = R.ep[r][HARM_X1DN], x1up = R.ep[r][HARM_X1UP], x2dn = R.ep[r][HARM_X2DN], x2up = R.ep[r][HARM_X2UP];
+  // where an event lies and whether it counts on this substage (-1: it does not)
+  auto stage = [&](int c, int h) {
+    const int u = t - 256, e = c * 256 + u;
+    if (u < 0 || e >= E) return;
+    const int slot = list[2 * e];
+    int cc = list[2 * e + 1], i, j;
+    if (slot < bB) {
+      const int f = slot >> 1;
+      i = f / (n2 + 2);
+      j = f % (n2 + 2) - 1;
+    } else if (slot < bC) {
+      const int f = (slot - bB) >> 1;
+      i = f / (n2 + 1) - 1;
+      j = f % (n2 + 1);
+    } else {
+      const int rel = slot - ((slot < bD) ? bC : bD);
+      if ((rel & 1) && !finalstep) cc = -1;  // COUNTFLOORACT / the repair: accounting substage only
+      const int f = rel >> 1;
+      i = f / n2;
+      j = f % n2;
     }
-    __syncthreads();
-    if (lane) {
-      const int cnt = min(256, E - base);
-      for (int m = 0; m < cnt; m++) {
-        const int c = ev_cat[m];
-        if (c < 0 || (cat >= 0 && c != cat)) continue;
-        if (!acct_in(ep, ev_i[m], ev_j[m])) continue;
-        s += ev_val[m * 8 + k];
+    ev_i[h][u] = i;
+    ev_j[h][u] = j;
+    ev_cat[h][u] = cc;
+#pragma unroll
+    for (int m = 0; m < 8; m++) ev_val[h][u * 8 + m] = cval[(size_t)e * 8 + m];
+  };
+  if (nch > 0) stage(0, 0);
+  __syncthreads();
+  for (int c = 0; c < nch; c++) {
+    const int h = c & 1;
+    if (t >= 256) {
+      if (c + 1 < nch) stage(c + 1, h ^ 1);
+    } else if (lane) {
+      const int cnt = min(256, E - c * 256);
+      for (int m0 = 0; m0 < cnt; m0 += 8) {  // eight events read ahead of the adds
+        int cc[8], ii[8], jj[8];
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const int m = min(m0 + u, 255);
+          cc[u] = (m0 + u < cnt) ? ev_cat[h][m] : -1;
+          ii[u] = ev_i[h][m];
+          jj[u] = ev_j[h][m];
+          v[u] = ev_val[h][m * 8 + k];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+          const bool take = cc[u] >= 0 && (cat < 0 || cc[u] == cat) && ii[u] >= x1dn && ii[u] <= x1up &&
+                            jj[u] >= x2dn && jj[u] <= x2up;
+          if (take) s += v[u];
+        }
       }
     }
     __syncthreads();
@@ -663,6 +740,7 @@ static int acct_alloc(harm_ctx* c) {
   if (e == hipSuccess) e = hipMemset(a->flag, 0, (size_t)a->nslots * sizeof(int));
   if (e == hipSuccess) e = hipMalloc(&a->val, (size_t)a->nslots * 8 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&a->list, (size_t)a->nslots * 2 * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&a->cval, (size_t)a->nslots * 8 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&a->cnt, (size_t)a->nblocks * sizeof(int));
   if (e == hipSuccess) e = hipMalloc(&a->off, (size_t)a->nblocks * sizeof(int));
   if (e == hipSuccess) e = hipMalloc(&a->total, sizeof(int));
@@ -674,8 +752,8 @@ static int acct_alloc(harm_ctx* c) {
 static void acct_free(harm_ctx* c) {
   AcctState* a = c->acct;
   if (!a) return;
-  void* bufs[13] = {a->acc, a->frdot, a->lumvsr, a->scratch, a->out,  a->faces, a->pre,
-                     a->flag, a->val,   a->list,   a->cnt,    a->off, a->total};
+  void* bufs[14] = {a->acc, a->frdot, a->lumvsr, a->scratch, a->out, a->faces, a->pre,
+                     a->flag, a->val,   a->list,   a->cval,  a->cnt, a->off,   a->total};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete a;
@@ -707,9 +785,9 @@ static int acct_faces(harm_ctx* c, int pb) {
 }
 
 // ... zones (after the update) ...
-static int acct_zones(harm_ctx* c) {
+static int acct_zones(harm_ctx* c, int finalstep) {
   const dim3 b(64, 4);
-  hipLaunchKernelGGL(k_acct_zoneg, grid2(c->kp.n2, c->kp.n1, b), b, 0, c->stream, c->kp, c->gC);
+  hipLaunchKernelGGL(k_acct_zoneg, grid2(c->kp.n2, c->kp.n1, b), b, 0, c->stream, c->kp, c->gC, finalstep);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -722,11 +800,12 @@ static int acct_merge(harm_ctx* c, int finalstep) {
     hipLaunchKernelGGL(k_acct_fixg, grid2(c->kp.n2, c->kp.n1, b), b, 0, c->stream, c->kp, c->gC);
   }
   hipLaunchKernelGGL(k_acct_count, dim3(a->nblocks), dim3(256), 0, c->stream, a->nslots, a->flag, a->cnt);
-  hipLaunchKernelGGL(k_acct_scan, dim3(1), dim3(64), 0, c->stream, a->nblocks, a->cnt, a->off, a->total);
-  hipLaunchKernelGGL(k_acct_compact, dim3(a->nblocks), dim3(256), 0, c->stream, a->nslots, a->flag, a->off, a->list);
+  hipLaunchKernelGGL(k_acct_scan, dim3(1), dim3(256), 0, c->stream, a->nblocks, a->cnt, a->off, a->total);
+  hipLaunchKernelGGL(k_acct_compact, dim3(a->nblocks), dim3(256), 0, c->stream, a->nslots, a->flag, a->off, a->list,
+                     a->val, a->cval);
   constexpr int nl = ACC_NREG * 8 * (1 + ACC_NFLOOR);
   static_assert(nl <= 256, "one workgroup holds every accumulator lane");
-  hipLaunchKernelGGL(k_acct_merge, dim3(1), dim3(256), 0, c->stream, c->kp, a->R, a->list, a->total, a->val, a->acc,
+  hipLaunchKernelGGL(k_acct_merge, dim3(1), dim3(512), 0, c->stream, c->kp, a->R, a->list, a->total, a->cval, a->acc,
                      finalstep);
   HIPCHK(hipGetLastError());
   return 0;

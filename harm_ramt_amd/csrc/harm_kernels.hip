@@ -1912,7 +1912,7 @@ static void acct_set_on(harm_ctx* c, bool on);
 static int acct_source(harm_ctx* c, int pi, int pb, int pf);
 static int acct_flux(harm_ctx* c, double Dt);
 static int acct_faces(harm_ctx* c, int pb);
-static int acct_zones(harm_ctx* c);
+static int acct_zones(harm_ctx* c, int finalstep);
 static int acct_merge(harm_ctx* c, int finalstep);
 
 extern "C" const char* harm_last_error(void) { return g_err; }
@@ -2405,7 +2405,7 @@ static int enqueue_step(harm_ctx* c) {
     if (to == nto - 1 && acct && acct_source(c, pii[to], pbb[to], pff[to])) return 1;
     if (acct && acct_faces(c, pbb[to])) return 1;
     if (substage(c, pii[to], pbb[to], pff[to], CUf[to], Cunew[to], to, nto)) return 1;
-    if (acct && acct_zones(c)) return 1;
+    if (acct && acct_zones(c, to == nto - 1)) return 1;
     // step_ch.c:104-124 bound_prim, post_fixup, bound_prim; then :174-183 after the last substage
     int mode = KB_BOUND | KB_FIXUP | KB_BOUND2 | ((to == nto - 1) ? KB_DT : 0);
     if (launch_boundary(c, pff[to], mode)) return 1;
