@@ -214,3 +214,33 @@ def test_gpu_accounting_off_and_explicit_diag_flux():
     c.diag_flux(0.5)
     d = c.acct()
     assert np.count_nonzero(d["pdot"][0]) and same(d["pcum"], d["pdot"] * 0.5)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,eno", [("ffde", 0), ("ks_mono", 0), ("ks", 1), ("mink", 2)])
+def test_gpu_accounting_matches_oracle_other_modes(kind, eno):
+    """force-free, CHECKSOLUTION (b^2 flags, checksolution repairs), DOENO 1
+    and 2: the device accumulators are the oracle's"""
+    n1, n2, nsteps = (32, 32, 8) if kind == "mink" else (32, 64, 8)
+    print(f"[case] {kind} eno={eno}", flush=True)
+    g, geom, p0, prm = problem(kind, n1, n2, eno=eno)
+    ep, df = H.ener_regions(g, 0.4)
+    dt0 = 1e-4 if kind == "mink" else 1e-3
+    o = Oracle(H.params_dict(prm))
+    o.set_geometry(*geom.tables())
+    o.set_array(0, p0)
+    o.bound_prim(0)
+    o.set_time(0.0, dt0, 1e30)
+    o.acct_setup(ep, df, 2 * np.pi)
+    c = H.Harm(prm, geom)
+    c.set_array(H.P, p0)
+    c.bound_prim(H.P)
+    c.set_time(0.0, dt0, 1e30)
+    c.acct_setup(ep, df, 2 * np.pi)
+    for _ in range(nsteps):
+        o.step()
+        c.step_ch()
+    assert same(c.get_array(H.P), o.get_array(0))
+    want, got = o.acct(), c.acct()
+    for k in STEP_KEYS + ["fladd", "fladdterms"]:
+        assert same(got[k], want[k]), k
