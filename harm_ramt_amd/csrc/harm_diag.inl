@@ -187,6 +187,9 @@ __global__ void k_dump(KParams P, const double* __restrict__ p, const double* __
 }  // namespace
 
 static bool cur_on(const harm_ctx* c) { return c->cur && c->cur->on; }
+static void cur_set_on(harm_ctx* c, bool on) {
+  if (c->cur) c->cur->on = on;
+}
 
 static void cur_free(harm_ctx* c) {
   CurState* s = c->cur;

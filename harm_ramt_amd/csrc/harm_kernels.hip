@@ -1904,6 +1904,7 @@ struct harm_ctx {
 
 static void cur_free(harm_ctx* c);
 static bool cur_on(const harm_ctx* c);
+static void cur_set_on(harm_ctx* c, bool on);
 static int cur_step(harm_ctx* c);
 
 static void acct_free(harm_ctx* c);
@@ -2571,9 +2572,12 @@ static int autotune(harm_ctx* c) {
     rc = seterr("autotune: copy");
   const bool prof = c->prof;
   c->prof = false;
-  // the timed replays must not accumulate: accounting is off meanwhile
+  // the timed replays must not accumulate nor advance the current's
+  // history: accounting and the current are off meanwhile
   const bool acct = acct_on(c);
   if (acct) acct_set_on(c, false);
+  const bool cur = cur_on(c);
+  if (cur) cur_set_on(c, false);
   float best = 1e30f;
   int bf = HARM_FLUX_FACE_64x4, bb = HARM_BOUND_KERNELS;
   const int fv[4] = {HARM_FLUX_FACE_64x4, HARM_FLUX_FACE_64x2, HARM_FLUX_PAIR, HARM_FLUX_SHARED};
@@ -2590,6 +2594,7 @@ static int autotune(harm_ctx* c) {
     }
   c->prof = prof;
   if (acct) acct_set_on(c, true);
+  if (cur) cur_set_on(c, true);
   c->flux_variant = bf;
   c->bound_variant = bb;
   hipError_t e = hipMemcpyAsync(c->arr[HARM_P], keep, pbytes, hipMemcpyDeviceToDevice, c->stream);
