@@ -2496,7 +2496,12 @@ extern "C" int harm_pre_fixup(harm_ctx* c, int sel) {
 extern "C" int harm_post_fixup(harm_ctx* c, int sel) {
   if (need_geom(c)) return 1;
   if (sel < 0 || sel > HARM_PK1) return seterr("sel must be HARM_P/PK0/PK1");
-  if (launch_boundary(c, sel, KB_FIXUP)) return 1;
+  // a repair outside a step is not accounted (no merge follows it)
+  const bool acct = acct_on(c);
+  if (acct) acct_set_on(c, false);
+  const int rc = launch_boundary(c, sel, KB_FIXUP);
+  if (acct) acct_set_on(c, true);
+  if (rc) return 1;
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
 }

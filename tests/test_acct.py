@@ -244,3 +244,32 @@ def test_gpu_accounting_matches_oracle_other_modes(kind, eno):
     want, got = o.acct(), c.acct()
     for k in STEP_KEYS + ["fladd", "fladdterms"]:
         assert same(got[k], want[k]), k
+
+
+@pytest.mark.gpu
+def test_gpu_standalone_post_fixup_is_not_accounted():
+    """a repair outside a step (fixup.c:37 called on its own) adds no fixup
+    events to the next step's accumulators"""
+    gd = load_golden(CASES[0])
+    g, geom, p0, prm = golden_problem(gd)
+    p = p0.copy()
+    p[10:14, 20:24, 0] = 1e-20  # below the density floor: the repair changes these zones
+    print("[case] standalone post_fixup with accounting", flush=True)
+    runs = []
+    for setup_first in (True, False):
+        c = H.Harm(prm, geom)
+        c.set_array(H.P, p)
+        c.bound_prim(H.P)
+        c.set_time(0.0, float(gd["dt0"]), 1e30)
+        if setup_first:
+            c.acct_setup(gd["enerpos"], gd["doflux"], 2 * np.pi)
+        c.post_fixup(H.P)
+        if not setup_first:
+            c.acct_setup(gd["enerpos"], gd["doflux"], 2 * np.pi)
+        c.step_ch()
+        runs.append((c.get_array(H.P), c.acct()))
+    (sa, aa), (sb, ab) = runs
+    assert not same(sa, p)  # the repair and the step ran
+    assert same(sa, sb)
+    for k in STEP_KEYS + ["fladd", "fladdterms"]:
+        assert same(aa[k], ab[k]), k
