@@ -176,7 +176,7 @@ __global__ void k_acct_srcterm(KParams P, AcctRegions R, const double* __restric
   geo_from_cent(gc, g);
   double pr[NPR], dU[NPR], ft[NPR], fd[NPR];
   load8(pb, z, pr);
-  source_conn(P, pr, g, gConn + (size_t)z * 64, dU);
+  source_conn(P, pr, g, gConn + z, dU, (size_t)(P.n1 + 2 * NG) * P.N2M);
   const double Dt = ds->dt;
 #pragma unroll
   for (int k = 0; k < NPR; k++) ft[k] = Dt * R.dVF * (0.0 + dU[k]);  // dUcomp[GEOMSOURCE] = 0 e + e dUconn

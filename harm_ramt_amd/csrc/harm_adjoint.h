@@ -413,7 +413,7 @@ HARM_HD int inversion_rev(const KParams& P, const double* p, const GeoU& g, cons
 
 // source_conn: dU[UU+l] = gdet sum_jk T^j_k Gamma^k_{lj} (conn[k][l][j])
 HARM_HD void source_rev(const KParams& P, const double* pr, const GeoU& g, const double* conn, const double* dUb,
-                        double* pb) {
+                        double* pb, size_t cs = 1) {
   State q;
   get_state(pr, g, q);
   StateBar qb;
@@ -422,7 +422,7 @@ HARM_HD void source_rev(const KParams& P, const double* pr, const GeoU& g, const
     double Tb[4];
     for (int kk = 0; kk < 4; kk++) {
       double s = 0.;
-      for (int l = 0; l < 4; l++) s += dUb[UU + l] * conn[(kk * 4 + l) * 4 + jj];
+      for (int l = 0; l < 4; l++) s += dUb[UU + l] * conn[(size_t)((kk * 4 + l) * 4 + jj) * cs];
       Tb[kk] = g.gdet * s;
     }
     mhd_rev(P, pr, jj, q, Tb, qb, pb);

@@ -315,7 +315,7 @@ __global__ __launch_bounds__(256) void k_adj_gather(KParams P, const double* con
     double pr[NPR], ldu[NPR];
     load8(pb, z, pr);
     load8(ldU, z, ldu);
-    source_rev(P, pr, g, gConn + (size_t)z * 64, ldu, l);
+    source_rev(P, pr, g, gConn + z, ldu, l, (size_t)(P.n1 + 2 * NG) * P.N2M);
   }
   store8(lp, z, l);
 }

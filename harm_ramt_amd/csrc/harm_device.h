@@ -253,8 +253,11 @@ HARM_HD void face_flux(const KParams& P, double* p_l, double* p_r, const GeoU& g
 }
 
 // source_conn (phys.c:475-619, WITHGDET): dU[UU+l] = gdet sum_jk T^j_k
-// Gamma^k_{lj} with the connection stored as conn[k][l][j]
-HARM_HD void source_conn(const KParams& P, const double* pr, const GeoU& g, const double* cn, double* dU) {
+// Gamma^k_{lj} with the connection stored as conn[k][l][j]; element e of
+// the zone's 64 at cn[e * cs] (cs = 1: the zone's own row; on the device the
+// table is component-major, cs = the number of zones)
+HARM_HD void source_conn(const KParams& P, const double* pr, const GeoU& g, const double* cn, double* dU,
+                         size_t cs = 1) {
   State qb;
   get_state(pr, g, qb);
   double mhd[4][4];
@@ -267,7 +270,7 @@ HARM_HD void source_conn(const KParams& P, const double* pr, const GeoU& g, cons
 #pragma unroll
     for (int jj = 0; jj < 4; jj++)
 #pragma unroll
-      for (int kk = 0; kk < 4; kk++) acc += mhd[jj][kk] * cn[(kk * 4 + l) * 4 + jj];
+      for (int kk = 0; kk < 4; kk++) acc += mhd[jj][kk] * cn[(size_t)((kk * 4 + l) * 4 + jj) * cs];
     dUc[l] = acc;
   }
   dU[RHO] = g.gdet * 0.;

@@ -982,7 +982,7 @@ __device__ __forceinline__ void update_body(KParams P, RKCoef C, const double* _
     }
     // source(pb): T^j_k Gamma^k_{lj} contraction (phys.c:475-619)
     double dUg[NPR];
-    source_conn(P, pbz, g, gConn + (size_t)z * 64, dUg);
+    source_conn(P, pbz, g, gConn + z, dUg, (size_t)(P.n1 + 2 * NG) * P.N2M);
     double d1[NPR], d2[NPR];
     if constexpr (ENO == 2) {
       eno_cvt_zone<ENO_A2C, 1>(P, eA, i, j, d1);
@@ -2097,7 +2097,12 @@ extern "C" int harm_set_geometry(harm_ctx* c, const double* gcov, const double* 
   }
   HIPCHK(hipMemcpy(c->gF, hF.data(), hF.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->gC, hC.data(), hC.size() * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->gConn, conn, nz * 64 * sizeof(double), hipMemcpyHostToDevice));
+  // the connection component-major ([64][zones]): the update's lanes (adjacent
+  // zones) then read adjacent words of each component
+  std::vector<double> hT((size_t)nz * 64);
+  for (size_t z = 0; z < (size_t)nz; z++)
+    for (int e = 0; e < 64; e++) hT[(size_t)e * nz + z] = conn[z * 64 + e];
+  HIPCHK(hipMemcpy(c->gConn, hT.data(), hT.size() * sizeof(double), hipMemcpyHostToDevice));
   c->geometry_set = true;
   return 0;
 }
