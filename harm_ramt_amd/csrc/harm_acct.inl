@@ -627,7 +627,9 @@ __global__ __launch_bounds__(512) void k_acct_merge(KParams P, AcctRegions R, co
                                                     const int* __restrict__ total, const double* __restrict__ cval,
                                                     double* __restrict__ acc, int finalstep) {
   __shared__ double ev_val[2][256 * 8];
-  __shared__ int ev_i[2][256], ev_j[2][256], ev_cat[2][256];
+  // an event's key: bit r set when region r holds it, category + 1 above the
+  // region bits; 0 when it does not count on this substage
+  __shared__ int ev_key[2][256];
   const int t = threadIdx.x;
   constexpr int NT = ACC_NREG * 8, NTT = ACC_NREG * ACC_NFLOOR * 8;
   const bool lane = t < NT + NTT;  // waves 0-3 add, waves 4-7 stage the next 256 events
@@ -647,7 +649,10 @@ __global__ __launch_bounds__(512) void k_acct_merge(KParams P, AcctRegions R, co
   double s = lane ? *dst : 0.0;
   const int E = *total;
   const int nch = (E + 255) / 256;
-  const int x1dn = R.ep[r][HARM_X1DN], x1up = R.ep[r][HARM_X1UP], x2dn = R.ep[r][HARM_X2DN], x2up = R.ep[r][HARM_X2UP];
+  // this lane takes events whose key has its region bit and, for a category
+  // lane, that category: (key & kmask) == kwant
+  const int kmask = (1 << r) | ((cat < 0) ? 0 : (~0 << ACC_NREG));
+  const int kwant = (1 << r) | ((cat < 0) ? 0 : ((cat + 1) << ACC_NREG));
   // where an event lies and whether it counts on this substage (-1: it does not)
   auto stage = [&](int c, int h) {
     const int u = t - 256, e = c * 256 + u;
@@ -669,9 +674,15 @@ __global__ __launch_bounds__(512) void k_acct_merge(KParams P, AcctRegions R, co
       i = f / n2;
       j = f % n2;
     }
-    ev_i[h][u] = i;
-    ev_j[h][u] = j;
-    ev_cat[h][u] = cc;
+    int key = 0;
+    if (cc >= 0) {
+#pragma unroll
+      for (int q = 0; q < ACC_NREG; q++)
+        if (i >= R.ep[q][HARM_X1DN] && i <= R.ep[q][HARM_X1UP] && j >= R.ep[q][HARM_X2DN] && j <= R.ep[q][HARM_X2UP])
+          key |= 1 << q;
+      if (key) key |= (cc + 1) << ACC_NREG;
+    }
+    ev_key[h][u] = key;
 #pragma unroll
     for (int m = 0; m < 8; m++) ev_val[h][u * 8 + m] = cval[(size_t)e * 8 + m];
   };
@@ -684,20 +695,17 @@ __global__ __launch_bounds__(512) void k_acct_merge(KParams P, AcctRegions R, co
     } else if (lane) {
       const int cnt = min(256, E - c * 256);
       for (int m0 = 0; m0 < cnt; m0 += 8) {  // eight events read ahead of the adds
-        int cc[8], ii[8], jj[8];
+        int kk[8];
         double v[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
           const int m = min(m0 + u, 255);
-          cc[u] = (m0 + u < cnt) ? ev_cat[h][m] : -1;
-          ii[u] = ev_i[h][m];
-          jj[u] = ev_j[h][m];
+          kk[u] = (m0 + u < cnt) ? ev_key[h][m] : 0;
           v[u] = ev_val[h][m * 8 + k];
         }
 #pragma unroll
         for (int u = 0; u < 8; u++) {
-          const bool take = cc[u] >= 0 && (cat < 0 || cc[u] == cat) && ii[u] >= x1dn && ii[u] <= x1up &&
-                            jj[u] >= x2dn && jj[u] <= x2up;
+          const bool take = (kk[u] & kmask) == kwant;
           if (take) s += v[u];
         }
       }
