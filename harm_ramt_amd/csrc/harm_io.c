@@ -76,7 +76,10 @@ static int write_header_text(FILE *f, const harm_restart_header *h) {
   return fprintf(f, "\n") < 0;
 }
 
-static int read_header_text(FILE *f, harm_restart_header *h) {
+/* The header readers return 0, 1 on a short or malformed header, or 2 when
+   the file's grid differs from n1 x n2: then nothing past the fixed fields is
+   read, so lumvsr_tot (n1 doubles) is never written beyond its n1 entries. */
+static int read_header_text(FILE *f, harm_restart_header *h, int n1, int n2) {
   int e, d, k, s, fl, n = 0;
   double dummy;
   n = fscanf(f, HDR_SCAN, &h->totalsize[0], &h->totalsize[1], &h->t, &h->tf, &h->nstep, &h->a, &h->gam, &h->cour,
@@ -90,6 +93,7 @@ static int read_header_text(FILE *f, harm_restart_header *h) {
              &h->binaryoutput, &h->sortedoutput, &h->defcon, &h->SAFE, &h->RHOMIN, &h->UUMIN, &h->RHOMINLIMIT,
              &h->UUMINLIMIT, &h->BSQORHOLIMIT, &h->BSQOULIMIT, &h->GAMMAMAX, &h->GAMMADAMP, &h->GAMMAFAIL);
   if (n != 68) return 1;
+  if (h->totalsize[0] != n1 || h->totalsize[1] != n2) return 2;
   for (e = 0; e < HARM_NUMENERREGIONS; e++)
     for (d = 0; d < HARM_NUMDIRS; d++)
       for (k = 0; k < NPR; k++) n += fscanf(f, "%lf", &h->pcumreg_tot[e][d][k]);
@@ -145,10 +149,12 @@ static int write_header_binary(FILE *f, const harm_restart_header *h) {
   return !ok;
 }
 
-static int read_header_binary(FILE *f, harm_restart_header *h) {
+static int read_header_binary(FILE *f, harm_restart_header *h, int n1, int n2) {
   int ok = 1, k;
   int dc[2];
   HDR_FIELDS(R)
+  if (!ok) return 1;
+  if (h->totalsize[0] != n1 || h->totalsize[1] != n2) return 2;
   R(dc);
   h->defcoord = dc[0];
   HDR_FIELDS2(R)
@@ -185,8 +191,7 @@ int harm_rdump_read_host(const char *path, int binary, harm_restart_header *h, i
   if (!path || !h || !p) return 1;
   FILE *f = fopen(path, binary ? "r" : "rt");
   if (!f) return 3;
-  int rc = binary ? read_header_binary(f, h) : read_header_text(f, h);
-  if (!rc && (h->totalsize[0] != n1 || h->totalsize[1] != n2)) rc = 2;
+  int rc = binary ? read_header_binary(f, h, n1, n2) : read_header_text(f, h, n1, n2);
   const int n2m = n2 + 2 * NG;
   for (int j = 0; j < n2 && !rc; j++)
     for (int i = 0; i < n1 && !rc; i++) {

@@ -161,3 +161,55 @@ def test_gpu_context_writes_the_reference_file(gd, tmp_path, binary):
         assert np.array_equal(got, want)
     else:
         assert np.allclose(got, want, rtol=1e-14, atol=0)
+
+
+_ASAN_DRIVER = r"""
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "harm_amd.h"
+/* writes a 64-wide restart file, then reads it into a 32-wide context's
+   buffers: the reader must refuse it (2) without writing past lumvsr[32] */
+int main(int argc, char **argv) {
+  const int w1 = 64, w2 = 8, n1 = 32, n2 = 8;
+  harm_restart_header h;
+  memset(&h, 0, sizeof(h));
+  h.totalsize[0] = w1; h.totalsize[1] = w2;
+  double *lw = calloc(w1, sizeof(double)), *pw = calloc((size_t)(w1 + 6) * (w2 + 6) * 8, sizeof(double));
+  for (int k = 0; k < w1; k++) lw[k] = k + 1;
+  h.lumvsr_tot = lw;
+  int rc = harm_rdump_write_host(argv[1], atoi(argv[2]), &h, w1, w2, pw);
+  if (rc) return 10 + rc;
+  harm_restart_header r;
+  memset(&r, 0, sizeof(r));
+  double *lum = malloc(n1 * sizeof(double)), *p = calloc((size_t)(n1 + 6) * (n2 + 6) * 8, sizeof(double));
+  r.lumvsr_tot = lum;
+  rc = harm_rdump_read_host(argv[1], atoi(argv[2]), &r, n1, n2, p);
+  printf("rc=%d\n", rc);
+  free(lw); free(pw); free(lum); free(p);
+  return rc == 2 ? 0 : 20 + rc;
+}
+"""
+
+
+@pytest.mark.parametrize("binary", [0, 1])
+def test_wider_file_is_refused_without_overflow(tmp_path, binary):
+    """A restart file from a wider grid is refused (error 2) before any of its
+    lumvsr values are stored: the reader is built with AddressSanitizer and
+    reads a 64-wide file into 32-wide buffers."""
+    import os
+    import shutil
+    import subprocess
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no gcc")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    drv = tmp_path / "drv.c"
+    drv.write_text(_ASAN_DRIVER)
+    exe = tmp_path / "drv"
+    subprocess.check_call([gcc, "-std=c11", "-D_GNU_SOURCE", "-g", "-fsanitize=address", "-fno-omit-frame-pointer",
+                           "-I", os.path.join(root, "include"), str(drv),
+                           os.path.join(root, "harm_ramt_amd", "csrc", "harm_io.c"), "-o", str(exe), "-lm"])
+    r = subprocess.run([str(exe), str(tmp_path / "wide"), str(binary)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "rc=2" in r.stdout and "AddressSanitizer" not in r.stderr
