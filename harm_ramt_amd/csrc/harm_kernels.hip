@@ -554,6 +554,60 @@ __global__ void k_ctapply(KParams P, double* __restrict__ F1, double* __restrict
   }
 }
 
+// fix_flux() (fixup.c:1357-1410) on the fluxes a context holds, for the
+// standalone harm_fix_flux() entry point (the step applies the same rules
+// inside the flux kernels).  One thread per face position (i in [-1,N1],
+// j in [-1,N2]) handles its F1 and its F2 face; no rule reads a value another
+// rule writes, so the reference's sequential order needs no ordering here.
+// The EMF quotients of every face whose B flux a rule changes follow F.
+__global__ void k_fixflux(KParams P, double* __restrict__ F1, double* __restrict__ F2, double* __restrict__ q1,
+                          double* __restrict__ q2, const double* __restrict__ gF) {
+  const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x) - 1;
+  const int i = (int)(blockIdx.y * blockDim.y + threadIdx.y) - 1;
+  if (i > P.n1 || j > P.n2) return;
+  const size_t nz = (size_t)(P.n1 + 2 * NG) * P.N2M;
+  const int z = zidx(P, i, j);
+  // F1[i][-1][B2] = -F1[i][0][B2], F1[i][N2][B2] = -F1[i][N2-1][B2], i in [-1,N1)
+  if (i < P.n1 && ((j == -1 && P.bc[HARM_X2DN] == HARM_POLARAXIS) || (j == P.n2 && P.bc[HARM_X2UP] == HARM_POLARAXIS))) {
+    const int zs = zidx(P, i, j == -1 ? 0 : P.n2 - 1);
+    const double v = -F1[(size_t)zs * 8 + B2];
+    F1[(size_t)z * 8 + B2] = v;
+    q1[z] = v / gF[(size_t)z * GF + 15];
+  }
+  // F2 at the polar faces (i in [-1,N1)) and the equator (i in [0,N1)):
+  // every component but U2 zeroed
+  const bool polar = i < P.n1 && ((j == 0 && P.bc[HARM_X2DN] == HARM_POLARAXIS) ||
+                                  (j == P.n2 && P.bc[HARM_X2UP] == HARM_POLARAXIS));
+  const bool equator = i >= 0 && i < P.n1 && j == (P.n2 - 1) / 2;
+  if (polar || equator) {
+    double* f = F2 + (size_t)z * 8;
+#pragma unroll
+    for (int k = 0; k < NPR; k++)
+      if (k != U2) f[k] = 0.;
+    q2[z] = f[B1] / gF[(nz + z) * GF + 15];
+  }
+  // no mass flux into the grid through an outflow x1 face, j in [-1,N2)
+  if (j < P.n2) {
+    double* f = F1 + (size_t)z * 8;
+    if (i == 0 && P.bc[HARM_X1DN] == HARM_OUTFLOW && f[RHO] > 0) f[RHO] = 0;
+    if (i == P.n1 && P.bc[HARM_X1UP] == HARM_OUTFLOW && f[RHO] < 0) f[RHO] = 0;
+  }
+}
+
+// EMF quotients F1[B2]/gdet(FACE1), F2[B1]/gdet(FACE2) of the fluxes a
+// context holds (step_ch.c:1228-1230), for the standalone harm_flux_ct():
+// x1 faces i in [0,N1], j in [-1,N2]; x2 faces i in [-1,N1], j in [0,N2]
+__global__ void k_qfromF(KParams P, const double* __restrict__ F1, const double* __restrict__ F2,
+                         double* __restrict__ q1, double* __restrict__ q2, const double* __restrict__ gF) {
+  const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x) - 1;
+  const int i = (int)(blockIdx.y * blockDim.y + threadIdx.y) - 1;
+  if (i > P.n1 || j > P.n2) return;
+  const size_t nz = (size_t)(P.n1 + 2 * NG) * P.N2M;
+  const int z = zidx(P, i, j);
+  if (i >= 0) q1[z] = F1[(size_t)z * 8 + B2] / gF[(size_t)z * GF + 15];
+  if (j >= 0) q2[z] = F2[(size_t)z * 8 + B1] / gF[(nz + z) * GF + 15];
+}
+
 struct RKCoef {
   double cuf0, cuf1, cuf2, cn0, cn1, cn2;
   int timeorder, last;
@@ -2225,28 +2279,32 @@ static int prof_end(harm_ctx* c) {
   return 0;
 }
 
-// fluxcalc for dir0..dir0+ndir-1 (both directions in one launch when ndir=2)
-static int launch_flux(harm_ctx* c, int pr_sel, int dir0, int ndir) {
+// fluxcalc for dir0..dir0+ndir-1 (both directions in one launch when ndir=2);
+// fix: apply fix_flux()'s edge rules in the same launch when the context
+// has them on (the step), or never (the standalone fluxcalc)
+static int launch_flux(harm_ctx* c, int pr_sel, int dir0, int ndir, bool fix = true) {
   const int nfj = c->kp.n2 + 2, nfi = c->kp.n1 + 2;  // covers both directions' face ranges
+  KParams kp = c->kp;
+  if (!fix) kp.fixupflux = 0;
   if (prof_begin(c, HARM_K_FLUX)) return 1;
   switch (c->flux_variant) {
     case HARM_FLUX_PAIR: {
       const dim3 b(64, 4);
-      hipLaunchKernelGGL(k_flux, dim3((nfj + 31) / 32, (nfi + 3) / 4, ndir), b, 0, c->stream, c->kp, c->arr[pr_sel],
+      hipLaunchKernelGGL(k_flux, dim3((nfj + 31) / 32, (nfi + 3) / 4, ndir), b, 0, c->stream, kp, c->arr[pr_sel],
                          c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2, c->gF, c->ds, dir0);
       break;
     }
     case HARM_FLUX_SHARED: {
       int gx, gy;
       flux_shared_grid(c->kp.n1, c->kp.n2, gx, gy);
-      hipLaunchKernelGGL(k_flux_shared, dim3(gx, gy, ndir), dim3(64, FSH_ROWS), 0, c->stream, c->kp,
+      hipLaunchKernelGGL(k_flux_shared, dim3(gx, gy, ndir), dim3(64, FSH_ROWS), 0, c->stream, kp,
                          c->arr[pr_sel], c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2, c->gF, c->ds, dir0);
       break;
     }
     default: {
       const dim3 b = (c->flux_variant == HARM_FLUX_FACE_64x2) ? dim3(64, 2) : dim3(64, 4);
       hipLaunchKernelGGL(k_flux_single, dim3((nfj + 63) / 64, (nfi + b.y - 1) / b.y, ndir), b, 0, c->stream,
-                         c->kp, c->arr[pr_sel], c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2, c->gF, c->ds, dir0);
+                         kp, c->arr[pr_sel], c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2, c->gF, c->ds, dir0);
       break;
     }
   }
@@ -2433,7 +2491,8 @@ extern "C" int harm_fluxcalc(harm_ctx* c, int pr_sel, int dir, double* ndt) {
   if (need_geom(c)) return 1;
   if (pr_sel < 0 || pr_sel > HARM_PK1) return seterr("pr_sel must be HARM_P/PK0/PK1");
   if (dir != 1 && dir != 2) return seterr("dir must be 1 or 2");
-  if (launch_flux(c, pr_sel, dir, 1)) return 1;
+  // the fluxes as fluxcalc() leaves them: fix_flux() is a call of its own
+  if (launch_flux(c, pr_sel, dir, 1, false)) return 1;
   unsigned long long bits;
   HIPCHK(hipMemcpyAsync(&bits, &c->ds->ndt_acc[dir - 1], 8, hipMemcpyDeviceToHost, c->stream));
   unsigned long long big;
@@ -2446,16 +2505,25 @@ extern "C" int harm_fluxcalc(harm_ctx* c, int pr_sel, int dir, double* ndt) {
 }
 
 extern "C" int harm_fix_flux(harm_ctx* c) {
-  // the fix_flux() edge rules are applied inside the flux kernel when
-  // params.fixupflux is set; nothing is left to do here
-  return need_geom(c);
+  if (need_geom(c)) return 1;
+  const dim3 b(64, 4);
+  hipLaunchKernelGGL(k_fixflux, grid2(c->kp.n2 + 2, c->kp.n1 + 2, b), b, 0, c->stream, c->kp, c->arr[HARM_F1],
+                     c->arr[HARM_F2], c->q1, c->q2, c->gF);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
 }
 
 extern "C" int harm_flux_ct(harm_ctx* c) {
   if (need_geom(c)) return 1;
+  // FLUXCTHLL: the reference's flux_ct() leaves the fluxes as they are
   if (c->prm.fluxb == HARM_FLUXCTHLL) return 0;
   const dim3 b(64, 4);
   if (prof_begin(c, HARM_K_EMF)) return 1;
+  // the EMFs from the fluxes the context holds (edits through harm_set_array
+  // included), as step_ch.c:1223-1234 reads F1, F2
+  hipLaunchKernelGGL(k_qfromF, grid2(c->kp.n2 + 2, c->kp.n1 + 2, b), b, 0, c->stream, c->kp, c->arr[HARM_F1],
+                     c->arr[HARM_F2], c->q1, c->q2, c->gF);
   hipLaunchKernelGGL(k_emf, grid2(c->kp.n2 + 1, c->kp.n1 + 1, b), b, 0, c->stream, c->kp, c->q1, c->q2, c->gC,
                      c->emf);
   HIPCHK(hipGetLastError());
@@ -2543,9 +2611,12 @@ static int build_graph(harm_ctx* c) {
 
 // Autotuner.  The kernel variants give bit-identical results; each
 // combination is captured as a one-step graph and timed over a few replays
-// from the live state, which is restored afterwards (P and the device
-// scalars: every other array is overwritten within a step before it is read).
-static int time_variant(harm_ctx* c, const double* keep, const DevScalars& save, float* ms) {
+// from the live state, which is restored before every replay pass and
+// afterwards: P, the device scalars and, with checksolution, the persistent
+// gamma table (written only at flagged zones, read at their neighbours).
+// Every other array is overwritten within a step before it is read;
+// accounting and the current are paused while the trials run.
+static int time_variant(harm_ctx* c, const double* keep, const double* keep_gc, const DevScalars& save, float* ms) {
   const size_t pbytes = (size_t)c->nz * 8 * sizeof(double);
   hipGraphExec_t ex;
   if (capture_step(c, &ex)) return 1;
@@ -2556,6 +2627,8 @@ static int time_variant(harm_ctx* c, const double* keep, const DevScalars& save,
   for (int pass = 0; pass < 2 && e == hipSuccess; pass++) {
     e = hipMemcpyAsync(c->arr[HARM_P], keep, pbytes, hipMemcpyDeviceToDevice, c->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(c->ds, &save, sizeof(save), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess && keep_gc)
+      e = hipMemcpyAsync(c->gammacheck, keep_gc, (size_t)c->nz * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
     if (e == hipSuccess && pass == 1) e = hipEventRecord(a, c->stream);
     for (int r = 0; r < (pass ? 4 : 1) && e == hipSuccess; r++) e = hipGraphLaunch(ex, c->stream);
     if (e == hipSuccess && pass == 1) e = hipEventRecord(b, c->stream);
@@ -2576,10 +2649,16 @@ static int autotune(harm_ctx* c) {
   HIPCHK(hipMemcpyAsync(&save, c->ds, sizeof(save), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   double* keep = nullptr;
+  double* keep_gc = nullptr;
   HIPCHK(hipMalloc(&keep, pbytes));
   int rc = 0;
   if (hipMemcpyAsync(keep, c->arr[HARM_P], pbytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
     rc = seterr("autotune: copy");
+  const size_t gbytes = (size_t)c->nz * sizeof(double);
+  if (!rc && c->kp.checksolution &&
+      (hipMalloc(&keep_gc, gbytes) != hipSuccess ||
+       hipMemcpyAsync(keep_gc, c->gammacheck, gbytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess))
+    rc = seterr("autotune: gamma table copy");
   const bool prof = c->prof;
   c->prof = false;
   // the timed replays must not accumulate nor advance the current's
@@ -2599,7 +2678,7 @@ static int autotune(harm_ctx* c) {
       float ms = 0.f;
       c->flux_variant = fv[f];
       c->bound_variant = bv[v];
-      rc = time_variant(c, keep, save, &ms);
+      rc = time_variant(c, keep, keep_gc, save, &ms);
       if (!rc && ms < best) { best = ms; bf = fv[f]; bb = bv[v]; }
     }
   c->prof = prof;
@@ -2609,8 +2688,10 @@ static int autotune(harm_ctx* c) {
   c->bound_variant = bb;
   hipError_t e = hipMemcpyAsync(c->arr[HARM_P], keep, pbytes, hipMemcpyDeviceToDevice, c->stream);
   if (e == hipSuccess) e = hipMemcpyAsync(c->ds, &save, sizeof(save), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess && keep_gc) e = hipMemcpyAsync(c->gammacheck, keep_gc, gbytes, hipMemcpyDeviceToDevice, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(keep);
+  if (keep_gc) (void)hipFree(keep_gc);
   if (rc) return 1;
   if (e != hipSuccess) return seterr("autotune: %s", hipGetErrorString(e));
   c->tuned = true;

@@ -139,9 +139,13 @@ int harm_get_time(harm_ctx *ctx, double *t, double *dt);
 
 /* step_ch.c:741 fluxcalc(stage, pr, F, dir, &ndt): fluxes of array pr_sel into
    F1 (dir=1) or F2 (dir=2) and the Courant limit ndt.  Includes the
-   reference's per-face limit_gamma, vchar, HLL/LAXF. */
+   reference's per-face limit_gamma, vchar, HLL/LAXF.  The fluxes are left as
+   fluxcalc() leaves them: fix_flux() is harm_fix_flux (harm_step and
+   harm_advance apply its rules inside their own flux launch). */
 int harm_fluxcalc(harm_ctx *ctx, int pr_sel, int dir, double *ndt);
-/* fixup.c:1357 fix_flux(pb, F1, F2) on the context fluxes. */
+/* fixup.c:1357 fix_flux(pb, F1, F2) on the fluxes the context holds,
+   whatever FIXUPFLUX says (the reference's caller decides): polar-axis x1
+   ghost-face B2 reflection and x2 polar/equator faces, outflow mass flux. */
 int harm_fix_flux(harm_ctx *ctx);
 /* fixup.c:25 pre_fixup(stage, pv): get_bsqflags (b^2/rho, b^2/u flags
    read by fixup_checksolution) from array sel; a no-op in effect unless
@@ -149,10 +153,15 @@ int harm_fix_flux(harm_ctx *ctx);
 int harm_pre_fixup(harm_ctx *ctx, int sel);
 /* fixup.c:37 post_fixup(stage, pv, finalstep): bound_pflag,
    fixup_checksolution (when checksolution is set) and fixup_utoprim (repair
-   of zones whose inversion failed or that the check failed) on array sel. */
+   of zones whose inversion failed or that the check failed) on array sel.
+   A standalone call is never accounted: the reference books fixup_utoprim's
+   repairs with diag_fixup when finalstep is set (fixup.c:908); with
+   accounting on, only the post_fixup inside harm_step/harm_step_n does. */
 int harm_post_fixup(harm_ctx *ctx, int sel);
-/* step_ch.c:1159 flux_ct(stage, pb, F1, F2): constrained-transport EMFs
-   rewritten into F1/F2. */
+/* step_ch.c:1159 flux_ct(stage, pb, F1, F2): Toth's corner EMFs formed from
+   the F1/F2 the context holds (CORNGDETVERSION 1) and rewritten into their
+   B components.  FLUXCTHLL: the fluxes are left unchanged, as the
+   reference's flux_ct() leaves them. */
 int harm_flux_ct(harm_ctx *ctx);
 /* advance.c:15 advance(stage, pi, ulast, pb, CUf, pf, Cunew, unew, timeorder,
    numtimeorders, &ndt): one RK substage.  pi/pb/pf may alias (advance.c:5-14). */

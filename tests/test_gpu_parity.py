@@ -72,6 +72,14 @@ def test_gpu_step_matches_reference(name):
     assert h.failures()[1] == 0
 
 
+def _same_fluxes(h, o, n1, n2, what):
+    """F1 over fluxcalc's x1 faces (i in [0,N1], j in [-1,N2]) and F2 over its
+    x2 faces (i in [-1,N1], j in [0,N2]), GPU against oracle"""
+    F1g, F2g, F1o, F2o = h.get_array(H.F1), h.get_array(H.F2), o.get_array(3), o.get_array(4)
+    assert same(F1g[NG:NG + n1 + 1, NG - 1:NG + n2 + 1], F1o[NG:NG + n1 + 1, NG - 1:NG + n2 + 1]), what + ": F1"
+    assert same(F2g[NG - 1:NG + n1 + 1, NG:NG + n2 + 1], F2o[NG - 1:NG + n1 + 1, NG:NG + n2 + 1]), what + ": F2"
+
+
 @pytest.mark.parametrize("kind,lim,flux", [("ks", H.PARA, H.HLLFLUX), ("ks", H.MC, H.LAXFFLUX),
                                            ("mink", H.MC, H.HLLFLUX)])
 def test_gpu_entry_points_match_oracle(kind, lim, flux):
@@ -81,15 +89,15 @@ def test_gpu_entry_points_match_oracle(kind, lim, flux):
     h = gpu_ctx(geom, prm, p0, 1e-3)
     o = oracle_ctx(geom, prm, p0, 1e-3)
     assert same(h.get_array(H.P), o.get_array(0)), "bound_prim"
-    # fluxcalc: the GPU applies fix_flux's edge rules inside the flux kernels
+    # fluxcalc: the fluxes before fix_flux, then fix_flux as a call of its own
     nd1, nd2 = h.fluxcalc(H.P, 1), h.fluxcalc(H.P, 2)
     od1, od2 = o.fluxcalc(0, 1), o.fluxcalc(0, 2)
-    if prm.fixupflux:
-        o.fix_flux()
     assert nd1 == od1 and nd2 == od2, "Courant limits"
-    F1g, F2g, F1o, F2o = h.get_array(H.F1), h.get_array(H.F2), o.get_array(3), o.get_array(4)
-    assert same(F1g[NG:NG + n1 + 1, NG - 1:NG + n2 + 1], F1o[NG:NG + n1 + 1, NG - 1:NG + n2 + 1]), "F1"
-    assert same(F2g[NG - 1:NG + n1 + 1, NG:NG + n2 + 1], F2o[NG - 1:NG + n1 + 1, NG:NG + n2 + 1]), "F2"
+    _same_fluxes(h, o, n1, n2, "fluxcalc")
+    if prm.fixupflux:
+        h.fix_flux()
+        o.fix_flux()
+        _same_fluxes(h, o, n1, n2, "fix_flux")
     # flux_ct
     h.flux_ct()
     o.flux_ct()
@@ -108,6 +116,38 @@ def test_gpu_entry_points_match_oracle(kind, lim, flux):
     assert h.advance(H.P, H.PK0, cu, H.PK1, cn, 1, 4) == o.advance(0, 1, cu, 2, cn, 1, 4)
     for sel in (H.PK1, H.ULAST, H.UNEW):
         assert same(h.get_array(sel)[NG:-NG, NG:-NG], o.get_array(sel)[NG:-NG, NG:-NG]), f"advance array {sel}"
+
+
+@pytest.mark.parametrize("kind", ["ks", "mink"])
+def test_gpu_fix_flux_and_flux_ct_read_the_held_fluxes(kind):
+    """fix_flux() and flux_ct() act on the fluxes the context holds: fluxes
+    edited through harm_set_array between fluxcalc and fix_flux, and between
+    fix_flux and flux_ct, give the oracle's results for the same edits."""
+    say(f"[case] fix_flux/flux_ct on edited fluxes, {kind}")
+    n1, n2 = 16, 32
+    g, geom, p0, prm = problem(kind, n1, n2)
+    h = gpu_ctx(geom, prm, p0, 1e-3)
+    o = oracle_ctx(geom, prm, p0, 1e-3)
+    for d in (1, 2):
+        assert h.fluxcalc(H.P, d) == o.fluxcalc(0, d)
+    rng = np.random.default_rng(5)
+    for sel in (H.F1, H.F2):
+        F = o.get_array(sel)
+        F *= 1.0 + 0.25 * rng.standard_normal(F.shape)
+        F[..., 0] = rng.standard_normal(F.shape[:2])   # both signs of the mass flux at the outflow faces
+        h.set_array(sel, F)
+        o.set_array(sel, F)
+    h.fix_flux()
+    o.fix_flux()
+    _same_fluxes(h, o, n1, n2, "fix_flux of edited fluxes")
+    for sel in (H.F1, H.F2):
+        F = o.get_array(sel)
+        F[..., 5:8] *= 1.0 + 0.5 * rng.standard_normal(F[..., 5:8].shape)
+        h.set_array(sel, F)
+        o.set_array(sel, F)
+    h.flux_ct()
+    o.flux_ct()
+    _same_fluxes(h, o, n1, n2, "flux_ct of edited fluxes")
 
 
 def test_gpu_utoprimgen_matches_oracle():
@@ -342,3 +382,30 @@ def test_gpu_checksolution_autotuned_graph():
     assert h.get_tuning()[1] != H.BOUND_BLOCK   # the one-workgroup pass has no fixup_checksolution
     assert same(h.get_array(H.P), o.get_array(0))
     assert h.get_time() == o.get_time()
+
+
+def test_gpu_checksolution_autotuned_region_grows():
+    """The autotuner's trial steps leave the persistent gamma table as they
+    found it.  With a 5e-3 split-monopole field the b^2-flagged region gains
+    zones in the third step (oracle: 342 flagged at the first step, 2 more at
+    the third), so trial steps write gamma values where the first real step
+    must still see the table's earlier contents.  The autotuned run must equal
+    a run forced to the same variants, and the oracle, bit for bit."""
+    say("[case] CHECKSOLUTION, flagged region growing, autotuned vs forced vs oracle, 12 steps")
+    g, geom, p0, prm = problem("ks", 32, 64)
+    p0 = p0.copy()
+    p0[..., 5] += H.init_monopole(geom, 5e-3)[..., 5]
+    prm = H.make_params(32, 64, g.dx1, g.dx2, lim=H.PARA, fluxmethod=H.HLLFLUX, timeorder=4, checksolution=1)
+    a = gpu_ctx(geom, prm, p0, 1e-3)
+    a.step_n(12)
+    fv, bv = a.get_tuning()
+    b = gpu_ctx(geom, prm, p0, 1e-3)
+    b.set_tuning(fv, bv)
+    b.step_n(12)
+    o = oracle_ctx(geom, prm, p0, 1e-3)
+    for _ in range(12):
+        o.step()
+    assert same(a.get_array(H.P), b.get_array(H.P))
+    assert same(a.get_array(H.P), o.get_array(0))
+    assert a.get_time() == o.get_time()
+    assert np.array_equal(a.get_pflag()[3:-3, 3:-3, 0], o.pflag()[3:-3, 3:-3, 0])
