@@ -77,16 +77,30 @@ def setup(n1, n2):
     return g, geom, p0, prm
 
 
-def cpu_baseline(geom, p0, prm, n1, n2, seconds):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
+def _oracle_rate(geom, p0, prm, n1, n2, threads, seconds):
+    """zone-updates/s of the oracle on `threads` OpenMP threads, stepping the
+    torus from t=0 for at least `seconds` (the first step, a first touch of
+    the arrays, untimed)"""
     import harm_ramt_amd as H
     from oracle.oracle import Oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
     o = Oracle(H.params_dict(prm), nthreads=threads)
     o.set_geometry(*geom.tables())
     o.set_array(0, p0)
     o.bound_prim(0)
     o.set_time(0.0, 1e-5, 2000.0)
-    o.step()  # first touch
+    o.step()
     steps = 0
     t0 = time.perf_counter()
     while True:
@@ -95,9 +109,28 @@ def cpu_baseline(geom, p0, prm, n1, n2, seconds):
         el = time.perf_counter() - t0
         if el >= seconds and steps >= 3:
             break
-    return {"value": steps * n1 * n2 / el, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+    return steps * n1 * n2 / el, steps, el
+
+
+def cpu_baseline(geom, p0, prm, n1, n2, seconds):
+    """The oracle on the host: every core the OS reports (os.cpu_count()) and
+    the process's own CPU set are both tried for a few seconds, and the faster
+    thread count is timed for `seconds`; both rates are reported."""
+    ncpu = os.cpu_count() or 1
+    try:
+        nset = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        nset = ncpu
+    trial = {}
+    for th in sorted({ncpu, nset}):
+        trial[th] = _oracle_rate(geom, p0, prm, n1, n2, th, 3.0)[0]
+    best = max(trial, key=trial.get)
+    rate, steps, el = _oracle_rate(geom, p0, prm, n1, n2, best, seconds)
+    tried = ", ".join(f"{k} threads {v:.3g}" for k, v in sorted(trial.items()))
+    return {"value": rate, "unit": "zone-updates/s", "cores": best, "kind": "port", "nproc": ncpu,
+            "affinity_cpus": nset, "cpu_model": cpu_model(),
             "sample": f"{steps} steps of the same {n1}x{n2} torus from t=0 (steps 2..{steps + 1}), "
-                      f"{el:.1f} s on {threads} OpenMP threads"}
+                      f"{el:.1f} s on {best} OpenMP threads (the faster of the 3 s trials: {tried} zone-updates/s)"}
 
 
 def batch_sweep(nruns, steps, warmup, seed=2024):
@@ -160,6 +193,72 @@ def batch_sweep(nruns, steps, warmup, seed=2024):
             "batched_zone_updates_per_s": vb, "separate_zone_updates_per_s": vs, "speedup": vb / vs,
             "batched_ms_per_step": 1e3 * 0.5 * (tb1 + tb2) / steps, "separate_ms_per_step": 1e3 * ts / steps,
             "variants": {"flux": fv, "boundary": bv}, "failed_runs": failed}
+
+
+def ffde_measure(steps, warmup, nspins=16, n=512, a_single=0.9):
+    """Force-free split monopole (EOMFFDE, MKS n x n, Rout 100 M, hslope 0.1,
+    MC, LAXF, midpoint RK2, flux-CT; the reference's force-free settings,
+    initbase.c:151-157, init.bz.c:44-55): zone-updates/s of one run at spin
+    a_single, and of nspins runs with spins evenly spaced in [0.1, 0.99]
+    advanced as one batch against the same contexts stepped one after
+    another (each with its own graph).  Synthetic initial state: B^r from the
+    split monopole, zero drift velocity."""
+    from concurrent.futures import ThreadPoolExecutor
+    import numpy as np
+    import harm_ramt_amd as H
+
+    def build(a):
+        g = H.grid_mks(n, n, a=a, Rout=100.0, hslope=0.1)
+        geom = H.Geometry(g)
+        p0 = H.init_monopole(geom)
+        prm = H.make_params(n, n, g.dx1, g.dx2, lim=H.MC, fluxmethod=H.LAXFFLUX, timeorder=2, eomtype=H.EOMFFDE,
+                            poledeath=0)
+        h = H.Harm(prm, geom)
+        h.set_array(H.P, p0)
+        h.bound_prim(H.P)
+        h.set_time(0.0, 1e-3, 1e30)
+        return h
+
+    h = build(a_single)
+    h.step_n(warmup)
+    h.get_time()
+    t0 = time.perf_counter()
+    h.step_n(steps)
+    h.get_time()
+    t_single = (time.perf_counter() - t0) / steps
+    single = {"spin": a_single, "zone_updates_per_s": n * n / t_single, "ms_per_step": 1e3 * t_single,
+              "tuning": list(h.get_tuning())}
+    h.close()
+    spins = [float(x) for x in np.linspace(0.1, 0.99, nspins)]
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        runs = list(ex.map(build, spins))
+    b = H.Batch(runs)
+    b.step_n(warmup)
+    for r in runs:
+        r.step_n(1)
+    zones = nspins * n * n
+    t0 = time.perf_counter()
+    b.step_n(steps)
+    tb1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for r in runs:
+        r.step_n(steps)
+    ts = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    b.step_n(steps)
+    tb2 = time.perf_counter() - t0
+    failed = sum(r.failures()[1] for r in runs)
+    vb, vs = steps * zones / (0.5 * (tb1 + tb2)), steps * zones / ts
+    out = {"workload": f"force-free split monopole, MKS {n}x{n}, Rout 100, MC+LAXF+RK2, flux-CT",
+           "single": single,
+           "batch": {"runs": nspins, "spins": [round(x, 6) for x in spins], "steps": steps,
+                     "batched_zone_updates_per_s": vb, "separate_zone_updates_per_s": vs, "speedup": vb / vs,
+                     "batched_ms_per_step": 1e3 * 0.5 * (tb1 + tb2) / steps, "separate_ms_per_step": 1e3 * ts / steps,
+                     "variants": list(b.get_tuning()), "failed_runs": failed}}
+    b.close()
+    for r in runs:
+        r.close()
+    return out
 
 
 def adjoint_measure(n=256, nsteps=256, K=16, seed=3):
@@ -325,6 +424,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-acct", action="store_true")
+    ap.add_argument("--no-ffde", action="store_true")
+    ap.add_argument("--ffde-steps", type=int, default=20)
     ap.add_argument("--tuning", default=None,
                     help="force the kernel variants as FLUX,BOUND (harm_set_tuning); default: autotuned")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -406,6 +507,7 @@ def main():
     h.close()
     batched = None if args.no_batch else batch_sweep(args.batch_runs, args.batch_steps, 3)
     adjoint = None if args.no_adjoint else adjoint_measure()
+    ffde = None if args.no_ffde else ffde_measure(args.ffde_steps, 3)
     cpu = None if args.no_cpu else cpu_baseline(geom, p0, prm, n1, n2, args.cpu_seconds)
     out = {
         "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -426,6 +528,7 @@ def main():
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adjoint,
+        "ffde": ffde,
         "accounting": accounting,
         "stages": stages,
         "eager_timed_ms_per_step": 1e3 * el_prof / prof_steps,

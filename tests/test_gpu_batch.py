@@ -111,3 +111,57 @@ def test_batch_rejects_mixed_time_orders():
     a, c = start(geom, p0, prm, 1e-4), start(geom2, p02, prm2, 1e-4)
     with pytest.raises(H.HarmError, match="TIMEORDER"):
         H.Batch([a, c])
+
+
+def test_config4_force_free_spin_batch_512():
+    """The force-free workload at full size: 16 split monopoles on MKS 512x512
+    (Rout 100 M, hslope 0.1) with spins evenly spaced in [0.1, 0.99], MC,
+    LAXF, midpoint RK2 (initbase.c:151-157), advanced as one batch for 3 steps.
+    Every run is bit-identical to its context stepped alone from the same
+    state, and the a=0.99 run to the oracle."""
+    from concurrent.futures import ThreadPoolExecutor
+    n, nsteps, dt0 = 512, 3, 1e-3
+    spins = [float(x) for x in np.linspace(0.1, 0.99, 16)]
+    say(f"[case] force-free batch: 16 spins in [0.1, 0.99], {n}x{n}, MC+LAXF+RK2, {nsteps} steps")
+
+    def build(a):
+        g = H.grid_mks(n, n, a=a, Rout=100.0, hslope=0.1)
+        geom = H.Geometry(g)
+        p0 = H.init_monopole(geom)
+        prm = H.make_params(n, n, g.dx1, g.dx2, lim=H.MC, fluxmethod=H.LAXFFLUX, timeorder=2, eomtype=H.EOMFFDE,
+                            poledeath=0)
+        return geom, p0, prm
+
+    runs, alone, keep = [], [], None
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        for k, (geom, p0, prm) in enumerate(ex.map(build, spins)):
+            h = start(geom, p0, prm, dt0)
+            h.step_n(nsteps)
+            alone.append((h.get_array(H.P), h.get_time(), h.failures()))
+            h.set_array(H.P, p0)
+            h.bound_prim(H.P)
+            h.set_time(0.0, dt0, 1e30)
+            runs.append(h)
+            if k == len(spins) - 1:
+                keep = (geom, p0, prm)
+    say("  each run stepped alone; now the batch")
+    b = H.Batch(runs)
+    b.step_n(nsteps)
+    for k, h in enumerate(runs):
+        P, t, fl = alone[k]
+        assert h.get_time() == t, f"spin {spins[k]}: t, dt"
+        assert same(h.get_array(H.P), P), f"spin {spins[k]}: state"
+        assert h.failures() == fl and fl[1] == 0
+    b.close()
+    geom, p0, prm = keep
+    o = Oracle(H.params_dict(prm))
+    o.set_geometry(*geom.tables())
+    o.set_array(0, p0)
+    o.bound_prim(0)
+    o.set_time(0.0, dt0, 1e30)
+    for _ in range(nsteps):
+        o.step()
+    assert runs[-1].get_time() == o.get_time()
+    assert same(runs[-1].get_array(H.P), o.get_array(0)), "a=0.99 run vs the oracle"
+    for h in runs:
+        h.close()
