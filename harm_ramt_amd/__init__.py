@@ -175,7 +175,7 @@ EXPORTS = [
     "harm_create", "harm_destroy", "harm_last_error", "harm_set_stream", "harm_set_geometry",
     "harm_set_array", "harm_get_array", "harm_array_ptr", "harm_get_pflag", "harm_set_time",
     "harm_get_time", "harm_fluxcalc", "harm_fix_flux", "harm_flux_ct", "harm_advance",
-    "harm_bound_prim", "harm_pre_fixup", "harm_post_fixup", "harm_step", "harm_step_n", "harm_utoprimgen", "harm_get_failures",
+    "harm_bound_prim", "harm_pre_fixup", "harm_post_fixup", "harm_step", "harm_step_n", "harm_utoprimgen", "harm_utoprimgen_noalloc", "harm_get_failures",
     "harm_get_nstroke", "harm_prof_enable", "harm_prof_get", "harm_prof_reset",
     "harm_set_tuning", "harm_get_tuning",
     "harm_batch_create", "harm_batch_destroy", "harm_batch_step_n", "harm_batch_set_tuning",
@@ -226,6 +226,7 @@ def load_library(path=LIB_PATH):
         "harm_step": [vp],
         "harm_step_n": [vp, ctypes.c_int],
         "harm_utoprimgen": [vp, ctypes.c_int, ip, dp, dp, ip],
+        "harm_utoprimgen_noalloc": [vp, ctypes.c_int, ip, dp, dp, ip],
         "harm_get_failures": [vp, ip, ip],
         "harm_get_nstroke": [vp, ctypes.POINTER(ctypes.c_longlong)],
         "harm_prof_enable": [vp, ctypes.c_int],

@@ -1680,6 +1680,19 @@ __global__ void k_utoprim_batch(KParams P, int n, const int* __restrict__ zone, 
   for (int k = 0; k < NPR; k++) pr[(size_t)t * 8 + k] = p[k];
 }
 
+// Staging memory of harm_utoprimgen_noalloc: the zone indices, U, the
+// guesses/solutions and the flags of up to HARM_UTOPRIM_NOALLOC_MAX zones,
+// laid out as [zones int2][U 8 doubles][pr 8 doubles][pflag int] per block.
+// Host code never passes the array's name to a kernel: it resolves the
+// array's device address per device once (hipGetSymbolAddress) and uses that.
+constexpr size_t NOALLOC_ZONE_OFF = 0;
+constexpr size_t NOALLOC_U_OFF = NOALLOC_ZONE_OFF + (size_t)HARM_UTOPRIM_NOALLOC_MAX * 2 * sizeof(int);
+constexpr size_t NOALLOC_PR_OFF = NOALLOC_U_OFF + (size_t)HARM_UTOPRIM_NOALLOC_MAX * 8 * sizeof(double);
+constexpr size_t NOALLOC_FL_OFF = NOALLOC_PR_OFF + (size_t)HARM_UTOPRIM_NOALLOC_MAX * 8 * sizeof(double);
+constexpr size_t NOALLOC_BYTES = NOALLOC_FL_OFF + (size_t)HARM_UTOPRIM_NOALLOC_MAX * sizeof(int);
+static_assert(NOALLOC_U_OFF % 256 == 0 && NOALLOC_PR_OFF % 256 == 0 && NOALLOC_FL_OFF % 256 == 0,
+              "staging blocks 256-byte aligned");
+
 // ---------------------------------------------------------------------------
 // CHECKSOLUTION: get_bsqflags (pre_fixup) and fixup_checksolution (post_fixup)
 // ---------------------------------------------------------------------------
@@ -1908,6 +1921,30 @@ __global__ void kb_fixapply(const RunDesc* __restrict__ runs, int sel) {
 
 }  // namespace
 
+// The staging array of harm_utoprimgen_noalloc (layout above); every GPU the
+// library runs on holds its own copy until the process ends.
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+// its device address on each device, resolved once per device with that
+// device current (the array's name is a host-side stand-in, not an address),
+// and a lock per device that keeps calls sharing one device's copy apart
+constexpr int NOALLOC_MAXDEV = 64;
+static unsigned char* g_scratch_ptr[NOALLOC_MAXDEV];
+static std::mutex g_scratch_mu[NOALLOC_MAXDEV];
+
+static int scratch_resolve(int dev, unsigned char** out) {
+  if (dev < 0 || dev >= NOALLOC_MAXDEV) return seterr("device %d beyond the staging table (%d)", dev, NOALLOC_MAXDEV);
+  std::lock_guard<std::mutex> lk(g_scratch_mu[dev]);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return seterr("hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+    g_scratch_ptr[dev] = (unsigned char*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 // ===========================================================================
 // C-ABI
 // ===========================================================================
@@ -1954,6 +1991,8 @@ struct harm_ctx {
   long long kcount[HARM_K_COUNT];
   AcctState* acct;  // null until accounting or a dump-time reduction is first used
   CurState* cur;    // null until the current is first enabled
+  int device;               // the GPU the context was created on
+  unsigned char* scratch;   // that device's staging array (harm_utoprimgen_noalloc)
 };
 
 static void cur_free(harm_ctx* c);
@@ -2059,8 +2098,13 @@ extern "C" int harm_create(const harm_params* prm, harm_ctx** out) {
   if (e == hipSuccess) e = hipMalloc(&c->gConn, nz * 64 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&c->ds, sizeof(DevScalars));
   if (e == hipSuccess) e = hipStreamCreate(&c->own_stream);
+  if (e == hipSuccess) e = hipGetDevice(&c->device);
   if (e != hipSuccess) {
     seterr("harm_create: %s", hipGetErrorString(e));
+    harm_destroy(c);
+    return 1;
+  }
+  if (scratch_resolve(c->device, &c->scratch)) {
     harm_destroy(c);
     return 1;
   }
@@ -2745,6 +2789,44 @@ extern "C" int harm_utoprimgen(harm_ctx* c, int n, const int* zone, const double
   (void)hipFree(dp);
   (void)hipFree(dfl);
   if (e != hipSuccess) return seterr("harm_utoprimgen: %s", hipGetErrorString(e));
+  return 0;
+}
+
+// harm_utoprimgen with its staging in the device's static array: no
+// allocation, at most HARM_UTOPRIM_NOALLOC_MAX zones.  Every staged value a
+// call reads it has written first, so nothing needs clearing; calls sharing a
+// device are serialised by that device's lock.
+extern "C" int harm_utoprimgen_noalloc(harm_ctx* c, int n, const int* zone, const double* U, double* pr,
+                                       int* pflag) {
+  if (need_geom(c)) return 1;
+  if (n <= 0) return 0;
+  if (!zone || !U || !pr || !pflag) return seterr("null argument");
+  if (n > HARM_UTOPRIM_NOALLOC_MAX)
+    return seterr("harm_utoprimgen_noalloc: %d zones exceed the staging capacity of %d (use harm_utoprimgen)", n,
+                  HARM_UTOPRIM_NOALLOC_MAX);
+  if (!c->scratch) return seterr("harm_utoprimgen_noalloc: no staging memory on device %d", c->device);
+  for (int t = 0; t < n; t++) {
+    int i = zone[2 * t], j = zone[2 * t + 1];
+    if (i < -NG || i >= c->prm.n1 + NG || j < -NG || j >= c->prm.n2 + NG)
+      return seterr("zone %d (%d,%d) out of range", t, i, j);
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu[c->device]);
+  int* dz = (int*)(c->scratch + NOALLOC_ZONE_OFF);
+  double* dU = (double*)(c->scratch + NOALLOC_U_OFF);
+  double* dp = (double*)(c->scratch + NOALLOC_PR_OFF);
+  int* dfl = (int*)(c->scratch + NOALLOC_FL_OFF);
+  hipError_t e = hipMemcpyAsync(dz, zone, (size_t)n * 2 * sizeof(int), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dU, U, (size_t)n * 8 * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dp, pr, (size_t)n * 8 * sizeof(double), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_utoprim_batch, dim3((n + 127) / 128), dim3(128), 0, c->stream, c->kp, n, dz, dU, dp, dfl,
+                       c->gC);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(pr, dp, (size_t)n * 8 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(pflag, dfl, (size_t)n * sizeof(int), hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return seterr("harm_utoprimgen_noalloc: %s", hipGetErrorString(e));
   return 0;
 }
 

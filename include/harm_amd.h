@@ -183,6 +183,17 @@ int harm_step_n(harm_ctx *ctx, int nsteps);
 int harm_utoprimgen(harm_ctx *ctx, int n, const int *zone, const double *U,
                     double *pr, int *pflag);
 
+/* (no reference counterpart) harm_utoprimgen for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving loops,
+   harnesses that forbid hipMalloc): the same arguments and results, staged
+   through a fixed device array of the library instead of per-call
+   allocations.  At most HARM_UTOPRIM_NOALLOC_MAX zones per call (more: error
+   1 with a message).  Each GPU holds one such array (about 2.2 MB) until the
+   process ends; calls on one GPU share it and are serialised by a lock. */
+#define HARM_UTOPRIM_NOALLOC_MAX 16384
+int harm_utoprimgen_noalloc(harm_ctx *ctx, int n, const int *zone, const double *U,
+                            double *pr, int *pflag);
+
 /* Failure reporting: number of zones whose last inversion set pflag, and the
    reference's global `failed` flag (fail.c:13). */
 int harm_get_failures(harm_ctx *ctx, int *nfailzones, int *failed);
