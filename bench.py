@@ -133,6 +133,29 @@ def cpu_baseline(geom, p0, prm, n1, n2, seconds):
                       f"{el:.1f} s on {best} OpenMP threads (the faster of the 3 s trials: {tried} zone-updates/s)"}
 
 
+def batch_draws(nruns, seed=2024):
+    """The batched sweep's runs, drawn from one seed: grids n1 in [64, 512]
+    and n2 in [64, 256] (multiples of 16), spin in [0, 0.99], r_max in
+    [10, 16], beta in [30, 300], and the seed of each run's +-2 % u
+    perturbation."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return [dict(n1=int(rng.integers(4, 33)) * 16, n2=int(rng.integers(4, 17)) * 16, a=float(rng.uniform(0, 0.99)),
+                 rmax=float(rng.uniform(10, 16)), beta=float(rng.uniform(30, 300)), seed=int(rng.integers(1, 2**31)))
+            for _ in range(nruns)]
+
+
+def build_torus(d):
+    """(geometry, initial state, parameters) of one drawn torus run: MKS
+    Rout 40 M, hslope 0.3, rin 6, PPM, HLL, RK4, Courant 0.9"""
+    import harm_ramt_amd as H
+    g = H.grid_mks(d["n1"], d["n2"], a=d["a"], Rout=40.0, hslope=0.3)
+    geom = H.Geometry(g)
+    p0 = H.init_fishmon(geom, rin=6.0, rmax=d["rmax"], beta=d["beta"], randfact=4e-2, seed=d["seed"])
+    prm = H.make_params(d["n1"], d["n2"], g.dx1, g.dx2, lim=H.PARA, fluxmethod=H.HLLFLUX, timeorder=4, cour=0.9)
+    return geom, p0, prm
+
+
 def batch_sweep(nruns, steps, warmup, seed=2024):
     """Batched sweep of independent FM tori (grids between 64x64 and 512x256,
     spin in [0, 0.99], r_max in [10, 16], beta in [30, 300], +-2 % u
@@ -142,19 +165,9 @@ def batch_sweep(nruns, steps, warmup, seed=2024):
     steps; the batched leg is timed before and after the separate leg and
     averaged, so the slow drift of the states does not favour either."""
     from concurrent.futures import ThreadPoolExecutor
-    import numpy as np
     import harm_ramt_amd as H
-    rng = np.random.default_rng(seed)
-    draws = [dict(n1=int(rng.integers(4, 33)) * 16, n2=int(rng.integers(4, 17)) * 16, a=float(rng.uniform(0, 0.99)),
-                  rmax=float(rng.uniform(10, 16)), beta=float(rng.uniform(30, 300)), seed=int(rng.integers(1, 2**31)))
-             for _ in range(nruns)]
-
-    def build(d):
-        g = H.grid_mks(d["n1"], d["n2"], a=d["a"], Rout=40.0, hslope=0.3)
-        geom = H.Geometry(g)
-        p0 = H.init_fishmon(geom, rin=6.0, rmax=d["rmax"], beta=d["beta"], randfact=4e-2, seed=d["seed"])
-        prm = H.make_params(d["n1"], d["n2"], g.dx1, g.dx2, lim=H.PARA, fluxmethod=H.HLLFLUX, timeorder=4, cour=0.9)
-        return geom, p0, prm
+    draws = batch_draws(nruns, seed)
+    build = build_torus
 
     with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
         built = list(ex.map(build, draws))
@@ -259,6 +272,89 @@ def ffde_measure(steps, warmup, nspins=16, n=512, a_single=0.9):
     for r in runs:
         r.close()
     return out
+
+
+def spin_derivative_measure(n=512, nsteps=2000, a0=0.9, r_flux=20.0, h_fd=1e-5):
+    """d L / d a for the force-free split monopole (config of ffde_measure at
+    spin a0): L = the outgoing energy flux through the zone row nearest
+    r_flux after nsteps steps.  The tangent-linear run (harm_tl: the step on
+    value-plus-tangent numbers, dt sequence of the values) starts from the
+    spin tangents of the geometry tables, the grid spacing and the initial
+    state (fourth-order differences of the host generators); checked against
+    the central difference of two plain runs at a0 -+ h_fd with their own
+    grids and initial states and the base run's dt sequence imposed.  Also
+    the cost of a tangent-linear step against a plain (graph) step."""
+    import numpy as np
+    import harm_ramt_amd as H
+
+    def grid(a):
+        return H.grid_mks(n, n, a=a, Rout=100.0, hslope=0.1)
+
+    def ctx(a):
+        g = grid(a)
+        geom = H.Geometry(g)
+        p0 = H.init_monopole(geom)
+        prm = H.make_params(n, n, g.dx1, g.dx2, lim=H.MC, fluxmethod=H.LAXFFLUX, timeorder=2, eomtype=H.EOMFFDE,
+                            poledeath=0)
+        h = H.Harm(prm, geom)
+        h.set_array(H.P, p0)
+        h.bound_prim(H.P)
+        h.set_time(0.0, 1e-3, 1e30)
+        h.set_tuning(H.FLUX_FACE_64x2, H.BOUND_FUSED)
+        return g, p0, h
+
+    g, p0, base = ctx(a0)
+    row = int(np.argmin([abs(H.zone_coords(g, i, n // 2)[2] - r_flux) for i in range(n)]))
+    t0 = time.perf_counter()
+    dgcov, dgcon, dgdet, dconn, dp0, ddx1, ddx2 = H.spin_tangents(grid, H.init_monopole, a0)
+    t_tables = time.perf_counter() - t0
+    tl = H.Tangent(base)
+    tl.set_geometry_tangent(dgcov, dgcon, dgdet, dconn)
+    tl.set_spacing_tangent(ddx1, ddx2)
+    tl.set_state_tangent(dp0)
+    del dgcov, dgcon, dgdet, dconn
+    dts = []
+    t0 = time.perf_counter()
+    for _ in range(nsteps):
+        dts.append(base.get_time()[1])
+        tl.step_n(1)
+    t_tl_run = time.perf_counter() - t0
+    L, dL = tl.energy_flux(row)
+
+    def fd_run(a):
+        _, _, h = ctx(a)
+        tt = 0.0
+        for dt in dts:
+            h.set_time(tt, dt, 1e30)
+            h.step_ch()
+            tt = h.get_time()[0]
+        tz = H.Tangent(h)
+        v = tz.energy_flux(row)[0]
+        tz.close()
+        h.close()
+        return v
+
+    fd = (fd_run(a0 + h_fd) - fd_run(a0 - h_fd)) / (2 * h_fd)
+    # steady-state cost: 50 tangent-linear steps against 50 plain graph steps
+    # continuing the same run
+    k = 50
+    base.step_n(2)
+    base.get_time()
+    t0 = time.perf_counter()
+    base.step_n(k)
+    base.get_time()
+    t_plain = (time.perf_counter() - t0) / k
+    t0 = time.perf_counter()
+    tl.step_n(k)
+    t_tl = (time.perf_counter() - t0) / k
+    tl.close()
+    base.close()
+    return {"workload": f"force-free split monopole a={a0}, MKS {n}x{n}, MC+LAXF+RK2, {nsteps} steps",
+            "functional": f"outgoing energy flux through zone row {row} (r ~ {r_flux} M) after the last step",
+            "L": L, "dL_da_tangent_linear": dL, "dL_da_central_difference": fd, "fd_h": h_fd,
+            "rel_diff": abs(dL - fd) / abs(fd) if fd != 0 else None,
+            "tangent_step_ms": 1e3 * t_tl, "plain_step_ms": 1e3 * t_plain, "tangent_cost_vs_plain_step": t_tl / t_plain,
+            "tangent_run_s": t_tl_run, "tangent_tables_s": t_tables}
 
 
 def adjoint_measure(n=256, nsteps=256, K=16, seed=3):
@@ -426,6 +522,8 @@ def main():
     ap.add_argument("--no-acct", action="store_true")
     ap.add_argument("--no-ffde", action="store_true")
     ap.add_argument("--ffde-steps", type=int, default=20)
+    ap.add_argument("--no-spin", action="store_true")
+    ap.add_argument("--spin-steps", type=int, default=2000)
     ap.add_argument("--tuning", default=None,
                     help="force the kernel variants as FLUX,BOUND (harm_set_tuning); default: autotuned")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -508,6 +606,8 @@ def main():
     batched = None if args.no_batch else batch_sweep(args.batch_runs, args.batch_steps, 3)
     adjoint = None if args.no_adjoint else adjoint_measure()
     ffde = None if args.no_ffde else ffde_measure(args.ffde_steps, 3)
+    if ffde is not None and not args.no_spin:
+        ffde["spin_derivative"] = spin_derivative_measure(nsteps=args.spin_steps)
     cpu = None if args.no_cpu else cpu_baseline(geom, p0, prm, n1, n2, args.cpu_seconds)
     out = {
         "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": args.steps,

@@ -176,6 +176,9 @@ EXPORTS = [
     "harm_set_array", "harm_get_array", "harm_array_ptr", "harm_get_pflag", "harm_set_time",
     "harm_get_time", "harm_fluxcalc", "harm_fix_flux", "harm_flux_ct", "harm_advance",
     "harm_bound_prim", "harm_pre_fixup", "harm_post_fixup", "harm_step", "harm_step_n", "harm_utoprimgen", "harm_utoprimgen_noalloc", "harm_get_failures",
+    "harm_tl_create", "harm_tl_destroy", "harm_tl_set_geometry_tangent", "harm_tl_set_spacing_tangent",
+    "harm_tl_set_state_tangent",
+    "harm_tl_get_state_tangent", "harm_tl_step_n", "harm_tl_energy_flux",
     "harm_get_nstroke", "harm_prof_enable", "harm_prof_get", "harm_prof_reset",
     "harm_set_tuning", "harm_get_tuning",
     "harm_batch_create", "harm_batch_destroy", "harm_batch_step_n", "harm_batch_set_tuning",
@@ -227,6 +230,14 @@ def load_library(path=LIB_PATH):
         "harm_step_n": [vp, ctypes.c_int],
         "harm_utoprimgen": [vp, ctypes.c_int, ip, dp, dp, ip],
         "harm_utoprimgen_noalloc": [vp, ctypes.c_int, ip, dp, dp, ip],
+        "harm_tl_create": [vp, ctypes.POINTER(vp)],
+        "harm_tl_destroy": [vp],
+        "harm_tl_set_geometry_tangent": [vp, dp, dp, dp, dp],
+        "harm_tl_set_spacing_tangent": [vp, ctypes.c_double, ctypes.c_double],
+        "harm_tl_set_state_tangent": [vp, dp],
+        "harm_tl_get_state_tangent": [vp, dp],
+        "harm_tl_step_n": [vp, ctypes.c_int],
+        "harm_tl_energy_flux": [vp, ctypes.c_int, dp, dp],
         "harm_get_failures": [vp, ip, ip],
         "harm_get_nstroke": [vp, ctypes.POINTER(ctypes.c_longlong)],
         "harm_prof_enable": [vp, ctypes.c_int],
@@ -634,6 +645,70 @@ class Batch:
         a, b = ctypes.c_int(), ctypes.c_int()
         _chk(self.lib.harm_batch_get_tuning(self.h, ctypes.byref(a), ctypes.byref(b)), "harm_batch_get_tuning")
         return a.value, b.value
+
+
+class Tangent:
+    """Tangent-linear run of a Harm context (harm_tl): the context's state
+    and its derivative with respect to one parameter advance together."""
+
+    def __init__(self, harm: "Harm"):
+        self.lib = load_library()
+        self.harm = harm
+        h = ctypes.c_void_p()
+        _chk(self.lib.harm_tl_create(harm.h, ctypes.byref(h)), "harm_tl_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.harm_tl_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_geometry_tangent(self, dgcov, dgcon, dgdet, dconn):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (dgcov, dgcon, dgdet, dconn)]
+        _chk(self.lib.harm_tl_set_geometry_tangent(self.h, *[_dptr(x) for x in a]), "harm_tl_set_geometry_tangent")
+
+    def set_spacing_tangent(self, ddx1, ddx2=0.0):
+        _chk(self.lib.harm_tl_set_spacing_tangent(self.h, ddx1, ddx2), "harm_tl_set_spacing_tangent")
+
+    def set_state_tangent(self, dp):
+        d = np.ascontiguousarray(dp, dtype=np.float64)
+        assert d.shape == self.harm.shape
+        _chk(self.lib.harm_tl_set_state_tangent(self.h, _dptr(d)), "harm_tl_set_state_tangent")
+
+    def state_tangent(self):
+        d = np.zeros(self.harm.shape)
+        _chk(self.lib.harm_tl_get_state_tangent(self.h, _dptr(d)), "harm_tl_get_state_tangent")
+        return d
+
+    def step_n(self, nsteps):
+        _chk(self.lib.harm_tl_step_n(self.h, nsteps), "harm_tl_step_n")
+
+    def energy_flux(self, i):
+        L, dL = ctypes.c_double(), ctypes.c_double()
+        _chk(self.lib.harm_tl_energy_flux(self.h, i, ctypes.byref(L), ctypes.byref(dL)), "harm_tl_energy_flux")
+        return L.value, dL.value
+
+
+def spin_tangents(grid_of_spin, init_of_geometry, a, h=1e-4):
+    """d/da of the geometry tables and of the initial state at spin a, by the
+    fourth-order central difference of the host generators,
+    (8 (T(a+h) - T(a-h)) - (T(a+2h) - T(a-2h))) / (12 h): the generators are
+    smooth in a (the horizon zone, which fixes Rin's formula, does not depend
+    on a), so the error is O(h^4) plus rounding of order 1e-16 / h.
+    grid_of_spin(a) -> HarmGrid, init_of_geometry(Geometry) -> state.
+    Returns (dgcov, dgcon, dgdet, dconn, dp0, ddx1, ddx2): the grid spacing
+    dx1 moves with a through Rin."""
+    def tabs(x):
+        g = Geometry(grid_of_spin(x))
+        return g.gcov, g.gcon, g.gdet, g.conn, init_of_geometry(g), g.grid.dx1, g.grid.dx2
+    tp, tm, t2p, t2m = tabs(a + h), tabs(a - h), tabs(a + 2 * h), tabs(a - 2 * h)
+    return tuple((8.0 * (p - m) - (p2 - m2)) / (12.0 * h) for p, m, p2, m2 in zip(tp, tm, t2p, t2m))
 
 
 class Adjoint:

@@ -41,32 +41,44 @@ struct KParams {
   double* acct_val;
 };
 
-struct GeoU {
-  double g00, g01, g02, g03, g11, g12, g13, g22, g23, g33;  // gcov upper
-  double n00, n01, n02, n03;                                // gcon^{0 mu}
-  double ndd;                                               // gcon^{dir dir}
-  double gdet;
+// The physics below is written once for a scalar type T: double for the
+// step itself, and a value-plus-tangent type for the tangent-linear step
+// (harm_tangent.h), whose values are the double step's to the bit.
+template <class T>
+struct GeoT {
+  T g00, g01, g02, g03, g11, g12, g13, g22, g23, g33;  // gcov upper
+  T n00, n01, n02, n03;                                // gcon^{0 mu}
+  T ndd;                                               // gcon^{dir dir}
+  T gdet;
 };
+using GeoU = GeoT<double>;
 
-struct State {
-  double ucon[4], ucov[4], bcon[4], bcov[4];
+template <class T>
+struct StateT {
+  T ucon[4], ucov[4], bcon[4], bcov[4];
 };
+using State = StateT<double>;
 
-HARM_HD double dmax(double a, double b) { return (a > b) ? a : b; }  // phys.c:1580
-HARM_HD double dmin(double a, double b) { return (a > b) ? b : a; }  // phys.c:1587
-HARM_HD double dsign(double a) {                                      // phys.c:1562
-  return (a > 0.0) ? 1.0 : ((a < 0.0) ? -1.0 : 0.0);
+template <class T>
+HARM_HD T dmax(T a, T b) { return (a > b) ? a : b; }  // phys.c:1580
+template <class T>
+HARM_HD T dmin(T a, T b) { return (a > b) ? b : a; }  // phys.c:1587
+template <class T>
+HARM_HD T dsign(T a) {                                // phys.c:1562
+  return (a > 0.0) ? T(1.0) : ((a < 0.0) ? T(-1.0) : T(0.0));
 }
 
 // phys.c:848-877 gamma_calc
-HARM_HD double gamma_calc(const double* pr, const GeoU& g) {
-  double qsq = g.g11 * pr[U1] * pr[U1] + g.g22 * pr[U2] * pr[U2] + g.g33 * pr[U3] * pr[U3] +
+template <class T>
+HARM_HD T gamma_calc(const T* pr, const GeoT<T>& g) {
+  T qsq = g.g11 * pr[U1] * pr[U1] + g.g22 * pr[U2] * pr[U2] + g.g33 * pr[U3] * pr[U3] +
                2. * (g.g12 * pr[U1] * pr[U2] + g.g13 * pr[U1] * pr[U3] + g.g23 * pr[U2] * pr[U3]);
   return sqrt(1. + qsq);
 }
 
 // phys.c:633-652 lower
-HARM_HD void lower(const double* v, const GeoU& g, double* w) {
+template <class T>
+HARM_HD void lower(const T* v, const GeoT<T>& g, T* w) {
   w[0] = g.g00 * v[0] + g.g01 * v[1] + g.g02 * v[2] + g.g03 * v[3];
   w[1] = g.g01 * v[0] + g.g11 * v[1] + g.g12 * v[2] + g.g13 * v[3];
   w[2] = g.g02 * v[0] + g.g12 * v[1] + g.g22 * v[2] + g.g23 * v[3];
@@ -74,12 +86,13 @@ HARM_HD void lower(const double* v, const GeoU& g, double* w) {
 }
 
 // phys.c:716-727 get_state with ucon_calc_rel4vel :830-845 and bcon_calc :285-294
-HARM_HD void get_state(const double* pr, const GeoU& g, State& q) {
-  double alpha = 1. / sqrt(-g.n00);
-  double beta1 = g.n01 * alpha * alpha;
-  double beta2 = g.n02 * alpha * alpha;
-  double beta3 = g.n03 * alpha * alpha;
-  double gamma = gamma_calc(pr, g);
+template <class T>
+HARM_HD void get_state(const T* pr, const GeoT<T>& g, StateT<T>& q) {
+  T alpha = 1. / sqrt(-g.n00);
+  T beta1 = g.n01 * alpha * alpha;
+  T beta2 = g.n02 * alpha * alpha;
+  T beta3 = g.n03 * alpha * alpha;
+  T gamma = gamma_calc(pr, g);
   q.ucon[0] = gamma / alpha;
   q.ucon[1] = pr[U1] - gamma * beta1 / alpha;
   q.ucon[2] = pr[U2] - gamma * beta2 / alpha;
@@ -92,14 +105,15 @@ HARM_HD void get_state(const double* pr, const GeoU& g, State& q) {
   lower(q.bcon, g, q.bcov);
 }
 
-HARM_HD double dot4(const double* a, const double* b) {
+template <class T>
+HARM_HD T dot4(const T* a, const T* b) {
   return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] + a[3] * b[3];
 }
 
 // phys.c:381-406 mhd_calc_0: T^dir_j, j=0..3 into mhd[0..3]
-HARM_HD void mhd_calc(const KParams& P, const double* pr, int dir, const State& q,
-                                         double* mhd) {
-  double r, u, Pg, w;
+template <class T>
+HARM_HD void mhd_calc(const KParams& P, const T* pr, int dir, const StateT<T>& q, T* mhd) {
+  T r, u, Pg, w;
   if (P.eomtype == 0) {
     r = pr[RHO];
     u = pr[UU];
@@ -109,24 +123,24 @@ HARM_HD void mhd_calc(const KParams& P, const double* pr, int dir, const State& 
     r = u = Pg = w = 0.0;
   }
   (void)r;
-  double bsq = dot4(q.bcon, q.bcov);
-  double eta = w + bsq;
-  double ptot = Pg + bsq * 0.5;
-  double ed = eta * q.ucon[dir];
-  double bd = q.bcon[dir];
+  T bsq = dot4(q.bcon, q.bcov);
+  T eta = w + bsq;
+  T ptot = Pg + bsq * 0.5;
+  T ed = eta * q.ucon[dir];
+  T bd = q.bcon[dir];
 #pragma unroll
   for (int j = 0; j < 4; j++) mhd[j] = ed * q.ucov[j] + ptot * ((dir == j) ? 1. : 0.) - bd * q.bcov[j];
 }
 
 // phys.c:16-56 primtoflux (massflux :59, PRIMMAXWELL dual Faraday :155-185,
 // UtoU UNOTHING->UEVOLVE :219-265 with e[k]=gdet)
-HARM_HD void primtoflux(const KParams& P, const double* pr, const State& q, int dir,
-                                           double gdet, double* F) {
-  double fin[NPR];
+template <class T>
+HARM_HD void primtoflux(const KParams& P, const T* pr, const StateT<T>& q, int dir, T gdet, T* F) {
+  T fin[NPR];
   fin[RHO] = pr[RHO] * q.ucon[dir];
   mhd_calc(P, pr, dir, q, &fin[UU]);
   if (dir > 0) {
-    double Bd = pr[B1 + dir - 1];
+    T Bd = pr[B1 + dir - 1];
     fin[B1] = (pr[B1] * q.ucon[dir] - Bd * q.ucon[1]) / q.ucon[0];
     fin[B2] = (pr[B2] * q.ucon[dir] - Bd * q.ucon[2]) / q.ucon[0];
     fin[B3] = (pr[B3] * q.ucon[dir] - Bd * q.ucon[3]) / q.ucon[0];
@@ -141,69 +155,69 @@ HARM_HD void primtoflux(const KParams& P, const double* pr, const State& q, int 
 }
 
 // vchar.c:37-135 vchar + :166-234 simplefast; returns 1 on the reference's
-// fail() conditions (COEFF_NEG, COEFF_SUP, VCHAR_DISCR)
-HARM_HD int vchar(const KParams& P, const double* pr, const State& q, int dir,
-                                     const GeoU& g, double& vmax, double& vmin) {
+// fail() conditions (COEFF_NEG, COEFF_SUP, VCHAR_DISCR).  Written as
+// selects between values computed on every path (the same operations as the
+// reference's branches, so the same bits), which keeps the FP64 code of the
+// calling kernels in straight runs the scheduler can interleave.
+template <class T>
+HARM_HD int vchar(const KParams& P, const T* pr, const StateT<T>& q, int dir, const GeoT<T>& g, T& vmax, T& vmin) {
   double gam = P.gam;
-  double bsq = dot4(q.bcon, q.bcov);
-  double rho = pr[RHO], u = pr[UU];
-  double WW = rho + gam * u, cs2, EF, va2, cms2;
-  int fail = 0;
-  if (WW > 0.0) cs2 = gam * (gam - 1.) * u / WW;
-  else WW = cs2 = 0.0;
-  EF = bsq + WW;
-  if (EF > 0.0) va2 = bsq / EF;
-  else EF = va2 = 0.0;
-  cms2 = cs2 + va2 - cs2 * va2;
-  if (cms2 < 0.0) {
-    if (cms2 / (cs2 + va2) > -NUMEPSILON * 100.0) cms2 = 0.0;
-    else fail = 1;
-  }
-  if (cms2 > 1.) {
-    if (cms2 < 1.0 + NUMEPSILON * 100.0) cms2 = 1.0;
-    else fail = 1;
-  }
+  T bsq = dot4(q.bcon, q.bcov);
+  T rho = pr[RHO], u = pr[UU];
+  T WW0 = rho + gam * u;
+  const bool wpos = WW0 > 0.0;
+  const T cs2w = gam * (gam - 1.) * u / WW0;
+  const T WW = wpos ? WW0 : T(0.0), cs2 = wpos ? cs2w : T(0.0);
+  T EF0 = bsq + WW;
+  const bool epos = EF0 > 0.0;
+  const T va2e = bsq / EF0;
+  const T va2 = epos ? va2e : T(0.0);
+  T cms2 = cs2 + va2 - cs2 * va2;
+  const T rneg = cms2 / (cs2 + va2);
+  const bool neg = cms2 < 0.0, negok = rneg > -NUMEPSILON * 100.0;
+  const bool sup = cms2 > 1., supok = cms2 < 1.0 + NUMEPSILON * 100.0;
+  int fail = (neg && !negok) || (!neg && sup && !supok);
+  cms2 = (neg && negok) ? T(0.0) : cms2;
+  cms2 = (!neg && sup && supok) ? T(1.0) : cms2;
   // simplefast: Acov=e_dir, Bcov=e_t raised with gcon (raise() keeps only the
   // selected gcon entries: the remaining terms are x*0)
-  double Asq = g.ndd, Bsq = g.n00, AB = (dir == 1) ? g.n01 : g.n02;
-  double Au = q.ucon[dir], Bu = q.ucon[0];
-  double Au2 = Au * Au, Bu2 = Bu * Bu, AuBu = Au * Bu;
-  double A = Bu2 - (Bsq + Bu2) * cms2;
-  double B = 2. * (AuBu - (AB + AuBu) * cms2);
-  double C = Au2 - (Asq + Au2) * cms2;
-  double discr = B * B - 4. * A * C;
-  if ((discr < 0.0) && (discr > -1E-10)) discr = 0.0;
-  else if (discr < -1E-10) {
-    fail = 1;
-    discr = 0.;
-  }
+  T Asq = g.ndd, Bsq = g.n00, AB = (dir == 1) ? g.n01 : g.n02;
+  T Au = q.ucon[dir], Bu = q.ucon[0];
+  T Au2 = Au * Au, Bu2 = Bu * Bu, AuBu = Au * Bu;
+  T A = Bu2 - (Bsq + Bu2) * cms2;
+  T B = 2. * (AuBu - (AB + AuBu) * cms2);
+  T C = Au2 - (Asq + Au2) * cms2;
+  T discr = B * B - 4. * A * C;
+  // (-1e-10, 0): zeroed; below -1e-10: fail and zeroed; -1e-10 itself kept
+  const bool dlow = discr < -1E-10, dsmall = (discr < 0.0) && (discr > -1E-10);
+  fail |= dlow;
+  discr = (dsmall || dlow) ? T(0.0) : discr;
   discr = sqrt(discr);
-  double vm = -(-B + discr) / (2. * A);
-  double vp = -(-B - discr) / (2. * A);
-  if (vp < vm) {
-    vmax = vm;
-    vmin = vp;
-  } else {
-    vmax = vp;
-    vmin = vm;
-  }
+  T vm = -(-B + discr) / (2. * A);
+  T vp = -(-B - discr) / (2. * A);
+  const bool sw = vp < vm;
+  vmax = sw ? vm : vp;
+  vmin = sw ? vp : vm;
   return fail;
 }
 
-// fixup.c:1298-1354 limit_gamma: -1 changed, 1 failure, 0 untouched
-HARM_HD int limit_gamma(double gammamax, double* pr, const GeoU& g) {
+// fixup.c:1298-1354 limit_gamma: -1 changed, 1 failure, 0 untouched (as
+// selects: the factor is formed on every path, applied where the reference
+// applies it)
+template <class T>
+HARM_HD int limit_gamma(double gammamax, T* pr, const GeoT<T>& g) {
   if (gammamax <= 1.0) return 0;
-  double gamma = gamma_calc(pr, g);
-  if ((gamma > gammamax && (gamma != 1.0))) {
-    double pref = (gammamax * gammamax - 1.) / (gamma * gamma - 1.);
-    if (pref < 0.0) return 1;
-    double f = sqrt(pref);
-    pr[U1] *= f;
-    pr[U2] *= f;
-    pr[U3] *= f;
-    return -1;
-  }
-  return 0;
+  T gamma = gamma_calc(pr, g);
+  const bool over = (gamma > gammamax && (gamma != 1.0));
+  T pref = (gammamax * gammamax - 1.) / (gamma * gamma - 1.);
+  const bool bad = pref < 0.0;
+  T f = sqrt(pref);
+  const bool apply = over && !bad;
+  const T u1 = pr[U1] * f, u2 = pr[U2] * f, u3 = pr[U3] * f;
+  pr[U1] = apply ? u1 : pr[U1];
+  pr[U2] = apply ? u2 : pr[U2];
+  pr[U3] = apply ? u3 : pr[U3];
+  return over ? (bad ? 1 : -1) : 0;
 }
 
 // The face flux of fluxcalc() (step_ch.c:930-1065) from the reconstructed
@@ -211,66 +225,64 @@ HARM_HD int limit_gamma(double gammamax, double* pr, const GeoU& g) {
 // both states (H6), get_state, F and U of both states, vchar, cminmax_calc
 // (:1095-1148) and HLL/LAXF (:1047-1054); dtij = cour dx / max(cmax, cmin)
 // (:1062-1065).  p_l, p_r are modified (averaging, limit_gamma).
-template <int DIR>
-HARM_HD void face_flux(const KParams& P, double* p_l, double* p_r, const GeoU& g, double* Fo, double& dtij,
-                       int& fail) {
+template <int DIR, class T>
+HARM_HD void face_flux(const KParams& P, T* p_l, T* p_r, const GeoT<T>& g, T* Fo, T& dtij, int& fail) {
   {
     constexpr int kb = B1 + DIR - 1;
-    double bavg = 0.5 * (p_l[kb] + p_r[kb]);
+    T bavg = 0.5 * (p_l[kb] + p_r[kb]);
     p_l[kb] = bavg;
     p_r[kb] = bavg;
   }
   if (limit_gamma(P.gammamax, p_l, g) >= 1) fail = 1;
   if (limit_gamma(P.gammamax, p_r, g) >= 1) fail = 1;
-  State sl, sr;
+  StateT<T> sl, sr;
   get_state(p_l, g, sl);
   get_state(p_r, g, sr);
-  double F_l[NPR], F_r[NPR], U_l[NPR], U_r[NPR];
+  T F_l[NPR], F_r[NPR], U_l[NPR], U_r[NPR];
   primtoflux(P, p_l, sl, DIR, g.gdet, F_l);
   primtoflux(P, p_r, sr, DIR, g.gdet, F_r);
   primtoflux(P, p_l, sl, 0, g.gdet, U_l);
   primtoflux(P, p_r, sr, 0, g.gdet, U_r);
-  double cmax_l, cmin_l, cmax_r, cmin_r;
+  T cmax_l, cmin_l, cmax_r, cmin_r;
   if (vchar(P, p_l, sl, DIR, g, cmax_l, cmin_l)) fail = 1;
   if (vchar(P, p_r, sr, DIR, g, cmax_r, cmin_r)) fail = 1;
-  double lmin = dmin(cmin_l, cmin_r), lmax = dmax(cmax_l, cmax_r);
-  double cmin = fabs(dmax(0., -lmin)), cmax = fabs(dmax(0., lmax));
-  double ctop = dmax(cmax, cmin);
+  T lmin = dmin(cmin_l, cmin_r), lmax = dmax(cmax_l, cmax_r);
+  T cmin = fabs(dmax(T(0.), -lmin)), cmax = fabs(dmax(T(0.), lmax));
+  T ctop = dmax(cmax, cmin);
   double hllfrac = (P.fluxmethod == 1) ? 1.0 : 0.0;
   double laxffrac = (P.fluxmethod == 0) ? 1.0 : 0.0;
-  if (cmax + cmin != 0.0) {
+  const bool spread = (cmax + cmin != 0.0);
 #pragma unroll
-    for (int k = 0; k < NPR; k++)
-      Fo[k] = hllfrac * ((cmax * F_l[k] + cmin * F_r[k] - cmax * cmin * (U_r[k] - U_l[k])) / (cmax + cmin)) +
-              laxffrac * (0.5 * (F_l[k] + F_r[k] - ctop * (U_r[k] - U_l[k])));
-  } else {
-#pragma unroll
-    for (int k = 0; k < NPR; k++) Fo[k] = (0.5 * (F_l[k] + F_r[k] - ctop * (U_r[k] - U_l[k])));
+  for (int k = 0; k < NPR; k++) {
+    const T lax = (0.5 * (F_l[k] + F_r[k] - ctop * (U_r[k] - U_l[k])));
+    const T mix = hllfrac * ((cmax * F_l[k] + cmin * F_r[k] - cmax * cmin * (U_r[k] - U_l[k])) / (cmax + cmin)) +
+                       laxffrac * lax;
+    Fo[k] = spread ? mix : lax;
   }
-  double cm = dmax(cmax, cmin);
+  T cm = dmax(cmax, cmin);
   dtij = P.cour * ((DIR == 1) ? P.dx1 : P.dx2) / cm;
   if (!(dtij < 1.e9)) dtij = 1.e9;
 }
 
 // source_conn (phys.c:475-619, WITHGDET): dU[UU+l] = gdet sum_jk T^j_k
-// Gamma^k_{lj} with the connection stored as conn[k][l][j]; element e of
-// the zone's 64 at cn[e * cs] (cs = 1: the zone's own row; on the device the
-// table is component-major, cs = the number of zones)
-HARM_HD void source_conn(const KParams& P, const double* pr, const GeoU& g, const double* cn, double* dU,
-                         size_t cs = 1) {
-  State qb;
+// Gamma^k_{lj} with the connection stored as conn[k][l][j]; element e of the
+// zone's 64 is cn(e) (an accessor: the zone's own row, or on the device the
+// component-major table)
+template <class T, class CN>
+HARM_HD void source_conn_at(const KParams& P, const T* pr, const GeoT<T>& g, const CN& cn, T* dU) {
+  StateT<T> qb;
   get_state(pr, g, qb);
-  double mhd[4][4];
+  T mhd[4][4];
 #pragma unroll
   for (int jj = 0; jj < 4; jj++) mhd_calc(P, pr, jj, qb, mhd[jj]);
-  double dUc[4];
+  T dUc[4];
 #pragma unroll
   for (int l = 0; l < 4; l++) {
-    double acc = 0;
+    T acc = 0;
 #pragma unroll
     for (int jj = 0; jj < 4; jj++)
 #pragma unroll
-      for (int kk = 0; kk < 4; kk++) acc += mhd[jj][kk] * cn[(size_t)((kk * 4 + l) * 4 + jj) * cs];
+      for (int kk = 0; kk < 4; kk++) acc += mhd[jj][kk] * cn((kk * 4 + l) * 4 + jj);
     dUc[l] = acc;
   }
   dU[RHO] = g.gdet * 0.;
@@ -283,28 +295,38 @@ HARM_HD void source_conn(const KParams& P, const double* pr, const GeoU& g, cons
   dU[B3] = g.gdet * 0.;
 }
 
+// the same with element e at cn[e * cs] (cs = 1: the zone's own row; on the
+// device the table is component-major, cs = the number of zones)
+template <class T>
+HARM_HD void source_conn(const KParams& P, const T* pr, const GeoT<T>& g, const T* cn, T* dU, size_t cs = 1) {
+  source_conn_at(P, pr, g, [&](int e) { return cn[(size_t)e * cs]; }, dU);
+}
+
 // ---------------- reconstruction (interp.c) ----------------
 
 // interp.h:13-14
-HARM_HD double minmod(double a, double b) {
-  return ((a) * (b) <= 0) ? 0.0 : ((fabs(a) < fabs(b)) ? (a) : (b));
+template <class T>
+HARM_HD T minmod(T a, T b) {
+  const T m = (fabs(a) < fabs(b)) ? (a) : (b);
+  return ((a) * (b) <= 0) ? T(0.0) : m;
 }
 
 // interp.c:106-149 slope_lim_3points
-HARM_HD double slope3(int lim, double yl, double yc, double yr) {
+template <class T>
+HARM_HD T slope3(int lim, T yl, T yc, T yr) {
   if (lim == 3) {  // MC
-    double Dqm = 2.0 * (yc - yl), Dqp = 2.0 * (yr - yc), Dqc = 0.5 * (yr - yl);
+    T Dqm = 2.0 * (yc - yl), Dqp = 2.0 * (yr - yc), Dqc = 0.5 * (yr - yl);
     return minmod(Dqc, minmod(Dqm, Dqp));
   } else if (lim == 1) {  // VANL
-    double Dqm = (yc - yl), Dqp = (yr - yc), s = Dqm * Dqp;
-    if (s <= 0.) return 0.;
-    return (2.0 * s / (Dqm + Dqp));
+    T Dqm = (yc - yl), Dqp = (yr - yc), s = Dqm * Dqp;
+    const T v = (2.0 * s / (Dqm + Dqp));
+    return (s <= 0.) ? T(0.) : v;
   } else if (lim == 2) {  // MINM
     return minmod(yc - yl, yr - yc);
   } else if (lim == 101) return 0.5 * (yr - yl);
   else if (lim == 102) return (yc - yl);
   else if (lim == 103) return (yr - yc);
-  return 0.0;  // DONOR
+  return T(0.0);  // DONOR
 }
 
 // interp.c:157-205 csslope
@@ -329,36 +351,38 @@ HARM_HD double csslope(int kk, double x1, double x2, double x3, double x4, doubl
                   Dql);
 }
 
-// interp.c:449-538 para4 (PARA2LIM=MC): left/right edge values of the middle zone
-HARM_HD void para4(double y0, double y1, double y2, double y3, double y4, double& lout,
-                                      double& rout) {
-  double y[5] = {y0, y1, y2, y3, y4}, dq[4];
+// interp.c:449-538 para4 (PARA2LIM=MC): left/right edge values of the middle
+// zone.  Every branch of the reference is a select between values formed on
+// every path (same operations, same bits), so a kernel's 16 reconstructions
+// form one straight run of independent FP64 work.
+template <class T>
+HARM_HD void para4(T y0, T y1, T y2, T y3, T y4, T& lout, T& rout) {
+  T y[5] = {y0, y1, y2, y3, y4}, dq[4];
 #pragma unroll
   for (int i = 1; i < 4; i++) {
-    double Dqm = 2.0 * (y[i] - y[i - 1]);
-    double Dqp = 2.0 * (y[i + 1] - y[i]);
-    double Dqc = 0.5 * (y[i + 1] - y[i - 1]);
-    double s = Dqm * Dqp;
-    dq[i] = (s <= 0.) ? 0. : dmin(fabs(Dqc), dmin(fabs(Dqm), fabs(Dqp))) * dsign(Dqc);
+    T Dqm = 2.0 * (y[i] - y[i - 1]);
+    T Dqp = 2.0 * (y[i + 1] - y[i]);
+    T Dqc = 0.5 * (y[i + 1] - y[i - 1]);
+    T s = Dqm * Dqp;
+    const T m = dmin(fabs(Dqc), dmin(fabs(Dqm), fabs(Dqp))) * dsign(Dqc);
+    dq[i] = (s <= 0.) ? T(0.) : m;
   }
-  double l = 0.5 * (y[2] + y[1]) - (dq[2] - dq[1]) / 8.0;
-  double r = 0.5 * (y[3] + y[2]) - (dq[3] - dq[2]) / 8.0;
+  T l = 0.5 * (y[2] + y[1]) - (dq[2] - dq[1]) / 8.0;
+  T r = 0.5 * (y[3] + y[2]) - (dq[3] - dq[2]) / 8.0;
   l = dmax(dmin(y[2], y[1]), l);
   l = dmin(dmax(y[2], y[1]), l);
   r = dmax(dmin(y[2], y[3]), r);
   r = dmin(dmax(y[2], y[3]), r);
-  double qa = (r - y[2]) * (y[2] - l);
-  double qd = (r - l);
-  double qe = 8.0 * (y[2] - 0.5 * (l + r));
-  if (qa <= 0.) {
-    l = y[2];
-    r = y[2];
-  } else {
-    if (qd * (qd - qe) < 0.0) l = 3.0 * y[2] - 2.0 * r;
-    if (qd * (qd + qe) < 0.0) r = 3.0 * y[2] - 2.0 * l;
-  }
-  lout = l;
-  rout = r;
+  T qa = (r - y[2]) * (y[2] - l);
+  T qd = (r - l);
+  T qe = 8.0 * (y[2] - 0.5 * (l + r));
+  const T l3 = 3.0 * y[2] - 2.0 * r;
+  const T l1 = (qd * (qd - qe) < 0.0) ? l3 : l;
+  const T r3 = 3.0 * y[2] - 2.0 * l1;
+  const T r1 = (qd * (qd + qe) < 0.0) ? r3 : r;
+  const bool flat = qa <= 0.;
+  lout = flat ? y[2] : l1;
+  rout = flat ? y[2] : r1;
 }
 
 // ---------------- GRMHD inversion (utoprim_2d_final.c) ----------------

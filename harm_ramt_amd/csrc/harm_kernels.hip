@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <stddef.h>
 #include "harm_device.h"
+#include "harm_tangent.h"
 #include "harm_adjoint.h"
 #include "harm_eno.h"
 #include "../../include/harm_amd.h"
@@ -44,17 +45,49 @@ constexpr int NG = 3;
 constexpr int GF = 16;   // doubles per face geometry record
 constexpr int GCN = 40;  // doubles per CENT geometry record
 
+// Accumulators that every wavefront of a launch updates are striped over
+// NSTRIPE slots, one 128-byte line apart: thousands of same-address atomics
+// from one launch serialise (measured on MI355X: 2,586 waves adding to one
+// address cost 20-27 us more than to 64 addresses), a slot per wavefront
+// group does not.  Readers combine the slots.
+constexpr int NSTRIPE = 64;
+constexpr int SSTR = 16;  // slot stride in 8-byte words
+
 struct DevScalars {
-  unsigned long long ndt_acc[2];  // running min of dtij bits, per direction
+  unsigned long long ndt_acc[2][NSTRIPE * SSTR];  // running min of dtij bits, per direction, striped
   double ndt_last[2];             // snapshot taken after both flux kernels
   double t, dt, tf;
   int failed;
   int nfail;
   int nlist;  // entries of the failed-zone list of the current substage
   int cs_changed;  // fixup_checksolution: the second pass differs from the first
-  unsigned long long nstroke;
   double dt_step;  // dt of the step just completed (diag_flux's Dt)
+  unsigned long long nstroke[NSTRIPE * SSTR];  // Newton iterations, striped
 };
+
+// the stripe of the calling wavefront (any spread of a launch's wavefronts)
+__device__ __forceinline__ int stripe_of(const dim3 bi) {
+  const unsigned w = (threadIdx.z * blockDim.y + threadIdx.y) * blockDim.x + threadIdx.x;
+  return (int)((bi.x * 7u + bi.y * 13u + bi.z * 29u + (w >> 6) * 5u) & (NSTRIPE - 1));
+}
+__device__ __forceinline__ void ndt_push(DevScalars* __restrict__ ds, int dir, double wm, const dim3 bi) {
+  atomicMin(&ds->ndt_acc[dir - 1][stripe_of(bi) * SSTR], (unsigned long long)__double_as_longlong(wm));
+}
+// the Courant limits of the substage's flux launch into ndt_last, every
+// slot reset for the next one (one thread, between the flux launches)
+__device__ __forceinline__ void ndt_take(DevScalars* __restrict__ ds) {
+  const unsigned long long big = (unsigned long long)__double_as_longlong(1.e9);
+#pragma unroll
+  for (int d = 0; d < 2; d++) {
+    unsigned long long m = big;
+    for (int k = 0; k < NSTRIPE; k++) {
+      const unsigned long long v = ds->ndt_acc[d][k * SSTR];
+      m = (v < m) ? v : m;
+      ds->ndt_acc[d][k * SSTR] = big;
+    }
+    ds->ndt_last[d] = __longlong_as_double((long long)m);
+  }
+}
 
 thread_local char g_err[512];
 
@@ -98,7 +131,8 @@ __device__ __forceinline__ void load_geo_face(const double* __restrict__ gF, int
 }
 
 // upper-triangle view of a full CENT record
-__device__ __forceinline__ void geo_from_cent(const double* c, GeoU& g) {
+template <class T>
+__device__ __forceinline__ void geo_from_cent(const T* c, GeoT<T>& g) {
   g.g00 = c[0]; g.g01 = c[1]; g.g02 = c[2]; g.g03 = c[3];
   g.g11 = c[5]; g.g12 = c[6]; g.g13 = c[7];
   g.g22 = c[10]; g.g23 = c[11]; g.g33 = c[15];
@@ -280,8 +314,7 @@ __device__ __forceinline__ void flux_body(KParams P, const double* __restrict__ 
     flux_face<2>(P, pr, F2, q2, gF + nz * GF, i, j, side, (i <= P.n1) && (j <= P.n2), dtij, fail);
   }
   double wm = wave_min(dtij);
-  if ((threadIdx.x & 63) == 0 && wm < 1.e9)
-    atomicMin(&ds->ndt_acc[dir - 1], (unsigned long long)__double_as_longlong(wm));
+  if ((threadIdx.x & 63) == 0 && wm < 1.e9) ndt_push(ds, dir, wm, bi);
   if (fail) atomicOr(&ds->failed, 1);
 }
 
@@ -418,18 +451,48 @@ __device__ __forceinline__ void flux_single_body(KParams P, const double* __rest
     if (i <= P.n1 && j <= P.n2) flux_face_single<2>(P, pr, F2, q2, gF + nz * GF, i, j, dtij, fail);
   }
   double wm = wave_min(dtij);
-  if ((threadIdx.x & 63) == 0 && wm < 1.e9)
-    atomicMin(&ds->ndt_acc[dir - 1], (unsigned long long)__double_as_longlong(wm));
+  if ((threadIdx.x & 63) == 0 && wm < 1.e9) ndt_push(ds, dir, wm, bi);
   if (fail) atomicOr(&ds->failed, 1);
 }
 
-__global__ __launch_bounds__(256) void k_flux_single(KParams P, const double* __restrict__ pr,
+// The scheme parameters of the common configurations as compile-time
+// constants (a negative template value keeps the runtime parameter): the
+// inlined physics then has no branches on the limiter, the equations of
+// motion or the flux method, and the scheduler sees longer straight runs.
+template <int LIMC, int EOMC, int FMC>
+__device__ __forceinline__ KParams specialize(KParams P) {
+  if (LIMC >= 0) P.lim = LIMC;
+  if (EOMC >= 0) P.eomtype = EOMC;
+  if (FMC >= 0) P.fluxmethod = FMC;
+  return P;
+}
+
+#ifndef HARM_FLUX_ATTR
+#define HARM_FLUX_ATTR
+#endif
+template <int LIMC, int EOMC, int FMC>
+__global__ __launch_bounds__(256) HARM_FLUX_ATTR void k_flux_single(KParams P, const double* __restrict__ pr,
                                                      double* __restrict__ F1, double* __restrict__ F2,
                                                      double* __restrict__ q1, double* __restrict__ q2,
                                                      const double* __restrict__ gF, DevScalars* __restrict__ ds,
                                                      int dir0) {
-  flux_single_body(P, pr, F1, F2, q1, q2, gF, ds, dir0, dim3(blockIdx.x, blockIdx.y, blockIdx.z),
-      dim3(gridDim.x, gridDim.y, gridDim.z));
+  flux_single_body(specialize<LIMC, EOMC, FMC>(P), pr, F1, F2, q1, q2, gF, ds, dir0,
+                   dim3(blockIdx.x, blockIdx.y, blockIdx.z), dim3(gridDim.x, gridDim.y, gridDim.z));
+}
+
+using FluxKernel = void (*)(KParams, const double*, double*, double*, double*, double*, const double*, DevScalars*,
+                            int);
+// the instantiation for a context's scheme: the torus (PARA, GRMHD, HLL), the
+// force-free problems (MC, FFDE, LAXF) and Orszag-Tang (MC, GRMHD, HLL), or
+// the general kernel
+FluxKernel flux_single_kernel(const KParams& P) {
+  if (P.lim == HARM_PARA && P.eomtype == HARM_EOMGRMHD && P.fluxmethod == HARM_HLLFLUX)
+    return k_flux_single<HARM_PARA, HARM_EOMGRMHD, HARM_HLLFLUX>;
+  if (P.lim == HARM_MC && P.eomtype == HARM_EOMFFDE && P.fluxmethod == HARM_LAXFFLUX)
+    return k_flux_single<HARM_MC, HARM_EOMFFDE, HARM_LAXFFLUX>;
+  if (P.lim == HARM_MC && P.eomtype == HARM_EOMGRMHD && P.fluxmethod == HARM_HLLFLUX)
+    return k_flux_single<HARM_MC, HARM_EOMGRMHD, HARM_HLLFLUX>;
+  return k_flux_single<-1, -1, -1>;
 }
 
 // Edge states of one zone along DIR from its 5-zone stencil s[0..4]
@@ -515,7 +578,7 @@ __global__ __launch_bounds__(64 * FSH_ROWS) void k_flux_shared(KParams P, const 
     }
   }
   double wm = wave_min(dtij);
-  if ((tx & 63) == 0 && wm < 1.e9) atomicMin(&ds->ndt_acc[dir - 1], (unsigned long long)__double_as_longlong(wm));
+  if ((tx & 63) == 0 && wm < 1.e9) ndt_push(ds, dir, wm, blockIdx);
   if (fail) atomicOr(&ds->failed, 1);
 }
 
@@ -614,7 +677,8 @@ struct RKCoef {
 };
 
 // lower() (phys.c:633-652, upper triangle of gcov) on a full CENT record
-__device__ __forceinline__ void lower_gc(const double* gc, const double* a, double* b) {
+template <class T>
+__device__ __forceinline__ void lower_gc(const T* gc, const T* a, T* b) {
   b[0] = gc[0] * a[0] + gc[1] * a[1] + gc[2] * a[2] + gc[3] * a[3];
   b[1] = gc[1] * a[0] + gc[5] * a[1] + gc[6] * a[2] + gc[7] * a[3];
   b[2] = gc[2] * a[0] + gc[6] * a[1] + gc[10] * a[2] + gc[11] * a[3];
@@ -627,42 +691,43 @@ __device__ __forceinline__ void lower_gc(const double* gc, const double* a, doub
 // Ug[B1..B3].  The drift velocity's u^i goes into pr[U1..U3] (vcon2pr,
 // transforms.c:381-386); pr[RHO], pr[UU] are not touched.  Returns 1 on the
 // space-like failure (phys.ffde.c:309-312), which is fatal in the reference.
-__device__ int utoprim_ffde(const KParams& P, const double* Ug, const double* gc, double* pr) {
-  const double* gcon = gc + 16;
-  const double gdet = gc[32];
-  const double alphasq = 1. / (-gcon[0]);
-  double Bcon[4] = {0.0, Ug[B1], Ug[B2], Ug[B3]}, Bcov[4];
+template <class T>
+__device__ int utoprim_ffde(const KParams& P, const T* Ug, const T* gc, T* pr) {
+  const T* gcon = gc + 16;
+  const T gdet = gc[32];
+  const T alphasq = 1. / (-gcon[0]);
+  T Bcon[4] = {T(0.0), Ug[B1], Ug[B2], Ug[B3]}, Bcov[4];
   lower_gc(gc, Bcon, Bcov);
-  double Bsq = 0.0;
+  T Bsq = 0.0;
 #pragma unroll
   for (int j = 1; j < 4; j++) Bsq += Bcov[j] * Bcon[j];
-  const double T1 = Ug[UU + 1], T2 = Ug[UU + 2], T3 = Ug[UU + 3];
-  const double o = 1.0 / (Bsq * gdet);
-  double Econ[4], Ecov[4];
+  const T T1 = Ug[UU + 1], T2 = Ug[UU + 2], T3 = Ug[UU + 3];
+  const T o = 1.0 / (Bsq * gdet);
+  T Econ[4], Ecov[4];
   Econ[0] = 0;
   Econ[1] = (Bcov[2] * T3 - Bcov[3] * T2) * (o);
   Econ[2] = (Bcov[3] * T1 - Bcov[1] * T3) * (o);
   Econ[3] = (Bcov[1] * T2 - Bcov[2] * T1) * (o);
-  double realBsq = Bsq;
+  T realBsq = Bsq;
   lower_gc(gc, Econ, Ecov);
-  double Esq = 0.0;
+  T Esq = 0.0;
 #pragma unroll
   for (int j = 1; j < 4; j++) Esq += Econ[j] * Ecov[j];
-  const double BsqmEsq = Bsq - Esq;
-  const double gammasq = Bsq / BsqmEsq;
+  const T BsqmEsq = Bsq - Esq;
+  const T gammasq = Bsq / BsqmEsq;
   if ((gammasq > P.gammamax * P.gammamax) || (gammasq < 1.0)) {
     const double Gsq = 1.0 - 1.0 / (P.gammamax * P.gammamax);
     realBsq = (sqrt(Esq * Bsq / Gsq));
   }
-  const double ob = 1.0 / (realBsq * gdet);
-  double v1 = alphasq * (-gcon[1] + (Ecov[2] * Bcov[3] - Ecov[3] * Bcov[2]) * ob);
-  double v2 = alphasq * (-gcon[2] + (Ecov[3] * Bcov[1] - Ecov[1] * Bcov[3]) * ob);
-  double v3 = alphasq * (-gcon[3] + (Ecov[1] * Bcov[2] - Ecov[2] * Bcov[1]) * ob);
+  const T ob = 1.0 / (realBsq * gdet);
+  T v1 = alphasq * (-gcon[1] + (Ecov[2] * Bcov[3] - Ecov[3] * Bcov[2]) * ob);
+  T v2 = alphasq * (-gcon[2] + (Ecov[3] * Bcov[1] - Ecov[1] * Bcov[3]) * ob);
+  T v3 = alphasq * (-gcon[3] + (Ecov[1] * Bcov[2] - Ecov[2] * Bcov[1]) * ob);
   // ucon_calc_3vel (phys.c:885-955)
-  double discr = +gc[0] + gc[5] * v1 * v1 + gc[10] * v2 * v2 + gc[15] * v3 * v3 +
+  T discr = +gc[0] + gc[5] * v1 * v1 + gc[10] * v2 * v2 + gc[15] * v3 * v3 +
                  2. * (gc[1] * v1 + gc[2] * v2 + gc[3] * v3 + gc[6] * v1 * v2 + gc[7] * v1 * v3 + gc[11] * v2 * v3);
   if (discr > 0.) return 1;
-  const double ut = 1. / sqrt(-discr);
+  const T ut = 1. / sqrt(-discr);
   pr[U1] = v1 * ut;
   pr[U2] = v2 * ut;
   pr[U3] = v3 * ut;
@@ -980,11 +1045,11 @@ __device__ __forceinline__ void invert_zone(const KParams& P, const double* U, c
 }
 
 // Newton iterations, failed zones and the fatal flag of a wavefront
-__device__ __forceinline__ void count_zone(DevScalars* __restrict__ ds, int niter, int failz, int fail) {
+__device__ __forceinline__ void count_zone(DevScalars* __restrict__ ds, int niter, int failz, int fail, const dim3 bi) {
   unsigned long long ns = wave_sum_u64((unsigned long long)niter);
   unsigned long long nf = wave_sum_u64((unsigned long long)failz);
   if ((threadIdx.x & 63) == 0) {
-    if (ns) atomicAdd(&ds->nstroke, ns);
+    if (ns) atomicAdd(&ds->nstroke[stripe_of(bi) * SSTR], ns);
     if (nf) atomicAdd(&ds->nfail, (int)nf);
   }
   if (fail) atomicOr(&ds->failed, 1);
@@ -1010,12 +1075,7 @@ __device__ __forceinline__ void update_body(KParams P, RKCoef C, const double* _
   const int j = bi.x * blockDim.x + threadIdx.x;
   const int i = bi.y * blockDim.y + threadIdx.y;
   int niter = 0, failz = 0, fail = 0;
-  if (i == 0 && j == 0) {  // this substage's Courant limits (advance.c:190 reads them)
-    ds->ndt_last[0] = __longlong_as_double((long long)ds->ndt_acc[0]);
-    ds->ndt_last[1] = __longlong_as_double((long long)ds->ndt_acc[1]);
-    ds->ndt_acc[0] = (unsigned long long)__double_as_longlong(1.e9);
-    ds->ndt_acc[1] = (unsigned long long)__double_as_longlong(1.e9);
-  }
+  if (i == 0 && j == 0) ndt_take(ds);  // this substage's Courant limits (advance.c:190 reads them)
   if (i < P.n1 && j < P.n2) {
     const int z = zidx(P, i, j);
     const double dt = ds->dt;
@@ -1069,11 +1129,14 @@ __device__ __forceinline__ void update_body(KParams P, RKCoef C, const double* _
       invert_zone(P, C.last ? un : Uf, pin, gc, g, z, pf, pflag, faillist, ds, niter, failz, fail);
     }
   }
-  count_zone(ds, niter, failz, fail);
+  count_zone(ds, niter, failz, fail, bi);
 }
 
+#ifndef HARM_UPDATE_ATTR
+#define HARM_UPDATE_ATTR
+#endif
 template <int ENO>
-__global__ __launch_bounds__(256) void k_update(KParams P, RKCoef C, const double* __restrict__ pi,
+__global__ __launch_bounds__(256) HARM_UPDATE_ATTR void k_update(KParams P, RKCoef C, const double* __restrict__ pi,
                                                 const double* __restrict__ pb, double* __restrict__ pf,
                                                 double* __restrict__ ulast, double* __restrict__ unew,
                                                 const double* __restrict__ F1, const double* __restrict__ F2,
@@ -1154,24 +1217,25 @@ __global__ __launch_bounds__(256) void k_eno_invert(KParams P, const double* __r
     eno_cvt_zone<ENO_A2C, 2>(P, in, i, j, U);
     invert_zone(P, U, pin, gc, g, z, pf, pflag, faillist, ds, niter, failz, fail);
   }
-  count_zone(ds, niter, failz, fail);
+  count_zone(ds, niter, failz, fail, blockIdx);
 }
 
 // inflow_check_rel4vel for x1 ghosts at CENT (fixup.c:1733-1841)
-__device__ int inflow_check_x1(const KParams& P, int i, double* pr, const double* gc, const GeoU& g) {
-  double alpha = 1. / sqrt(-g.n00);
-  double gamma = gamma_calc(pr, g);
-  double ucon1 = pr[U1] - gamma * (g.n01 * alpha * alpha) / alpha;
+template <class T>
+__device__ int inflow_check_x1(const KParams& P, int i, T* pr, const T* gc, const GeoT<T>& g) {
+  T alpha = 1. / sqrt(-g.n00);
+  T gamma = gamma_calc(pr, g);
+  T ucon1 = pr[U1] - gamma * (g.n01 * alpha * alpha) / alpha;
   bool dofix = ((i <= -1) && (P.bc[HARM_X1DN] == HARM_OUTFLOW) && (ucon1 > 0.)) ||
                ((i >= P.n1) && (P.bc[HARM_X1UP] == HARM_OUTFLOW) && (ucon1 < 0.));
   if (!dofix) return 0;
   pr[U1] /= gamma;
   pr[U2] /= gamma;
   pr[U3] /= gamma;
-  double alpha2 = 1. / sqrt(-gc[16]);
-  double betacon = gc[17] * alpha2 * alpha2;
+  T alpha2 = 1. / sqrt(-gc[16]);
+  T betacon = gc[17] * alpha2 * alpha2;
   pr[U1] = betacon / alpha2;
-  double vsq = 0.;
+  T vsq = 0.;
 #pragma unroll
   for (int a = 1; a < 4; a++)
 #pragma unroll
@@ -2112,8 +2176,8 @@ extern "C" int harm_create(const harm_params* prm, harm_ctx** out) {
   DevScalars h;
   memset(&h, 0, sizeof(h));
   double big = 1.e9;
-  memcpy(&h.ndt_acc[0], &big, 8);
-  memcpy(&h.ndt_acc[1], &big, 8);
+  for (int d = 0; d < 2; d++)
+    for (int k = 0; k < NSTRIPE * SSTR; k++) memcpy(&h.ndt_acc[d][k], &big, 8);
   h.ndt_last[0] = h.ndt_last[1] = big;
   h.tf = 1e30;
   e = hipMemcpy(c->ds, &h, sizeof(h), hipMemcpyHostToDevice);
@@ -2168,11 +2232,15 @@ extern "C" int harm_set_stream(harm_ctx* c, void* s) {
   return 0;
 }
 
-extern "C" int harm_set_geometry(harm_ctx* c, const double* gcov, const double* gcon, const double* gdet,
-                                 const double* conn, const double* floors) {
-  if (!c || !gcov || !gcon || !gdet || !conn || !floors) return seterr("null argument");
-  size_t nz = (size_t)c->nz;
-  std::vector<double> hF(2 * nz * GF), hC(nz * GCN, 0.0);
+// the reference's set_grid() tables packed into the device records: face
+// records hF [2][nz][GF], CENT records hC [nz][GCN], the connection
+// component-major hT [64][nz] (floors may be null: zero)
+static void pack_geometry(size_t nz, const double* gcov, const double* gcon, const double* gdet, const double* conn,
+                          const double* floors, std::vector<double>& hF, std::vector<double>& hC,
+                          std::vector<double>& hT) {
+  hF.assign(2 * nz * GF, 0.0);
+  hC.assign(nz * GCN, 0.0);
+  hT.assign(nz * 64, 0.0);
   for (size_t z = 0; z < nz; z++) {
     for (int pos = 0; pos < 2; pos++) {
       const double* gc = gcov + (z * 4 + pos) * 16;
@@ -2189,17 +2257,23 @@ extern "C" int harm_set_geometry(harm_ctx* c, const double* gcov, const double* 
     memcpy(o, gcov + (z * 4 + 3) * 16, 16 * sizeof(double));
     memcpy(o + 16, gcon + (z * 4 + 3) * 16, 16 * sizeof(double));
     o[32] = gdet[z * 4 + 3];
-    o[33] = floors[z * 2 + 0];
-    o[34] = floors[z * 2 + 1];
+    o[33] = floors ? floors[z * 2 + 0] : 0.0;
+    o[34] = floors ? floors[z * 2 + 1] : 0.0;
     o[35] = gdet[z * 4 + 2];
   }
-  HIPCHK(hipMemcpy(c->gF, hF.data(), hF.size() * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->gC, hC.data(), hC.size() * sizeof(double), hipMemcpyHostToDevice));
   // the connection component-major ([64][zones]): the update's lanes (adjacent
   // zones) then read adjacent words of each component
-  std::vector<double> hT((size_t)nz * 64);
-  for (size_t z = 0; z < (size_t)nz; z++)
+  for (size_t z = 0; z < nz; z++)
     for (int e = 0; e < 64; e++) hT[(size_t)e * nz + z] = conn[z * 64 + e];
+}
+
+extern "C" int harm_set_geometry(harm_ctx* c, const double* gcov, const double* gcon, const double* gdet,
+                                 const double* conn, const double* floors) {
+  if (!c || !gcov || !gcon || !gdet || !conn || !floors) return seterr("null argument");
+  std::vector<double> hF, hC, hT;
+  pack_geometry((size_t)c->nz, gcov, gcon, gdet, conn, floors, hF, hC, hT);
+  HIPCHK(hipMemcpy(c->gF, hF.data(), hF.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->gC, hC.data(), hC.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->gConn, hT.data(), hT.size() * sizeof(double), hipMemcpyHostToDevice));
   c->geometry_set = true;
   return 0;
@@ -2347,8 +2421,9 @@ static int launch_flux(harm_ctx* c, int pr_sel, int dir0, int ndir, bool fix = t
     }
     default: {
       const dim3 b = (c->flux_variant == HARM_FLUX_FACE_64x2) ? dim3(64, 2) : dim3(64, 4);
-      hipLaunchKernelGGL(k_flux_single, dim3((nfj + 63) / 64, (nfi + b.y - 1) / b.y, ndir), b, 0, c->stream,
-                         kp, c->arr[pr_sel], c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2, c->gF, c->ds, dir0);
+      hipLaunchKernelGGL(flux_single_kernel(kp), dim3((nfj + 63) / 64, (nfi + b.y - 1) / b.y, ndir), b, 0,
+                         c->stream, kp, c->arr[pr_sel], c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2, c->gF,
+                         c->ds, dir0);
       break;
     }
   }
@@ -2537,12 +2612,18 @@ extern "C" int harm_fluxcalc(harm_ctx* c, int pr_sel, int dir, double* ndt) {
   if (dir != 1 && dir != 2) return seterr("dir must be 1 or 2");
   // the fluxes as fluxcalc() leaves them: fix_flux() is a call of its own
   if (launch_flux(c, pr_sel, dir, 1, false)) return 1;
-  unsigned long long bits;
-  HIPCHK(hipMemcpyAsync(&bits, &c->ds->ndt_acc[dir - 1], 8, hipMemcpyDeviceToHost, c->stream));
+  // the striped running minimum: combined here, then every slot reset
+  std::vector<unsigned long long> slots(NSTRIPE * SSTR);
+  const size_t sbytes = slots.size() * sizeof(unsigned long long);
+  HIPCHK(hipMemcpyAsync(slots.data(), &c->ds->ndt_acc[dir - 1][0], sbytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   unsigned long long big;
   double b = 1.e9;
   memcpy(&big, &b, 8);
-  HIPCHK(hipMemcpyAsync(&c->ds->ndt_acc[dir - 1], &big, 8, hipMemcpyHostToDevice, c->stream));
+  unsigned long long bits = big;
+  for (int k = 0; k < NSTRIPE; k++) bits = std::min(bits, slots[(size_t)k * SSTR]);
+  std::fill(slots.begin(), slots.end(), big);
+  HIPCHK(hipMemcpyAsync(&c->ds->ndt_acc[dir - 1][0], slots.data(), sbytes, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (ndt) memcpy(ndt, &bits, 8);
   return 0;
@@ -2842,10 +2923,13 @@ extern "C" int harm_get_failures(harm_ctx* c, int* nfailzones, int* failed) {
 
 extern "C" int harm_get_nstroke(harm_ctx* c, long long* nstroke) {
   if (!c || !nstroke) return seterr("null argument");
-  unsigned long long v;
-  HIPCHK(hipMemcpyAsync(&v, &c->ds->nstroke, 8, hipMemcpyDeviceToHost, c->stream));
+  std::vector<unsigned long long> v(NSTRIPE * SSTR);
+  HIPCHK(hipMemcpyAsync(v.data(), &c->ds->nstroke[0], v.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                        c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  *nstroke = (long long)v;
+  unsigned long long sum = 0;
+  for (int k = 0; k < NSTRIPE; k++) sum += v[(size_t)k * SSTR];
+  *nstroke = (long long)sum;
   return 0;
 }
 
@@ -3163,3 +3247,4 @@ namespace {
 #include "harm_adjoint.inl"
 #include "harm_acct.inl"
 #include "harm_diag.inl"
+#include "harm_tangent.inl"

@@ -290,6 +290,35 @@ int harm_adj_get_lambda(harm_adj *adj, double *lambda);
    repaired in the recomputation: both 0 means the gradient is exact */
 int harm_adj_get_nonsmooth(harm_adj *adj, int *count, int *recompute_failures);
 
+/* ---- tangent-linear (forward-mode) step (no reference counterpart) ----
+   Advances a context's state together with its derivative with respect to
+   one parameter s (for instance the black-hole spin), given the derivative
+   of the geometry tables and of the initial state: the step's own physics
+   evaluated on value-plus-tangent numbers, so the values are the plain
+   step's to the bit and the tangents follow by the chain rule on the
+   branches the values take.  The time steps are the values' own (adaptive
+   dt), i.e. the derivative of the run with its dt sequence held fixed.
+   Supported: EOMFFDE (the explicit force-free inversion), every limiter but
+   CSSLOPE, HLL/LAXF, flux-CT, OUTFLOW/POLARAXIS/PERIODIC boundaries,
+   FIXUPFLUX on or off; no DOENO, CHECKSOLUTION, accounting or current. */
+typedef struct harm_tl harm_tl;
+int harm_tl_create(harm_ctx *ctx, harm_tl **out);
+int harm_tl_destroy(harm_tl *tl);
+/* d/ds of the set_grid() tables, in harm_set_geometry's layout (zero until set) */
+int harm_tl_set_geometry_tangent(harm_tl *tl, const double *dgcov, const double *dgcon,
+                                 const double *dgdet, const double *dconn);
+/* d/ds of the grid spacings dx1, dx2 (zero until set): the flux divergence
+   divides by them (the Courant limit, which sets dt, is not differentiated) */
+int harm_tl_set_spacing_tangent(harm_tl *tl, double ddx1, double ddx2);
+/* d/ds of the context's state HARM_P, [N1M][N2M][NPR] (zero until set) */
+int harm_tl_set_state_tangent(harm_tl *tl, const double *dp);
+int harm_tl_get_state_tangent(harm_tl *tl, double *dp);
+/* nsteps calls of step_ch() on the context's state and its tangent */
+int harm_tl_step_n(harm_tl *tl, int nsteps);
+/* the outgoing energy flux through zone row i of the context's state,
+   L = -2 pi dx2 sum_j gdet T^1_0 (zone-centred, over j in [0,N2)), and dL/ds */
+int harm_tl_energy_flux(harm_tl *tl, int i, double *L, double *dL);
+
 /* ---- host-side problem and geometry generation (no GPU use) ---- */
 
 /* Grid/coordinate description (coord.c: set_points, bl_coord, dxdxprim). */

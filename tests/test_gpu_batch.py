@@ -165,3 +165,56 @@ def test_config4_force_free_spin_batch_512():
     assert same(runs[-1].get_array(H.P), o.get_array(0)), "a=0.99 run vs the oracle"
     for h in runs:
         h.close()
+
+
+def test_config2_sixty_four_run_draw():
+    """The batched sweep's own 64 runs (seed 2024: grids 64x64 .. 512x256,
+    2.81 M zones, spins in [0, 0.99]) advanced as one batch for 3 steps:
+    every run bit-identical to its context stepped alone from the same
+    state, and the two smallest runs to the oracle."""
+    from concurrent.futures import ThreadPoolExecutor
+    import bench
+    draws = bench.batch_draws(64, 2024)
+    nsteps, dt0 = 3, 1e-5
+    say(f"[case] the batched sweep's 64-run draw (seed 2024), {sum(d['n1'] * d['n2'] for d in draws)} zones, "
+        f"{nsteps} steps")
+    small = sorted(range(64), key=lambda k: draws[k]["n1"] * draws[k]["n2"])[:2]
+    runs, alone, keep = [], [], {}
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        for k, (geom, p0, prm) in enumerate(ex.map(bench.build_torus, draws)):
+            h = H.Harm(prm, geom)
+            h.set_array(H.P, p0)
+            h.bound_prim(H.P)
+            h.set_time(0.0, dt0, 2000.0)
+            h.step_n(nsteps)
+            alone.append((h.get_array(H.P), h.get_time(), h.nstroke(), h.failures()))
+            h.set_array(H.P, p0)
+            h.bound_prim(H.P)
+            h.set_time(0.0, dt0, 2000.0)
+            runs.append(h)
+            if k in small:
+                keep[k] = (geom, p0, prm)
+    say("  each run stepped alone; now the batch")
+    b = H.Batch(runs)
+    b.step_n(nsteps)
+    for k, h in enumerate(runs):
+        P, t, ns, fl = alone[k]
+        g = f"run {k} ({draws[k]['n1']}x{draws[k]['n2']}, a={draws[k]['a']:.3f})"
+        assert h.get_time() == t, g + ": t, dt"
+        assert h.nstroke() - ns == ns, g + ": Newton iterations"
+        assert same(h.get_array(H.P), P), g + ": state"
+        assert h.failures()[1] == fl[1] == 0
+    b.close()
+    for k in small:
+        geom, p0, prm = keep[k]
+        o = Oracle(H.params_dict(prm))
+        o.set_geometry(*geom.tables())
+        o.set_array(0, p0)
+        o.bound_prim(0)
+        o.set_time(0.0, dt0, 2000.0)
+        for _ in range(nsteps):
+            o.step()
+        assert runs[k].get_time() == o.get_time(), f"run {k}: t, dt vs the oracle"
+        assert same(runs[k].get_array(H.P), o.get_array(0)), f"run {k}: state vs the oracle"
+    for h in runs:
+        h.close()
