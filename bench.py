@@ -156,7 +156,7 @@ def build_torus(d):
     return geom, p0, prm
 
 
-def batch_sweep(nruns, steps, warmup, seed=2024):
+def batch_sweep(nruns, steps, warmup, seed=2024, tuning=None):
     """Batched sweep of independent FM tori (grids between 64x64 and 512x256,
     spin in [0, 0.99], r_max in [10, 16], beta in [30, 300], +-2 % u
     perturbations, all drawn from one seed): zone-updates/s of the batch
@@ -181,6 +181,11 @@ def batch_sweep(nruns, steps, warmup, seed=2024):
     del built
     zones = sum(d["n1"] * d["n2"] for d in draws)
     b = H.Batch(runs)
+    if tuning:                             # forced variants (profiling runs): no autotuner
+        b.set_tuning(tuning[0] if tuning[0] in (H.FLUX_FACE_64x4, H.FLUX_FACE_64x2) else H.FLUX_FACE_64x2,
+                     tuning[1] if tuning[1] in (H.BOUND_KERNELS, H.BOUND_BLOCK) else H.BOUND_KERNELS)
+        for h in runs:
+            h.set_tuning(*tuning)
     b.step_n(warmup)                       # includes the batch autotuner
     for h in runs:
         h.step_n(1)                        # each context's own autotuner and graph
@@ -274,16 +279,21 @@ def ffde_measure(steps, warmup, nspins=16, n=512, a_single=0.9):
     return out
 
 
-def spin_derivative_measure(n=512, nsteps=2000, a0=0.9, r_flux=20.0, h_fd=1e-5):
+def spin_derivative_measure(n=512, nsteps=100, a0=0.9, r_flux=20.0, growth_steps=400):
     """d L / d a for the force-free split monopole (config of ffde_measure at
     spin a0): L = the outgoing energy flux through the zone row nearest
     r_flux after nsteps steps.  The tangent-linear run (harm_tl: the step on
     value-plus-tangent numbers, dt sequence of the values) starts from the
     spin tangents of the geometry tables, the grid spacing and the initial
     state (fourth-order differences of the host generators); checked against
-    the central difference of two plain runs at a0 -+ h_fd with their own
-    grids and initial states and the base run's dt sequence imposed.  Also
-    the cost of a tangent-linear step against a plain (graph) step."""
+    the central differences of two plain runs at a0 -+ h (h = 1e-5, 1e-7) with
+    their own grids and initial states and the base run's dt sequence
+    imposed.  The run is then continued to growth_steps steps recording the
+    largest state tangent: the linearised force-free step amplifies
+    perturbations at the equatorial current sheet by a roughly constant
+    factor per step, which bounds the horizon over which a derivative of this
+    problem is meaningful.  Also the cost of a tangent-linear step against a
+    plain (graph) step."""
     import numpy as np
     import harm_ramt_amd as H
 
@@ -334,7 +344,16 @@ def spin_derivative_measure(n=512, nsteps=2000, a0=0.9, r_flux=20.0, h_fd=1e-5):
         h.close()
         return v
 
-    fd = (fd_run(a0 + h_fd) - fd_run(a0 - h_fd)) / (2 * h_fd)
+    fd = {f"{hh:g}": (fd_run(a0 + hh) - fd_run(a0 - hh)) / (2 * hh) for hh in (1e-5, 1e-7)}
+    # growth of the tangent beyond the horizon: largest |dp/da| over the
+    # interior every 50 steps, and where it sits at the end
+    growth = []
+    for s in range(nsteps, growth_steps, 50):
+        tl.step_n(50)
+        d = np.abs(tl.state_tangent()[3:-3, 3:-3])
+        growth.append([s + 50, float(np.nanmax(d)) if np.isfinite(d).any() else None])
+    d = np.nan_to_num(np.abs(tl.state_tangent()[3:-3, 3:-3]), nan=np.inf).max(axis=2)
+    big = np.argwhere(d >= np.sort(d, axis=None)[-min(100, d.size)])
     # steady-state cost: 50 tangent-linear steps against 50 plain graph steps
     # continuing the same run
     k = 50
@@ -351,8 +370,11 @@ def spin_derivative_measure(n=512, nsteps=2000, a0=0.9, r_flux=20.0, h_fd=1e-5):
     base.close()
     return {"workload": f"force-free split monopole a={a0}, MKS {n}x{n}, MC+LAXF+RK2, {nsteps} steps",
             "functional": f"outgoing energy flux through zone row {row} (r ~ {r_flux} M) after the last step",
-            "L": L, "dL_da_tangent_linear": dL, "dL_da_central_difference": fd, "fd_h": h_fd,
-            "rel_diff": abs(dL - fd) / abs(fd) if fd != 0 else None,
+            "L": L, "dL_da_tangent_linear": dL, "dL_da_central_difference": fd,
+            "rel_diff": {k: abs(dL - v) / abs(v) for k, v in fd.items()},
+            "tangent_max_by_step": growth,
+            "largest_tangents_zone_range": {"i": [int(big[:, 0].min()), int(big[:, 0].max())],
+                                            "j": [int(big[:, 1].min()), int(big[:, 1].max())]},
             "tangent_step_ms": 1e3 * t_tl, "plain_step_ms": 1e3 * t_plain, "tangent_cost_vs_plain_step": t_tl / t_plain,
             "tangent_run_s": t_tl_run, "tangent_tables_s": t_tables}
 
@@ -451,22 +473,38 @@ def adjoint_measure(n=256, nsteps=256, K=16, seed=3):
     w = np.zeros_like(p0)
     w[3:-3, 3:-3, 0] = rng.normal(0, 1, (n, n))
 
-    def checks_of(lam, pr, dts):
+    def perturbed(name, e):
+        if name == "B0":
+            return state(B0 + e, vamp)
+        if name == "v_amplitude":
+            return state(B0, vamp + e)
+        return p0 + e * w
+
+    def checks_of(lam, pr, dts, sweep=False):
         out = {}
-        for name, plus, minus, dpdq in (
-                ("B0", state(B0 + hh, vamp), state(B0 - hh, vamp),
-                 (state(B0 + hh, vamp) - state(B0 - hh, vamp)) / (2 * hh)),
-                ("v_amplitude", state(B0, vamp + hh), state(B0, vamp - hh),
-                 (state(B0, vamp + hh) - state(B0, vamp - hh)) / (2 * hh)),
-                ("rho_perturbation_direction", p0 + hh * w, p0 - hh * w, w)):
+        for name in ("B0", "v_amplitude", "rho_perturbation_direction"):
+            dpdq = w if name == "rho_perturbation_direction" else (perturbed(name, hh) - perturbed(name, -hh)) / (2 * hh)
             ad = float((lam * dpdq).sum())
-            fd = (J_of(plus, pr, dts) - J_of(minus, pr, dts)) / (2 * hh)
+            Jp, Jm = J_of(perturbed(name, hh), pr, dts), J_of(perturbed(name, -hh), pr, dts)
+            fd = (Jp - Jm) / (2 * hh)
             scale = float(np.abs(lam * dpdq).sum())
             out[name] = {"adjoint": ad, "finite_difference": fd, "abs_diff_over_term_scale": abs(ad - fd) / scale,
                          "rel_diff": abs(ad - fd) / abs(fd)}
+            if sweep:
+                # a kink of J (a limiter branch switch) within h of the base
+                # state adds an error that shrinks with h, unlike a wrong
+                # derivative; one-sided differences straddle it
+                J0 = J_of(p0 if name == "rho_perturbation_direction" else perturbed(name, 0.0), pr, dts)
+                out[name]["one_sided_plus"] = (Jp - J0) / hh
+                out[name]["one_sided_minus"] = (J0 - Jm) / hh
+                rels = {}
+                for e in (1e-7, 1e-8):
+                    f = (J_of(perturbed(name, e), pr, dts) - J_of(perturbed(name, -e), pr, dts)) / (2 * e)
+                    rels[f"{e:g}"] = abs(ad - f) / abs(f)
+                out[name]["rel_diff_vs_fd_step"] = rels
         return out
 
-    checks = checks_of(lam, prm, dts)
+    checks = checks_of(lam, prm, dts, sweep=True)
     # MC's slope switches are kinks of J: finite differences straddle them.
     # The same checks with a smooth limiter (NLIMCENT), where central
     # differences converge to the derivative
@@ -523,7 +561,9 @@ def main():
     ap.add_argument("--no-ffde", action="store_true")
     ap.add_argument("--ffde-steps", type=int, default=20)
     ap.add_argument("--no-spin", action="store_true")
-    ap.add_argument("--spin-steps", type=int, default=2000)
+    ap.add_argument("--spin-steps", type=int, default=100)
+    ap.add_argument("--eager", action="store_true",
+                    help="launch the timed steps eagerly instead of replaying a graph (for profilers)")
     ap.add_argument("--tuning", default=None,
                     help="force the kernel variants as FLUX,BOUND (harm_set_tuning); default: autotuned")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -548,6 +588,8 @@ def main():
     if args.tuning:
         h.set_tuning(*[int(v) for v in args.tuning.split(",")])
 
+    if args.eager:
+        h.prof_enable(True)         # eager launches, per-stage events
     h.step_n(args.warmup)           # the first call also times the kernel variants and keeps the fastest
     h.get_time()                    # synchronises
     fv, bv = h.get_tuning()
@@ -603,11 +645,13 @@ def main():
         h.acct_off()
 
     h.close()
-    batched = None if args.no_batch else batch_sweep(args.batch_runs, args.batch_steps, 3)
+    forced = [int(v) for v in args.tuning.split(",")] if args.tuning else None
+    batched = None if args.no_batch else batch_sweep(args.batch_runs, args.batch_steps, 3, tuning=forced)
     adjoint = None if args.no_adjoint else adjoint_measure()
     ffde = None if args.no_ffde else ffde_measure(args.ffde_steps, 3)
     if ffde is not None and not args.no_spin:
         ffde["spin_derivative"] = spin_derivative_measure(nsteps=args.spin_steps)
+    # the spin derivative's growth diagnostic continues the tangent run
     cpu = None if args.no_cpu else cpu_baseline(geom, p0, prm, n1, n2, args.cpu_seconds)
     out = {
         "metric": METRIC, "value": value, "unit": "zone-updates/s", "n_gpus": 1, "steps": args.steps,

@@ -484,7 +484,7 @@ __global__ __launch_bounds__(256) void k_cons_sum(KParams P, AcctRegions R, int 
 
 // limit_gamma on the face states of fluxcalc (step_ch.c:978-979): the event
 // of each state it changes (faces and their order as the flux kernels)
-__global__ void k_acct_faceg(KParams P, const double* __restrict__ pr, const double* __restrict__ gF) {
+__global__ __launch_bounds__(256) void k_acct_faceg(KParams P, const double* __restrict__ pr, const double* __restrict__ gF) {
   const int dir = 1 + blockIdx.z;
   const int jf = blockIdx.x * blockDim.x + threadIdx.x;
   const int ifc = blockIdx.y * blockDim.y + threadIdx.y;
@@ -512,7 +512,7 @@ __global__ void k_acct_faceg(KParams P, const double* __restrict__ pr, const dou
 
 // fixup1zone's events (fixup.c:274-377) from the state before it: limit_gamma
 // (COUNTLIMITGAMMAACT), then the whole change (COUNTFLOORACT, final substage)
-__global__ void k_acct_zoneg(KParams P, const double* __restrict__ gC, int finalstep) {
+__global__ __launch_bounds__(256) void k_acct_zoneg(KParams P, const double* __restrict__ gC, int finalstep) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= P.n1 || j >= P.n2) return;
@@ -545,7 +545,7 @@ __global__ void k_acct_zoneg(KParams P, const double* __restrict__ gC, int final
 
 // the failed zones' events from the states fixup_zone kept: limit_gamma's
 // (slot flag -1: it changed the averaged state) and the repair's
-__global__ void k_acct_fixg(KParams P, const double* __restrict__ gC) {
+__global__ __launch_bounds__(256) void k_acct_fixg(KParams P, const double* __restrict__ gC) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= P.n1 || j >= P.n2) return;

@@ -134,10 +134,12 @@ __global__ void k_adj_seed_bsq(KParams P, const double* p, const double* gC, dou
 }
 
 // inversion (implicit-function theorem) + RK combination, interior zones
-__global__ void k_adj_zone(KParams P, AdjRK R, const double* const* __restrict__ slot, const double* __restrict__ dts,
+template <int LIMC, int FMC>
+__global__ __launch_bounds__(256) void k_adj_zone(KParams P0, AdjRK R, const double* const* __restrict__ slot, const double* __restrict__ dts,
                            const int* __restrict__ ctr, const double* __restrict__ gC,
                            const double* __restrict__ lp, double* __restrict__ lunew, double* __restrict__ lulast,
                            double* __restrict__ lUi, double* __restrict__ ldU, int* __restrict__ nonsmooth) {
+  const KParams P = specialize<LIMC, HARM_EOMGRMHD, FMC>(P0);
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y * blockDim.y + threadIdx.y;
   if (i >= P.n1 || j >= P.n2) return;
@@ -245,11 +247,13 @@ __device__ __forceinline__ void adj_face(const KParams& P, const double* __restr
   store8(lf, 2 * z0 + 1, prb);
 }
 
-__global__ __launch_bounds__(256) void k_adj_face(KParams P, const double* const* __restrict__ slot,
+template <int LIMC, int FMC>
+__global__ __launch_bounds__(256) void k_adj_face(KParams P0, const double* const* __restrict__ slot,
                                                   const double* __restrict__ gF, const double* __restrict__ gC,
                                                   const double* __restrict__ ldU, const double* __restrict__ lemf,
                                                   double* __restrict__ lf1, double* __restrict__ lf2,
                                                   int* __restrict__ nonsmooth) {
+  const KParams P = specialize<LIMC, HARM_EOMGRMHD, FMC>(P0);
   const double* __restrict__ pb = slot[0];
   const int dir = 1 + blockIdx.z;
   const int jf = blockIdx.x * blockDim.x + threadIdx.x;
@@ -269,10 +273,12 @@ __global__ __launch_bounds__(256) void k_adj_face(KParams P, const double* const
 // gather: adjoint of every zone of pb from the face-state adjoints of the
 // faces whose reconstruction stencil holds it (pl(f) = p(f-1) + dq(f-1)/2,
 // pr(f) = p(f) - dq(f)/2, dq on the limiter's branch), plus the source term
-__global__ __launch_bounds__(256) void k_adj_gather(KParams P, const double* const* __restrict__ slot,
+template <int LIMC, int FMC>
+__global__ __launch_bounds__(256) void k_adj_gather(KParams P0, const double* const* __restrict__ slot,
                                                     const double* __restrict__ gC, const double* __restrict__ gConn,
                                                     const double* __restrict__ ldU, const double* __restrict__ lf1,
                                                     const double* __restrict__ lf2, double* __restrict__ lp) {
+  const KParams P = specialize<LIMC, HARM_EOMGRMHD, FMC>(P0);
   const double* __restrict__ pb = slot[0];
   const int j = (int)(blockIdx.x * blockDim.x + threadIdx.x) - NG;
   const int i = (int)(blockIdx.y * blockDim.y + threadIdx.y) - NG;
@@ -322,7 +328,7 @@ __global__ __launch_bounds__(256) void k_adj_gather(KParams P, const double* con
 
 // Ui = U(p_n) transposed into the adjoint of p_n; clears lUi, lunew, lulast
 // (the adjoints of the substage accumulators, which start each step at zero)
-__global__ void k_adj_ui(KParams P, const double* const* __restrict__ slot, const double* __restrict__ gC,
+__global__ __launch_bounds__(256) void k_adj_ui(KParams P, const double* const* __restrict__ slot, const double* __restrict__ gC,
                          double* __restrict__ lUi, double* __restrict__ lunew, double* __restrict__ lulast,
                          double* __restrict__ lp) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -483,17 +489,34 @@ static int adj_boundT(harm_adj* a);
 
 // the reverse of step ctr[0] (its substage states in the segment store),
 // then ctr[0]--
+// the reverse kernels specialised for the Orszag-Tang scheme (MC with HLL or
+// LAXF), or the general ones
+template <int LIMC, int FMC>
+static void adj_launch_substage(harm_adj* a, const AdjRK& R, bool first) {
+  harm_ctx* c = a->c;
+  const dim3 b(64, 4);
+  const dim3 gz = grid2(c->kp.n2, c->kp.n1, b);
+  const dim3 gc2 = grid2(c->kp.n2 + 1, c->kp.n1 + 1, b);
+  const dim3 gf((c->kp.n2 + 2 + 63) / 64, (c->kp.n1 + 2 + 3) / 4, 2);
+  const dim3 gall = grid2(c->kp.n2 + 2 * NG, c->kp.n1 + 2 * NG, b);
+  hipLaunchKernelGGL((k_adj_zone<LIMC, FMC>), gz, b, 0, c->stream, c->kp, R, a->d_slot, a->d_dts, a->d_ctr, c->gC,
+                     a->lp, a->lunew, a->lulast, a->lUi, a->ldU, a->d_ns);
+  if (c->prm.fluxb == HARM_FLUXCTTOTH)
+    hipLaunchKernelGGL(k_adj_emf, gc2, b, 0, c->stream, c->kp, a->ldU, a->lemf);
+  hipLaunchKernelGGL((k_adj_face<LIMC, FMC>), gf, b, 0, c->stream, c->kp, a->d_slot, c->gF, c->gC, a->ldU, a->lemf,
+                     a->lf1, a->lf2, a->d_ns);
+  hipLaunchKernelGGL((k_adj_gather<LIMC, FMC>), gall, b, 0, c->stream, c->kp, a->d_slot, c->gC, c->gConn, a->ldU,
+                     a->lf1, a->lf2, a->lp);
+  if (first)
+    hipLaunchKernelGGL(k_adj_ui, gz, b, 0, c->stream, c->kp, a->d_slot, c->gC, a->lUi, a->lunew, a->lulast, a->lp);
+}
+
 static int adj_reverse_step(harm_adj* a) {
   harm_ctx* c = a->c;
   const size_t nz8 = (size_t)c->nz * 8;
   double CUf[4][3], Cunew[4][3];
   int pii[4], pbb[4], pff[4], nto;
   rk_setup(c->prm.timeorder, &nto, CUf, Cunew, pii, pbb, pff);
-  const dim3 b(64, 4);
-  const dim3 gz = grid2(c->kp.n2, c->kp.n1, b);
-  const dim3 gc2 = grid2(c->kp.n2 + 1, c->kp.n1 + 1, b);
-  const dim3 gf((c->kp.n2 + 2 + 63) / 64, (c->kp.n1 + 2 + 3) / 4, 2);
-  const dim3 gall = grid2(c->kp.n2 + 2 * NG, c->kp.n1 + 2 * NG, b);
   for (int s = nto - 1; s >= 0; s--) {
     AdjRK R;
     R.a0 = CUf[s][0]; R.a1 = CUf[s][1]; R.a2 = CUf[s][2];
@@ -502,16 +525,12 @@ static int adj_reverse_step(harm_adj* a) {
     R.last = (s == nto - 1);
     if (adj_boundT(a)) return 1;
     hipLaunchKernelGGL(k_adj_slot, dim3(1), dim3(1), 0, c->stream, a->seg, a->d_ctr, nto, s, nz8, a->d_slot);
-    hipLaunchKernelGGL(k_adj_zone, gz, b, 0, c->stream, c->kp, R, a->d_slot, a->d_dts, a->d_ctr, c->gC, a->lp,
-                       a->lunew, a->lulast, a->lUi, a->ldU, a->d_ns);
-    if (c->prm.fluxb == HARM_FLUXCTTOTH)
-      hipLaunchKernelGGL(k_adj_emf, gc2, b, 0, c->stream, c->kp, a->ldU, a->lemf);
-    hipLaunchKernelGGL(k_adj_face, gf, b, 0, c->stream, c->kp, a->d_slot, c->gF, c->gC, a->ldU, a->lemf, a->lf1,
-                       a->lf2, a->d_ns);
-    hipLaunchKernelGGL(k_adj_gather, gall, b, 0, c->stream, c->kp, a->d_slot, c->gC, c->gConn, a->ldU, a->lf1,
-                       a->lf2, a->lp);
-    if (s == 0)
-      hipLaunchKernelGGL(k_adj_ui, gz, b, 0, c->stream, c->kp, a->d_slot, c->gC, a->lUi, a->lunew, a->lulast, a->lp);
+    if (c->kp.lim == HARM_MC && c->kp.fluxmethod == HARM_HLLFLUX)
+      adj_launch_substage<HARM_MC, HARM_HLLFLUX>(a, R, s == 0);
+    else if (c->kp.lim == HARM_MC && c->kp.fluxmethod == HARM_LAXFFLUX)
+      adj_launch_substage<HARM_MC, HARM_LAXFFLUX>(a, R, s == 0);
+    else
+      adj_launch_substage<-1, -1>(a, R, s == 0);
   }
   hipLaunchKernelGGL(k_adj_next, dim3(1), dim3(1), 0, c->stream, a->d_ctr, -1);
   HIPCHK(hipGetLastError());

@@ -467,11 +467,8 @@ __device__ __forceinline__ KParams specialize(KParams P) {
   return P;
 }
 
-#ifndef HARM_FLUX_ATTR
-#define HARM_FLUX_ATTR
-#endif
 template <int LIMC, int EOMC, int FMC>
-__global__ __launch_bounds__(256) HARM_FLUX_ATTR void k_flux_single(KParams P, const double* __restrict__ pr,
+__global__ __launch_bounds__(256) void k_flux_single(KParams P, const double* __restrict__ pr,
                                                      double* __restrict__ F1, double* __restrict__ F2,
                                                      double* __restrict__ q1, double* __restrict__ q2,
                                                      const double* __restrict__ gF, DevScalars* __restrict__ ds,
@@ -1132,11 +1129,10 @@ __device__ __forceinline__ void update_body(KParams P, RKCoef C, const double* _
   count_zone(ds, niter, failz, fail, bi);
 }
 
-#ifndef HARM_UPDATE_ATTR
-#define HARM_UPDATE_ATTR
-#endif
-template <int ENO>
-__global__ __launch_bounds__(256) HARM_UPDATE_ATTR void k_update(KParams P, RKCoef C, const double* __restrict__ pi,
+// EOMC >= 0: the equations of motion as a compile-time constant (the
+// inversion and the fluxes without the other branch)
+template <int ENO, int EOMC = -1>
+__global__ __launch_bounds__(256) void k_update(KParams P, RKCoef C, const double* __restrict__ pi,
                                                 const double* __restrict__ pb, double* __restrict__ pf,
                                                 double* __restrict__ ulast, double* __restrict__ unew,
                                                 const double* __restrict__ F1, const double* __restrict__ F2,
@@ -1144,8 +1140,9 @@ __global__ __launch_bounds__(256) HARM_UPDATE_ATTR void k_update(KParams P, RKCo
                                                 const double* __restrict__ gC, const double* __restrict__ gConn,
                                                 int* __restrict__ pflag, int* __restrict__ faillist,
                                                 DevScalars* __restrict__ ds, double* eA, const double* eB) {
-  update_body<ENO>(P, C, pi, pb, pf, ulast, unew, F1, F2, q1, q2, gC, gConn, pflag, faillist, ds, eA, eB,
-              dim3(blockIdx.x, blockIdx.y, blockIdx.z), dim3(gridDim.x, gridDim.y, gridDim.z));
+  update_body<ENO>(specialize<-1, EOMC, -1>(P), C, pi, pb, pf, ulast, unew, F1, F2, q1, q2, gC, gConn, pflag,
+                   faillist, ds, eA, eB, dim3(blockIdx.x, blockIdx.y, blockIdx.z),
+                   dim3(gridDim.x, gridDim.y, gridDim.z));
 }
 
 // ---- DOENO kernels (interior zones, thread per zone) ----
@@ -1724,7 +1721,7 @@ __global__ __launch_bounds__(1024) void k_fixall(KParams P, double* __restrict__
   }
 }
 
-__global__ void k_utoprim_batch(KParams P, int n, const int* __restrict__ zone, const double* __restrict__ U,
+__global__ __launch_bounds__(128) void k_utoprim_batch(KParams P, int n, const int* __restrict__ zone, const double* __restrict__ U,
                                 double* __restrict__ pr, int* __restrict__ pfl, const double* __restrict__ gC) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
@@ -2449,7 +2446,11 @@ static int launch_update(harm_ctx* c, int pi, int pb, int pf, const RKCoef& C) {
     hipLaunchKernelGGL(k_eno_fluxdiv, gz, b, 0, c->stream, c->kp, c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2,
                        c->gC, c->enoA, c->enoB);
   }
-  auto* kup = (c->kp.eno == 1) ? k_update<1> : (c->kp.eno == 2) ? k_update<2> : k_update<0>;
+  auto* kup = (c->kp.eno == 1)                        ? k_update<1>
+              : (c->kp.eno == 2)                      ? k_update<2>
+              : (c->kp.eomtype == HARM_EOMGRMHD)      ? k_update<0, HARM_EOMGRMHD>
+              : (c->kp.eomtype == HARM_EOMFFDE)       ? k_update<0, HARM_EOMFFDE>
+                                                      : k_update<0>;
   hipLaunchKernelGGL(kup, gz, b, 0, c->stream, c->kp, C, c->arr[pi], c->arr[pbs], c->arr[pf], c->arr[HARM_ULAST],
                      c->arr[HARM_UNEW], c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2, c->gC, c->gConn, c->pflag,
                      c->faillist, c->ds, c->enoA, c->enoB);
@@ -2741,7 +2742,7 @@ static int build_graph(harm_ctx* c) {
 // gamma table (written only at flagged zones, read at their neighbours).
 // Every other array is overwritten within a step before it is read;
 // accounting and the current are paused while the trials run.
-static int time_variant(harm_ctx* c, const double* keep, const double* keep_gc, const DevScalars& save, float* ms) {
+static int time_variant(harm_ctx* c, const double* keep, const double* keep_gc, const DevScalars* keep_ds, float* ms) {
   const size_t pbytes = (size_t)c->nz * 8 * sizeof(double);
   hipGraphExec_t ex;
   if (capture_step(c, &ex)) return 1;
@@ -2751,7 +2752,7 @@ static int time_variant(harm_ctx* c, const double* keep, const double* keep_gc, 
   hipError_t e = hipSuccess;
   for (int pass = 0; pass < 2 && e == hipSuccess; pass++) {
     e = hipMemcpyAsync(c->arr[HARM_P], keep, pbytes, hipMemcpyDeviceToDevice, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(c->ds, &save, sizeof(save), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->ds, keep_ds, sizeof(DevScalars), hipMemcpyDeviceToDevice, c->stream);
     if (e == hipSuccess && keep_gc)
       e = hipMemcpyAsync(c->gammacheck, keep_gc, (size_t)c->nz * sizeof(double), hipMemcpyDeviceToDevice, c->stream);
     if (e == hipSuccess && pass == 1) e = hipEventRecord(a, c->stream);
@@ -2770,14 +2771,14 @@ static int time_variant(harm_ctx* c, const double* keep, const double* keep_gc, 
 static int autotune(harm_ctx* c) {
   if (c->tuned) return 0;
   const size_t pbytes = (size_t)c->nz * 8 * sizeof(double);
-  DevScalars save;
-  HIPCHK(hipMemcpyAsync(&save, c->ds, sizeof(save), hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
   double* keep = nullptr;
   double* keep_gc = nullptr;
+  DevScalars* keep_ds = nullptr;  // the device scalars, kept on the device
   HIPCHK(hipMalloc(&keep, pbytes));
   int rc = 0;
-  if (hipMemcpyAsync(keep, c->arr[HARM_P], pbytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+  if (hipMalloc(&keep_ds, sizeof(DevScalars)) != hipSuccess ||
+      hipMemcpyAsync(keep_ds, c->ds, sizeof(DevScalars), hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+      hipMemcpyAsync(keep, c->arr[HARM_P], pbytes, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
     rc = seterr("autotune: copy");
   const size_t gbytes = (size_t)c->nz * sizeof(double);
   if (!rc && c->kp.checksolution &&
@@ -2803,7 +2804,7 @@ static int autotune(harm_ctx* c) {
       float ms = 0.f;
       c->flux_variant = fv[f];
       c->bound_variant = bv[v];
-      rc = time_variant(c, keep, keep_gc, save, &ms);
+      rc = time_variant(c, keep, keep_gc, keep_ds, &ms);
       if (!rc && ms < best) { best = ms; bf = fv[f]; bb = bv[v]; }
     }
   c->prof = prof;
@@ -2812,10 +2813,12 @@ static int autotune(harm_ctx* c) {
   c->flux_variant = bf;
   c->bound_variant = bb;
   hipError_t e = hipMemcpyAsync(c->arr[HARM_P], keep, pbytes, hipMemcpyDeviceToDevice, c->stream);
-  if (e == hipSuccess) e = hipMemcpyAsync(c->ds, &save, sizeof(save), hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess && keep_ds)
+    e = hipMemcpyAsync(c->ds, keep_ds, sizeof(DevScalars), hipMemcpyDeviceToDevice, c->stream);
   if (e == hipSuccess && keep_gc) e = hipMemcpyAsync(c->gammacheck, keep_gc, gbytes, hipMemcpyDeviceToDevice, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipFree(keep);
+  if (keep_ds) (void)hipFree(keep_ds);
   if (keep_gc) (void)hipFree(keep_gc);
   if (rc) return 1;
   if (e != hipSuccess) return seterr("autotune: %s", hipGetErrorString(e));
@@ -3144,12 +3147,13 @@ extern "C" int harm_batch_create(harm_ctx* const* ctxs, int n, harm_batch** out)
 static int batch_autotune(harm_batch* b) {
   if (b->tuned) return 0;
   const int n = b->n;
-  std::vector<DevScalars> save(n);
+  std::vector<DevScalars*> save(n, nullptr);  // device copies of each run's scalars
   std::vector<double*> keep(n, nullptr);
   int rc = 0;
   for (int r = 0; r < n && !rc; r++) {
     harm_ctx* c = (*b->ctx)[r];
-    if (hipMemcpy(&save[r], c->ds, sizeof(DevScalars), hipMemcpyDeviceToHost) != hipSuccess ||
+    if (hipMalloc(&save[r], sizeof(DevScalars)) != hipSuccess ||
+        hipMemcpy(save[r], c->ds, sizeof(DevScalars), hipMemcpyDeviceToDevice) != hipSuccess ||
         hipMalloc(&keep[r], (size_t)c->nz * 8 * sizeof(double)) != hipSuccess ||
         hipMemcpy(keep[r], c->arr[HARM_P], (size_t)c->nz * 8 * sizeof(double), hipMemcpyDeviceToDevice) != hipSuccess)
       rc = seterr("batch autotune: save");
@@ -3160,7 +3164,7 @@ static int batch_autotune(harm_batch* b) {
       harm_ctx* c = (*b->ctx)[r];
       e = hipMemcpyAsync(c->arr[HARM_P], keep[r], (size_t)c->nz * 8 * sizeof(double), hipMemcpyDeviceToDevice,
                          b->stream);
-      if (e == hipSuccess) e = hipMemcpyAsync(c->ds, &save[r], sizeof(DevScalars), hipMemcpyHostToDevice, b->stream);
+      if (e == hipSuccess) e = hipMemcpyAsync(c->ds, save[r], sizeof(DevScalars), hipMemcpyDeviceToDevice, b->stream);
     }
     return e;
   };
@@ -3198,7 +3202,10 @@ static int batch_autotune(harm_batch* b) {
   }
   if (ea) (void)hipEventDestroy(ea);
   if (eb) (void)hipEventDestroy(eb);
-  for (int r = 0; r < n; r++) (void)hipFree(keep[r]);
+  for (int r = 0; r < n; r++) {
+    (void)hipFree(keep[r]);
+    if (save[r]) (void)hipFree(save[r]);
+  }
   if (!rc) b->tuned = true;
   return rc;
 }

@@ -194,11 +194,15 @@ __global__ __launch_bounds__(256) void k_tl_update(KParams P, RKCoef C, TLArrays
       get_state(pin, g, q);
       primtoflux(P, pin, q, 0, g.gdet, Ui);
     }
+    // phases kept in order (each phase's loads issued where it starts): the
+    // value-plus-tangent state of the whole update does not fit in registers
+    __builtin_amdgcn_sched_barrier(0);
     // source(pb) with the component-major connection and its tangent
     const size_t nzz = (size_t)(P.n1 + 2 * NG) * P.N2M;
     Dual dUg[NPR];
     source_conn_at(P, pbz, g,
                    [&](int e) { return Dual(A.gConn[(size_t)e * nzz + z], A.dgConn[(size_t)e * nzz + z]); }, dUg);
+    __builtin_amdgcn_sched_barrier(0);
     // flux2dUavg with the flux-CT B fluxes from the corner EMFs
     Dual f1a[NPR], f1b[NPR], f2a[NPR], f2b[NPR];
     load8t(A.F1, A.dF1, z, f1a);
@@ -219,6 +223,7 @@ __global__ __launch_bounds__(256) void k_tl_update(KParams P, RKCoef C, TLArrays
       f2b[B1] = -0.5 * (e01 + e11);
       f2b[B2] = 0.;
     }
+    __builtin_amdgcn_sched_barrier(0);
     const Dual idx1 = 1.0 / Dual(P.dx1, A.ddx1), idx2 = 1.0 / Dual(P.dx2, A.ddx2);
     Dual ul[NPR], un[NPR], Uf[NPR];
     if (C.timeorder == 0) {
@@ -239,12 +244,12 @@ __global__ __launch_bounds__(256) void k_tl_update(KParams P, RKCoef C, TLArrays
     }
     store8t(A.ul, A.dul, z, ul);
     store8t(A.un, A.dun, z, un);
+    __builtin_amdgcn_sched_barrier(0);
     // Utoprimgen (force-free) into pf from the guess pi, then fixup1zone
-    const Dual* U = C.last ? un : Uf;
     Dual Ug[NPR], pout[NPR];
 #pragma unroll
     for (int k = 0; k < NPR; k++) {
-      Ug[k] = U[k] / gc[32];
+      Ug[k] = (C.last ? un[k] : Uf[k]) / gc[32];
       pout[k] = pin[k];
     }
     if (utoprim_ffde(P, Ug, gc, pout)) fail = 1;  // space-like drift: fatal in the reference
