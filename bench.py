@@ -667,7 +667,8 @@ def main():
                                                     H.BOUND_BLOCK: "one_workgroup",
                                                     H.BOUND_KERNELS_SEQCHECK: "wide_launches_seqcheck",
                                                     H.BOUND_FIXONE: "wide_fills_one_workgroup_fixup",
-                                                    H.BOUND_FUSED: "one_launch_fills_one_workgroup_fixup"}[bv]}},
+                                                    H.BOUND_FUSED: "one_launch_fills_one_workgroup_fixup",
+                                                    H.BOUND_INUPDATE: "first_fill_in_update_one_workgroup_fixup"}[bv]}},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,

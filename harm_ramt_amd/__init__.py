@@ -30,7 +30,7 @@ X1UP, X1DN, X2UP, X2DN = 0, 1, 2, 3
 P, PK0, PK1, F1, F2, ULAST, UNEW = 0, 1, 2, 3, 4, 5, 6
 METRIC_MINK, METRIC_KS = 0, 2
 FLUX_FACE_64x4, FLUX_FACE_64x2, FLUX_PAIR, FLUX_SHARED = 0, 1, 2, 3
-BOUND_KERNELS, BOUND_BLOCK, BOUND_KERNELS_SEQCHECK, BOUND_FIXONE, BOUND_FUSED = 0, 1, 2, 3, 4
+BOUND_KERNELS, BOUND_BLOCK, BOUND_KERNELS_SEQCHECK, BOUND_FIXONE, BOUND_FUSED, BOUND_INUPDATE = 0, 1, 2, 3, 4, 5
 K_FLUX, K_EMF, K_UPDATE, K_BOUNDARY = 0, 2, 3, 4
 STAGE_NAMES = {K_FLUX: "flux", K_EMF: "flux_ct", K_UPDATE: "update", K_BOUNDARY: "boundary"}
 

@@ -23,6 +23,7 @@
 //   gC[nz][40]     CENT: gcov 16, gcon 16, gdet, floor rho, floor u, gdet(CORN)
 //   gConn[nz][64]  CENT connection Gamma^k_{lj} as conn[k][l][j]
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <string.h>
 #include <stdio.h>
 #include <stdarg.h>
@@ -63,6 +64,12 @@ struct DevScalars {
   int cs_changed;  // fixup_checksolution: the second pass differs from the first
   double dt_step;  // dt of the step just completed (diag_flux's Dt)
   unsigned long long nstroke[NSTRIPE * SSTR];  // Newton iterations, striped
+  // HARM_BOUND_INUPDATE with N1 >= 5: the x1 ghost columns of the pole rows'
+  // outflow edge zones as the second bound_prim() computes them (from the row
+  // after POLEDEATH), written by k_update, copied by k_fixall where the edge
+  // zone was not repaired; [e][g]: e&1 the x1 side, e&2 the x2 side, column g+1
+  double pole_x1[4][NG][8];
+  int pole_fail[4][NG];
 };
 
 // the stripe of the calling wavefront (any spread of a launch's wavefronts)
@@ -1059,7 +1066,31 @@ __device__ __forceinline__ void count_zone(DevScalars* __restrict__ ds, int nite
 //      unew on the last substage) goes to eA, inverted later by k_eno_invert
 //   2  DOENO=2 (advance.c:142-158): the flux divergence of each direction is
 //      converted average -> centre (a2c of eA along x1, of eB along x2)
-template <int ENO>
+constexpr int NIMG = 1 + 2 * NG;  // image parts of an interior zone (bound_image_part)
+__device__ void bound_image_part(const KParams& P, double* __restrict__ pv, const double* __restrict__ gC, int i,
+                                 int j, const double* v, int part, int& fail);
+__device__ __forceinline__ bool pole_row(const KParams& P, int j);
+__device__ __forceinline__ bool near_edge(const KParams& P, int i, int j);
+__device__ __forceinline__ int x1_image_column(const KParams& P, int i, int part);
+__device__ __forceinline__ void x1_value(const KParams& P, const double* __restrict__ gC, int i, int j, double* v,
+                                         int& fail);
+
+// The edge block of a k_update launch (64x4 workgroups) whose other three
+// rows write the x1 ghost columns of its outflow edge row: 0 the down side
+// (row 0), 1 the up side (row N1-1), -1 none (both edges in one block: the
+// edge zones write their columns themselves).
+__device__ __forceinline__ int edge_block_side(const KParams& P, const dim3 bi) {
+  const bool dn = P.bc[HARM_X1DN] == HARM_OUTFLOW && bi.y == 0;
+  const bool up = P.bc[HARM_X1UP] == HARM_OUTFLOW && (int)bi.y == (P.n1 - 1) / 4;
+  return (dn && up) ? -1 : dn ? 0 : up ? 1 : -1;
+}
+static_assert(NG == 3, "an edge block has NG rows besides the edge row");
+
+// BND: the first bound_prim() of the substage done here (ENO 0 only): each
+// zone near an edge writes its ghost images after its inversion; an outflow
+// edge row hands its values to the other rows of its workgroup (LDS, one
+// barrier), which write the NG x1 ghost columns, one column per row
+template <int ENO, int BND = 0>
 __device__ __forceinline__ void update_body(KParams P, RKCoef C, const double* __restrict__ pi,
                                                 const double* __restrict__ pb, double* __restrict__ pf,
                                                 double* __restrict__ ulast, double* __restrict__ unew,
@@ -1072,6 +1103,8 @@ __device__ __forceinline__ void update_body(KParams P, RKCoef C, const double* _
   const int j = bi.x * blockDim.x + threadIdx.x;
   const int i = bi.y * blockDim.y + threadIdx.y;
   int niter = 0, failz = 0, fail = 0;
+  __shared__ double edge_v[BND ? 64 * NPR : 1];
+  const int eside = BND ? edge_block_side(P, bi) : -1;
   if (i == 0 && j == 0) ndt_take(ds);  // this substage's Courant limits (advance.c:190 reads them)
   if (i < P.n1 && j < P.n2) {
     const int z = zidx(P, i, j);
@@ -1124,6 +1157,58 @@ __device__ __forceinline__ void update_body(KParams P, RKCoef C, const double* _
       store8(eA, z, C.last ? un : Uf);
     } else {
       invert_zone(P, C.last ? un : Uf, pin, gc, g, z, pf, pflag, faillist, ds, niter, failz, fail);
+      if constexpr (BND == 1) {  // (the edge block's other rows continue below)
+        if (near_edge(P, i, j)) {
+          double v[NPR];
+          load8(pf, z, v);
+          const bool handed = eside >= 0 && i == (eside ? P.n1 - 1 : 0);
+          for (int part = 0; part < NIMG; part++)
+            if (!(handed && part > 0 && (part > NG) == (eside == 1)))
+              bound_image_part(P, pf, gC, i, j, v, part, fail);
+          if (handed) {
+#pragma unroll
+            for (int k = 0; k < NPR; k++) edge_v[threadIdx.x * NPR + k] = v[k];
+          }
+          if (pole_row(P, j)) {  // POLEDEATH of the pole row, after its images
+            pf[(size_t)z * 8 + U2] *= 0.;
+            pf[(size_t)z * 8 + B2] *= 0.;
+          }
+        }
+      }
+    }
+  }
+  if constexpr (BND == 1) {
+    if (eside >= 0) {  // block-uniform
+      __syncthreads();  // the edge row's values in edge_v
+      const int orow = eside ? (P.n1 - 1) - (int)(bi.y * blockDim.y) : 0;
+      const int ty = threadIdx.y, src = eside ? P.n1 - 1 : 0;
+      if (ty != orow && j < P.n2) {  // the first bound_prim()'s column, one per row
+        const int part = (eside ? NG + 1 : 1) + (ty < orow ? ty : ty - 1);
+        double v[NPR];
+#pragma unroll
+        for (int k = 0; k < NPR; k++) v[k] = edge_v[threadIdx.x * NPR + k];
+        bound_image_part(P, pf, gC, src, j, v, part, fail);
+      } else if (ty == orow && threadIdx.x >= 1 && threadIdx.x <= 2 * NG && P.n1 >= 5) {
+        // the edge row's own lanes 1..2NG: the second bound_prim()'s value of
+        // column g of pole row p (0: j = 0, 1: j = N2-1), from the zeroed row
+        const int p = (threadIdx.x - 1) / NG, g = (threadIdx.x - 1) % NG;
+        const int jp = p ? P.n2 - 1 : 0, lane = jp - (int)(bi.x * blockDim.x);
+        if (lane >= 0 && lane < (int)blockDim.x && pole_row(P, jp)) {
+          double v[NPR];
+#pragma unroll
+          for (int k = 0; k < NPR; k++) v[k] = edge_v[lane * NPR + k];
+          const int part = (eside ? NG + 1 : 1) + g, e = eside | (p ? 2 : 0);
+          int f2 = 0;
+          v[U2] *= 0.;
+          v[B2] *= 0.;
+          x1_value(P, gC, x1_image_column(P, src, part), jp, v, f2);
+          v[U2] *= 0.;
+          v[B2] *= 0.;
+#pragma unroll
+          for (int k = 0; k < NPR; k++) ds->pole_x1[e][g][k] = v[k];
+          ds->pole_fail[e][g] = f2;
+        }
+      }
     }
   }
   count_zone(ds, niter, failz, fail, bi);
@@ -1131,7 +1216,7 @@ __device__ __forceinline__ void update_body(KParams P, RKCoef C, const double* _
 
 // EOMC >= 0: the equations of motion as a compile-time constant (the
 // inversion and the fluxes without the other branch)
-template <int ENO, int EOMC = -1>
+template <int ENO, int EOMC = -1, int BND = 0>
 __global__ __launch_bounds__(256) void k_update(KParams P, RKCoef C, const double* __restrict__ pi,
                                                 const double* __restrict__ pb, double* __restrict__ pf,
                                                 double* __restrict__ ulast, double* __restrict__ unew,
@@ -1140,7 +1225,7 @@ __global__ __launch_bounds__(256) void k_update(KParams P, RKCoef C, const doubl
                                                 const double* __restrict__ gC, const double* __restrict__ gConn,
                                                 int* __restrict__ pflag, int* __restrict__ faillist,
                                                 DevScalars* __restrict__ ds, double* eA, const double* eB) {
-  update_body<ENO>(specialize<-1, EOMC, -1>(P), C, pi, pb, pf, ulast, unew, F1, F2, q1, q2, gC, gConn, pflag,
+  update_body<ENO, BND>(specialize<-1, EOMC, -1>(P), C, pi, pb, pf, ulast, unew, F1, F2, q1, q2, gC, gConn, pflag,
                    faillist, ds, eA, eB, dim3(blockIdx.x, blockIdx.y, blockIdx.z),
                    dim3(gridDim.x, gridDim.y, gridDim.z));
 }
@@ -1307,6 +1392,87 @@ __device__ void bound_x2_cell(const KParams& P, double* __restrict__ pr, int i, 
   }
 }
 
+// bound_prim() from the writer's side: every ghost zone whose value the
+// composed rules above (x1 first, a corner from its x1 value) take from
+// interior zone (i, j), given that zone's value v before POLEDEATH zeroes it.
+// The images come in parts, each a set of ghosts of its own: part 0 the x2
+// images of the zone (polar reflections with POLEDEATH's zeroing, periodic
+// wraps), part 1..NG the x1 ghost column -part (down side), part
+// NG+1..2NG the column N1-1+part-NG (up side), each with its own x2 images
+// (the corners).  An x1 part exists when zone i is that column's source
+// (outflow: the edge zone, through x1_value; periodic: the wrapped zone).
+// Each ghost has one source, so writers of different (zone, part) pairs
+// never collide; only x1 OUTFLOW/PERIODIC and x2 POLARAXIS/PERIODIC are
+// covered (bound_images_ok).  The zeroing of the pole row itself is the
+// caller's, after every part has read v.
+__host__ __device__ __forceinline__ bool bound_images_ok(const KParams& P) {
+  for (int s = 0; s < 2; s++) {
+    const int b1 = P.bc[s ? HARM_X1UP : HARM_X1DN], b2 = P.bc[s ? HARM_X2UP : HARM_X2DN];
+    if (b1 != HARM_OUTFLOW && b1 != HARM_PERIODIC) return false;
+    if (b2 != HARM_POLARAXIS && b2 != HARM_PERIODIC) return false;
+  }
+  return true;
+}
+__device__ __forceinline__ bool pole_row(const KParams& P, int j) {
+  return P.poledeath && ((j == 0 && P.bc[HARM_X2DN] == HARM_POLARAXIS) ||
+                         (j == P.n2 - 1 && P.bc[HARM_X2UP] == HARM_POLARAXIS));
+}
+__device__ __forceinline__ bool near_edge(const KParams& P, int i, int j) {
+  return i < NG || i >= P.n1 - NG || j < NG || j >= P.n2 - NG;
+}
+// the x2 images of (ii, j) holding value u
+__device__ __forceinline__ void bound_x2_images(const KParams& P, double* __restrict__ pv, int ii, int j,
+                                                const double* u) {
+  for (int s = 0; s < 2; s++) {
+    const int bc = s ? P.bc[HARM_X2UP] : P.bc[HARM_X2DN];
+    int jg;
+    if (bc == HARM_POLARAXIS) {
+      if (s ? (j < P.n2 - NG) : (j >= NG)) continue;
+      jg = s ? 2 * P.n2 - 1 - j : -1 - j;
+    } else {  // periodic
+      if (s ? (j >= NG) : (j < P.n2 - NG)) continue;
+      jg = s ? j + P.n2 : j - P.n2;
+    }
+    double t[NPR];
+#pragma unroll
+    for (int k = 0; k < NPR; k++) t[k] = u[k];
+    if (bc == HARM_POLARAXIS && P.poledeath) {
+      t[U2] *= 0.;
+      t[B2] *= 0.;
+    }
+    store8(pv, zidx(P, ii, jg), t);
+  }
+}
+// the x1 ghost column of image part 1..2NG if zone column i is its source, else -1000
+__device__ __forceinline__ int x1_image_column(const KParams& P, int i, int part) {
+  const bool up = part > NG;
+  const int g = up ? part - NG : part;  // 1..NG
+  const int ig = up ? P.n1 - 1 + g : -g;
+  const int bc = up ? P.bc[HARM_X1UP] : P.bc[HARM_X1DN];
+  if (bc == HARM_OUTFLOW) return (i == (up ? P.n1 - 1 : 0)) ? ig : -1000;
+  return (i == (up ? ig - P.n1 : ig + P.n1)) ? ig : -1000;  // periodic
+}
+// one part of the images of interior zone (i, j)
+__device__ void bound_image_part(const KParams& P, double* __restrict__ pv, const double* __restrict__ gC, int i,
+                                 int j, const double* v, int part, int& fail) {
+  if (part == 0) {
+    bound_x2_images(P, pv, i, j, v);
+    return;
+  }
+  const int ig = x1_image_column(P, i, part);
+  if (ig == -1000) return;
+  double w[NPR];
+#pragma unroll
+  for (int k = 0; k < NPR; k++) w[k] = v[k];
+  x1_value(P, gC, ig, j, w, fail);
+  if (pole_row(P, j)) {
+    w[U2] *= 0.;
+    w[B2] *= 0.;
+  }
+  store8(pv, zidx(P, ig, j), w);
+  bound_x2_images(P, pv, ig, j, w);
+}
+
 // bound_prim() by one workgroup: x1 faces, then x2 faces over all i (corners
 // from the x1 values), then POLEDEATH of the two interior pole rows, in the
 // reference's order (barriers between the phases)
@@ -1355,7 +1521,7 @@ __device__ __forceinline__ bool interior(const KParams& P, int i, int j) {
 // ghost whose source zone was failed by it still counts as good.
 __device__ void fixup_zone(const KParams& P, const double* __restrict__ pv, const int* __restrict__ pflag,
                            const int* __restrict__ cs, double* __restrict__ pfix, const double* __restrict__ gC,
-                           int z, int& fail) {
+                           int z, int& fail, double* keep = nullptr) {
   const int i = z / P.N2M - NG, j = z % P.N2M - NG;
   double gc[36];
 #pragma unroll
@@ -1431,15 +1597,20 @@ __device__ void fixup_zone(const KParams& P, const double* __restrict__ pv, cons
     P.acct_flag[slot] = (lg == -1) ? -1 : -2;  // states stored; k_acct_fixg makes the events
     P.acct_flag[slot + 1] = cat + 1;
   }
-  double* o = pfix + (size_t)z * 8;
+  double* o = pfix + (size_t)z * 8;  // the whole repaired state (the fields unchanged)
 #pragma unroll
-  for (int k = 0; k <= U3; k++) o[k] = pz[k];
+  for (int k = 0; k < NPR; k++) o[k] = pz[k];
+  if (keep) {
+#pragma unroll
+    for (int k = 0; k < NPR; k++) keep[k] = pz[k];
+  }
 }
 
 constexpr int KB_BOUND = 1;     // bound_prim()
 constexpr int KB_FIXUP = 2;     // post_fixup(): bound_pflag + fixup_utoprim on the failed-zone list
 constexpr int KB_BOUND2 = 8;    // bound_prim() after the fixup (step_ch.c:124)
 constexpr int KB_DT = 4;        // t,dt update (step_ch.c:174-183)
+constexpr int KB_INUPDATE = 16; // the first bound_prim() done by k_update (HARM_BOUND_INUPDATE)
 
 // step_ch.c:174-183 with advance.c:190: the new dt from the last substage's
 // Courant limits, clipped to SAFE*dt and to tf
@@ -1717,6 +1888,87 @@ __global__ __launch_bounds__(1024) void k_fixall(KParams P, double* __restrict__
   if (fail) atomicOr(&ds->failed, 1);
   if (threadIdx.x == 0) {  // every thread read nlist before the barrier
     ds->nlist = 0;
+    if (do_dt) dt_update(P, ds);
+  }
+}
+
+// The rest of a substage after a k_update that did the first bound_prim()
+// (HARM_BOUND_INUPDATE), in one workgroup.  FIX: post_fixup's repair of the
+// failed-zone list, every repaired state computed from the unrepaired array
+// (kept in LDS, the first FIXKEEP of them, else read back from pfix), then,
+// after one barrier, all at once: each repaired zone written (with
+// POLEDEATH's zeroing on a pole row), and the second bound_prim() as the
+// ghost images of what the repair and POLEDEATH can change: the repaired
+// zones' images from their repaired states, and the x1 columns of the pole
+// rows' outflow edge zones, which the reference recomputes from the zeroed
+// rows (copied from what k_update computed for them when N1 >= 5, else
+// computed here).  Every other ghost keeps the value it holds.  Without FIX
+// (no post_fixup: force-free) only the pole rows' columns, the list left as
+// it is.  Then the t,dt update after the last substage.
+constexpr int FIXKEEP = 512;
+template <bool FIX>
+__global__ __launch_bounds__(1024) void k_fixbound(KParams P, double* __restrict__ pv, int* __restrict__ pflag,
+                                                   const int* __restrict__ faillist, double* __restrict__ pfix,
+                                                   const double* __restrict__ gC, DevScalars* __restrict__ ds,
+                                                   int do_dt) {
+  __shared__ double keep[FIX ? FIXKEEP * NPR : 1];
+  __shared__ int keepz[FIX ? FIXKEEP : 1];
+  int n = FIX ? ds->nlist : 0, fail = 0;
+  if (n > P.n1 * P.n2) n = P.n1 * P.n2;
+  for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    const int z = faillist[t];
+    fixup_zone(P, pv, pflag, nullptr, pfix, gC, z, fail, t < FIXKEEP ? keep + (size_t)t * NPR : nullptr);
+    if (t < FIXKEEP) keepz[t] = z;
+  }
+  __syncthreads();  // every repaired state computed from the unrepaired array
+  auto entry = [&](int q, double* v) {
+    const int z = (q < FIXKEEP) ? keepz[q] : faillist[q];
+    if (q < FIXKEEP) {
+#pragma unroll
+      for (int k = 0; k < NPR; k++) v[k] = keep[q * NPR + k];
+    } else {
+      load8(pfix, z, v);
+    }
+    return z;
+  };
+  const int nimg = n * NIMG;
+  for (int t = threadIdx.x; t < n + nimg + 4 * NG; t += blockDim.x) {
+    double v[NPR];
+    if (t < n) {  // the repaired zone
+      const int z = entry(t, v);
+      const bool pole = pole_row(P, z % P.N2M - NG);
+#pragma unroll
+      for (int k = 0; k <= U3; k++) pv[(size_t)z * 8 + k] = (pole && k == U2) ? v[k] * 0. : v[k];
+      if (pole) pv[(size_t)z * 8 + B2] = v[B2] * 0.;
+      pflag[z] = -1;  // UTOPRIMFAILFIXED (global.h:524)
+    } else if (t < n + nimg) {  // one part of its images
+      const int q = (t - n) / NIMG, part = (t - n) % NIMG;
+      const int z = entry(q, v);
+      bound_image_part(P, pv, gC, z / P.N2M - NG, z % P.N2M - NG, v, part, fail);
+    } else {  // column g of pole-row edge zone e (e&1 the x1 side, e&2 the x2 side)
+      const int e = (t - n - nimg) / NG, g = (t - n - nimg) % NG;
+      const int i = (e & 1) ? P.n1 - 1 : 0, j = (e & 2) ? P.n2 - 1 : 0;
+      if (!pole_row(P, j) || P.bc[(e & 1) ? HARM_X1UP : HARM_X1DN] != HARM_OUTFLOW) continue;
+      const int ze = zidx(P, i, j);
+      bool repaired = false;  // then its images are the list's
+      for (int q = 0; q < n && !repaired; q++) repaired = ((q < FIXKEEP) ? keepz[q] : faillist[q]) == ze;
+      if (repaired) continue;
+      const int ig = (e & 1) ? P.n1 + g : -1 - g;
+      if (P.n1 >= 5) {  // computed by k_update from the zeroed row
+#pragma unroll
+        for (int k = 0; k < NPR; k++) v[k] = ds->pole_x1[e][g][k];
+        if (ds->pole_fail[e][g]) fail = 1;
+        store8(pv, zidx(P, ig, j), v);
+        bound_x2_images(P, pv, ig, j, v);
+      } else {
+        load8(pv, ze, v);
+        bound_image_part(P, pv, gC, i, j, v, (e & 1) ? NG + 1 + g : 1 + g, fail);
+      }
+    }
+  }
+  if (fail) atomicOr(&ds->failed, 1);
+  if (threadIdx.x == 0) {  // every thread read nlist before the barrier
+    if (FIX) ds->nlist = 0;
     if (do_dt) dt_update(P, ds);
   }
 }
@@ -2048,6 +2300,8 @@ struct harm_ctx {
   std::vector<hipEvent_t>* evpool;  // pairs (start, end) recorded without waiting
   std::vector<int>* evkid;          // kernel id of each recorded pair
   int evused;
+  int pkid;                         // stage whose kernels are being timed, -1: none
+  int perr;                         // a timed launch could not get its events
   double ktime[HARM_K_COUNT];
   long long kcount[HARM_K_COUNT];
   AcctState* acct;  // null until accounting or a dump-time reduction is first used
@@ -2125,6 +2379,8 @@ extern "C" int harm_create(const harm_params* prm, harm_ctx** out) {
   k.dx1 = prm->dx1; k.dx2 = prm->dx2;
   c->evpool = new std::vector<hipEvent_t>();
   c->evkid = new std::vector<int>();
+  c->pkid = -1;
+  c->perr = 0;
   size_t nz = (size_t)c->nz;
   hipError_t e = hipSuccess;
   for (int a = 0; a < HARM_NARRAYS && e == hipSuccess; a++) {
@@ -2356,42 +2612,71 @@ extern "C" int harm_get_time(harm_ctx* c, double* t, double* dt) {
 
 static inline dim3 grid2(int nj, int ni, dim3 b) { return dim3((nj + b.x - 1) / b.x, (ni + b.y - 1) / b.y); }
 
-// Per-kernel timing: a start/end event pair is recorded around each timed
-// launch without waiting; pairs are drained (synchronise + elapsed time) when
-// the pool fills or when the totals are read.
+// Per-kernel timing: while a timed stage is open (prof_begin .. prof_end),
+// each of its kernels is launched with its own start/stop event pair through
+// hipExtLaunchKernelGGL, whose events take the kernel's own begin and end
+// timestamps (the dispatch gap before a kernel is not counted).  A stage's
+// time is the sum over its kernels, its count one per stage call.  Pairs are
+// drained (synchronise + elapsed time) when the pool fills or when the
+// totals are read.
 static int prof_drain(harm_ctx* c) {
   if (!c->evused) return 0;
   HIPCHK(hipEventSynchronize((*c->evpool)[2 * (c->evused - 1) + 1]));
   for (int k = 0; k < c->evused; k++) {
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, (*c->evpool)[2 * k], (*c->evpool)[2 * k + 1]));
-    int kid = (*c->evkid)[k];
-    c->ktime[kid] += ms;
-    c->kcount[kid] += 1;
+    c->ktime[(*c->evkid)[k]] += ms;
   }
   c->evused = 0;
   return 0;
 }
 static int prof_begin(harm_ctx* c, int kid) {
   if (!c->prof) return 0;
-  if (c->evused == 2048 && prof_drain(c)) return 1;
+  c->pkid = kid;
+  c->kcount[kid] += 1;
+  return 0;
+}
+static int prof_end(harm_ctx* c) {
+  c->pkid = -1;
+  if (c->perr) {
+    c->perr = 0;
+    return 1;
+  }
+  return 0;
+}
+// the next event pair of the pool for a kernel of the open stage
+static bool prof_pair(harm_ctx* c, hipEvent_t* e0, hipEvent_t* e1) {
+  if (c->evused == 2048 && prof_drain(c)) return false;
   if ((int)c->evpool->size() < 2 * (c->evused + 1)) {
     hipEvent_t a, b;
-    HIPCHK(hipEventCreate(&a));
-    HIPCHK(hipEventCreate(&b));
+    if (hipEventCreate(&a) != hipSuccess) return false;
+    if (hipEventCreate(&b) != hipSuccess) {
+      (void)hipEventDestroy(a);
+      return false;
+    }
     c->evpool->push_back(a);
     c->evpool->push_back(b);
     c->evkid->push_back(0);
   }
-  (*c->evkid)[c->evused] = kid;
-  HIPCHK(hipEventRecord((*c->evpool)[2 * c->evused], c->stream));
-  return 0;
-}
-static int prof_end(harm_ctx* c) {
-  if (!c->prof) return 0;
-  HIPCHK(hipEventRecord((*c->evpool)[2 * c->evused + 1], c->stream));
+  (*c->evkid)[c->evused] = c->pkid;
+  *e0 = (*c->evpool)[2 * c->evused];
+  *e1 = (*c->evpool)[2 * c->evused + 1];
   c->evused++;
-  return 0;
+  return true;
+}
+// a launch on the context's stream, timed when a stage is open
+template <typename F, typename... A>
+static void tlaunch(harm_ctx* c, F kernel, dim3 g, dim3 b, unsigned shm, A... args) {
+  hipEvent_t e0, e1;
+  if (c->prof && c->pkid >= 0) {
+    if (prof_pair(c, &e0, &e1)) {
+      hipExtLaunchKernelGGL(kernel, g, b, shm, c->stream, e0, e1, 0, args...);
+      return;
+    }
+    if (!c->perr) seterr("per-kernel timing: event pool");
+    c->perr = 1;
+  }
+  hipLaunchKernelGGL(kernel, g, b, shm, c->stream, args...);
 }
 
 // fluxcalc for dir0..dir0+ndir-1 (both directions in one launch when ndir=2);
@@ -2405,21 +2690,20 @@ static int launch_flux(harm_ctx* c, int pr_sel, int dir0, int ndir, bool fix = t
   switch (c->flux_variant) {
     case HARM_FLUX_PAIR: {
       const dim3 b(64, 4);
-      hipLaunchKernelGGL(k_flux, dim3((nfj + 31) / 32, (nfi + 3) / 4, ndir), b, 0, c->stream, kp, c->arr[pr_sel],
+      tlaunch(c,k_flux, dim3((nfj + 31) / 32, (nfi + 3) / 4, ndir), b, 0, kp, c->arr[pr_sel],
                          c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2, c->gF, c->ds, dir0);
       break;
     }
     case HARM_FLUX_SHARED: {
       int gx, gy;
       flux_shared_grid(c->kp.n1, c->kp.n2, gx, gy);
-      hipLaunchKernelGGL(k_flux_shared, dim3(gx, gy, ndir), dim3(64, FSH_ROWS), 0, c->stream, kp,
+      tlaunch(c,k_flux_shared, dim3(gx, gy, ndir), dim3(64, FSH_ROWS), 0, kp,
                          c->arr[pr_sel], c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2, c->gF, c->ds, dir0);
       break;
     }
     default: {
       const dim3 b = (c->flux_variant == HARM_FLUX_FACE_64x2) ? dim3(64, 2) : dim3(64, 4);
-      hipLaunchKernelGGL(flux_single_kernel(kp), dim3((nfj + 63) / 64, (nfi + b.y - 1) / b.y, ndir), b, 0,
-                         c->stream, kp, c->arr[pr_sel], c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2, c->gF,
+      tlaunch(c,flux_single_kernel(kp), dim3((nfj + 63) / 64, (nfi + b.y - 1) / b.y, ndir), b, 0, kp, c->arr[pr_sel], c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2, c->gF,
                          c->ds, dir0);
       break;
     }
@@ -2432,7 +2716,7 @@ static int launch_flux(harm_ctx* c, int pr_sel, int dir0, int ndir, bool fix = t
 // along x1 A -> B, update (c2a along x2 of B; averaged Uf -> A), a2c along
 // x1 A -> B, inversion (a2c along x2 of B).  DOENO=2: the two flux
 // divergences -> A, B, then the update converts them.
-static int launch_update(harm_ctx* c, int pi, int pb, int pf, const RKCoef& C) {
+static int launch_update(harm_ctx* c, int pi, int pb, int pf, const RKCoef& C, bool bnd = false) {
   const dim3 b(64, 4);
   const dim3 gz = grid2(c->kp.n2, c->kp.n1, b);
   // the reference sets pf = pi before it reads pb zone by zone (advance.c:121,
@@ -2440,23 +2724,25 @@ static int launch_update(harm_ctx* c, int pi, int pb, int pf, const RKCoef& C) {
   const int pbs = (pf == pb) ? pi : pb;
   if (prof_begin(c, HARM_K_UPDATE)) return 1;
   if (c->kp.eno == 1) {
-    hipLaunchKernelGGL(k_eno_ucenter, gz, b, 0, c->stream, c->kp, c->arr[pi], c->gC, c->enoA);
-    hipLaunchKernelGGL(k_eno_cvt1<ENO_C2A>, gz, b, 0, c->stream, c->kp, c->enoA, c->enoB);
+    tlaunch(c,k_eno_ucenter, gz, b, 0, c->kp, c->arr[pi], c->gC, c->enoA);
+    tlaunch(c,k_eno_cvt1<ENO_C2A>, gz, b, 0, c->kp, c->enoA, c->enoB);
   } else if (c->kp.eno == 2) {
-    hipLaunchKernelGGL(k_eno_fluxdiv, gz, b, 0, c->stream, c->kp, c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2,
+    tlaunch(c,k_eno_fluxdiv, gz, b, 0, c->kp, c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2,
                        c->gC, c->enoA, c->enoB);
   }
   auto* kup = (c->kp.eno == 1)                        ? k_update<1>
               : (c->kp.eno == 2)                      ? k_update<2>
-              : (c->kp.eomtype == HARM_EOMGRMHD)      ? k_update<0, HARM_EOMGRMHD>
-              : (c->kp.eomtype == HARM_EOMFFDE)       ? k_update<0, HARM_EOMFFDE>
-                                                      : k_update<0>;
-  hipLaunchKernelGGL(kup, gz, b, 0, c->stream, c->kp, C, c->arr[pi], c->arr[pbs], c->arr[pf], c->arr[HARM_ULAST],
+              : (c->kp.eomtype == HARM_EOMGRMHD)      ? (bnd ? k_update<0, HARM_EOMGRMHD, 1>
+                                                             : k_update<0, HARM_EOMGRMHD>)
+              : (c->kp.eomtype == HARM_EOMFFDE)       ? (bnd ? k_update<0, HARM_EOMFFDE, 1>
+                                                             : k_update<0, HARM_EOMFFDE>)
+                                                      : (bnd ? k_update<0, -1, 1> : k_update<0>);
+  tlaunch(c,kup, gz, b, 0, c->kp, C, c->arr[pi], c->arr[pbs], c->arr[pf], c->arr[HARM_ULAST],
                      c->arr[HARM_UNEW], c->arr[HARM_F1], c->arr[HARM_F2], c->q1, c->q2, c->gC, c->gConn, c->pflag,
                      c->faillist, c->ds, c->enoA, c->enoB);
   if (c->kp.eno == 1) {
-    hipLaunchKernelGGL(k_eno_cvt1<ENO_A2C>, gz, b, 0, c->stream, c->kp, c->enoA, c->enoB);
-    hipLaunchKernelGGL(k_eno_invert, gz, b, 0, c->stream, c->kp, c->arr[pi], c->arr[pf], c->enoB, c->gC, c->pflag,
+    tlaunch(c,k_eno_cvt1<ENO_A2C>, gz, b, 0, c->kp, c->enoA, c->enoB);
+    tlaunch(c,k_eno_invert, gz, b, 0, c->kp, c->arr[pi], c->arr[pf], c->enoB, c->gC, c->pflag,
                        c->faillist, c->ds);
   }
   HIPCHK(hipGetLastError());
@@ -2471,8 +2757,17 @@ static void launch_bsqflags(harm_ctx* c, int sel) {
 static int launch_boundary(harm_ctx* c, int sel, int mode) {
   if (c->prm.eomtype != HARM_EOMGRMHD) mode &= ~KB_FIXUP;
   if (prof_begin(c, HARM_K_BOUNDARY)) return 1;
-  if (c->bound_variant == HARM_BOUND_BLOCK && !c->kp.checksolution) {
-    hipLaunchKernelGGL(k_boundary, dim3(1), dim3(1024), 0, c->stream, c->kp, c->arr[sel], c->gC, c->pflag,
+  if (mode & KB_INUPDATE) {  // the first bound_prim() was k_update's
+    const int do_dt = (mode & KB_DT) ? 1 : 0;
+    if (mode & KB_FIXUP) {
+      tlaunch(c, k_fixbound<true>, dim3(1), dim3(1024), 0, c->kp, c->arr[sel], c->pflag, c->faillist, c->pfix,
+              c->gC, c->ds, do_dt);
+    } else if ((mode & KB_BOUND2) || do_dt) {
+      tlaunch(c, k_fixbound<false>, dim3(1), dim3(1024), 0, c->kp, c->arr[sel], c->pflag, c->faillist, c->pfix,
+              c->gC, c->ds, do_dt);
+    }
+  } else if (c->bound_variant == HARM_BOUND_BLOCK && !c->kp.checksolution) {
+    tlaunch(c,k_boundary, dim3(1), dim3(1024), 0, c->kp, c->arr[sel], c->gC, c->pflag,
                        c->faillist, c->pfix, c->ds, mode);
   } else {
     const dim3 gx1((c->kp.n2 + 63) / 64, 2 * NG), gx2((c->kp.n1 + 2 * NG + 63) / 64, 2 * NG + 2);
@@ -2480,46 +2775,46 @@ static int launch_boundary(harm_ctx* c, int sel, int mode) {
     const bool bound2 = (mode & KB_BOUND2) != 0, fix = (mode & KB_FIXUP) != 0;
     const bool fused = (c->bound_variant == HARM_BOUND_FUSED);
     if ((mode & KB_BOUND) && fused) {
-      hipLaunchKernelGGL(k_bound, gb, dim3(256), 0, c->stream, c->kp, c->arr[sel], c->gC, c->ds, 0,
+      tlaunch(c,k_bound, gb, dim3(256), 0, c->kp, c->arr[sel], c->gC, c->ds, 0,
                          (!fix && !bound2 && (mode & KB_DT)) ? 1 : 0);
     } else if (mode & KB_BOUND) {
-      hipLaunchKernelGGL(k_bx1, gx1, dim3(64), 0, c->stream, c->kp, c->arr[sel], c->gC, c->ds, 0);
-      hipLaunchKernelGGL(k_bx2, gx2, dim3(64), 0, c->stream, c->kp, c->arr[sel], c->ds,
+      tlaunch(c,k_bx1, gx1, dim3(64), 0, c->kp, c->arr[sel], c->gC, c->ds, 0);
+      tlaunch(c,k_bx2, gx2, dim3(64), 0, c->kp, c->arr[sel], c->ds,
                          (!fix && !bound2 && (mode & KB_DT)) ? 1 : 0);
     }
     if (fix && c->kp.checksolution) {  // fixup_checksolution (fixup.c:388-488)
       const dim3 b(64, 4);
       const dim3 gi = grid2(c->kp.n2, c->kp.n1, b);
-      hipLaunchKernelGGL(k_gammacheck, grid2(c->kp.n2 + 2, c->kp.n1 + 2, b), b, 0, c->stream, c->kp, c->arr[sel],
+      tlaunch(c,k_gammacheck, grid2(c->kp.n2 + 2, c->kp.n1 + 2, b), b, 0, c->kp, c->arr[sel],
                          c->gC, c->pflag, c->bsqf, c->gammacheck);
-      hipLaunchKernelGGL(k_cs_eval, gi, b, 0, c->stream, c->kp, c->arr[sel], c->pflag, c->bsqf, c->gammacheck,
+      tlaunch(c,k_cs_eval, gi, b, 0, c->kp, c->arr[sel], c->pflag, c->bsqf, c->gammacheck,
                          c->csB, c->csA, c->ds, 1);
-      hipLaunchKernelGGL(k_cs_eval, gi, b, 0, c->stream, c->kp, c->arr[sel], c->pflag, c->bsqf, c->gammacheck,
+      tlaunch(c,k_cs_eval, gi, b, 0, c->kp, c->arr[sel], c->pflag, c->bsqf, c->gammacheck,
                          c->csA, c->csB, c->ds, 0);
-      hipLaunchKernelGGL(k_cs_sequential, dim3(1), dim3(1), 0, c->stream, c->kp, c->arr[sel], c->pflag, c->bsqf,
+      tlaunch(c,k_cs_sequential, dim3(1), dim3(1), 0, c->kp, c->arr[sel], c->pflag, c->bsqf,
                          c->gammacheck, c->csB, c->ds, c->bound_variant == HARM_BOUND_KERNELS_SEQCHECK ? 1 : 0);
-      hipLaunchKernelGGL(k_cs_apply, gi, b, 0, c->stream, c->kp, c->csB, c->pflag, c->faillist, c->ds);
+      tlaunch(c,k_cs_apply, gi, b, 0, c->kp, c->csB, c->pflag, c->faillist, c->ds);
     }
     const bool fixone = (c->bound_variant == HARM_BOUND_FIXONE) || fused;
     if (fix && fixone) {  // list repaired and cleared by one workgroup
-      hipLaunchKernelGGL(k_fixall, dim3(1), dim3(1024), 0, c->stream, c->kp, c->arr[sel], c->pflag,
+      tlaunch(c,k_fixall, dim3(1), dim3(1024), 0, c->kp, c->arr[sel], c->pflag,
                          c->kp.checksolution ? c->csB : nullptr, c->faillist, c->pfix, c->gC, c->ds,
                          (!bound2 && (mode & KB_DT)) ? 1 : 0);
     } else if (fix) {
-      hipLaunchKernelGGL(k_fixlist, dim3(32), dim3(64), 0, c->stream, c->kp, c->arr[sel], c->pflag,
+      tlaunch(c,k_fixlist, dim3(32), dim3(64), 0, c->kp, c->arr[sel], c->pflag,
                          c->kp.checksolution ? c->csB : nullptr, c->faillist, c->pfix, c->gC, c->ds);
-      hipLaunchKernelGGL(k_fixapply, dim3(32), dim3(64), 0, c->stream, c->kp, c->arr[sel], c->pflag, c->faillist,
+      tlaunch(c,k_fixapply, dim3(32), dim3(64), 0, c->kp, c->arr[sel], c->pflag, c->faillist,
                          c->pfix, c->ds);
     }
     if (bound2 && fused) {
-      hipLaunchKernelGGL(k_bound, gb, dim3(256), 0, c->stream, c->kp, c->arr[sel], c->gC, c->ds, 0,
+      tlaunch(c,k_bound, gb, dim3(256), 0, c->kp, c->arr[sel], c->gC, c->ds, 0,
                          (mode & KB_DT) ? 1 : 0);
     } else if (bound2) {
-      hipLaunchKernelGGL(k_bx1, gx1, dim3(64), 0, c->stream, c->kp, c->arr[sel], c->gC, c->ds,
+      tlaunch(c,k_bx1, gx1, dim3(64), 0, c->kp, c->arr[sel], c->gC, c->ds,
                          (fix && !fixone) ? 1 : 0);
-      hipLaunchKernelGGL(k_bx2, gx2, dim3(64), 0, c->stream, c->kp, c->arr[sel], c->ds, (mode & KB_DT) ? 1 : 0);
+      tlaunch(c,k_bx2, gx2, dim3(64), 0, c->kp, c->arr[sel], c->ds, (mode & KB_DT) ? 1 : 0);
     } else if (fix && !fixone) {
-      hipLaunchKernelGGL(k_tail, dim3(1), dim3(1), 0, c->stream, c->kp, c->ds, (mode & KB_DT) ? 1 : 0);
+      tlaunch(c,k_tail, dim3(1), dim3(1), 0, c->kp, c->ds, (mode & KB_DT) ? 1 : 0);
     }
   }
   HIPCHK(hipGetLastError());
@@ -2565,15 +2860,24 @@ static void rk_setup(int timeorder, int* nto, double CUf[4][3], double Cunew[4][
 }
 
 // advance(): fluxcalc both directions (+ fix_flux, flux_ct formed in k_update), update
-static int substage(harm_ctx* c, int pi, int pb, int pf, const double* CUf, const double* Cunew, int to, int nto) {
+static int substage(harm_ctx* c, int pi, int pb, int pf, const double* CUf, const double* Cunew, int to, int nto,
+                    bool bnd = false) {
   RKCoef C;
   C.cuf0 = CUf[0]; C.cuf1 = CUf[1]; C.cuf2 = CUf[2];
   C.cn0 = Cunew[0]; C.cn1 = Cunew[1]; C.cn2 = Cunew[2];
   C.timeorder = to;
   C.last = (to == nto - 1);
   if (launch_flux(c, pb, 1, 2)) return 1;
-  if (launch_update(c, pi, pb, pf, C)) return 1;
+  if (launch_update(c, pi, pb, pf, C, bnd)) return 1;
   return 0;
+}
+
+// HARM_BOUND_INUPDATE where it applies: the plain update (no ENO), no
+// fixup_checksolution between the fills, no accounting (which reads the
+// updated zones before bound_prim), boundary kinds the images cover
+static bool bound_in_update(const harm_ctx* c) {
+  return c->bound_variant == HARM_BOUND_INUPDATE && c->kp.eno == 0 && !c->kp.checksolution && !acct_on(c) &&
+         bound_images_ok(c->kp);
 }
 
 static int enqueue_step(harm_ctx* c) {
@@ -2588,10 +2892,11 @@ static int enqueue_step(harm_ctx* c) {
     // advance.c:138-140: diag_source on the last substage
     if (to == nto - 1 && acct && acct_source(c, pii[to], pbb[to], pff[to])) return 1;
     if (acct && acct_faces(c, pbb[to])) return 1;
-    if (substage(c, pii[to], pbb[to], pff[to], CUf[to], Cunew[to], to, nto)) return 1;
+    const bool bnd = bound_in_update(c);
+    if (substage(c, pii[to], pbb[to], pff[to], CUf[to], Cunew[to], to, nto, bnd)) return 1;
     if (acct && acct_zones(c, to == nto - 1)) return 1;
     // step_ch.c:104-124 bound_prim, post_fixup, bound_prim; then :174-183 after the last substage
-    int mode = KB_BOUND | KB_FIXUP | KB_BOUND2 | ((to == nto - 1) ? KB_DT : 0);
+    int mode = KB_BOUND | KB_FIXUP | KB_BOUND2 | ((to == nto - 1) ? KB_DT : 0) | (bnd ? KB_INUPDATE : 0);
     if (launch_boundary(c, pff[to], mode)) return 1;
     if (acct && acct_merge(c, to == nto - 1)) return 1;
   }
@@ -2648,9 +2953,9 @@ extern "C" int harm_flux_ct(harm_ctx* c) {
   if (prof_begin(c, HARM_K_EMF)) return 1;
   // the EMFs from the fluxes the context holds (edits through harm_set_array
   // included), as step_ch.c:1223-1234 reads F1, F2
-  hipLaunchKernelGGL(k_qfromF, grid2(c->kp.n2 + 2, c->kp.n1 + 2, b), b, 0, c->stream, c->kp, c->arr[HARM_F1],
+  tlaunch(c,k_qfromF, grid2(c->kp.n2 + 2, c->kp.n1 + 2, b), b, 0, c->kp, c->arr[HARM_F1],
                      c->arr[HARM_F2], c->q1, c->q2, c->gF);
-  hipLaunchKernelGGL(k_emf, grid2(c->kp.n2 + 1, c->kp.n1 + 1, b), b, 0, c->stream, c->kp, c->q1, c->q2, c->gC,
+  tlaunch(c,k_emf, grid2(c->kp.n2 + 1, c->kp.n1 + 1, b), b, 0, c->kp, c->q1, c->q2, c->gC,
                      c->emf);
   HIPCHK(hipGetLastError());
   if (prof_end(c)) return 1;
@@ -2796,9 +3101,10 @@ static int autotune(harm_ctx* c) {
   float best = 1e30f;
   int bf = HARM_FLUX_FACE_64x4, bb = HARM_BOUND_KERNELS;
   const int fv[4] = {HARM_FLUX_FACE_64x4, HARM_FLUX_FACE_64x2, HARM_FLUX_PAIR, HARM_FLUX_SHARED};
-  // the one-workgroup pass has no fixup_checksolution: it is tried last
-  const int bv[4] = {HARM_BOUND_KERNELS, HARM_BOUND_FIXONE, HARM_BOUND_FUSED, HARM_BOUND_BLOCK};
-  const int nbv = c->kp.checksolution ? 3 : 4;
+  // the one-workgroup pass has no fixup_checksolution and the fill inside
+  // the update does not apply with it: both are tried last
+  const int bv[5] = {HARM_BOUND_KERNELS, HARM_BOUND_FIXONE, HARM_BOUND_FUSED, HARM_BOUND_INUPDATE, HARM_BOUND_BLOCK};
+  const int nbv = c->kp.checksolution ? 3 : 5;
   for (int f = 0; f < 4 && !rc; f++)
     for (int v = 0; v < nbv && !rc; v++) {
       float ms = 0.f;
@@ -2964,7 +3270,7 @@ extern "C" int harm_prof_reset(harm_ctx* c) {
 extern "C" int harm_set_tuning(harm_ctx* c, int flux_variant, int bound_variant) {
   if (!c) return seterr("null ctx");
   if (flux_variant < 0 || flux_variant > HARM_FLUX_SHARED) return seterr("bad flux variant %d", flux_variant);
-  if (bound_variant < 0 || bound_variant > HARM_BOUND_FUSED)
+  if (bound_variant < 0 || bound_variant > HARM_BOUND_INUPDATE)
     return seterr("bad boundary variant %d", bound_variant);
   c->flux_variant = flux_variant;
   c->bound_variant = bound_variant;

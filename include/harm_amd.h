@@ -200,8 +200,11 @@ int harm_get_failures(harm_ctx *ctx, int *nfailzones, int *failed);
 /* Newton-iteration counter (the reference's nstroke, main.c:72-78). */
 int harm_get_nstroke(harm_ctx *ctx, long long *nstroke);
 
-/* (no reference counterpart) Per-stage timing with HIP events recorded on
-   the launch stream around each stage's launches, stage ids:
+/* (no reference counterpart) Per-stage timing with HIP events: each kernel
+   of a timed stage is launched with its own event pair on the launch stream
+   (hipExtLaunchKernelGGL; the events hold the kernel's begin and end, launch
+   gaps excluded), a stage's time is the sum over its kernels, its count one
+   per stage call (eager launches while on), stage ids:
    HARM_K_FLUX     fluxcalc, both directions in one launch
                    (k_flux_single or k_flux, see harm_set_tuning)
    HARM_K_EMF      corner EMFs + CT fluxes (standalone harm_flux_ct only)
@@ -232,6 +235,13 @@ int harm_prof_enable(harm_ctx *ctx, int on);
                                   in one workgroup (compute, barrier, write) */
 #define HARM_BOUND_FUSED 4     /* each bound_prim() in one launch, the repair
                                   in one workgroup */
+#define HARM_BOUND_INUPDATE 5  /* the first bound_prim() inside the update
+                                  (each edge zone writes its ghost images),
+                                  the repair and the second bound_prim() (the
+                                  images of what the repair and POLEDEATH can
+                                  change) in one workgroup; HARM_BOUND_FUSED
+                                  where it does not apply: ENO, checksolution,
+                                  accounting on, other boundary kinds */
 int harm_set_tuning(harm_ctx *ctx, int flux_variant, int bound_variant);
 int harm_get_tuning(harm_ctx *ctx, int *flux_variant, int *bound_variant);
 int harm_prof_get(harm_ctx *ctx, int kernel, double *total_ms, long long *count);

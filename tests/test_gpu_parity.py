@@ -263,7 +263,7 @@ def test_gpu_fullsize_ot_invariants():
 
 
 @pytest.mark.parametrize("fv", [H.FLUX_FACE_64x4, H.FLUX_FACE_64x2, H.FLUX_PAIR, H.FLUX_SHARED])
-@pytest.mark.parametrize("bv", [H.BOUND_KERNELS, H.BOUND_BLOCK, H.BOUND_FIXONE, H.BOUND_FUSED])
+@pytest.mark.parametrize("bv", [H.BOUND_KERNELS, H.BOUND_BLOCK, H.BOUND_FIXONE, H.BOUND_FUSED, H.BOUND_INUPDATE])
 def test_gpu_variants_match_oracle(fv, bv):
     say(f"[case] kernel variants flux={fv} bound={bv}: 32x64 torus, 30 steps (with repaired inversions)")
     g, geom, p0, prm = problem("ks", 32, 64)
@@ -304,7 +304,7 @@ def test_gpu_shared_flux_every_limiter(lim):
 
 
 @pytest.mark.parametrize("case", ["mink", "ffde", "ks_x1periodic_5", "ks_x1periodic_12", "ks_nopoledeath"])
-@pytest.mark.parametrize("bv", [H.BOUND_FUSED, H.BOUND_FIXONE])
+@pytest.mark.parametrize("bv", [H.BOUND_FUSED, H.BOUND_FIXONE, H.BOUND_INUPDATE])
 def test_gpu_boundary_variants_other_boundaries(case, bv):
     """The one-launch ghost fill on the other boundary combinations: periodic
     both ways, polar without POLEDEATH, and periodic x1 with polar x2 and
@@ -334,6 +334,67 @@ def test_gpu_boundary_variants_other_boundaries(case, bv):
     assert h.nstroke() == o.nstroke()
 
 
+def strong_field_at_edges(p, n1, n2):
+    """B raised by ten times the local sqrt(rho) in the three zones next to
+    both polar axes (B2) and next to the inner radial edge (B1): inversions
+    then fail on the pole rows and the edge columns, so the repair and the
+    second bound_prim() act on zones that ghost zones are copied from."""
+    s = slice(NG, NG + n1)
+    for js in (slice(NG, 2 * NG), slice(n2, n2 + NG)):
+        p[s, js, 6] += 10.0 * np.sqrt(p[s, js, 0])
+    p[NG:2 * NG, NG:NG + n2, 5] += 10.0 * np.sqrt(p[NG:2 * NG, NG:NG + n2, 0])
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.int64)
+
+
+@pytest.mark.parametrize("case", ["poledeath", "nopoledeath", "x1periodic_12", "x1periodic_5", "outflow_5",
+                                  "outflow_4"])
+def test_gpu_fill_in_update_with_failures_at_the_edges(case):
+    """HARM_BOUND_INUPDATE (the first bound_prim() written by the update's
+    edge zones, the second one as the images of the repaired zones and of
+    the pole rows' edge zones) against the oracle and, bit for bit with the
+    signs of zeros, against the one-launch fills, on a torus whose
+    inversions fail on the pole rows and at the inner edge.  N1 = 5: the up
+    edge's workgroup rows beyond the grid write its ghost columns; N1 = 4:
+    both edges in one workgroup, each edge zone writes its own columns and
+    the repair pass recomputes the pole rows' columns itself."""
+    n1 = int(case.rsplit("_", 1)[1]) if case.startswith(("x1periodic", "outflow")) else 32
+    n2 = 64
+    say(f"[case] fill in the update, {case}: torus {n1}x{n2} with failures at the edges, 6 steps")
+    g, geom, p0, prm = problem("ks", n1, n2)
+    if case == "nopoledeath":
+        prm.poledeath = 0
+    if case.startswith("x1periodic"):
+        prm.bc[H.X1UP] = prm.bc[H.X1DN] = H.PERIODIC
+    p0 = p0.copy()
+    strong_field_at_edges(p0, n1, n2)
+    a = gpu_ctx(geom, prm, p0, 1e-3)
+    a.set_tuning(H.FLUX_SHARED, H.BOUND_INUPDATE)
+    b = gpu_ctx(geom, prm, p0, 1e-3)
+    b.set_tuning(H.FLUX_SHARED, H.BOUND_FUSED)
+    o = oracle_ctx(geom, prm, p0, 1e-3)
+    pole = edge = 0
+    for s in range(6):
+        a.step_ch()
+        b.step_ch()
+        o.step()
+        assert a.get_time() == o.get_time() == b.get_time(), f"step {s + 1}"
+        pa = a.get_array(H.P)
+        assert same(pa, o.get_array(0)), f"step {s + 1}: state vs oracle"
+        assert np.array_equal(bits(pa), bits(b.get_array(H.P))), f"step {s + 1}: bits vs one-launch fills"
+        fl = o.pflag()[NG:NG + n1, NG:NG + n2, 0]
+        assert np.array_equal(a.get_pflag()[NG:NG + n1, NG:NG + n2, 0], fl), f"step {s + 1}: pflag"
+        pole += int(np.count_nonzero(fl[:, [0, n2 - 1]]))
+        edge += int(np.count_nonzero(fl[[0, n1 - 1], :]))
+    assert a.nstroke() == o.nstroke() == b.nstroke()
+    nfail, failed = a.failures()
+    assert failed == o.failed() == 0
+    say(f"  flagged zones over the run: pole rows {pole}, x1 edge columns {edge}")
+    assert pole > 0 and edge > 0, "the run is meant to repair pole-row and edge zones"
+
+
 def test_gpu_autotune_picks_a_variant():
     say("[case] autotuner on the 64x64 torus")
     g, geom, p0, prm = problem("ks", 64, 64)
@@ -342,7 +403,8 @@ def test_gpu_autotune_picks_a_variant():
     h.step_n(3)
     fv, bv = h.get_tuning()
     say(f"  chosen flux={fv} bound={bv}")
-    assert fv in (H.FLUX_FACE_64x4, H.FLUX_FACE_64x2, H.FLUX_PAIR, H.FLUX_SHARED) and bv in (H.BOUND_KERNELS, H.BOUND_BLOCK, H.BOUND_FIXONE, H.BOUND_FUSED)
+    assert fv in (H.FLUX_FACE_64x4, H.FLUX_FACE_64x2, H.FLUX_PAIR, H.FLUX_SHARED)
+    assert bv in (H.BOUND_KERNELS, H.BOUND_BLOCK, H.BOUND_FIXONE, H.BOUND_FUSED, H.BOUND_INUPDATE)
     for _ in range(3):
         o.step()
     assert same(h.get_array(H.P), o.get_array(0))
@@ -350,7 +412,8 @@ def test_gpu_autotune_picks_a_variant():
 
 
 @pytest.mark.parametrize("fv", [H.FLUX_FACE_64x4, H.FLUX_PAIR, H.FLUX_SHARED])
-@pytest.mark.parametrize("bv", [H.BOUND_KERNELS, H.BOUND_KERNELS_SEQCHECK, H.BOUND_FIXONE, H.BOUND_FUSED])
+@pytest.mark.parametrize("bv", [H.BOUND_KERNELS, H.BOUND_KERNELS_SEQCHECK, H.BOUND_FIXONE, H.BOUND_FUSED,
+                                H.BOUND_INUPDATE])
 def test_gpu_checksolution_matches_oracle(fv, bv):
     say(f"[case] CHECKSOLUTION: field-threaded torus 32x64, 25 steps, flux={fv} bound={bv}")
     g, geom, p0, prm = problem("ks_mono", 32, 64)
