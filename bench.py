@@ -51,19 +51,23 @@ def kernel_of(stage, flux_variant):
     return {H.FLUX_PAIR: "k_flux", H.FLUX_SHARED: "k_flux_shared"}.get(flux_variant, "k_flux_single")
 
 
-def traffic_from_profiles(kernel, variants):
-    """HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE
-    (profiles/pmc_traffic.json, counters scaled by a copy kernel of known
-    size), or None if no profile of this kernel variant has been taken.  The
-    flux and update kernels' traffic does not depend on the boundary variant."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+F64_PEAK_TFLOPS = 78.6   # MI355X FP64 vector (and matrix) peak
+
+
+def profile_of(kernel, variants):
+    """The committed counter profile of `kernel` (profiles/pmc_roofline.json:
+    rocprofv3 --pmc runs of this workload with these kernel variants; bytes
+    from FETCH_SIZE and WRITE_SIZE scaled by a copy kernel of known size,
+    FP64 flops from SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes), or None if none was
+    taken for these variants."""
+    path = os.path.join(ROOT, "profiles", "pmc_roofline.json")
     if not os.path.exists(path):
         return None
     try:
         d = json.load(open(path))
-        if (d.get("variants") or {}).get("flux") != variants.get("flux"):
+        if d.get("variants") != variants:
             return None
-        return d.get("per_launch_bytes", {}).get(kernel)
+        return d.get("kernels", {}).get(kernel)
     except (OSError, ValueError):
         return None
 
@@ -622,10 +626,17 @@ def main():
     bytes_launch = BYTES_PER_UNIT[dom] * units_per_launch(dom, n1, n2)
     achieved = bytes_launch / (stages[dom]["avg_us"] * 1e-6) / 1e9
     variants = {"flux": fv, "boundary": bv}
+    prof = profile_of(kname, variants)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(kname, variants),
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": prof["traffic_bytes_per_launch"] if prof else None,
                 "kernel": kname, "avg_launch_us": round(stages[dom]["avg_us"], 3),
                 "algorithmic_bytes_per_launch": bytes_launch}
+    if prof:  # the FP64 VALU rate beside the HBM one: profiled flops per launch / live launch time
+        tf = prof["f64_flops_per_launch"] / (stages[dom]["avg_us"] * 1e-6) / 1e12
+        roofline["valu_f64"] = {"achieved": round(tf, 3), "peak": F64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                                "frac": round(tf / F64_PEAK_TFLOPS, 4),
+                                "flops_per_launch": prof["f64_flops_per_launch"]}
 
     # the same run continued with the reference's conservation accounting on
     # (diag_flux, diag_source, diag_fixup every step; not part of `value`)
