@@ -25,7 +25,8 @@ NLIMCENT, NLIMUP, NLIMDOWN = 101, 102, 103
 LAXFFLUX, HLLFLUX = 0, 1
 FLUXCTHLL, FLUXCTTOTH = 0, 1
 EOMGRMHD, EOMFFDE = 0, 1
-OUTFLOW, POLARAXIS, PERIODIC = 0, 4, 100
+OUTFLOW, FIXED, POLARAXIS, PERIODIC = 0, 3, 4, 100
+BOUNDS_FISHMON, BOUNDS_DISK = 0, 1
 X1UP, X1DN, X2UP, X2DN = 0, 1, 2, 3
 P, PK0, PK1, F1, F2, ULAST, UNEW = 0, 1, 2, 3, 4, 5, 6
 METRIC_MINK, METRIC_KS = 0, 2
@@ -47,6 +48,7 @@ class HarmParams(ctypes.Structure):
         ("defcon", ctypes.c_double), ("gammamax", ctypes.c_double), ("gammafail", ctypes.c_double),
         ("rhominlimit", ctypes.c_double), ("uuminlimit", ctypes.c_double),
         ("dx1", ctypes.c_double), ("dx2", ctypes.c_double),
+        ("boundsfile", ctypes.c_int), ("extrap", ctypes.c_int), ("disk_first", ctypes.c_int),
     ]
 
 
@@ -110,7 +112,8 @@ class HarmGrid(ctypes.Structure):
         ("a", ctypes.c_double), ("R0", ctypes.c_double), ("Rin", ctypes.c_double), ("Rout", ctypes.c_double),
         ("hslope", ctypes.c_double), ("startx1", ctypes.c_double), ("startx2", ctypes.c_double),
         ("dx1", ctypes.c_double), ("dx2", ctypes.c_double), ("rhomin", ctypes.c_double),
-        ("uumin", ctypes.c_double),
+        ("uumin", ctypes.c_double), ("Zin", ctypes.c_double), ("Zout", ctypes.c_double),
+        ("npow", ctypes.c_double),
     ]
 
 
@@ -186,7 +189,8 @@ EXPORTS = [
     "harm_adj_create", "harm_adj_destroy", "harm_adj_forward", "harm_adj_get_dts",
     "harm_adj_seed_magnetic_energy", "harm_adj_seed", "harm_adj_backward", "harm_adj_get_lambda",
     "harm_adj_get_nonsmooth",
-    "harm_grid_mks", "harm_grid_cartesian", "harm_geometry", "harm_init_fishmon",
+    "harm_grid_mks", "harm_grid_cartesian", "harm_grid_cylmink", "harm_geometry", "harm_init_fishmon",
+    "harm_set_analytic",
     "harm_init_orszag_tang", "harm_init_monopole",
     "harm_rdump_write_host", "harm_rdump_read_host", "harm_rdump_write", "harm_rdump_read",
     "harm_ener_regions", "harm_acct_setup", "harm_acct_off", "harm_acct_read", "harm_diag_flux",
@@ -262,6 +266,9 @@ def load_library(path=LIB_PATH):
         "harm_grid_mks": [ctypes.POINTER(HarmGrid), ctypes.c_int, ctypes.c_int, ctypes.c_double,
                           ctypes.c_double, ctypes.c_double, ctypes.c_double],
         "harm_grid_cartesian": [ctypes.POINTER(HarmGrid), ctypes.c_int, ctypes.c_int],
+        "harm_grid_cylmink": [ctypes.POINTER(HarmGrid), ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                              ctypes.c_double, ctypes.c_double, ctypes.c_double],
+        "harm_set_analytic": [vp, dp, dp],
         "harm_geometry": [ctypes.POINTER(HarmGrid), dp, dp, dp, dp, dp],
         "harm_init_fishmon": [ctypes.POINTER(HarmGrid), dp, dp, dp, ctypes.c_double, ctypes.c_double,
                               ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -351,6 +358,14 @@ def grid_cartesian(n1, n2):
     return g
 
 
+def grid_cylmink(n1, n2, Rin=1.0, Rout=1e3, Zin=-1e3, Zout=1e3):
+    """Cylindrical Minkowski (R, z) on the bi-exponential grid defcoord 666,
+    the reference's default problem's grid (init.c:54-60)."""
+    g = HarmGrid()
+    _chk(load_library().harm_grid_cylmink(ctypes.byref(g), n1, n2, Rin, Rout, Zin, Zout), "harm_grid_cylmink")
+    return g
+
+
 def init_fishmon(geom: Geometry, gam=4.0 / 3.0, rin=6.0, rmax=12.0, kappa=1e-3, beta=100.0,
                  randfact=4e-2, seed=7):
     g = geom.grid
@@ -378,7 +393,8 @@ def init_monopole(geom: Geometry, B0=1.0):
 
 def make_params(n1, n2, dx1, dx2, *, lim=PARA, fluxmethod=HLLFLUX, timeorder=4, bc=None, fixupflux=1,
                 poledeath=1, checksolution=0, eno=0, gam=4.0 / 3.0, cour=0.9, safe=1.3, defcon=1.0, gammamax=25.0,
-                gammafail=None, rhominlimit=1e-20, uuminlimit=1e-20, fluxb=FLUXCTTOTH, eomtype=EOMGRMHD):
+                gammafail=None, rhominlimit=1e-20, uuminlimit=1e-20, fluxb=FLUXCTTOTH, eomtype=EOMGRMHD,
+                boundsfile=0, extrap=0, disk_first=0):
     """Parameter block with the reference's GRMHD defaults (initbase.c:119-243)."""
     prm = HarmParams()
     prm.n1, prm.n2 = n1, n2
@@ -392,6 +408,7 @@ def make_params(n1, n2, dx1, dx2, *, lim=PARA, fluxmethod=HLLFLUX, timeorder=4, 
     prm.gammafail = 100.0 * gammamax if gammafail is None else gammafail
     prm.rhominlimit, prm.uuminlimit = rhominlimit, uuminlimit
     prm.dx1, prm.dx2 = dx1, dx2
+    prm.boundsfile, prm.extrap, prm.disk_first = boundsfile, extrap, disk_first
     return prm
 
 
@@ -434,6 +451,17 @@ class Harm:
         a = np.ascontiguousarray(a, dtype=np.float64)
         assert a.shape == self.shape
         _chk(self.lib.harm_set_array(self.h, sel, _dptr(a)), "harm_set_array")
+
+    def set_analytic(self, panalytic, omegaf=None):
+        """harm_set_analytic: the analytic solution FIXED boundaries and
+        bound_disk() read (panalytic, omegafanalytic; [N1M][N2M][NPR])."""
+        pa = np.ascontiguousarray(panalytic, dtype=np.float64)
+        assert pa.shape == self.shape
+        om = None if omegaf is None else np.ascontiguousarray(omegaf, dtype=np.float64)
+        if om is not None:
+            assert om.shape == self.shape
+        _chk(self.lib.harm_set_analytic(self.h, _dptr(pa), _dptr(om) if om is not None else None),
+             "harm_set_analytic")
 
     def get_array(self, sel):
         a = np.zeros(self.shape)

@@ -31,6 +31,7 @@ struct KParams {
   int eomtype, lim, fluxmethod, fluxb;
   int bc[4];
   int fixupflux, poledeath, checksolution, eno;
+  int boundsfile, extrap;  // bounds.*.c (HARM_BOUNDS_*), the outflow fill's EXTRAP
   double gam, cour, defcon, safe, gammamax, gammafail, rhominlimit, uuminlimit;
   double dx1, dx2;
   // floor/failure accounting (diag_fixup), set only while it is on: dVF,

@@ -70,6 +70,7 @@ struct DevScalars {
   // zone was not repaired; [e][g]: e&1 the x1 side, e&2 the x2 side, column g+1
   double pole_x1[4][NG][8];
   int pole_fail[4][NG];
+  int disk_first;  // bound_disk()'s firsttime: its next call skips the symmetrisation
 };
 
 // the stripe of the calling wavefront (any spread of a launch's wavefronts)
@@ -1406,6 +1407,7 @@ __device__ void bound_x2_cell(const KParams& P, double* __restrict__ pr, int i, 
 // covered (bound_images_ok).  The zeroing of the pole row itself is the
 // caller's, after every part has read v.
 __host__ __device__ __forceinline__ bool bound_images_ok(const KParams& P) {
+  if (P.extrap != 0 || P.boundsfile != HARM_BOUNDS_FISHMON) return false;
   for (int s = 0; s < 2; s++) {
     const int b1 = P.bc[s ? HARM_X1UP : HARM_X1DN], b2 = P.bc[s ? HARM_X2UP : HARM_X2DN];
     if (b1 != HARM_OUTFLOW && b1 != HARM_PERIODIC) return false;
@@ -1973,6 +1975,191 @@ __global__ __launch_bounds__(1024) void k_fixbound(KParams P, double* __restrict
   }
 }
 
+// ---- bound_prim() with the general rules: the outflow extrapolation
+// (EXTRAP 1), FIXED sides, and bounds.c's bound_disk().  One launch per
+// phase of the reference's loops, in its order: x1 faces, x2 faces,
+// POLEDEATH, the disk.  Taken for every schedule variant when the context
+// uses any of them (general_bounds()). ----
+
+// the outflow ghost (i, j) from edge zone (ri, j) before the inflow check:
+// EXTRAP 0 a copy; EXTRAP 1 the densities and B1 scaled by gdet, the other
+// components extrapolated linearly, the up side's U1 with twice the slope
+// (bounds.fishmon.c:37-56, 194-217; bounds.c:40-56, 129-146)
+__device__ __forceinline__ void outflow_value(const KParams& P, const double* __restrict__ pv,
+                                              const double* __restrict__ gC, int i, int j, int ri, double* v) {
+  const double* src = pv + (size_t)zidx(P, ri, j) * 8;
+#pragma unroll
+  for (int k = 0; k < NPR; k++) v[k] = src[k];
+  if (!P.extrap) return;
+  const double gr = gC[(size_t)zidx(P, ri, j) * GCN + 32], gi = gC[(size_t)zidx(P, i, j) * GCN + 32];
+  const bool up = ri > 0;
+  const double dx1 = P.dx1;
+  for (int k = RHO; k <= UU; k++) v[k] = src[k] * gr / gi;
+  v[U1] = up ? src[U1] * (1. - 2 * (i - ri) * dx1) : src[U1] * (1. - (i - ri) * dx1);
+  for (int k = U2; k <= U3; k++) v[k] = up ? src[k] * (1. - (i - ri) * dx1) : src[k] * (1. + (i - ri) * dx1);
+  v[B1] = src[B1] * gr / gi;
+  for (int k = B2; k <= B3; k++) v[k] = up ? src[k] * (1. - (i - ri) * dx1) : src[k] * (1. + (i - ri) * dx1);
+}
+
+// x1 ghost columns: thread (column g, j), j over [-NG, N2+NG): outflow and
+// periodic fill interior j; FIXED copies panalytic over every j with the
+// bounds.c rules and leaves the ghosts alone with bounds.fishmon.c's
+__global__ void k_bg_x1(KParams P, double* __restrict__ pv, const double* __restrict__ gC,
+                        const double* __restrict__ pan, DevScalars* __restrict__ ds) {
+  const int nj = P.n2 + 2 * NG;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * NG * nj) return;
+  const int g = t / nj, j = t % nj - NG;
+  const bool up = g >= NG;
+  const int i = up ? P.n1 + (g - NG) : -1 - g;
+  const int bc = up ? P.bc[HARM_X1UP] : P.bc[HARM_X1DN];
+  double v[NPR];
+  int fail = 0;
+  if (bc == HARM_FIXED) {
+    if (P.boundsfile != HARM_BOUNDS_DISK) return;
+    load8(pan, zidx(P, i, j), v);
+  } else if (j < 0 || j >= P.n2) {
+    return;
+  } else if (bc == HARM_PERIODIC) {
+    load8(pv, zidx(P, up ? i - P.n1 : i + P.n1, j), v);
+  } else if (bc == HARM_OUTFLOW) {
+    outflow_value(P, pv, gC, i, j, up ? P.n1 - 1 : 0, v);
+    x1_value(P, gC, i, j, v, fail);
+  } else {
+    return;
+  }
+  store8(pv, zidx(P, i, j), v);
+  if (fail) atomicOr(&ds->failed, 1);
+}
+
+// x2 ghost rows over i in [-NG, N1+NG): periodic wrap, polar reflection
+// (bounds.fishmon.c:263-279), FIXED from panalytic (bounds.c rules)
+__global__ void k_bg_x2(KParams P, double* __restrict__ pv, const double* __restrict__ pan) {
+  const int ni = P.n1 + 2 * NG;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * NG * ni) return;
+  const int g = t / ni, i = t % ni - NG;
+  const bool up = g >= NG;
+  const int j = up ? P.n2 + (g - NG) : -1 - g;
+  const int bc = up ? P.bc[HARM_X2UP] : P.bc[HARM_X2DN];
+  double v[NPR];
+  if (bc == HARM_PERIODIC) {
+    load8(pv, zidx(P, i, up ? j - P.n2 : j + P.n2), v);
+  } else if (bc == HARM_POLARAXIS) {
+    load8(pv, zidx(P, i, up ? (P.n2 - 1) + ((P.n2 - 1) - j + 1) : 0 + (0 - j - 1)), v);
+  } else if (bc == HARM_FIXED && P.boundsfile == HARM_BOUNDS_DISK) {
+    load8(pan, zidx(P, i, j), v);
+  } else {
+    return;
+  }
+  store8(pv, zidx(P, i, j), v);
+}
+
+// POLEDEATH (bounds.fishmon.c:322-355): U2, B2 zeroed on the ghost rows and
+// the pole row of a polar side, every i
+__global__ void k_bg_pole(KParams P, double* __restrict__ pv) {
+  const int ni = P.n1 + 2 * NG;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 2 * (NG + 1) * ni) return;
+  const int r = t / ni, i = t % ni - NG;
+  const bool up = r > NG;
+  if ((up ? P.bc[HARM_X2UP] : P.bc[HARM_X2DN]) != HARM_POLARAXIS) return;
+  const int j = up ? P.n2 - 1 + (r - NG - 1) : -NG + r;
+  double* a = pv + (size_t)zidx(P, i, j) * 8;
+  a[U2] *= 0.;
+  a[B2] *= 0.;
+}
+
+constexpr int DISK_DELTAJ = 2;        // bounds.c:378
+constexpr double DISK_TSTATIC = 15.0;  // bounds.c:387
+
+// bounds.c:350-760 bound_disk(), one thread per (i, pair of zones mirrored
+// across the equator, DELTAJ pairs): after the first call the pair made
+// exactly symmetric (B1, B3, U2 antisymmetric); then per zone B1, B3 from
+// panalytic until t > TSTATIC and from the zone DELTAJ+1 off the equator
+// after, B2 from panalytic; the velocity from E_phi = 0 with the field
+// line's angular velocity omegafanalytic[.][0] (get_omegastar, init.c:739),
+// through vcon2pr, or U2 = 0 when that fails (clearing the failure flag as
+// the reference's failed=0 does).  The rows DELTAJ+1 off the equator are
+// only read, so the pairs are independent.  RDISKINNER 0, RDISKOUTER 1e30:
+// the reference's radius tests hold for every zone of a cylindrical-
+// Minkowski grid and are not restated.  N2 even (checked at harm_create).
+__global__ void k_bg_disk(KParams P, double* __restrict__ pv, const double* __restrict__ pan,
+                          const double* __restrict__ omf, const double* __restrict__ gC, int* __restrict__ pflag,
+                          DevScalars* __restrict__ ds) {
+  const int ni = P.n1 + 2 * NG;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= DISK_DELTAJ * ni) return;
+  const int i = t % ni - NG, q = t / ni;
+  const int half = P.n2 / 2;
+  const int ja = half - DISK_DELTAJ + q, jb = half - ja + (P.n2 - 1) / 2;
+  double* a = pv + (size_t)zidx(P, i, ja) * 8;
+  double* b = pv + (size_t)zidx(P, i, jb) * 8;
+  if (!ds->disk_first) {
+#pragma unroll
+    for (int k = 0; k < NPR; k++) {
+      if ((k == B1) || (k == B3) || (k == U2)) {
+        a[k] = 0.5 * (a[k] - b[k]);
+        b[k] = 0.5 * (b[k] - a[k]);
+      } else {
+        a[k] = 0.5 * (a[k] + b[k]);
+        b[k] = 0.5 * (a[k] + b[k]);
+      }
+    }
+  }
+  const double fun = (ds->t <= DISK_TSTATIC) ? 0.0 : 1.0;
+  for (int side = 0; side < 2; side++) {
+    const int j = side ? jb : ja;
+    const int rj = side ? DISK_DELTAJ + half : -(DISK_DELTAJ + 1) + half;
+    double* p = side ? b : a;
+    const double* an = pan + (size_t)zidx(P, i, j) * 8;
+    const double* rp = pv + (size_t)zidx(P, i, rj) * 8;
+    p[B1] = (1.0 - fun) * an[B1] + fun * rp[B1];
+    p[B3] = (1.0 - fun) * an[B3] + fun * rp[B3];
+    p[B2] = an[B2];
+    double gc[36];
+    load_cent(gC, zidx(P, i, j), gc);
+    GeoU g;
+    geo_from_cent(gc, g);
+    double Bcon[4], Bcov[4], Bsq;
+    Bcon[0] = 0;
+    Bcon[1] = p[B1];
+    Bcon[2] = p[B2];
+    Bcon[3] = p[B3];
+    lower(Bcon, g, Bcov);
+    Bsq = 0.0;
+    for (int k = 0; k <= 3; k++) Bsq += Bcon[k] * Bcov[k];
+    const double Om = omf[(size_t)zidx(P, i, j) * 8];
+    const double v1 = -Bcon[1] * (Bcov[0] + Om * Bcov[3]) / Bsq;
+    const double v2 = -Bcon[2] * (Bcov[0] + Om * Bcov[3]) / Bsq;
+    const double v3 = Om - Bcon[3] * (Bcov[0] + Om * Bcov[3]) / Bsq;
+    // vcon2pr(VELREL4): ucon_calc_3vel (phys.c:885-955), u^i returned
+    const double discr = +gc[0] + gc[5] * v1 * v1 + gc[10] * v2 * v2 + gc[15] * v3 * v3 +
+                         2. * (gc[1] * v1 + gc[2] * v2 + gc[3] * v3 + gc[6] * v1 * v2 + gc[7] * v1 * v3 +
+                               gc[11] * v2 * v3);
+    bool sane = true;
+    if (discr > 0.) {
+      ds->failed = 0;
+      sane = false;
+    } else {
+      const double ut = 1. / sqrt(-discr);
+      const double w1 = v1 * ut, w2 = v2 * ut, w3 = v3 * ut;
+      if (isnan(w1) || isnan(w2) || isnan(w3)) {
+        sane = false;
+      } else {
+        p[U1] = w1;
+        p[U2] = w2;
+        p[U3] = w3;
+      }
+    }
+    if (!sane) p[U2] = 0.0;
+    if (sane) pflag[zidx(P, i, j)] = 0;
+  }
+}
+
+// after a bound_disk() call: the next one symmetrises
+__global__ void k_bg_diskdone(DevScalars* __restrict__ ds) { ds->disk_first = 0; }
+
 __global__ __launch_bounds__(128) void k_utoprim_batch(KParams P, int n, const int* __restrict__ zone, const double* __restrict__ U,
                                 double* __restrict__ pr, int* __restrict__ pfl, const double* __restrict__ gC) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2301,6 +2488,8 @@ struct harm_ctx {
   std::vector<int>* evkid;          // kernel id of each recorded pair
   int evused;
   int pkid;                         // stage whose kernels are being timed, -1: none
+  double* pan;                      // panalytic [nz][8] (harm_set_analytic), FIXED and bound_disk
+  double* omf;                      // omegafanalytic [nz][8], bound_disk
   int perr;                         // a timed launch could not get its events
   double ktime[HARM_K_COUNT];
   long long kcount[HARM_K_COUNT];
@@ -2341,8 +2530,17 @@ static int check_prm(const harm_params* p) {
     return seterr("FLUXB %d not supported (FLUXCTTOTH=1, FLUXCTHLL=0)", p->fluxb);
   if (p->timeorder < 1 || p->timeorder > 4) return seterr("timeorder must be 1..4");
   for (int k = 0; k < 4; k++)
-    if (p->bc[k] != HARM_OUTFLOW && p->bc[k] != HARM_POLARAXIS && p->bc[k] != HARM_PERIODIC)
+    if (p->bc[k] != HARM_OUTFLOW && p->bc[k] != HARM_POLARAXIS && p->bc[k] != HARM_PERIODIC &&
+        p->bc[k] != HARM_FIXED)
       return seterr("boundary type %d not supported", p->bc[k]);
+  if (p->boundsfile != HARM_BOUNDS_FISHMON && p->boundsfile != HARM_BOUNDS_DISK)
+    return seterr("boundsfile must be HARM_BOUNDS_FISHMON or HARM_BOUNDS_DISK");
+  if (p->extrap != 0 && p->extrap != 1) return seterr("extrap must be 0 or 1 (EXTRAP 2 is not built)");
+  if (p->boundsfile == HARM_BOUNDS_DISK) {
+    if (p->bc[HARM_X2DN] == HARM_POLARAXIS || p->bc[HARM_X2UP] == HARM_POLARAXIS)
+      return seterr("bounds.c's polar axis (sign flips) is not built: use HARM_BOUNDS_FISHMON");
+    if (p->n2 % 2) return seterr("bound_disk needs an even n2 (an equator between two rows)");
+  }
   if ((p->bc[HARM_X1DN] == HARM_POLARAXIS) || (p->bc[HARM_X1UP] == HARM_POLARAXIS))
     return seterr("POLARAXIS is an x2 boundary");
   if ((p->bc[HARM_X2DN] == HARM_OUTFLOW) || (p->bc[HARM_X2UP] == HARM_OUTFLOW))
@@ -2377,6 +2575,7 @@ extern "C" int harm_create(const harm_params* prm, harm_ctx** out) {
   k.gammamax = prm->gammamax; k.gammafail = prm->gammafail;
   k.rhominlimit = prm->rhominlimit; k.uuminlimit = prm->uuminlimit;
   k.dx1 = prm->dx1; k.dx2 = prm->dx2;
+  k.boundsfile = prm->boundsfile; k.extrap = prm->extrap;
   c->evpool = new std::vector<hipEvent_t>();
   c->evkid = new std::vector<int>();
   c->pkid = -1;
@@ -2433,6 +2632,7 @@ extern "C" int harm_create(const harm_params* prm, harm_ctx** out) {
     for (int k = 0; k < NSTRIPE * SSTR; k++) memcpy(&h.ndt_acc[d][k], &big, 8);
   h.ndt_last[0] = h.ndt_last[1] = big;
   h.tf = 1e30;
+  h.disk_first = prm->disk_first ? 1 : 0;
   e = hipMemcpy(c->ds, &h, sizeof(h), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     seterr("harm_create: %s", hipGetErrorString(e));
@@ -2440,6 +2640,19 @@ extern "C" int harm_create(const harm_params* prm, harm_ctx** out) {
     return 1;
   }
   *out = c;
+  return 0;
+}
+
+extern "C" int harm_set_analytic(harm_ctx* c, const double* panalytic, const double* omegaf) {
+  if (!c) return seterr("null ctx");
+  if (!panalytic) return seterr("null panalytic");
+  const size_t bytes = (size_t)c->nz * 8 * sizeof(double);
+  if (!c->pan) HIPCHK(hipMalloc(&c->pan, bytes));
+  HIPCHK(hipMemcpy(c->pan, panalytic, bytes, hipMemcpyHostToDevice));
+  if (omegaf) {
+    if (!c->omf) HIPCHK(hipMalloc(&c->omf, bytes));
+    HIPCHK(hipMemcpy(c->omf, omegaf, bytes, hipMemcpyHostToDevice));
+  }
   return 0;
 }
 
@@ -2463,6 +2676,8 @@ extern "C" int harm_destroy(harm_ctx* c) {
   if (c->gC) (void)hipFree(c->gC);
   if (c->gConn) (void)hipFree(c->gConn);
   if (c->ds) (void)hipFree(c->ds);
+  if (c->pan) (void)hipFree(c->pan);
+  if (c->omf) (void)hipFree(c->omf);
   if (c->evpool) {
     for (hipEvent_t e : *c->evpool) (void)hipEventDestroy(e);
     delete c->evpool;
@@ -2754,8 +2969,44 @@ static void launch_bsqflags(harm_ctx* c, int sel) {
   hipLaunchKernelGGL(k_bsqflags, grid2(c->kp.n2, c->kp.n1, b), b, 0, c->stream, c->kp, c->arr[sel], c->gC, c->bsqf);
 }
 
+// the ghost fill needs the general rules (outflow extrapolation, a FIXED
+// side or bounds.c's disk): then every schedule variant runs them
+static bool general_bounds(const harm_ctx* c) {
+  bool fixed = false;
+  for (int b = 0; b < 4; b++) fixed = fixed || (c->prm.bc[b] == HARM_FIXED);
+  return fixed || c->prm.extrap != 0 || c->prm.boundsfile != HARM_BOUNDS_FISHMON;
+}
+
+static void launch_general_fill(harm_ctx* c, int sel) {
+  const int ni = c->kp.n1 + 2 * NG, nj = c->kp.n2 + 2 * NG;
+  tlaunch(c, k_bg_x1, dim3((2 * NG * nj + 255) / 256), dim3(256), 0, c->kp, c->arr[sel], c->gC, c->pan, c->ds);
+  tlaunch(c, k_bg_x2, dim3((2 * NG * ni + 255) / 256), dim3(256), 0, c->kp, c->arr[sel], c->pan);
+  if (c->kp.poledeath) tlaunch(c, k_bg_pole, dim3((2 * (NG + 1) * ni + 255) / 256), dim3(256), 0, c->kp, c->arr[sel]);
+  if (c->prm.boundsfile == HARM_BOUNDS_DISK) {
+    tlaunch(c, k_bg_disk, dim3((DISK_DELTAJ * ni + 255) / 256), dim3(256), 0, c->kp, c->arr[sel], c->pan, c->omf,
+            c->gC, c->pflag, c->ds);
+    tlaunch(c, k_bg_diskdone, dim3(1), dim3(1), 0, c->ds);
+  }
+}
+
 static int launch_boundary(harm_ctx* c, int sel, int mode) {
   if (c->prm.eomtype != HARM_EOMGRMHD) mode &= ~KB_FIXUP;
+  if (general_bounds(c)) {
+    bool needs_analytic = c->prm.boundsfile == HARM_BOUNDS_DISK;
+    if (needs_analytic && (!c->pan || !c->omf))
+      return seterr("bound_prim: the analytic solution is not set (harm_set_analytic)");
+    if (c->kp.checksolution && (mode & KB_FIXUP))
+      return seterr("checksolution with extrapolated, FIXED or disk boundaries is not built");
+    if (prof_begin(c, HARM_K_BOUNDARY)) return 1;
+    if (mode & KB_BOUND) launch_general_fill(c, sel);
+    if (mode & KB_FIXUP)
+      tlaunch(c, k_fixall, dim3(1), dim3(1024), 0, c->kp, c->arr[sel], c->pflag, nullptr, c->faillist, c->pfix,
+              c->gC, c->ds, 0);
+    if (mode & KB_BOUND2) launch_general_fill(c, sel);
+    if (mode & KB_DT) tlaunch(c, k_tail, dim3(1), dim3(1), 0, c->kp, c->ds, 1);
+    HIPCHK(hipGetLastError());
+    return prof_end(c);
+  }
   if (prof_begin(c, HARM_K_BOUNDARY)) return 1;
   if (mode & KB_INUPDATE) {  // the first bound_prim() was k_update's
     const int do_dt = (mode & KB_DT) ? 1 : 0;
@@ -3402,6 +3653,8 @@ extern "C" int harm_batch_create(harm_ctx* const* ctxs, int n, harm_batch** out)
       return seterr("context %d: every run of a batch must use the same TIMEORDER", r);
     if (ctxs[r]->prm.checksolution) return seterr("context %d: CHECKSOLUTION is not batched", r);
     if (ctxs[r]->prm.eno) return seterr("context %d: DOENO is not batched", r);
+    if (general_bounds(ctxs[r]))
+      return seterr("context %d: extrapolated, FIXED or disk boundaries are not batched", r);
   }
   HIPCHK(hipDeviceSynchronize());
   harm_batch* b = new harm_batch();

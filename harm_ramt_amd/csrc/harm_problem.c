@@ -7,7 +7,9 @@
  * operation order so that the values agree bit for bit where the libm calls
  * agree:
  *   - Kerr-Schild metric in modified Kerr-Schild coordinates (defcoord 0),
- *   - Minkowski metric on the unit square (Cartesian).
+ *   - Minkowski metric on the unit square (Cartesian),
+ *   - cylindrical Minkowski in (R, z) on the bi-exponential grid defcoord 666
+ *     (the reference's default problem's grid).
  * It also writes the two initial states (Fishbone-Moncrief torus,
  * Orszag-Tang vortex).  No GPU use.
  */
@@ -56,19 +58,52 @@ static void coord(const harm_grid *g, int i, int j, int loc, double *X) {
   X[0] = X[3] = 0.0;
 }
 
-/* coord.c:334-455 bl_coord(), defcoord 0 (POSDEFMETRIC=0, COORDSINGFIX=1) */
+/* coord.c:334-455 bl_coord(), defcoord 0 and 666 (POSDEFMETRIC=0,
+   COORDSINGFIX=1); for 666 r is the cylindrical R and th the height z */
 static void bl_coord(const harm_grid *g, const double *X, double *r, double *th) {
-  *r = g->R0 + exp(X[1]);
-  *th = M_PI * X[2] + ((1. - g->hslope) / 2.) * sin(2. * M_PI * X[2]);
+  if (g->defcoord == 666) {
+    *r = ((g->Rout - g->Rin) * exp(g->npow * X[1]) + g->Rin * exp(g->npow) - g->Rout) / (exp(g->npow) - 1.0);
+    if (X[2] > 0.0)
+      *th = ((g->Zout - 0) * exp(g->npow * fabs(X[2])) + 0 * exp(g->npow) - g->Zout) / (exp(g->npow) - 1.0);
+    else
+      *th = ((g->Zin - 0) * exp(g->npow * fabs(X[2])) + 0 * exp(g->npow) - g->Zin) / (exp(g->npow) - 1.0);
+  } else {
+    *r = g->R0 + exp(X[1]);
+    *th = M_PI * X[2] + ((1. - g->hslope) / 2.) * sin(2. * M_PI * X[2]);
+  }
   if (fabs(*th) < SINGSMALL) {
     if (*th >= 0) *th = SINGSMALL;
     if (*th < 0) *th = -SINGSMALL;
   }
 }
 
-/* coord.c:476-482 dxdxp_analytic(), defcoord 0 */
+/* coord.c:647-678 dxdxp_numerical (GAMMIEDERIVATIVE, DXDELTA 1e-5), the
+   Jacobian for defcoord > 10 */
+#define DXDELTA 1E-5
+static void dxdxp_numerical(const harm_grid *g, const double *X, double dxdxp[NDIM][NDIM]) {
+  for (int k = 0; k < NDIM; k++)
+    for (int j = 0; j < NDIM; j++) {
+      if ((j == 0) || (k == 0) || (j == 3) || (k == 3)) {
+        dxdxp[j][k] = (j != k) ? 0.0 : 1.0;
+      } else {
+        double Xh[NDIM], Xl[NDIM], ch[NDIM], cl[NDIM];
+        for (int l = 0; l < NDIM; l++) Xl[l] = Xh[l] = X[l];
+        Xh[k] += DXDELTA;
+        Xl[k] -= DXDELTA;
+        bl_coord(g, Xh, &ch[1], &ch[2]);
+        bl_coord(g, Xl, &cl[1], &cl[2]);
+        dxdxp[j][k] = (ch[j] - cl[j]) / (Xh[k] - Xl[k]);
+      }
+    }
+}
+
+/* coord.c:462-476 dxdxprim: coord.c:476-482 dxdxp_analytic() for defcoord 0 */
 static void dxdxprim(const harm_grid *g, const double *X, double r, double th, double dxdxp[NDIM][NDIM]) {
   (void)th;
+  if (g->defcoord > 10) {
+    dxdxp_numerical(g, X, dxdxp);
+    return;
+  }
   for (int j = 0; j < NDIM; j++)
     for (int k = 0; k < NDIM; k++) dxdxp[j][k] = 0.0;
   for (int j = 0; j < NDIM; j++) dxdxp[j][j] = 1.0;
@@ -120,6 +155,30 @@ static void gcon_ks(double a, double r, double th, double gcon[NDIM][NDIM]) {
   gcon[3][1] = gcon[1][3];
   gcon[3][2] = gcon[2][3];
   gcon[3][3] = (1. / (rho2 * s2));
+}
+
+/* metric.c:468-505 set_gcov_cylminkmetric (Mass = 0, RSTAR = 0.1) in (R, z) */
+static void gcov_cylmink(double R, double z, double gcov[NDIM][NDIM]) {
+  double Mass = 0.0, RSTAR = 0.1, phi;
+  double r = sqrt(R * R + z * z);
+  if (r < RSTAR) phi = (Mass / (2.0 * RSTAR)) * ((r / RSTAR) * (r / RSTAR) - 3.0);
+  else phi = -Mass / r;
+  gcov[0][0] = -1.0 - 2.0 * phi;
+  gcov[0][1] = 0.0;
+  gcov[0][2] = 0.0;
+  gcov[0][3] = 0.0;
+  gcov[1][0] = 0.0;
+  gcov[1][1] = 1.0 - 2.0 * phi * R * R / (r * r);
+  gcov[1][2] = -2.0 * phi * R * z / (r * r);
+  gcov[1][3] = 0.0;
+  gcov[2][0] = gcov[0][2];
+  gcov[2][1] = gcov[1][2];
+  gcov[2][2] = 1.0 - 2.0 * phi * z * z / (r * r);
+  gcov[2][3] = 0.0;
+  gcov[3][0] = gcov[0][3];
+  gcov[3][1] = gcov[1][3];
+  gcov[3][2] = gcov[2][3];
+  gcov[3][3] = R * R;
 }
 
 /* metric.c:441-466 set_gcov_minkcartmetric */
@@ -246,7 +305,8 @@ static void gcov_func(const harm_grid *g, const double *X, double gcov[NDIM][NDI
   double r, th, dxdxp[NDIM][NDIM];
   if (g->metric == HARM_METRIC_MINK) { gcov_mink(gcov); return; }
   bl_coord(g, X, &r, &th);
-  gcov_ks(g->a, r, th, gcov);
+  if (g->metric == HARM_METRIC_CYLMINK) gcov_cylmink(r, th, gcov);
+  else gcov_ks(g->a, r, th, gcov);
   dxdxprim(g, X, r, th, dxdxp);
   gcov2gcovprim(dxdxp, gcov);
 }
@@ -254,7 +314,11 @@ static void gcov_func(const harm_grid *g, const double *X, double gcov[NDIM][NDI
 /* metric.c:75-115 gcon_func(getprim=1, MCOORD, X, gcov) */
 static void gcon_func(const harm_grid *g, const double *X, const double gcov[NDIM][NDIM], double gcon[NDIM][NDIM]) {
   double r, th, dxdxp[NDIM][NDIM];
-  if (g->metric == HARM_METRIC_MINK) { gauss_jordan_inverse(gcov, gcon); return; }
+  /* no analytic gcon: the numerical inverse of the grid-coordinate gcov */
+  if (g->metric == HARM_METRIC_MINK || g->metric == HARM_METRIC_CYLMINK) {
+    gauss_jordan_inverse(gcov, gcon);
+    return;
+  }
   bl_coord(g, X, &r, &th);
   gcon_ks(g->a, r, th, gcon);
   dxdxprim(g, X, r, th, dxdxp);
@@ -308,6 +372,26 @@ int harm_grid_mks(harm_grid *g, int n1, int n2, double a, double Rout, double hs
   g->dx1 = log((Rout - R0) / (g->Rin - R0)) / n1;
   g->dx2 = 1. / n2;
   g->rhomin = 1.e-4;   /* initbase.c:207-208 */
+  g->uumin = 1.e-6;
+  return 0;
+}
+
+/* init.c:54-60 init_grid() (CYLMINKMETRIC), coord.c:134-136
+   set_coord_parms() (npow 10) and coord.c:798-804 set_points() */
+int harm_grid_cylmink(harm_grid *g, int n1, int n2, double Rin, double Rout, double Zin, double Zout) {
+  if (!g || n1 < 2 || n2 < 2) return 1;
+  memset(g, 0, sizeof(*g));
+  g->n1 = n1; g->n2 = n2;
+  g->metric = HARM_METRIC_CYLMINK;
+  g->defcoord = 666;
+  g->Rin = Rin; g->Rout = Rout; g->Zin = Zin; g->Zout = Zout;
+  g->npow = 10.0;
+  g->hslope = 0.3;   /* init_defgrid (initbase.c:112); unused by defcoord 666 */
+  g->startx1 = 0.;
+  g->startx2 = -1.0;
+  g->dx1 = 1.0 / n1;
+  g->dx2 = 2.0 / n2;
+  g->rhomin = 1.e-4;
   g->uumin = 1.e-6;
   return 0;
 }

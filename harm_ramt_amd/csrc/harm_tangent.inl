@@ -387,6 +387,7 @@ static int tl_check(const harm_ctx* c) {
   if (p.eno) return seterr("tangent-linear step: DOENO has no tangent");
   if (p.checksolution) return seterr("tangent-linear step: CHECKSOLUTION has no tangent");
   if (acct_on(c) || cur_on(c)) return seterr("tangent-linear step: accounting and the current must be off");
+  if (general_bounds(c)) return seterr("tangent-linear step: extrapolated, FIXED or disk boundaries have no tangent");
   return 0;
 }
 

@@ -54,8 +54,15 @@ extern "C" {
 /* boundary types (global.h:616-622); PERIODIC is the single-GPU stand-in for
    the reference's periodic MPI halo (init_mpi.c:305-313) */
 #define HARM_OUTFLOW 0
+#define HARM_FIXED 3
 #define HARM_POLARAXIS 4
 #define HARM_PERIODIC 100
+/* boundary files (the reference swaps bounds.*.c; harm_params.boundsfile) */
+#define HARM_BOUNDS_FISHMON 0  /* bounds.fishmon.c: outflow, polar axis with
+                                  POLEDEATH, FIXED leaves the ghosts alone */
+#define HARM_BOUNDS_DISK 1     /* bounds.c (the reference's default file):
+                                  FIXED from panalytic, outflow, and
+                                  bound_disk() after every fill */
 /* boundary slots (global.h:85-88) */
 #define HARM_X1UP 0
 #define HARM_X1DN 1
@@ -106,6 +113,12 @@ typedef struct {
   double rhominlimit;  /* RHOMINLIMIT */
   double uuminlimit;   /* UUMINLIMIT */
   double dx1, dx2;     /* dx[1], dx[2] */
+  int boundsfile;      /* HARM_BOUNDS_*: which bounds.*.c bound_prim() is */
+  int extrap;          /* EXTRAP of the outflow fill (bounds.fishmon.c:8,
+                          bounds.c:5): 0 copy, 1 gdet / linear extrapolation */
+  int disk_first;      /* bound_disk()'s function-static firsttime
+                          (bounds.c:373): 1 if its next call is the first of
+                          the process (no equatorial symmetrisation) */
 } harm_params;
 
 typedef struct harm_ctx harm_ctx;
@@ -125,6 +138,13 @@ int harm_set_stream(harm_ctx *ctx, void *hip_stream);
 int harm_set_geometry(harm_ctx *ctx, const double *gcov, const double *gcon,
                       const double *gdet, const double *conn,
                       const double *floors);
+
+/* The analytic solution the FIXED boundaries and bound_disk() read
+   (initbase.c:519-523: panalytic, the initial state after its own
+   bound_prim(); init.c:663: omegafanalytic, the field-line angular velocity
+   of each zone in component 0), both [N1M][N2M][NPR] host arrays; omegaf
+   may be NULL when bound_disk() is not used.  Copied to the device. */
+int harm_set_analytic(harm_ctx *ctx, const double *panalytic, const double *omegaf);
 
 /* Host <-> device copies of a context array (selector HARM_*). */
 int harm_set_array(harm_ctx *ctx, int sel, const double *host);
@@ -334,6 +354,7 @@ int harm_tl_energy_flux(harm_tl *tl, int i, double *L, double *dL);
 /* Grid/coordinate description (coord.c: set_points, bl_coord, dxdxprim). */
 #define HARM_METRIC_MINK 0  /* MINKMETRIC (metric.c:441) */
 #define HARM_METRIC_KS 2    /* KSCOORDS (metric.c:540,666) */
+#define HARM_METRIC_CYLMINK 4  /* CYLMINKMETRIC (metric.h:7, metric.c:468) */
 typedef struct {
   int n1, n2;
   int metric;          /* HARM_METRIC_* */
@@ -342,6 +363,8 @@ typedef struct {
   double R0, Rin, Rout, hslope;
   double startx1, startx2, dx1, dx2;
   double rhomin, uumin;  /* floor coefficients RHOMIN, UUMIN */
+  double Zin, Zout, npow;  /* defcoord 666: cylindrical height range and the
+                              exponential rate (coord.c:134-136, 421-431) */
 } harm_grid;
 
 /* MKS grid for a Kerr hole as the reference's init_grid() does for the torus:
@@ -350,6 +373,11 @@ int harm_grid_mks(harm_grid *g, int n1, int n2, double a, double Rout,
                   double hslope, double R0);
 /* Unit-square Cartesian grid with Minkowski metric (X in [0,1)^2). */
 int harm_grid_cartesian(harm_grid *g, int n1, int n2);
+/* init.c:54-60 init_grid() for CYLMINKMETRIC: cylindrical Minkowski in
+   (R, z) on the bi-exponential grid defcoord 666 (coord.c:421-431, 798-804),
+   R in [Rin, Rout], z in [Zin, Zout]; the metric's conversion to the grid
+   coordinates uses the numerical Jacobian (coord.c:647-678) */
+int harm_grid_cylmink(harm_grid *g, int n1, int n2, double Rin, double Rout, double Zin, double Zout);
 /* Geometry tables of set_grid() (set_grid.c:6-88) for the grid. */
 int harm_geometry(const harm_grid *g, double *gcov, double *gcon, double *gdet,
                   double *conn, double *floors);

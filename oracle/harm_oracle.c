@@ -83,6 +83,9 @@ struct oracle_ctx {
   double *jcon;      /* [nz][4] */
   long long calls;   /* current_calc_1's statics */
   double lastdt;
+  double *panalytic; /* [nz][8], FIXED boundaries and bound_disk */
+  double *omegaf;    /* [nz][8], omegafanalytic (component 0 used) */
+  int disk_first;    /* bound_disk's static firsttime */
 };
 
 typedef struct {
@@ -794,10 +797,28 @@ static int Utoprim_2d_final(const oracle_ctx *c, const double U[NPR], const geom
    B^i in U[B1..B3].  pr[RHO], pr[UU] are not touched.  ACCOUNTDISS only feeds
    diagnostics (diss_account, phys.ffde.c:572-607) and is left out.
    Returns 1 on the space-like failure of vcon2pr (phys.ffde.c:309-312). */
+/* vcon2pr(VELREL4) -> ucon_calc_3vel (transforms.c:348-396, phys.c:885-955):
+   u^t from the normalisation of the coordinate velocity vcon; the reference
+   returns u^i itself here (transforms.c:381-386).  1 if vcon is space-like. */
+static int vcon2pr_rel4(const geom_t *g, const double vcon[NDIM], double *pr) {
+  double discr, ucon[NDIM];
+  const double (*gc)[NDIM] = g->gcov;
+  discr = +gc[0][0] + gc[1][1] * vcon[1] * vcon[1] + gc[2][2] * vcon[2] * vcon[2] + gc[3][3] * vcon[3] * vcon[3] +
+          2. * (gc[0][1] * vcon[1] + gc[0][2] * vcon[2] + gc[0][3] * vcon[3] + gc[1][2] * vcon[1] * vcon[2] +
+                gc[1][3] * vcon[1] * vcon[3] + gc[2][3] * vcon[2] * vcon[3]);
+  if (discr > 0.) return 1;
+  ucon[0] = 1. / sqrt(-discr);
+  for (int j = 1; j < NDIM; j++) ucon[j] = vcon[j] * ucon[0];
+  pr[U1] = ucon[1];
+  pr[U2] = ucon[2];
+  pr[U3] = ucon[3];
+  return 0;
+}
+
 static int Utoprim_ffde(const oracle_ctx *c, const double U[NPR], const geom_t *g, double *pr) {
   double alphasq, BconOalpha[NDIM], BcovOalpha[NDIM], BsqOalphasq, T0[NDIM];
   double EconOalpha[NDIM], EcovOalpha[NDIM], EsqOalphasq, BsqmEsqOalphasq, realBsqOalphasq;
-  double gammasq, Gsq, oBsqOalphasqgeom, oBsqgeom, vcon[NDIM], discr, ucon[NDIM];
+  double gammasq, Gsq, oBsqOalphasqgeom, oBsqgeom, vcon[NDIM];
   int j;
   alphasq = 1. / (-g->gcon[0][0]);
   /* U2BBsq (phys.ffde.c:1261-1283) */
@@ -829,21 +850,7 @@ static int Utoprim_ffde(const oracle_ctx *c, const double U[NPR], const geom_t *
   vcon[1] = alphasq * (-g->gcon[0][1] + (EcovOalpha[2] * BcovOalpha[3] - EcovOalpha[3] * BcovOalpha[2]) * oBsqgeom);
   vcon[2] = alphasq * (-g->gcon[0][2] + (EcovOalpha[3] * BcovOalpha[1] - EcovOalpha[1] * BcovOalpha[3]) * oBsqgeom);
   vcon[3] = alphasq * (-g->gcon[0][3] + (EcovOalpha[1] * BcovOalpha[2] - EcovOalpha[2] * BcovOalpha[1]) * oBsqgeom);
-  /* vcon2pr(VELREL4) -> ucon_calc_3vel (transforms.c:348-396, phys.c:885-955);
-     the reference returns u^i itself here (transforms.c:381-386) */
-  {
-    const double (*gc)[NDIM] = g->gcov;
-    discr = +gc[0][0] + gc[1][1] * vcon[1] * vcon[1] + gc[2][2] * vcon[2] * vcon[2] +
-            gc[3][3] * vcon[3] * vcon[3] +
-            2. * (gc[0][1] * vcon[1] + gc[0][2] * vcon[2] + gc[0][3] * vcon[3] + gc[1][2] * vcon[1] * vcon[2] +
-                  gc[1][3] * vcon[1] * vcon[3] + gc[2][3] * vcon[2] * vcon[3]);
-  }
-  if (discr > 0.) return 1;
-  ucon[0] = 1. / sqrt(-discr);
-  for (j = 1; j < NDIM; j++) ucon[j] = vcon[j] * ucon[0];
-  pr[U1] = ucon[1];
-  pr[U2] = ucon[2];
-  pr[U3] = ucon[3];
+  if (vcon2pr_rel4(g, vcon, pr)) return 1;
   pr[B1] = U[B1];
   pr[B2] = U[B2];
   pr[B3] = U[B3];
@@ -1187,19 +1194,127 @@ static int inflow_check_rel4vel_x1(const oracle_ctx *c, double *pr, const geom_t
   return -1;
 }
 
-/* ghost fill: bounds.fishmon.c:13-355 (outflow EXTRAP=0 + polar axis + POLEDEATH)
-   and periodic wrap (the single-rank equivalent of the reference's MPI halo,
-   boundmpi.c:5-130: x1 faces first, then x2 faces over the full x1 range) */
+/* the outflow ghost (i, j) from edge zone (ri, j): EXTRAP 0 copies,
+   EXTRAP 1 scales the densities and B1 by gdet and the other components
+   linearly (bounds.fishmon.c:37-56 and 194-217, bounds.c:40-56 and 129-146;
+   the up side's U1 with twice the slope) */
+static void outflow_value(const oracle_ctx *c, double *pr, int i, int j, int ri) {
+  const double *src = PA(c, pr, ri, j);
+  double *dst = PA(c, pr, i, j);
+  if (!c->p.extrap) {
+    memcpy(dst, src, sizeof(double) * NPR);
+    return;
+  }
+  const double gr = c->gdet[(size_t)zi(c, ri, j) * 4 + CENT], gi = c->gdet[(size_t)zi(c, i, j) * 4 + CENT];
+  const double dx1 = c->p.dx1;
+  const int up = ri > 0;
+  for (int k = RHO; k <= UU; k++) dst[k] = src[k] * gr / gi;
+  dst[U1] = up ? src[U1] * (1. - 2 * (i - ri) * dx1) : src[U1] * (1. - (i - ri) * dx1);
+  for (int k = U2; k <= U3; k++) dst[k] = up ? src[k] * (1. - (i - ri) * dx1) : src[k] * (1. + (i - ri) * dx1);
+  dst[B1] = src[B1] * gr / gi;
+  for (int k = B2; k <= B3; k++) dst[k] = up ? src[k] * (1. - (i - ri) * dx1) : src[k] * (1. + (i - ri) * dx1);
+}
+
+#define DISK_DELTAJ 2      /* bounds.c:378 */
+#define DISK_TSTATIC 15.0  /* bounds.c:387 */
+/* bounds.c:350-760 bound_disk(): the equatorial conductor of the disk
+   problem, on the DELTAJ zones each side of the equator, every i (ghosts
+   included): after the first call, the state made exactly symmetric across
+   the equator (B1, B3, U2 antisymmetric); B1, B3 from panalytic until
+   t > TSTATIC, then from the zone DELTAJ+1 away from the equator; B2 from
+   panalytic; the velocity from E_phi = 0 with the field line's angular
+   velocity omegafanalytic (get_omegastar, init.c:739, PROBLEMTYPE 7) through
+   vcon2pr, or U2 = 0 if that fails (which also clears the failure flag, as
+   the reference's failed=0 does).  RDISKINNER = 0 and RDISKOUTER = 1e30
+   make the reference's radius tests true for every zone with 0 < r < 1e30,
+   which holds on every cylindrical-Minkowski grid (R > (Rin e^npow -
+   Rout)/(e^npow - 1) > 0), and its RNEARBLACKHOLE branch (r <= 0) is never
+   taken: neither is restated.  Needs an even N2. */
+static void bound_disk(oracle_ctx *c, double *pr) {
+  const int n1 = c->p.n1, n2 = c->p.n2, half = n2 / 2;
+  if (!c->disk_first) {
+    for (int j = -NG; j < n2 + NG; j++)
+      for (int i = -NG; i < n1 + NG; i++) {
+        if (!((j - half < 0) && (j - half >= -DISK_DELTAJ))) continue;
+        const int rj = half - j + (n2 - 1) / 2;
+        double *a = PA(c, pr, i, j), *b = PA(c, pr, i, rj);
+        for (int k = 0; k < NPR; k++) {
+          if ((k == B1) || (k == B3) || (k == U2)) {
+            a[k] = 0.5 * (a[k] - b[k]);
+            b[k] = 0.5 * (b[k] - a[k]);
+          } else {
+            a[k] = 0.5 * (a[k] + b[k]);
+            b[k] = 0.5 * (a[k] + b[k]);
+          }
+        }
+      }
+  }
+  double fun = (c->t <= DISK_TSTATIC) ? 0.0 : 1.0;
+  for (int j = -NG; j < n2 + NG; j++)
+    for (int i = -NG; i < n1 + NG; i++) {
+      int rj;
+      if ((j - half < 0) && (j - half >= -DISK_DELTAJ)) rj = -(DISK_DELTAJ + 1) + half;
+      else if ((j - half <= DISK_DELTAJ - 1) && (j - half >= 0)) rj = DISK_DELTAJ + half;
+      else continue;
+      double *a = PA(c, pr, i, j);
+      const double *an = PA(c, c->panalytic, i, j);
+      a[B1] = (1.0 - fun) * an[B1] + fun * PA(c, pr, i, rj)[B1];
+      a[B3] = (1.0 - fun) * an[B3] + fun * PA(c, pr, i, rj)[B3];
+      a[B2] = an[B2];
+    }
+  for (int j = -NG; j < n2 + NG; j++)
+    for (int i = -NG; i < n1 + NG; i++) {
+      if (!((j - half <= DISK_DELTAJ - 1) && (j - half >= -DISK_DELTAJ))) continue;
+      geom_t g;
+      get_geometry(c, i, j, CENT, &g);
+      double *a = PA(c, pr, i, j);
+      double Bcon[NDIM], Bcov[NDIM], Bsq, vcon[NDIM], primtest[NPR];
+      Bcon[0] = 0;
+      Bcon[1] = a[B1];
+      Bcon[2] = a[B2];
+      Bcon[3] = a[B3];
+      lower(Bcon, &g, Bcov);
+      Bsq = 0.0;
+      for (int k = 0; k <= 3; k++) Bsq += Bcon[k] * Bcov[k];
+      const double Om = c->omegaf[(size_t)zi(c, i, j) * NPR + 0];
+      vcon[1] = -Bcon[1] * (Bcov[0] + Om * Bcov[3]) / Bsq;
+      vcon[2] = -Bcon[2] * (Bcov[0] + Om * Bcov[3]) / Bsq;
+      vcon[3] = Om - Bcon[3] * (Bcov[0] + Om * Bcov[3]) / Bsq;
+      int sanity = 1;
+      if (vcon2pr_rel4(&g, vcon, primtest)) {
+        c->failed = 0;
+        sanity = 0;
+      } else {
+        for (int k = U1; k <= U3; k++)
+          if (isnan(primtest[k])) sanity = 0;
+        if (sanity) {
+          a[U1] = primtest[U1];
+          a[U2] = primtest[U2];
+          a[U3] = primtest[U3];
+        }
+      }
+      if (!sanity) a[U2] = 0.0;
+      if (sanity) c->pflag[(size_t)zi(c, i, j) * NUMPFLAGS + 0] = 0;
+    }
+  c->disk_first = 0;
+}
+
+/* ghost fill: bounds.fishmon.c:13-355 (outflow, polar axis + POLEDEATH; a
+   FIXED side left alone) or bounds.c:11-343 (outflow, FIXED from panalytic,
+   then bound_disk), and the periodic wrap (the single-rank equivalent of
+   the reference's MPI halo, boundmpi.c:5-130: x1 faces first, then x2
+   faces over the full x1 range) */
 int or_bound_prim(oracle_ctx *c, int sel) {
   double *pr = c->arr[sel];
   int n1 = c->p.n1, n2 = c->p.n2, fail = 0;
+  const int fixed_from_analytic = (c->p.boundsfile == 1);
   /* x1 */
   if (c->p.bc[OR_X1DN] == OR_PERIODIC) {
     for (int j = 0; j < n2; j++)
       for (int i = -NG; i < 0; i++) memcpy(PA(c, pr, i, j), PA(c, pr, i + n1, j), sizeof(double) * NPR);
   } else if (c->p.bc[OR_X1DN] == OR_OUTFLOW) {
     for (int j = 0; j < n2; j++) {
-      for (int i = -1; i >= -NG; i--) memcpy(PA(c, pr, i, j), PA(c, pr, 0, j), sizeof(double) * NPR);
+      for (int i = -1; i >= -NG; i--) outflow_value(c, pr, i, j, 0);
       for (int i = -1; i >= -NG; i--) {
         geom_t g;
         get_geometry(c, i, j, CENT, &g);
@@ -1207,13 +1322,16 @@ int or_bound_prim(oracle_ctx *c, int sel) {
         if (limit_gamma(c->p.gammamax, PA(c, pr, i, j), &g) >= 1) fail = 1;
       }
     }
+  } else if (c->p.bc[OR_X1DN] == OR_FIXED && fixed_from_analytic) {
+    for (int j = -NG; j < n2 + NG; j++)
+      for (int i = -1; i >= -NG; i--) memcpy(PA(c, pr, i, j), PA(c, c->panalytic, i, j), sizeof(double) * NPR);
   }
   if (c->p.bc[OR_X1UP] == OR_PERIODIC) {
     for (int j = 0; j < n2; j++)
       for (int i = n1; i < n1 + NG; i++) memcpy(PA(c, pr, i, j), PA(c, pr, i - n1, j), sizeof(double) * NPR);
   } else if (c->p.bc[OR_X1UP] == OR_OUTFLOW) {
     for (int j = 0; j < n2; j++) {
-      for (int i = n1; i <= n1 - 1 + NG; i++) memcpy(PA(c, pr, i, j), PA(c, pr, n1 - 1, j), sizeof(double) * NPR);
+      for (int i = n1; i <= n1 - 1 + NG; i++) outflow_value(c, pr, i, j, n1 - 1);
       for (int i = n1; i <= n1 - 1 + NG; i++) {
         geom_t g;
         get_geometry(c, i, j, CENT, &g);
@@ -1221,6 +1339,10 @@ int or_bound_prim(oracle_ctx *c, int sel) {
         if (limit_gamma(c->p.gammamax, PA(c, pr, i, j), &g) >= 1) fail = 1;
       }
     }
+  } else if (c->p.bc[OR_X1UP] == OR_FIXED && fixed_from_analytic) {
+    for (int j = -NG; j < n2 + NG; j++)
+      for (int i = n1; i <= n1 - 1 + NG; i++)
+        memcpy(PA(c, pr, i, j), PA(c, c->panalytic, i, j), sizeof(double) * NPR);
   }
   /* x2 */
   if (c->p.bc[OR_X2DN] == OR_PERIODIC) {
@@ -1229,6 +1351,9 @@ int or_bound_prim(oracle_ctx *c, int sel) {
   } else if (c->p.bc[OR_X2DN] == OR_POLARAXIS) {
     for (int i = -NG; i <= n1 - 1 + NG; i++)
       for (int j = -NG; j <= -1; j++) memcpy(PA(c, pr, i, j), PA(c, pr, i, 0 + (0 - j - 1)), sizeof(double) * NPR);
+  } else if (c->p.bc[OR_X2DN] == OR_FIXED && fixed_from_analytic) {
+    for (int i = -NG; i <= n1 - 1 + NG; i++)
+      for (int j = -NG; j <= -1; j++) memcpy(PA(c, pr, i, j), PA(c, c->panalytic, i, j), sizeof(double) * NPR);
   }
   if (c->p.bc[OR_X2UP] == OR_PERIODIC) {
     for (int i = -NG; i < n1 + NG; i++)
@@ -1237,6 +1362,9 @@ int or_bound_prim(oracle_ctx *c, int sel) {
     for (int i = -NG; i <= n1 - 1 + NG; i++)
       for (int j = n2; j <= n2 - 1 + NG; j++)
         memcpy(PA(c, pr, i, j), PA(c, pr, i, (n2 - 1) + ((n2 - 1) - j + 1)), sizeof(double) * NPR);
+  } else if (c->p.bc[OR_X2UP] == OR_FIXED && fixed_from_analytic) {
+    for (int i = -NG; i <= n1 - 1 + NG; i++)
+      for (int j = n2; j <= n2 - 1 + NG; j++) memcpy(PA(c, pr, i, j), PA(c, c->panalytic, i, j), sizeof(double) * NPR);
   }
   if (c->p.poledeath) {
     if (c->p.bc[OR_X2DN] == OR_POLARAXIS)
@@ -1246,8 +1374,17 @@ int or_bound_prim(oracle_ctx *c, int sel) {
       for (int i = -NG; i <= n1 - 1 + NG; i++)
         for (int j = n2 - 1; j <= n2 - 1 + NG; j++) { PA(c, pr, i, j)[U2] *= 0.; PA(c, pr, i, j)[B2] *= 0.; }
   }
+  /* bounds.c:330-339: the disk's equatorial conditions after the fills */
+  if (c->p.boundsfile == 1) bound_disk(c, pr);
   if (fail) c->failed = 1;
   return fail;
+}
+
+void or_set_analytic(oracle_ctx *c, const double *panalytic, const double *omegaf) {
+  if (!c->panalytic) c->panalytic = (double *)calloc((size_t)c->nz * NPR, sizeof(double));
+  if (!c->omegaf) c->omegaf = (double *)calloc((size_t)c->nz * NPR, sizeof(double));
+  memcpy(c->panalytic, panalytic, sizeof(double) * (size_t)c->nz * NPR);
+  if (omegaf) memcpy(c->omegaf, omegaf, sizeof(double) * (size_t)c->nz * NPR);
 }
 
 /* boundsint.c:4-60 bound_pflag (FLAGUTOPRIMFAIL only): x1 outflow copies over
@@ -1269,10 +1406,11 @@ static void bound_pflag(oracle_ctx *c) {
       for (int i = -NG; i < 0; i++) PFC(i, j, i + n1, j);
       for (int i = n1; i < n1 + NG; i++) PFC(i, j, i - n1, j);
     }
-  if (c->p.bc[OR_X2DN] == OR_POLARAXIS)
+  /* the polar reflection is unconditional in the reference (boundsint.c:37-52) */
+  if (c->p.bc[OR_X2DN] != OR_PERIODIC)
     for (int i = -NG; i <= n1 - 1 + NG; i++)
       for (int j = -NG; j <= -1; j++) PFC(i, j, i, 0 + (0 - j - 1));
-  if (c->p.bc[OR_X2UP] == OR_POLARAXIS)
+  if (c->p.bc[OR_X2UP] != OR_PERIODIC)
     for (int i = -NG; i <= n1 - 1 + NG; i++)
       for (int j = n2; j <= n2 - 1 + NG; j++) PFC(i, j, i, (n2 - 1) + ((n2 - 1) - j + 1));
   if (c->p.bc[OR_X2DN] == OR_PERIODIC)
@@ -1580,6 +1718,7 @@ oracle_ctx *or_create(const oracle_params *prm) {
   c->frdot = (double *)calloc((size_t)prm->n1 * NPR, sizeof(double));
   c->lumvsr = (double *)calloc((size_t)prm->n1, sizeof(double));
   c->tf = 1e30;
+  c->disk_first = prm->disk_first;
 #ifdef _OPENMP
   if (prm->nthreads > 0) omp_set_num_threads(prm->nthreads);
 #endif
@@ -1594,6 +1733,7 @@ void or_destroy(oracle_ctx *c) {
   free(c->pflag);
   free(c->frdot); free(c->lumvsr);
   free(c->cf); free(c->fcon); free(c->jcon);
+  free(c->panalytic); free(c->omegaf);
   free(c);
 }
 

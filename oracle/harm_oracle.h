@@ -27,7 +27,7 @@ enum { OR_DONOR = 0, OR_VANL = 1, OR_MINM = 2, OR_MC = 3, OR_PARA = 4,
        OR_CSSLOPE = 90, OR_NLIMCENT = 101, OR_NLIMUP = 102, OR_NLIMDOWN = 103 };
 enum { OR_LAXF = 0, OR_HLL = 1 };
 enum { OR_FLUXCTHLL = 0, OR_FLUXCTTOTH = 1, OR_FLUXCD = 2 };
-enum { OR_OUTFLOW = 0, OR_POLARAXIS = 4, OR_PERIODIC = 100 };
+enum { OR_OUTFLOW = 0, OR_FIXED = 3, OR_POLARAXIS = 4, OR_PERIODIC = 100 };
 enum { OR_X1UP = 0, OR_X1DN = 1, OR_X2UP = 2, OR_X2DN = 3 };
 enum { OR_GRMHD = 0, OR_FFDE = 1 };
 
@@ -42,6 +42,9 @@ typedef struct {
   double gam, cour, safe, defcon, gammamax, gammafail;
   double rhominlimit, uuminlimit;
   double dx1, dx2;
+  int boundsfile;       /* 0 bounds.fishmon.c, 1 bounds.c (FIXED from panalytic, bound_disk) */
+  int extrap;           /* EXTRAP of the outflow fill (bounds.fishmon.c:8, bounds.c:5) */
+  int disk_first;       /* bound_disk's static firsttime (bounds.c:373) */
   int nthreads;         /* OpenMP threads, 0 = default */
 } oracle_params;
 
@@ -71,6 +74,9 @@ int or_advance(oracle_ctx *c, int pi_sel, int pb_sel, const double *CUf,
                int pf_sel, const double *Cunew, int timeorder,
                int numtimeorders, double *ndt);
 int or_bound_prim(oracle_ctx *c, int sel);
+/* panalytic and omegafanalytic ([N1M][N2M][NPR], omegaf may be NULL): the
+   analytic solution FIXED boundaries and bound_disk read (copied) */
+void or_set_analytic(oracle_ctx *c, const double *panalytic, const double *omegaf);
 int or_post_fixup(oracle_ctx *c, int sel);
 int or_step(oracle_ctx *c);
 

@@ -24,6 +24,7 @@ class OracleParams(ctypes.Structure):
         ("gam", ctypes.c_double), ("cour", ctypes.c_double), ("safe", ctypes.c_double), ("defcon", ctypes.c_double),
         ("gammamax", ctypes.c_double), ("gammafail", ctypes.c_double), ("rhominlimit", ctypes.c_double),
         ("uuminlimit", ctypes.c_double), ("dx1", ctypes.c_double), ("dx2", ctypes.c_double),
+        ("boundsfile", ctypes.c_int), ("extrap", ctypes.c_int), ("disk_first", ctypes.c_int),
         ("nthreads", ctypes.c_int),
     ]
 
@@ -100,6 +101,7 @@ def load():
         "or_current": ([vp], None),
         "or_current_get": ([vp, dp, dp], None),
         "or_dump_content": ([vp, dp], None),
+        "or_set_analytic": ([vp, dp, dp], None),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -139,6 +141,11 @@ class Oracle:
         if getattr(self, "h", None):
             self.lib.or_destroy(self.h)
             self.h = None
+
+    def set_analytic(self, panalytic, omegaf=None):
+        pa = np.ascontiguousarray(panalytic, dtype=np.float64)
+        om = None if omegaf is None else np.ascontiguousarray(omegaf, dtype=np.float64)
+        self.lib.or_set_analytic(self.h, _d(pa), _d(om) if om is not None else None)
 
     def set_geometry(self, gcov, gcon, gdet, conn, floors):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (gcov, gcon, gdet, conn, floors)]

@@ -18,6 +18,12 @@
  * With "rdump 1" it also writes the final state's restart files, with
  * "dump 1" its 56-column dump, with "diag 1" the conservation accumulators
  * and dump-time reductions (see the end of main()).
+ *
+ * Built with HARNESS_REFINIT it runs the reference's own init() instead
+ * (initbase.c:297-560 with the compiled problem file's init_grid(),
+ * init_global(), initial state and analytic solution), writes what it set up
+ * (geometry tables, the state, panalytic, omegafanalytic) and then steps from
+ * it: the reference's default problem end to end.
  */
 #include "defs.h"
 #include "step_ch.h"
@@ -68,6 +74,74 @@ static void readall(const char *path, void *buf, size_t bytes)
   fclose(f);
 }
 
+#ifdef HARNESS_REFINIT
+/* The reference's init() (initbase.c:297-560) for a fresh start, in its
+   order, with the reference's own functions.  Two things differ: MPI is
+   initialised first (pre_init() calls makedirs(), whose MPI_Barrier comes
+   before its own init_mpi() and aborts under MPICH, initbase.c:13-54 with
+   diag.c:928-940), and the dump files init() writes (dump(9997..9999)) and
+   the diagnostics set-up of post_init() (initbase.c:621-660: horizon,
+   flux regions, the current's history) are skipped -- they write files or
+   diagnostics and do not touch the state. */
+static int refinit(char **margv)
+{
+  extern void filterffde(int i, int j, FTYPE *pr);
+  extern int init_vpot2field(SFTYPE A[][N2M], FTYPE pr[][N2M][NPR]);
+  extern int init_postfield(FTYPE pr[][N2M][NPR]);
+  extern int normalize_densities(FTYPE p[][N2M][NPR]);
+  extern int normalize_field(FTYPE p[][N2M][NPR]);
+  extern int init_dsandvels(int *whichvel, int *whichcoord, int ii, int jj, FTYPE *pr);
+  extern int init_vpot(int ii, int jj, FTYPE *A);
+  extern int pre_init_specific_init(void);
+  extern int post_init_specific_init(void);
+  extern int init_dumps(void);
+  int whichvel, whichcoord, i, j, k;
+  FTYPE (*A)[N2M];
+
+  ranc(0);
+  binaryoutput = TEXTOUTPUT;
+  sortedoutput = SORTED;
+  init_mpi(3, margv);
+  set_arrays();
+  init_dumps();
+  for (k = 0; k < NDIM; k++) FULLLOOP jcon[i][j][k] = 0.0;
+  pre_init_specific_init();
+  init_defglobal();
+  init_defconsts();
+  init_defgrid();
+  init_grid();
+  set_coord_parms();
+  init_global();
+  Rhor = rhor_calc(0);
+  Risco = rmso_calc(PROGRADERISCO);
+  set_grid();
+  ZSLOOP(-N1BND, N1 - 1 + N1BND, -N2BND, N2 - 1 + N2BND) {
+    if (init_dsandvels(&whichvel, &whichcoord, i, j, p[i][j]) > 0) return 1;
+    if (bl2met2metp2v(whichvel, whichcoord, p[i][j], i, j) >= 1) return 1;
+  }
+  normalize_densities(p);
+  A = emf;
+  ZSLOOP(-N1BND, N1 - 1 + N1BND, -N2BND, N2 - 1 + N2BND) A[i][j] = 0.;
+  ZSLOOP(-N1BND, N1 - 1 + N1BND, -N2BND, N2 - 1 + N2BND) init_vpot(i, j, &A[i][j]);
+  init_vpot2field(A, p);
+  normalize_field(p);
+  init_postfield(p);
+  ZSLOOP(-N1BND, N1 - 1 + N1BND, -N2BND, N2 - 1 + N2BND) filterffde(i, j, p[i][j]);
+  normalize_field(p);
+  init_postfield(p);
+  ZSLOOP(-N1BND, N1 - 1 + N1BND, -N2BND, N2 - 1 + N2BND) filterffde(i, j, p[i][j]);
+  ZSLOOP(-N1BND, N1 - 1 + N1BND, -N2BND, N2 - 1 + N2BND) PLOOP panalytic[i][j][k] = p[i][j][k];
+  if (bound_prim(STAGEM1, panalytic) >= 1) return 1;
+#if(FIXUPAFTERINIT)
+  if (fixup(STAGEM1, p, 0) >= 1) return 1;
+#endif
+  if (bound_prim(STAGEM1, p) >= 1) return 1;
+  if (pre_fixup(STAGEM1, p) >= 1) return 1;
+  post_init_specific_init();
+  return failed ? 1 : 0;
+}
+#endif
+
 int main(int argc, char *argv[])
 {
   char *margv[4];
@@ -80,6 +154,38 @@ int main(int argc, char *argv[])
   read_config(argv[1]);
 
   margv[0] = argv[0]; margv[1] = "1"; margv[2] = "1"; margv[3] = NULL;
+#ifdef HARNESS_REFINIT
+  {
+    FILE *f;
+    if (refinit(margv) >= 1) { fprintf(stderr, "init failed\n"); return 4; }
+    if (cfg("tables_out", NULL)) {
+      f = fopen(cfg("tables_out", ""), "wb");
+      fwrite(&a_gcov[0][0][0][0][0], sizeof(FTYPE), (size_t)N1M * N2M * NPG * NDIM * NDIM, f);
+      fwrite(&a_gcon[0][0][0][0][0], sizeof(FTYPE), (size_t)N1M * N2M * NPG * NDIM * NDIM, f);
+      fwrite(&a_gdet[0][0][0], sizeof(FTYPE), (size_t)N1M * N2M * NPG, f);
+      fwrite(&a_conn[0][0][0][0][0], sizeof(FTYPE), (size_t)N1M * N2M * NDIM * NDIM * NDIM, f);
+      fwrite(&Rin, sizeof(FTYPE), 1, f);
+      fwrite(&startx[1], sizeof(FTYPE), 2, f);
+      fwrite(&dx[1], sizeof(FTYPE), 2, f);
+      fclose(f);
+    }
+    f = fopen(cfg("init_out", "init.bin"), "wb");
+    fwrite(&a_p[0][0][0], sizeof(FTYPE), (size_t)N1M * N2M * NPR, f);
+    fwrite(&a_panalytic[0][0][0], sizeof(FTYPE), (size_t)N1M * N2M * NPR, f);
+    fwrite(&a_omegafanalytic[0][0][0], sizeof(FTYPE), (size_t)N1M * N2M * NPR, f);
+    fclose(f);
+    cour = cfgd("cour", cour);
+    /* the run-time parameters init() chose */
+    f = fopen(cfg("params_out", "params.txt"), "w");
+    fprintf(f, "lim %d\nfluxmethod %d\ntimeorder %d\nFLUXB %d\ncour %.17g\ngam %.17g\n", lim, fluxmethod,
+            TIMEORDER, FLUXB, cour, gam);
+    fprintf(f, "bc0 %d\nbc1 %d\nbc2 %d\nbc3 %d\n", BCtype[0], BCtype[1], BCtype[2], BCtype[3]);
+    fprintf(f, "Rin %.17g\nRout %.17g\nZin %.17g\nZout %.17g\ndefcoord %d\n", Rin, Rout, Zin, Zout, defcoord);
+    fprintf(f, "GAMMAMAX %.17g\nGAMMAFAIL %.17g\nSAFE %.17g\ndefcon %.17g\ntf %.17g\ndt %.17g\n", GAMMAMAX,
+            GAMMAFAIL, SAFE, defcon, tf, dt);
+    fclose(f);
+  }
+#else
   /* output modes as init() sets them before init_mpi() (initbase.c:42-50,
      USEROMIO 0); init_mpi() picks the file-combine method from them */
   binaryoutput = TEXTOUTPUT;
@@ -151,6 +257,7 @@ int main(int argc, char *argv[])
 
   readall(cfg("prim", ""), &a_p[0][0][0], sizeof(FTYPE) * (size_t)N1M * N2M * NPR);
   bound_prim(STAGEM1, p);
+#endif
 
   /* conservation accounting (every step runs diag_flux, step_ch.c:160, and
      diag_source, advance.c:140, whatever this option says): with "diag 1"
@@ -171,7 +278,7 @@ int main(int argc, char *argv[])
     for (ii = 0; ii < ncpux1 * N1; ii++) lumvsr[ii] = 0.0;
   }
 
-  t = 0.0;
+  t = cfgd("t0", 0.0);
   dt = cfgd("dt", 1e-4);
   tf = cfgd("tf", 1e30);
   nsteps = cfgi("nsteps", 1);

@@ -54,6 +54,10 @@ CASES = {
     "ot_eno_u_mc_hll_rk4": ("mink", 32, 32, H.MC, H.HLLFLUX, 4, 30, 1e-4, (1, 30)),
     "ot_eno_du_para_hll_rk3": ("mink", 32, 32, H.PARA, H.HLLFLUX, 3, 30, 1e-4, (30,)),
 }
+# bounds.fishmon.c's outflow extrapolation EXTRAP of the cases that set it
+# (the reference build ksx1: a configured copy of bounds.fishmon.c)
+EXTRAP = {"torus_extrap1_para_hll_rk4": 1}
+CASES["torus_extrap1_para_hll_rk4"] = ("ks", 32, 64, H.PARA, H.HLLFLUX, 4, 30, 1e-3, (1, 30))
 # DOENO of the cases that set it (the reference is built once per value)
 ENO = {"torus_eno_u_para_hll_rk4": 1, "torus_eno_du_mc_laxf_rk2": 2, "ot_eno_u_mc_hll_rk4": 1,
        "ot_eno_du_para_hll_rk3": 2}
@@ -94,7 +98,7 @@ def bc_of(kind):
 
 
 def run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, eno=0, rdump=False, diag=False,
-                  gammamax=None, dump=False):
+                  gammamax=None, dump=False, binary=None):
     p0.tofile(os.path.join(work, "p0.bin"))
     cfg = dict(prim=os.path.join(work, "p0.bin"), dt=dt0, nsteps=nsteps, out=os.path.join(work, "o.bin"),
                lim=lim, fluxmethod=flux, timeorder=to)
@@ -125,7 +129,7 @@ def run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, eno=0, rd
     with open(path, "w") as f:
         for k, v in cfg.items():
             f.write(f"{k} {v!r}\n" if isinstance(v, float) else f"{k} {v}\n")
-    binary = {"ks_mono": "ref_kscs"}.get(kind, "ref_" + kind) + (f"eno{eno}" if eno else "")
+    binary = binary or {"ks_mono": "ref_kscs"}.get(kind, "ref_" + kind) + (f"eno{eno}" if eno else "")
     subprocess.run([os.path.join(REF_DIR, binary), path], cwd=work, check=True, capture_output=True)
     d = np.fromfile(os.path.join(work, "o.bin"), dtype=np.float64)[4:]
     nz = (g.n1 + 6) * (g.n2 + 6) * 8
@@ -220,6 +224,54 @@ def make_dump(name, work):
     print(name, "text", len(text), "bytes, binary", len(binary), "bytes")
 
 
+# the reference's default problem as shipped (init.c PROBLEMTYPE 7, bounds.c,
+# CYLMINKMETRIC, EOMFFDE) from its own init(): the initial state, panalytic
+# and omegafanalytic it sets up, its set_grid() tables (pinned by sha256
+# against harm_grid_cylmink), and its steps.  t0 > 15 (TSTATIC) takes
+# bound_disk's outflow branch for B1, B3.
+DISK_CASES = {"disk_default_mc_laxf_rk2": (32, 64, 40, 1e-3, 0.0, (1, 10, 40)),
+              "disk_late_mc_laxf_rk2": (32, 64, 20, 1e-2, 20.0, (1, 20))}
+
+
+def make_disk(name, work):
+    n1, n2, nsteps, dt0, t0, saved = DISK_CASES[name]
+    ref = "/root/reference/nu.75_m.5.txt"
+    link = os.path.join(work, "nu.75_m.5.txt")
+    if not os.path.exists(link):
+        os.symlink(ref, link)
+    cfg = dict(tables_out=os.path.join(work, "dtables.bin"), init_out=os.path.join(work, "dinit.bin"),
+               params_out=os.path.join(work, "dparams.txt"), dt=dt0, t0=t0, nsteps=nsteps,
+               out=os.path.join(work, "do.bin"))
+    path = os.path.join(work, "disk.cfg")
+    with open(path, "w") as f:
+        for k, v in cfg.items():
+            f.write(f"{k} {v!r}\n" if isinstance(v, float) else f"{k} {v}\n")
+    subprocess.run([os.path.join(REF_DIR, "ref_disk"), path], cwd=work, check=True, capture_output=True)
+    m1, m2 = n1 + 6, n2 + 6
+    init = np.fromfile(os.path.join(work, "dinit.bin")).reshape(3, m1, m2, 8)
+    prm = dict(l.split() for l in open(os.path.join(work, "dparams.txt")))
+    tb = np.fromfile(os.path.join(work, "dtables.bin"))
+    nz = m1 * m2
+    tables = tb[:nz * (64 + 64 + 4 + 64)]
+    g = H.grid_cylmink(n1, n2, float(prm["Rin"]), float(prm["Rout"]), float(prm["Zin"]), float(prm["Zout"]))
+    mine = np.concatenate([a.ravel() for a in H.Geometry(g).tables()[:4]])
+    assert np.array_equal(mine.view(np.int64), tables.view(np.int64)), "harm_grid_cylmink differs from set_grid()"
+    d = np.fromfile(os.path.join(work, "do.bin"))[4:]
+    recs = d.reshape(nsteps, 4 + nz * 8)
+    out = dict(kind="disk", n1=n1, n2=n2, lim=int(prm["lim"]), fluxmethod=int(prm["fluxmethod"]),
+               timeorder=int(prm["timeorder"]), fluxb=int(prm["FLUXB"]), cour=float(prm["cour"]),
+               gam=float(prm["gam"]), bc=np.array([int(prm[f"bc{k}"]) for k in range(4)]),
+               Rin=float(prm["Rin"]), Rout=float(prm["Rout"]), Zin=float(prm["Zin"]), Zout=float(prm["Zout"]),
+               gammamax=float(prm["GAMMAMAX"]), nsteps=nsteps, dt0=dt0, t0=t0,
+               p0=init[0], panalytic=init[1], omegaf=init[2][..., 0],
+               tables_sha=hashlib.sha256(tables.tobytes()).hexdigest(),
+               t=recs[:, 0], dt=recs[:, 1], nstroke=recs[:, 2], failcode=recs[:, 3], saved_steps=np.array(saved))
+    for s in saved:
+        out[f"p_{s}"] = recs[s - 1, 4:].reshape(m1, m2, 8)
+    np.savez_compressed(os.path.join(OUT_DIR, f"{name}.npz"), **out)
+    print(name, "t", recs[-1, 0], "dt", recs[-1, 1])
+
+
 def main():
     subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle", "ref"), "-s"])
     with tempfile.TemporaryDirectory() as work:
@@ -229,19 +281,25 @@ def main():
         du = [a for a in sys.argv[1:] if a in DUMP_CASES] or ([] if sys.argv[1:] else list(DUMP_CASES))
         for name in du:
             make_dump(name, work)
+        dk = [a for a in sys.argv[1:] if a in DISK_CASES] or ([] if sys.argv[1:] else list(DISK_CASES))
+        for name in dk:
+            make_disk(name, work)
         alld = {**DIAG_CASES, **DIAG_GAMMAMAX}
         dc = [a for a in sys.argv[1:] if a in alld] or ([] if sys.argv[1:] else list(alld))
         for name in dc:
             make_diag(name, work)
-        names = [a for a in sys.argv[1:] if a not in RDUMP_CASES and a not in alld and a not in DUMP_CASES] \
+        names = [a for a in sys.argv[1:] if a not in RDUMP_CASES and a not in alld and a not in DUMP_CASES
+                 and a not in DISK_CASES] \
             if sys.argv[1:] else list(CASES)
         for name in names:
             kind, n1, n2, lim, flux, to, nsteps, dt0, saved = CASES[name]
             g, geom, p0, extra = problem(kind, n1, n2)
             eno = ENO.get(name, 0)
-            recs, states = run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, eno)
+            extrap = EXTRAP.get(name, 0)
+            recs, states = run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, eno,
+                                         binary="ref_ksx1" if extrap else None)
             out = dict(kind=kind, n1=n1, n2=n2, lim=lim, fluxmethod=flux, timeorder=to, nsteps=nsteps, dt0=dt0,
-                       eno=eno,
+                       eno=eno, extrap=extrap,
                        input_sha=sha(p0), geom_sha=sha(np.concatenate([a.ravel() for a in geom.tables()])),
                        t=recs[:, 0], dt=recs[:, 1], nstroke=recs[:, 2], failcode=recs[:, 3],
                        saved_steps=np.array(saved))

@@ -44,7 +44,30 @@ def problem(kind, n1, n2, lim=H.PARA, fluxmethod=H.HLLFLUX, timeorder=4, seed=7,
 
 def golden_cases():
     return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "*.npz"))
-                  if not os.path.basename(p).startswith(("ks_geometry", "rdump_", "diag_", "dump_")))
+                  if not os.path.basename(p).startswith(("ks_geometry", "rdump_", "diag_", "dump_", "disk_")))
+
+
+def disk_cases():
+    return sorted(os.path.basename(p)[:-4] for p in glob.glob(os.path.join(GOLDEN, "disk_*.npz")))
+
+
+def disk_problem(gd):
+    """The reference's default problem from a disk golden: the cylindrical-
+    Minkowski grid, the state, panalytic and omegafanalytic its init() set
+    up (omegaf expanded to [N1M][N2M][NPR], component 0), and the parameters
+    it ran with (bounds.c: FIXED sides from panalytic, bound_disk, whose
+    first call the reference's init() already made)."""
+    n1, n2 = int(gd["n1"]), int(gd["n2"])
+    g = H.grid_cylmink(n1, n2, float(gd["Rin"]), float(gd["Rout"]), float(gd["Zin"]), float(gd["Zout"]))
+    geom = H.Geometry(g)
+    om = np.zeros_like(gd["panalytic"])
+    om[..., 0] = gd["omegaf"]
+    prm = H.make_params(n1, n2, g.dx1, g.dx2, lim=int(gd["lim"]), fluxmethod=int(gd["fluxmethod"]),
+                        timeorder=int(gd["timeorder"]), fluxb=int(gd["fluxb"]), cour=float(gd["cour"]),
+                        gam=float(gd["gam"]), gammamax=float(gd["gammamax"]), eomtype=H.EOMFFDE,
+                        bc=tuple(int(b) for b in gd["bc"]), poledeath=0, boundsfile=H.BOUNDS_DISK, extrap=1,
+                        disk_first=0)
+    return g, geom, np.array(gd["p0"]), np.array(gd["panalytic"]), om, prm
 
 
 def load_golden(name):
@@ -56,6 +79,8 @@ def golden_problem(gd):
     eno = int(gd["eno"]) if "eno" in gd.files else 0
     g, geom, p0, prm = problem(str(gd["kind"]), int(gd["n1"]), int(gd["n2"]), lim=int(gd["lim"]),
                                fluxmethod=int(gd["fluxmethod"]), timeorder=int(gd["timeorder"]), eno=eno)
+    if "extrap" in gd.files:
+        prm.extrap = int(gd["extrap"])
     if "gammamax" in gd.files:  # GAMMAMAX and GAMMAFAIL = 100 GAMMAMAX (initbase.c:218-219)
         prm.gammamax = float(gd["gammamax"])
         prm.gammafail = 100.0 * prm.gammamax
