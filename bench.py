@@ -383,7 +383,7 @@ def spin_derivative_measure(n=512, nsteps=100, a0=0.9, r_flux=20.0, growth_steps
             "tangent_run_s": t_tl_run, "tangent_tables_s": t_tables}
 
 
-def adjoint_measure(n=256, nsteps=256, K=16, seed=3):
+def adjoint_measure(n=256, nsteps=256, K=None, seed=3):
     """Discrete adjoint on the Orszag-Tang problem (Minkowski, periodic, MC,
     HLL, RK4, flux-CT) at n x n over nsteps steps with checkpoints every K:
     J = magnetic energy at the end, gradients with respect to the field
@@ -391,7 +391,10 @@ def adjoint_measure(n=256, nsteps=256, K=16, seed=3):
     perturbation field (sigma 1e-3, also part of the state).  Reports the
     cost of forward + reverse sweep against the plain step (steady state: step
     graphs captured and kernel variants chosen before the timed sweep), and the adjoint
-    against central finite differences (h = 1e-6, same time steps)."""
+    against central finite differences (h = 1e-6, same time steps).  K
+    (checkpoint interval) defaults to nsteps when every substage state of the
+    run fits in 32 GB of HBM: the forward run then keeps them all and the
+    reverse sweep recomputes nothing."""
     import numpy as np
     import harm_ramt_amd as H
     g = H.grid_cartesian(n, n)
@@ -420,6 +423,9 @@ def adjoint_measure(n=256, nsteps=256, K=16, seed=3):
 
     B0, vamp = 1.0 / np.sqrt(4 * np.pi), 0.5
     p0 = state(B0, vamp)
+    if K is None:
+        state_bytes = (n + 6) * (n + 6) * 8 * 8
+        K = nsteps if (nsteps * 4 + 1) * state_bytes <= 32 * 2**30 else 16
 
     def ctx(p, pr=None):
         h = H.Harm(pr or prm, geom)

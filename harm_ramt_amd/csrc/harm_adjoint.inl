@@ -10,7 +10,9 @@
 //   differentiated through the implicit-function theorem at its converged
 //   solution; floors on their active branch; periodic bound_prim transposed.
 // Reverse sweep: checkpoints every K steps; each K-step segment is recomputed
-// forward once, storing every substage state, then reversed step by step.
+// forward once, storing every substage state, then reversed step by step --
+// except the last segment, whose states the forward run stores as it goes
+// (K = nsteps: no recompute at all, one state per substage kept in HBM).
 // Supported: GRMHD, PERIODIC boundaries, FIXUPFLUX off, limiters DONOR, VANL,
 // MINM, MC, NLIM*; HLL or LAXF; FLUXCTTOTH or no CT.  Points where the
 // primal is not differentiable (limit_gamma active, failed inversions, clamped
@@ -293,21 +295,43 @@ __global__ __launch_bounds__(256) void k_adj_gather(KParams P0, const double* co
     const int c = (dir == 1) ? i : j;                 // position along the line
     const int nmax = ((dir == 1) ? P.n1 : P.n2) + NG;  // one past the last zone
     const int stride = (dir == 1) ? P.N2M : 1;
-    // lpl(f) = lf[2 z(f)], lpr(f) = lf[2 z(f) + 1]; faces exist for f in [-NG, nmax)
-    auto lpl = [&](int f, int k) { return (f >= -NG && f < nmax) ? lf[(size_t)(2 * (z + (f - c) * stride)) * 8 + k] : 0.; };
-    auto lpr = [&](int f, int k) {
-      return (f >= -NG && f < nmax) ? lf[(size_t)(2 * (z + (f - c) * stride) + 1) * 8 + k] : 0.;
-    };
-    auto pv = [&](int w, int k) { return pb[(size_t)(z + (w - c) * stride) * 8 + k]; };
+    // lpl(f) = lf[2 z(f)], lpr(f) = lf[2 z(f) + 1]; faces exist for f in
+    // [-NG, nmax).  The line's zones c-2..c+2 and faces c-1..c+2 are loaded
+    // once, as whole records; out-of-range entries are never used.
+    double pz[5][NPR], fl[4][NPR], fr[4][NPR];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      const int w = c - 2 + m;
+      if (w >= -NG && w < nmax) {
+        load8(pb, z + (w - c) * stride, pz[m]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < NPR; k++) pz[m][k] = 0.;
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const int f = c - 1 + m;
+      if (f >= -NG && f < nmax) {
+        load8(lf, 2 * (z + (f - c) * stride), fl[m]);
+        load8(lf, 2 * (z + (f - c) * stride) + 1, fr[m]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < NPR; k++) fl[m][k] = fr[m][k] = 0.;
+      }
+    }
+#pragma unroll
     for (int k = 0; k < NPR; k++) {
-      double acc = lpl(c + 1, k) + lpr(c, k);
-      for (int w = c - 1; w <= c + 1; w++) {
+      double acc = fl[2][k] + fr[1][k];  // lpl(c+1) + lpr(c)
+#pragma unroll
+      for (int dw = -1; dw <= 1; dw++) {
+        const int w = c + dw;
         if (w - 1 < -NG || w + 1 >= nmax) continue;
-        const double ldq = 0.5 * lpl(w + 1, k) - 0.5 * lpr(w, k);
+        const double ldq = 0.5 * fl[dw + 2][k] - 0.5 * fr[dw + 1][k];  // lpl(w+1), lpr(w)
         if (ldq == 0.) continue;
         double cl, cc, cr;
-        slope3_coef(P.lim, pv(w - 1, k), pv(w, k), pv(w + 1, k), cl, cc, cr);
-        acc += ((w == c - 1) ? cr : ((w == c) ? cc : cl)) * ldq;
+        slope3_coef(P.lim, pz[dw + 1][k], pz[dw + 2][k], pz[dw + 3][k], cl, cc, cr);
+        acc += ((dw == -1) ? cr : ((dw == 0) ? cc : cl)) * ldq;
       }
       l[k] += acc;
     }
@@ -378,9 +402,10 @@ struct harm_adj {
   int* d_ns;
   int* d_ctr;                      // step index, first step of the segment
   const double** d_slot;           // pb and pf of the substage being reversed
-  hipGraphExec_t gx[4];            // steps: record dt, impose dt, recompute, reverse
-  bool gx_ok[4];
+  hipGraphExec_t gx[5];            // steps: record dt, impose dt, impose + store, reverse, record + store
+  bool gx_ok[5];
   bool forward_done, seeded;
+  bool kept_last;                  // seg holds the last segment's states from the forward run
   int recompute_failures;
 };
 
@@ -399,7 +424,7 @@ static int adj_check(const harm_ctx* c) {
 
 extern "C" int harm_adj_destroy(harm_adj* a) {
   if (!a) return 0;
-  for (int k = 0; k < 4; k++)
+  for (int k = 0; k < 5; k++)
     if (a->gx_ok[k]) (void)hipGraphExecDestroy(a->gx[k]);
   if (a->d_ctr) (void)hipFree(a->d_ctr);
   if (a->d_slot) (void)hipFree((void*)a->d_slot);
@@ -474,8 +499,11 @@ static int adj_step(harm_adj* a, int impose, bool store) {
   rk_setup(c->prm.timeorder, &nto, CUf, Cunew, pii, pbb, pff);
   const size_t nz8 = (size_t)c->nz * 8;
   for (int to = 0; to < nto; to++) {
-    if (substage(c, pii[to], pbb[to], pff[to], CUf[to], Cunew[to], to, nto)) return 1;
-    if (launch_boundary(c, pff[to], KB_BOUND | KB_FIXUP | KB_BOUND2 | ((to == nto - 1) ? KB_DT : 0))) return 1;
+    const bool bnd = bound_in_update(c);  // the step's own boundary schedule
+    if (substage(c, pii[to], pbb[to], pff[to], CUf[to], Cunew[to], to, nto, bnd)) return 1;
+    if (launch_boundary(c, pff[to],
+                        KB_BOUND | KB_FIXUP | KB_BOUND2 | ((to == nto - 1) ? KB_DT : 0) | (bnd ? KB_INUPDATE : 0)))
+      return 1;
     if (store)
       hipLaunchKernelGGL(k_adj_store, dim3(512), dim3(256), 0, c->stream, c->arr[pff[to]], a->seg, a->d_ctr, nto, to,
                          nz8);
@@ -538,13 +566,14 @@ static int adj_reverse_step(harm_adj* a) {
 }
 
 // graph k: 0 forward step recording dt, 1 forward step imposing dt,
-// 2 recompute step (imposed dt, states stored), 3 reverse step
+// 2 recompute step (imposed dt, states stored), 3 reverse step, 4 forward
+// step recording dt with its states stored
 static int adj_graph(harm_adj* a, int k) {
   if (a->gx_ok[k]) return 0;
   harm_ctx* c = a->c;
   hipGraph_t g;
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  int rc = (k == 3) ? adj_reverse_step(a) : adj_step(a, k == 0 ? 0 : 1, k == 2);
+  int rc = (k == 3) ? adj_reverse_step(a) : adj_step(a, (k == 0 || k == 4) ? 0 : 1, k == 2 || k == 4);
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (rc) {
     if (e == hipSuccess) (void)hipGraphDestroy(g);
@@ -570,8 +599,10 @@ extern "C" int harm_adj_forward(harm_adj* a, const double* dts) {
     memcpy(a->dts->data(), dts, sizeof(double) * a->nsteps);
     HIPCHK(hipMemcpy(a->d_dts, dts, sizeof(double) * a->nsteps, hipMemcpyHostToDevice));
   }
-  const int gk = dts ? 1 : 0;
+  const int gk = dts ? 1 : 0, gks = dts ? 2 : 4;  // plain steps, and the last segment's (states stored)
+  const int n0last = (a->nseg - 1) * a->K;
   int rc = adj_graph(a, gk);
+  if (!rc) rc = adj_graph(a, gks);
   if (!rc) hipLaunchKernelGGL(k_adj_setctr, dim3(1), dim3(1), 0, c->stream, a->d_ctr, 0, 0);
   for (int n = 0; n < a->nsteps && !rc; n++) {
     if (n % a->K == 0) {
@@ -580,7 +611,13 @@ extern "C" int harm_adj_forward(harm_adj* a, const double* dts) {
           hipMemcpyAsync(a->d_ckds + m, c->ds, sizeof(DevScalars), hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
         rc = seterr("adjoint forward: checkpoint copy");
     }
-    if (!rc && hipGraphLaunch(a->gx[gk], c->stream) != hipSuccess) rc = seterr("adjoint forward: graph launch");
+    if (!rc && n == n0last) {  // the last segment: its initial state, then every substage's
+      if (hipMemcpyAsync(a->seg, c->arr[HARM_P], sb, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+        rc = seterr("adjoint forward: segment store");
+      hipLaunchKernelGGL(k_adj_setctr, dim3(1), dim3(1), 0, c->stream, a->d_ctr, n0last, n0last);
+    }
+    if (!rc && hipGraphLaunch(a->gx[n >= n0last ? gks : gk], c->stream) != hipSuccess)
+      rc = seterr("adjoint forward: graph launch");
   }
   c->prof = was_prof;
   if (rc) return 1;
@@ -590,6 +627,7 @@ extern "C" int harm_adj_forward(harm_adj* a, const double* dts) {
   if (!dts) HIPCHK(hipMemcpy(a->dts->data(), a->d_dts, sizeof(double) * a->nsteps, hipMemcpyDeviceToHost));
   a->forward_done = true;
   a->seeded = false;
+  a->kept_last = true;
   return 0;
 }
 
@@ -658,6 +696,12 @@ extern "C" int harm_adj_backward(harm_adj* a) {
   a->recompute_failures = 0;
   for (int m = a->nseg - 1; m >= 0 && !rc; m--) {
     const int n0 = m * a->K, n1 = (n0 + a->K < a->nsteps) ? n0 + a->K : a->nsteps;
+    if (m == a->nseg - 1 && a->kept_last) {  // stored by the forward run
+      hipLaunchKernelGGL(k_adj_setctr, dim3(1), dim3(1), 0, c->stream, a->d_ctr, n1 - 1, n0);
+      for (int n = n1 - 1; n >= n0 && !rc; n--)
+        if (hipGraphLaunch(a->gx[3], c->stream) != hipSuccess) rc = seterr("adjoint reverse: graph launch");
+      continue;
+    }
     // recompute the segment, storing every substage state
     HIPCHK(hipMemcpyAsync(c->arr[HARM_P], (*a->ckpt)[m], sb, hipMemcpyDeviceToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(c->ds, a->d_ckds + m, sizeof(DevScalars), hipMemcpyDeviceToDevice, c->stream));
@@ -676,6 +720,7 @@ extern "C" int harm_adj_backward(harm_adj* a) {
       if (hipGraphLaunch(a->gx[3], c->stream) != hipSuccess) rc = seterr("adjoint reverse: graph launch");
   }
   c->prof = was_prof;
+  if (a->nseg > 1) a->kept_last = false;  // the earlier segments' recomputes overwrote it
   if (rc) return 1;
   if (adj_boundT(a)) return 1;
   // leave the context at the end of the forward run

@@ -3024,7 +3024,8 @@ static int launch_boundary(harm_ctx* c, int sel, int mode) {
     const dim3 gx1((c->kp.n2 + 63) / 64, 2 * NG), gx2((c->kp.n1 + 2 * NG + 63) / 64, 2 * NG + 2);
     const dim3 gb((bound_threads(c->kp) + 255) / 256);
     const bool bound2 = (mode & KB_BOUND2) != 0, fix = (mode & KB_FIXUP) != 0;
-    const bool fused = (c->bound_variant == HARM_BOUND_FUSED);
+    // the in-update fill's fallback where it does not apply is the fused one
+    const bool fused = (c->bound_variant == HARM_BOUND_FUSED) || (c->bound_variant == HARM_BOUND_INUPDATE);
     if ((mode & KB_BOUND) && fused) {
       tlaunch(c,k_bound, gb, dim3(256), 0, c->kp, c->arr[sel], c->gC, c->ds, 0,
                          (!fix && !bound2 && (mode & KB_DT)) ? 1 : 0);

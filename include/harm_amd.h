@@ -293,7 +293,9 @@ int harm_batch_get_tuning(harm_batch *batch, int *flux_variant, int *bound_varia
    on the branch the forward run took; the inversion through the
    implicit-function theorem at its converged solution.  Checkpoints every
    checkpoint_every steps; each segment is recomputed once, keeping every
-   RK substage state, and then reversed.  Supported: EOMGRMHD, PERIODIC
+   RK substage state, and then reversed; the last segment's states are kept
+   by the forward run itself (checkpoint_every = nsteps: no recompute, at
+   the memory of nsteps x substages + 1 states).  Supported: EOMGRMHD, PERIODIC
    boundaries, FIXUPFLUX off, limiters DONOR/VANL/MINM/MC/NLIM*, HLL or
    LAXF, with or without flux-CT (harm_adj_create reports anything else).
    Typical use: set the state (harm_set_array + harm_bound_prim) and time,

@@ -191,6 +191,29 @@ def test_adjoint_is_independent_of_the_checkpoint_interval():
     assert np.array_equal(lams[0], lams[1]) and np.array_equal(lams[0], lams[2])
 
 
+@pytest.mark.parametrize("K", [3, 7])
+def test_adjoint_second_backward_gives_the_same_gradient(K):
+    """the last segment's states come from the forward run the first time;
+    a second backward sweep (re-seeded) recomputes what the first overwrote"""
+    n, nsteps = 32, 7
+    say(f"[case] adjoint OT {n}x{n}, {nsteps} steps, K={K}: two backward sweeps after one forward")
+    g, geom, prm = ot_setup(n)
+    p0 = ot_state(g)
+    h = H.Harm(prm, geom)
+    h.set_array(H.P, p0)
+    h.bound_prim(H.P)
+    h.set_time(0.0, 1e-3, 1e30)
+    a = H.Adjoint(h, nsteps, K)
+    a.forward()
+    a.seed_magnetic_energy()
+    lam1 = a.backward()
+    a.seed_magnetic_energy()
+    lam2 = a.backward()
+    a.close()
+    h.close()
+    assert np.array_equal(lam1, lam2)
+
+
 def test_adjoint_parameter_gradients():
     n, nsteps = 32, 10
     say(f"[case] adjoint OT {n}x{n} NLIMCENT, {nsteps} steps: dJ/dB0, dJ/dv_amplitude, dJ/d(rho perturbation)")
