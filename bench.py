@@ -603,11 +603,19 @@ def main():
     h.step_n(args.warmup)           # the first call also times the kernel variants and keeps the fastest
     h.get_time()                    # synchronises
     fv, bv = h.get_tuning()
+    ns0, wm0 = h.newton_spread()    # synchronises
     t0 = time.perf_counter()
     h.step_n(args.steps)            # HIP-graph replay, no host sync inside
     t_now, dt_now = h.get_time()    # synchronises
     el = time.perf_counter() - t0
     value = args.steps * n1 * n2 / el
+    ns1, wm1 = h.newton_spread()
+    # the Newton iterations of the timed steps: needed per zone inversion, and
+    # issued (each wavefront runs its slowest zone's count) over needed
+    ninv = args.steps * 4 * n1 * n2
+    newton = {"iterations_per_inversion": (ns1 - ns0) / ninv,
+              "wavefront_max_per_inversion": 64 * (wm1 - wm0) / ninv,
+              "issued_over_needed": 64 * (wm1 - wm0) / max(1, ns1 - ns0)}
     nfail, failed = h.failures()
     if failed:
         print(f"bench.py: the run set the failure flag (t={t_now})", file=sys.stderr)
@@ -693,6 +701,7 @@ def main():
         "ffde": ffde,
         "accounting": accounting,
         "stages": stages,
+        "newton": newton,
         "eager_timed_ms_per_step": 1e3 * el_prof / prof_steps,
     }
     print(json.dumps(out), flush=True)

@@ -182,7 +182,7 @@ EXPORTS = [
     "harm_tl_create", "harm_tl_destroy", "harm_tl_set_geometry_tangent", "harm_tl_set_spacing_tangent",
     "harm_tl_set_state_tangent",
     "harm_tl_get_state_tangent", "harm_tl_step_n", "harm_tl_energy_flux",
-    "harm_get_nstroke", "harm_prof_enable", "harm_prof_get", "harm_prof_reset",
+    "harm_get_nstroke", "harm_get_newton_spread", "harm_prof_enable", "harm_prof_get", "harm_prof_reset",
     "harm_set_tuning", "harm_get_tuning",
     "harm_batch_create", "harm_batch_destroy", "harm_batch_step_n", "harm_batch_set_tuning",
     "harm_batch_get_tuning",
@@ -244,6 +244,7 @@ def load_library(path=LIB_PATH):
         "harm_tl_energy_flux": [vp, ctypes.c_int, dp, dp],
         "harm_get_failures": [vp, ip, ip],
         "harm_get_nstroke": [vp, ctypes.POINTER(ctypes.c_longlong)],
+        "harm_get_newton_spread": [vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
         "harm_prof_enable": [vp, ctypes.c_int],
         "harm_prof_get": [vp, ctypes.c_int, dp, ctypes.POINTER(ctypes.c_longlong)],
         "harm_prof_reset": [vp],
@@ -613,6 +614,12 @@ class Harm:
         n, f = ctypes.c_int(), ctypes.c_int()
         _chk(self.lib.harm_get_failures(self.h, ctypes.byref(n), ctypes.byref(f)), "harm_get_failures")
         return n.value, f.value
+
+    def newton_spread(self):
+        """(Newton iterations, sum over wavefronts of each one's largest count)"""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        _chk(self.lib.harm_get_newton_spread(self.h, ctypes.byref(a), ctypes.byref(b)), "harm_get_newton_spread")
+        return a.value, b.value
 
     def nstroke(self):
         v = ctypes.c_longlong()

@@ -63,7 +63,8 @@ struct DevScalars {
   int nlist;  // entries of the failed-zone list of the current substage
   int cs_changed;  // fixup_checksolution: the second pass differs from the first
   double dt_step;  // dt of the step just completed (diag_flux's Dt)
-  unsigned long long nstroke[NSTRIPE * SSTR];  // Newton iterations, striped
+  unsigned long long nstroke[NSTRIPE * SSTR];  // Newton iterations, striped; word 8 of a
+                                               // slot: the sum of each wavefront's largest count
   // HARM_BOUND_INUPDATE with N1 >= 5: the x1 ghost columns of the pole rows'
   // outflow edge zones as the second bound_prim() computes them (from the row
   // after POLEDEATH), written by k_update, copied by k_fixall where the edge
@@ -157,6 +158,11 @@ __device__ __forceinline__ double wave_min(double v) {
 __device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+__device__ __forceinline__ int wave_max_i32(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
   return v;
 }
 
@@ -1053,8 +1059,12 @@ __device__ __forceinline__ void invert_zone(const KParams& P, const double* U, c
 __device__ __forceinline__ void count_zone(DevScalars* __restrict__ ds, int niter, int failz, int fail, const dim3 bi) {
   unsigned long long ns = wave_sum_u64((unsigned long long)niter);
   unsigned long long nf = wave_sum_u64((unsigned long long)failz);
+  const int wm = wave_max_i32(niter);  // the wavefront runs its slowest zone's iterations
   if ((threadIdx.x & 63) == 0) {
-    if (ns) atomicAdd(&ds->nstroke[stripe_of(bi) * SSTR], ns);
+    if (ns) {
+      atomicAdd(&ds->nstroke[stripe_of(bi) * SSTR], ns);
+      atomicAdd(&ds->nstroke[stripe_of(bi) * SSTR + 8], (unsigned long long)wm);
+    }
     if (nf) atomicAdd(&ds->nfail, (int)nf);
   }
   if (fail) atomicOr(&ds->failed, 1);
@@ -3491,6 +3501,22 @@ extern "C" int harm_get_nstroke(harm_ctx* c, long long* nstroke) {
   unsigned long long sum = 0;
   for (int k = 0; k < NSTRIPE; k++) sum += v[(size_t)k * SSTR];
   *nstroke = (long long)sum;
+  return 0;
+}
+
+extern "C" int harm_get_newton_spread(harm_ctx* c, long long* nstroke, long long* wave_max_sum) {
+  if (!c || !nstroke || !wave_max_sum) return seterr("null argument");
+  std::vector<unsigned long long> v(NSTRIPE * SSTR);
+  HIPCHK(hipMemcpyAsync(v.data(), &c->ds->nstroke[0], v.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  unsigned long long a = 0, b = 0;
+  for (int k = 0; k < NSTRIPE; k++) {
+    a += v[(size_t)k * SSTR];
+    b += v[(size_t)k * SSTR + 8];
+  }
+  *nstroke = (long long)a;
+  *wave_max_sum = (long long)b;
   return 0;
 }
 

@@ -265,6 +265,11 @@ int harm_prof_enable(harm_ctx *ctx, int on);
 int harm_set_tuning(harm_ctx *ctx, int flux_variant, int bound_variant);
 int harm_get_tuning(harm_ctx *ctx, int *flux_variant, int *bound_variant);
 int harm_prof_get(harm_ctx *ctx, int kernel, double *total_ms, long long *count);
+/* (no reference counterpart) The Newton iteration count of harm_get_nstroke
+   and the sum over wavefronts (64 zones) of each one's largest count: a
+   wavefront runs its slowest zone's iterations, so 64 wave_max_sum /
+   nstroke >= 1 is the iterations issued per iteration needed. */
+int harm_get_newton_spread(harm_ctx *ctx, long long *nstroke, long long *wave_max_sum);
 int harm_prof_reset(harm_ctx *ctx);
 
 /* ---- batched runs (no reference counterpart: the reference advances one

@@ -395,6 +395,18 @@ def test_gpu_fill_in_update_with_failures_at_the_edges(case):
     assert pole > 0 and edge > 0, "the run is meant to repair pole-row and edge zones"
 
 
+def test_gpu_newton_spread_counts():
+    """harm_get_newton_spread: its iteration total is harm_get_nstroke's, and
+    the wavefront maxima bound it from above (64 lanes a wavefront)"""
+    say("[case] Newton spread counters on the 64x64 torus, 5 steps")
+    g, geom, p0, prm = problem("ks", 64, 64)
+    h = gpu_ctx(geom, prm, p0, 1e-3)
+    h.step_n(5)
+    ns, wm = h.newton_spread()
+    assert ns == h.nstroke() > 0
+    assert 64 * wm >= ns and wm > 0
+
+
 def test_gpu_autotune_picks_a_variant():
     say("[case] autotuner on the 64x64 torus")
     g, geom, p0, prm = problem("ks", 64, 64)
