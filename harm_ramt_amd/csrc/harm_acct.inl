@@ -49,6 +49,7 @@ struct AcctState {
   size_t scratch_n;
   // fixup events of a substage (slots of acct_base) and their compaction
   double* pre;      // zone states before fixup1zone [nz][8]
+  double* uin;      // the conserved state each force-free inversion took [nz][8]
   int* flag;        // [nslots], category + 1 of the event in the slot
   double* val;      // [nslots][8]
   int* list;        // [nslots][2]: slot, category of the events in order
@@ -510,8 +511,120 @@ __global__ __launch_bounds__(256) void k_acct_faceg(KParams P, const double* __r
   }
 }
 
-// fixup1zone's events (fixup.c:274-377) from the state before it: limit_gamma
-// (COUNTLIMITGAMMAACT), then the whole change (COUNTFLOORACT, final substage)
+// The force-free inversion's dissipation account (phys.ffde.c:354-364,
+// ACCOUNTDISS 1; diss_account :572-607, ACCOUNTNONCONS 1) into slot `slot`:
+// the geometry-free Ug the inversion took and the state pr it returned.  E/alpha
+// and the Lorentz-factor test are formed from Ug as utoprim_ffde forms them; the
+// initial T^t_mu from E and B alone (FitUtoT :917-935: Max_con :1472-1552,
+// lower_A :1654-1682, ffdestresstensor :1430-1462), the final U from pr
+// (primtoU UDIAG), the energy the non-conservative inversion lost added back.
+// COUNTUTOPRIMFAILCONV when the cap acted, else COUNTFLOORACT; it counts on
+// every substage (diag_fixup_U, fixup.c:234-270, forces its finalstep test).
+__device__ void acct_diss_event(const KParams& P, const double* gc, const GeoU& g, const double* Ug,
+                                const double* pr, int slot) {
+  const double* gcn = gc + 16;
+  const double gdet = gc[32];
+  double Bcon[4] = {0.0, Ug[B1], Ug[B2], Ug[B3]}, Bcov[4];
+  lower_gc(gc, Bcon, Bcov);
+  double Bsq = 0.0;
+#pragma unroll
+  for (int j = 1; j < 4; j++) Bsq += Bcov[j] * Bcon[j];
+  const double o = 1.0 / (Bsq * gdet);
+  double E[4], Ecv[4];
+  E[0] = 0;
+  E[1] = (Bcov[2] * Ug[UU + 3] - Bcov[3] * Ug[UU + 2]) * (o);
+  E[2] = (Bcov[3] * Ug[UU + 1] - Bcov[1] * Ug[UU + 3]) * (o);
+  E[3] = (Bcov[1] * Ug[UU + 2] - Bcov[2] * Ug[UU + 1]) * (o);
+  lower_gc(gc, E, Ecv);
+  double Esq = 0.0;
+#pragma unroll
+  for (int j = 1; j < 4; j++) Esq += E[j] * Ecv[j];
+  const double gammasq = Bsq / (Bsq - Esq);
+  const int modified = (gammasq > P.gammamax * P.gammamax) || (gammasq < 1.0);
+  // Maxwell tensor M^{ab} of F^{it} = -E^i/alpha and M^{ti} = -B^i, eta_a = (-alpha, 0, 0, 0)
+  const double alpha = 1. / sqrt(-gcn[0]);
+  const double et[4] = {-alpha, 0.0, 0.0, 0.0};
+  double en[4];
+  en[0] = gcn[0] * et[0] + gcn[1] * et[1] + gcn[2] * et[2] + gcn[3] * et[3];
+  en[1] = gcn[1] * et[0] + gcn[5] * et[1] + gcn[6] * et[2] + gcn[7] * et[3];
+  en[2] = gcn[2] * et[0] + gcn[6] * et[1] + gcn[10] * et[2] + gcn[11] * et[3];
+  en[3] = gcn[3] * et[0] + gcn[7] * et[1] + gcn[11] * et[2] + gcn[15] * et[3];
+  double Fit[4], Bm[4], Em[4], Emc[4];
+  Fit[0] = 0.0;
+  Bm[0] = 0.0;
+#pragma unroll
+  for (int j = 1; j < 4; j++) {
+    Fit[j] = -E[j];
+    Bm[j] = et[0] * (-Ug[B1 + j - 1]);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) Em[j] = et[0] * Fit[j];
+  lower_gc(gc, Em, Emc);
+  double M[4][4], Mc[4][4];
+  M[0][0] = M[1][1] = M[2][2] = M[3][3] = 0.;
+  M[1][2] = -en[1] * Bm[2] + en[2] * Bm[1] - (et[0] / gdet) * Emc[3];
+  M[2][3] = -en[2] * Bm[3] + en[3] * Bm[2] - (et[0] / gdet) * Emc[1];
+  M[3][1] = -en[3] * Bm[1] + en[1] * Bm[3] - (et[0] / gdet) * Emc[2];
+  M[2][1] = -M[1][2];
+  M[3][2] = -M[2][3];
+  M[1][3] = -M[3][1];
+#pragma unroll
+  for (int j = 1; j < 4; j++) {
+    M[0][j] = Bm[j] / et[0];
+    M[j][0] = -M[0][j];
+  }
+  // lower_A: the six upper entries of M_ab, summed over (j, k) in loop order
+  double a01 = 0., a02 = 0., a03 = 0., a12 = 0., a23 = 0., a31 = 0.;
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      a01 += (gc[0 * 4 + j]) * (gc[1 * 4 + k]) * M[j][k];
+      a02 += (gc[0 * 4 + j]) * (gc[2 * 4 + k]) * M[j][k];
+      a03 += (gc[0 * 4 + j]) * (gc[3 * 4 + k]) * M[j][k];
+      a12 += (gc[1 * 4 + j]) * (gc[2 * 4 + k]) * M[j][k];
+      a23 += (gc[2 * 4 + j]) * (gc[3 * 4 + k]) * M[j][k];
+      a31 += (gc[3 * 4 + j]) * (gc[1 * 4 + k]) * M[j][k];
+    }
+  Mc[0][0] = Mc[1][1] = Mc[2][2] = Mc[3][3] = 0.;
+  Mc[0][1] = a01; Mc[0][2] = a02; Mc[0][3] = a03; Mc[1][2] = a12; Mc[2][3] = a23; Mc[3][1] = a31;
+  Mc[1][0] = -a01; Mc[2][0] = -a02; Mc[3][0] = -a03; Mc[2][1] = -a12; Mc[3][2] = -a23; Mc[1][3] = -a31;
+  // ffdestresstensor, row t: T^t_b = M^{tc} M_{bc} - 1/4 delta^t_b (M_cd M^cd)
+  double Msq = 0.;
+#pragma unroll
+  for (int j = 0; j < 4; j++)
+#pragma unroll
+    for (int k = 0; k < 4; k++) Msq += Mc[j][k] * M[j][k];
+  double Tt[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    double t = 0.;
+#pragma unroll
+    for (int i = 0; i < 4; i++) t += M[0][i] * Mc[k][i];
+    Tt[k] = t;
+  }
+  Tt[0] -= 0.25 * Msq;
+  double Ui[NPR], Uf[NPR];
+  Ui[RHO] = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) Ui[UU + j] = gdet * Tt[j];
+#pragma unroll
+  for (int j = 1; j < 4; j++) Ui[B1 + j - 1] = gdet * Ug[B1 + j - 1];
+  State q;
+  get_state(pr, g, q);
+  primtoU_diag(P, pr, q, gdet, Uf);
+  Uf[RHO] = 0.0;
+  Uf[UU] -= (gdet * Ug[UU] - Ui[UU]);
+  double* out = P.acct_val + (size_t)slot * 8;
+#pragma unroll
+  for (int k = 0; k < NPR; k++) out[k] = P.dVF * (Uf[k] - Ui[k]);
+  P.acct_flag[slot] = (modified ? 0 : 1) + 1;
+}
+
+// the zone's events in the reference's order: the force-free inversion's
+// dissipation account, then fixup1zone's (fixup.c:274-377) from the state
+// before it: limit_gamma (COUNTLIMITGAMMAACT), then the whole change
+// (COUNTFLOORACT, final substage)
 __global__ __launch_bounds__(256) void k_acct_zoneg(KParams P, const double* __restrict__ gC, int finalstep) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y * blockDim.y + threadIdx.y;
@@ -523,6 +636,18 @@ __global__ __launch_bounds__(256) void k_acct_zoneg(KParams P, const double* __r
   geo_from_cent(gc, g);
   double p0[NPR], p1[NPR], p2[NPR];
   load8(P.acct_pre, z, p0);
+  const int slot = acct_base(P.n1, P.n2, 2) + (i * P.n2 + j) * 3;
+  if (P.eomtype == HARM_EOMFFDE) {
+    double U[NPR], Ug[NPR], pt[NPR];
+    load8(P.acct_uin, z, U);
+#pragma unroll
+    for (int k = 0; k < NPR; k++) {
+      Ug[k] = U[k] / gc[32];
+      pt[k] = p0[k];
+    }
+    // no account when the inversion failed (space-like drift, fatal)
+    if (!utoprim_ffde(P, Ug, gc, pt)) acct_diss_event(P, gc, g, Ug, p0, slot);
+  }
 #pragma unroll
   for (int k = 0; k < NPR; k++) p1[k] = p0[k];
   int changed = 0;
@@ -535,12 +660,11 @@ __global__ __launch_bounds__(256) void k_acct_zoneg(KParams P, const double* __r
   }
 #pragma unroll
   for (int k = 0; k < NPR; k++) p2[k] = p1[k];
-  const int slot = acct_base(P.n1, P.n2, 2) + (i * P.n2 + j) * 2;
   if (limit_gamma(P.gammamax, p2, g) == -1) {
-    acct_event(P, g, p1, p2, slot, 2);
+    acct_event(P, g, p1, p2, slot + 1, 2);
     changed = 1;
   }
-  if (changed && finalstep) acct_event(P, g, p0, p2, slot + 1, 1);
+  if (changed && finalstep) acct_event(P, g, p0, p2, slot + 2, 1);
 }
 
 // the failed zones' events from the states fixup_zone kept: limit_gamma's
@@ -668,9 +792,12 @@ __global__ __launch_bounds__(512) void k_acct_merge(KParams P, AcctRegions R, co
       i = f / (n2 + 1) - 1;
       j = f % (n2 + 1);
     } else {
-      const int rel = slot - ((slot < bD) ? bC : bD);
-      if ((rel & 1) && !finalstep) cc = -1;  // COUNTFLOORACT / the repair: accounting substage only
-      const int f = rel >> 1;
+      // a zone's third slot (the floors) and a failed zone's second (the
+      // repair) count on the accounting substage only
+      const bool zone = slot < bD;
+      const int rel = slot - (zone ? bC : bD);
+      const int f = zone ? rel / 3 : rel >> 1, sub = zone ? rel - 3 * f : rel & 1;
+      if (sub == (zone ? 2 : 1) && !finalstep) cc = -1;
       i = f / n2;
       j = f % n2;
     }
@@ -724,6 +851,7 @@ static void acct_set_on(harm_ctx* c, bool on) {
   if (a) a->on = on;
   const bool k = a && on;
   c->kp.acct_pre = k ? a->pre : nullptr;
+  c->kp.acct_uin = k ? a->uin : nullptr;
   c->kp.acct_flag = k ? a->flag : nullptr;
   c->kp.acct_val = k ? a->val : nullptr;
 }
@@ -744,6 +872,7 @@ static int acct_alloc(harm_ctx* c) {
   a->nslots = acct_base(n1, n2, 4);
   a->nblocks = (a->nslots + 255) / 256;
   if (e == hipSuccess) e = hipMalloc(&a->pre, (size_t)c->nz * 8 * sizeof(double));
+  if (e == hipSuccess && c->prm.eomtype == HARM_EOMFFDE) e = hipMalloc(&a->uin, (size_t)c->nz * 8 * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&a->flag, (size_t)a->nslots * sizeof(int));
   if (e == hipSuccess) e = hipMemset(a->flag, 0, (size_t)a->nslots * sizeof(int));
   if (e == hipSuccess) e = hipMalloc(&a->val, (size_t)a->nslots * 8 * sizeof(double));
@@ -760,8 +889,8 @@ static int acct_alloc(harm_ctx* c) {
 static void acct_free(harm_ctx* c) {
   AcctState* a = c->acct;
   if (!a) return;
-  void* bufs[14] = {a->acc, a->frdot, a->lumvsr, a->scratch, a->out, a->faces, a->pre,
-                     a->flag, a->val,   a->list,   a->cval,  a->cnt, a->off,   a->total};
+  void* bufs[15] = {a->acc,  a->frdot, a->lumvsr, a->scratch, a->out, a->faces, a->pre,  a->uin,
+                     a->flag, a->val,   a->list,   a->cval,    a->cnt, a->off,   a->total};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete a;

@@ -35,9 +35,11 @@ struct KParams {
   double gam, cour, defcon, safe, gammamax, gammafail, rhominlimit, uuminlimit;
   double dx1, dx2;
   // floor/failure accounting (diag_fixup), set only while it is on: dVF,
-  // the zone states before fixup1zone, and the event slots of a substage
+  // the zone states before fixup1zone, the conserved state each force-free
+  // inversion took (its dissipation account), and the event slots of a substage
   double dVF;
   double* acct_pre;
+  double* acct_uin;
   int* acct_flag;
   double* acct_val;
 };

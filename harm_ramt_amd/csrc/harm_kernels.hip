@@ -982,7 +982,8 @@ __device__ __forceinline__ void eno_cvt_zone(const KParams& P, const double* __r
 // - U(before)) in UDIAG form, booked in the energy regions holding that zone
 // or face.  The events of a substage have fixed slots in the order the
 // reference makes them: x1 faces (FZLOOP, left state then right), x2 faces,
-// zones (limit_gamma, then floors), failed zones (limit_gamma, then the repair).
+// zones (the force-free inversion's dissipation account, limit_gamma, then
+// floors), failed zones (limit_gamma, then the repair).
 // A flag (category + 1) marks a slot in use; the slots are compacted in order
 // and summed after the substage.
 
@@ -1014,10 +1015,12 @@ __device__ void acct_event(const KParams& P, const GeoU& g, const double* pr0, c
   P.acct_flag[slot] = cat + 1;
 }
 
-// slot bases of the four event classes of a substage
+// slot bases of the four event classes of a substage: x1 and x2 faces (two
+// states each), zones (three: the force-free inversion's dissipation account,
+// limit_gamma, the floors), failed zones (two: limit_gamma, the repair)
 __host__ __device__ __forceinline__ int acct_base(int n1, int n2, int cls) {
-  const int nA = (n1 + 1) * (n2 + 2) * 2, nB = (n1 + 2) * (n2 + 1) * 2, nC = n1 * n2 * 2;
-  return cls == 0 ? 0 : cls == 1 ? nA : cls == 2 ? nA + nB : cls == 3 ? nA + nB + nC : nA + nB + 2 * nC;
+  const int nA = (n1 + 1) * (n2 + 2) * 2, nB = (n1 + 2) * (n2 + 1) * 2, nC = n1 * n2 * 3, nD = n1 * n2 * 2;
+  return cls == 0 ? 0 : cls == 1 ? nA : cls == 2 ? nA + nB : cls == 3 ? nA + nB + nC : nA + nB + nC + nD;
 }
 
 // Utoprimgen of U into pf[z] from the guess pin (the pf=pi assignment,
@@ -1042,7 +1045,10 @@ __device__ __forceinline__ void invert_zone(const KParams& P, const double* U, c
     int slot = atomicAdd(&ds->nlist, 1);
     if (slot < P.n1 * P.n2) faillist[slot] = z;
   }
-  if (P.acct_pre) store8(P.acct_pre, z, pout);
+  if (P.acct_pre) {
+    store8(P.acct_pre, z, pout);
+    if (P.eomtype == HARM_EOMFFDE) store8(P.acct_uin, z, U);
+  }
   // fixup1zone: floors then limit_gamma
   if (P.eomtype == 0) {
     double fr = gc[33], fu = gc[34];
@@ -3698,6 +3704,7 @@ extern "C" int harm_batch_create(harm_ctx* const* ctxs, int n, harm_batch** out)
     RunDesc& d = hr[r];
     d.P = c->kp;
     d.P.acct_pre = nullptr;  // a batch does no accounting
+    d.P.acct_uin = nullptr;
     d.P.acct_flag = nullptr;
     d.P.acct_val = nullptr;
     for (int a = 0; a < HARM_NARRAYS; a++) d.arr[a] = c->arr[a];

@@ -72,6 +72,7 @@ struct oracle_ctx {
   long nstroke;
   int failed;
   int acct;          /* conservation accounting on */
+  int ffde_pr0_zero; /* force-free fixup1zone measures from pr0 = 0 (see or_acct_ffde_pr0_zero) */
   int finalstep;     /* the substage being advanced is the step's last */
   or_acct A;
   double dx3;
@@ -875,11 +876,13 @@ static int Utoprimgen(const oracle_ctx *c, const double *U, const geom_t *g, dou
 /* fixup.c:274-377 fixup1zone (floors from the precomputed table, then
    limit_gamma); a change is accounted as COUNTFLOORACT on the final substage
    (measured from the state before the floors).  Force-free leaves pr0
-   unset in the reference (fixup.c:293-313); here it is the incoming state. */
+   unset in the reference (fixup.c:293-313); here it is the incoming state,
+   or zero with ffde_pr0_zero (the reference with zero-initialised locals). */
 static int fixup1zone(oracle_ctx *c, double *pr, const geom_t *g, int finalstep) {
   double pr0[NPR];
   int changed = 0;
   memcpy(pr0, pr, sizeof(pr0));
+  if (c->p.eomtype == OR_FFDE && c->ffde_pr0_zero) memset(pr0, 0, sizeof(pr0));
   if (c->p.eomtype == OR_GRMHD) {
     const double *fl = c->floors + (size_t)zi(c, g->i, g->j) * 2;
     double prfloor[2] = {fl[0], fl[1]};
@@ -1034,10 +1037,128 @@ static void dUtoU(const double *dUgeom, const double *dUriemann, const double *C
     un[k] += Cunew[0] * Ui[k] + Cunew[1] * dt * (dUriemann[k] + dUgeom[k]) + Cunew[2] * Uf[k];
 }
 
+/* fixup.c:234-270 diag_fixup_U: dVF (Uf - Ui) in every region holding the
+   zone, on every substage (its finalstep test is forced true) */
+static void diag_fixup_U(oracle_ctx *c, const double *Ui, const double *Uf, const geom_t *g, int whocalled) {
+  for (int r = 0; r < OR_NREG; r++) {
+    if (!in_region(c->A.enerpos[r], g->i, g->j)) continue;
+    for (int k = 0; k < NPR; k++) {
+      double ftemp = c->A.dVF * (Uf[k] - Ui[k]);
+      c->A.fladdterms[r][whocalled][k] += ftemp;
+      c->A.fladd[r][k] += ftemp;
+    }
+  }
+}
+
+/* The force-free inversion's dissipation account (phys.ffde.c:354-364 with
+   ACCOUNTDISS 1, diss_account :572-607 with ACCOUNTNONCONS 1), for the
+   geometry-free Ug the inversion took and the state pr it returned.  The
+   inversion's E/alpha and its Lorentz-factor test are formed again from Ug
+   as Utoprim_ffde forms them.  Initial T^t_mu from E and B alone (FitUtoT,
+   :917-935: Max_con :1472-1552, ffdestresstensor :1430-1462, lower_A
+   :1654-1682), final U from pr (primtoU UDIAG), the energy lost by the
+   non-conservative inversion added back; COUNTUTOPRIMFAILCONV when the cap
+   acted, COUNTFLOORACT otherwise. */
+static void diss_account(oracle_ctx *c, const double *Ug, const geom_t *g, const double *pr) {
+  const double (*gcv)[NDIM] = g->gcov;
+  const double (*gcn)[NDIM] = g->gcon;
+  double BconOalpha[NDIM], BcovOalpha[NDIM], BsqOalphasq, EconOalpha[NDIM], EcovOalpha[NDIM], EsqOalphasq;
+  int j, k, i, modified = 0;
+  BconOalpha[0] = 0.0;
+  for (j = 1; j < NDIM; j++) BconOalpha[j] = Ug[B1 + j - 1];
+  lower(BconOalpha, g, BcovOalpha);
+  BsqOalphasq = 0.0;
+  for (j = 1; j < NDIM; j++) BsqOalphasq += BcovOalpha[j] * BconOalpha[j];
+  {
+    double o = 1.0 / (BsqOalphasq * g->g);
+    EconOalpha[0] = 0;
+    EconOalpha[1] = (BcovOalpha[2] * Ug[UU + 3] - BcovOalpha[3] * Ug[UU + 2]) * (o);
+    EconOalpha[2] = (BcovOalpha[3] * Ug[UU + 1] - BcovOalpha[1] * Ug[UU + 3]) * (o);
+    EconOalpha[3] = (BcovOalpha[1] * Ug[UU + 2] - BcovOalpha[2] * Ug[UU + 1]) * (o);
+  }
+  lower(EconOalpha, g, EcovOalpha);
+  EsqOalphasq = 0.0;
+  for (j = 1; j < NDIM; j++) EsqOalphasq += EconOalpha[j] * EcovOalpha[j];
+  {
+    double gammasq = BsqOalphasq / (BsqOalphasq - EsqOalphasq);
+    if ((gammasq > c->p.gammamax * c->p.gammamax) || (gammasq < 1.0)) modified = 1;
+  }
+  /* FitUtoT: prffde U1..U3 = F^{it} = -E^i/alpha, B1..B3 = -B^i */
+  double Fit[NDIM], Bm[NDIM], etacov[NDIM], etacon[NDIM], Econ[NDIM], Ecov[NDIM];
+  double Mcon[NDIM][NDIM], Mcov[NDIM][NDIM], T[NDIM][NDIM], Msq;
+  double alpha = 1. / sqrt(-gcn[0][0]);
+  etacov[0] = -alpha;
+  for (j = 1; j < NDIM; j++) etacov[j] = 0.0;
+  etacon[0] = gcn[0][0] * etacov[0] + gcn[0][1] * etacov[1] + gcn[0][2] * etacov[2] + gcn[0][3] * etacov[3];
+  etacon[1] = gcn[0][1] * etacov[0] + gcn[1][1] * etacov[1] + gcn[1][2] * etacov[2] + gcn[1][3] * etacov[3];
+  etacon[2] = gcn[0][2] * etacov[0] + gcn[1][2] * etacov[1] + gcn[2][2] * etacov[2] + gcn[2][3] * etacov[3];
+  etacon[3] = gcn[0][3] * etacov[0] + gcn[1][3] * etacov[1] + gcn[2][3] * etacov[2] + gcn[3][3] * etacov[3];
+  Fit[0] = 0.0;
+  for (j = 1; j < NDIM; j++) Fit[j] = -EconOalpha[j];
+  Bm[0] = 0.0;
+  for (j = 1; j < NDIM; j++) Bm[j] = etacov[0] * (-Ug[B1 + j - 1]);
+  for (j = 0; j < NDIM; j++) Econ[j] = etacov[0] * Fit[j];
+  lower(Econ, g, Ecov);
+  Mcon[0][0] = Mcon[1][1] = Mcon[2][2] = Mcon[3][3] = 0.;
+  Mcon[1][2] = -etacon[1] * Bm[2] + etacon[2] * Bm[1] - (etacov[0] / g->g) * Ecov[3];
+  Mcon[2][3] = -etacon[2] * Bm[3] + etacon[3] * Bm[2] - (etacov[0] / g->g) * Ecov[1];
+  Mcon[3][1] = -etacon[3] * Bm[1] + etacon[1] * Bm[3] - (etacov[0] / g->g) * Ecov[2];
+  Mcon[2][1] = -Mcon[1][2];
+  Mcon[3][2] = -Mcon[2][3];
+  Mcon[1][3] = -Mcon[3][1];
+  for (j = 1; j < NDIM; j++) {
+    Mcon[0][j] = Bm[j] / etacov[0];
+    Mcon[j][0] = -Mcon[0][j];
+  }
+  /* lower_A: the six upper entries summed over (j, k) in loop order */
+  for (j = 0; j < NDIM; j++)
+    for (k = 0; k < NDIM; k++) Mcov[j][k] = 0.;
+  for (j = 0; j < NDIM; j++)
+    for (k = 0; k < NDIM; k++) {
+      Mcov[0][1] += (gcv[0][j]) * (gcv[1][k]) * Mcon[j][k];
+      Mcov[0][2] += (gcv[0][j]) * (gcv[2][k]) * Mcon[j][k];
+      Mcov[0][3] += (gcv[0][j]) * (gcv[3][k]) * Mcon[j][k];
+      Mcov[1][2] += (gcv[1][j]) * (gcv[2][k]) * Mcon[j][k];
+      Mcov[2][3] += (gcv[2][j]) * (gcv[3][k]) * Mcon[j][k];
+      Mcov[3][1] += (gcv[3][j]) * (gcv[1][k]) * Mcon[j][k];
+    }
+  Mcov[1][0] = -Mcov[0][1];
+  Mcov[2][0] = -Mcov[0][2];
+  Mcov[3][0] = -Mcov[0][3];
+  Mcov[2][1] = -Mcov[1][2];
+  Mcov[3][2] = -Mcov[2][3];
+  Mcov[1][3] = -Mcov[3][1];
+  /* ffdestresstensor: T^a_b = M^ac M_bc - 1/4 delta (M_cd M^cd) */
+  Msq = 0.;
+  for (j = 0; j < NDIM; j++)
+    for (k = 0; k < NDIM; k++) Msq += Mcov[j][k] * Mcon[j][k];
+  for (j = 0; j < NDIM; j++)
+    for (k = 0; k < NDIM; k++) {
+      T[j][k] = 0.;
+      for (i = 0; i < NDIM; i++) T[j][k] += Mcon[j][i] * Mcov[k][i];
+    }
+  for (j = 0; j < NDIM; j++) T[j][j] -= 0.25 * Msq;
+  double Ui[NPR], Uf[NPR];
+  state_t q;
+  Ui[RHO] = 0.0;
+  for (j = 0; j < NDIM; j++) Ui[UU + j] = g->g * T[0][j];
+  for (j = 1; j < NDIM; j++) Ui[B1 + j - 1] = g->g * Ug[B1 + j - 1];
+  get_state(pr, g, &q);
+  primtoU_diag(c, pr, &q, g, Uf);
+  Uf[RHO] = 0.0;
+  Uf[UU] -= (g->g * Ug[UU] - Ui[UU]);
+  diag_fixup_U(c, Ui, Uf, g, modified ? COUNTUTOPRIMFAILCONV : COUNTFLOORACT);
+}
+
 /* Utoprimgen of U into pf (guess: pf as it stands) and fixup1zone
    (advance.c:164-180 / 460-476); returns the failure flag */
 static int invert_zone(oracle_ctx *c, int i, int j, const geom_t *g, const double *U, double *pfz, long *nstroke) {
   int pfl, fail = Utoprimgen(c, U, g, pfz, &pfl, nstroke);
+  if (c->acct && c->p.eomtype == OR_FFDE && !fail) {
+    double Ug[NPR];
+    for (int k = 0; k < NPR; k++) Ug[k] = U[k] / g->e[k];
+    diss_account(c, Ug, g, pfz);
+  }
   c->pflag[(size_t)zi(c, i, j) * NUMPFLAGS + 0] = pfl;
   if (fixup1zone(c, pfz, g, c->finalstep)) fail |= 1;
   return fail;
@@ -1767,6 +1888,8 @@ void or_acct_setup(oracle_ctx *c, const int *enerpos, const int *doflux, double 
 }
 
 void or_acct_off(oracle_ctx *c) { c->acct = 0; }
+
+void or_acct_ffde_pr0_zero(oracle_ctx *c, int on) { c->ffde_pr0_zero = on; }
 
 void or_acct_get(const oracle_ctx *c, or_acct *out, double *frdot, double *lumvsr) {
   if (out) *out = c->A;

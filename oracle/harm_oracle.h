@@ -123,6 +123,11 @@ typedef struct {
    steps then run their zone loops in the reference's order, one thread) */
 void or_acct_setup(oracle_ctx *c, const int *enerpos, const int *doflux, double dx3);
 void or_acct_off(oracle_ctx *c);
+/* Force-free fixup1zone's COUNTFLOORACT event measured from pr0 = 0 instead
+   of the incoming state: the reference declares pr0 and didchangeprim without
+   setting them under EOMFFDE (fixup.c:283-313), and with its locals
+   zero-initialised (oracle/ref ffdezi build) that is what it books. */
+void or_acct_ffde_pr0_zero(oracle_ctx *c, int on);
 /* the accumulators, frdot[n1][8] and lumvsr[n1] */
 void or_acct_get(const oracle_ctx *c, or_acct *out, double *frdot, double *lumvsr);
 /* diag_flux() of the current state and fluxes with time step Dt (the step

@@ -93,6 +93,7 @@ def load():
         "or_eno_weight": ([ctypes.c_int] * 4, ctypes.c_double),
         "or_acct_setup": ([vp, ip, ip, ctypes.c_double], None),
         "or_acct_off": ([vp], None),
+        "or_acct_ffde_pr0_zero": ([vp, ctypes.c_int], None),
         "or_acct_get": ([vp, ctypes.POINTER(OracleAcct), dp, dp], None),
         "or_diag_flux": ([vp, ctypes.c_double], None),
         "or_divbmaxavg": ([vp, ctypes.c_int, dp, dp, ip, ip], None),
@@ -254,6 +255,11 @@ class Oracle:
 
     def acct_off(self):
         self.lib.or_acct_off(self.h)
+
+    def acct_ffde_pr0_zero(self, on=True):
+        """force-free fixup1zone events measured from pr0 = 0, as the
+        reference books them with zero-initialised locals"""
+        self.lib.or_acct_ffde_pr0_zero(self.h, int(on))
 
     def acct(self):
         """the accumulators as a dict of arrays (plus frdot[n1][8], lumvsr[n1])"""

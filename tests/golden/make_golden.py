@@ -168,6 +168,19 @@ DIAG_CASES = {
 # GAMMAMAX lowered (initbase.c:218) so that limit_gamma acts on face states
 # and zones: its accounting (COUNTLIMITGAMMAACT) is exercised
 DIAG_GAMMAMAX = {"diag_ot_gammamax_mc_hll_rk2": ("mink", 32, 32, H.MC, H.HLLFLUX, 2, 10, 1e-4, 1.1)}
+# the other equation and conversion modes with accounting: force-free
+# (EOMFFDE), CHECKSOLUTION 1 (b^2 flags and checksolution repairs), DOENO 1
+# and 2 (the source and flux terms of the converted update)
+DIAG_CASES.update({
+    "diag_ffde_monopole_mc_laxf_rk2": ("ffde", 32, 64, H.MC, H.LAXFFLUX, 2, 20, 1e-3),
+    "diag_torus_field_checksolution_rk4": ("ks_mono", 32, 64, H.PARA, H.HLLFLUX, 4, 20, 1e-3),
+    "diag_torus_eno_u_para_hll_rk4": ("ks", 32, 64, H.PARA, H.HLLFLUX, 4, 12, 1e-3),
+    "diag_ot_eno_du_mc_hll_rk2": ("mink", 32, 32, H.MC, H.HLLFLUX, 2, 10, 1e-4),
+})
+DIAG_ENO = {"diag_torus_eno_u_para_hll_rk4": 1, "diag_ot_eno_du_mc_hll_rk2": 2}
+# force-free: fixup1zone() books COUNTFLOORACT from locals it never sets under
+# EOMFFDE (fixup.c:283-313); the ffdezi build zero-initialises them
+DIAG_BINARY = {"diag_ffde_monopole_mc_laxf_rk2": "ref_ffdezi"}
 NREG, NFL, NSRC, NFLOOR = 3, 7, 7, 9
 DIAG_LAYOUT = [("enerpos", (NREG, 4)), ("doflux", (NREG, 4)), ("pdot", (NREG, 4, 8)),
                ("pdotterms", (NREG, 4, NFL, 8)), ("pcum", (NREG, 4, 8)), ("fladd", (NREG, 8)),
@@ -196,9 +209,15 @@ def make_diag(name, work):
     else:
         (kind, n1, n2, lim, flux, to, nsteps, dt0), gmax = DIAG_CASES[name], None
     g, geom, p0, extra = problem(kind, n1, n2)
-    recs, states = run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, diag=True, gammamax=gmax)
+    eno = DIAG_ENO.get(name, 0)
+    recs, states = run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, eno=eno, diag=True,
+                                 gammamax=gmax, binary=DIAG_BINARY.get(name))
     dg = read_diag(os.path.join(work, "diag.bin"), n1)
     more = {} if gmax is None else {"gammamax": gmax}
+    if eno:
+        more["eno"] = eno
+    if name in DIAG_BINARY:
+        more["zero_init_locals"] = 1
     np.savez_compressed(os.path.join(OUT_DIR, f"{name}.npz"), kind=kind, n1=n1, n2=n2, lim=lim, fluxmethod=flux,
                         timeorder=to, nsteps=nsteps, dt0=dt0, input_sha=sha(p0), t=recs[:, 0], dt=recs[:, 1],
                         fail=recs[:, 3], p=states[-1], h_over_r_jet=0.4, **more, **dg)

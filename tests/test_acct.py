@@ -11,7 +11,15 @@ divbmaxavg()/constotal() of the final state).
   rho dVF N, the energy total -rho-u per zone; a field with B1 a function of
   x2 only and B2 of x1 only has zero corner divergence.
 - The cases cover floors and repaired inversion failures (torus) and
-  limit_gamma on face and zone states (Orszag-Tang with GAMMAMAX 1.1).
+  limit_gamma on face and zone states (Orszag-Tang with GAMMAMAX 1.1),
+  CHECKSOLUTION (b^2 flags and checksolution repairs), DOENO 1 and 2, and
+  force-free, whose every inversion books its dissipation
+  (phys.ffde.c:572-607).  Force-free fixup1zone() books COUNTFLOORACT from
+  locals the reference never sets under EOMFFDE (fixup.c:283-313); the
+  force-free reference case is the reference built with those locals zero
+  (oracle/ref, ffdezi), which the oracle reproduces in full with pr0 = 0;
+  with its own choice (pr0 = the incoming state) every accumulator but
+  that category and fladd is the reference's.
 - The device path (-m gpu): the same accumulators bit for bit on the
   reference's cases, at a larger grid against the oracle, with the state of
   the steps unchanged by the accounting, and the dump-time reductions of the
@@ -24,11 +32,13 @@ import harm_ramt_amd as H
 from oracle.oracle import Oracle
 from tests.helpers import golden_problem, load_golden, problem, same, sha
 
-CASES = ["diag_torus_para_hll_rk4", "diag_ot_mc_hll_rk2", "diag_ot_gammamax_mc_hll_rk2"]
+CASES = ["diag_torus_para_hll_rk4", "diag_ot_mc_hll_rk2", "diag_ot_gammamax_mc_hll_rk2",
+         "diag_torus_field_checksolution_rk4", "diag_torus_eno_u_para_hll_rk4", "diag_ot_eno_du_mc_hll_rk2"]
+FFDE_CASE = "diag_ffde_monopole_mc_laxf_rk2"
 STEP_KEYS = ["pdot", "pdotterms", "pcum", "sourceadd", "sourceaddterms", "pdottermsjet2", "frdot", "lumvsr"]
 
 
-def _oracle_run(gd, nsteps=None):
+def _oracle_run(gd, nsteps=None, pr0_zero=False):
     g, geom, p0, prm = golden_problem(gd)
     assert sha(p0) == str(gd["input_sha"])
     o = Oracle(H.params_dict(prm))
@@ -37,12 +47,14 @@ def _oracle_run(gd, nsteps=None):
     o.bound_prim(0)
     o.set_time(0.0, float(gd["dt0"]), 1e30)
     o.acct_setup(gd["enerpos"], gd["doflux"], 2 * np.pi)
+    if pr0_zero:
+        o.acct_ffde_pr0_zero()
     for _ in range(int(gd["nsteps"]) if nsteps is None else nsteps):
         o.step()
     return o
 
 
-@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("name", CASES + [FFDE_CASE])
 def test_regions_match_reference(name):
     gd = load_golden(name)
     g, geom, p0, prm = golden_problem(gd)
@@ -69,6 +81,44 @@ def test_oracle_accounting_matches_reference(name):
     assert mx == gd["divbmax"] and av == gd["divbavg"]
     for r in range(3):
         assert same(o.constotal(r), gd["constotal"][r]), r
+
+
+def test_oracle_force_free_accounting_matches_reference():
+    """every accumulator of the force-free reference case, with fixup1zone's
+    events measured from pr0 = 0 as the zero-initialised reference books them"""
+    gd = load_golden(FFDE_CASE)
+    assert int(gd["zero_init_locals"]) == 1
+    o = _oracle_run(gd, pr0_zero=True)
+    assert same(o.get_array(0), gd["p"])
+    a = o.acct()
+    for k in STEP_KEYS + ["fladd", "fladdterms"]:
+        assert same(a[k], gd[k]), k
+    mx, av, im, jm = o.divbmaxavg()
+    assert mx == gd["divbmax"] and av == gd["divbavg"]
+    for r in range(3):
+        assert same(o.constotal(r), gd["constotal"][r]), r
+
+
+def test_oracle_force_free_accounting_own_choice():
+    """with pr0 = the incoming state only fixup1zone's COUNTFLOORACT events
+    differ: the dissipation account (categories 0 and 1 of the inversions),
+    limit_gamma's and every flux and source accumulator are the reference's;
+    the jets' regions hold no fixup1zone event and match in full"""
+    gd = load_golden(FFDE_CASE)
+    o = _oracle_run(gd)
+    a = o.acct()
+    for k in STEP_KEYS:
+        assert same(a[k], gd[k]), k
+    ft, want = a["fladdterms"], gd["fladdterms"]
+    for r in range(3):
+        for cat in range(ft.shape[1]):
+            if (r, cat) != (0, 1):
+                assert same(ft[r][cat], want[r][cat]), (r, cat)
+    assert same(a["fladd"][1:], gd["fladd"][1:])
+    assert not same(ft[0][1], want[0][1])  # limit_gamma acted in region 0
+    # the dissipation account: energy and momenta, never the field
+    assert np.count_nonzero(want[0][0][1:5]) and not np.count_nonzero(want[0][0][5:])
+    assert np.count_nonzero(want[1][1][1:5])
 
 
 def test_reference_cases_exercise_the_accumulators():
@@ -159,6 +209,29 @@ def test_gpu_accounting_matches_reference(name, graph):
     assert mx == gd["divbmax"] and av == gd["divbavg"]
     for r in range(3):
         assert same(c.constotal(r), gd["constotal"][r]), r
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("graph", [False, True])
+def test_gpu_force_free_accounting_matches_reference(graph):
+    """force-free: the dissipation account of every inversion and the rest of
+    the accumulators are the reference's; fixup1zone's COUNTFLOORACT events
+    (region 0 only here) are the oracle's, measured from the incoming state"""
+    gd = load_golden(FFDE_CASE)
+    print(f"[case] {FFDE_CASE} graph={graph}", flush=True)
+    c = _gpu_run(gd, graph=graph)
+    assert same(c.get_array(H.P), gd["p"]), "accounting changed the step"
+    a = c.acct()
+    for k in STEP_KEYS:
+        assert same(a[k], gd[k]), k
+    ft, want = a["fladdterms"], gd["fladdterms"]
+    for r in range(3):
+        for cat in range(ft.shape[1]):
+            if (r, cat) != (0, 1):
+                assert same(ft[r][cat], want[r][cat]), (r, cat)
+    assert same(a["fladd"][1:], gd["fladd"][1:])
+    o = _oracle_run(gd).acct()
+    assert same(a["fladd"], o["fladd"]) and same(ft, o["fladdterms"])
 
 
 @pytest.mark.gpu
