@@ -33,11 +33,12 @@ HBM_PEAK_GBS = 8000.0
 #   k_flux_single, k_flux  per face (both directions in one launch):
 #              p of one zone (64) + packed face geometry (16 doubles, 128)
 #              + F (64) + EMF quotient q (8)                         = 264 B
-#   k_update   per zone: pi, pb (128) + CENT geometry (36 doubles, 288)
-#              + connection (64 doubles, 512) + F1, F2 (128) + q1, q2 (16)
-#              + ulast, unew read (128) + pf, ulast, unew written (192)
-#              + pflag (4)                                          = 1396 B
-BYTES_PER_UNIT = {"flux": 264, "update": 1396}
+#   k_update   per zone: pi, pb (128) + CENT geometry (gcov upper
+#              triangle, gcon, gdet, two floors, corner gdet: 30 doubles,
+#              240) + connection (Gamma^k_{lj}, l <= j: 40 doubles, 320)
+#              + F1, F2 (128) + q1, q2 (16) + ulast, unew read (128)
+#              + pf, ulast, unew written (192) + pflag (4)          = 1156 B
+BYTES_PER_UNIT = {"flux": 264, "update": 1156}
 
 
 def units_per_launch(stage, n1, n2):

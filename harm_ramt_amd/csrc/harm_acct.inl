@@ -177,7 +177,7 @@ __global__ void k_acct_srcterm(KParams P, AcctRegions R, const double* __restric
   geo_from_cent(gc, g);
   double pr[NPR], dU[NPR], ft[NPR], fd[NPR];
   load8(pb, z, pr);
-  source_conn(P, pr, g, gConn + z, dU, (size_t)(P.n1 + 2 * NG) * P.N2M);
+  source_conn_sym(P, pr, g, gConn + z, dU, (size_t)(P.n1 + 2 * NG) * P.N2M);
   const double Dt = ds->dt;
 #pragma unroll
   for (int k = 0; k < NPR; k++) ft[k] = Dt * R.dVF * (0.0 + dU[k]);  // dUcomp[GEOMSOURCE] = 0 e + e dUconn
@@ -387,8 +387,8 @@ __device__ double corner_divb(const KParams& P, const double* __restrict__ p, co
   if ((j != 0) && (j != P.n2)) {
     const double dx1 = P.dx1, dx2 = P.dx2;
     const int z00 = zidx(P, i, j), z0m = zidx(P, i, j - 1), zm0 = zidx(P, i - 1, j), zmm = zidx(P, i - 1, j - 1);
-    const double g00 = gC[(size_t)z00 * GCN + 32], g0m = gC[(size_t)z0m * GCN + 32];
-    const double gm0 = gC[(size_t)zm0 * GCN + 32], gmm = gC[(size_t)zmm * GCN + 32];
+    const double g00 = gC[(size_t)z00 * GCN + GC_GDET], g0m = gC[(size_t)z0m * GCN + GC_GDET];
+    const double gm0 = gC[(size_t)zm0 * GCN + GC_GDET], gmm = gC[(size_t)zmm * GCN + GC_GDET];
     const double b100 = p[(size_t)z00 * 8 + B1], b10m = p[(size_t)z0m * 8 + B1];
     const double b1m0 = p[(size_t)zm0 * 8 + B1], b1mm = p[(size_t)zmm * 8 + B1];
     const double b200 = p[(size_t)z00 * 8 + B2], b20m = p[(size_t)z0m * 8 + B2];
@@ -398,7 +398,7 @@ __device__ double corner_divb(const KParams& P, const double* __restrict__ p, co
                         0.5 * (b200 * g00 + b2m0 * gm0 - b20m * g0m - b2mm * gmm) / dx2);
     const double sum = b100 * g00 + b10m * g0m + b1m0 * gm0 + b1mm * gmm + b200 * g00 + b2m0 * gm0 + b20m * g0m +
                        b2mm * gmm;
-    divb = fabs(num / (gC[(size_t)z00 * GCN + 35] * 0.125 * fabs(fabs(sum) + 1.e-20)));
+    divb = fabs(num / (gC[(size_t)z00 * GCN + GC_GDETC] * 0.125 * fabs(fabs(sum) + 1.e-20)));
   }
   return divb;
 }

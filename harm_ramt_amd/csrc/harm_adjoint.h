@@ -411,9 +411,11 @@ HARM_HD int inversion_rev(const KParams& P, const double* p, const GeoU& g, cons
   return 0;
 }
 
-// source_conn: dU[UU+l] = gdet sum_jk T^j_k Gamma^k_{lj} (conn[k][l][j])
-HARM_HD void source_rev(const KParams& P, const double* pr, const GeoU& g, const double* conn, const double* dUb,
-                        double* pb, size_t cs = 1) {
+// source_conn: dU[UU+l] = gdet sum_jk T^j_k Gamma^k_{lj} (conn[k][l][j]);
+// element e of the zone's 64 is cn(e)
+template <class CN>
+HARM_HD void source_rev_at(const KParams& P, const double* pr, const GeoU& g, const CN& cn, const double* dUb,
+                           double* pb) {
   State q;
   get_state(pr, g, q);
   StateBar qb;
@@ -422,12 +424,18 @@ HARM_HD void source_rev(const KParams& P, const double* pr, const GeoU& g, const
     double Tb[4];
     for (int kk = 0; kk < 4; kk++) {
       double s = 0.;
-      for (int l = 0; l < 4; l++) s += dUb[UU + l] * conn[(size_t)((kk * 4 + l) * 4 + jj) * cs];
+      for (int l = 0; l < 4; l++) s += dUb[UU + l] * cn((kk * 4 + l) * 4 + jj);
       Tb[kk] = g.gdet * s;
     }
     mhd_rev(P, pr, jj, q, Tb, qb, pb);
   }
   get_state_rev(pr, g, q, qb, pb);
+}
+
+// the same with element e at conn[e] (the zone's own row of 64)
+HARM_HD void source_rev(const KParams& P, const double* pr, const GeoU& g, const double* conn, const double* dUb,
+                        double* pb) {
+  source_rev_at(P, pr, g, [&](int e) { return conn[e]; }, dUb, pb);
 }
 
 // d(0.5 b^2 gdet dV)/dp at a zone (the magnetic-energy functional)

@@ -108,8 +108,7 @@ __global__ void k_adj_seed_bsq(KParams P, const double* p, const double* gC, dou
   double l[NPR] = {0., 0., 0., 0., 0., 0., 0., 0.};
   if (i >= 0 && i < P.n1 && j >= 0 && j < P.n2) {
     double gc[36];
-#pragma unroll
-    for (int m = 0; m < 36; m++) gc[m] = gC[(size_t)z * GCN + m];
+    load_cent(gC, z, gc);
     GeoU g;
     geo_from_cent(gc, g);
     double pr[NPR];
@@ -122,7 +121,7 @@ __global__ void k_adj_seed_bsq(KParams P, const double* p, const double* gC, dou
     for (int jj = 0; jj < P.n2; jj++) {
       const int zz = zidx(P, i, jj);
       double gc[36];
-      for (int m = 0; m < 36; m++) gc[m] = gC[(size_t)zz * GCN + m];
+      load_cent(gC, zz, gc);
       GeoU g;
       geo_from_cent(gc, g);
       double pr[NPR];
@@ -149,8 +148,7 @@ __global__ __launch_bounds__(256) void k_adj_zone(KParams P0, AdjRK R, const dou
   R.dt = dts[ctr[0]];
   const int z = zidx(P, i, j);
   double gc[36];
-#pragma unroll
-  for (int m = 0; m < 36; m++) gc[m] = gC[(size_t)z * GCN + m];
+  load_cent(gC, z, gc);
   GeoU g;
   geo_from_cent(gc, g);
   double p[NPR], lpf[NPR];
@@ -221,7 +219,7 @@ __device__ __forceinline__ void adj_face(const KParams& P, const double* __restr
   load_geo_face(gF, z0, g);
   if (P.fluxb == HARM_FLUXCTTOTH) {
     auto le = [&](int a, int b) {
-      return (a >= 0 && a <= P.n1 && b >= 0 && b <= P.n2) ? gC[(size_t)zidx(P, a, b) * GCN + 35] * lemf[zidx(P, a, b)]
+      return (a >= 0 && a <= P.n1 && b >= 0 && b <= P.n2) ? gC[(size_t)zidx(P, a, b) * GCN + GC_GDETC] * lemf[zidx(P, a, b)]
                                                            : 0.;
     };
     if (DIR == 1) {
@@ -338,14 +336,16 @@ __global__ __launch_bounds__(256) void k_adj_gather(KParams P0, const double* co
   }
   if (i >= 0 && i < P.n1 && j >= 0 && j < P.n2) {
     double gc[36];
-#pragma unroll
-    for (int m = 0; m < 36; m++) gc[m] = gC[(size_t)z * GCN + m];
+    load_cent(gC, z, gc);
     GeoU g;
     geo_from_cent(gc, g);
     double pr[NPR], ldu[NPR];
     load8(pb, z, pr);
     load8(ldU, z, ldu);
-    source_rev(P, pr, g, gConn + z, ldu, l, (size_t)(P.n1 + 2 * NG) * P.N2M);
+    {
+      const size_t cs = (size_t)(P.n1 + 2 * NG) * P.N2M;
+      source_rev_at(P, pr, g, [&](int e) { return gConn[(size_t)conn_sym(e) * cs + z]; }, ldu, l);
+    }
   }
   store8(lp, z, l);
 }
@@ -361,8 +361,7 @@ __global__ __launch_bounds__(256) void k_adj_ui(KParams P, const double* const* 
   const double* __restrict__ pn = slot[0];
   const int z = zidx(P, i, j);
   double gc[36];
-#pragma unroll
-  for (int m = 0; m < 36; m++) gc[m] = gC[(size_t)z * GCN + m];
+  load_cent(gC, z, gc);
   GeoU g;
   geo_from_cent(gc, g);
   double pr[NPR], lui[NPR], l[NPR];

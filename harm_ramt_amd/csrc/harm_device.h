@@ -298,11 +298,27 @@ HARM_HD void source_conn_at(const KParams& P, const T* pr, const GeoT<T>& g, con
   dU[B3] = g.gdet * 0.;
 }
 
-// the same with element e at cn[e * cs] (cs = 1: the zone's own row; on the
-// device the table is component-major, cs = the number of zones)
+// the same with element e at cn[e * cs] (cs = 1: the zone's own row of 64)
 template <class T>
 HARM_HD void source_conn(const KParams& P, const T* pr, const GeoT<T>& g, const T* cn, T* dU, size_t cs = 1) {
   source_conn_at(P, pr, g, [&](int e) { return cn[(size_t)e * cs]; }, dU);
+}
+
+// conn[k][l][j] is symmetric in (l, j) (set_grid() computes it so, to the
+// bit): the device table keeps each k's ten entries with l <= j, component-
+// major ([NCONN][zones]).  Element e = (k*4 + l)*4 + j of the full table is
+// entry conn_sym(e) there.
+constexpr int NCONN = 40;
+HARM_HD constexpr int conn_sym(int e) {
+  const int k = e >> 4, l = (e >> 2) & 3, j = e & 3;
+  const int a = (l < j) ? l : j, b = (l < j) ? j : l;
+  return k * 10 + a * 4 - (a * (a - 1)) / 2 + (b - a);
+}
+
+// source_conn from the symmetric component-major table (cs = the number of zones)
+template <class T>
+HARM_HD void source_conn_sym(const KParams& P, const T* pr, const GeoT<T>& g, const T* cn, T* dU, size_t cs) {
+  source_conn_at(P, pr, g, [&](int e) { return cn[(size_t)conn_sym(e) * cs]; }, dU);
 }
 
 // ---------------- reconstruction (interp.c) ----------------

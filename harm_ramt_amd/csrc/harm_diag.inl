@@ -74,7 +74,7 @@ __global__ void k_cur_calc(KParams P, const double* __restrict__ cf, const doubl
   const bool full = st[0] >= 2.0;
   const double dtl = st[1], dtr = ds->dt_step;
   const double idx1 = 1.0 / (2.0 * P.dx1), idx2 = 1.0 / (2.0 * P.dx2);
-  auto GD = [&](int a, int b) { return gC[(size_t)zidx(P, a, b) * GCN + 32]; };
+  auto GD = [&](int a, int b) { return gC[(size_t)zidx(P, a, b) * GCN + GC_GDET]; };
   auto CF = [&](int a, int b, int s, int m) { return cf[(size_t)zidx(P, a, b) * 15 + s * 3 + m]; };
   const double gt = GD(i, j), gr = GD(i, j), grp1 = GD(i + 1, j), ghp1 = GD(i, j + 1), grm1 = GD(i - 1, j),
                ghm1 = GD(i, j - 1);

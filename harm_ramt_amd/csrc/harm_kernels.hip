@@ -20,8 +20,10 @@
 // Device layout (HBM):
 //   primitive-shaped arrays: the reference's AoS p[N1M][N2M][8] (64 B/zone);
 //   gF[2][nz][16]  FACE1/FACE2: gcov upper 10, gcon^{00..03}, gcon^{dd}, gdet
-//   gC[nz][40]     CENT: gcov 16, gcon 16, gdet, floor rho, floor u, gdet(CORN)
-//   gConn[nz][64]  CENT connection Gamma^k_{lj} as conn[k][l][j]
+//   gC[nz][30]     CENT: gcov upper 10 (gcov is symmetric to the bit), gcon 16
+//                  (whose row 0 can hold -0 where its column holds +0), gdet,
+//                  floor rho, floor u, gdet(CORN)
+//   gConn[40][nz]  CENT connection Gamma^k_{lj}, l <= j (conn_sym), component-major
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 #include <string.h>
@@ -44,7 +46,8 @@ namespace {
 
 constexpr int NG = 3;
 constexpr int GF = 16;   // doubles per face geometry record
-constexpr int GCN = 40;  // doubles per CENT geometry record
+constexpr int GCN = 30;  // doubles per CENT geometry record
+constexpr int GC_GDET = 26, GC_GDETC = 29;  // gdet at CENT and at CORN in the record
 
 // Accumulators that every wavefront of a launch updates are striped over
 // NSTRIPE slots, one 128-byte line apart: thousands of same-address atomics
@@ -598,7 +601,7 @@ __global__ __launch_bounds__(64 * FSH_ROWS) void k_flux_shared(KParams P, const 
 __device__ __forceinline__ double emf_at(const KParams& P, const double* __restrict__ q1,
                                          const double* __restrict__ q2, const double* __restrict__ gC, int i, int j) {
   const int z = zidx(P, i, j);
-  return 0.25 * (gC[(size_t)z * GCN + 35]) *
+  return 0.25 * (gC[(size_t)z * GCN + GC_GDETC]) *
          (q1[z] + q1[zidx(P, i, j - 1)] - q2[z] - q2[zidx(P, i - 1, j)]);
 }
 
@@ -909,15 +912,32 @@ __device__ int utoprimgen(const KParams& P, const double* U, const double* gc, d
 // DOENO forms (advance.c:101-158 and 298-487), zones i<N1, j<N2.
 // ---------------------------------------------------------------------------
 
+// the full CENT record (gcov 16, gcon 16, gdet, floor rho, floor u,
+// gdet(CORN)) from the stored one, whose element m is r(m)
+template <class T, class R>
+__device__ __forceinline__ void cent_expand(const R& r, T* gc) {
+  constexpr int up[16] = {0, 1, 2, 3, 1, 4, 5, 6, 2, 5, 7, 8, 3, 6, 8, 9};  // [a][b] -> upper entry
+#pragma unroll
+  for (int m = 0; m < 16; m++) {
+    gc[m] = r(up[m]);
+    gc[16 + m] = r(10 + m);
+  }
+#pragma unroll
+  for (int m = 0; m < 4; m++) gc[32 + m] = r(GC_GDET + m);
+}
+
 // full CENT geometry record of zone z
 __device__ __forceinline__ void load_cent(const double* __restrict__ gC, int z, double* gc) {
   const double2* a = reinterpret_cast<const double2*>(gC + (size_t)z * GCN);
+  double v[GCN];
 #pragma unroll
-  for (int m = 0; m < 18; m++) {
-    double2 t = a[m];
-    gc[2 * m] = t.x;
-    gc[2 * m + 1] = t.y;
+  for (int m = 0; m < GCN / 2; m++) {
+    const double2 t = a[m];
+    v[2 * m] = t.x;
+    v[2 * m + 1] = t.y;
   }
+  cent_expand(
+      [&](int m) { return v[m]; }, gc);
 }
 
 // flux2dUavg (advance.c:204-265): each direction's flux difference, with the
@@ -1143,7 +1163,7 @@ __device__ __forceinline__ void update_body(KParams P, RKCoef C, const double* _
     }
     // source(pb): T^j_k Gamma^k_{lj} contraction (phys.c:475-619)
     double dUg[NPR];
-    source_conn(P, pbz, g, gConn + z, dUg, (size_t)(P.n1 + 2 * NG) * P.N2M);
+    source_conn_sym(P, pbz, g, gConn + z, dUg, (size_t)(P.n1 + 2 * NG) * P.N2M);
     double d1[NPR], d2[NPR];
     if constexpr (ENO == 2) {
       eno_cvt_zone<ENO_A2C, 1>(P, eA, i, j, d1);
@@ -1542,8 +1562,7 @@ __device__ void fixup_zone(const KParams& P, const double* __restrict__ pv, cons
                            int z, int& fail, double* keep = nullptr) {
   const int i = z / P.N2M - NG, j = z % P.N2M - NG;
   double gc[36];
-#pragma unroll
-  for (int m = 0; m < 36; m++) gc[m] = gC[(size_t)z * GCN + m];
+  load_cent(gC, z, gc);
   GeoU g;
   geo_from_cent(gc, g);
   double pz[NPR];
@@ -2007,7 +2026,7 @@ __device__ __forceinline__ void outflow_value(const KParams& P, const double* __
 #pragma unroll
   for (int k = 0; k < NPR; k++) v[k] = src[k];
   if (!P.extrap) return;
-  const double gr = gC[(size_t)zidx(P, ri, j) * GCN + 32], gi = gC[(size_t)zidx(P, i, j) * GCN + 32];
+  const double gr = gC[(size_t)zidx(P, ri, j) * GCN + GC_GDET], gi = gC[(size_t)zidx(P, i, j) * GCN + GC_GDET];
   const bool up = ri > 0;
   const double dx1 = P.dx1;
   for (int k = RHO; k <= UU; k++) v[k] = src[k] * gr / gi;
@@ -2182,8 +2201,7 @@ __global__ __launch_bounds__(128) void k_utoprim_batch(KParams P, int n, const i
   if (t >= n) return;
   int z = zidx(P, zone[2 * t], zone[2 * t + 1]);
   double gc[36];
-#pragma unroll
-  for (int m = 0; m < 36; m++) gc[m] = gC[(size_t)z * GCN + m];
+  load_cent(gC, z, gc);
   double u[NPR], p[NPR];
 #pragma unroll
   for (int k = 0; k < NPR; k++) {
@@ -2232,8 +2250,7 @@ __global__ void k_bsqflags(KParams P, const double* __restrict__ pv, const doubl
   int f0 = 0, f3 = 0;
   if (P.eomtype == HARM_EOMGRMHD) {
     double gc[36];
-#pragma unroll
-    for (int m = 0; m < 36; m++) gc[m] = gC[(size_t)z * GCN + m];
+    load_cent(gC, z, gc);
     GeoU g;
     geo_from_cent(gc, g);
     double pr[NPR];
@@ -2262,8 +2279,7 @@ __global__ void k_gammacheck(KParams P, const double* __restrict__ pv, const dou
   if (bsqf[zm] == 0 || pflag[zm] != 0) return;
   const int z = zidx(P, i, j);
   double gc[36];
-#pragma unroll
-  for (int m = 0; m < 36; m++) gc[m] = gC[(size_t)z * GCN + m];
+  load_cent(gC, z, gc);
   GeoU g;
   geo_from_cent(gc, g);
   double pr[NPR];
@@ -2627,7 +2643,7 @@ extern "C" int harm_create(const harm_params* prm, harm_ctx** out) {
   }
   if (e == hipSuccess) e = hipMalloc(&c->gF, 2 * nz * GF * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&c->gC, nz * GCN * sizeof(double));
-  if (e == hipSuccess) e = hipMalloc(&c->gConn, nz * 64 * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc(&c->gConn, nz * NCONN * sizeof(double));
   if (e == hipSuccess) e = hipMalloc(&c->ds, sizeof(DevScalars));
   if (e == hipSuccess) e = hipStreamCreate(&c->own_stream);
   if (e == hipSuccess) e = hipGetDevice(&c->device);
@@ -2718,14 +2734,28 @@ extern "C" int harm_set_stream(harm_ctx* c, void* s) {
 
 // the reference's set_grid() tables packed into the device records: face
 // records hF [2][nz][GF], CENT records hC [nz][GCN], the connection
-// component-major hT [64][nz] (floors may be null: zero)
-static void pack_geometry(size_t nz, const double* gcov, const double* gcon, const double* gdet, const double* conn,
+// component-major hT [NCONN][nz] (floors may be null: zero).  The records keep
+// one triangle of gcov and of the connection's lower index pair, so those
+// must be symmetric to the bit (set_grid()'s are); returns false if not.
+static bool pack_geometry(size_t nz, const double* gcov, const double* gcon, const double* gdet, const double* conn,
                           const double* floors, std::vector<double>& hF, std::vector<double>& hC,
                           std::vector<double>& hT) {
+  static const int ua[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3}, ub[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
+  auto same = [](double x, double y) { return memcmp(&x, &y, sizeof(double)) == 0; };
   hF.assign(2 * nz * GF, 0.0);
   hC.assign(nz * GCN, 0.0);
-  hT.assign(nz * 64, 0.0);
+  hT.assign(nz * NCONN, 0.0);
   for (size_t z = 0; z < nz; z++) {
+    for (int pos = 0; pos < 4; pos++)
+      for (int a = 0; a < 4; a++)
+        for (int b = a + 1; b < 4; b++) {
+          const size_t m = (z * 4 + pos) * 16;
+          if (!same(gcov[m + a * 4 + b], gcov[m + b * 4 + a])) return false;
+        }
+    for (int k = 0; k < 4; k++)
+      for (int l = 0; l < 4; l++)
+        for (int j = l + 1; j < 4; j++)
+          if (!same(conn[z * 64 + (k * 4 + l) * 4 + j], conn[z * 64 + (k * 4 + j) * 4 + l])) return false;
     for (int pos = 0; pos < 2; pos++) {
       const double* gc = gcov + (z * 4 + pos) * 16;
       const double* gn = gcon + (z * 4 + pos) * 16;
@@ -2738,24 +2768,29 @@ static void pack_geometry(size_t nz, const double* gcov, const double* gcon, con
       o[15] = gdet[z * 4 + pos];
     }
     double* o = &hC[z * GCN];
-    memcpy(o, gcov + (z * 4 + 3) * 16, 16 * sizeof(double));
-    memcpy(o + 16, gcon + (z * 4 + 3) * 16, 16 * sizeof(double));
-    o[32] = gdet[z * 4 + 3];
-    o[33] = floors ? floors[z * 2 + 0] : 0.0;
-    o[34] = floors ? floors[z * 2 + 1] : 0.0;
-    o[35] = gdet[z * 4 + 2];
+    const double* gc = gcov + (z * 4 + 3) * 16;
+    const double* gn = gcon + (z * 4 + 3) * 16;
+    for (int u = 0; u < 10; u++) o[u] = gc[ua[u] * 4 + ub[u]];
+    for (int m = 0; m < 16; m++) o[10 + m] = gn[m];
+    o[GC_GDET] = gdet[z * 4 + 3];
+    o[GC_GDET + 1] = floors ? floors[z * 2 + 0] : 0.0;
+    o[GC_GDET + 2] = floors ? floors[z * 2 + 1] : 0.0;
+    o[GC_GDETC] = gdet[z * 4 + 2];
   }
-  // the connection component-major ([64][zones]): the update's lanes (adjacent
-  // zones) then read adjacent words of each component
+  // the connection component-major ([NCONN][zones]): the update's lanes
+  // (adjacent zones) then read adjacent words of each component
   for (size_t z = 0; z < nz; z++)
-    for (int e = 0; e < 64; e++) hT[(size_t)e * nz + z] = conn[z * 64 + e];
+    for (int k = 0; k < 4; k++)
+      for (int u = 0; u < 10; u++) hT[(size_t)(k * 10 + u) * nz + z] = conn[z * 64 + (k * 4 + ua[u]) * 4 + ub[u]];
+  return true;
 }
 
 extern "C" int harm_set_geometry(harm_ctx* c, const double* gcov, const double* gcon, const double* gdet,
                                  const double* conn, const double* floors) {
   if (!c || !gcov || !gcon || !gdet || !conn || !floors) return seterr("null argument");
   std::vector<double> hF, hC, hT;
-  pack_geometry((size_t)c->nz, gcov, gcon, gdet, conn, floors, hF, hC, hT);
+  if (!pack_geometry((size_t)c->nz, gcov, gcon, gdet, conn, floors, hF, hC, hT))
+    return seterr("geometry tables: gcov and the connection's lower index pair must be symmetric to the bit");
   HIPCHK(hipMemcpy(c->gF, hF.data(), hF.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->gC, hC.data(), hC.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->gConn, hT.data(), hT.size() * sizeof(double), hipMemcpyHostToDevice));

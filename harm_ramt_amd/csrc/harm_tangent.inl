@@ -52,8 +52,7 @@ __device__ __forceinline__ void load_geo_face_t(const double* __restrict__ gF, c
 
 __device__ __forceinline__ void load_cent_t(const double* __restrict__ gC, const double* __restrict__ dgC, int z,
                                             Dual* gc) {
-#pragma unroll
-  for (int m = 0; m < 36; m++) gc[m] = Dual(gC[(size_t)z * GCN + m], dgC[(size_t)z * GCN + m]);
+  cent_expand([&](int m) { return Dual(gC[(size_t)z * GCN + m], dgC[(size_t)z * GCN + m]); }, gc);
 }
 
 // ---- fluxcalc: one thread per face (flux_face_single / face_finish) ----
@@ -161,7 +160,7 @@ __device__ __forceinline__ Dual tl_emf(const KParams& P, const double* __restric
                                        const double* __restrict__ q2, const double* __restrict__ dq2,
                                        const double* __restrict__ gC, const double* __restrict__ dgC, int i, int j) {
   const int z = zidx(P, i, j), zj = zidx(P, i, j - 1), zi = zidx(P, i - 1, j);
-  const Dual gc(gC[(size_t)z * GCN + 35], dgC[(size_t)z * GCN + 35]);
+  const Dual gc(gC[(size_t)z * GCN + GC_GDETC], dgC[(size_t)z * GCN + GC_GDETC]);
   return 0.25 * gc * (Dual(q1[z], dq1[z]) + Dual(q1[zj], dq1[zj]) - Dual(q2[z], dq2[z]) - Dual(q2[zi], dq2[zi]));
 }
 
@@ -201,7 +200,11 @@ __global__ __launch_bounds__(256) void k_tl_update(KParams P, RKCoef C, TLArrays
     const size_t nzz = (size_t)(P.n1 + 2 * NG) * P.N2M;
     Dual dUg[NPR];
     source_conn_at(P, pbz, g,
-                   [&](int e) { return Dual(A.gConn[(size_t)e * nzz + z], A.dgConn[(size_t)e * nzz + z]); }, dUg);
+                   [&](int e) {
+                     const size_t o = (size_t)conn_sym(e) * nzz + z;
+                     return Dual(A.gConn[o], A.dgConn[o]);
+                   },
+                   dUg);
     __builtin_amdgcn_sched_barrier(0);
     // flux2dUavg with the flux-CT B fluxes from the corner EMFs
     Dual f1a[NPR], f1b[NPR], f2a[NPR], f2b[NPR];
@@ -412,7 +415,7 @@ extern "C" int harm_tl_create(harm_ctx* c, harm_tl** out) {
   struct { double** p; size_t n; } bufs[] = {
       {&t->dA[0], nz * 8}, {&t->dA[1], nz * 8}, {&t->dA[2], nz * 8}, {&t->dF1, nz * 8}, {&t->dF2, nz * 8},
       {&t->dq1, nz}, {&t->dq2, nz}, {&t->dul, nz * 8}, {&t->dun, nz * 8}, {&t->dgF, 2 * nz * GF},
-      {&t->dgC, nz * GCN}, {&t->dgConn, nz * 64}, {&t->red, 2 * (size_t)c->kp.n2 + 2}};
+      {&t->dgC, nz * GCN}, {&t->dgConn, nz * NCONN}, {&t->red, 2 * (size_t)c->kp.n2 + 2}};
   hipError_t e = hipSuccess;
   for (auto& b : bufs) {
     if (e == hipSuccess) e = hipMalloc(b.p, b.n * sizeof(double));
@@ -430,7 +433,8 @@ extern "C" int harm_tl_set_geometry_tangent(harm_tl* t, const double* dgcov, con
                                             const double* dconn) {
   if (!t || !dgcov || !dgcon || !dgdet || !dconn) return seterr("null argument");
   std::vector<double> hF, hC, hT;
-  pack_geometry((size_t)t->c->nz, dgcov, dgcon, dgdet, dconn, nullptr, hF, hC, hT);
+  if (!pack_geometry((size_t)t->c->nz, dgcov, dgcon, dgdet, dconn, nullptr, hF, hC, hT))
+    return seterr("geometry tangents: gcov and the connection's lower index pair must be symmetric to the bit");
   HIPCHK(hipMemcpy(t->dgF, hF.data(), hF.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(t->dgC, hC.data(), hC.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(t->dgConn, hT.data(), hT.size() * sizeof(double), hipMemcpyHostToDevice));
