@@ -666,8 +666,18 @@ def main():
         h.step_n(na)
         h.get_time()
         ms_acct = 1e3 * (time.perf_counter() - t2) / na
-        accounting = {"steps": na, "ms_per_step": ms_acct, "vs_plain_step": ms_acct / (1e3 * el / args.steps),
-                      "regions": 3}
+        # the run continued with the tree sums (harm_acct_set_order)
+        h.acct_setup(ep, df)
+        h.acct_set_order(H.ACCT_TREE)
+        h.step_n(2)
+        h.get_time()
+        t2 = time.perf_counter()
+        h.step_n(na)
+        h.get_time()
+        ms_tree = 1e3 * (time.perf_counter() - t2) / na
+        plain = 1e3 * el / args.steps
+        accounting = {"steps": na, "ms_per_step": ms_acct, "vs_plain_step": ms_acct / plain, "regions": 3,
+                      "tree_sums": {"ms_per_step": ms_tree, "vs_plain_step": ms_tree / plain}}
         h.acct_off()
 
     h.close()

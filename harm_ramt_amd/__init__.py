@@ -32,6 +32,7 @@ P, PK0, PK1, F1, F2, ULAST, UNEW = 0, 1, 2, 3, 4, 5, 6
 METRIC_MINK, METRIC_KS = 0, 2
 FLUX_FACE_64x4, FLUX_FACE_64x2, FLUX_PAIR, FLUX_SHARED = 0, 1, 2, 3
 BOUND_KERNELS, BOUND_BLOCK, BOUND_KERNELS_SEQCHECK, BOUND_FIXONE, BOUND_FUSED, BOUND_INUPDATE = 0, 1, 2, 3, 4, 5
+ACCT_ORDERED, ACCT_TREE = 0, 1
 K_FLUX, K_EMF, K_UPDATE, K_BOUNDARY = 0, 2, 3, 4
 STAGE_NAMES = {K_FLUX: "flux", K_EMF: "flux_ct", K_UPDATE: "update", K_BOUNDARY: "boundary"}
 
@@ -193,7 +194,8 @@ EXPORTS = [
     "harm_set_analytic",
     "harm_init_orszag_tang", "harm_init_monopole",
     "harm_rdump_write_host", "harm_rdump_read_host", "harm_rdump_write", "harm_rdump_read",
-    "harm_ener_regions", "harm_acct_setup", "harm_acct_off", "harm_acct_read", "harm_diag_flux",
+    "harm_ener_regions", "harm_acct_setup", "harm_acct_off", "harm_acct_set_order", "harm_acct_read",
+    "harm_diag_flux",
     "harm_divbmaxavg", "harm_constotal",
     "harm_current_enable", "harm_get_current", "harm_dump_write_host", "harm_dump_write", "harm_zone_coords",
 ]
@@ -285,6 +287,7 @@ def load_library(path=LIB_PATH):
         "harm_ener_regions": [ctypes.POINTER(HarmGrid), ctypes.c_double, ip, ip],
         "harm_acct_setup": [vp, ip, ip, ctypes.c_double],
         "harm_acct_off": [vp],
+        "harm_acct_set_order": [vp, ctypes.c_int],
         "harm_acct_read": [vp, ctypes.POINTER(EnerRegion), dp, dp, dp],
         "harm_diag_flux": [vp, ctypes.c_double],
         "harm_divbmaxavg": [vp, ctypes.c_int, dp, dp, ip, ip],
@@ -554,6 +557,11 @@ class Harm:
 
     def acct_off(self):
         _chk(self.lib.harm_acct_off(self.h), "harm_acct_off")
+
+    def acct_set_order(self, order):
+        """ACCT_ORDERED (the reference's serial sums, bitwise) or ACCT_TREE
+        (source and fixup totals by reduction trees, for throughput)"""
+        _chk(self.lib.harm_acct_set_order(self.h, order), "harm_acct_set_order")
 
     def acct(self):
         """the accumulators as a dict of arrays with a leading region axis,

@@ -58,6 +58,8 @@ struct AcctState {
   int* off;
   int* total;
   int nslots, nblocks;
+  bool tree;        // HARM_ACCT_TREE: source and fixup totals by reduction trees
+  double* part;     // their per-block partial sums
 };
 
 __device__ __forceinline__ bool acct_in(const int* ep, int i, int j) {
@@ -745,6 +747,62 @@ __global__ __launch_bounds__(256) void k_acct_compact(int nslots, int* __restric
   }
 }
 
+// an event's key: bit r set when region r holds it, category + 1 above the
+// region bits; 0 when it does not count on this substage (floor and repair
+// events count on the last substage only)
+__device__ __forceinline__ int acct_slot_key(int n1, int n2, const AcctRegions& R, int slot, int cc, int finalstep) {
+  const int bB = acct_base(n1, n2, 1), bC = acct_base(n1, n2, 2), bD = acct_base(n1, n2, 3);
+  int i, j;
+  if (slot < bB) {
+    const int f = slot >> 1;
+    i = f / (n2 + 2);
+    j = f % (n2 + 2) - 1;
+  } else if (slot < bC) {
+    const int f = (slot - bB) >> 1;
+    i = f / (n2 + 1) - 1;
+    j = f % (n2 + 1);
+  } else {
+    // a zone's third slot (the floors) and a failed zone's second (the
+    // repair) count on the accounting substage only
+    const bool zone = slot < bD;
+    const int rel = slot - (zone ? bC : bD);
+    const int f = zone ? rel / 3 : rel >> 1, sub = zone ? rel - 3 * f : rel & 1;
+    if (sub == (zone ? 2 : 1) && !finalstep) return 0;
+    i = f / n2;
+    j = f % n2;
+  }
+  int key = 0;
+#pragma unroll
+  for (int q = 0; q < ACC_NREG; q++)
+    if (i >= R.ep[q][HARM_X1DN] && i <= R.ep[q][HARM_X1UP] && j >= R.ep[q][HARM_X2DN] && j <= R.ep[q][HARM_X2UP])
+      key |= 1 << q;
+  return key ? key | ((cc + 1) << ACC_NREG) : 0;
+}
+
+// accumulator lane t of the fixup sums: t < 24 fladd[r][k], then
+// fladdterms[r][cat][k] (cat = -1 for fladd); the event keys it takes are
+// those with (key & kmask) == kwant
+struct FixLane {
+  int r, cat, k, kmask, kwant;
+  __device__ explicit FixLane(int t) {
+    constexpr int NT = ACC_NREG * 8;
+    if (t < NT) {
+      r = t / 8;
+      cat = -1;
+      k = t % 8;
+    } else {
+      const int u = t - NT;
+      r = u / (ACC_NFLOOR * 8);
+      cat = (u / 8) % ACC_NFLOOR;
+      k = u % 8;
+    }
+    kmask = (1 << r) | ((cat < 0) ? 0 : (int)(~0u << ACC_NREG));
+    kwant = (1 << r) | ((cat < 0) ? 0 : ((cat + 1) << ACC_NREG));
+  }
+  __device__ int dst() const { return r * REG_D + ((cat < 0) ? OFF_FLADD + k : OFF_FLT + cat * 8 + k); }
+};
+constexpr int FIX_LANES = ACC_NREG * 8 * (1 + ACC_NFLOOR);
+
 // the substage's events added in order: lanes 0-23 fladd[r][k], then
 // fladdterms[r][cat][k]; floor and repair events count on the last substage
 __global__ __launch_bounds__(512) void k_acct_merge(KParams P, AcctRegions R, const int* __restrict__ list,
@@ -755,61 +813,19 @@ __global__ __launch_bounds__(512) void k_acct_merge(KParams P, AcctRegions R, co
   // region bits; 0 when it does not count on this substage
   __shared__ int ev_key[2][256];
   const int t = threadIdx.x;
-  constexpr int NT = ACC_NREG * 8, NTT = ACC_NREG * ACC_NFLOOR * 8;
-  const bool lane = t < NT + NTT;  // waves 0-3 add, waves 4-7 stage the next 256 events
-  int r = 0, cat = -1, k = 0;
-  if (t < NT) {
-    r = t / 8;
-    k = t % 8;
-  } else if (lane) {
-    const int u = t - NT;
-    r = u / (ACC_NFLOOR * 8);
-    cat = (u / 8) % ACC_NFLOOR;
-    k = u % 8;
-  }
+  const bool lane = t < FIX_LANES;  // waves 0-3 add, waves 4-7 stage the next 256 events
+  const FixLane L(lane ? t : 0);
   const int n1 = P.n1, n2 = P.n2;
-  const int bB = acct_base(n1, n2, 1), bC = acct_base(n1, n2, 2), bD = acct_base(n1, n2, 3);
-  double* dst = acc + r * REG_D + ((cat < 0) ? OFF_FLADD + k : OFF_FLT + cat * 8 + k);
+  double* dst = acc + L.dst();
   double s = lane ? *dst : 0.0;
   const int E = *total;
   const int nch = (E + 255) / 256;
-  // this lane takes events whose key has its region bit and, for a category
-  // lane, that category: (key & kmask) == kwant
-  const int kmask = (1 << r) | ((cat < 0) ? 0 : (~0 << ACC_NREG));
-  const int kwant = (1 << r) | ((cat < 0) ? 0 : ((cat + 1) << ACC_NREG));
-  // where an event lies and whether it counts on this substage (-1: it does not)
+  const int kmask = L.kmask, kwant = L.kwant;
+  // where an event lies and whether it counts on this substage
   auto stage = [&](int c, int h) {
     const int u = t - 256, e = c * 256 + u;
     if (u < 0 || e >= E) return;
-    const int slot = list[2 * e];
-    int cc = list[2 * e + 1], i, j;
-    if (slot < bB) {
-      const int f = slot >> 1;
-      i = f / (n2 + 2);
-      j = f % (n2 + 2) - 1;
-    } else if (slot < bC) {
-      const int f = (slot - bB) >> 1;
-      i = f / (n2 + 1) - 1;
-      j = f % (n2 + 1);
-    } else {
-      // a zone's third slot (the floors) and a failed zone's second (the
-      // repair) count on the accounting substage only
-      const bool zone = slot < bD;
-      const int rel = slot - (zone ? bC : bD);
-      const int f = zone ? rel / 3 : rel >> 1, sub = zone ? rel - 3 * f : rel & 1;
-      if (sub == (zone ? 2 : 1) && !finalstep) cc = -1;
-      i = f / n2;
-      j = f % n2;
-    }
-    int key = 0;
-    if (cc >= 0) {
-#pragma unroll
-      for (int q = 0; q < ACC_NREG; q++)
-        if (i >= R.ep[q][HARM_X1DN] && i <= R.ep[q][HARM_X1UP] && j >= R.ep[q][HARM_X2DN] && j <= R.ep[q][HARM_X2UP])
-          key |= 1 << q;
-      if (key) key |= (cc + 1) << ACC_NREG;
-    }
-    ev_key[h][u] = key;
+    ev_key[h][u] = acct_slot_key(n1, n2, R, list[2 * e], list[2 * e + 1], finalstep);
 #pragma unroll
     for (int m = 0; m < 8; m++) ev_val[h][u * 8 + m] = cval[(size_t)e * 8 + m];
   };
@@ -828,7 +844,7 @@ __global__ __launch_bounds__(512) void k_acct_merge(KParams P, AcctRegions R, co
         for (int u = 0; u < 8; u++) {
           const int m = min(m0 + u, 255);
           kk[u] = (m0 + u < cnt) ? ev_key[h][m] : 0;
-          v[u] = ev_val[h][m * 8 + k];
+          v[u] = ev_val[h][m * 8 + L.k];
         }
 #pragma unroll
         for (int u = 0; u < 8; u++) {
@@ -840,6 +856,104 @@ __global__ __launch_bounds__(512) void k_acct_merge(KParams P, AcctRegions R, co
     __syncthreads();
   }
   if (lane) *dst = s;
+}
+
+// ---- tree sums (HARM_ACCT_TREE) ----
+// The same accumulators, each substage's total formed by a fixed tree: every
+// block sums its items (one thread per item, then wavefront butterflies and
+// the four wavefronts in order) into a row of partial sums, then one
+// workgroup per accumulator sums the rows (strided per thread, then the same
+// tree) and adds the total.  Reproducible run to run; not the reference's
+// serial order.
+constexpr int TREE_SRC_NB = 256;  // blocks of the source sum
+constexpr int TREE_SRC_N = ACC_NREG * 8;
+
+__device__ __forceinline__ double block_sum(double v, double* lds4) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) lds4[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return ((lds4[0] + lds4[1]) + (lds4[2] + lds4[3]));
+}
+
+// source terms of the interior zones, lane (region, k) of each block
+__global__ __launch_bounds__(256) void k_acct_srcsum_part(KParams P, AcctRegions R, const double* __restrict__ term,
+                                                          double* __restrict__ part) {
+  __shared__ double lds4[4];
+  double s[TREE_SRC_N];
+#pragma unroll
+  for (int a = 0; a < TREE_SRC_N; a++) s[a] = 0.0;
+  const int nzz = P.n1 * P.n2;
+  for (int z = blockIdx.x * 256 + threadIdx.x; z < nzz; z += TREE_SRC_NB * 256) {
+    const int i = z / P.n2, j = z - i * P.n2;
+    double v[NPR];
+    load8(term, z, v);
+#pragma unroll
+    for (int r = 0; r < ACC_NREG; r++) {
+      const bool in = i >= R.ep[r][HARM_X1DN] && i <= R.ep[r][HARM_X1UP] && j >= R.ep[r][HARM_X2DN] &&
+                      j <= R.ep[r][HARM_X2UP];
+#pragma unroll
+      for (int k = 0; k < NPR; k++)
+        if (in) s[r * 8 + k] += v[k];
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < TREE_SRC_N; a++) {
+    const double b = block_sum(s[a], lds4);
+    if (threadIdx.x == 0) part[(size_t)blockIdx.x * TREE_SRC_N + a] = b;
+  }
+}
+
+// the substage's fixup events, 256 slots per block: lane FixLane(t) of each
+// block over the block's events in slot order (used slots cleared)
+__global__ __launch_bounds__(256) void k_acct_merge_part(KParams P, AcctRegions R, int nslots, int* __restrict__ flag,
+                                                         const double* __restrict__ val, double* __restrict__ part,
+                                                         int finalstep) {
+  __shared__ int key[256];
+  __shared__ double ev[256 * 8];
+  const int t = threadIdx.x, slot = blockIdx.x * 256 + t;
+  const int f = (slot < nslots) ? flag[slot] : 0;
+  int kk = 0;
+  if (f != 0) {
+    kk = acct_slot_key(P.n1, P.n2, R, slot, f - 1, finalstep);
+#pragma unroll
+    for (int m = 0; m < 8; m++) ev[t * 8 + m] = val[(size_t)slot * 8 + m];
+    flag[slot] = 0;
+  }
+  key[t] = kk;
+  const int any = __syncthreads_count(kk != 0);
+  if (t < FIX_LANES) {
+    const FixLane L(t);
+    double s = 0.0;
+    if (any)
+      for (int m = 0; m < 256; m++)
+        if ((key[m] & L.kmask) == L.kwant) s += ev[m * 8 + L.k];
+    part[(size_t)blockIdx.x * FIX_LANES + t] = s;
+  }
+}
+
+// one workgroup per accumulator a: the sum of its nb partial rows added to
+// the accumulator (mode 0: sourceadd[r][k] and sourceaddterms[r][0][k];
+// mode 1: fladd / fladdterms)
+__global__ __launch_bounds__(256) void k_acct_tree_final(const double* __restrict__ part, int nb, int nacc, int mode,
+                                                         double* __restrict__ acc) {
+  __shared__ double lds4[4];
+  const int a = blockIdx.x;
+  double v = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 256) v += part[(size_t)b * nacc + a];
+  const double tot = block_sum(v, lds4);
+  if (threadIdx.x == 0) {
+    if (mode == 0) {
+      const int r = a / 8, k = a % 8;
+      const double n = acc[r * REG_D + OFF_SRC + k] + tot;
+      acc[r * REG_D + OFF_SRC + k] = n;
+      acc[r * REG_D + OFF_SRCT + k] = n;
+    } else {
+      const int d = FixLane(a).dst();
+      acc[d] = acc[d] + tot;
+    }
+  }
 }
 
 }  // namespace
@@ -881,6 +995,8 @@ static int acct_alloc(harm_ctx* c) {
   if (e == hipSuccess) e = hipMalloc(&a->cnt, (size_t)a->nblocks * sizeof(int));
   if (e == hipSuccess) e = hipMalloc(&a->off, (size_t)a->nblocks * sizeof(int));
   if (e == hipSuccess) e = hipMalloc(&a->total, sizeof(int));
+  if (e == hipSuccess)
+    e = hipMalloc(&a->part, (size_t)std::max(TREE_SRC_NB * TREE_SRC_N, a->nblocks * FIX_LANES) * sizeof(double));
   c->acct = a;
   if (e != hipSuccess) return seterr("accounting buffers: %s", hipGetErrorString(e));
   return 0;
@@ -889,8 +1005,8 @@ static int acct_alloc(harm_ctx* c) {
 static void acct_free(harm_ctx* c) {
   AcctState* a = c->acct;
   if (!a) return;
-  void* bufs[15] = {a->acc,  a->frdot, a->lumvsr, a->scratch, a->out, a->faces, a->pre,  a->uin,
-                     a->flag, a->val,   a->list,   a->cval,    a->cnt, a->off,   a->total};
+  void* bufs[16] = {a->acc,  a->frdot, a->lumvsr, a->scratch, a->out, a->faces, a->pre,  a->uin,
+                     a->flag, a->val,   a->list,   a->cval,    a->cnt, a->off,   a->total, a->part};
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   delete a;
@@ -905,7 +1021,14 @@ static int acct_source(harm_ctx* c, int pi, int pb, int pf) {
   const dim3 b(64, 4);
   hipLaunchKernelGGL(k_acct_srcterm, grid2(c->kp.n2, c->kp.n1, b), b, 0, c->stream, c->kp, a->R, c->arr[pbs], c->gC,
                      c->gConn, c->ds, a->scratch);
-  hipLaunchKernelGGL(k_acct_srcsum, dim3(1), dim3(256), 0, c->stream, c->kp, a->R, a->scratch, a->acc);
+  if (a->tree) {
+    hipLaunchKernelGGL(k_acct_srcsum_part, dim3(TREE_SRC_NB), dim3(256), 0, c->stream, c->kp, a->R, a->scratch,
+                       a->part);
+    hipLaunchKernelGGL(k_acct_tree_final, dim3(TREE_SRC_N), dim3(256), 0, c->stream, a->part, TREE_SRC_NB,
+                       TREE_SRC_N, 0, a->acc);
+  } else {
+    hipLaunchKernelGGL(k_acct_srcsum, dim3(1), dim3(256), 0, c->stream, c->kp, a->R, a->scratch, a->acc);
+  }
   hipLaunchKernelGGL(k_acct_lumsum, dim3((c->kp.n1 + 63) / 64), dim3(64), 0, c->stream, c->kp, a->R, a->scratch,
                      a->lumvsr);
   HIPCHK(hipGetLastError());
@@ -936,12 +1059,19 @@ static int acct_merge(harm_ctx* c, int finalstep) {
     const dim3 b(64, 4);
     hipLaunchKernelGGL(k_acct_fixg, grid2(c->kp.n2, c->kp.n1, b), b, 0, c->stream, c->kp, c->gC);
   }
+  if (a->tree) {
+    hipLaunchKernelGGL(k_acct_merge_part, dim3(a->nblocks), dim3(256), 0, c->stream, c->kp, a->R, a->nslots, a->flag,
+                       a->val, a->part, finalstep);
+    hipLaunchKernelGGL(k_acct_tree_final, dim3(FIX_LANES), dim3(256), 0, c->stream, a->part, a->nblocks, FIX_LANES,
+                       1, a->acc);
+    HIPCHK(hipGetLastError());
+    return 0;
+  }
   hipLaunchKernelGGL(k_acct_count, dim3(a->nblocks), dim3(256), 0, c->stream, a->nslots, a->flag, a->cnt);
   hipLaunchKernelGGL(k_acct_scan, dim3(1), dim3(256), 0, c->stream, a->nblocks, a->cnt, a->off, a->total);
   hipLaunchKernelGGL(k_acct_compact, dim3(a->nblocks), dim3(256), 0, c->stream, a->nslots, a->flag, a->off, a->list,
                      a->val, a->cval);
-  constexpr int nl = ACC_NREG * 8 * (1 + ACC_NFLOOR);
-  static_assert(nl <= 256, "one workgroup holds every accumulator lane");
+  static_assert(FIX_LANES <= 256, "one workgroup holds every accumulator lane");
   hipLaunchKernelGGL(k_acct_merge, dim3(1), dim3(512), 0, c->stream, c->kp, a->R, a->list, a->total, a->cval, a->acc,
                      finalstep);
   HIPCHK(hipGetLastError());
@@ -1020,6 +1150,15 @@ extern "C" int harm_acct_setup(harm_ctx* c, const int* enerpos, const int* doflu
   HIPCHK(hipStreamSynchronize(c->stream));
   c->kp.dVF = R.dVF;
   acct_set_on(c, true);
+  invalidate_graph(c);
+  return 0;
+}
+
+extern "C" int harm_acct_set_order(harm_ctx* c, int order) {
+  if (!c) return seterr("null ctx");
+  if (!c->acct) return seterr("accounting was never set up (harm_acct_setup)");
+  if (order != HARM_ACCT_ORDERED && order != HARM_ACCT_TREE) return seterr("bad accounting order %d", order);
+  c->acct->tree = (order == HARM_ACCT_TREE);
   invalidate_graph(c);
   return 0;
 }

@@ -493,6 +493,18 @@ int harm_ener_regions(const harm_grid *g, double h_over_r_jet, int enerpos[HARM_
    boxes are clipped to the interior as ZLOOP clips them. */
 int harm_acct_setup(harm_ctx *ctx, const int *enerpos, const int *doflux, double dx3);
 int harm_acct_off(harm_ctx *ctx);
+/* No counterpart in the reference: the order of the accounting's sums.
+   HARM_ACCT_ORDERED (what harm_acct_setup sets) adds every term in the
+   reference's serial loop order, one dependent chain per accumulator: the
+   reference's totals to the bit.  HARM_ACCT_TREE, for throughput runs, forms
+   each substage's volume-source and fixup totals by a fixed reduction tree
+   (per-block partial sums, then a tree per accumulator) and adds them to the
+   accumulators: the same from run to run, within rounding of the ordered
+   totals (tested to 1e-12 of each accumulator array's largest value).  The
+   surface integrals (diag_flux), frdot and lumvsr keep the serial order. */
+#define HARM_ACCT_ORDERED 0
+#define HARM_ACCT_TREE 1
+int harm_acct_set_order(harm_ctx *ctx, int order);
 /* the accumulators: regions[HARM_NUMENERREGIONS], pdottermsjet2[4][7][8],
    frdot[N1][8] (radial flux per radius), lumvsr[N1]; any may be NULL */
 int harm_acct_read(harm_ctx *ctx, harm_ener_region *regions, double *pdottermsjet2, double *frdot,

@@ -235,6 +235,44 @@ def test_gpu_force_free_accounting_matches_reference(graph):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("graph", [False, True])
+def test_gpu_tree_sums_match_ordered_sums(graph):
+    """HARM_ACCT_TREE on a 96x128 torus over 6 steps: the state and the
+    serial accumulators (surface integrals, frdot, lumvsr) are the ordered
+    mode's to the bit; the source and fixup totals agree to 1e-12 of each
+    array's largest value; a second tree run gives the same bits"""
+    n1, n2, nsteps = 96, 128, 6
+    g, geom, p0, prm = problem("ks", n1, n2)
+    ep, df = H.ener_regions(g, 0.4)
+    out = []
+    for order in (H.ACCT_ORDERED, H.ACCT_TREE, H.ACCT_TREE):
+        print(f"[case] torus {n1}x{n2} accounting order {order} graph={graph}", flush=True)
+        c = H.Harm(prm, geom)
+        c.set_array(H.P, p0)
+        c.bound_prim(H.P)
+        c.set_time(0.0, 1e-3, 1e30)
+        c.acct_setup(ep, df, 2 * np.pi)
+        c.acct_set_order(order)
+        if graph:
+            c.step_n(nsteps)
+        else:
+            for _ in range(nsteps):
+                c.step_ch()
+        out.append((c.get_array(H.P), c.acct()))
+        c.close()
+    (p_o, a_o), (p_t, a_t), (p_t2, a_t2) = out
+    assert same(p_o, p_t)
+    for k in STEP_KEYS:
+        if k not in ("sourceadd", "sourceaddterms"):
+            assert same(a_t[k], a_o[k]), k
+    for k in ("sourceadd", "sourceaddterms", "fladd", "fladdterms"):
+        scale = np.abs(a_o[k]).max()
+        assert scale > 0, k
+        assert np.abs(a_t[k] - a_o[k]).max() <= 1e-12 * scale, k
+        assert same(a_t2[k], a_t[k]), k
+
+
+@pytest.mark.gpu
 def test_gpu_accounting_matches_oracle_larger_grid():
     """a 96x128 torus over 6 steps, every flux and boundary variant pair in turn
     gives the oracle's accumulators"""
