@@ -470,15 +470,28 @@ def adjoint_measure(n=256, nsteps=256, K=None, seed=3):
     a.close()
     h.close()
 
-    def J_of(p, pr, dts):
+    def J_of(p, pr, dts, keep=False):
         hh = ctx(p, pr)
         hh.set_time(0.0, dts[0], 1e30)
         aa = H.Adjoint(hh, nsteps, nsteps)
         aa.forward(dts)
         val = aa.seed_magnetic_energy()
+        if keep:  # every state kept (checkpoint_every = nsteps) for branch_switches
+            return val, (hh, aa)
         aa.close()
         hh.close()
         return val
+
+    def J_pair(name, e, pr, dts):
+        """J(+e), J(-e) and the limiter branches that differ between the two
+        forward runs (every substage's reconstruction input)"""
+        Jp, rp = J_of(perturbed(name, e), pr, dts, keep=True)
+        Jm, rm = J_of(perturbed(name, -e), pr, dts, keep=True)
+        sw, cmp_ = rp[1].branch_switches(rm[1])
+        for hh, aa in (rp, rm):
+            aa.close()
+            hh.close()
+        return Jp, Jm, sw, cmp_
 
     hh = 1e-6
     w = np.zeros_like(p0)
@@ -496,23 +509,27 @@ def adjoint_measure(n=256, nsteps=256, K=None, seed=3):
         for name in ("B0", "v_amplitude", "rho_perturbation_direction"):
             dpdq = w if name == "rho_perturbation_direction" else (perturbed(name, hh) - perturbed(name, -hh)) / (2 * hh)
             ad = float((lam * dpdq).sum())
-            Jp, Jm = J_of(perturbed(name, hh), pr, dts), J_of(perturbed(name, -hh), pr, dts)
+            Jp, Jm, sw, ncmp = J_pair(name, hh, pr, dts)
             fd = (Jp - Jm) / (2 * hh)
             scale = float(np.abs(lam * dpdq).sum())
             out[name] = {"adjoint": ad, "finite_difference": fd, "abs_diff_over_term_scale": abs(ad - fd) / scale,
-                         "rel_diff": abs(ad - fd) / abs(fd)}
+                         "rel_diff": abs(ad - fd) / abs(fd), "limiter_branch_switches": sw,
+                         "slopes_compared": ncmp}
             if sweep:
-                # a kink of J (a limiter branch switch) within h of the base
-                # state adds an error that shrinks with h, unlike a wrong
-                # derivative; one-sided differences straddle it
+                # a kink of J (a limiter branch switch between the +h and -h
+                # runs) adds an error that shrinks with h, unlike a wrong
+                # derivative: the differences and the switches per step size
                 J0 = J_of(p0 if name == "rho_perturbation_direction" else perturbed(name, 0.0), pr, dts)
                 out[name]["one_sided_plus"] = (Jp - J0) / hh
                 out[name]["one_sided_minus"] = (J0 - Jm) / hh
-                rels = {}
-                for e in (1e-7, 1e-8):
-                    f = (J_of(perturbed(name, e), pr, dts) - J_of(perturbed(name, -e), pr, dts)) / (2 * e)
+                rels, sws = {}, {}
+                for e in (1e-7, 1e-8, 1e-9):
+                    jp, jm, swe, _ = J_pair(name, e, pr, dts)
+                    f = (jp - jm) / (2 * e)
                     rels[f"{e:g}"] = abs(ad - f) / abs(f)
+                    sws[f"{e:g}"] = swe
                 out[name]["rel_diff_vs_fd_step"] = rels
+                out[name]["limiter_branch_switches_vs_fd_step"] = sws
         return out
 
     checks = checks_of(lam, prm, dts, sweep=True)

@@ -189,7 +189,7 @@ EXPORTS = [
     "harm_batch_get_tuning",
     "harm_adj_create", "harm_adj_destroy", "harm_adj_forward", "harm_adj_get_dts",
     "harm_adj_seed_magnetic_energy", "harm_adj_seed", "harm_adj_backward", "harm_adj_get_lambda",
-    "harm_adj_get_nonsmooth",
+    "harm_adj_get_nonsmooth", "harm_adj_branch_switches",
     "harm_grid_mks", "harm_grid_cartesian", "harm_grid_cylmink", "harm_geometry", "harm_init_fishmon",
     "harm_set_analytic",
     "harm_init_orszag_tang", "harm_init_monopole",
@@ -266,6 +266,7 @@ def load_library(path=LIB_PATH):
         "harm_adj_backward": [vp],
         "harm_adj_get_lambda": [vp, dp],
         "harm_adj_get_nonsmooth": [vp, ip, ip],
+        "harm_adj_branch_switches": [vp, vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)],
         "harm_grid_mks": [ctypes.POINTER(HarmGrid), ctypes.c_int, ctypes.c_int, ctypes.c_double,
                           ctypes.c_double, ctypes.c_double, ctypes.c_double],
         "harm_grid_cartesian": [ctypes.POINTER(HarmGrid), ctypes.c_int, ctypes.c_int],
@@ -809,3 +810,11 @@ class Adjoint:
         a, b = ctypes.c_int(), ctypes.c_int()
         _chk(self.lib.harm_adj_get_nonsmooth(self.h, ctypes.byref(a), ctypes.byref(b)), "harm_adj_get_nonsmooth")
         return a.value, b.value
+
+    def branch_switches(self, other: "Adjoint"):
+        """(limiter branches that differ between this forward run and
+        other's, slopes compared); both with checkpoint_every = nsteps"""
+        n, c = ctypes.c_longlong(), ctypes.c_longlong()
+        _chk(self.lib.harm_adj_branch_switches(self.h, other.h, ctypes.byref(n), ctypes.byref(c)),
+             "harm_adj_branch_switches")
+        return n.value, c.value

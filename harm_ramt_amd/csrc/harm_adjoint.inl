@@ -735,6 +735,91 @@ extern "C" int harm_adj_get_lambda(harm_adj* a, double* host) {
   return 0;
 }
 
+// the branch a slope limiter takes on (yl, yc, yr) (interp.c:106-149): for
+// MC, which argument each of its two minmods returned (0: the zero branch);
+// MINM the same for its one minmod; VANL whether the slopes agree in sign;
+// 0 for the limiters without branches
+__device__ __forceinline__ int slope_branch(int lim, double yl, double yc, double yr) {
+  auto mm = [](double a, double b, double& m) {
+    if (a * b <= 0) {
+      m = 0.0;
+      return 0;
+    }
+    m = (fabs(a) < fabs(b)) ? a : b;
+    return (fabs(a) < fabs(b)) ? 1 : 2;
+  };
+  double m, m2;
+  if (lim == HARM_MC) {
+    const int in = mm(2.0 * (yc - yl), 2.0 * (yr - yc), m);
+    return in | (mm(0.5 * (yr - yl), m, m2) << 2);
+  }
+  if (lim == HARM_MINM) return mm(yc - yl, yr - yc, m);
+  if (lim == HARM_VANL) return ((yc - yl) * (yr - yc) <= 0.) ? 0 : 1;
+  return 0;
+}
+
+// limiter branches that differ between two runs' reconstruction inputs
+// (state z of each store): every zone whose slope a face uses (i in -1..N1
+// along x1, j in -1..N2 along x2), every component
+__global__ __launch_bounds__(256) void k_adj_branch_diff(KParams P, const double* __restrict__ sa,
+                                                         const double* __restrict__ sb, size_t nz8,
+                                                         unsigned long long* __restrict__ out) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x - 1;
+  const int i = blockIdx.y * blockDim.y + threadIdx.y - 1;
+  const double* a = sa + blockIdx.z * nz8;
+  const double* b = sb + blockIdx.z * nz8;
+  unsigned long long n = 0, c = 0;
+  if (i <= P.n1 && j <= P.n2) {
+#pragma unroll
+    for (int dir = 1; dir <= 2; dir++) {
+      if ((dir == 1 && (j < 0 || j >= P.n2)) || (dir == 2 && (i < 0 || i >= P.n1))) continue;
+      const int zl = (dir == 1) ? zidx(P, i - 1, j) : zidx(P, i, j - 1), z = zidx(P, i, j);
+      const int zr = (dir == 1) ? zidx(P, i + 1, j) : zidx(P, i, j + 1);
+      for (int k = 0; k < NPR; k++) {
+        const int ba = slope_branch(P.lim, a[(size_t)zl * 8 + k], a[(size_t)z * 8 + k], a[(size_t)zr * 8 + k]);
+        const int bb = slope_branch(P.lim, b[(size_t)zl * 8 + k], b[(size_t)z * 8 + k], b[(size_t)zr * 8 + k]);
+        n += (ba != bb);
+        c++;
+      }
+    }
+  }
+  n = wave_sum_u64(n);
+  c = wave_sum_u64(c);
+  if ((threadIdx.x & 63) == 0 && c) {
+    atomicAdd(&out[0], n);
+    atomicAdd(&out[1], c);
+  }
+}
+
+extern "C" int harm_adj_branch_switches(harm_adj* a, harm_adj* b, long long* switches, long long* compared) {
+  if (!a || !b) return seterr("null adjoint");
+  const harm_ctx *ca = a->c, *cb = b->c;
+  if (!a->forward_done || !b->forward_done || !a->kept_last || !b->kept_last || a->nseg != 1 || b->nseg != 1)
+    return seterr("branch switches: both runs need a forward run that kept every state (checkpoint_every = nsteps)");
+  if (ca->prm.n1 != cb->prm.n1 || ca->prm.n2 != cb->prm.n2 || ca->prm.lim != cb->prm.lim || a->nsteps != b->nsteps ||
+      a->nto != b->nto)
+    return seterr("branch switches: the runs differ in grid, limiter or steps");
+  unsigned long long* d = nullptr;
+  HIPCHK(hipMalloc(&d, 2 * sizeof(unsigned long long)));
+  HIPCHK(hipMemsetAsync(d, 0, 2 * sizeof(unsigned long long), ca->stream));
+  const size_t nz8 = (size_t)ca->nz * 8;
+  const int ns = a->nsteps * a->nto;  // the substages' reconstruction inputs: states 0 .. ns-1
+  const dim3 blk(64, 4);
+  for (int s0 = 0; s0 < ns; s0 += 1024) {
+    const dim3 g((ca->prm.n2 + 2 + 63) / 64, (ca->prm.n1 + 2 + 3) / 4, std::min(1024, ns - s0));
+    hipLaunchKernelGGL(k_adj_branch_diff, g, blk, 0, ca->stream, ca->kp, a->seg + s0 * nz8, b->seg + s0 * nz8, nz8, d);
+  }
+  unsigned long long h[2] = {0, 0};
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, ca->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ca->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return seterr("branch switches: %s", hipGetErrorString(e));
+  if (switches) *switches = (long long)h[0];
+  if (compared) *compared = (long long)h[1];
+  return 0;
+}
+
 extern "C" int harm_adj_get_nonsmooth(harm_adj* a, int* count, int* recompute_failures) {
   if (!a) return seterr("null adjoint");
   int ns = 0;

@@ -326,6 +326,15 @@ int harm_adj_get_lambda(harm_adj *adj, double *lambda);
    clamped wave speeds, singular inversion Jacobian) and failed inversions
    repaired in the recomputation: both 0 means the gradient is exact */
 int harm_adj_get_nonsmooth(harm_adj *adj, int *count, int *recompute_failures);
+/* Limiter branches (MC: which argument each of its minmods returned; MINM,
+   VANL likewise) that differ between two forward runs of the same grid,
+   limiter and steps, over every substage's reconstruction input, every
+   zone whose slope a face uses and every component: where J(p) is not
+   differentiable between the two inputs (a central difference straddling
+   such a switch measures a secant, not the derivative).  Both runs need
+   checkpoint_every = nsteps (every state kept).  compared: the slopes
+   compared. */
+int harm_adj_branch_switches(harm_adj *a, harm_adj *b, long long *switches, long long *compared);
 
 /* ---- tangent-linear (forward-mode) step (no reference counterpart) ----
    Advances a context's state together with its derivative with respect to

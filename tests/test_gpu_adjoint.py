@@ -330,3 +330,67 @@ def test_torch_autograd_harness_parameter_gradients():
         say(f"  dJ/d{name}: torch.autograd through the GPU adjoint {ad:.13e}, forward mode {float(dJ):.13e}")
         assert float(Jt) == pytest.approx(float(J), rel=1e-12)
         assert abs(ad - float(dJ)) <= 1e-9 * scale
+
+
+def _host_branch(lim, yl, yc, yr):
+    """slope_branch on numpy arrays (interp.c:106-149 minmod choices)"""
+    def mm(a, b):
+        code = np.where(np.abs(a) < np.abs(b), 1, 2)
+        m = np.where(np.abs(a) < np.abs(b), a, b)
+        zero = a * b <= 0
+        return np.where(zero, 0, code), np.where(zero, 0.0, m)
+    if lim == H.MC:
+        cin, m = mm(2.0 * (yc - yl), 2.0 * (yr - yc))
+        cout, _ = mm(0.5 * (yr - yl), m)
+        return cin | (cout << 2)
+    if lim == H.MINM:
+        return mm(yc - yl, yr - yc)[0]
+    if lim == H.VANL:
+        return np.where((yc - yl) * (yr - yc) <= 0.0, 0, 1)
+    return np.zeros_like(yc, dtype=int)
+
+
+def _host_switches(lim, a, b):
+    """branch codes of two bounded states compared where the faces use them"""
+    n1, n2 = a.shape[0] - 6, a.shape[1] - 6
+    n = c = 0
+    # x1 slopes of zones i = -1..N1 (rows 2..N1+3), j interior; x2 likewise
+    ca = _host_branch(lim, a[1:n1 + 3, 3:n2 + 3], a[2:n1 + 4, 3:n2 + 3], a[3:n1 + 5, 3:n2 + 3])
+    cb = _host_branch(lim, b[1:n1 + 3, 3:n2 + 3], b[2:n1 + 4, 3:n2 + 3], b[3:n1 + 5, 3:n2 + 3])
+    n += int((ca != cb).sum())
+    c += ca.size
+    ca = _host_branch(lim, a[3:n1 + 3, 1:n2 + 3], a[3:n1 + 3, 2:n2 + 4], a[3:n1 + 3, 3:n2 + 5])
+    cb = _host_branch(lim, b[3:n1 + 3, 1:n2 + 3], b[3:n1 + 3, 2:n2 + 4], b[3:n1 + 3, 3:n2 + 5])
+    n += int((ca != cb).sum())
+    c += ca.size
+    return n, c
+
+
+@pytest.mark.parametrize("lim", [H.MC, H.MINM, H.VANL])
+def test_branch_switches_count_limiter_branches(lim):
+    """harm_adj_branch_switches against the branch codes formed on the host
+    from the two bounded initial states (one forward-Euler step: its only
+    reconstruction input); zero for a run against itself and for a limiter
+    without branches"""
+    n = 24
+    say(f"[case] limiter branch switches, OT {n}x{n}, lim={lim}")
+    g, geom, prm = ot_setup(n, lim=lim, timeorder=1)
+    base = generic(ot_state(g))
+    w = np.random.default_rng(5).normal(0, 1, base.shape)
+    runs, states = [], []
+    for e in (1e-3, -1e-3, 1e-3):
+        h = H.Harm(prm, geom)
+        h.set_array(H.P, base + e * w)
+        h.bound_prim(H.P)
+        states.append(h.get_array(H.P))
+        h.set_time(0.0, 1e-4, 1e30)
+        a = H.Adjoint(h, 1, 1)
+        a.forward([1e-4])
+        runs.append((h, a))
+    got = runs[0][1].branch_switches(runs[1][1])
+    want = _host_switches(lim, states[0], states[1])
+    assert got == want and want[0] > 0
+    assert runs[0][1].branch_switches(runs[2][1]) == (0, want[1])
+    for h, a in runs:
+        a.close()
+        h.close()
