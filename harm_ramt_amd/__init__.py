@@ -50,6 +50,7 @@ class HarmParams(ctypes.Structure):
         ("rhominlimit", ctypes.c_double), ("uuminlimit", ctypes.c_double),
         ("dx1", ctypes.c_double), ("dx2", ctypes.c_double),
         ("boundsfile", ctypes.c_int), ("extrap", ctypes.c_int), ("disk_first", ctypes.c_int),
+        ("rescaletype", ctypes.c_int),
     ]
 
 
@@ -191,7 +192,7 @@ EXPORTS = [
     "harm_adj_seed_magnetic_energy", "harm_adj_seed", "harm_adj_backward", "harm_adj_get_lambda",
     "harm_adj_get_nonsmooth", "harm_adj_branch_switches",
     "harm_grid_mks", "harm_grid_cartesian", "harm_grid_cylmink", "harm_geometry", "harm_init_fishmon",
-    "harm_set_analytic",
+    "harm_set_analytic", "harm_set_radius",
     "harm_init_orszag_tang", "harm_init_monopole",
     "harm_rdump_write_host", "harm_rdump_read_host", "harm_rdump_write", "harm_rdump_read",
     "harm_ener_regions", "harm_acct_setup", "harm_acct_off", "harm_acct_set_order", "harm_acct_read",
@@ -273,6 +274,7 @@ def load_library(path=LIB_PATH):
         "harm_grid_cylmink": [ctypes.POINTER(HarmGrid), ctypes.c_int, ctypes.c_int, ctypes.c_double,
                               ctypes.c_double, ctypes.c_double, ctypes.c_double],
         "harm_set_analytic": [vp, dp, dp],
+        "harm_set_radius": [vp, dp],
         "harm_geometry": [ctypes.POINTER(HarmGrid), dp, dp, dp, dp, dp],
         "harm_init_fishmon": [ctypes.POINTER(HarmGrid), dp, dp, dp, ctypes.c_double, ctypes.c_double,
                               ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
@@ -350,6 +352,18 @@ class Geometry:
     def tables(self):
         return self.gcov, self.gcon, self.gdet, self.conn, self.floors
 
+    @property
+    def radius(self):
+        """r of every zone centre [N1M][N2M] (harm_zone_coords), for EXTRAP 2"""
+        if getattr(self, "_radius", None) is None:
+            g = self.grid
+            r = np.zeros((g.n1 + 2 * NG, g.n2 + 2 * NG))
+            for i in range(-NG, g.n1 + NG):
+                for j in range(-NG, g.n2 + NG):
+                    r[i + NG, j + NG] = zone_coords(g, i, j)[2]
+            self._radius = r
+        return self._radius
+
 
 def grid_mks(n1, n2, a=0.9375, Rout=40.0, hslope=0.3, R0=0.0):
     g = HarmGrid()
@@ -399,7 +413,7 @@ def init_monopole(geom: Geometry, B0=1.0):
 def make_params(n1, n2, dx1, dx2, *, lim=PARA, fluxmethod=HLLFLUX, timeorder=4, bc=None, fixupflux=1,
                 poledeath=1, checksolution=0, eno=0, gam=4.0 / 3.0, cour=0.9, safe=1.3, defcon=1.0, gammamax=25.0,
                 gammafail=None, rhominlimit=1e-20, uuminlimit=1e-20, fluxb=FLUXCTTOTH, eomtype=EOMGRMHD,
-                boundsfile=0, extrap=0, disk_first=0):
+                boundsfile=0, extrap=0, disk_first=0, rescaletype=1):
     """Parameter block with the reference's GRMHD defaults (initbase.c:119-243)."""
     prm = HarmParams()
     prm.n1, prm.n2 = n1, n2
@@ -413,7 +427,7 @@ def make_params(n1, n2, dx1, dx2, *, lim=PARA, fluxmethod=HLLFLUX, timeorder=4, 
     prm.gammafail = 100.0 * gammamax if gammafail is None else gammafail
     prm.rhominlimit, prm.uuminlimit = rhominlimit, uuminlimit
     prm.dx1, prm.dx2 = dx1, dx2
-    prm.boundsfile, prm.extrap, prm.disk_first = boundsfile, extrap, disk_first
+    prm.boundsfile, prm.extrap, prm.disk_first, prm.rescaletype = boundsfile, extrap, disk_first, rescaletype
     return prm
 
 
@@ -436,6 +450,8 @@ class Harm:
         self.h = h
         if geom is not None:
             self.set_geometry(*geom.tables())
+            if prm.extrap == 2:
+                self.set_radius(geom.radius)
 
     def close(self):
         if self.h:
@@ -447,6 +463,11 @@ class Harm:
             self.close()
         except Exception:
             pass
+
+    def set_radius(self, r):
+        """EXTRAP 2: r of every zone centre [N1M][N2M] (Geometry.radius)"""
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        _chk(self.lib.harm_set_radius(self.h, _dptr(r)), "harm_set_radius")
 
     def set_geometry(self, gcov, gcon, gdet, conn, floors):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (gcov, gcon, gdet, conn, floors)]

@@ -32,6 +32,7 @@ struct KParams {
   int bc[4];
   int fixupflux, poledeath, checksolution, eno;
   int boundsfile, extrap;  // bounds.*.c (HARM_BOUNDS_*), the outflow fill's EXTRAP
+  const double* rscale;    // EXTRAP 2: rescale() factors [nz][8] (harm_set_radius)
   double gam, cour, defcon, safe, gammamax, gammafail, rhominlimit, uuminlimit;
   double dx1, dx2;
   // floor/failure accounting (diag_fixup), set only while it is on: dVF,

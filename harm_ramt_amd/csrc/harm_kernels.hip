@@ -2026,6 +2026,13 @@ __device__ __forceinline__ void outflow_value(const KParams& P, const double* __
 #pragma unroll
   for (int k = 0; k < NPR; k++) v[k] = src[k];
   if (!P.extrap) return;
+  if (P.extrap == 2) {  // rescale(1, ...) of the edge zone, rescale(-1, ...) at the ghost
+    const double* sr = P.rscale + (size_t)zidx(P, ri, j) * 8;
+    const double* si = P.rscale + (size_t)zidx(P, i, j) * 8;
+#pragma unroll
+    for (int k = 0; k < NPR; k++) v[k] = (src[k] / sr[k]) * si[k];
+    return;
+  }
   const double gr = gC[(size_t)zidx(P, ri, j) * GCN + GC_GDET], gi = gC[(size_t)zidx(P, i, j) * GCN + GC_GDET];
   const bool up = ri > 0;
   const double dx1 = P.dx1;
@@ -2082,6 +2089,12 @@ __global__ void k_bg_x2(KParams P, double* __restrict__ pv, const double* __rest
     load8(pv, zidx(P, i, up ? j - P.n2 : j + P.n2), v);
   } else if (bc == HARM_POLARAXIS) {
     load8(pv, zidx(P, i, up ? (P.n2 - 1) + ((P.n2 - 1) - j + 1) : 0 + (0 - j - 1)), v);
+    if (P.boundsfile == HARM_BOUNDS_DISK) {  // bounds.c:214-233, 268-278: u and b antisymmetric across the pole
+      v[U2] *= -1.;
+      v[U3] *= 1.;
+      v[B2] *= -1.;
+      v[B3] *= 1.;
+    }
   } else if (bc == HARM_FIXED && P.boundsfile == HARM_BOUNDS_DISK) {
     load8(pan, zidx(P, i, j), v);
   } else {
@@ -2522,6 +2535,7 @@ struct harm_ctx {
   int pkid;                         // stage whose kernels are being timed, -1: none
   double* pan;                      // panalytic [nz][8] (harm_set_analytic), FIXED and bound_disk
   double* omf;                      // omegafanalytic [nz][8], bound_disk
+  double* rscale;                   // EXTRAP 2 rescale() factors [nz][8] (harm_set_radius)
   int perr;                         // a timed launch could not get its events
   double ktime[HARM_K_COUNT];
   long long kcount[HARM_K_COUNT];
@@ -2567,12 +2581,10 @@ static int check_prm(const harm_params* p) {
       return seterr("boundary type %d not supported", p->bc[k]);
   if (p->boundsfile != HARM_BOUNDS_FISHMON && p->boundsfile != HARM_BOUNDS_DISK)
     return seterr("boundsfile must be HARM_BOUNDS_FISHMON or HARM_BOUNDS_DISK");
-  if (p->extrap != 0 && p->extrap != 1) return seterr("extrap must be 0 or 1 (EXTRAP 2 is not built)");
-  if (p->boundsfile == HARM_BOUNDS_DISK) {
-    if (p->bc[HARM_X2DN] == HARM_POLARAXIS || p->bc[HARM_X2UP] == HARM_POLARAXIS)
-      return seterr("bounds.c's polar axis (sign flips) is not built: use HARM_BOUNDS_FISHMON");
-    if (p->n2 % 2) return seterr("bound_disk needs an even n2 (an equator between two rows)");
-  }
+  if (p->extrap < 0 || p->extrap > 2) return seterr("extrap must be 0, 1 or 2");
+  if (p->extrap == 2 && p->rescaletype != 0 && p->rescaletype != 1) return seterr("rescaletype must be 0 or 1");
+  if (p->boundsfile == HARM_BOUNDS_DISK && p->n2 % 2)
+    return seterr("bound_disk needs an even n2 (an equator between two rows)");
   if ((p->bc[HARM_X1DN] == HARM_POLARAXIS) || (p->bc[HARM_X1UP] == HARM_POLARAXIS))
     return seterr("POLARAXIS is an x2 boundary");
   if ((p->bc[HARM_X2DN] == HARM_OUTFLOW) || (p->bc[HARM_X2UP] == HARM_OUTFLOW))
@@ -2688,6 +2700,36 @@ extern "C" int harm_set_analytic(harm_ctx* c, const double* panalytic, const dou
   return 0;
 }
 
+// interp.c:559-620 rescale(), dir 1, VARTOINTERP PRIMTOINTERP: the factors of
+// every zone from its r, in the reference's operations (glibc pow on the host)
+extern "C" int harm_set_radius(harm_ctx* c, const double* r) {
+  if (!c) return seterr("null ctx");
+  if (!r) return seterr("null r");
+  const int rt = c->prm.rescaletype;
+  if (rt != 0 && rt != 1) return seterr("rescaletype must be 0 or 1");
+  std::vector<double> h((size_t)c->nz * 8);
+  for (size_t z = 0; z < (size_t)c->nz; z++) {
+    const double rr = r[z];
+    double* s = &h[z * 8];
+    s[RHO] = (rt == 0) ? pow(rr, -1.5) : pow(rr, -2.7);
+    s[UU] = s[RHO] / rr;
+    s[U1] = s[RHO];
+    s[U2] = 1.0;
+    s[U3] = 1.0 / (rr * rr);
+    s[B1] = (rt == 0) ? s[U3] : pow(rr, -2.4);
+    s[B2] = s[B1];
+    s[B3] = s[B1];
+  }
+  if (!c->rscale) HIPCHK(hipMalloc(&c->rscale, h.size() * sizeof(double)));
+  HIPCHK(hipMemcpy(c->rscale, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  c->kp.rscale = c->rscale;
+  if (c->graph_ok) {  // captured launches hold the previous parameters
+    (void)hipGraphExecDestroy(c->graph);
+    c->graph_ok = false;
+  }
+  return 0;
+}
+
 extern "C" int harm_destroy(harm_ctx* c) {
   if (!c) return 0;
   if (c->graph_ok) (void)hipGraphExecDestroy(c->graph);
@@ -2710,6 +2752,7 @@ extern "C" int harm_destroy(harm_ctx* c) {
   if (c->ds) (void)hipFree(c->ds);
   if (c->pan) (void)hipFree(c->pan);
   if (c->omf) (void)hipFree(c->omf);
+  if (c->rscale) (void)hipFree(c->rscale);
   if (c->evpool) {
     for (hipEvent_t e : *c->evpool) (void)hipEventDestroy(e);
     delete c->evpool;
@@ -3046,6 +3089,7 @@ static int launch_boundary(harm_ctx* c, int sel, int mode) {
     bool needs_analytic = c->prm.boundsfile == HARM_BOUNDS_DISK;
     if (needs_analytic && (!c->pan || !c->omf))
       return seterr("bound_prim: the analytic solution is not set (harm_set_analytic)");
+    if (c->prm.extrap == 2 && !c->rscale) return seterr("bound_prim: EXTRAP 2 needs the zone radii (harm_set_radius)");
     if (c->kp.checksolution && (mode & KB_FIXUP))
       return seterr("checksolution with extrapolated, FIXED or disk boundaries is not built");
     if (prof_begin(c, HARM_K_BOUNDARY)) return 1;

@@ -115,10 +115,14 @@ typedef struct {
   double dx1, dx2;     /* dx[1], dx[2] */
   int boundsfile;      /* HARM_BOUNDS_*: which bounds.*.c bound_prim() is */
   int extrap;          /* EXTRAP of the outflow fill (bounds.fishmon.c:8,
-                          bounds.c:5): 0 copy, 1 gdet / linear extrapolation */
+                          bounds.c:5): 0 copy, 1 gdet / linear extrapolation,
+                          2 power laws in r (rescale(), interp.c:559-620;
+                          needs harm_set_radius) */
   int disk_first;      /* bound_disk()'s function-static firsttime
                           (bounds.c:373): 1 if its next call is the first of
                           the process (no equatorial symmetrisation) */
+  int rescaletype;     /* rescaletype (initbase.c:200, default 1): the
+                          powers of EXTRAP 2's rescale() */
 } harm_params;
 
 typedef struct harm_ctx harm_ctx;
@@ -144,6 +148,11 @@ int harm_set_geometry(harm_ctx *ctx, const double *gcov, const double *gcon,
    bound_prim(); init.c:663: omegafanalytic, the field-line angular velocity
    of each zone in component 0), both [N1M][N2M][NPR] host arrays; omegaf
    may be NULL when bound_disk() is not used.  Copied to the device. */
+/* EXTRAP 2: r of every zone centre ([N1M][N2M], bl_coord of coord(i, j,
+   CENT), e.g. from harm_zone_coords); the rescale() factors of every zone
+   are formed from it on the host (interp.c:559-620, dir 1, PRIMTOINTERP,
+   the context's rescaletype) */
+int harm_set_radius(harm_ctx *ctx, const double *r);
 int harm_set_analytic(harm_ctx *ctx, const double *panalytic, const double *omegaf);
 
 /* Host <-> device copies of a context array (selector HARM_*). */

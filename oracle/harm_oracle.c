@@ -86,6 +86,7 @@ struct oracle_ctx {
   double lastdt;
   double *panalytic; /* [nz][8], FIXED boundaries and bound_disk */
   double *omegaf;    /* [nz][8], omegafanalytic (component 0 used) */
+  double *radius;    /* [nz], r of the zone centres (EXTRAP 2) */
   int disk_first;    /* bound_disk's static firsttime */
 };
 
@@ -1319,11 +1320,35 @@ static int inflow_check_rel4vel_x1(const oracle_ctx *c, double *pr, const geom_t
    EXTRAP 1 scales the densities and B1 by gdet and the other components
    linearly (bounds.fishmon.c:37-56 and 194-217, bounds.c:40-56 and 129-146;
    the up side's U1 with twice the slope) */
+/* interp.c:559-620 rescale(), dir 1, VARTOINTERP PRIMTOINTERP: the factors
+   at zone (i, j) of CENT from its r */
+static void rescale_factors(const oracle_ctx *c, int i, int j, double *scale) {
+  const double r = c->radius[zi(c, i, j)];
+  if (c->p.rescaletype == 0) scale[RHO] = pow(r, -1.5);
+  else scale[RHO] = pow(r, -2.7);
+  scale[UU] = scale[RHO] / r;
+  scale[U1] = scale[RHO];
+  scale[U2] = 1.0;
+  scale[U3] = 1.0 / (r * r);
+  if (c->p.rescaletype == 0) scale[B1] = scale[U3];
+  else scale[B1] = pow(r, -2.4);
+  scale[B2] = scale[B1];
+  scale[B3] = scale[B1];
+}
+
 static void outflow_value(const oracle_ctx *c, double *pr, int i, int j, int ri) {
   const double *src = PA(c, pr, ri, j);
   double *dst = PA(c, pr, i, j);
   if (!c->p.extrap) {
     memcpy(dst, src, sizeof(double) * NPR);
+    return;
+  }
+  if (c->p.extrap == 2) { /* bounds.fishmon.c:56-65, 219-228: rescale(1) at the edge, rescale(-1) here */
+    double sr[NPR], si[NPR], pre[NPR];
+    rescale_factors(c, ri, j, sr);
+    for (int k = 0; k < NPR; k++) pre[k] = src[k] / sr[k];
+    rescale_factors(c, i, j, si);
+    for (int k = 0; k < NPR; k++) dst[k] = pre[k] * si[k];
     return;
   }
   const double gr = c->gdet[(size_t)zi(c, ri, j) * 4 + CENT], gi = c->gdet[(size_t)zi(c, i, j) * 4 + CENT];
@@ -1429,6 +1454,10 @@ int or_bound_prim(oracle_ctx *c, int sel) {
   double *pr = c->arr[sel];
   int n1 = c->p.n1, n2 = c->p.n2, fail = 0;
   const int fixed_from_analytic = (c->p.boundsfile == 1);
+  if (c->p.extrap == 2 && !c->radius) { /* EXTRAP 2 without or_set_radius */
+    c->failed = 1;
+    return 1;
+  }
   /* x1 */
   if (c->p.bc[OR_X1DN] == OR_PERIODIC) {
     for (int j = 0; j < n2; j++)
@@ -1487,6 +1516,22 @@ int or_bound_prim(oracle_ctx *c, int sel) {
     for (int i = -NG; i <= n1 - 1 + NG; i++)
       for (int j = n2; j <= n2 - 1 + NG; j++) memcpy(PA(c, pr, i, j), PA(c, c->panalytic, i, j), sizeof(double) * NPR);
   }
+  /* bounds.c:214-233, 268-278: its polar axis makes u^2 and b^2
+     antisymmetric across the pole (bounds.fishmon.c leaves them, POSDEFMETRIC 0) */
+  if (c->p.boundsfile == 1) {
+    if (c->p.bc[OR_X2DN] == OR_POLARAXIS)
+      for (int i = -NG; i <= n1 - 1 + NG; i++)
+        for (int j = -NG; j < 0; j++) {
+          double *z = PA(c, pr, i, j);
+          z[U2] *= -1.; z[U3] *= 1.; z[B2] *= -1.; z[B3] *= 1.;
+        }
+    if (c->p.bc[OR_X2UP] == OR_POLARAXIS)
+      for (int i = -NG; i <= n1 - 1 + NG; i++)
+        for (int j = n2; j <= n2 - 1 + NG; j++) {
+          double *z = PA(c, pr, i, j);
+          z[U1] *= 1.; z[U2] *= -1.; z[U3] *= 1.; z[B1] *= 1.; z[B2] *= -1.; z[B3] *= 1.;
+        }
+  }
   if (c->p.poledeath) {
     if (c->p.bc[OR_X2DN] == OR_POLARAXIS)
       for (int i = -NG; i <= n1 - 1 + NG; i++)
@@ -1499,6 +1544,11 @@ int or_bound_prim(oracle_ctx *c, int sel) {
   if (c->p.boundsfile == 1) bound_disk(c, pr);
   if (fail) c->failed = 1;
   return fail;
+}
+
+void or_set_radius(oracle_ctx *c, const double *r) {
+  if (!c->radius) c->radius = (double *)calloc((size_t)c->nz, sizeof(double));
+  memcpy(c->radius, r, sizeof(double) * (size_t)c->nz);
 }
 
 void or_set_analytic(oracle_ctx *c, const double *panalytic, const double *omegaf) {
@@ -1854,7 +1904,7 @@ void or_destroy(oracle_ctx *c) {
   free(c->pflag);
   free(c->frdot); free(c->lumvsr);
   free(c->cf); free(c->fcon); free(c->jcon);
-  free(c->panalytic); free(c->omegaf);
+  free(c->panalytic); free(c->omegaf); free(c->radius);
   free(c);
 }
 

@@ -45,6 +45,7 @@ typedef struct {
   int boundsfile;       /* 0 bounds.fishmon.c, 1 bounds.c (FIXED from panalytic, bound_disk) */
   int extrap;           /* EXTRAP of the outflow fill (bounds.fishmon.c:8, bounds.c:5) */
   int disk_first;       /* bound_disk's static firsttime (bounds.c:373) */
+  int rescaletype;      /* rescaletype (initbase.c:200) of EXTRAP 2's rescale() */
   int nthreads;         /* OpenMP threads, 0 = default */
 } oracle_params;
 
@@ -76,6 +77,8 @@ int or_advance(oracle_ctx *c, int pi_sel, int pb_sel, const double *CUf,
 int or_bound_prim(oracle_ctx *c, int sel);
 /* panalytic and omegafanalytic ([N1M][N2M][NPR], omegaf may be NULL): the
    analytic solution FIXED boundaries and bound_disk read (copied) */
+/* EXTRAP 2: r of every zone centre [N1M][N2M] (bl_coord of coord(i,j,CENT)) */
+void or_set_radius(oracle_ctx *c, const double *r);
 void or_set_analytic(oracle_ctx *c, const double *panalytic, const double *omegaf);
 int or_post_fixup(oracle_ctx *c, int sel);
 int or_step(oracle_ctx *c);

@@ -25,7 +25,7 @@ class OracleParams(ctypes.Structure):
         ("gammamax", ctypes.c_double), ("gammafail", ctypes.c_double), ("rhominlimit", ctypes.c_double),
         ("uuminlimit", ctypes.c_double), ("dx1", ctypes.c_double), ("dx2", ctypes.c_double),
         ("boundsfile", ctypes.c_int), ("extrap", ctypes.c_int), ("disk_first", ctypes.c_int),
-        ("nthreads", ctypes.c_int),
+        ("rescaletype", ctypes.c_int), ("nthreads", ctypes.c_int),
     ]
 
 
@@ -103,6 +103,7 @@ def load():
         "or_current_get": ([vp, dp, dp], None),
         "or_dump_content": ([vp, dp], None),
         "or_set_analytic": ([vp, dp, dp], None),
+        "or_set_radius": ([vp, dp], None),
     }
     for name, (args, res) in sig.items():
         f = getattr(lib, name)
@@ -147,6 +148,10 @@ class Oracle:
         pa = np.ascontiguousarray(panalytic, dtype=np.float64)
         om = None if omegaf is None else np.ascontiguousarray(omegaf, dtype=np.float64)
         self.lib.or_set_analytic(self.h, _d(pa), _d(om) if om is not None else None)
+
+    def set_radius(self, r):
+        """EXTRAP 2: r of every zone centre [N1M][N2M]"""
+        self.lib.or_set_radius(self.h, _d(np.ascontiguousarray(r, dtype=np.float64)))
 
     def set_geometry(self, gcov, gcon, gdet, conn, floors):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (gcov, gcon, gdet, conn, floors)]

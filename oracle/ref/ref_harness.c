@@ -158,6 +158,10 @@ int main(int argc, char *argv[])
   {
     FILE *f;
     if (refinit(margv) >= 1) { fprintf(stderr, "init failed\n"); return 4; }
+    /* x2 boundary types other than init()'s, from the first step on (init()'s
+       own bound_prim() calls have run with its types) */
+    if (cfg("bc2", NULL)) BCtype[X2UP] = cfgi("bc2", BCtype[X2UP]);
+    if (cfg("bc3", NULL)) BCtype[X2DN] = cfgi("bc3", BCtype[X2DN]);
     if (cfg("tables_out", NULL)) {
       f = fopen(cfg("tables_out", ""), "wb");
       fwrite(&a_gcov[0][0][0][0][0], sizeof(FTYPE), (size_t)N1M * N2M * NPG * NDIM * NDIM, f);

@@ -56,8 +56,9 @@ CASES = {
 }
 # bounds.fishmon.c's outflow extrapolation EXTRAP of the cases that set it
 # (the reference build ksx1: a configured copy of bounds.fishmon.c)
-EXTRAP = {"torus_extrap1_para_hll_rk4": 1}
+EXTRAP = {"torus_extrap1_para_hll_rk4": 1, "torus_extrap2_mc_hll_rk2": 2}
 CASES["torus_extrap1_para_hll_rk4"] = ("ks", 32, 64, H.PARA, H.HLLFLUX, 4, 30, 1e-3, (1, 30))
+CASES["torus_extrap2_mc_hll_rk2"] = ("ks", 32, 64, H.MC, H.HLLFLUX, 2, 30, 1e-3, (1, 30))
 # DOENO of the cases that set it (the reference is built once per value)
 ENO = {"torus_eno_u_para_hll_rk4": 1, "torus_eno_du_mc_laxf_rk2": 2, "ot_eno_u_mc_hll_rk4": 1,
        "ot_eno_du_para_hll_rk3": 2}
@@ -250,6 +251,10 @@ def make_dump(name, work):
 # bound_disk's outflow branch for B1, B3.
 DISK_CASES = {"disk_default_mc_laxf_rk2": (32, 64, 40, 1e-3, 0.0, (1, 10, 40)),
               "disk_late_mc_laxf_rk2": (32, 64, 20, 1e-2, 20.0, (1, 20))}
+# the same problem with bounds.c's polar axis on both x2 sides (u^2, b^2
+# antisymmetric across the pole, POLEDEATH 1) from the first step on
+DISK_BC = {"disk_polar_mc_laxf_rk2": {"bc2": H.POLARAXIS, "bc3": H.POLARAXIS}}
+DISK_CASES["disk_polar_mc_laxf_rk2"] = (32, 64, 30, 1e-3, 0.0, (1, 30))
 
 
 def make_disk(name, work):
@@ -260,7 +265,7 @@ def make_disk(name, work):
         os.symlink(ref, link)
     cfg = dict(tables_out=os.path.join(work, "dtables.bin"), init_out=os.path.join(work, "dinit.bin"),
                params_out=os.path.join(work, "dparams.txt"), dt=dt0, t0=t0, nsteps=nsteps,
-               out=os.path.join(work, "do.bin"))
+               out=os.path.join(work, "do.bin"), **DISK_BC.get(name, {}))
     path = os.path.join(work, "disk.cfg")
     with open(path, "w") as f:
         for k, v in cfg.items():
@@ -316,7 +321,7 @@ def main():
             eno = ENO.get(name, 0)
             extrap = EXTRAP.get(name, 0)
             recs, states = run_reference(kind, g, geom, p0, lim, flux, to, nsteps, dt0, work, eno,
-                                         binary="ref_ksx1" if extrap else None)
+                                         binary=f"ref_ksx{extrap}" if extrap else None)
             out = dict(kind=kind, n1=n1, n2=n2, lim=lim, fluxmethod=flux, timeorder=to, nsteps=nsteps, dt0=dt0,
                        eno=eno, extrap=extrap,
                        input_sha=sha(p0), geom_sha=sha(np.concatenate([a.ravel() for a in geom.tables()])),

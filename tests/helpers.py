@@ -65,7 +65,9 @@ def disk_problem(gd):
     prm = H.make_params(n1, n2, g.dx1, g.dx2, lim=int(gd["lim"]), fluxmethod=int(gd["fluxmethod"]),
                         timeorder=int(gd["timeorder"]), fluxb=int(gd["fluxb"]), cour=float(gd["cour"]),
                         gam=float(gd["gam"]), gammamax=float(gd["gammamax"]), eomtype=H.EOMFFDE,
-                        bc=tuple(int(b) for b in gd["bc"]), poledeath=0, boundsfile=H.BOUNDS_DISK, extrap=1,
+                        bc=tuple(int(b) for b in gd["bc"]),
+                        poledeath=1 if H.POLARAXIS in tuple(int(b) for b in gd["bc"][2:]) else 0,
+                        boundsfile=H.BOUNDS_DISK, extrap=1,
                         disk_first=0)
     return g, geom, np.array(gd["p0"]), np.array(gd["panalytic"]), om, prm
 

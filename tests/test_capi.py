@@ -49,16 +49,16 @@ def test_parameter_errors_are_reported_before_any_gpu_call():
     prm = H.make_params(16, 16, 0.1, 0.1, bc=(H.PERIODIC, H.OUTFLOW, H.POLARAXIS, H.POLARAXIS))
     assert lib.harm_create(ctypes.byref(prm), ctypes.byref(h)) != 0
     assert b"periodic" in lib.harm_last_error()
-    # bounds.c rules: no polar axis (its sign flips are not built), an even N2
-    prm = H.make_params(16, 16, 0.1, 0.1, boundsfile=H.BOUNDS_DISK)
-    assert lib.harm_create(ctypes.byref(prm), ctypes.byref(h)) != 0
-    assert b"polar axis" in lib.harm_last_error()
+    # bounds.c rules: an even N2 (bound_disk's equator between two rows)
     prm = H.make_params(16, 15, 0.1, 0.1, boundsfile=H.BOUNDS_DISK, bc=(H.FIXED,) * 4)
     assert lib.harm_create(ctypes.byref(prm), ctypes.byref(h)) != 0
     assert b"even n2" in lib.harm_last_error()
-    prm = H.make_params(16, 16, 0.1, 0.1, extrap=2)
+    prm = H.make_params(16, 16, 0.1, 0.1, extrap=3)
     assert lib.harm_create(ctypes.byref(prm), ctypes.byref(h)) != 0
-    assert b"EXTRAP 2" in lib.harm_last_error()
+    assert b"extrap" in lib.harm_last_error()
+    prm = H.make_params(16, 16, 0.1, 0.1, extrap=2, rescaletype=2)
+    assert lib.harm_create(ctypes.byref(prm), ctypes.byref(h)) != 0
+    assert b"rescaletype" in lib.harm_last_error()
 
 
 def test_host_generators_run_without_gpu():

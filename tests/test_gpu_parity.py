@@ -34,6 +34,8 @@ def gpu_ctx(geom, prm, p0, dt0):
 def oracle_ctx(geom, prm, p0, dt0):
     o = Oracle(H.params_dict(prm))
     o.set_geometry(*geom.tables())
+    if prm.extrap == 2:
+        o.set_radius(geom.radius)
     o.set_array(0, p0)
     o.bound_prim(0)
     o.set_time(0.0, dt0, 1e30)

@@ -18,6 +18,8 @@ def test_oracle_matches_reference(name):
     assert sha(np.concatenate([a.ravel() for a in geom.tables()])) == str(gd["geom_sha"])
     o = Oracle(H.params_dict(prm))
     o.set_geometry(*geom.tables())
+    if prm.extrap == 2:
+        o.set_radius(geom.radius)
     o.set_array(0, p0)
     o.bound_prim(0)
     o.set_time(0.0, float(gd["dt0"]), 1e30)
