@@ -86,19 +86,31 @@ __device__ __forceinline__ void ndt_push(DevScalars* __restrict__ ds, int dir, d
   atomicMin(&ds->ndt_acc[dir - 1][stripe_of(bi) * SSTR], (unsigned long long)__double_as_longlong(wm));
 }
 // the Courant limits of the substage's flux launch into ndt_last, every
-// slot reset for the next one (one thread, between the flux launches)
-__device__ __forceinline__ void ndt_take(DevScalars* __restrict__ ds) {
+// slot reset for the next one (between the flux launches): one wavefront,
+// lane k taking slot k of both directions (one round trip, the minimum by
+// butterflies; a minimum does not depend on its order)
+static_assert(NSTRIPE == 64, "one slot per lane of a wavefront");
+__device__ __forceinline__ void ndt_take_wave(DevScalars* __restrict__ ds, int lane) {
   const unsigned long long big = (unsigned long long)__double_as_longlong(1.e9);
+  unsigned long long m[2] = {ds->ndt_acc[0][lane * SSTR], ds->ndt_acc[1][lane * SSTR]};
 #pragma unroll
   for (int d = 0; d < 2; d++) {
-    unsigned long long m = big;
-    for (int k = 0; k < NSTRIPE; k++) {
-      const unsigned long long v = ds->ndt_acc[d][k * SSTR];
-      m = (v < m) ? v : m;
-      ds->ndt_acc[d][k * SSTR] = big;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned long long o = __shfl_xor(m[d], off);
+      m[d] = (o < m[d]) ? o : m[d];
     }
-    ds->ndt_last[d] = __longlong_as_double((long long)m);
+    ds->ndt_acc[d][lane * SSTR] = big;
   }
+  if (lane == 0) {
+    ds->ndt_last[0] = __longlong_as_double((long long)m[0]);
+    ds->ndt_last[1] = __longlong_as_double((long long)m[1]);
+  }
+}
+// the first wavefront of workgroup (0, 0) of a zone launch takes them
+__device__ __forceinline__ void ndt_take_first_wave(DevScalars* __restrict__ ds, const dim3 bi) {
+  const int t = threadIdx.x + threadIdx.y * blockDim.x;
+  if (bi.x == 0 && bi.y == 0 && t < 64) ndt_take_wave(ds, t);
 }
 
 thread_local char g_err[512];
@@ -1142,7 +1154,7 @@ __device__ __forceinline__ void update_body(KParams P, RKCoef C, const double* _
   int niter = 0, failz = 0, fail = 0;
   __shared__ double edge_v[BND ? 64 * NPR : 1];
   const int eside = BND ? edge_block_side(P, bi) : -1;
-  if (i == 0 && j == 0) ndt_take(ds);  // this substage's Courant limits (advance.c:190 reads them)
+  ndt_take_first_wave(ds, bi);  // this substage's Courant limits (advance.c:190 reads them)
   if (i < P.n1 && j < P.n2) {
     const int z = zidx(P, i, j);
     const double dt = ds->dt;

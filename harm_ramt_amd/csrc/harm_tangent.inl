@@ -177,7 +177,7 @@ __global__ __launch_bounds__(256) void k_tl_update(KParams P, RKCoef C, TLArrays
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = blockIdx.y * blockDim.y + threadIdx.y;
   int fail = 0;
-  if (i == 0 && j == 0) ndt_take(ds);  // this substage's Courant limits (advance.c:190 reads them)
+  ndt_take_first_wave(ds, blockIdx);  // this substage's Courant limits (advance.c:190 reads them)
   if (i < P.n1 && j < P.n2) {
     const int z = zidx(P, i, j);
     const double dt = ds->dt;
