@@ -1100,12 +1100,7 @@ static int acct_flux(harm_ctx* c, double Dt) {
   return 0;
 }
 
-static void invalidate_graph(harm_ctx* c) {
-  if (c->graph_ok) {
-    (void)hipGraphExecDestroy(c->graph);
-    c->graph_ok = false;
-  }
-}
+static void invalidate_graph(harm_ctx* c) { drop_graphs(c); }
 
 extern "C" int harm_acct_setup(harm_ctx* c, const int* enerpos, const int* doflux, double dx3) {
   if (!c || !enerpos || !doflux) return seterr("null argument");

@@ -2534,8 +2534,10 @@ struct harm_ctx {
   DevScalars* ds;
   hipStream_t stream;
   hipStream_t own_stream;
-  hipGraphExec_t graph;
+  hipGraphExec_t graph;             // one step_ch() as a graph
   bool graph_ok;
+  hipGraphExec_t graphK;            // STEPS_PER_GRAPH steps in one graph (harm_step_n)
+  bool graphK_ok;
   bool geometry_set;
   bool prof;
   int flux_variant;   // HARM_FLUX_* chosen by the autotuner or the caller
@@ -2712,6 +2714,13 @@ extern "C" int harm_set_analytic(harm_ctx* c, const double* panalytic, const dou
   return 0;
 }
 
+// captured launches hold the parameters, stream and variants of their capture
+static void drop_graphs(harm_ctx* c) {
+  if (c->graph_ok) (void)hipGraphExecDestroy(c->graph);
+  if (c->graphK_ok) (void)hipGraphExecDestroy(c->graphK);
+  c->graph_ok = c->graphK_ok = false;
+}
+
 // interp.c:559-620 rescale(), dir 1, VARTOINTERP PRIMTOINTERP: the factors of
 // every zone from its r, in the reference's operations (glibc pow on the host)
 extern "C" int harm_set_radius(harm_ctx* c, const double* r) {
@@ -2735,16 +2744,13 @@ extern "C" int harm_set_radius(harm_ctx* c, const double* r) {
   if (!c->rscale) HIPCHK(hipMalloc(&c->rscale, h.size() * sizeof(double)));
   HIPCHK(hipMemcpy(c->rscale, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
   c->kp.rscale = c->rscale;
-  if (c->graph_ok) {  // captured launches hold the previous parameters
-    (void)hipGraphExecDestroy(c->graph);
-    c->graph_ok = false;
-  }
+  drop_graphs(c);
   return 0;
 }
 
 extern "C" int harm_destroy(harm_ctx* c) {
   if (!c) return 0;
-  if (c->graph_ok) (void)hipGraphExecDestroy(c->graph);
+  drop_graphs(c);
   for (int a = 0; a < HARM_NARRAYS; a++)
     if (c->arr[a]) (void)hipFree(c->arr[a]);
   double* dbufs[4] = {c->q1, c->q2, c->emf, c->pfix};
@@ -2780,10 +2786,7 @@ extern "C" int harm_destroy(harm_ctx* c) {
 extern "C" int harm_set_stream(harm_ctx* c, void* s) {
   if (!c) return seterr("null ctx");
   c->stream = s ? (hipStream_t)s : c->own_stream;
-  if (c->graph_ok) {
-    (void)hipGraphExecDestroy(c->graph);
-    c->graph_ok = false;
-  }
+  drop_graphs(c);
   return 0;
 }
 
@@ -3379,10 +3382,11 @@ extern "C" int harm_step(harm_ctx* c) {
   return 0;
 }
 
-static int capture_step(harm_ctx* c, hipGraphExec_t* ex) {
+static int capture_step(harm_ctx* c, hipGraphExec_t* ex, int nsteps = 1) {
   hipGraph_t g;
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  int rc = enqueue_step(c);
+  int rc = 0;
+  for (int s = 0; s < nsteps && !rc; s++) rc = enqueue_step(c);
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (rc) return 1;
   if (e != hipSuccess) return seterr("graph capture: %s", hipGetErrorString(e));
@@ -3396,6 +3400,16 @@ static int build_graph(harm_ctx* c) {
   if (c->graph_ok) return 0;
   if (capture_step(c, &c->graph)) return 1;
   c->graph_ok = true;
+  return 0;
+}
+
+// harm_step_n replays STEPS_PER_GRAPH steps per graph launch: a launch of a
+// whole graph costs ~8 us before its first kernel, a kernel inside one none
+constexpr int STEPS_PER_GRAPH = 8;
+static int build_graphK(harm_ctx* c) {
+  if (c->graphK_ok) return 0;
+  if (capture_step(c, &c->graphK, STEPS_PER_GRAPH)) return 1;
+  c->graphK_ok = true;
   return 0;
 }
 
@@ -3500,7 +3514,12 @@ extern "C" int harm_step_n(harm_ctx* c, int nsteps) {
       if (enqueue_step(c)) return 1;
   } else {
     if (build_graph(c)) return 1;
-    for (int s = 0; s < nsteps; s++) HIPCHK(hipGraphLaunch(c->graph, c->stream));
+    int s = 0;
+    if (nsteps >= STEPS_PER_GRAPH) {
+      if (build_graphK(c)) return 1;
+      for (; s + STEPS_PER_GRAPH <= nsteps; s += STEPS_PER_GRAPH) HIPCHK(hipGraphLaunch(c->graphK, c->stream));
+    }
+    for (; s < nsteps; s++) HIPCHK(hipGraphLaunch(c->graph, c->stream));
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   return 0;
@@ -3650,10 +3669,7 @@ extern "C" int harm_set_tuning(harm_ctx* c, int flux_variant, int bound_variant)
   c->flux_variant = flux_variant;
   c->bound_variant = bound_variant;
   c->tuned = true;
-  if (c->graph_ok) {
-    (void)hipGraphExecDestroy(c->graph);
-    c->graph_ok = false;
-  }
+  drop_graphs(c);
   return 0;
 }
 

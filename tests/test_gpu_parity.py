@@ -199,7 +199,10 @@ def test_gpu_ffde_utoprimgen_matches_oracle():
 
 
 def test_gpu_graph_replay_equals_single_steps():
-    say("[case] harm_step_n (HIP graph) vs harm_step, 64x64 torus, 12 steps")
+    """harm_step_n replays graphs of 8 steps and of one step: 12 and then
+    19 steps (two 8-step replays and three single ones) equal the same
+    steps launched one harm_step at a time"""
+    say("[case] harm_step_n (HIP graph) vs harm_step, 64x64 torus, 12 + 19 steps")
     g, geom, p0, prm = problem("ks", 64, 64)
     a = gpu_ctx(geom, prm, p0, 1e-3)
     b = gpu_ctx(geom, prm, p0, 1e-3)
@@ -208,6 +211,11 @@ def test_gpu_graph_replay_equals_single_steps():
     b.step_n(12)
     assert same(a.get_array(H.P), b.get_array(H.P))
     assert a.get_time() == b.get_time()
+    for _ in range(19):
+        a.step_ch()
+    b.step_n(19)
+    assert same(a.get_array(H.P), b.get_array(H.P))
+    assert a.get_time() == b.get_time() and a.nstroke() == b.nstroke()
 
 
 @pytest.mark.parametrize("kind,n", [("ks", 256), ("mink", 256), ("ffde", 512)])
