@@ -1144,6 +1144,7 @@ extern "C" int harm_acct_setup(harm_ctx* c, const int* enerpos, const int* doflu
   HIPCHK(hipMemsetAsync(a->lumvsr, 0, (size_t)n1 * sizeof(double), c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   c->kp.dVF = R.dVF;
+  a->tree = false;  // the reference's order until harm_acct_set_order says otherwise
   acct_set_on(c, true);
   invalidate_graph(c);
   return 0;
