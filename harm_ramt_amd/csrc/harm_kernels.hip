@@ -1569,10 +1569,27 @@ __device__ __forceinline__ bool interior(const KParams& P, int i, int j) {
 // cs (nullable): zones failed by fixup_checksolution in this pass.  Ghost
 // neighbours carry the flags bound_pflag copied before that check, so a
 // ghost whose source zone was failed by it still counts as good.
+// PF: the neighbours' densities and velocities (components RHO..U3, the ones
+// the repair averages) loaded with the flags (one memory round trip instead
+// of two; the values are those the branch then reads)
+template <bool PF = false>
 __device__ void fixup_zone(const KParams& P, const double* __restrict__ pv, const int* __restrict__ pflag,
                            const int* __restrict__ cs, double* __restrict__ pfix, const double* __restrict__ gC,
                            int z, int& fail, double* keep = nullptr) {
   const int i = z / P.N2M - NG, j = z % P.N2M - NG;
+  constexpr int NA = U3 + 1;
+  double nb[PF ? 9 : 1][PF ? NA : 1];
+  if constexpr (PF) {
+#pragma unroll
+    for (int di = -1; di <= 1; di++)
+#pragma unroll
+      for (int dj = -1; dj <= 1; dj++) {
+        const double* a = pv + (size_t)zidx(P, i + di, j + dj) * 8;
+        double* o = nb[(di + 1) * 3 + (dj + 1)];
+#pragma unroll
+        for (int k = 0; k < NA; k++) o[k] = a[k];
+      }
+  }
   double gc[36];
   load_cent(gC, z, gc);
   GeoU g;
@@ -1587,7 +1604,10 @@ __device__ void fixup_zone(const KParams& P, const double* __restrict__ pv, cons
     if (cs && !interior(P, a, b) && cs[zm]) f = 0;
     return f == 0;
   };
-  auto val = [&](int di, int dj, int k) { return pv[(size_t)zidx(P, i + di, j + dj) * 8 + k]; };
+  auto val = [&](int di, int dj, int k) {
+    if constexpr (PF) return nb[(di + 1) * 3 + (dj + 1)][k];
+    else return pv[(size_t)zidx(P, i + di, j + dj) * 8 + k];
+  };
   auto avg41 = [&](int k) { return 0.25 * (val(0, 1, k) + val(0, -1, k) + val(-1, 0, k) + val(1, 0, k)); };
   auto avg42 = [&](int k) { return 0.25 * (val(1, 1, k) + val(1, -1, k) + val(-1, 1, k) + val(-1, -1, k)); };
   int mode;
@@ -1956,17 +1976,20 @@ __global__ __launch_bounds__(1024) void k_fixall(KParams P, double* __restrict__
 // it is.  Then the t,dt update after the last substage.
 constexpr int FIXKEEP = 512;
 template <bool FIX>
-__global__ __launch_bounds__(1024) void k_fixbound(KParams P, double* __restrict__ pv, int* __restrict__ pflag,
-                                                   const int* __restrict__ faillist, double* __restrict__ pfix,
-                                                   const double* __restrict__ gC, DevScalars* __restrict__ ds,
-                                                   int do_dt) {
+__global__ __launch_bounds__(256) void k_fixbound(KParams P, double* __restrict__ pv, int* __restrict__ pflag,
+                                                  const int* __restrict__ faillist, double* __restrict__ pfix,
+                                                  const double* __restrict__ gC, DevScalars* __restrict__ ds,
+                                                  int do_dt) {
   __shared__ double keep[FIX ? FIXKEEP * NPR : 1];
   __shared__ int keepz[FIX ? FIXKEEP : 1];
+  // the t, dt update after the last substage reads only the step's scalars:
+  // one thread of the last wavefront, beside the repair
+  if (do_dt && threadIdx.x == blockDim.x - 1) dt_update(P, ds);
   int n = FIX ? ds->nlist : 0, fail = 0;
   if (n > P.n1 * P.n2) n = P.n1 * P.n2;
   for (int t = threadIdx.x; t < n; t += blockDim.x) {
     const int z = faillist[t];
-    fixup_zone(P, pv, pflag, nullptr, pfix, gC, z, fail, t < FIXKEEP ? keep + (size_t)t * NPR : nullptr);
+    fixup_zone<true>(P, pv, pflag, nullptr, pfix, gC, z, fail, t < FIXKEEP ? keep + (size_t)t * NPR : nullptr);
     if (t < FIXKEEP) keepz[t] = z;
   }
   __syncthreads();  // every repaired state computed from the unrepaired array
@@ -2016,10 +2039,7 @@ __global__ __launch_bounds__(1024) void k_fixbound(KParams P, double* __restrict
     }
   }
   if (fail) atomicOr(&ds->failed, 1);
-  if (threadIdx.x == 0) {  // every thread read nlist before the barrier
-    if (FIX) ds->nlist = 0;
-    if (do_dt) dt_update(P, ds);
-  }
+  if (FIX && threadIdx.x == 0) ds->nlist = 0;  // every thread read nlist before the barrier
 }
 
 // ---- bound_prim() with the general rules: the outflow extrapolation
@@ -3121,10 +3141,10 @@ static int launch_boundary(harm_ctx* c, int sel, int mode) {
   if (mode & KB_INUPDATE) {  // the first bound_prim() was k_update's
     const int do_dt = (mode & KB_DT) ? 1 : 0;
     if (mode & KB_FIXUP) {
-      tlaunch(c, k_fixbound<true>, dim3(1), dim3(1024), 0, c->kp, c->arr[sel], c->pflag, c->faillist, c->pfix,
+      tlaunch(c, k_fixbound<true>, dim3(1), dim3(256), 0, c->kp, c->arr[sel], c->pflag, c->faillist, c->pfix,
               c->gC, c->ds, do_dt);
     } else if ((mode & KB_BOUND2) || do_dt) {
-      tlaunch(c, k_fixbound<false>, dim3(1), dim3(1024), 0, c->kp, c->arr[sel], c->pflag, c->faillist, c->pfix,
+      tlaunch(c, k_fixbound<false>, dim3(1), dim3(256), 0, c->kp, c->arr[sel], c->pflag, c->faillist, c->pfix,
               c->gC, c->ds, do_dt);
     }
   } else if (c->bound_variant == HARM_BOUND_BLOCK && !c->kp.checksolution) {
