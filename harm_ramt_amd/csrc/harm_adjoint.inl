@@ -403,6 +403,8 @@ struct harm_adj {
   const double** d_slot;           // pb and pf of the substage being reversed
   hipGraphExec_t gx[5];            // steps: record dt, impose dt, impose + store, reverse, record + store
   bool gx_ok[5];
+  hipGraphExec_t gx8[5];           // the same, ADJ_STEPS_PER_GRAPH steps per graph
+  bool gx8_ok[5];
   bool forward_done, seeded;
   bool kept_last;                  // seg holds the last segment's states from the forward run
   int recompute_failures;
@@ -423,8 +425,10 @@ static int adj_check(const harm_ctx* c) {
 
 extern "C" int harm_adj_destroy(harm_adj* a) {
   if (!a) return 0;
-  for (int k = 0; k < 5; k++)
+  for (int k = 0; k < 5; k++) {
     if (a->gx_ok[k]) (void)hipGraphExecDestroy(a->gx[k]);
+    if (a->gx8_ok[k]) (void)hipGraphExecDestroy(a->gx8[k]);
+  }
   if (a->d_ctr) (void)hipFree(a->d_ctr);
   if (a->d_slot) (void)hipFree((void*)a->d_slot);
   if (a->ckpt)
@@ -567,22 +571,44 @@ static int adj_reverse_step(harm_adj* a) {
 // graph k: 0 forward step recording dt, 1 forward step imposing dt,
 // 2 recompute step (imposed dt, states stored), 3 reverse step, 4 forward
 // step recording dt with its states stored
-static int adj_graph(harm_adj* a, int k) {
-  if (a->gx_ok[k]) return 0;
+// (each step advances the device step counter itself, so a graph of several
+// steps is that many steps)
+constexpr int ADJ_STEPS_PER_GRAPH = 8;
+static int adj_graph(harm_adj* a, int k, bool multi = false) {
+  hipGraphExec_t* ex = multi ? &a->gx8[k] : &a->gx[k];
+  bool* ok = multi ? &a->gx8_ok[k] : &a->gx_ok[k];
+  if (*ok) return 0;
   harm_ctx* c = a->c;
   hipGraph_t g;
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  int rc = (k == 3) ? adj_reverse_step(a) : adj_step(a, (k == 0 || k == 4) ? 0 : 1, k == 2 || k == 4);
+  int rc = 0;
+  for (int s = 0; s < (multi ? ADJ_STEPS_PER_GRAPH : 1) && !rc; s++)
+    rc = (k == 3) ? adj_reverse_step(a) : adj_step(a, (k == 0 || k == 4) ? 0 : 1, k == 2 || k == 4);
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (rc) {
     if (e == hipSuccess) (void)hipGraphDestroy(g);
     return 1;
   }
   if (e != hipSuccess) return seterr("adjoint graph capture: %s", hipGetErrorString(e));
-  e = hipGraphInstantiate(&a->gx[k], g, nullptr, nullptr, 0);
+  e = hipGraphInstantiate(ex, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) return seterr("adjoint graph instantiate: %s", hipGetErrorString(e));
-  a->gx_ok[k] = true;
+  *ok = true;
+  return 0;
+}
+
+// count consecutive steps of graph k: multi-step replays, then single ones
+static int adj_launch_steps(harm_adj* a, int k, int count) {
+  harm_ctx* c = a->c;
+  int n = 0;
+  if (count >= ADJ_STEPS_PER_GRAPH) {
+    if (adj_graph(a, k, true)) return 1;
+    for (; n + ADJ_STEPS_PER_GRAPH <= count; n += ADJ_STEPS_PER_GRAPH)
+      if (hipGraphLaunch(a->gx8[k], c->stream) != hipSuccess) return seterr("adjoint: graph launch");
+  }
+  if (n < count && adj_graph(a, k)) return 1;
+  for (; n < count; n++)
+    if (hipGraphLaunch(a->gx[k], c->stream) != hipSuccess) return seterr("adjoint: graph launch");
   return 0;
 }
 
@@ -603,20 +629,19 @@ extern "C" int harm_adj_forward(harm_adj* a, const double* dts) {
   int rc = adj_graph(a, gk);
   if (!rc) rc = adj_graph(a, gks);
   if (!rc) hipLaunchKernelGGL(k_adj_setctr, dim3(1), dim3(1), 0, c->stream, a->d_ctr, 0, 0);
-  for (int n = 0; n < a->nsteps && !rc; n++) {
-    if (n % a->K == 0) {
-      const int m = n / a->K;
-      if (hipMemcpyAsync((*a->ckpt)[m], c->arr[HARM_P], sb, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
-          hipMemcpyAsync(a->d_ckds + m, c->ds, sizeof(DevScalars), hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
-        rc = seterr("adjoint forward: checkpoint copy");
-    }
+  for (int n = 0; n < a->nsteps && !rc;) {  // one checkpoint segment at a time
+    const int m = n / a->K;
+    if (hipMemcpyAsync((*a->ckpt)[m], c->arr[HARM_P], sb, hipMemcpyDeviceToDevice, c->stream) != hipSuccess ||
+        hipMemcpyAsync(a->d_ckds + m, c->ds, sizeof(DevScalars), hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
+      rc = seterr("adjoint forward: checkpoint copy");
     if (!rc && n == n0last) {  // the last segment: its initial state, then every substage's
       if (hipMemcpyAsync(a->seg, c->arr[HARM_P], sb, hipMemcpyDeviceToDevice, c->stream) != hipSuccess)
         rc = seterr("adjoint forward: segment store");
       hipLaunchKernelGGL(k_adj_setctr, dim3(1), dim3(1), 0, c->stream, a->d_ctr, n0last, n0last);
     }
-    if (!rc && hipGraphLaunch(a->gx[n >= n0last ? gks : gk], c->stream) != hipSuccess)
-      rc = seterr("adjoint forward: graph launch");
+    const int next = (n + a->K < a->nsteps) ? n + a->K : a->nsteps;
+    if (!rc) rc = adj_launch_steps(a, n >= n0last ? gks : gk, next - n);
+    n = next;
   }
   c->prof = was_prof;
   if (rc) return 1;
@@ -697,8 +722,7 @@ extern "C" int harm_adj_backward(harm_adj* a) {
     const int n0 = m * a->K, n1 = (n0 + a->K < a->nsteps) ? n0 + a->K : a->nsteps;
     if (m == a->nseg - 1 && a->kept_last) {  // stored by the forward run
       hipLaunchKernelGGL(k_adj_setctr, dim3(1), dim3(1), 0, c->stream, a->d_ctr, n1 - 1, n0);
-      for (int n = n1 - 1; n >= n0 && !rc; n--)
-        if (hipGraphLaunch(a->gx[3], c->stream) != hipSuccess) rc = seterr("adjoint reverse: graph launch");
+      rc = adj_launch_steps(a, 3, n1 - n0);
       continue;
     }
     // recompute the segment, storing every substage state
@@ -708,15 +732,13 @@ extern "C" int harm_adj_backward(harm_adj* a) {
     int nf0 = 0, nf1 = 0, dummy = 0;
     if (harm_get_failures(c, &nf0, &dummy)) return 1;
     hipLaunchKernelGGL(k_adj_setctr, dim3(1), dim3(1), 0, c->stream, a->d_ctr, n0, n0);
-    for (int n = n0; n < n1 && !rc; n++)
-      if (hipGraphLaunch(a->gx[2], c->stream) != hipSuccess) rc = seterr("adjoint recompute: graph launch");
+    rc = adj_launch_steps(a, 2, n1 - n0);
     if (rc) break;
     if (harm_get_failures(c, &nf1, &dummy)) return 1;
     a->recompute_failures += nf1 - nf0;
     // reverse the steps of the segment, last first
     hipLaunchKernelGGL(k_adj_setctr, dim3(1), dim3(1), 0, c->stream, a->d_ctr, n1 - 1, n0);
-    for (int n = n1 - 1; n >= n0 && !rc; n--)
-      if (hipGraphLaunch(a->gx[3], c->stream) != hipSuccess) rc = seterr("adjoint reverse: graph launch");
+    rc = adj_launch_steps(a, 3, n1 - n0);
   }
   c->prof = was_prof;
   if (a->nseg > 1) a->kept_last = false;  // the earlier segments' recomputes overwrote it

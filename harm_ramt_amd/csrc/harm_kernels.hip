@@ -1985,10 +1985,13 @@ __global__ __launch_bounds__(256) void k_fixbound(KParams P, double* __restrict_
   // the t, dt update after the last substage reads only the step's scalars:
   // one thread of the last wavefront, beside the repair
   if (do_dt && threadIdx.x == blockDim.x - 1) dt_update(P, ds);
+  const int cap = P.n1 * P.n2;
   int n = FIX ? ds->nlist : 0, fail = 0;
-  if (n > P.n1 * P.n2) n = P.n1 * P.n2;
+  // the list's first entries read beside its length (used only below n)
+  const int z0 = (FIX && (int)threadIdx.x < cap) ? faillist[threadIdx.x] : 0;
+  if (n > cap) n = cap;
   for (int t = threadIdx.x; t < n; t += blockDim.x) {
-    const int z = faillist[t];
+    const int z = (t == (int)threadIdx.x) ? z0 : faillist[t];
     fixup_zone<true>(P, pv, pflag, nullptr, pfix, gC, z, fail, t < FIXKEEP ? keep + (size_t)t * NPR : nullptr);
     if (t < FIXKEEP) keepz[t] = z;
   }
