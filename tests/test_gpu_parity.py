@@ -236,6 +236,23 @@ def test_gpu_fullsize_matches_oracle(kind, n):
     assert h.nstroke() == o.nstroke()
 
 
+def test_gpu_largest_grid_matches_oracle():
+    """a 1024x1024 torus (1.06 M zones with ghosts, 9.5 M fixup-event slots
+    when accounting) for 2 steps through the autotuned graph path, bitwise
+    against the oracle: index arithmetic at four times the benchmark's size"""
+    n = 1024
+    say(f"[case] torus {n}x{n}, 2 steps via harm_step_n vs oracle")
+    g, geom, p0, prm = problem("ks", n, n)
+    h = gpu_ctx(geom, prm, p0, 1e-3)
+    o = oracle_ctx(geom, prm, p0, 1e-3)
+    h.step_n(2)
+    for _ in range(2):
+        o.step()
+    assert h.get_time() == o.get_time()
+    assert same(h.get_array(H.P), o.get_array(0))
+    assert h.nstroke() == o.nstroke()
+
+
 @pytest.mark.parametrize("kind,eno", [("ks", 1), ("ks", 2), ("mink", 1), ("mink", 2)])
 def test_gpu_eno_graph_matches_oracle(kind, eno):
     """DOENO through the autotuned graph path at 128x128, bitwise vs the oracle."""
