@@ -82,6 +82,32 @@ def setup(n1, n2):
     return g, geom, p0, prm
 
 
+def grid_scan(sizes=(512, 1024), steps=40):
+    """The same torus at larger grids (not part of `value`): zone-updates/s
+    once the grid holds more wavefronts than the chip has SIMDs (256^2 is
+    1,024 update wavefronts for 1,024 SIMDs)"""
+    import harm_ramt_amd as H
+    out = {}
+    for n in sizes:
+        g, geom, p0, prm = setup(n, n)
+        h = H.Harm(prm, geom)
+        h.set_array(H.P, p0)
+        h.bound_prim(H.P)
+        h.set_time(0.0, 1e-5, 2000.0)
+        h.step_n(8)                        # autotuner, graphs
+        h.get_time()
+        t0 = time.perf_counter()
+        h.step_n(steps)
+        h.get_time()
+        el = time.perf_counter() - t0
+        fv, bv = h.get_tuning()
+        out[f"{n}x{n}"] = {"zone_updates_per_s": steps * n * n / el, "ms_per_step": 1e3 * el / steps,
+                           "steps": steps, "variants": [fv, bv]}
+        h.close()
+        del geom, p0
+    return out
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -584,6 +610,7 @@ def main():
     ap.add_argument("--batch-runs", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-grid-scan", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-acct", action="store_true")
     ap.add_argument("--no-ffde", action="store_true")
@@ -700,6 +727,7 @@ def main():
     h.close()
     forced = [int(v) for v in args.tuning.split(",")] if args.tuning else None
     batched = None if args.no_batch else batch_sweep(args.batch_runs, args.batch_steps, 3, tuning=forced)
+    scan = None if args.no_grid_scan else grid_scan()
     adjoint = None if args.no_adjoint else adjoint_measure()
     ffde = None if args.no_ffde else ffde_measure(args.ffde_steps, 3)
     if ffde is not None and not args.no_spin:
@@ -725,6 +753,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "grid_scan": scan,
         "adjoint": adjoint,
         "ffde": ffde,
         "accounting": accounting,
