@@ -1548,11 +1548,15 @@ __device__ void bound_block(const KParams& P, double* __restrict__ pr, const dou
 // pflag of a zone at distance <= 1 from the interior, as bound_pflag()
 // (boundsint.c:4-60) leaves it: x2 mapping first (polar reflection / periodic
 // wrap over all i), then x1 (outflow copy of the edge zone / periodic wrap).
+// A FIXED x1 side is left alone there ("if fixed BC: do nothing",
+// boundsint.c:34), so its ghosts keep the flag they started with: the ghost
+// zone itself, whose entry nothing writes (zero, UTOPRIMNOFAIL).
 __device__ __forceinline__ int pflag_zone(const KParams& P, int i, int j) {
   if (j < 0) j = (P.bc[HARM_X2DN] == HARM_PERIODIC) ? j + P.n2 : (0 + (0 - j - 1));
   else if (j >= P.n2) j = (P.bc[HARM_X2UP] == HARM_PERIODIC) ? j - P.n2 : ((P.n2 - 1) + ((P.n2 - 1) - j + 1));
-  if (i < 0) i = (P.bc[HARM_X1DN] == HARM_PERIODIC) ? i + P.n1 : 0;
-  else if (i >= P.n1) i = (P.bc[HARM_X1UP] == HARM_PERIODIC) ? i - P.n1 : P.n1 - 1;
+  if (i < 0) i = (P.bc[HARM_X1DN] == HARM_PERIODIC) ? i + P.n1 : (P.bc[HARM_X1DN] == HARM_FIXED ? i : 0);
+  else if (i >= P.n1)
+    i = (P.bc[HARM_X1UP] == HARM_PERIODIC) ? i - P.n1 : (P.bc[HARM_X1UP] == HARM_FIXED ? i : P.n1 - 1);
   return zidx(P, i, j);
 }
 __device__ __forceinline__ int pflag_at(const KParams& P, const int* __restrict__ pflag, int i, int j) {

@@ -422,6 +422,36 @@ def test_gpu_fill_in_update_with_failures_at_the_edges(case):
     assert pole > 0 and edge > 0, "the run is meant to repair pole-row and edge zones"
 
 
+def test_gpu_fixed_x1_sides_repair_reads_untouched_ghost_flags():
+    """A torus with FIXED x1 sides (bounds.fishmon.c leaves those ghosts
+    alone) whose inversions fail next to both sides: bound_pflag does
+    nothing on a FIXED side (boundsint.c:34), so the repair of an edge zone
+    counts its ghost neighbours as good, not as copies of the edge zone's
+    own failure flag.  Bitwise against the oracle, with failures on the
+    edge columns."""
+    n1, n2 = 32, 64
+    say(f"[case] FIXED x1 sides: torus {n1}x{n2} with failures at the edges, 6 steps")
+    g, geom, p0, prm = problem("ks", n1, n2)
+    prm.bc[H.X1DN] = prm.bc[H.X1UP] = H.FIXED
+    p0 = p0.copy()
+    strong_field_at_edges(p0, n1, n2)
+    p0[n1:n1 + NG, NG:NG + n2, 5] += 10.0 * np.sqrt(p0[n1:n1 + NG, NG:NG + n2, 0])
+    h = gpu_ctx(geom, prm, p0, 1e-3)
+    o = oracle_ctx(geom, prm, p0, 1e-3)
+    edge = 0
+    for s in range(6):
+        h.step_ch()
+        o.step()
+        assert h.get_time() == o.get_time(), f"step {s + 1}"
+        assert same(h.get_array(H.P), o.get_array(0)), f"step {s + 1}: state vs oracle"
+        fl = o.pflag()[NG:NG + n1, NG:NG + n2, 0]
+        assert np.array_equal(h.get_pflag()[NG:NG + n1, NG:NG + n2, 0], fl), f"step {s + 1}: pflag"
+        edge += int(np.count_nonzero(fl[[0, n1 - 1], :]))
+    assert h.nstroke() == o.nstroke()
+    say(f"  flagged zones on the x1 edge columns over the run: {edge}")
+    assert edge > 0, "the run is meant to repair edge zones"
+
+
 def test_gpu_newton_spread_counts():
     """harm_get_newton_spread: its iteration total is harm_get_nstroke's, and
     the wavefront maxima bound it from above (64 lanes a wavefront)"""
