@@ -101,8 +101,23 @@ def grid_scan(sizes=(512, 1024), steps=40):
         h.get_time()
         el = time.perf_counter() - t0
         fv, bv = h.get_tuning()
+        # the flux and update kernels' HBM rates at this size, from HIP events
+        # around each launch of an eagerly launched continuation of the run
+        h.prof_enable(True)
+        h.prof_reset()
+        h.step_n(min(steps, 20))
+        h.get_time()
+        kernels = {}
+        for kid, name in H.STAGE_NAMES.items():
+            ms, cnt = h.prof_get(kid)
+            if cnt and name in BYTES_PER_UNIT:
+                avg = 1e3 * ms / cnt
+                gbs = BYTES_PER_UNIT[name] * units_per_launch(name, n, n) / (avg * 1e-6) / 1e9
+                kernels[kernel_of(name, fv)] = {"avg_launch_us": round(avg, 3), "achieved_GBs": round(gbs, 1),
+                                                "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        h.prof_enable(False)
         out[f"{n}x{n}"] = {"zone_updates_per_s": steps * n * n / el, "ms_per_step": 1e3 * el / steps,
-                           "steps": steps, "variants": [fv, bv]}
+                           "steps": steps, "variants": [fv, bv], "kernels": kernels}
         h.close()
         del geom, p0
     return out
