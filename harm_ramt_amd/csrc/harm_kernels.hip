@@ -1548,16 +1548,21 @@ __device__ void bound_block(const KParams& P, double* __restrict__ pr, const dou
 // pflag of a zone at distance <= 1 from the interior, as bound_pflag()
 // (boundsint.c:4-60) leaves it: x2 mapping first (polar reflection / periodic
 // wrap over all i), then x1 (outflow copy of the edge zone / periodic wrap).
-// A FIXED x1 side is left alone there ("if fixed BC: do nothing",
-// boundsint.c:34), so its ghosts keep the flag they started with: the ghost
-// zone itself, whose entry nothing writes (zero, UTOPRIMNOFAIL).
+// (A FIXED x1 side is left alone there: fixed_x1_ghost.)
 __device__ __forceinline__ int pflag_zone(const KParams& P, int i, int j) {
   if (j < 0) j = (P.bc[HARM_X2DN] == HARM_PERIODIC) ? j + P.n2 : (0 + (0 - j - 1));
   else if (j >= P.n2) j = (P.bc[HARM_X2UP] == HARM_PERIODIC) ? j - P.n2 : ((P.n2 - 1) + ((P.n2 - 1) - j + 1));
-  if (i < 0) i = (P.bc[HARM_X1DN] == HARM_PERIODIC) ? i + P.n1 : (P.bc[HARM_X1DN] == HARM_FIXED ? i : 0);
-  else if (i >= P.n1)
-    i = (P.bc[HARM_X1UP] == HARM_PERIODIC) ? i - P.n1 : (P.bc[HARM_X1UP] == HARM_FIXED ? i : P.n1 - 1);
+  if (i < 0) i = (P.bc[HARM_X1DN] == HARM_PERIODIC) ? i + P.n1 : 0;
+  else if (i >= P.n1) i = (P.bc[HARM_X1UP] == HARM_PERIODIC) ? i - P.n1 : P.n1 - 1;
   return zidx(P, i, j);
+}
+// A ghost column of a FIXED x1 side: bound_pflag does nothing there ("if
+// fixed BC: do nothing", boundsint.c:34) and nothing else writes its flags,
+// so they keep set_arrays()'s initial -100 (set_arrays.c:86), which the
+// repair counts as a failed neighbour.
+constexpr int PFLAG_INIT = -100;
+__device__ __forceinline__ bool fixed_x1_ghost(const KParams& P, int i) {
+  return (i < 0 && P.bc[HARM_X1DN] == HARM_FIXED) || (i >= P.n1 && P.bc[HARM_X1UP] == HARM_FIXED);
 }
 __device__ __forceinline__ int pflag_at(const KParams& P, const int* __restrict__ pflag, int i, int j) {
   return pflag[pflag_zone(P, i, j)];
@@ -1604,7 +1609,7 @@ __device__ void fixup_zone(const KParams& P, const double* __restrict__ pv, cons
   if (gamma > P.gammamax) gamma = P.gammamax;
   auto good = [&](int di, int dj) {
     const int a = i + di, b = j + dj, zm = pflag_zone(P, a, b);
-    int f = pflag[zm];
+    int f = fixed_x1_ghost(P, a) ? PFLAG_INIT : pflag[zm];
     if (cs && !interior(P, a, b) && cs[zm]) f = 0;
     return f == 0;
   };
@@ -2622,6 +2627,11 @@ static int check_prm(const harm_params* p) {
       return seterr("boundary type %d not supported", p->bc[k]);
   if (p->boundsfile != HARM_BOUNDS_FISHMON && p->boundsfile != HARM_BOUNDS_DISK)
     return seterr("boundsfile must be HARM_BOUNDS_FISHMON or HARM_BOUNDS_DISK");
+  // bounds.fishmon.c has no FIXED branch: the reference leaves such ghosts
+  // alone, so its Runge-Kutta stage buffers keep set_arrays()'s zeros there
+  for (int k = 0; k < 4; k++)
+    if (p->bc[k] == HARM_FIXED && p->boundsfile != HARM_BOUNDS_DISK)
+      return seterr("FIXED sides need HARM_BOUNDS_DISK (bounds.c fills them from panalytic)");
   if (p->extrap < 0 || p->extrap > 2) return seterr("extrap must be 0, 1 or 2");
   if (p->extrap == 2 && p->rescaletype != 0 && p->rescaletype != 1) return seterr("rescaletype must be 0 or 1");
   if (p->boundsfile == HARM_BOUNDS_DISK && p->n2 % 2)

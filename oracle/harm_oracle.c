@@ -1884,6 +1884,14 @@ oracle_ctx *or_create(const oracle_params *prm) {
   c->floors = (double *)calloc((size_t)c->nz * 2, sizeof(double));
   for (int a = 0; a < 12; a++) c->arr[a] = (double *)calloc((size_t)c->nz * NPR, sizeof(double));
   c->pflag = (int *)calloc((size_t)c->nz * NUMPFLAGS, sizeof(int));
+  /* set_arrays.c:86: every zone's flags start at -100.  The interior's
+     failure flags are written by each step's inversion before anything reads
+     them, and bound_pflag overwrites the ghosts of every side but a FIXED x1
+     one, whose ghosts keep this value. */
+  if (c->pflag)
+    for (int i = -NG; i < c->p.n1 + NG; i++)
+      for (int j = -NG; j < c->p.n2 + NG; j++)
+        if (i < 0 || i >= c->p.n1) c->pflag[(size_t)zi(c, i, j) * NUMPFLAGS + 0] = -100;
   c->gammacheck = (double *)calloc((size_t)(prm->n1 + 2) * (prm->n2 + 2), sizeof(double));
   for (int a = 0; a < 3; a++) c->eno[a] = (double *)calloc((size_t)c->nz * NPR, sizeof(double));
   c->frdot = (double *)calloc((size_t)prm->n1 * NPR, sizeof(double));

@@ -59,6 +59,10 @@ def test_parameter_errors_are_reported_before_any_gpu_call():
     prm = H.make_params(16, 16, 0.1, 0.1, extrap=2, rescaletype=2)
     assert lib.harm_create(ctypes.byref(prm), ctypes.byref(h)) != 0
     assert b"rescaletype" in lib.harm_last_error()
+    # bounds.fishmon.c has no FIXED branch
+    prm = H.make_params(16, 16, 0.1, 0.1, bc=(H.FIXED, H.FIXED, H.POLARAXIS, H.POLARAXIS))
+    assert lib.harm_create(ctypes.byref(prm), ctypes.byref(h)) != 0
+    assert b"FIXED sides need HARM_BOUNDS_DISK" in lib.harm_last_error()
 
 
 def test_host_generators_run_without_gpu():

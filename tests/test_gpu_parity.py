@@ -422,22 +422,46 @@ def test_gpu_fill_in_update_with_failures_at_the_edges(case):
     assert pole > 0 and edge > 0, "the run is meant to repair pole-row and edge zones"
 
 
-def test_gpu_fixed_x1_sides_repair_reads_untouched_ghost_flags():
-    """A torus with FIXED x1 sides (bounds.fishmon.c leaves those ghosts
-    alone) whose inversions fail next to both sides: bound_pflag does
-    nothing on a FIXED side (boundsint.c:34), so the repair of an edge zone
-    counts its ghost neighbours as good, not as copies of the edge zone's
-    own failure flag.  Bitwise against the oracle, with failures on the
-    edge columns."""
-    n1, n2 = 32, 64
-    say(f"[case] FIXED x1 sides: torus {n1}x{n2} with failures at the edges, 6 steps")
+def fixed_x1_torus(n1, n2):
+    """A torus with FIXED x1 sides filled from panalytic (bounds.c; panalytic
+    = the initial state, omegaf = 0) whose field is raised next to both x1
+    edges (in the polar quarters, away from bound_disk's equator rows) and
+    both poles, so that inversions fail on the edge columns."""
     g, geom, p0, prm = problem("ks", n1, n2)
     prm.bc[H.X1DN] = prm.bc[H.X1UP] = H.FIXED
+    prm.boundsfile = H.BOUNDS_DISK
+    prm.disk_first = 0
     p0 = p0.copy()
-    strong_field_at_edges(p0, n1, n2)
-    p0[n1:n1 + NG, NG:NG + n2, 5] += 10.0 * np.sqrt(p0[n1:n1 + NG, NG:NG + n2, 0])
-    h = gpu_ctx(geom, prm, p0, 1e-3)
-    o = oracle_ctx(geom, prm, p0, 1e-3)
+    s = slice(NG, NG + n1)
+    for js in (slice(NG, 2 * NG), slice(n2, n2 + NG)):
+        p0[s, js, 6] += 10.0 * np.sqrt(p0[s, js, 0])
+    for i0 in (NG, n1):
+        for js in (slice(NG, NG + n2 // 4), slice(NG + 3 * n2 // 4, NG + n2)):
+            p0[i0:i0 + NG, js, 5] += 10.0 * np.sqrt(p0[i0:i0 + NG, js, 0])
+    return g, geom, p0, prm
+
+
+def test_gpu_fixed_x1_sides_repair_counts_ghost_flags_as_failed():
+    """bound_pflag does nothing on a FIXED side (boundsint.c:34), so those
+    ghosts keep set_arrays()'s initial flag -100 (set_arrays.c:86) and the
+    repair of an edge zone counts them as failed neighbours, not as copies
+    of the edge zone's flag.  Bitwise against the oracle over 6 steps, with
+    failures on both edge columns."""
+    n1, n2 = 32, 64
+    say(f"[case] FIXED x1 sides: torus {n1}x{n2} with failures at the edges, 6 steps")
+    g, geom, p0, prm = fixed_x1_torus(n1, n2)
+    om = np.zeros_like(p0)
+    h = H.Harm(prm, geom)
+    h.set_analytic(p0, om)
+    h.set_array(H.P, p0)
+    h.bound_prim(H.P)
+    h.set_time(0.0, 1e-3, 1e30)
+    o = Oracle(H.params_dict(prm))
+    o.set_geometry(*geom.tables())
+    o.set_analytic(p0, om)
+    o.set_array(0, p0)
+    o.bound_prim(0)
+    o.set_time(0.0, 1e-3, 1e30)
     edge = 0
     for s in range(6):
         h.step_ch()
