@@ -59,7 +59,8 @@ extern "C" {
 #define HARM_PERIODIC 100
 /* boundary files (the reference swaps bounds.*.c; harm_params.boundsfile) */
 #define HARM_BOUNDS_FISHMON 0  /* bounds.fishmon.c: outflow, polar axis with
-                                  POLEDEATH, FIXED leaves the ghosts alone */
+                                  POLEDEATH; it has no FIXED branch, so
+                                  harm_create refuses FIXED sides with it */
 #define HARM_BOUNDS_DISK 1     /* bounds.c (the reference's default file):
                                   FIXED from panalytic, outflow, and
                                   bound_disk() after every fill */
